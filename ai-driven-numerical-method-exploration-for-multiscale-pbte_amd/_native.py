@@ -1,0 +1,104 @@
+"""ctypes binding of ``libpbte_hip.so`` (declared in ``include/pbte_hip.h``).
+
+The library is loaded from this package directory only; if it is missing
+the import fails loudly -- there is no CPU fallback for any kernel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libpbte_hip.so"
+
+PBTE_FP64 = 0
+PBTE_FP32 = 1
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+class PbteDims(C.Structure):
+    _fields_ = [("ndir", C.c_int32), ("n_tris", C.c_int32), ("nd", C.c_int32),
+                ("nf", C.c_int32), ("n_faces", C.c_int32), ("nbc", C.c_int32)]
+
+
+class PbteSetup(C.Structure):
+    _fields_ = [("cxv", _f64p), ("cyv", _f64p), ("domega", _f64p),
+                ("cv", C.c_double), ("vg", C.c_double), ("tau_r", C.c_double),
+                ("tau_n", C.c_double), ("tau_c", C.c_double), ("pi", C.c_double),
+                ("mass", _f64p), ("gx", _f64p), ("gy", _f64p), ("fcm", _f64p),
+                ("ttfc", _f64p), ("nx", _f64p), ("ny", _f64p), ("int_tri", _f64p),
+                ("tri_faces", _i32p), ("face_bc", _i32p), ("neighbour", _i32p),
+                ("periodic_tri", _i32p), ("other_lfc", _i32p), ("bc_type", _i32p),
+                ("bc_temp", _f64p), ("face_tri", _i32p), ("face_lfc", _i32p),
+                ("order", _i32p), ("max_bundle", C.c_int32), ("block", C.c_int32)]
+
+
+_SIGS = {
+    "pbte_build_order": (C.c_int, [C.c_int32, C.c_int32, _i32p, _f64p, _f64p, _f64p, _f64p,
+                                   C.c_int32, _i32p, _i32p]),
+    "pbte_ctx_create": (C.c_void_p, [C.POINTER(PbteDims), C.POINTER(PbteSetup), C.c_void_p,
+                                     C.c_char_p, C.c_int32]),
+    "pbte_ctx_destroy": (None, [C.c_void_p]),
+    "pbte_last_error": (C.c_char_p, [C.c_void_p]),
+    "pbte_schedule_info": (C.c_int, [C.c_void_p, _i64p]),
+    "pbte_factorise": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pbte_sweep": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]),
+    "pbte_moments": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_wall_flux": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "pbte_transport_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, _f64p]),
+    "pbte_residual_iterate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _f64p, _f64p]),
+    "pbte_vec_multidot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
+                                    C.c_int64, _f64p]),
+    "pbte_vec_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, _f64p,
+                                  C.c_void_p, C.c_int64]),
+    "pbte_vec_axpby": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.c_void_p,
+                                 C.c_int64]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with build_native.py "
+                               "(there is no CPU fallback)")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def f64(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_f64p)
+
+
+def i32(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_i32p)
+
+
+class NativeError(RuntimeError):
+    """A HIP or usage error reported by the native library."""
+
+
+def check(ctx, status: int, what: str) -> int:
+    if status < 0:
+        msg = lib().pbte_last_error(ctx)
+        raise NativeError(f"{what}: {msg.decode() if msg else 'unknown error'}")
+    return status

@@ -1,0 +1,1256 @@
+// pbte_kernels.hip -- CDNA4 (gfx950) kernels of the gray linear Callaway DG
+// discrete-ordinates solve and the C-ABI that drives them.
+//
+// Data layout in HBM
+//   vdf        [D][N][P]   the reference's layout (sweep.py:132-158); a
+//                          cell's P values are contiguous, so the sweep
+//                          writes and the upwind gathers are 12-120 B runs and
+//                          the moment reduction reads coalesced rows.
+//   LU factors per bundle [P*P][slot][dir-in-bundle]: element-major, the
+//                          slots of one level contiguous, so a level's LU
+//                          streams as full 64-lane coalesced loads.
+//   pivots     per bundle [slot][dir] uint64, 4 bits per row.
+//   cell data  CellGeo{normals, across-face element, face code} and the
+//                          compact Q x Q upwind face blocks, gathered by cell.
+//
+// A bundle is a set of directions with the same inflow sign on every
+// interior and periodic face; such directions have the same reference
+// order and the same level schedule, so one workgroup sweeps all of them
+// together and loads each cell's direction-independent data once.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "pbte_hip.h"
+
+int pbte_detail_levels(int n, const int32_t* order, const int32_t* nb, const int32_t* tri_faces,
+                       const int32_t* face_bc, const int32_t* bc_type, const int32_t* periodic_tri,
+                       const double* nx, const double* ny, double cx, double cy,
+                       std::vector<int32_t>& level_of);
+
+namespace {
+
+// --------------------------------------------------------------------------
+// element node numbering: node (i/deg, j/deg) has index j(deg+1) - j(j-1)/2 + i
+// local face il runs from vertex il to vertex il+1 of (0,0),(1,0),(0,1)
+// --------------------------------------------------------------------------
+__host__ __device__ constexpr int node_id(int deg, int i, int j)
+{
+    return j * (deg + 1) - j * (j - 1) / 2 + i;
+}
+__host__ __device__ constexpr int face_node(int deg, int il, int a)
+{
+    return il == 0 ? node_id(deg, a, 0) : (il == 1 ? node_id(deg, deg - a, a) : node_id(deg, 0, deg - a));
+}
+template <int P> struct Deg;
+template <> struct Deg<3> { static constexpr int v = 1; };
+template <> struct Deg<6> { static constexpr int v = 2; };
+template <> struct Deg<10> { static constexpr int v = 3; };
+template <> struct Deg<15> { static constexpr int v = 4; };
+
+// face codes of CellGeo::info
+enum { FACE_INTERIOR = 0, FACE_PERIODIC = 1, FACE_ADIABATIC = 2, FACE_THERMAL = 3 };
+
+template <typename T> struct alignas(16) CellGeo {
+    T n[6];            // (nx, ny) of local faces 0, 1, 2
+    int32_t other[3];  // element across (interior / periodic) or face id (walls)
+    int32_t info[3];   // code | (other local face << 2) | (face id << 4)
+};
+
+#define HIP_TRY(expr)                                                       \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess) {                                             \
+            throw std::string(#expr " failed: ") + hipGetErrorString(e_);  \
+        }                                                                   \
+    } while (0)
+
+template <typename T> T* dalloc(size_t n)
+{
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+}  // namespace
+
+struct pbte_ctx {
+    pbte_dims dm{};
+    int P = 0, Q = 0, N = 0, D = 0, F = 0, deg = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double cv = 1, vg = 1, tau_r = 1, tau_n = 1, tau_c = 1, pi = M_PI;
+    int block = 256;
+
+    // direction tables
+    double *cxv = nullptr, *cyv = nullptr, *domega = nullptr;
+    float *cxv32 = nullptr, *cyv32 = nullptr;
+    // full FP64 tensors (factorisation, residual)
+    double *mass = nullptr, *gx = nullptr, *gy = nullptr, *fcm = nullptr, *int_tri = nullptr;
+    double* nrm = nullptr;     // [N][3][2] outward normals
+    // compact sweep data
+    CellGeo<double>* geo64 = nullptr;
+    CellGeo<float>* geo32 = nullptr;
+    double* K64 = nullptr;
+    float* K32 = nullptr;
+    double* bvec64 = nullptr;   // [F][P] tw * ttfc[:, 0] on thermal walls
+    float* bvec32 = nullptr;
+    int n_bc_th = 0;
+    // adiabatic walls
+    int n_wall = 0;
+    int32_t *wall_face = nullptr, *wall_tri = nullptr, *wall_lfc = nullptr;
+    // schedule
+    int nb = 0;
+    int64_t total_levels = 0, max_width = 0;
+    int32_t *block_bundle = nullptr, *bundle_ndir = nullptr, *bundle_dir_off = nullptr,
+            *bundle_dirs = nullptr, *slot_cell = nullptr, *lev_off = nullptr, *lev_ptr = nullptr;
+    int64_t *lu_off = nullptr, *piv_off = nullptr;
+    std::vector<int64_t> h_lu_off;
+    int64_t lu_elems = 0, piv_elems = 0;
+    // factors
+    double* lu64 = nullptr;
+    float* lu32 = nullptr;
+    uint64_t* piv = nullptr;
+    bool have64 = false, have32 = false;
+    // scratch
+    double* src64 = nullptr;
+    float* src32 = nullptr;
+    double* ssraw = nullptr;
+    double* red = nullptr;     // reduction partials
+    size_t red_len = 0;
+    double* small = nullptr;   // small device vector (GMRES coefficients)
+    int32_t* counter = nullptr;
+    double* hbuf = nullptr;    // pinned host staging
+    size_t hbuf_len = 0;
+
+    std::vector<void*> owned;
+    template <typename T> T* alloc(size_t n)
+    {
+        T* p = dalloc<T>(n);
+        owned.push_back(p);
+        return p;
+    }
+    template <typename T> T* upload(const T* h, size_t n)
+    {
+        T* p = alloc<T>(n);
+        HIP_TRY(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    ~pbte_ctx()
+    {
+        for (void* p : owned) (void)hipFree(p);
+        if (hbuf) (void)hipHostFree(hbuf);
+    }
+    double* host_staging(size_t n)
+    {
+        if (n > hbuf_len) {
+            if (hbuf) HIP_TRY(hipHostFree(hbuf));
+            hbuf = nullptr;
+            HIP_TRY(hipHostMalloc((void**)&hbuf, n * sizeof(double), hipHostMallocDefault));
+            hbuf_len = n;
+        }
+        return hbuf;
+    }
+};
+
+namespace {
+
+// ==========================================================================
+// batched element LU: build A_SOL (_kernels.py:106-125) and factorise it
+// with DGETF2's pivoting (_kernels.py:43-76), both in binary64; store in the
+// sweep layout with the reciprocal of U's diagonal in the diagonal slot.
+// ==========================================================================
+struct FactorArgs {
+    const int32_t *slot_cell, *bundle_ndir, *bundle_dir_off, *bundle_dirs;
+    const int64_t *lu_off, *piv_off;
+    const double *cxv, *cyv, *mass, *gx, *gy, *fcm;
+    const CellGeo<double>* geo;
+    double inv_tau_c;
+    int N;
+    double* lu64;
+    float* lu32;
+    uint64_t* piv;
+    int32_t* singular;
+};
+
+template <int P>
+__global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
+{
+    const int bundle = blockIdx.y;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.N) return;
+    const int B = a.bundle_ndir[bundle];
+    const int i = a.slot_cell[(size_t)bundle * a.N + s];
+    const int N = a.N;
+    const CellGeo<double> g = a.geo[i];
+    const double* fc = a.fcm + (size_t)i * 3 * P * P;
+    for (int b = 0; b < B; ++b) {
+        const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
+        const double cx = a.cxv[d], cy = a.cyv[d];
+        double w[3];
+#pragma unroll
+        for (int il = 0; il < 3; ++il) {
+            double sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
+            w[il] = 0.5 * (sp + fabs(sp));
+        }
+        double A[P][P];
+#pragma unroll
+        for (int l = 0; l < P; ++l)
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                double v = (a.inv_tau_c * a.mass[((size_t)m * P + l) * N + i] -
+                            cx * a.gx[((size_t)i * P + m) * P + l] - cy * a.gy[((size_t)i * P + m) * P + l]);
+                v = v + w[0] * fc[(0 * P + m) * P + l];
+                v = v + w[1] * fc[(1 * P + m) * P + l];
+                v = v + w[2] * fc[(2 * P + m) * P + l];
+                A[m][l] = v;
+            }
+        uint64_t pv = 0;
+        int info = 0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            int p = k;
+            double big = fabs(A[k][k]);
+#pragma unroll
+            for (int r = k + 1; r < P; ++r) {
+                double v = fabs(A[r][k]);
+                if (v > big) { big = v; p = r; }
+            }
+            pv |= (uint64_t)p << (4 * k);
+#pragma unroll
+            for (int r = k + 1; r < P; ++r) {
+                if (p == r) {
+#pragma unroll
+                    for (int j = 0; j < P; ++j) {
+                        double t = A[k][j]; A[k][j] = A[r][j]; A[r][j] = t;
+                    }
+                }
+            }
+            if (A[k][k] == 0.0) {
+                if (info == 0) info = k + 1;
+                continue;
+            }
+            const double rcp = 1.0 / A[k][k];
+#pragma unroll
+            for (int r = k + 1; r < P; ++r) A[r][k] = A[r][k] * rcp;
+#pragma unroll
+            for (int r = k + 1; r < P; ++r) {
+                const double ark = A[r][k];
+#pragma unroll
+                for (int j = k + 1; j < P; ++j) A[r][j] -= ark * A[k][j];
+            }
+            A[k][k] = rcp;   // the solve multiplies by the reciprocal
+        }
+        if (info) atomicAdd(a.singular, 1);
+        const int64_t base = a.lu_off[bundle];
+#pragma unroll
+        for (int r = 0; r < P; ++r)
+#pragma unroll
+            for (int c = 0; c < P; ++c) {
+                const size_t off = (size_t)base + ((size_t)(r * P + c) * N + s) * B + b;
+                if (a.lu64) a.lu64[off] = A[r][c];
+                if (a.lu32) a.lu32[off] = (float)A[r][c];
+            }
+        a.piv[a.piv_off[bundle] + (size_t)s * B + b] = pv;
+    }
+}
+
+// ==========================================================================
+// source vectors: src = U + cx VX + cy VY per cell (the Callaway scattering
+// source of _kernels.py:153-167 split by its dependence on the direction)
+// ==========================================================================
+template <int P, typename T>
+__global__ void __launch_bounds__(256) source_kernel(const double* __restrict__ mass,
+                                                     const double* __restrict__ ts,
+                                                     const double* __restrict__ qxs,
+                                                     const double* __restrict__ qys, T* __restrict__ src,
+                                                     int N, double cv, double pi, double tau_r,
+                                                     double tau_n, double vg)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N * P) return;
+    const int i = g / P, m = g - (g / P) * P;
+    double u = 0.0, vx = 0.0, vy = 0.0;
+#pragma unroll
+    for (int l = 0; l < P; ++l) {
+        const double mlm = mass[((size_t)l * P + m) * N + i];
+        const double t = ts[(size_t)l * N + i];
+        const double al = cv * t / 4.0 / pi / tau_r + 1.0 / tau_n * (cv * t / 4.0 / pi);
+        const double bx = 1.0 / tau_n * (3.0 / 4.0 / pi * qxs[(size_t)l * N + i] / vg / vg);
+        const double by = 1.0 / tau_n * (3.0 / 4.0 / pi * qys[(size_t)l * N + i] / vg / vg);
+        u += mlm * al;
+        vx += mlm * bx;
+        vy += mlm * by;
+    }
+    T* o = src + (size_t)i * 3 * P;
+    o[m] = (T)u;
+    o[P + m] = (T)vx;
+    o[2 * P + m] = (T)vy;
+}
+
+// ==========================================================================
+// the level-set sweep (_kernels.py:217-239 with the inflow of 169-215 and
+// the triangular solve of 78-101).  One workgroup per direction bundle;
+// thread t owns direction t % B of the bundle for the whole sweep and slot
+// t / B of every level; a barrier separates levels.
+// ==========================================================================
+template <typename T> struct SweepArgs {
+    const int32_t *block_bundle, *bundle_ndir, *bundle_dir_off, *bundle_dirs, *slot_cell,
+        *lev_off, *lev_ptr;
+    const int64_t *lu_off, *piv_off;
+    const T *lu;
+    const uint64_t* piv;
+    const T *cxv, *cyv;
+    const CellGeo<T>* geo;
+    const T* K;
+    const T* bvec;
+    const T* src;
+    const double* flux_wall;
+    const T* pbuf;
+    T* vdf;
+    int N, F, use_pbuf, homogeneous;
+};
+
+template <int P, typename T>
+__global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
+{
+    constexpr int DEG = Deg<P>::v;
+    constexpr int Q = DEG + 1;
+    const int bundle = a.block_bundle[blockIdx.x];
+    const int B = a.bundle_ndir[bundle];
+    const int t = threadIdx.x;
+    const int b = t % B;
+    const int c = t / B;
+    const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
+    const T cx = a.cxv[d], cy = a.cyv[d];
+    const int N = a.N;
+    const int32_t* cells = a.slot_cell + (size_t)bundle * N;
+    const T* lu = a.lu + a.lu_off[bundle];
+    const uint64_t* piv = a.piv + a.piv_off[bundle];
+    T* vdf_d = a.vdf + (size_t)d * N * P;
+    const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
+
+    for (int lv = l0; lv < l1; ++lv) {
+        const int s = a.lev_ptr[lv] + c;
+        if (s < a.lev_ptr[lv + 1]) {
+            const int i = cells[s];
+            T x[P];
+            const T* sv = a.src + (size_t)i * 3 * P;
+#pragma unroll
+            for (int m = 0; m < P; ++m) x[m] = sv[m] + cx * sv[P + m] + cy * sv[2 * P + m];
+            const CellGeo<T> g = a.geo[i];
+#pragma unroll
+            for (int il = 0; il < 3; ++il) {
+                const T sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
+                const T w = T(0.5) * (sp - fabs(sp));
+                if (w == T(0)) continue;
+                const int info = g.info[il];
+                const int code = info & 3;
+                if (code <= FACE_PERIODIC) {
+                    const int ilp = (info >> 2) & 3;
+                    const T* fe = (code == FACE_PERIODIC && a.use_pbuf)
+                                      ? a.pbuf + ((size_t)d * a.F + (info >> 4)) * P
+                                      : vdf_d + (size_t)g.other[il] * P;
+                    T fb[Q];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) fb[q] = fe[face_node(DEG, ilp, q)];
+                    const T* Kf = a.K + ((size_t)i * 3 + il) * Q * Q;
+#pragma unroll
+                    for (int r = 0; r < Q; ++r) {
+                        T acc = x[face_node(DEG, il, r)];
+#pragma unroll
+                        for (int q = 0; q < Q; ++q) acc -= (w * Kf[r * Q + q]) * fb[q];
+                        x[face_node(DEG, il, r)] = acc;
+                    }
+                } else if (code == FACE_ADIABATIC) {
+                    const double* fw = a.flux_wall + (size_t)g.other[il] * P;
+#pragma unroll
+                    for (int m = 0; m < P; ++m) x[m] += w * (T)fw[m];
+                } else if (!a.homogeneous) {
+                    const T* bv = a.bvec + (size_t)g.other[il] * P;
+#pragma unroll
+                    for (int m = 0; m < P; ++m) x[m] -= w * bv[m];
+                }
+            }
+            // P b: DGETRS row interchanges
+            const uint64_t pv = piv[(size_t)s * B + b];
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const int p = (int)((pv >> (4 * k)) & 15u);
+#pragma unroll
+                for (int r = k + 1; r < P; ++r)
+                    if (p == r) { T tmp = x[k]; x[k] = x[r]; x[r] = tmp; }
+            }
+            const size_t ld = (size_t)N * B;
+            const T* e = lu + (size_t)s * B + b;
+            // unit lower solve
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                const T xk = x[k];
+#pragma unroll
+                for (int r = k + 1; r < P; ++r) x[r] -= xk * e[(size_t)(r * P + k) * ld];
+            }
+            // upper solve, accumulated downwards (DTRSM column order)
+#pragma unroll
+            for (int k = P - 1; k >= 0; --k) {
+                x[k] = x[k] * e[(size_t)(k * P + k) * ld];
+                const T xk = x[k];
+#pragma unroll
+                for (int r = 0; r < k; ++r) x[r] -= xk * e[(size_t)(r * P + k) * ld];
+            }
+            T* o = vdf_d + (size_t)i * P;
+#pragma unroll
+            for (int m = 0; m < P; ++m) o[m] = x[m];
+        }
+        __syncthreads();
+    }
+}
+
+// ==========================================================================
+// moments: per (cell, dof) the direction sums in index order, binary64
+// (_kernels.py:311-338); element integrals in a second pass
+// ==========================================================================
+template <typename T>
+__global__ void __launch_bounds__(256) moments_kernel(const T* __restrict__ vdf,
+                                                      const double* __restrict__ cxv,
+                                                      const double* __restrict__ cyv,
+                                                      const double* __restrict__ dw, int D, int N,
+                                                      int P, double cv, double* ts, double* qxs,
+                                                      double* qys, double* ssraw)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N * P) return;
+    const int i = g / P, l = g - (g / P) * P;
+    double ss = 0.0, ssx = 0.0, ssy = 0.0;
+    const size_t stride = (size_t)N * P;
+    const T* f = vdf + g;
+    for (int d = 0; d < D; ++d) {
+        const double v = (double)f[(size_t)d * stride];
+        const double w = dw[d];
+        ss += v * w;
+        ssx += cxv[d] * v * w;
+        ssy += cyv[d] * v * w;
+    }
+    ts[(size_t)l * N + i] = ss / cv;
+    qxs[(size_t)l * N + i] = ssx;
+    qys[(size_t)l * N + i] = ssy;
+    ssraw[(size_t)l * N + i] = ss;
+}
+
+__global__ void __launch_bounds__(256) integrals_kernel(const double* ssraw, const double* qxs,
+                                                        const double* qys, const double* int_tri,
+                                                        int N, int P, double cv, double* temp,
+                                                        double* qx, double* qy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double t = 0.0, x = 0.0, y = 0.0;
+    for (int l = 0; l < P; ++l) {
+        const double w = int_tri[(size_t)l * N + i];
+        t += ssraw[(size_t)l * N + i] * w / cv;
+        x += qxs[(size_t)l * N + i] * w;
+        y += qys[(size_t)l * N + i] * w;
+    }
+    temp[i] = t;
+    qx[i] = x;
+    qy[i] = y;
+}
+
+// ==========================================================================
+// adiabatic wall emission (_kernels.py:363-397): one thread per (wall face, m)
+// ==========================================================================
+template <typename T>
+__global__ void __launch_bounds__(256) wall_flux_kernel(const T* __restrict__ vdf, const double* cxv,
+                                                        const double* cyv, const double* dw,
+                                                        const double* fcm, const double* nx3,
+                                                        const int32_t* wall_face, const int32_t* wall_tri,
+                                                        const int32_t* wall_lfc, int n_wall, int D,
+                                                        int N, int P, double* flux_wall)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_wall * P) return;
+    const int k = g / P, m = g - (g / P) * P;
+    const int tri = wall_tri[k], lfc = wall_lfc[k];
+    const double n1 = -nx3[2 * (tri * 3 + lfc)], n2 = -nx3[2 * (tri * 3 + lfc) + 1];
+    const double* fc = fcm + ((size_t)tri * 3 + lfc) * P * P + (size_t)m * P;
+    double ss1 = 0.0, ss2 = 0.0;
+    for (int d = 0; d < D; ++d) {
+        const double u = cxv[d] * n1 + cyv[d] * n2;
+        if (u < 0.0) {
+            const T* f = vdf + ((size_t)d * N + tri) * P;
+            double acc = 0.0;
+            for (int l = 0; l < P; ++l) acc += (double)f[l] * fc[l];
+            ss1 += u * dw[d] * acc;
+        } else {
+            ss2 -= u * dw[d];
+        }
+    }
+    flux_wall[(size_t)wall_face[k] * P + m] = -ss1 / ss2;
+}
+
+// ==========================================================================
+// transport residual ||A f - b||^2 (_kernels.py:274-306), binary64: one
+// thread per (cell, direction), workgroup partials per direction
+// ==========================================================================
+template <int P>
+__global__ void __launch_bounds__(256) residual_kernel(
+    const double* __restrict__ vdf, const double* __restrict__ src, const double* cxv, const double* cyv,
+    const double* dw, const double* mass, const double* gx, const double* gy, const double* fcm,
+    const CellGeo<double>* geo, const double* K, const double* bvec, const double* flux_wall,
+    double inv_tau_c, int N, double* partial)
+{
+    constexpr int DEG = Deg<P>::v;
+    constexpr int Q = DEG + 1;
+    const int d = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double num = 0.0, den = 0.0;
+    if (i < N) {
+        const double cx = cxv[d], cy = cyv[d];
+        const double* vd = vdf + (size_t)d * N * P;
+        double b[P], f[P];
+        const double* sv = src + (size_t)i * 3 * P;
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            b[m] = sv[m] + cx * sv[P + m] + cy * sv[2 * P + m];
+            f[m] = vd[(size_t)i * P + m];
+        }
+        const CellGeo<double> g = geo[i];
+        double wout[3];
+#pragma unroll
+        for (int il = 0; il < 3; ++il) {
+            const double sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
+            wout[il] = 0.5 * (sp + fabs(sp));
+            const double w = 0.5 * (sp - fabs(sp));
+            if (w == 0.0) continue;
+            const int info = g.info[il];
+            const int code = info & 3;
+            if (code <= FACE_PERIODIC) {
+                const int ilp = (info >> 2) & 3;
+                const double* fe = vd + (size_t)g.other[il] * P;
+                const double* Kf = K + ((size_t)i * 3 + il) * Q * Q;
+#pragma unroll
+                for (int r = 0; r < Q; ++r) {
+                    double acc = b[face_node(DEG, il, r)];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) acc -= (w * Kf[r * Q + q]) * fe[face_node(DEG, ilp, q)];
+                    b[face_node(DEG, il, r)] = acc;
+                }
+            } else if (code == FACE_ADIABATIC) {
+#pragma unroll
+                for (int m = 0; m < P; ++m) b[m] += w * flux_wall[(size_t)g.other[il] * P + m];
+            } else {
+#pragma unroll
+                for (int m = 0; m < P; ++m) b[m] -= w * bvec[(size_t)g.other[il] * P + m];
+            }
+        }
+        const double* fc = fcm + (size_t)i * 3 * P * P;
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            double r = -b[m];
+#pragma unroll
+            for (int l = 0; l < P; ++l) {
+                double a = (inv_tau_c * mass[((size_t)m * P + l) * N + i] - cx * gx[((size_t)i * P + m) * P + l] -
+                            cy * gy[((size_t)i * P + m) * P + l]);
+                a = a + wout[0] * fc[(0 * P + m) * P + l];
+                a = a + wout[1] * fc[(1 * P + m) * P + l];
+                a = a + wout[2] * fc[(2 * P + m) * P + l];
+                r += a * f[l];
+            }
+            num += r * r * dw[d];
+            den += b[m] * b[m] * dw[d];
+        }
+    }
+    __shared__ double sn[256], sd[256];
+    sn[threadIdx.x] = num;
+    sd[threadIdx.x] = den;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sn[threadIdx.x] += sn[threadIdx.x + h];
+            sd[threadIdx.x] += sd[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[((size_t)d * gridDim.x + blockIdx.x) * 2 + 0] = sn[0];
+        partial[((size_t)d * gridDim.x + blockIdx.x) * 2 + 1] = sd[0];
+    }
+}
+
+// sum partials per row in index order: out[r] = sum_k in[r][k]
+__global__ void rowsum_kernel(const double* in, int nrow, int ncol, int stride_k, int comps,
+                              double* out)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nrow * comps) return;
+    const int r = g / comps, c = g - (g / comps) * comps;
+    double s = 0.0;
+    for (int k = 0; k < ncol; ++k) s += in[((size_t)r * ncol + k) * stride_k + c];
+    out[g] = s;
+}
+
+// sequential index-order sums of the iterate residual (_kernels.py:340-361)
+__global__ void iterate_residual_kernel(const double* temp, const double* t_old, int n, double* out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s1 = 0.0, s2 = 0.0, m = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double dlt = temp[i] - t_old[i];
+        s1 += dlt * dlt;
+        s2 += temp[i] * temp[i];
+        m += temp[i];
+    }
+    out[0] = sqrt(s1 / s2);
+    out[1] = m;
+}
+
+// ==========================================================================
+// Krylov vector kernels: fixed partition, fixed reduction order
+// ==========================================================================
+constexpr int kRows = 8;
+constexpr int kDotBlocks = 1024;
+
+__global__ void __launch_bounds__(256) multidot_kernel(const double* __restrict__ Qm, int64_t ld, int m,
+                                                       const double* __restrict__ w, int64_t n,
+                                                       double* partial)
+{
+    const int r0 = blockIdx.y * kRows;
+    double acc[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double wi = w[i];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+            if (r0 + r < m) acc[r] += Qm[(size_t)(r0 + r) * ld + i] * wi;
+    }
+    __shared__ double sh[kRows][256];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) sh[r][threadIdx.x] = acc[r];
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + h];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < kRows && r0 + (int)threadIdx.x < m)
+        partial[(size_t)(r0 + threadIdx.x) * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) update_kernel(const double* __restrict__ Qm, int64_t ld, int m,
+                                                     const double* __restrict__ c, double* w, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < m; ++k) s += c[k] * Qm[(size_t)k * ld + i];
+    w[i] -= s;
+}
+
+__global__ void __launch_bounds__(256) axpby_kernel(double a, const double* x, double b, double* y, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = a * x[i] + b * y[i];
+}
+
+inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename F>
+int guarded(pbte_ctx* ctx, F&& f)
+{
+    if (!ctx) return -1;
+    try {
+        return f();
+    } catch (const std::string& e) {
+        ctx->err = e;
+        return -2;
+    }
+}
+
+}  // namespace
+
+// ==========================================================================
+// C-ABI
+// ==========================================================================
+extern "C" const char* pbte_last_error(pbte_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" void pbte_ctx_destroy(pbte_ctx* ctx) { delete ctx; }
+
+extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su, void* stream,
+                                     char* err, int32_t errlen)
+{
+    auto fail = [&](const std::string& m) -> pbte_ctx* {
+        if (err && errlen > 0) {
+            std::strncpy(err, m.c_str(), (size_t)errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        return nullptr;
+    };
+    if (!dims || !su) return fail("null dims/setup");
+    const int P = dims->nd, Q = dims->nf, N = dims->n_tris, D = dims->ndir, F = dims->n_faces;
+    if (!((P == 3 && Q == 2) || (P == 6 && Q == 3) || (P == 10 && Q == 4) || (P == 15 && Q == 5)))
+        return fail("unsupported (ndof_tri, ndof_fc): degrees 1-4 only");
+    if (N <= 0 || D <= 0) return fail("empty mesh or velocity set");
+    pbte_ctx* ctx = new pbte_ctx();
+    try {
+        ctx->dm = *dims;
+        ctx->P = P; ctx->Q = Q; ctx->N = N; ctx->D = D; ctx->F = F; ctx->deg = Q - 1;
+        ctx->stream = (hipStream_t)stream;
+        ctx->cv = su->cv; ctx->vg = su->vg; ctx->tau_r = su->tau_r; ctx->tau_n = su->tau_n;
+        ctx->tau_c = su->tau_c; ctx->pi = su->pi;
+        ctx->block = su->block > 0 ? su->block : 256;
+        if (ctx->block % 64 || ctx->block > 256) throw std::string("block must be a multiple of 64, <= 256");
+        const int deg = Q - 1;
+
+        // --- direction tables -------------------------------------------------
+        ctx->cxv = ctx->upload(su->cxv, D);
+        ctx->cyv = ctx->upload(su->cyv, D);
+        ctx->domega = ctx->upload(su->domega, D);
+        std::vector<float> f32(D);
+        for (int d = 0; d < D; ++d) f32[d] = (float)su->cxv[d];
+        ctx->cxv32 = ctx->upload(f32.data(), D);
+        for (int d = 0; d < D; ++d) f32[d] = (float)su->cyv[d];
+        ctx->cyv32 = ctx->upload(f32.data(), D);
+
+        ctx->mass = ctx->upload(su->mass, (size_t)P * P * N);
+        ctx->gx = ctx->upload(su->gx, (size_t)N * P * P);
+        ctx->gy = ctx->upload(su->gy, (size_t)N * P * P);
+        ctx->fcm = ctx->upload(su->fcm, (size_t)N * 3 * P * P);
+        ctx->int_tri = ctx->upload(su->int_tri, (size_t)P * N);
+        {
+            std::vector<double> nrm((size_t)N * 6);
+            for (int i = 0; i < N; ++i)
+                for (int il = 0; il < 3; ++il) {
+                    nrm[(size_t)i * 6 + 2 * il] = su->nx[i * 3 + il];
+                    nrm[(size_t)i * 6 + 2 * il + 1] = su->ny[i * 3 + il];
+                }
+            ctx->nrm = ctx->upload(nrm.data(), nrm.size());
+        }
+
+        // --- compact cell data ------------------------------------------------
+        std::vector<CellGeo<double>> g64(N);
+        std::vector<CellGeo<float>> g32(N);
+        std::vector<double> K((size_t)N * 3 * Q * Q, 0.0);
+        std::vector<double> bvec((size_t)std::max(F, 1) * P, 0.0);
+        double worst = 0.0;
+        for (int i = 0; i < N; ++i) {
+            for (int il = 0; il < 3; ++il) {
+                g64[i].n[2 * il] = su->nx[i * 3 + il];
+                g64[i].n[2 * il + 1] = su->ny[i * 3 + il];
+                const int fc = su->tri_faces[i * 3 + il];
+                const int bc = su->face_bc[fc];
+                int code, other, ilp = 0;
+                if (bc < 0) {
+                    code = FACE_INTERIOR;
+                    other = su->neighbour[i * 3 + il];
+                    ilp = su->other_lfc[i * 3 + il];
+                } else {
+                    const int typ = su->bc_type[bc];
+                    if (typ == 3) {
+                        code = FACE_PERIODIC;
+                        other = su->periodic_tri[fc];
+                        ilp = su->other_lfc[i * 3 + il];
+                    } else if (typ == 2) {
+                        code = FACE_ADIABATIC;
+                        other = fc;
+                    } else if (typ == 1) {
+                        code = FACE_THERMAL;
+                        other = fc;
+                        const double tw = su->cv / 4.0 / su->pi * su->bc_temp[bc];
+                        const double* tt = su->ttfc + ((size_t)i * 3 + il) * P * P;
+                        for (int m = 0; m < P; ++m) bvec[(size_t)fc * P + m] = tw * tt[m * P + 0];
+                    } else {
+                        throw std::string("boundary type not supported by the sweep");
+                    }
+                }
+                if (code <= FACE_PERIODIC) {
+                    if (other < 0 || ilp < 0 || ilp > 2) throw std::string("bad across-face element");
+                    const double* tt = su->ttfc + ((size_t)i * 3 + il) * P * P;
+                    double bmax = 0.0, omax = 0.0;
+                    std::vector<char> onr(P, 0), onc(P, 0);
+                    for (int a = 0; a < Q; ++a) {
+                        onr[face_node(deg, il, a)] = 1;
+                        onc[face_node(deg, ilp, a)] = 1;
+                    }
+                    for (int m = 0; m < P; ++m)
+                        for (int l = 0; l < P; ++l) {
+                            const double v = std::fabs(tt[m * P + l]);
+                            if (onr[m] && onc[l]) bmax = std::max(bmax, v);
+                            else omax = std::max(omax, v);
+                        }
+                    if (bmax > 0) worst = std::max(worst, omax / bmax);
+                    for (int a = 0; a < Q; ++a)
+                        for (int q = 0; q < Q; ++q)
+                            K[(((size_t)i * 3 + il) * Q + a) * Q + q] =
+                                tt[face_node(deg, il, a) * P + face_node(deg, ilp, q)];
+                }
+                g64[i].other[il] = other;
+                g64[i].info[il] = code | (ilp << 2) | (fc << 4);
+            }
+            for (int k = 0; k < 6; ++k) g32[i].n[k] = (float)g64[i].n[k];
+            for (int k = 0; k < 3; ++k) {
+                g32[i].other[k] = g64[i].other[k];
+                g32[i].info[k] = g64[i].info[k];
+            }
+        }
+        if (worst > 1e-9)
+            throw std::string("element x neighbour face tensor is not confined to the shared face "
+                              "(non-conforming mesh?)");
+        ctx->geo64 = ctx->upload(g64.data(), N);
+        ctx->geo32 = ctx->upload(g32.data(), N);
+        ctx->K64 = ctx->upload(K.data(), K.size());
+        std::vector<float> K32(K.begin(), K.end());
+        ctx->K32 = ctx->upload(K32.data(), K32.size());
+        ctx->bvec64 = ctx->upload(bvec.data(), bvec.size());
+        std::vector<float> bv32(bvec.begin(), bvec.end());
+        ctx->bvec32 = ctx->upload(bv32.data(), bv32.size());
+
+        // --- adiabatic walls --------------------------------------------------
+        std::vector<int32_t> wf, wt, wl;
+        for (int f = 0; f < F; ++f) {
+            const int bc = su->face_bc[f];
+            if (bc < 0 || su->bc_type[bc] != 2) continue;
+            int tri = su->face_tri[f * 2], lfc = su->face_lfc[f * 2];
+            if (tri < 0) { tri = su->face_tri[f * 2 + 1]; lfc = su->face_lfc[f * 2 + 1]; }
+            wf.push_back(f); wt.push_back(tri); wl.push_back(lfc);
+        }
+        ctx->n_wall = (int)wf.size();
+        ctx->wall_face = ctx->upload(wf.data(), wf.size());
+        ctx->wall_tri = ctx->upload(wt.data(), wt.size());
+        ctx->wall_lfc = ctx->upload(wl.data(), wl.size());
+
+        // --- bundles: directions with identical inflow signs ------------------
+        std::vector<uint64_t> hsh(D);
+        std::vector<std::vector<uint8_t>> sig(D);
+        for (int d = 0; d < D; ++d) {
+            auto& sg = sig[d];
+            sg.resize((size_t)N * 3);
+            uint64_t h = 1469598103934665603ull;
+            for (int i = 0; i < N; ++i)
+                for (int il = 0; il < 3; ++il) {
+                    const double sp = su->cxv[d] * su->nx[i * 3 + il] + su->cyv[d] * su->ny[i * 3 + il];
+                    const uint8_t v = sp < 0.0 ? 1 : (sp > 0.0 ? 2 : 0);
+                    sg[(size_t)i * 3 + il] = v;
+                    h = (h ^ v) * 1099511628211ull;
+                }
+            hsh[d] = h;
+        }
+        int bmax = su->max_bundle > 0 ? su->max_bundle : 1;
+        bmax = std::min(bmax, 8);
+        while (ctx->block % bmax) --bmax;
+        std::vector<std::vector<int>> groups;
+        {
+            std::vector<char> used(D, 0);
+            for (int d = 0; d < D; ++d) {
+                if (used[d]) continue;
+                std::vector<int> grp{d};
+                used[d] = 1;
+                for (int e = d + 1; e < D; ++e)
+                    if (!used[e] && hsh[e] == hsh[d] && sig[e] == sig[d] &&
+                        std::memcmp(su->order + (size_t)e * N, su->order + (size_t)d * N,
+                                    (size_t)N * sizeof(int32_t)) == 0) {
+                        grp.push_back(e);
+                        used[e] = 1;
+                    }
+                groups.push_back(grp);
+            }
+        }
+        std::vector<std::vector<int>> bundles;
+        for (auto& grp : groups) {
+            size_t k = 0;
+            while (k < grp.size()) {
+                int take = bmax;
+                while (take > 1 && (size_t)take > grp.size() - k) take >>= 1;
+                bundles.emplace_back(grp.begin() + k, grp.begin() + k + take);
+                k += take;
+            }
+        }
+        const int nb = (int)bundles.size();
+        ctx->nb = nb;
+        // levels per bundle (all directions of a bundle share them)
+        std::vector<int32_t> h_slot((size_t)nb * N), h_lev_off(nb + 1, 0), h_lev_ptr;
+        std::vector<int32_t> h_ndir(nb), h_dir_off(nb + 1, 0), h_dirs;
+        std::vector<int64_t> h_lu_off(nb + 1, 0), h_piv_off(nb + 1, 0);
+        std::vector<std::vector<int32_t>> slots(nb);
+        std::vector<std::vector<int32_t>> lptr(nb);
+        std::atomic_int next{0};
+        std::vector<std::string> errs(nb);
+        auto work = [&]() {
+            std::vector<int32_t> level_of;
+            for (int k = next++; k < nb; k = next++) {
+                const int d = bundles[k][0];
+                const int32_t* order = su->order + (size_t)d * N;
+                int nlev = pbte_detail_levels(N, order, su->neighbour, su->tri_faces, su->face_bc,
+                                              su->bc_type, su->periodic_tri, su->nx, su->ny, su->cxv[d],
+                                              su->cyv[d], level_of);
+                std::vector<int32_t> cnt(nlev + 1, 0);
+                for (int i = 0; i < N; ++i) ++cnt[level_of[i] + 1];
+                for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
+                std::vector<int32_t> start(cnt.begin(), cnt.end() - 1);
+                auto& sl = slots[k];
+                sl.assign(N, -1);
+                // within a level keep the reference order
+                for (int p = 0; p < N; ++p) {
+                    const int i = order[p];
+                    sl[start[level_of[i]]++] = i;
+                }
+                const int B = (int)bundles[k].size();
+                const int width = ctx->block / B;
+                auto& lp = lptr[k];
+                lp.clear();
+                for (int l = 0; l < nlev; ++l)
+                    for (int s = cnt[l]; s < cnt[l + 1]; s += width) lp.push_back(s);
+                lp.push_back(N);
+            }
+        };
+        {
+            unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < hw; ++t) th.emplace_back(work);
+            for (auto& x : th) x.join();
+        }
+        int64_t maxw = 0;
+        for (int k = 0; k < nb; ++k) {
+            const int B = (int)bundles[k].size();
+            h_ndir[k] = B;
+            h_dir_off[k + 1] = h_dir_off[k] + B;
+            for (int d : bundles[k]) h_dirs.push_back(d);
+            std::copy(slots[k].begin(), slots[k].end(), h_slot.begin() + (size_t)k * N);
+            // lev_ptr is shared storage: each bundle's boundaries, its last
+            // entry doubling as the end of its final level
+            h_lev_off[k] = (int32_t)h_lev_ptr.size();
+            for (size_t j = 0; j + 1 < lptr[k].size(); ++j) {
+                h_lev_ptr.push_back(lptr[k][j]);
+                maxw = std::max<int64_t>(maxw, (int64_t)(lptr[k][j + 1] - lptr[k][j]) * B);
+            }
+            h_lev_ptr.push_back(N);
+            h_lu_off[k + 1] = h_lu_off[k] + (int64_t)P * P * N * B;
+            h_piv_off[k + 1] = h_piv_off[k] + (int64_t)N * B;
+        }
+        // lev_off[k]..lev_off[k]+nlev_k-1 index level starts; lev_ptr[j+1] is
+        // the end of level j; the sentinel N closes every bundle's last level.
+        int64_t tot = 0;
+        for (int k = 0; k < nb; ++k) tot += (int64_t)lptr[k].size() - 1;
+        ctx->total_levels = tot;
+        ctx->max_width = maxw;
+        // kernel loops lv in [lev_off[k], lev_end[k]); pass the end through
+        // lev_off[k+1] by storing per-bundle (start, end) pairs instead
+        std::vector<int32_t> h_lev_se(2 * nb);
+        for (int k = 0; k < nb; ++k) {
+            h_lev_se[2 * k] = h_lev_off[k];
+            h_lev_se[2 * k + 1] = h_lev_off[k] + (int32_t)lptr[k].size() - 1;
+        }
+        // XCD-aware block -> bundle map: blocks are dispatched round robin over
+        // 8 XCDs; give each XCD a contiguous range of bundles sorted by the
+        // in-plane angle so its L2 sees similar sweep fronts.
+        std::vector<int> sorted(nb);
+        std::iota(sorted.begin(), sorted.end(), 0);
+        std::stable_sort(sorted.begin(), sorted.end(), [&](int p, int q) {
+            const int dp = bundles[p][0], dq = bundles[q][0];
+            return std::atan2(su->cyv[dp], su->cxv[dp]) < std::atan2(su->cyv[dq], su->cxv[dq]);
+        });
+        std::vector<int32_t> h_block_bundle(nb);
+        {
+            const int X = 8;
+            std::vector<int> count(X, 0), startx(X + 1, 0);
+            for (int g = 0; g < nb; ++g) ++count[g % X];
+            for (int x = 0; x < X; ++x) startx[x + 1] = startx[x] + count[x];
+            for (int g = 0; g < nb; ++g) h_block_bundle[g] = sorted[startx[g % X] + g / X];
+        }
+        ctx->block_bundle = ctx->upload(h_block_bundle.data(), nb);
+        ctx->bundle_ndir = ctx->upload(h_ndir.data(), nb);
+        ctx->bundle_dir_off = ctx->upload(h_dir_off.data(), nb + 1);
+        ctx->bundle_dirs = ctx->upload(h_dirs.data(), h_dirs.size());
+        ctx->slot_cell = ctx->upload(h_slot.data(), h_slot.size());
+        ctx->lev_off = ctx->upload(h_lev_se.data(), h_lev_se.size());
+        ctx->lev_ptr = ctx->upload(h_lev_ptr.data(), h_lev_ptr.size());
+        ctx->lu_off = ctx->upload(h_lu_off.data(), nb + 1);
+        ctx->piv_off = ctx->upload(h_piv_off.data(), nb + 1);
+        ctx->h_lu_off = h_lu_off;
+        ctx->lu_elems = h_lu_off[nb];
+        ctx->piv_elems = h_piv_off[nb];
+
+        // --- scratch -----------------------------------------------------------
+        ctx->src64 = ctx->alloc<double>((size_t)N * 3 * P);
+        ctx->src32 = ctx->alloc<float>((size_t)N * 3 * P);
+        ctx->ssraw = ctx->alloc<double>((size_t)N * P);
+        ctx->red_len = std::max<size_t>((size_t)D * nblocks(N, 256) * 2, (size_t)64 * kDotBlocks) + 64;
+        ctx->red = ctx->alloc<double>(ctx->red_len);
+        ctx->small = ctx->alloc<double>(4096);
+        ctx->counter = ctx->alloc<int32_t>(16);
+        HIP_TRY(hipDeviceSynchronize());
+    } catch (const std::string& e) {
+        delete ctx;
+        return fail(e);
+    } catch (const std::exception& e) {
+        delete ctx;
+        return fail(e.what());
+    }
+    return ctx;
+}
+
+extern "C" int pbte_schedule_info(pbte_ctx* ctx, int64_t* out4)
+{
+    return guarded(ctx, [&]() {
+        out4[0] = ctx->total_levels;
+        out4[1] = ctx->nb;
+        out4[2] = ctx->max_width;
+        out4[3] = ctx->lu_elems;
+        return 0;
+    });
+}
+
+namespace {
+template <int P>
+void launch_factor(pbte_ctx* ctx, double* lu64, float* lu32)
+{
+    FactorArgs a{};
+    a.slot_cell = ctx->slot_cell; a.bundle_ndir = ctx->bundle_ndir; a.bundle_dir_off = ctx->bundle_dir_off;
+    a.bundle_dirs = ctx->bundle_dirs; a.lu_off = ctx->lu_off; a.piv_off = ctx->piv_off;
+    a.cxv = ctx->cxv; a.cyv = ctx->cyv; a.mass = ctx->mass; a.gx = ctx->gx; a.gy = ctx->gy; a.fcm = ctx->fcm;
+    a.geo = ctx->geo64; a.inv_tau_c = 1.0 / ctx->tau_c; a.N = ctx->N;
+    a.lu64 = lu64; a.lu32 = lu32; a.piv = ctx->piv; a.singular = ctx->counter;
+    dim3 grid(nblocks(ctx->N, 128), ctx->nb);
+    hipLaunchKernelGGL(factor_kernel<P>, grid, dim3(128), 0, ctx->stream, a);
+    HIP_TRY(hipGetLastError());
+}
+
+template <int P, typename T>
+void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const CellGeo<T>* geo, const T* K,
+                  const T* bvec, const T* src, const double* flux_wall, const T* pbuf, T* vdf, int use_pbuf,
+                  int homogeneous)
+{
+    SweepArgs<T> a{};
+    a.block_bundle = ctx->block_bundle; a.bundle_ndir = ctx->bundle_ndir; a.bundle_dir_off = ctx->bundle_dir_off;
+    a.bundle_dirs = ctx->bundle_dirs; a.slot_cell = ctx->slot_cell; a.lev_off = ctx->lev_off;
+    a.lev_ptr = ctx->lev_ptr; a.lu_off = ctx->lu_off; a.piv_off = ctx->piv_off; a.lu = lu; a.piv = ctx->piv;
+    a.cxv = cx; a.cyv = cy; a.geo = geo; a.K = K; a.bvec = bvec; a.src = src; a.flux_wall = flux_wall;
+    a.pbuf = pbuf; a.vdf = vdf; a.N = ctx->N; a.F = ctx->F; a.use_pbuf = use_pbuf; a.homogeneous = homogeneous;
+    hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
+    HIP_TRY(hipGetLastError());
+}
+
+template <typename T>
+void launch_source(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, T* out)
+{
+    const int P = ctx->P;
+    const unsigned g = nblocks((int64_t)ctx->N * P, 256);
+#define SRC_CASE(PP)                                                                                 \
+    case PP:                                                                                         \
+        hipLaunchKernelGGL((source_kernel<PP, T>), dim3(g), dim3(256), 0, ctx->stream, ctx->mass, ts, qxs, \
+                           qys, out, ctx->N, ctx->cv, ctx->pi, ctx->tau_r, ctx->tau_n, ctx->vg);    \
+        break;
+    switch (P) { SRC_CASE(3) SRC_CASE(6) SRC_CASE(10) SRC_CASE(15) }
+#undef SRC_CASE
+    HIP_TRY(hipGetLastError());
+}
+}  // namespace
+
+extern "C" int pbte_factorise(pbte_ctx* ctx, int32_t precision)
+{
+    return guarded(ctx, [&]() {
+        if (precision != PBTE_FP64 && precision != PBTE_FP32) throw std::string("bad precision");
+        if (!ctx->piv) ctx->piv = ctx->alloc<uint64_t>((size_t)ctx->piv_elems);
+        double* l64 = nullptr;
+        float* l32 = nullptr;
+        if (precision == PBTE_FP64) {
+            if (!ctx->lu64) ctx->lu64 = ctx->alloc<double>((size_t)ctx->lu_elems);
+            l64 = ctx->lu64;
+        } else {
+            if (!ctx->lu32) ctx->lu32 = ctx->alloc<float>((size_t)ctx->lu_elems);
+            l32 = ctx->lu32;
+        }
+        HIP_TRY(hipMemsetAsync(ctx->counter, 0, sizeof(int32_t), ctx->stream));
+        switch (ctx->P) {
+        case 3: launch_factor<3>(ctx, l64, l32); break;
+        case 6: launch_factor<6>(ctx, l64, l32); break;
+        case 10: launch_factor<10>(ctx, l64, l32); break;
+        case 15: launch_factor<15>(ctx, l64, l32); break;
+        }
+        int32_t bad = 0;
+        HIP_TRY(hipMemcpyAsync(&bad, ctx->counter, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (precision == PBTE_FP64) ctx->have64 = true; else ctx->have32 = true;
+        return (int)bad;
+    });
+}
+
+extern "C" int pbte_sweep(pbte_ctx* ctx, int32_t precision, const double* ts, const double* qxs,
+                          const double* qys, void* vdf, const double* flux_wall, const void* pbuf,
+                          int32_t use_pbuf, int32_t homogeneous)
+{
+    return guarded(ctx, [&]() {
+        if (use_pbuf && !pbuf) throw std::string("use_pbuf without a buffer");
+        if (ctx->n_wall && !flux_wall) throw std::string("adiabatic walls need flux_wall");
+        if (precision == PBTE_FP64) {
+            if (!ctx->have64) throw std::string("FP64 factors not built; call pbte_factorise");
+            launch_source<double>(ctx, ts, qxs, qys, ctx->src64);
+            switch (ctx->P) {
+#define SW(PP) case PP: launch_sweep<PP, double>(ctx, ctx->lu64, ctx->cxv, ctx->cyv, ctx->geo64, ctx->K64, \
+                    ctx->bvec64, ctx->src64, flux_wall, (const double*)pbuf, (double*)vdf, use_pbuf, homogeneous); break;
+            SW(3) SW(6) SW(10) SW(15)
+#undef SW
+            }
+        } else if (precision == PBTE_FP32) {
+            if (!ctx->have32) throw std::string("FP32 factors not built; call pbte_factorise");
+            launch_source<float>(ctx, ts, qxs, qys, ctx->src32);
+            switch (ctx->P) {
+#define SW(PP) case PP: launch_sweep<PP, float>(ctx, ctx->lu32, ctx->cxv32, ctx->cyv32, ctx->geo32, ctx->K32, \
+                    ctx->bvec32, ctx->src32, flux_wall, (const float*)pbuf, (float*)vdf, use_pbuf, homogeneous); break;
+            SW(3) SW(6) SW(10) SW(15)
+#undef SW
+            }
+        } else {
+            throw std::string("bad precision");
+        }
+        return 0;
+    });
+}
+
+extern "C" int pbte_moments(pbte_ctx* ctx, int32_t precision, const void* vdf, double* ts, double* qxs,
+                            double* qys, double* temp, double* qx, double* qy)
+{
+    return guarded(ctx, [&]() {
+        const unsigned g = nblocks((int64_t)ctx->N * ctx->P, 256);
+        if (precision == PBTE_FP64)
+            hipLaunchKernelGGL(moments_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
+                               ctx->cxv, ctx->cyv, ctx->domega, ctx->D, ctx->N, ctx->P, ctx->cv, ts, qxs, qys,
+                               ctx->ssraw);
+        else
+            hipLaunchKernelGGL(moments_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (const float*)vdf,
+                               ctx->cxv, ctx->cyv, ctx->domega, ctx->D, ctx->N, ctx->P, ctx->cv, ts, qxs, qys,
+                               ctx->ssraw);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(integrals_kernel, dim3(nblocks(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->ssraw,
+                           qxs, qys, ctx->int_tri, ctx->N, ctx->P, ctx->cv, temp, qx, qy);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_wall_flux(pbte_ctx* ctx, int32_t precision, const void* vdf, double* flux_wall)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->n_wall) return 0;
+        const unsigned g = nblocks((int64_t)ctx->n_wall * ctx->P, 256);
+        if (precision == PBTE_FP64)
+            hipLaunchKernelGGL(wall_flux_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
+                               ctx->cxv, ctx->cyv, ctx->domega, ctx->fcm, ctx->nrm, ctx->wall_face,
+                               ctx->wall_tri, ctx->wall_lfc, ctx->n_wall, ctx->D, ctx->N, ctx->P, flux_wall);
+        else
+            hipLaunchKernelGGL(wall_flux_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (const float*)vdf,
+                               ctx->cxv, ctx->cyv, ctx->domega, ctx->fcm, ctx->nrm, ctx->wall_face,
+                               ctx->wall_tri, ctx->wall_lfc, ctx->n_wall, ctx->D, ctx->N, ctx->P, flux_wall);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_transport_residual(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys,
+                                       const double* vdf, const double* flux_wall, double* out_host)
+{
+    return guarded(ctx, [&]() {
+        if (ctx->n_wall && !flux_wall) throw std::string("adiabatic walls need flux_wall");
+        launch_source<double>(ctx, ts, qxs, qys, ctx->src64);
+        const unsigned gx = nblocks(ctx->N, 256);
+        dim3 grid(gx, ctx->D);
+        double* part = ctx->red;
+        double* out = ctx->red + (size_t)ctx->D * gx * 2;
+        if ((size_t)ctx->D * gx * 2 + 2 * (size_t)ctx->D > ctx->red_len) throw std::string("reduction buffer too small");
+#define RS(PP) case PP: hipLaunchKernelGGL(residual_kernel<PP>, grid, dim3(256), 0, ctx->stream, vdf, ctx->src64, \
+        ctx->cxv, ctx->cyv, ctx->domega, ctx->mass, ctx->gx, ctx->gy, ctx->fcm, ctx->geo64, ctx->K64, ctx->bvec64, \
+        flux_wall, 1.0 / ctx->tau_c, ctx->N, part); break;
+        switch (ctx->P) { RS(3) RS(6) RS(10) RS(15) }
+#undef RS
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(rowsum_kernel, dim3(nblocks((int64_t)ctx->D * 2, 256)), dim3(256), 0, ctx->stream,
+                           part, ctx->D, (int)gx, 2, 2, out);
+        HIP_TRY(hipGetLastError());
+        double* h = ctx->host_staging((size_t)2 * ctx->D);
+        HIP_TRY(hipMemcpyAsync(h, out, sizeof(double) * 2 * ctx->D, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int d = 0; d < ctx->D; ++d) {
+            out_host[d] = h[2 * d];
+            out_host[ctx->D + d] = h[2 * d + 1];
+        }
+        return 0;
+    });
+}
+
+extern "C" int pbte_residual_iterate(pbte_ctx* ctx, const double* temp, const double* t_old, double* res_host,
+                                     double* mass_host)
+{
+    return guarded(ctx, [&]() {
+        hipLaunchKernelGGL(iterate_residual_kernel, dim3(1), dim3(64), 0, ctx->stream, temp, t_old, ctx->N,
+                           ctx->small);
+        HIP_TRY(hipGetLastError());
+        double* h = ctx->host_staging(2);
+        HIP_TRY(hipMemcpyAsync(h, ctx->small, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (res_host) *res_host = h[0];
+        if (mass_host) *mass_host = h[1];
+        return 0;
+    });
+}
+
+extern "C" int pbte_vec_multidot(pbte_ctx* ctx, const double* Q, int64_t ld, int32_t m, const double* w,
+                                 int64_t n, double* c_host)
+{
+    return guarded(ctx, [&]() {
+        if (m <= 0) return 0;
+        if (m > 4096 || (size_t)m * kDotBlocks + m > ctx->red_len) throw std::string("too many basis vectors");
+        const int nbk = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + 255) / 256));
+        dim3 grid(nbk, (m + kRows - 1) / kRows);
+        hipLaunchKernelGGL(multidot_kernel, grid, dim3(256), 0, ctx->stream, Q, ld, m, w, n, ctx->red);
+        HIP_TRY(hipGetLastError());
+        double* out = ctx->red + (size_t)m * nbk;
+        hipLaunchKernelGGL(rowsum_kernel, dim3(nblocks(m, 256)), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1,
+                           out);
+        HIP_TRY(hipGetLastError());
+        double* h = ctx->host_staging((size_t)m);
+        HIP_TRY(hipMemcpyAsync(h, out, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::memcpy(c_host, h, sizeof(double) * m);
+        return 0;
+    });
+}
+
+extern "C" int pbte_vec_update(pbte_ctx* ctx, const double* Q, int64_t ld, int32_t m, const double* c_host,
+                               double* w, int64_t n)
+{
+    return guarded(ctx, [&]() {
+        if (m <= 0) return 0;
+        if (m > 4096) throw std::string("too many basis vectors");
+        double* h = ctx->host_staging((size_t)m);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));   // staging buffer may still feed a copy
+        std::memcpy(h, c_host, sizeof(double) * m);
+        HIP_TRY(hipMemcpyAsync(ctx->small, h, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(update_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, Q, ld, m, ctx->small,
+                           w, n);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+extern "C" int pbte_vec_axpby(pbte_ctx* ctx, double a, const double* x, double b, double* y, int64_t n)
+{
+    return guarded(ctx, [&]() {
+        hipLaunchKernelGGL(axpby_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, a, x, b, y, n);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}

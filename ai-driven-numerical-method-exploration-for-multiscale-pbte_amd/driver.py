@@ -1,0 +1,200 @@
+"""``Solver``: set up a case and run the outer iteration on the device.
+
+Same constructor, methods and record as the reference's
+``solver-python/pybte/driver.py:41-290``::
+
+    STEP = 1
+    DO
+        adiabatic walls: refresh the diffuse emission      (driver.py:150-153)
+        transport sweep                                    (ctx.sweep)
+        CIS: moments | GSIS: HoT source, trace solve, local recovery, correction
+        iterate residual, mass
+        EXIT when residual < tol or STEP >= tmax
+    END DO
+
+The distribution and moments stay in HBM for the whole run; ``vdf`` and the
+``Moments`` attributes copy to the host on access.
+"""
+from __future__ import annotations
+
+import hashlib
+import time
+from pathlib import Path
+
+import numpy as np
+
+from .config import Case
+from .device import dev_zeros, require_gpu, torch
+from .io_output import RunRecord
+from .moments import Moments
+from .problem import build_problem
+from .sweep import SweepContext
+
+__all__ = ["Solver"]
+
+
+class Solver:
+    def __init__(self, case: Case, validate_order: bool = False, on_cycle: str | None = None,
+                 max_bundle: int = 0, block: int = 0):
+        t0 = time.perf_counter()
+        require_gpu()
+        self.case = c = case
+        pb = build_problem(c, on_cycle=on_cycle, validate_order=validate_order)
+        self.problem = pb
+        self.mesh, self.basis, self.vel = pb.mesh, pb.basis, pb.vel
+        self.integrals, self.order, self.bcdata = pb.integrals, pb.order, pb.bcdata
+        self.ndir, self.n_tris, self.ndof = self.vel.ndir, self.mesh.n_tris, c.ndof_tri
+        self.ctx = SweepContext.build(c, self.mesh, self.integrals, self.vel, self.order,
+                                      self.bcdata, max_bundle=max_bundle, block=block)
+        self.factorisation_count = self.ndir * self.n_tris
+        self.precision = c.performance.precision
+        self.vdf_d = dev_zeros((self.ndir, self.n_tris, self.ndof))
+        self.vdf32_d = (dev_zeros((self.ndir, self.n_tris, self.ndof), dtype=torch.float32)
+                        if self.precision == "fp32" else None)
+        self.mom = Moments.zeros(self.ndof, self.n_tris)
+        self.mom._ctx = self.ctx
+        self.t_old_d = dev_zeros(self.n_tris)
+        self.restart_info = self._apply_restart()
+        self.acc = None
+        if c.scheme.accflag == 1:
+            from .acceleration import Acceleration
+            self.acc = Acceleration(self)
+        torch.cuda.synchronize()
+        self.setup_seconds = time.perf_counter() - t0
+
+    # -- state views ---------------------------------------------------------
+    @property
+    def sweep_vdf(self):
+        """The device tensor the sweeps run on (binary32 in fp32 mode)."""
+        return self.vdf32_d if self.precision == "fp32" else self.vdf_d
+
+    @property
+    def vdf(self) -> np.ndarray:
+        """Host copy of the distribution ``[D, N, P]`` (binary64)."""
+        return self.sweep_vdf.detach().to(torch.float64).cpu().numpy()
+
+    @vdf.setter
+    def vdf(self, value) -> None:
+        self.sweep_vdf.copy_(torch.as_tensor(np.ascontiguousarray(value)).to(self.sweep_vdf.dtype))
+
+    def vdf_fortran(self) -> np.ndarray:
+        npole, nazim = self.case.velmesh.npole, self.case.velmesh.nazim
+        return np.transpose(self.vdf.reshape(nazim, npole, self.n_tris, self.ndof), (3, 2, 1, 0))
+
+    # -- restart (driver.py:100-139) ------------------------------------------
+    def _restart_stem(self) -> str:
+        c = self.case
+        return (f"VDFP{c.dg.deg}T{self.n_tris:3d}"
+                f"NP{c.velmesh.npole:2d}NA{c.velmesh.nazim:2d}").replace(" ", "")
+
+    def _apply_restart(self) -> dict:
+        r = self.case.restart
+        default = Path(self.case.basedir) / f"{self._restart_stem()}.out"
+        if not r.enabled:
+            if r.error_on_stale and default.exists():
+                raise FileExistsError(f"a restart file {default} is present but restart is "
+                                      "disabled; delete it or set restart.enabled / "
+                                      "restart.error_on_stale explicitly")
+            return {"enabled": False, "path": None, "sha256": None}
+        path = Path(r.path) if r.path else default
+        if not path.is_absolute():
+            path = Path(self.case.basedir) / path
+        if not path.exists():
+            raise FileNotFoundError(f"restart enabled but {path} does not exist")
+        raw = path.read_bytes()
+        want = self.ndof * self.n_tris * self.ndir * 8
+        if len(raw) != want:
+            raise ValueError(f"{path}: {len(raw)} bytes, expected {want} "
+                             "(wrong DEG / mesh / angular resolution?)")
+        arr = np.frombuffer(raw, dtype="<f8").reshape(
+            (self.ndof, self.n_tris, self.case.velmesh.npole, self.case.velmesh.nazim), order="F")
+        self.vdf = np.transpose(arr, (3, 2, 1, 0)).reshape(self.ndir, self.n_tris, self.ndof)
+        return {"enabled": True, "path": str(path), "sha256": hashlib.sha256(raw).hexdigest()}
+
+    def save_restart(self, path=None) -> Path:
+        path = Path(path) if path else Path(self.case.basedir) / f"{self._restart_stem()}.out"
+        path.parent.mkdir(parents=True, exist_ok=True)
+        path.write_bytes(self.vdf_fortran().ravel(order="F").tobytes())
+        return path
+
+    # -- one outer iteration (driver.py:149-161) -------------------------------
+    def step(self) -> None:
+        v = self.sweep_vdf
+        if self.bcdata.has_nonthermalising:
+            self.ctx.wall_flux(v)
+        self.ctx.sweep(self.mom, v)
+        if self.acc is None:
+            self.ctx.moments(v, self.mom)
+        else:
+            self.acc.apply()
+
+    def run(self, callback=None, progress_every: int = 0) -> RunRecord:
+        if self.case.scheme.method == "krylov":
+            from .krylov import run_krylov
+            return run_krylov(self, callback, progress_every)
+        return self._run_source(callback, progress_every)
+
+    def _run_source(self, callback=None, progress_every: int = 0) -> RunRecord:
+        c = self.case
+        tol, tmax = c.iteration.tol, c.iteration.tmax
+        want_true = c.iteration.true_residual
+        every = max(1, int(c.iteration.true_residual_every))
+        residuals, masses, true_res = [], [], []
+        t0 = time.perf_counter()
+        step = 0
+        converged = False
+        while True:
+            step += 1
+            self.step()
+            res, mass = self.ctx.residual_iterate(self.mom.temp_d, self.t_old_d)
+            self.t_old_d.copy_(self.mom.temp_d)
+            residuals.append(res)
+            masses.append(mass)
+            if want_true:
+                true_res.append(self.ctx.true_residual(self.mom, self.sweep_vdf)
+                                if step % every == 0 else np.nan)
+            if progress_every and step % progress_every == 0:
+                print(f"  step {step:8d}  residual {res:.6e}  mass {mass:.10f}", flush=True)
+            if callback is not None:
+                callback(step, res, self)
+            if res < tol:
+                if (self.acc is not None and self.acc.omega and not self.acc.defect_every
+                        and self.acc.refresh_defect() > tol):
+                    continue
+                converged = True
+                break
+            if step >= tmax:
+                break
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return self._record(step, converged, residuals, masses,
+                            true_res if want_true else None, wall,
+                            scheme="GSIS" if c.scheme.accflag == 1 else "CIS")
+
+    def _record(self, iterations, converged, residuals, masses, true_res, wall, scheme,
+                extra=None) -> RunRecord:
+        c = self.case
+        mom = self.mom
+        rec = RunRecord(
+            iterations=iterations, converged=converged,
+            residual_history=np.asarray(residuals),
+            residual_true=np.asarray(true_res) if true_res is not None else None,
+            mass_history=np.asarray(masses), temp=mom.temp, qx=mom.qx, qy=mom.qy,
+            temp_dofs=mom.ts, qx_dofs=mom.qxs, qy_dofs=mom.qys,
+            sweep_count=self.ctx.sweep_count, factorisation_count=self.factorisation_count,
+            wall_clock=wall, setup_wall_clock=self.setup_seconds, config_hash=c.config_hash,
+            scheme=scheme, acc_variant=c.scheme.acc_variant if c.scheme.accflag == 1 else None,
+            tol=c.iteration.tol, tmax=c.iteration.tmax, tau_r=c.flow.tau_r, tau_n=c.flow.tau_n,
+            tau_c=c.tau_c, deg=c.dg.deg, npole=c.velmesh.npole, nazim=c.velmesh.nazim,
+            n_tris=self.n_tris, n_faces=self.mesh.n_faces, mesh_file=str(c.mesh_path()),
+            restart=self.restart_info, environment=RunRecord.environment_fingerprint(),
+            diagnostics={"sweep_mode": self.ctx.mode, "precision": self.precision,
+                         "cycles_broken": int(self.order.cycles_broken.sum()),
+                         "hmin": float(self.mesh.hmin), "n_faces_b": int(self.mesh.n_faces_b),
+                         "boundary_types": [b.type for b in c.boundaries],
+                         **{f"schedule_{k}": v for k, v in self.ctx.schedule_info().items()}})
+        if self.acc is not None:
+            rec.diagnostics.update(self.acc.diagnostics())
+        if extra:
+            rec.diagnostics.update(extra)
+        return rec

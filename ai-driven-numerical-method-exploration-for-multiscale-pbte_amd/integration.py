@@ -1,0 +1,292 @@
+"""Precomputed element and face integrals of the nodal DG discretisation.
+
+Index orders are the reference's (``solver-python/pybte/integration.py:159-196``,
+``docs/INDEXING.md:41-56``)::
+
+    int_tri[L, I]               int_tri_tri[M, L, I]
+    int_tri_tri_x[I, M, L]      int_tri_tri_y[I, M, L]
+    int_tri_fc[I, IL, M, L]     int_tri_fc_fc[M, L, I, IL]
+    int_fc_fc[M, L, F]          int_fc[M, F]
+    int_tri_tri_fc[I, IL, M, L]
+
+Two techniques, as in the reference (``integration.py:199-404``):
+
+* analytic monomial moments ``int_T xi^a eta^b = a! b!/(a+b+2)!`` (and edge
+  analogues) contracted with the basis coefficients -- mass, derivative and
+  same-element edge tensors;
+* a 15-point Gauss-Legendre rule along each face -- the element x face-basis
+  tensor and the element x neighbour tensor (``int_tri_tri_fc``); on a wall
+  face the latter holds ``int phi_M ds`` broadcast over ``L``.
+
+The contraction loops keep the reference's summation orders (they matter in
+the last bit); the per-element work is vectorised over elements in chunks,
+which changes no arithmetic.  Monomial powers use ``numpy.power`` (libm
+``pow`` for exponents >= 3, correctly rounded products below).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .basis import Basis
+from .constants import BC_PERIODIC
+from .mesh import Mesh
+from .quadrature import gauss_legendre_core, tri_quadrature
+
+__all__ = ["Integrals", "build_integrals", "fac_div"]
+
+
+def fac_div(ns: int, ne: int, ds: int, de: int) -> float:
+    """``prod(ns..ne) / prod(ds..de)`` evaluated term by term."""
+    numn = ne - ns + 1
+    numd = de - ds + 1
+    ans = 1.0
+    for i in range(1, min(numn, numd) + 1):
+        ans = ans * float(ns + i - 1) / float(ds + i - 1)
+    for i in range(numd + 1, numn + 1):
+        ans = ans * float(ns + i - 1)
+    for i in range(numn + 1, numd + 1):
+        ans = ans / float(ds + i - 1)
+    return ans
+
+
+def _affine(x, y):
+    """Physical -> reference map ``xi = b x + c y + d``, ``eta = f x + g y + h``."""
+    x1, x2, x3 = x[..., 0], x[..., 1], x[..., 2]
+    y1, y2, y3 = y[..., 0], y[..., 1], y[..., 2]
+    A = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1)
+    B = y3 - y1
+    C = x1 - x3
+    D = (x3 - x1) * y1 - (y3 - y1) * x1
+    E = (x3 - x1) * (y2 - y1) - (x2 - x1) * (y3 - y1)
+    F = y2 - y1
+    G = x1 - x2
+    H = (x2 - x1) * y1 - (y2 - y1) * x1
+    return (B / A, C / A, D / A), (F / E, G / E, H / E)
+
+
+def _pow(x, e: int):
+    if e == 0:
+        return np.ones_like(x)
+    if e == 1:
+        return x.copy()
+    return np.power(x, float(e))
+
+
+def _eval_mono(C, mono, xi, eta):
+    """``sum_k C[m,k] xi^a_k eta^b_k`` accumulated in ascending ``k``."""
+    out = np.zeros(np.broadcast(xi, eta).shape + (C.shape[0],))
+    for k, (a, b) in enumerate(mono):
+        out += (C[:, k] * _pow(xi, int(a))[..., None]) * _pow(eta, int(b))[..., None]
+    return out
+
+
+def _contract2(C, E, il_outer: bool):
+    n, m = C.shape[0], E.shape[0]
+    out = np.zeros((n, n))
+    if il_outer:
+        for il in range(m):
+            for im in range(m):
+                out += (C[:, il][None, :] * C[:, im][:, None]) * E[im, il]
+    else:
+        for im in range(m):
+            for il in range(m):
+                out += (C[:, im][:, None] * C[:, il][None, :]) * E[im, il]
+    return out
+
+
+def _contract1(C, e):
+    out = np.zeros(C.shape[0])
+    for im in range(e.size):
+        out += C[:, im] * e[im]
+    return out
+
+
+@dataclass
+class Integrals:
+    int_tri: np.ndarray
+    int_tri_tri: np.ndarray
+    int_tri_tri_x: np.ndarray
+    int_tri_tri_y: np.ndarray
+    int_tri_fc: np.ndarray
+    int_tri_fc_fc: np.ndarray
+    int_fc_fc: np.ndarray
+    int_fc: np.ndarray
+    int_tri_tri_fc: np.ndarray
+    qua_abs_x: np.ndarray
+    qua_abs_y: np.ndarray
+    qua_wei: np.ndarray
+    nodfun_qua_p: np.ndarray
+    ref_tri: np.ndarray
+    ref_tri_tri: np.ndarray
+
+
+def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
+                    bc_codes=None, bc_xoff=None, bc_yoff=None,
+                    chunk: int = 16384) -> Integrals:
+    P = basis.ndof_tri
+    Q = basis.ndof_fc
+    deg = basis.deg
+    mono = basis.mono
+    C = basis.nodfun_tri
+    Cf = basis.nodfun_fc
+    nodes = mesh.nodes
+    n_tris = mesh.n_tris
+    area2 = 2.0 * mesh.tri_area
+    ix, iy = mono[:, 0], mono[:, 1]
+
+    e_vec = np.array([fac_div(1, int(iy[m]), int(ix[m]) + 1, int(ix[m] + iy[m]) + 2)
+                      for m in range(P)])
+    e_mass = np.empty((P, P))
+    e_dx = np.zeros((P, P))
+    e_dy = np.zeros((P, P))
+    e_e1 = np.zeros((P, P))
+    e_e2 = np.empty((P, P))
+    e_e3 = np.zeros((P, P))
+    sqrt2 = np.sqrt(2.0)
+    for m in range(P):
+        im, jm = int(ix[m]), int(iy[m])
+        for l in range(P):
+            il, jl = int(ix[l]), int(iy[l])
+            e_mass[m, l] = fac_div(1, jl + jm, il + im + 1, il + im + jl + jm + 2)
+            if im > 0:
+                e_dx[m, l] = im * fac_div(1, jl + jm, il + im, il + im - 1 + jl + jm + 2)
+            if jm > 0:
+                e_dy[m, l] = jm * fac_div(1, jl + jm - 1, il + im + 1,
+                                          il + im - 1 + jl + jm + 2)
+            if jm == 0 and jl == 0:
+                e_e1[m, l] = 1.0 / float(im + il + 1)
+            e_e2[m, l] = sqrt2 * fac_div(1, jm + jl, im + il + 1, im + il + jm + jl + 1)
+            if im == 0 and il == 0:
+                e_e3[m, l] = 1.0 / float(jm + jl + 1)
+
+    ref_tri = _contract1(C, e_vec)
+    ref_tri_tri = _contract2(C, e_mass, True)
+    ref_dx = _contract2(C, e_dx, True)
+    ref_dy = _contract2(C, e_dy, True)
+    r1 = _contract2(C, e_e1, False)
+    r2 = _contract2(C, e_e2, False)
+    r3 = _contract2(C, e_e3, False)
+
+    int_tri = ref_tri[:, None] * area2[None, :]
+    int_tri_tri = ref_tri_tri[:, :, None] * area2[None, None, :]
+
+    x = nodes[mesh.tri_nodes, 0]
+    y = nodes[mesh.tri_nodes, 1]
+    x1, x2, x3 = x[:, 0], x[:, 1], x[:, 2]
+    y1, y2, y3 = y[:, 0], y[:, 1], y[:, 2]
+    ss = (x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1)
+    int_tri_tri_x = (((y3 - y1) / ss)[:, None, None] * ref_dx[None]
+                     + ((y1 - y2) / ss)[:, None, None] * ref_dy[None]) * area2[:, None, None]
+    int_tri_tri_y = (((x1 - x3) / ss)[:, None, None] * ref_dx[None]
+                     + ((x2 - x1) / ss)[:, None, None] * ref_dy[None]) * area2[:, None, None]
+
+    len12 = np.sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1))
+    len23 = np.sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2))
+    len31 = np.sqrt((x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1))
+    int_tri_fc = np.empty((n_tris, 3, P, P))
+    int_tri_fc[:, 0] = len12[:, None, None] * r1[None]
+    int_tri_fc[:, 1] = (len23 / sqrt2)[:, None, None] * r2[None]
+    int_tri_fc[:, 2] = len31[:, None, None] * r3[None]
+
+    e_line = np.zeros((Q, Q))
+    for m in range(deg + 1):
+        for l in range(deg + 1):
+            if (m + l) % 2 == 0:
+                e_line[m, l] = 2.0 / float(m + l + 1)
+    ref_fc_fc = _contract2(Cf, e_line, False)
+    ref_fc = _contract1(Cf, e_line[:, 0])
+    half_len = mesh.face_len * 0.5
+    int_fc_fc = ref_fc_fc[:, :, None] * half_len[None, None, :]
+    int_fc = ref_fc[:, None] * half_len[None, :]
+
+    # -- face rules ---------------------------------------------------------
+    yq, lpq = gauss_legendre_core(np_fc)
+    n1, n2 = np_fc, np_fc + 1
+
+    def face_rule(length):
+        absc = (0.0 * (1.0 - yq) + length[:, None] * (1.0 + yq)) / 2.0
+        weig = n2 * n2 * (length[:, None] - 0.0) / ((1.0 - yq * yq) * lpq * lpq) / n1 / n1
+        return absc, weig
+
+    bc_codes = np.asarray(bc_codes) if bc_codes is not None else np.zeros(0, dtype=int)
+    bc_xoff = np.asarray(bc_xoff, dtype=float) if bc_xoff is not None else np.zeros(0)
+    bc_yoff = np.asarray(bc_yoff, dtype=float) if bc_yoff is not None else np.zeros(0)
+
+    # the element across each face: interior neighbour, or for a periodic
+    # face the element behind the partner face shifted by the partner's offset
+    adj_tri, _ = mesh.adjacent_arrays()
+    int_tri_fc_fc = np.zeros((P, Q, n_tris, 3))
+    int_tri_tri_fc = np.zeros((n_tris, 3, P, P))
+    for j2 in range(3):
+        fcid = mesh.tri_faces[:, j2].astype(np.int64)
+        fbc = mesh.face_bc[fcid].astype(np.int64)
+        per = np.zeros(n_tris, dtype=bool)
+        if bc_codes.size:
+            per = (fbc >= 0) & (bc_codes[np.clip(fbc, 0, None)] == BC_PERIODIC)
+        interior = fbc < 0
+        t1 = mesh.face_tri[fcid, 0]
+        t2 = mesh.face_tri[fcid, 1]
+        other = np.where(t1 == np.arange(n_tris), t2, t1).astype(np.int64)
+        ox = np.zeros(n_tris)
+        oy = np.zeros(n_tris)
+        if per.any():
+            pair = mesh.face_pair[fcid[per]].astype(np.int64)
+            other[per] = adj_tri[pair]
+            bcp = mesh.face_bc[pair].astype(np.int64)
+            ox[per] = bc_xoff[bcp]
+            oy[per] = bc_yoff[bcp]
+        for s0 in range(0, n_tris, chunk):
+            sl = slice(s0, min(n_tris, s0 + chunk))
+            f = fcid[sl]
+            ln = mesh.face_len[f]
+            absc, weig = face_rule(ln)
+            fx1 = nodes[mesh.face_nodes[f, 0], 0][:, None]
+            fy1 = nodes[mesh.face_nodes[f, 0], 1][:, None]
+            fx2 = nodes[mesh.face_nodes[f, 1], 0][:, None]
+            fy2 = nodes[mesh.face_nodes[f, 1], 1][:, None]
+            lnc = ln[:, None]
+            xp = (fx2 - fx1) / lnc * absc + fx1
+            yp = (fy2 - fy1) / lnc * absc + fy1
+            (bA, cA, dA), (fE, gE, hE) = _affine(x[sl], y[sl])
+            xi = bA[:, None] * xp + cA[:, None] * yp + dA[:, None]
+            eta = fE[:, None] * xp + gE[:, None] * yp + hE[:, None]
+            pm = _eval_mono(C, mono, xi, eta)                  # (c, q, P)
+            t = 2.0 * absc / lnc - 1.0
+            pl = np.zeros(t.shape + (Q,))
+            for k in range(Q):
+                pl += Cf[:, k] * _pow(t, k)[..., None]
+            acc = np.zeros((pm.shape[0], P, Q))
+            for q in range(np_fc):
+                acc += (pm[:, q, :, None] * pl[:, q, None, :]) * weig[:, q, None, None]
+            int_tri_fc_fc[:, :, sl, j2] = np.transpose(acc, (1, 2, 0))
+
+            # element x neighbour (or wall) tensor
+            two = (interior | per)[sl]
+            acc2 = np.zeros((pm.shape[0], P, P))
+            if two.any():
+                o = other[sl]
+                xo = nodes[mesh.tri_nodes[o], 0] + ox[sl][:, None]
+                yo = nodes[mesh.tri_nodes[o], 1] + oy[sl][:, None]
+                (bB, cB, dB), (fF, gF, hF) = _affine(xo, yo)
+                xi2 = bB[:, None] * xp + cB[:, None] * yp + dB[:, None]
+                eta2 = fF[:, None] * xp + gF[:, None] * yp + hF[:, None]
+                pn = _eval_mono(C, mono, xi2, eta2)
+                for q in range(np_fc):
+                    acc2 += (pm[:, q, :, None] * pn[:, q, None, :]) * weig[:, q, None, None]
+            if (~two).any():
+                accw = np.zeros((pm.shape[0], P))
+                for q in range(np_fc):
+                    accw += pm[:, q, :] * weig[:, q, None]
+                acc2 = np.where(two[:, None, None], acc2, accw[:, :, None])
+            int_tri_tri_fc[sl, j2] = acc2
+
+    qax, qay, qw = tri_quadrature(np_tri)
+    nodfun_qua_p = _eval_mono(C, mono, qax, qay).T
+    return Integrals(int_tri=int_tri, int_tri_tri=int_tri_tri,
+                     int_tri_tri_x=int_tri_tri_x, int_tri_tri_y=int_tri_tri_y,
+                     int_tri_fc=int_tri_fc, int_tri_fc_fc=int_tri_fc_fc,
+                     int_fc_fc=int_fc_fc, int_fc=int_fc, int_tri_tri_fc=int_tri_tri_fc,
+                     qua_abs_x=qax, qua_abs_y=qay, qua_wei=qw, nodfun_qua_p=nodfun_qua_p,
+                     ref_tri=ref_tri, ref_tri_tri=ref_tri_tri)

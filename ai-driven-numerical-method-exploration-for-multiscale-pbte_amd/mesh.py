@@ -1,0 +1,380 @@
+"""Triangle meshes: gmsh 2.2 ASCII reading, face numbering, connectivity,
+geometry, and writers for the structured / jittered unit-square meshes.
+
+Face numbering follows ``solver-python/pybte/mesh/topology.py:125-274``:
+elements are visited in file order; a line element appends a boundary face
+(tagged with the *last* matching boundary index); a triangle appends each of
+its edges not yet present as an undirected node pair.  Triangle ids are the
+tie-breaker of the sweep order, so the numbering is load-bearing.
+
+Geometry follows ``mesh/geometry.py:35-79``: signed area
+``0.5(x3y1 - x2y1 + x1y2 - x3y2 - x1y3 + x2y3)``, outward unit normal of edge
+``j`` (node j -> node j+1) ``(-dy, dx)/L`` with ``d = p_j - p_{j+1}``, and
+heights ``2A/L``.
+
+The writers follow the reference's mesh family convention
+(``solver-python/tools/make_meshes.py:60-140``): nodes numbered x-major
+(``id = i*(n+1) + j + 1``), boundary lines south (+x), north (-x), east (+y),
+west (-y) with physical ids 11..14, and per-quad diagonals chosen by
+``(2i >= n) == (2j >= n)`` (on ``j`` only for the periodic channel).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from .constants import BC_SYMMETRY, BC_THERMALISING
+
+__all__ = ["Mesh", "GmshMesh", "read_gmsh22", "build_mesh", "triangle_geometry",
+           "write_square_mesh", "square_mesh_text"]
+
+LINE_ELEMENT = 1
+TRIANGLE_ELEMENT = 2
+
+
+@dataclass
+class GmshMesh:
+    nodes: np.ndarray
+    elem_type: np.ndarray
+    elem_phyid: np.ndarray
+    elem_nodes: list
+    physical_names: dict
+
+
+def read_gmsh22(path) -> GmshMesh:
+    """Read ``$Nodes``/``$Elements`` (types 1 and 2 only, >= 2 tags)."""
+    path = Path(path)
+    lines = path.read_text().splitlines()
+    nodes = None
+    etype_l, phy_l, en_l = [], [], []
+    names = {}
+    i, n = 0, len(lines)
+    while i < n:
+        tag = lines[i].strip()
+        if tag == "$MeshFormat":
+            ver = lines[i + 1].split()
+            if not ver[0].startswith("2.2"):
+                raise ValueError(f"{path}: expected gmsh ASCII 2.2, got {ver[0]}")
+            if ver[1] != "0":
+                raise ValueError(f"{path}: binary gmsh files are not supported")
+            i += 2
+        elif tag == "$PhysicalNames":
+            cnt = int(lines[i + 1])
+            for k in range(cnt):
+                p = lines[i + 2 + k].split(None, 2)
+                names[(int(p[0]), int(p[1]))] = p[2].strip().strip('"')
+            i += 2 + cnt
+        elif tag == "$Nodes":
+            cnt = int(lines[i + 1])
+            block = np.array([lines[i + 2 + k].split()[:3] for k in range(cnt)],
+                             dtype=np.float64).reshape(cnt, 3)
+            ids = block[:, 0].astype(np.int64)
+            if not np.array_equal(ids, np.arange(1, cnt + 1)):
+                raise ValueError(f"{path}: node ids are not 1..{cnt} in order")
+            nodes = np.ascontiguousarray(block[:, 1:3])
+            i += 2 + cnt
+        elif tag == "$Elements":
+            cnt = int(lines[i + 1])
+            for k in range(cnt):
+                p = lines[i + 2 + k].split()
+                et, ntags = int(p[1]), int(p[2])
+                if et not in (LINE_ELEMENT, TRIANGLE_ELEMENT):
+                    raise ValueError(f"{path}: element {p[0]} has unsupported type {et}")
+                if ntags < 2:
+                    raise ValueError(f"{path}: element {p[0]} has {ntags} tags, need >= 2")
+                nn = 2 if et == LINE_ELEMENT else 3
+                etype_l.append(et)
+                phy_l.append(int(p[3]))
+                en_l.append(tuple(int(v) - 1 for v in p[3 + ntags:3 + ntags + nn]))
+            i += 2 + cnt
+        else:
+            i += 1
+    if nodes is None:
+        raise ValueError(f"{path}: no $Nodes section")
+    return GmshMesh(nodes=nodes, elem_type=np.asarray(etype_l, dtype=np.int64),
+                    elem_phyid=np.asarray(phy_l, dtype=np.int64), elem_nodes=en_l,
+                    physical_names=names)
+
+
+def triangle_geometry(nodes: np.ndarray, tri_nodes: np.ndarray):
+    """``(area, normal[n,3,2], hmin_per_tri, hmin_global)``."""
+    x = nodes[tri_nodes, 0]
+    y = nodes[tri_nodes, 1]
+    x1, x2, x3 = x[:, 0], x[:, 1], x[:, 2]
+    y1, y2, y3 = y[:, 0], y[:, 1], y[:, 2]
+    area = 0.5 * (x3 * y1 - x2 * y1 + x1 * y2 - x3 * y2 - x1 * y3 + x2 * y3)
+    if np.any(area < 0):
+        bad = np.flatnonzero(area < 0)
+        raise ValueError(f"{bad.size} triangles have negative area; first: {bad[:5].tolist()}")
+    xn = np.concatenate([x, x[:, :1]], axis=1)
+    yn = np.concatenate([y, y[:, :1]], axis=1)
+    dx = xn[:, :3] - xn[:, 1:]
+    dy = yn[:, :3] - yn[:, 1:]
+    length = np.sqrt(dx * dx + dy * dy)
+    normal = np.empty((tri_nodes.shape[0], 3, 2))
+    normal[:, :, 0] = -dy / length
+    normal[:, :, 1] = dx / length
+    heights = 2.0 * area[:, None] / np.sqrt(dx ** 2 + dy ** 2)
+    return area, normal, heights.min(axis=1), min(1.0, float(heights.min()))
+
+
+@dataclass
+class Mesh:
+    nodes: np.ndarray
+    tri_nodes: np.ndarray
+    tri_faces: np.ndarray
+    tri_area: np.ndarray
+    tri_normal: np.ndarray
+    tri_hmin: np.ndarray
+    hmin: float
+    face_nodes: np.ndarray
+    face_bc: np.ndarray
+    face_pair: np.ndarray
+    face_tri: np.ndarray
+    face_lfc: np.ndarray
+    face_len: np.ndarray
+    tri_neighbour: np.ndarray
+    bc_codes: np.ndarray
+
+    @property
+    def n_nodes(self) -> int:
+        return self.nodes.shape[0]
+
+    @property
+    def n_tris(self) -> int:
+        return self.tri_nodes.shape[0]
+
+    @property
+    def n_faces(self) -> int:
+        return self.face_nodes.shape[0]
+
+    @property
+    def n_faces_b(self) -> int:
+        return int(np.count_nonzero(self.face_bc >= 0))
+
+    @property
+    def n_faces_i(self) -> int:
+        return self.n_faces - self.n_faces_b
+
+    def adjacent(self, face: int):
+        """``(triangle, local edge)`` touching a boundary face; the (+) side
+        is preferred, as in ``topology.py:115-122``."""
+        if self.face_tri[face, 0] < 0:
+            return int(self.face_tri[face, 1]), int(self.face_lfc[face, 1])
+        return int(self.face_tri[face, 0]), int(self.face_lfc[face, 0])
+
+    def adjacent_arrays(self):
+        """Vectorised :meth:`adjacent` over all faces."""
+        minus = self.face_tri[:, 0] < 0
+        tri = np.where(minus, self.face_tri[:, 1], self.face_tri[:, 0])
+        lfc = np.where(minus, self.face_lfc[:, 1], self.face_lfc[:, 0])
+        return tri.astype(np.int64), lfc.astype(np.int64)
+
+
+def build_mesh(path, bc_phyid, bc_codes, bc_names=None, bc_xoff=None, bc_yoff=None) -> Mesh:
+    g = read_gmsh22(path)
+    nbc = len(bc_phyid)
+    bc_phyid = np.asarray(bc_phyid, dtype=np.int64)
+    bc_codes = np.asarray(bc_codes, dtype=np.int64)
+    bc_names = list(bc_names) if bc_names is not None else [""] * nbc
+    bc_xoff = np.asarray(bc_xoff if bc_xoff is not None else np.zeros(nbc), dtype=float)
+    bc_yoff = np.asarray(bc_yoff if bc_yoff is not None else np.zeros(nbc), dtype=float)
+
+    face_nodes: list = []
+    face_bc: list = []
+    key_to_face: dict = {}
+    tri_nodes: list = []
+    for et, phy, nds in zip(g.elem_type, g.elem_phyid, g.elem_nodes):
+        if et == LINE_ELEMENT:
+            a, b = nds
+            bc = -1
+            for j in range(nbc):
+                if phy == bc_phyid[j]:
+                    bc = j
+            key = (a, b) if a < b else (b, a)
+            if key in key_to_face:
+                raise ValueError(f"{path}: duplicate boundary edge {(a + 1, b + 1)}")
+            key_to_face[key] = len(face_nodes)
+            face_nodes.append((a, b))
+            face_bc.append(bc)
+        else:
+            tri_nodes.append(nds)
+            for j in range(3):
+                a, b = nds[j], nds[(j + 1) % 3]
+                key = (a, b) if a < b else (b, a)
+                if key not in key_to_face:
+                    key_to_face[key] = len(face_nodes)
+                    face_nodes.append((a, b))
+                    face_bc.append(-1)
+    n_tris = len(tri_nodes)
+    if n_tris == 0:
+        raise ValueError(f"{path}: no triangles")
+    n_faces = len(face_nodes)
+    tri_nodes_a = np.asarray(tri_nodes, dtype=np.int32)
+    face_nodes_a = np.asarray(face_nodes, dtype=np.int32)
+    face_bc_a = np.asarray(face_bc, dtype=np.int32)
+    face_pair = np.full(n_faces, -1, dtype=np.int32)
+
+    # periodic pairing between the boundaries named 'Master' and 'Slave'
+    id_master = next((k for k, s in enumerate(bc_names) if str(s).strip() == "Master"), -1)
+    id_slave = next((k for k, s in enumerate(bc_names) if str(s).strip() == "Slave"), -1)
+    if id_master >= 0:
+        if id_slave < 0:
+            raise ValueError("a 'Master' boundary needs a matching 'Slave'")
+        masters = np.flatnonzero(face_bc_a == id_master)
+        slaves = np.flatnonzero(face_bc_a == id_slave)
+        xo, yo = bc_xoff[id_master], bc_yoff[id_master]
+        P = g.nodes
+        for m in masters:
+            a, b = face_nodes_a[m]
+            pa = (P[a, 0] + xo, P[a, 1] + yo)
+            pb = (P[b, 0] + xo, P[b, 1] + yo)
+            for s in slaves:
+                c, d = face_nodes_a[s]
+                pc = (P[c, 0], P[c, 1])
+                pd = (P[d, 0], P[d, 1])
+                if (pa == pc and pb == pd) or (pa == pd and pb == pc):
+                    face_pair[m] = s
+                    face_pair[s] = m
+                    break
+        unpaired = [int(k) for k in np.concatenate([masters, slaves]) if face_pair[k] < 0]
+        if unpaired:
+            raise ValueError(f"unpaired periodic faces: {unpaired[:8]}")
+
+    area, normal, hmin_tri, hmin = triangle_geometry(g.nodes, tri_nodes_a)
+    fn = g.nodes[face_nodes_a]
+    d = fn[:, 0, :] - fn[:, 1, :]
+    face_len = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1])
+
+    tri_faces = np.full((n_tris, 3), -1, dtype=np.int32)
+    face_tri = np.full((n_faces, 2), -1, dtype=np.int32)
+    face_lfc = np.full((n_faces, 2), -1, dtype=np.int32)
+    for i, nds in enumerate(tri_nodes):
+        for j in range(3):
+            a, b = nds[j], nds[(j + 1) % 3]
+            k = key_to_face[(a, b) if a < b else (b, a)]
+            tri_faces[i, j] = k
+            if face_nodes_a[k, 0] == a and face_nodes_a[k, 1] == b:
+                face_tri[k, 0] = i
+                face_lfc[k, 0] = j
+            else:
+                face_tri[k, 1] = i
+                face_lfc[k, 1] = j
+
+    # wall / symmetry queue index overwrites face_pair (topology.py:238-243)
+    counter = 0
+    for k in range(n_faces):
+        bc = face_bc_a[k]
+        if bc >= 0 and bc_codes[bc] in (BC_THERMALISING, BC_SYMMETRY):
+            face_pair[k] = counter
+            counter += 1
+
+    t1 = face_tri[tri_faces, 0]
+    t2 = face_tri[tri_faces, 1]
+    own = np.arange(n_tris)[:, None]
+    tri_neighbour = np.where(t1 == own, t2, np.where(t2 == own, t1, -1)).astype(np.int32)
+
+    mesh = Mesh(nodes=g.nodes, tri_nodes=tri_nodes_a, tri_faces=tri_faces, tri_area=area,
+                tri_normal=normal, tri_hmin=hmin_tri, hmin=hmin, face_nodes=face_nodes_a,
+                face_bc=face_bc_a, face_pair=face_pair, face_tri=face_tri,
+                face_lfc=face_lfc, face_len=face_len, tri_neighbour=tri_neighbour,
+                bc_codes=bc_codes)
+    interior = mesh.face_bc < 0
+    if np.any((mesh.face_tri[interior] < 0).any(axis=1)):
+        bad = np.flatnonzero(interior & (mesh.face_tri < 0).any(axis=1))
+        raise ValueError(f"interior faces with a missing neighbour: {bad[:8].tolist()}")
+    boundary = ~interior
+    if np.any((mesh.face_tri[boundary] >= 0).all(axis=1)):
+        bad = np.flatnonzero(boundary & (mesh.face_tri >= 0).all(axis=1))
+        raise ValueError(f"boundary faces with two neighbours: {bad[:8].tolist()}")
+    return mesh
+
+
+# ---------------------------------------------------------------------------
+# writers
+# ---------------------------------------------------------------------------
+def _smootherstep(t):
+    return t ** 3 * (10.0 - 15.0 * t + 6.0 * t ** 2)
+
+
+_HEADER = ("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$PhysicalNames\n5\n"
+           '1 11 "{s} (physical id 11)"\n1 12 "{n} (physical id 12)"\n'
+           '1 13 "{e} (physical id 13)"\n1 14 "{w} (physical id 14)"\n'
+           '2 15 "Fluid (physical id 15)"\n$EndPhysicalNames\n')
+
+
+def square_mesh_text(n: int, layout: str = "cavity", distribution: str = "uniform",
+                     jitter: float = 0.0, seed: int = 0) -> str:
+    """gmsh 2.2 text of an ``n x n`` triangulated unit square.
+
+    ``jitter`` moves every interior node by ``U(-jitter*h, +jitter*h)`` in x and
+    y, drawn from ``numpy.random.default_rng(seed)`` in node-id order (x draw
+    then y draw per node); ``h = 1/n``.  Boundary nodes stay on the boundary
+    lines so that wall physical ids are unchanged.
+    """
+    if distribution not in ("uniform", "smootherstep"):
+        raise ValueError(f"unknown distribution {distribution!r}")
+    t = np.arange(n + 1) / n
+    cy = _smootherstep(t) if distribution == "smootherstep" else t.astype(float)
+    cx = cy.copy() if layout != "channel" else t.astype(float)
+    nn = n + 1
+    X = np.repeat(cx[:, None], nn, axis=1)          # X[i, j] = x_i
+    Y = np.repeat(cy[None, :], nn, axis=0)          # Y[i, j] = y_j
+    if jitter:
+        rng = np.random.default_rng(seed)
+        h = 1.0 / n
+        d = rng.uniform(-jitter * h, jitter * h, size=(nn, nn, 2))
+        inner = np.zeros((nn, nn), dtype=bool)
+        inner[1:n, 1:n] = True
+        X = np.where(inner, X + d[:, :, 0], X)
+        Y = np.where(inner, Y + d[:, :, 1], Y)
+    names = {"s": "Swall", "n": "NWall", "e": "EWall", "w": "WWall"}
+    if layout == "channel":
+        names = {"s": "Swall", "n": "NWall", "e": "Master", "w": "Slave"}
+    elif layout != "cavity":
+        raise ValueError(f"unknown layout {layout!r}")
+
+    def nid(i, j):
+        return i * nn + j + 1
+
+    out = [_HEADER.format(**names), "$Nodes\n", f"{nn * nn:6d}\n"]
+    ids = np.arange(1, nn * nn + 1)
+    xs = X.reshape(-1)
+    ys = Y.reshape(-1)
+    out.extend(f"{k:6d} {x:.16f} {y:.16f} 0\n" for k, x, y in zip(ids, xs, ys))
+    out.append("$EndNodes\n")
+    lines = []
+    lines += [(11, 5, nid(i, 0), nid(i + 1, 0)) for i in range(n)]
+    lines += [(12, 6, nid(i, n), nid(i - 1, n)) for i in range(n, 0, -1)]
+    lines += [(13, 7, nid(n, j), nid(n, j + 1)) for j in range(n)]
+    lines += [(14, 8, nid(0, j), nid(0, j - 1)) for j in range(n, 0, -1)]
+    flip_on_x = layout != "channel"
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            if ((2 * i >= n) if flip_on_x else False) == (2 * j >= n):
+                tris.append((nid(i, j), nid(i + 1, j), nid(i + 1, j + 1)))
+                tris.append((nid(i, j), nid(i + 1, j + 1), nid(i, j + 1)))
+            else:
+                tris.append((nid(i, j), nid(i + 1, j), nid(i, j + 1)))
+                tris.append((nid(i, j + 1), nid(i + 1, j), nid(i + 1, j + 1)))
+    out.append("$Elements\n")
+    out.append(f"{len(lines) + len(tris):6d}\n")
+    k = 0
+    for phy, geo, a, b in lines:
+        k += 1
+        out.append(f"{k:6d} 1 2 {phy} {geo} {a:5d} {b:5d}\n")
+    for a, b, c in tris:
+        k += 1
+        out.append(f"{k:6d} 2 2 15 10 {a:5d} {b:5d} {c:5d}\n")
+    out.append("$EndElements\n")
+    return "".join(out)
+
+
+def write_square_mesh(path, n: int, **kw) -> Path:
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    path.write_text(square_mesh_text(n, **kw))
+    return path

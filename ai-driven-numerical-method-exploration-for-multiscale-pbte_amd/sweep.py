@@ -1,0 +1,229 @@
+"""``SweepContext``: the operator boundary of the solve on the device.
+
+Replaces the reference's ``SweepContext`` (``solver-python/pybte/sweep.py:37-197``)
+with the same method names and side effects:
+
+* ``sweep(mom, vdf)``       one transport sweep, in place on ``vdf``,
+                            ``sweep_count += 1`` (``sweep.py:132-158``);
+* ``true_residual(mom, vdf)`` the dw-weighted ``||A f - b|| / ||b||`` with no
+                            sweep (``sweep.py:161-197``);
+* ``factorise()``           LU of every (direction, element) operator, once
+                            (``sweep.py:116-129``); singular blocks raise
+                            ``numpy.linalg.LinAlgError``.
+
+``vdf`` is a device tensor ``[D, N, P]``; its dtype selects the sweep
+precision (float64: binary64 LU and accumulation; float32: binary32 LU and
+accumulation, moments still summed in binary64).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native
+from .constants import PI
+from .device import dev_zeros, ptr, stream_handle, torch
+
+__all__ = ["SweepContext"]
+
+_PREC = {torch.float64: _native.PBTE_FP64, torch.float32: _native.PBTE_FP32}
+
+
+def _other_lfc(mesh, bcdata) -> np.ndarray:
+    """Local face index, in the element across, of each (element, face)."""
+    n = mesh.n_tris
+    out = np.full((n, 3), -1, dtype=np.int32)
+    f = mesh.tri_faces
+    t0 = mesh.face_tri[f, 0]
+    own = np.arange(n)[:, None]
+    interior = mesh.face_bc[f] < 0
+    lfc_other = np.where(t0 == own, mesh.face_lfc[f, 1], mesh.face_lfc[f, 0])
+    out[interior] = lfc_other[interior]
+    if bcdata.has_periodic:
+        adj_tri, adj_lfc = mesh.adjacent_arrays()
+        for i in range(n):
+            for il in range(3):
+                fc = int(f[i, il])
+                bc = int(mesh.face_bc[fc])
+                if bc >= 0 and bcdata.bc_type[bc] == 3:
+                    out[i, il] = adj_lfc[int(mesh.face_pair[fc])]
+    return out
+
+
+class SweepContext:
+    def __init__(self):
+        self.handle = None
+        self.sweep_count = 0
+        self.mode = "precomputed"
+        self.kernel = "hip"
+        self.use_pbuf = False
+        self.pbuf = None
+
+    # -- construction ------------------------------------------------------
+    @classmethod
+    def build(cls, case, mesh, integrals, vel, sweep_order, bcdata, precompute=None,
+              max_bytes: float | None = None, max_bundle: int = 0, block: int = 0):
+        self = cls()
+        cxv, cyv, domega = vel.flat()
+        self.cxv, self.cyv, self.domega = cxv, cyv, domega
+        self.ndir, self.n_tris, self.nd = cxv.size, mesh.n_tris, case.ndof_tri
+        self.n_faces = mesh.n_faces
+        self.cv, self.vg = float(case.flow.cv), float(case.flow.vg)
+        self.tau_r, self.tau_n, self.tau_c = float(case.flow.tau_r), float(case.flow.tau_n), case.tau_c
+        self.has_walls = bool(bcdata.has_nonthermalising)
+        self.has_periodic = bool(bcdata.has_periodic)
+        keep = {}
+
+        def f64(a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep[id(a)] = a
+            return _native.f64(a)
+
+        def i32(a):
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            keep[id(a)] = a
+            return _native.i32(a)
+
+        dims = _native.PbteDims(ndir=self.ndir, n_tris=self.n_tris, nd=self.nd, nf=case.ndof_fc,
+                                n_faces=self.n_faces, nbc=len(bcdata.bc_type))
+        su = _native.PbteSetup()
+        su.cxv, su.cyv, su.domega = f64(cxv), f64(cyv), f64(domega)
+        su.cv, su.vg, su.tau_r, su.tau_n, su.tau_c, su.pi = (self.cv, self.vg, self.tau_r,
+                                                             self.tau_n, self.tau_c, PI)
+        su.mass = f64(integrals.int_tri_tri)
+        su.gx, su.gy = f64(integrals.int_tri_tri_x), f64(integrals.int_tri_tri_y)
+        su.fcm, su.ttfc = f64(integrals.int_tri_fc), f64(integrals.int_tri_tri_fc)
+        su.nx, su.ny = f64(mesh.tri_normal[:, :, 0]), f64(mesh.tri_normal[:, :, 1])
+        su.int_tri = f64(integrals.int_tri)
+        su.tri_faces, su.face_bc = i32(mesh.tri_faces), i32(mesh.face_bc)
+        su.neighbour, su.periodic_tri = i32(mesh.tri_neighbour), i32(bcdata.periodic_tri)
+        su.other_lfc = i32(_other_lfc(mesh, bcdata))
+        su.bc_type, su.bc_temp = i32(bcdata.bc_type), f64(bcdata.bc_temp)
+        su.face_tri, su.face_lfc = i32(mesh.face_tri), i32(mesh.face_lfc)
+        su.order = i32(sweep_order.order)
+        su.max_bundle, su.block = int(max_bundle), int(block)
+        err = C.create_string_buffer(512)
+        torch.cuda.synchronize()
+        h = _native.lib().pbte_ctx_create(C.byref(dims), C.byref(su), C.c_void_p(stream_handle()),
+                                          err, 512)
+        if not h:
+            raise _native.NativeError(f"pbte_ctx_create: {err.value.decode()}")
+        self.handle = h
+        self.flux_wall = dev_zeros((max(self.n_faces, 1), self.nd))
+        self.precision = case.performance.precision
+        self.factorise("fp64")
+        if self.precision == "fp32":
+            self.factorise("fp32")
+        return self
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _native.lib().pbte_ctx_destroy(self.handle)
+            self.handle = None
+
+    def schedule_info(self) -> dict:
+        out = (C.c_int64 * 4)()
+        _native.check(self.handle, _native.lib().pbte_schedule_info(self.handle, out), "schedule")
+        return {"levels": int(out[0]), "bundles": int(out[1]), "max_width": int(out[2]),
+                "lu_elements": int(out[3])}
+
+    def operator_bytes(self, n_tris: int, ndof: int) -> float:
+        return float(self.ndir) * n_tris * ndof * ndof * 8.0
+
+    def factorise(self, precision: str = "fp64") -> None:
+        code = _native.PBTE_FP64 if precision == "fp64" else _native.PBTE_FP32
+        bad = _native.check(self.handle, _native.lib().pbte_factorise(self.handle, code), "factorise")
+        if bad:
+            raise np.linalg.LinAlgError(
+                f"{bad} element operators are singular; check TAU_C and the mesh")
+        self.mode = "precomputed"
+
+    def ensure_pbuf(self, dtype):
+        if self.pbuf is None or self.pbuf.dtype != dtype:
+            self.pbuf = dev_zeros((self.ndir, max(self.n_faces, 1), self.nd), dtype=dtype)
+        return self.pbuf
+
+    # -- operators ---------------------------------------------------------
+    def sweep(self, mom, vdf, homogeneous: bool = False, flux_wall=None, src=None) -> None:
+        """One transport sweep in place on the device tensor ``vdf``.
+
+        ``src`` optionally overrides the moment source with a tuple of device
+        ``[P, N]`` binary64 tensors ``(ts, qxs, qys)``."""
+        prec = _PREC[vdf.dtype]
+        ts, qxs, qys = src if src is not None else (mom.ts_d, mom.qxs_d, mom.qys_d)
+        fw = self.flux_wall if flux_wall is None else flux_wall
+        pb = None
+        if self.use_pbuf:
+            pb = self.ensure_pbuf(vdf.dtype)
+        st = _native.lib().pbte_sweep(self.handle, prec, ptr(ts), ptr(qxs), ptr(qys), ptr(vdf),
+                                      ptr(fw), ptr(pb), 1 if self.use_pbuf else 0,
+                                      1 if homogeneous else 0)
+        _native.check(self.handle, st, "sweep")
+        self.sweep_count += 1
+
+    def moments(self, vdf, mom, out=None) -> None:
+        """Moments of ``vdf`` into ``mom`` (or into ``out`` = 6 device tensors)."""
+        prec = _PREC[vdf.dtype]
+        o = out if out is not None else (mom.ts_d, mom.qxs_d, mom.qys_d, mom.temp_d, mom.qx_d, mom.qy_d)
+        st = _native.lib().pbte_moments(self.handle, prec, ptr(vdf), *(ptr(t) for t in o))
+        _native.check(self.handle, st, "moments")
+
+    def wall_flux(self, vdf, out=None) -> None:
+        prec = _PREC[vdf.dtype]
+        st = _native.lib().pbte_wall_flux(self.handle, prec, ptr(vdf),
+                                          ptr(self.flux_wall if out is None else out))
+        _native.check(self.handle, st, "wall_flux")
+
+    def transport_residual_parts(self, mom, vdf):
+        if vdf.dtype != torch.float64:
+            vdf = vdf.to(torch.float64)
+        out = np.zeros(2 * self.ndir)
+        st = _native.lib().pbte_transport_residual(
+            self.handle, ptr(mom.ts_d), ptr(mom.qxs_d), ptr(mom.qys_d), ptr(vdf),
+            ptr(self.flux_wall), out.ctypes.data_as(C.POINTER(C.c_double)))
+        _native.check(self.handle, st, "transport_residual")
+        return out[:self.ndir], out[self.ndir:]
+
+    def true_residual(self, mom, vdf) -> float:
+        num, den = self.transport_residual_parts(mom, vdf)
+        d = float(den.sum())
+        if d <= 0.0:
+            return float("nan")
+        return float(np.sqrt(num.sum() / d))
+
+    def residual_iterate(self, temp_d, t_old_d):
+        res = C.c_double()
+        mass = C.c_double()
+        st = _native.lib().pbte_residual_iterate(self.handle, ptr(temp_d), ptr(t_old_d),
+                                                 C.byref(res), C.byref(mass))
+        _native.check(self.handle, st, "residual_iterate")
+        return res.value, mass.value
+
+    def mass(self, mom) -> float:
+        return self.residual_iterate(mom.temp_d, mom.temp_d)[1]
+
+    # -- Krylov vector kernels --------------------------------------------
+    def multidot(self, Q, m: int, w) -> np.ndarray:
+        out = np.zeros(max(m, 1))
+        st = _native.lib().pbte_vec_multidot(self.handle, ptr(Q), Q.stride(0), m, ptr(w),
+                                             w.numel(), out.ctypes.data_as(C.POINTER(C.c_double)))
+        _native.check(self.handle, st, "multidot")
+        return out[:m]
+
+    def update(self, Q, m: int, c: np.ndarray, w) -> None:
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        st = _native.lib().pbte_vec_update(self.handle, ptr(Q), Q.stride(0), m,
+                                           c.ctypes.data_as(C.POINTER(C.c_double)), ptr(w),
+                                           w.numel())
+        _native.check(self.handle, st, "update")
+
+    def axpby(self, a: float, x, b: float, y) -> None:
+        st = _native.lib().pbte_vec_axpby(self.handle, float(a), ptr(x), float(b), ptr(y), y.numel())
+        _native.check(self.handle, st, "axpby")
+
+    def norm(self, w) -> float:
+        return float(np.sqrt(max(self.multidot(w.view(1, -1), 1, w)[0], 0.0)))

@@ -1,0 +1,154 @@
+/*
+ * pbte_hip.h -- C-ABI of the MI355X gray linear Callaway DG sweep library
+ * (libpbte_hip.so).  Plain pointers and sizes only; device pointers are
+ * allocated by the caller (PyTorch-ROCm tensors in the shipped host layer)
+ * and are never freed by the library.  A context owns only its private
+ * workspaces: element LU factors, the level schedule, compact face data.
+ *
+ * Every call is stream-ordered on the stream given to pbte_ctx_create.  A
+ * context is not thread-safe: one host thread per context.  Status codes are
+ * 0 on success, > 0 for a numerical condition the caller maps to an
+ * exception (singular element operators, broken cycles) and < 0 for misuse or
+ * a HIP error; pbte_last_error() returns the message.
+ *
+ * Each entry point cites the reference interface it replaces
+ * (solver-python/pybte/...).
+ */
+#ifndef PBTE_HIP_H
+#define PBTE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pbte_ctx pbte_ctx;
+
+/* precision selector of the per-(cell, direction) data a sweep touches */
+enum { PBTE_FP64 = 0, PBTE_FP32 = 1 };
+
+typedef struct pbte_dims {
+    int32_t ndir;     /* D = npole * nazim, d = j1 + npole * j2          */
+    int32_t n_tris;   /* N                                               */
+    int32_t nd;       /* P = ndof_tri, element DOFs (3, 6, 10, 15)       */
+    int32_t nf;       /* Q = ndof_fc = deg + 1                            */
+    int32_t n_faces;  /* F                                               */
+    int32_t nbc;      /* boundary conditions                             */
+} pbte_dims;
+
+/* Host arrays read once by pbte_ctx_create (index orders are the
+ * reference's, docs/INDEXING.md:41-56; sweep.py:37-72). */
+typedef struct pbte_setup {
+    const double *cxv, *cyv, *domega;     /* [D]                            */
+    double cv, vg, tau_r, tau_n, tau_c, pi;
+    const double *mass;                   /* int_tri_tri   [P][P][N]        */
+    const double *gx, *gy;                /* int_tri_tri_x [N][P][P]        */
+    const double *fcm;                    /* int_tri_fc    [N][3][P][P]     */
+    const double *ttfc;                   /* int_tri_tri_fc[N][3][P][P]     */
+    const double *nx, *ny;                /* outward normals [N][3]         */
+    const double *int_tri;                /* [P][N]                         */
+    const int32_t *tri_faces;             /* [N][3]                         */
+    const int32_t *face_bc;               /* [F], -1 interior              */
+    const int32_t *neighbour;             /* [N][3], -1 on boundaries      */
+    const int32_t *periodic_tri;          /* [F], element behind the pair  */
+    const int32_t *other_lfc;             /* [N][3] local face of the
+                                             element across (interior or
+                                             periodic), -1 on walls        */
+    const int32_t *bc_type;               /* [nbc] 1 thermal, 2 adiabatic,
+                                             3 periodic                    */
+    const double *bc_temp;                /* [nbc]                          */
+    const int32_t *face_tri, *face_lfc;   /* [F][2]                         */
+    const int32_t *order;                 /* reference sweep order [D][N]  */
+    int32_t max_bundle;                   /* directions per workgroup cap,
+                                             0 = automatic                 */
+    int32_t block;                        /* threads per sweep workgroup,
+                                             0 = automatic                 */
+} pbte_setup;
+
+/* ---- setup ------------------------------------------------------------ */
+
+/* Per-direction element order, bit-exact with the reference's Fortran-scan
+ * topological sort; replaces ordering._order_kernel / build_sweep_order
+ * (solver-python/pybte/ordering.py:61-175).  Host only, no device needed.
+ * Returns 0, or -1 when a cycle is found and allow_break == 0. */
+int pbte_build_order(int32_t ndir, int32_t n_tris, const int32_t *neighbour,
+                     const double *nx, const double *ny, const double *cx,
+                     const double *cy, int32_t allow_break, int32_t *order,
+                     int32_t *cycles);
+
+/* Uploads the setup, builds the level schedule and direction bundles.
+ * Replaces SweepContext.build (sweep.py:74-111).  stream is a hipStream_t
+ * (NULL = default stream).  Returns NULL on failure with the message copied
+ * into err. */
+pbte_ctx *pbte_ctx_create(const pbte_dims *dims, const pbte_setup *setup, void *stream,
+                          char *err, int32_t errlen);
+
+void pbte_ctx_destroy(pbte_ctx *ctx);
+
+const char *pbte_last_error(pbte_ctx *ctx);
+
+/* Schedule statistics: levels per bundle summed, bundles, max level width. */
+int pbte_schedule_info(pbte_ctx *ctx, int64_t *out4);
+
+/* Factorise A_SOL for every (direction, element) in the given precision.
+ * Replaces SweepContext.factorise / _kernels.build_operators
+ * (sweep.py:116-129, _kernels.py:43-148).  Returns 0, or the number of
+ * singular element operators (> 0). */
+int pbte_factorise(pbte_ctx *ctx, int32_t precision);
+
+/* One transport sweep, in place on vdf (device, [D][N][P], double for
+ * PBTE_FP64 and float for PBTE_FP32).  ts/qxs/qys: device [P][N] double.
+ * flux_wall: device [F][P] double (adiabatic emission), pbuf: device
+ * [D][F][P] in the sweep precision, read instead of the partner element when
+ * use_pbuf != 0.  homogeneous != 0 drops the thermalising wall inflow.
+ * Replaces SweepContext.sweep / _kernels.sweep_precomputed
+ * (sweep.py:132-158, _kernels.py:153-239). */
+int pbte_sweep(pbte_ctx *ctx, int32_t precision, const double *ts, const double *qxs,
+               const double *qys, void *vdf, const double *flux_wall, const void *pbuf,
+               int32_t use_pbuf, int32_t homogeneous);
+
+/* Angular moments of vdf (precision as in pbte_sweep) in binary64,
+ * directions summed in index order.  Replaces moments.compute_moments /
+ * _kernels.moments_kernel (moments.py:51-55, _kernels.py:311-338). */
+int pbte_moments(pbte_ctx *ctx, int32_t precision, const void *vdf, double *ts, double *qxs,
+                 double *qys, double *temp, double *qx, double *qy);
+
+/* Diffuse adiabatic wall emission flux_wall[F][P] (device, double).
+ * Replaces BoundaryData.update_wall_flux / _kernels.wall_flux_kernel
+ * (bc.py:47-58, _kernels.py:363-397). */
+int pbte_wall_flux(pbte_ctx *ctx, int32_t precision, const void *vdf, double *flux_wall);
+
+/* Per-direction sum_i ||A f - b||^2 dw and ||b||^2 dw in binary64 at the
+ * current state, no sweep (vdf device double [D][N][P]); out: host [2][D].
+ * Replaces SweepContext.true_residual / _kernels.transport_residual
+ * (sweep.py:161-197, _kernels.py:274-306). */
+int pbte_transport_residual(pbte_ctx *ctx, const double *ts, const double *qxs,
+                            const double *qys, const double *vdf, const double *flux_wall,
+                            double *out_host);
+
+/* sqrt(sum (T - T_old)^2 / sum T^2) and sum T in index order, device
+ * inputs [N], host outputs.  Replaces moments.residual_iterate and
+ * Moments.mass (moments.py:40-41, 58-72; _kernels.py:340-361). */
+int pbte_residual_iterate(pbte_ctx *ctx, const double *temp, const double *t_old,
+                          double *res_host, double *mass_host);
+
+/* ---- device vector kernels of the Krylov solver (krylov.py:150-246) --- */
+
+/* c[k] = sum_i Q[k][i] w[i], k < m (Q device [m][n], row stride ld);
+ * result on the host, reduction order fixed. */
+int pbte_vec_multidot(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m,
+                      const double *w, int64_t n, double *c_host);
+
+/* w[i] -= sum_k c[k] Q[k][i] (c from the host). */
+int pbte_vec_update(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m,
+                    const double *c_host, double *w, int64_t n);
+
+/* y[i] = a x[i] + b y[i] */
+int pbte_vec_axpby(pbte_ctx *ctx, double a, const double *x, double b, double *y, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBTE_HIP_H */
