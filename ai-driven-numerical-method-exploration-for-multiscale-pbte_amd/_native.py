@@ -37,6 +37,18 @@ class PbteSetup(C.Structure):
                 ("order", _i32p), ("max_bundle", C.c_int32), ("block", C.c_int32)]
 
 
+class PbteGsisData(C.Structure):
+    _fields_ = [("cxv", _f64p), ("cyv", _f64p), ("domega", _f64p),
+                ("inv_aa_sol", _f64p), ("aa_trace", _f64p), ("ba_sol", _f64p),
+                ("tri_fc_fc", _f64p), ("tri_hmin", _f64p), ("tri_faces", _i32p),
+                ("t1", _i32p), ("l1", _i32p), ("t2", _i32p), ("l2", _i32p),
+                ("flip", C.POINTER(C.c_uint8)), ("n_wall", C.c_int32), ("n_wall_th", C.c_int32),
+                ("wall_face", _i32p), ("wall_tri", _i32p), ("wall_lfc", _i32p),
+                ("wall_nx", _f64p), ("wall_ny", _f64p),
+                ("tc5", C.c_double), ("tau_thr", C.c_double), ("drift", C.c_double),
+                ("variant_b", C.c_int32), ("use_hmin", C.c_int32)]
+
+
 _SIGS = {
     "pbte_build_order": (C.c_int, [C.c_int32, C.c_int32, _i32p, _f64p, _f64p, _f64p, _f64p,
                                    C.c_int32, _i32p, _i32p]),
@@ -54,12 +66,22 @@ _SIGS = {
     "pbte_transport_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, _f64p]),
     "pbte_residual_iterate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _f64p, _f64p]),
+    "pbte_periodic_gather": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "pbte_periodic_scatter": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "pbte_vec_multidot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                     C.c_int64, _f64p]),
     "pbte_vec_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, _f64p,
                                   C.c_void_p, C.c_int64]),
     "pbte_vec_axpby": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.c_void_p,
                                  C.c_int64]),
+    "pbte_gsis_setup": (C.c_int, [C.c_void_p, C.POINTER(PbteGsisData)]),
+    "pbte_gsis_moments": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_source": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_rhs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_local": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_correct": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -109,6 +109,9 @@ struct pbte_ctx {
     // adiabatic walls
     int n_wall = 0;
     int32_t *wall_face = nullptr, *wall_tri = nullptr, *wall_lfc = nullptr;
+    // periodic faces (Krylov state)
+    int n_per = 0;
+    int32_t *per_face = nullptr, *per_tri = nullptr;
     // schedule
     int nb = 0;
     int64_t total_levels = 0, max_width = 0;
@@ -132,6 +135,17 @@ struct pbte_ctx {
     int32_t* counter = nullptr;
     double* hbuf = nullptr;    // pinned host staging
     size_t hbuf_len = 0;
+
+    // GSIS data
+    bool gsis = false;
+    int gs_nw = 0, gs_nth = 0, gs_variant_b = 0, gs_use_hmin = 1;
+    double gs_tc5 = 0, gs_tau_thr = 1, gs_drift = 0;
+    double *gs_hw = nullptr, *gs_inv4 = nullptr, *gs_aa_trace = nullptr, *gs_ba_sol = nullptr,
+           *gs_wtfc = nullptr, *gs_wnx = nullptr, *gs_wny = nullptr, *gs_hmin = nullptr,
+           *gs_work = nullptr, *gs_wmom = nullptr;
+    int32_t *gs_t1 = nullptr, *gs_l1 = nullptr, *gs_t2 = nullptr, *gs_l2 = nullptr, *gs_wface = nullptr,
+            *gs_wtri = nullptr, *gs_tri_faces = nullptr;
+    uint8_t* gs_flip = nullptr;
 
     std::vector<void*> owned;
     template <typename T> T* alloc(size_t n)
@@ -619,6 +633,7 @@ __global__ void iterate_residual_kernel(const double* temp, const double* t_old,
 // ==========================================================================
 constexpr int kRows = 8;
 constexpr int kDotBlocks = 1024;
+constexpr int kMaxBasis = 512;   // rows of one multidot / update call
 
 __global__ void __launch_bounds__(256) multidot_kernel(const double* __restrict__ Qm, int64_t ld, int m,
                                                        const double* __restrict__ w, int64_t n,
@@ -664,6 +679,270 @@ __global__ void __launch_bounds__(256) axpby_kernel(double a, const double* x, d
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     y[i] = a * x[i] + b * y[i];
+}
+
+// periodic partner DOFs as Krylov state: gather vdf[d][ptri[k]][:] into a
+// binary64 block [D][K][P], and scatter such a block into pbuf[d][pface[k]][:]
+template <typename T>
+__global__ void __launch_bounds__(256) periodic_gather_kernel(const T* __restrict__ vdf, const int32_t* ptri,
+                                                              int K, int D, int N, int P, double* out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)D * K * P) return;
+    const int l = (int)(g % P);
+    const int k = (int)((g / P) % K);
+    const int d = (int)(g / ((int64_t)P * K));
+    out[g] = (double)vdf[((size_t)d * N + ptri[k]) * P + l];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) periodic_scatter_kernel(const double* __restrict__ in, const int32_t* pface,
+                                                               int K, int D, int F, int P, T* pbuf)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)D * K * P) return;
+    const int l = (int)(g % P);
+    const int k = (int)((g / P) % K);
+    const int d = (int)(g / ((int64_t)P * K));
+    pbuf[((size_t)d * F + pface[k]) * P + l] = (T)in[g];
+}
+
+// ==========================================================================
+// GSIS with the HDG synthetic system (acceleration/*.py), per iteration:
+//   moments7 -> hot work -> AA_SRC -> trace RHS (+ wall traces) -> [trace
+//   solve] -> local recovery -> kinetic correction
+// ==========================================================================
+
+// seven angular moments per (cell, dof), binary64, directions in index
+// order: 1, cx, cy (as moments_kernel) and the four third-order weights of
+// the higher-order-term source, cx(5cx^2-3), cy(5cx^2-1), cx(5cy^2-1),
+// cy(5cy^2-3), each times dw (hot_source.py:29-58 summed over l first)
+template <typename T>
+__global__ void __launch_bounds__(256) moments7_kernel(const T* __restrict__ vdf, const double* __restrict__ cxv,
+                                                       const double* __restrict__ cyv,
+                                                       const double* __restrict__ dw,
+                                                       const double* __restrict__ hw, int D, int N, int P,
+                                                       double* __restrict__ out)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N * P) return;
+    const int i = g / P, l = g - (g / P) * P;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
+    const size_t stride = (size_t)N * P;
+    for (int d = 0; d < D; ++d) {
+        const double f = (double)vdf[(size_t)d * stride + g];
+        const double w = dw[d];
+        a0 += f * w;
+        a1 += cxv[d] * f * w;
+        a2 += cyv[d] * f * w;
+        a3 += hw[d] * f;
+        a4 += hw[D + d] * f;
+        a5 += hw[2 * D + d] * f;
+        a6 += hw[3 * D + d] * f;
+    }
+    const size_t PN = (size_t)P * N, o = (size_t)l * N + i;
+    out[o] = a0;
+    out[PN + o] = a1;
+    out[2 * PN + o] = a2;
+    out[3 * PN + o] = a3;
+    out[4 * PN + o] = a4;
+    out[5 * PN + o] = a5;
+    out[6 * PN + o] = a6;
+}
+
+// stress rows of the HoT source per (cell, m) (hot_source.py:29-58)
+__global__ void __launch_bounds__(256) hot_work_kernel(const double* __restrict__ m7, const double* __restrict__ gx,
+                                                       const double* __restrict__ gy, int N, int P, double tc5,
+                                                       double* __restrict__ work)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N * P) return;
+    const int i = g / P, m = g - (g / P) * P;
+    const size_t PN = (size_t)P * N;
+    double pixx = 0, pixy = 0, piyy = 0;
+    for (int l = 0; l < P; ++l) {
+        const double ax = gx[((size_t)i * P + l) * P + m], ay = gy[((size_t)i * P + l) * P + m];
+        const double w1 = m7[3 * PN + (size_t)l * N + i], w2 = m7[4 * PN + (size_t)l * N + i];
+        const double w3 = m7[5 * PN + (size_t)l * N + i], w4 = m7[6 * PN + (size_t)l * N + i];
+        pixx += ax * w1 + ay * w2;
+        pixy += ax * w2 + ay * w3;
+        piyy += ax * w3 + ay * w4;
+    }
+    double* o = work + (size_t)i * 4 * P;
+    o[m] = (pixx + 0.5 * piyy) * tc5;
+    o[P + m] = 0.5 * pixy * tc5;
+    o[2 * P + m] = 0.5 * pixy * tc5;
+    o[3 * P + m] = (0.5 * pixx + piyy) * tc5;
+}
+
+// AA_SRC[p][i] = sum_q inv(AA_SOL)[i][p][3P + q] work[i][q]  (hot_source.py:74)
+__global__ void __launch_bounds__(256) hot_apply_kernel(const double* __restrict__ inv4, const double* __restrict__ work,
+                                                        int N, int P, double* __restrict__ aa_src)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int R = 7 * P;
+    if (g >= N * R) return;
+    const int i = g / R, p = g - (g / R) * R;
+    const double* a = inv4 + ((size_t)i * R + p) * 4 * P;
+    const double* w = work + (size_t)i * 4 * P;
+    double s = 0.0;
+    for (int q = 0; q < 4 * P; ++q) s += a[q] * w[q];
+    aa_src[(size_t)p * N + i] = s;
+}
+
+// wall trace moments (global_solve.py:31-53): per (wall face, face node)
+template <typename T>
+__global__ void __launch_bounds__(256) wall_trace_kernel(const T* __restrict__ vdf, const double* cxv, const double* cyv,
+                                                         const double* dw, const double* __restrict__ wtfc,
+                                                         const int32_t* wtri, int nw, int D, int N, int P, int Q,
+                                                         double* out)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nw * Q) return;
+    const int k = g / Q, m = g - (g / Q) * Q;
+    const int tri = wtri[k];
+    const double* c = wtfc + ((size_t)k * Q + m) * P;
+    double tb = 0, qxb = 0, qyb = 0;
+    for (int d = 0; d < D; ++d) {
+        const T* f = vdf + ((size_t)d * N + tri) * P;
+        double t = 0.0;
+        for (int l = 0; l < P; ++l) t += (double)f[l] * c[l];
+        tb += t * dw[d];
+        qxb += cxv[d] * t * dw[d];
+        qyb += cyv[d] * t * dw[d];
+    }
+    out[(size_t)k * 3 * Q + m] = tb;
+    out[(size_t)k * 3 * Q + Q + m] = qxb;
+    out[(size_t)k * 3 * Q + 2 * Q + m] = qyb;
+}
+
+// trace right-hand side per (face, row) (global_solve.py:128-186)
+__global__ void __launch_bounds__(256) trace_rhs_kernel(const double* __restrict__ ba_sol, const double* __restrict__ aa_src,
+                                                        const int32_t* t1, const int32_t* l1, const int32_t* t2,
+                                                        const int32_t* l2, const uint8_t* flip, int F, int N, int P,
+                                                        int Q, double* ffa)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int R = 3 * Q, C = 7 * P;
+    if (g >= F * R) return;
+    const int f = g / R, r = g - (g / R) * R;
+    double s = 0.0;
+    {
+        const int t = t1[f], l = l1[f];
+        const double* b = ba_sol + (((size_t)t * 3 + l) * R + r) * C;
+        for (int q = 0; q < C; ++q) s += b[q] * aa_src[(size_t)q * N + t];
+    }
+    if (t2[f] >= 0) {
+        const int t = t2[f], l = l2[f];
+        int rr = r;
+        if (flip[f]) {   // mirror_permutation: k -> Q-1-k within each field block
+            const int blk = r / Q, k = r - blk * Q;
+            rr = blk * Q + (Q - 1 - k);
+        }
+        const double* b = ba_sol + (((size_t)t * 3 + l) * R + rr) * C;
+        double s2 = 0.0;
+        for (int q = 0; q < C; ++q) s2 += b[q] * aa_src[(size_t)q * N + t];
+        s += s2;
+    }
+    ffa[(size_t)f * R + r] = s;
+}
+
+// walls impose the trace from the kinetic solution (global_solve.py:151-186)
+__global__ void __launch_bounds__(256) trace_wall_kernel(const double* __restrict__ mom, const int32_t* wface,
+                                                         const double* wnx, const double* wny, int nw, int n_th,
+                                                         int Q, double cv, int variant_b, double* ffa)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nw * Q) return;
+    const int k = g / Q, m = g - (g / Q) * Q;
+    const int R = 3 * Q;
+    double* o = ffa + (size_t)wface[k] * R;
+    const double* w = mom + (size_t)k * R;
+    o[m] = w[m] / cv;
+    if (k < n_th) {
+        if (variant_b) {
+            o[Q + m] = w[Q + m];
+            o[2 * Q + m] = w[2 * Q + m];
+        }
+    } else {
+        const double qxb = w[Q + m], qyb = w[2 * Q + m], nx = wnx[k], ny = wny[k];
+        o[Q + m] = qxb * (ny * ny) - qyb * (nx * ny);
+        o[2 * Q + m] = -qxb * (nx * ny) + qyb * (nx * nx);
+    }
+}
+
+// UQ = AA_SRC + sum_il AA_TRACE[i][il] U_TRACE[face(i, il)]  (local_solve.py:16-26)
+__global__ void __launch_bounds__(256) local_solve_kernel(const double* __restrict__ aa_trace, const double* __restrict__ aa_src,
+                                                          const double* __restrict__ u, const int32_t* tri_faces,
+                                                          int N, int P, int Q, double* uq)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int R = 7 * P, C = 3 * Q;
+    if (g >= N * R) return;
+    const int i = g / R, p = g - (g / R) * R;
+    double s = 0.0;
+    for (int il = 0; il < 3; ++il) {
+        const double* a = aa_trace + (((size_t)i * 3 + il) * R + p) * C;
+        const double* x = u + (size_t)tri_faces[i * 3 + il] * C;
+        for (int r = 0; r < C; ++r) s += a[r] * x[r];
+    }
+    uq[(size_t)p * N + i] = aa_src[(size_t)p * N + i] + s;
+}
+
+// damped kinetic correction (correction.py:29-75) per (cell, dof)
+template <typename T>
+__global__ void __launch_bounds__(256) correct_kernel(T* vdf, const double* __restrict__ m7, const double* __restrict__ uq,
+                                                      const double* __restrict__ defect, const double* cxv,
+                                                      const double* cyv, const double* hmin, int D, int N, int P,
+                                                      double cv, double pi, double tau_r, double tau_thr,
+                                                      double drift, int use_hmin, double* ts, double* qxs,
+                                                      double* qys, double* mkin)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N * P) return;
+    const int i = g / P, m = g - (g / P) * P;
+    const double tau_loc = use_hmin ? tau_r / hmin[i] : tau_r;
+    const double beta = fmin(tau_loc, tau_thr) / tau_loc;
+    const size_t PN = (size_t)P * N, o = (size_t)m * N + i;
+    const double t_vdf = m7[o] / cv, qx_vdf = m7[PN + o], qy_vdf = m7[2 * PN + o];
+    if (mkin) {
+        mkin[o] = t_vdf;
+        mkin[PN + o] = qx_vdf;
+        mkin[2 * PN + o] = qy_vdf;
+    }
+    const double d0 = defect ? defect[o] : 0.0, d1 = defect ? defect[PN + o] : 0.0,
+                 d2 = defect ? defect[2 * PN + o] : 0.0;
+    const double l_t = (uq[o] - d0 - t_vdf) * beta;
+    const double l_qx = (uq[PN + o] - d1 - qx_vdf) * beta;
+    const double l_qy = (uq[2 * PN + o] - d2 - qy_vdf) * beta;
+    const double eq = l_t * cv / 4.0 / pi;
+    const size_t stride = (size_t)N * P;
+    for (int d = 0; d < D; ++d) {
+        T* f = vdf + (size_t)d * stride + g;
+        *f = (T)((double)*f + eq + (cxv[d] * l_qx + cyv[d] * l_qy) * drift);
+    }
+    ts[o] = l_t + t_vdf;
+    qxs[o] = l_qx + qx_vdf;
+    qys[o] = l_qy + qy_vdf;
+}
+
+// element integrals of corrected dofs: temp[i] = sum_m ts[m][i] int_tri[m][i]
+__global__ void __launch_bounds__(256) dof_integrals_kernel(const double* ts, const double* qxs, const double* qys,
+                                                            const double* int_tri, int N, int P, double* temp,
+                                                            double* qx, double* qy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double t = 0, x = 0, y = 0;
+    for (int m = 0; m < P; ++m) {
+        const double w = int_tri[(size_t)m * N + i];
+        t += ts[(size_t)m * N + i] * w;
+        x += qxs[(size_t)m * N + i] * w;
+        y += qys[(size_t)m * N + i] * w;
+    }
+    temp[i] = t;
+    qx[i] = x;
+    qy[i] = y;
 }
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -831,6 +1110,12 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->wall_face = ctx->upload(wf.data(), wf.size());
         ctx->wall_tri = ctx->upload(wt.data(), wt.size());
         ctx->wall_lfc = ctx->upload(wl.data(), wl.size());
+        std::vector<int32_t> pf, pt;
+        for (int f = 0; f < F; ++f)
+            if (su->periodic_tri[f] >= 0) { pf.push_back(f); pt.push_back(su->periodic_tri[f]); }
+        ctx->n_per = (int)pf.size();
+        ctx->per_face = ctx->upload(pf.data(), pf.size());
+        ctx->per_tri = ctx->upload(pt.data(), pt.size());
 
         // --- bundles: directions with identical inflow signs ------------------
         std::vector<uint64_t> hsh(D);
@@ -987,7 +1272,8 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->src64 = ctx->alloc<double>((size_t)N * 3 * P);
         ctx->src32 = ctx->alloc<float>((size_t)N * 3 * P);
         ctx->ssraw = ctx->alloc<double>((size_t)N * P);
-        ctx->red_len = std::max<size_t>((size_t)D * nblocks(N, 256) * 2, (size_t)64 * kDotBlocks) + 64;
+        ctx->red_len = std::max<size_t>((size_t)D * nblocks(N, 256) * 2 + 2 * (size_t)D,
+                                        (size_t)kMaxBasis * kDotBlocks + kMaxBasis) + 64;
         ctx->red = ctx->alloc<double>(ctx->red_len);
         ctx->small = ctx->alloc<double>(4096);
         ctx->counter = ctx->alloc<int32_t>(16);
@@ -1211,7 +1497,7 @@ extern "C" int pbte_vec_multidot(pbte_ctx* ctx, const double* Q, int64_t ld, int
 {
     return guarded(ctx, [&]() {
         if (m <= 0) return 0;
-        if (m > 4096 || (size_t)m * kDotBlocks + m > ctx->red_len) throw std::string("too many basis vectors");
+        if (m > kMaxBasis || (size_t)m * kDotBlocks + m > ctx->red_len) throw std::string("too many basis vectors");
         const int nbk = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + 255) / 256));
         dim3 grid(nbk, (m + kRows - 1) / kRows);
         hipLaunchKernelGGL(multidot_kernel, grid, dim3(256), 0, ctx->stream, Q, ld, m, w, n, ctx->red);
@@ -1233,7 +1519,7 @@ extern "C" int pbte_vec_update(pbte_ctx* ctx, const double* Q, int64_t ld, int32
 {
     return guarded(ctx, [&]() {
         if (m <= 0) return 0;
-        if (m > 4096) throw std::string("too many basis vectors");
+        if (m > kMaxBasis) throw std::string("too many basis vectors");
         double* h = ctx->host_staging((size_t)m);
         HIP_TRY(hipStreamSynchronize(ctx->stream));   // staging buffer may still feed a copy
         std::memcpy(h, c_host, sizeof(double) * m);
@@ -1250,6 +1536,188 @@ extern "C" int pbte_vec_axpby(pbte_ctx* ctx, double a, const double* x, double b
 {
     return guarded(ctx, [&]() {
         hipLaunchKernelGGL(axpby_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, a, x, b, y, n);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_periodic_gather(pbte_ctx* ctx, int32_t precision, const void* vdf, double* out)
+{
+    return guarded(ctx, [&]() {
+        const int64_t n = (int64_t)ctx->D * ctx->n_per * ctx->P;
+        if (n == 0) return 0;
+        if (precision == PBTE_FP64)
+            hipLaunchKernelGGL(periodic_gather_kernel<double>, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
+                               (const double*)vdf, ctx->per_tri, ctx->n_per, ctx->D, ctx->N, ctx->P, out);
+        else
+            hipLaunchKernelGGL(periodic_gather_kernel<float>, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
+                               (const float*)vdf, ctx->per_tri, ctx->n_per, ctx->D, ctx->N, ctx->P, out);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_periodic_scatter(pbte_ctx* ctx, int32_t precision, const double* in, void* pbuf)
+{
+    return guarded(ctx, [&]() {
+        const int64_t n = (int64_t)ctx->D * ctx->n_per * ctx->P;
+        if (n == 0) return 0;
+        if (precision == PBTE_FP64)
+            hipLaunchKernelGGL(periodic_scatter_kernel<double>, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
+                               in, ctx->per_face, ctx->n_per, ctx->D, ctx->F, ctx->P, (double*)pbuf);
+        else
+            hipLaunchKernelGGL(periodic_scatter_kernel<float>, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream,
+                               in, ctx->per_face, ctx->n_per, ctx->D, ctx->F, ctx->P, (float*)pbuf);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_setup(pbte_ctx* ctx, const pbte_gsis_data* g)
+{
+    return guarded(ctx, [&]() {
+        const int P = ctx->P, Q = ctx->Q, N = ctx->N, F = ctx->F, D = ctx->D;
+        std::vector<double> hw((size_t)4 * D);
+        for (int d = 0; d < D; ++d) {
+            const double cx = g->cxv[d], cy = g->cyv[d], w = g->domega[d];
+            hw[d] = cx * (5.0 * cx * cx - 3.0) * w;
+            hw[D + d] = cy * (5.0 * cx * cx - 1.0) * w;
+            hw[2 * D + d] = cx * (5.0 * cy * cy - 1.0) * w;
+            hw[3 * D + d] = cy * (5.0 * cy * cy - 3.0) * w;
+        }
+        ctx->gs_hw = ctx->upload(hw.data(), hw.size());
+        // stress columns of inv(AA_SOL): [N][7P][4P]
+        std::vector<double> inv4((size_t)N * 7 * P * 4 * P);
+        for (int i = 0; i < N; ++i)
+            for (int p = 0; p < 7 * P; ++p)
+                std::memcpy(&inv4[((size_t)i * 7 * P + p) * 4 * P], g->inv_aa_sol + ((size_t)i * 7 * P + p) * 7 * P + 3 * P,
+                            sizeof(double) * 4 * P);
+        ctx->gs_inv4 = ctx->upload(inv4.data(), inv4.size());
+        ctx->gs_aa_trace = ctx->upload(g->aa_trace, (size_t)N * 3 * 7 * P * 3 * Q);
+        ctx->gs_ba_sol = ctx->upload(g->ba_sol, (size_t)N * 3 * 3 * Q * 7 * P);
+        ctx->gs_t1 = ctx->upload(g->t1, F);
+        ctx->gs_l1 = ctx->upload(g->l1, F);
+        ctx->gs_t2 = ctx->upload(g->t2, F);
+        ctx->gs_l2 = ctx->upload(g->l2, F);
+        ctx->gs_flip = ctx->upload(g->flip, F);
+        ctx->gs_nw = g->n_wall;
+        ctx->gs_nth = g->n_wall_th;
+        ctx->gs_wface = ctx->upload(g->wall_face, std::max(g->n_wall, 1));
+        ctx->gs_wtri = ctx->upload(g->wall_tri, std::max(g->n_wall, 1));
+        ctx->gs_wnx = ctx->upload(g->wall_nx, std::max(g->n_wall, 1));
+        ctx->gs_wny = ctx->upload(g->wall_ny, std::max(g->n_wall, 1));
+        // wall face slices of int_tri_fc_fc [P][Q][N][3] -> [k][Q][P]
+        std::vector<double> wt((size_t)std::max(g->n_wall, 1) * Q * P, 0.0);
+        for (int k = 0; k < g->n_wall; ++k)
+            for (int m = 0; m < Q; ++m)
+                for (int l = 0; l < P; ++l)
+                    wt[((size_t)k * Q + m) * P + l] =
+                        g->tri_fc_fc[(((size_t)l * Q + m) * N + g->wall_tri[k]) * 3 + g->wall_lfc[k]];
+        ctx->gs_wtfc = ctx->upload(wt.data(), wt.size());
+        ctx->gs_hmin = ctx->upload(g->tri_hmin, N);
+        ctx->gs_tri_faces = ctx->upload(g->tri_faces, (size_t)N * 3);
+        ctx->gs_work = ctx->alloc<double>((size_t)N * 4 * P);
+        ctx->gs_wmom = ctx->alloc<double>((size_t)std::max(g->n_wall, 1) * 3 * Q);
+        ctx->gs_tc5 = g->tc5;
+        ctx->gs_tau_thr = g->tau_thr;
+        ctx->gs_drift = g->drift;
+        ctx->gs_variant_b = g->variant_b;
+        ctx->gs_use_hmin = g->use_hmin;
+        ctx->gsis = true;
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_moments(pbte_ctx* ctx, int32_t precision, const void* vdf, double* mom7)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
+        const unsigned g = nblocks((int64_t)ctx->N * ctx->P, 256);
+        if (precision == PBTE_FP64)
+            hipLaunchKernelGGL(moments7_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
+                               ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_hw, ctx->D, ctx->N, ctx->P, mom7);
+        else
+            hipLaunchKernelGGL(moments7_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (const float*)vdf,
+                               ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_hw, ctx->D, ctx->N, ctx->P, mom7);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_source(pbte_ctx* ctx, const double* mom7, double* aa_src)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
+        const int N = ctx->N, P = ctx->P;
+        hipLaunchKernelGGL(hot_work_kernel, dim3(nblocks((int64_t)N * P, 256)), dim3(256), 0, ctx->stream, mom7,
+                           ctx->gx, ctx->gy, N, P, ctx->gs_tc5, ctx->gs_work);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(hot_apply_kernel, dim3(nblocks((int64_t)N * 7 * P, 256)), dim3(256), 0, ctx->stream,
+                           ctx->gs_inv4, ctx->gs_work, N, P, aa_src);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_rhs(pbte_ctx* ctx, int32_t precision, const void* vdf, const double* aa_src, double* ffa)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
+        const int N = ctx->N, P = ctx->P, Q = ctx->Q, F = ctx->F;
+        hipLaunchKernelGGL(trace_rhs_kernel, dim3(nblocks((int64_t)F * 3 * Q, 256)), dim3(256), 0, ctx->stream,
+                           ctx->gs_ba_sol, aa_src, ctx->gs_t1, ctx->gs_l1, ctx->gs_t2, ctx->gs_l2, ctx->gs_flip, F, N,
+                           P, Q, ffa);
+        HIP_TRY(hipGetLastError());
+        if (ctx->gs_nw) {
+            const unsigned g = nblocks((int64_t)ctx->gs_nw * Q, 256);
+            if (precision == PBTE_FP64)
+                hipLaunchKernelGGL(wall_trace_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
+                                   ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_wtfc, ctx->gs_wtri, ctx->gs_nw, ctx->D,
+                                   N, P, Q, ctx->gs_wmom);
+            else
+                hipLaunchKernelGGL(wall_trace_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (const float*)vdf,
+                                   ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_wtfc, ctx->gs_wtri, ctx->gs_nw, ctx->D,
+                                   N, P, Q, ctx->gs_wmom);
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(trace_wall_kernel, dim3(g), dim3(256), 0, ctx->stream, ctx->gs_wmom, ctx->gs_wface,
+                               ctx->gs_wnx, ctx->gs_wny, ctx->gs_nw, ctx->gs_nth, Q, ctx->cv, ctx->gs_variant_b, ffa);
+            HIP_TRY(hipGetLastError());
+        }
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_local(pbte_ctx* ctx, const double* aa_src, const double* u_trace, double* uq)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
+        const int N = ctx->N, P = ctx->P;
+        hipLaunchKernelGGL(local_solve_kernel, dim3(nblocks((int64_t)N * 7 * P, 256)), dim3(256), 0, ctx->stream,
+                           ctx->gs_aa_trace, aa_src, u_trace, ctx->gs_tri_faces, N, P, ctx->Q, uq);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_correct(pbte_ctx* ctx, int32_t precision, void* vdf, const double* mom7, const double* uq,
+                                 const double* defect, double* ts, double* qxs, double* qys, double* temp,
+                                 double* qx, double* qy, double* mkin)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
+        const int N = ctx->N, P = ctx->P;
+        const unsigned g = nblocks((int64_t)N * P, 256);
+        if (precision == PBTE_FP64)
+            hipLaunchKernelGGL(correct_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (double*)vdf, mom7, uq,
+                               defect, ctx->cxv, ctx->cyv, ctx->gs_hmin, ctx->D, N, P, ctx->cv, ctx->pi, ctx->tau_r,
+                               ctx->gs_tau_thr, ctx->gs_drift, ctx->gs_use_hmin, ts, qxs, qys, mkin);
+        else
+            hipLaunchKernelGGL(correct_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (float*)vdf, mom7, uq,
+                               defect, ctx->cxv, ctx->cyv, ctx->gs_hmin, ctx->D, N, P, ctx->cv, ctx->pi, ctx->tau_r,
+                               ctx->gs_tau_thr, ctx->gs_drift, ctx->gs_use_hmin, ts, qxs, qys, mkin);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(dof_integrals_kernel, dim3(nblocks(N, 256)), dim3(256), 0, ctx->stream, ts, qxs, qys,
+                           ctx->int_tri, N, P, temp, qx, qy);
         HIP_TRY(hipGetLastError());
         return 0;
     });

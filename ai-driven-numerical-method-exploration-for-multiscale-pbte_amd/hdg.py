@@ -1,0 +1,209 @@
+"""HDG macroscopic synthetic system of GSIS: local operators, face operator
+and the sparse trace matrix (host, once per case).
+
+Follows ``solver-python/pybte/acceleration/assembly.py:44-268``: seven fields
+per element DOF ``UQ = [T, qx, qy, Lxx, Lxy, Lyx, Lyy]``, three trace fields
+per face DOF; ``inv(AA_SOL)`` per element, ``AA_TRACE`` premultiplied by it,
+``BA_SOL`` (interior / periodic average-plus-jump rows, thermalising-wall
+momentum rows for variant A), and the trace matrix assembled block by block
+(identity-like ``int_fc_fc`` diagonal minus ``BA_SOL @ AA_TRACE`` from each
+side, partner rows mirrored across a reversed periodic pair).  The face
+loop of the reference is vectorised over faces; duplicates are summed in
+the reference's block order (diagonal, side 1, side 2; local faces 0..2).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .constants import BC_PERIODIC, BC_THERMALISING
+
+__all__ = ["build_local_operators", "build_ba_sol", "mirror_permutation",
+           "assemble_global_matrix", "face_sides"]
+
+
+def build_local_operators(mesh, integrals, nd, nf, cv, tau_r, tau_c, st, variant="A"):
+    """``(inv_aa_sol [N,7P,7P], aa_trace [N,3,7P,3Q])``."""
+    n = mesh.n_tris
+    scale = tau_r if variant == "B" else 1.0
+    aa = integrals.int_tri_tri_x
+    bb = integrals.int_tri_tri_y
+    cc = np.transpose(aa, (0, 2, 1))
+    dd = np.transpose(bb, (0, 2, 1))
+    pp = integrals.int_tri_fc.sum(axis=1)
+    oo = np.transpose(integrals.int_tri_tri, (2, 0, 1))
+    A = np.zeros((n, 7 * nd, 7 * nd))
+
+    def put(r, c, v):
+        A[:, r * nd:(r + 1) * nd, c * nd:(c + 1) * nd] = v
+
+    put(0, 0, st[0] * pp)
+    put(0, 1, cc)
+    put(0, 2, dd)
+    put(1, 0, -aa * cv / 3.0 * scale)
+    put(1, 1, (oo + st[1] * pp) if variant == "B" else (st[1] * pp + oo / tau_r))
+    put(1, 3, -4.0 / 3.0 * cc)
+    put(1, 4, -dd)
+    put(1, 5, -dd)
+    put(1, 6, 2.0 / 3.0 * cc)
+    put(2, 0, -bb * cv / 3.0 * scale)
+    put(2, 2, (oo + st[2] * pp) if variant == "B" else (st[2] * pp + oo / tau_r))
+    put(2, 3, 2.0 / 3.0 * dd)
+    put(2, 4, -cc)
+    put(2, 5, -cc)
+    put(2, 6, -4.0 / 3.0 * dd)
+    tc5 = tau_c * scale / 5.0
+    put(3, 1, aa * tc5)
+    put(3, 3, oo)
+    put(4, 1, bb * tc5)
+    put(4, 4, oo)
+    put(5, 2, aa * tc5)
+    put(5, 5, oo)
+    put(6, 2, bb * tc5)
+    put(6, 6, oo)
+    inv_aa_sol = np.linalg.inv(A)
+
+    ww = np.transpose(integrals.int_tri_fc_fc, (2, 3, 0, 1))          # (I, IL, P, Q)
+    nxe = mesh.tri_normal[:, :, 0][:, :, None, None]
+    nye = mesh.tri_normal[:, :, 1][:, :, None, None]
+    T = np.zeros((n, 3, 7 * nd, 3 * nf))
+
+    def tput(r, c, v):
+        T[:, :, r * nd:(r + 1) * nd, c * nf:(c + 1) * nf] = v
+
+    tput(0, 0, st[0] * ww)
+    tput(1, 0, -nxe * ww * cv / 3.0 * scale)
+    tput(1, 1, st[1] * ww)
+    tput(2, 0, -nye * ww * cv / 3.0 * scale)
+    tput(2, 2, st[2] * ww)
+    tput(3, 1, nxe * ww * tc5)
+    tput(4, 1, nye * ww * tc5)
+    tput(5, 2, nxe * ww * tc5)
+    tput(6, 2, nye * ww * tc5)
+    aa_trace = np.einsum("ipq,ilqr->ilpr", inv_aa_sol, T, optimize=True)
+    return inv_aa_sol, aa_trace
+
+
+def build_ba_sol(mesh, integrals, nd, nf, st, bc_type, variant="A"):
+    """``BA_SOL [N,3,3Q,7P]``."""
+    n = mesh.n_tris
+    B = np.zeros((n, 3, 3 * nf, 7 * nd))
+    bb2 = np.transpose(integrals.int_tri_fc_fc, (2, 3, 1, 0))        # (I, IL, Q, P)
+    nx = mesh.tri_normal[:, :, 0][:, :, None, None]
+    ny = mesh.tri_normal[:, :, 1][:, :, None, None]
+    face_bc = mesh.face_bc[mesh.tri_faces]
+    code = np.where(face_bc >= 0, np.asarray(bc_type)[np.clip(face_bc, 0, None)], 0)
+    interior = face_bc < 0
+    periodic = (~interior) & (code == BC_PERIODIC)
+    wall_th = (~interior) & (code == BC_THERMALISING)
+    std = (interior | periodic)[:, :, None, None]
+
+    def blk(r, c):
+        return B[:, :, r * nf:(r + 1) * nf, c * nd:(c + 1) * nd]
+
+    def put(r, c, v):
+        blk(r, c)[:] = np.where(std, v, 0.0)
+
+    put(0, 0, 0.5 * bb2)
+    put(0, 1, nx / 2.0 / st[0] * bb2)
+    put(0, 2, ny / 2.0 / st[0] * bb2)
+    put(1, 1, 0.5 * bb2)
+    put(1, 3, -2.0 * nx / 3.0 / st[1] * bb2)
+    put(1, 4, -ny / 2.0 / st[1] * bb2)
+    put(1, 5, -ny / 2.0 / st[1] * bb2)
+    put(1, 6, nx / 3.0 / st[1] * bb2)
+    put(2, 2, 0.5 * bb2)
+    put(2, 3, ny / 3.0 / st[2] * bb2)
+    put(2, 4, -nx / 2.0 / st[2] * bb2)
+    put(2, 5, -nx / 2.0 / st[2] * bb2)
+    put(2, 6, -2.0 * ny / 3.0 / st[2] * bb2)
+    if variant == "A":
+        w = wall_th[:, :, None, None]
+
+        def wput(r, c, v):
+            blk(r, c)[:] = np.where(w, v, blk(r, c))
+
+        wput(1, 1, bb2)
+        wput(1, 3, -4.0 * nx / 3.0 / st[1] * bb2)
+        wput(1, 4, -ny / st[1] * bb2)
+        wput(1, 5, -ny / st[1] * bb2)
+        wput(1, 6, 2.0 * nx / 3.0 / st[1] * bb2)
+        wput(2, 2, bb2)
+        wput(2, 3, 2.0 * ny / 3.0 / st[2] * bb2)
+        wput(2, 4, -nx / st[2] * bb2)
+        wput(2, 5, -nx / st[2] * bb2)
+        wput(2, 6, -4.0 * ny / 3.0 / st[2] * bb2)
+    return B
+
+
+def mirror_permutation(nf: int) -> np.ndarray:
+    base = np.arange(nf)[::-1]
+    return np.concatenate([base + b * nf for b in range(3)])
+
+
+def face_sides(mesh, bc_type):
+    """Per face: (t1, l1, t2, l2) contributing (element, local face) pairs,
+    ``-1`` where absent (``global_solve.py:89-106``)."""
+    F = mesh.n_faces
+    bc = mesh.face_bc
+    code = np.where(bc >= 0, np.asarray(bc_type)[np.clip(bc, 0, None)], 0)
+    interior = bc < 0
+    periodic = (~interior) & (code == BC_PERIODIC)
+    adj_tri, adj_lfc = mesh.adjacent_arrays()
+    t1 = np.where(interior, mesh.face_tri[:, 0], adj_tri).astype(np.int64)
+    l1 = np.where(interior, mesh.face_lfc[:, 0], adj_lfc).astype(np.int64)
+    t2 = np.full(F, -1, dtype=np.int64)
+    l2 = np.full(F, -1, dtype=np.int64)
+    t2[interior] = mesh.face_tri[interior, 1]
+    l2[interior] = mesh.face_lfc[interior, 1]
+    if periodic.any():
+        pair = mesh.face_pair[periodic].astype(np.int64)
+        t2[periodic] = adj_tri[pair]
+        l2[periodic] = adj_lfc[pair]
+    return t1, l1, t2, l2, periodic
+
+
+def assemble_global_matrix(mesh, integrals, ba_sol, aa_trace, nf, bc_type, drop_tol=1e-30,
+                           periodic_flip=None):
+    """CSR trace matrix ``K`` (``assembly.py:200-268``) and the entry count
+    before dropping."""
+    from scipy.sparse import coo_matrix
+
+    nf3 = 3 * nf
+    F = mesh.n_faces
+    n = F * nf3
+    perm = mirror_permutation(nf)
+    flip = np.zeros(F, dtype=bool) if periodic_flip is None else np.asarray(periodic_flip)
+    t1, l1, t2, l2, periodic = face_sides(mesh, bc_type)
+    rr, cc = np.meshgrid(np.arange(nf3), np.arange(nf3), indexing="ij")
+    rows, cols, vals = [], [], []
+    # diagonal int_fc_fc blocks, one per field
+    diag = np.zeros((F, nf3, nf3))
+    for k in range(3):
+        diag[:, k * nf:(k + 1) * nf, k * nf:(k + 1) * nf] = np.transpose(integrals.int_fc_fc, (2, 0, 1))
+    fidx = np.arange(F)
+    rows.append((fidx[:, None, None] * nf3 + rr[None]).ravel())
+    cols.append((fidx[:, None, None] * nf3 + cc[None]).ravel())
+    vals.append(diag.ravel())
+    for side, (t, l) in enumerate(((t1, l1), (t2, l2))):
+        has = t >= 0
+        f = fidx[has]
+        tt, ll = t[has], l[has]
+        mir = flip[has] & periodic[has] & (side == 1)
+        for il in range(3):
+            g = mesh.tri_faces[tt, il].astype(np.int64)
+            d = np.einsum("frq,fqc->frc", ba_sol[tt, ll], aa_trace[tt, il], optimize=True)
+            if mir.any():
+                d[mir] = d[mir][:, perm]
+            rows.append((f[:, None, None] * nf3 + rr[None]).ravel())
+            cols.append((g[:, None, None] * nf3 + cc[None]).ravel())
+            vals.append((-d).ravel())
+    rows = np.concatenate(rows)
+    cols = np.concatenate(cols)
+    vals = np.concatenate(vals)
+    K = coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+    K.sum_duplicates()
+    nnz_before = int(np.count_nonzero(np.abs(K.data) > 0.0))
+    K.data[np.abs(K.data) <= drop_tol] = 0.0
+    K.eliminate_zeros()
+    K.sort_indices()
+    return K, nnz_before

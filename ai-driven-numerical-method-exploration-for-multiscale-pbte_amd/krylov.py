@@ -1,0 +1,398 @@
+"""Krylov solution of the outer iteration on the device
+(``solver-python/pybte/krylov.py:55-363``).
+
+One source-iteration step is the affine map ``u -> F(u) = T u + g`` on the
+state ``u = [ts, qxs, qys (, flux_wall) (, periodic partner DOFs)]``; GMRES
+solves ``(I - T) u = g`` with one sweep per product.  Everything of size
+``n`` stays in HBM: the basis ``Q`` (binary64 tensor), the products, and the
+orthogonalisation -- classical Gram-Schmidt applied twice through the
+``pbte_vec_multidot`` / ``pbte_vec_update`` kernels, whose reductions have a
+fixed order.  Only the ``j+2`` Hessenberg entries per product and the
+Givens recurrence run on the host, as scalars.
+
+Precision (``performance.precision``):
+
+``fp64``  the reference's algorithm: ``A v = v - (F(v) - g)`` with binary64
+          sweeps, staged targets, a closing sweep and the true transport
+          residual as the convergence decision (``krylov.py:285-363``).
+``fp32``  FP64 defect correction around FP32 sweeps.  The outer loop keeps
+          ``x`` in binary64 and evaluates ``r = F64(x) - x`` (one binary64
+          sweep, ``= g - (I - T) x``); the correction ``(I - T) e = r`` is
+          solved by GMRES whose products are homogeneous binary32 sweeps
+          (boundary sources removed, moments summed in binary64); ``x += e``.
+          The last ``F64(x)`` is the closing sweep.  Convergence is decided
+          by the binary64 transport residual, as in the reference.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _native
+from .device import dev_zeros, ptr, torch
+
+__all__ = ["FixedPointOperator", "applicable", "gmres_full", "run_krylov", "krylov_basis_cap"]
+
+_ENOUGH_FACTOR = 1.0e-2
+_FLOOR = 1.0e-14
+
+
+def applicable(solver) -> tuple[bool, str]:
+    if int(solver.order.cycles_broken.sum()):
+        return False, ("the sweep ordering has broken cycles: part of the inflow is lagged, "
+                       "so one sweep is not a function of the state alone")
+    return True, ""
+
+
+class FixedPointOperator:
+    """``u -> F(u)`` on device vectors, one sweep apiece."""
+
+    def __init__(self, solver):
+        self.s = solver
+        self.ctx = solver.ctx
+        self.nd, self.nt = solver.ndof, solver.n_tris
+        self.nmom = 3 * self.nd * self.nt
+        self.wall = bool(solver.bcdata.has_nonthermalising)
+        self.nwall = solver.mesh.n_faces * self.nd if self.wall else 0
+        self.per = bool(solver.bcdata.has_periodic)
+        self.npf = int(np.count_nonzero(solver.bcdata.periodic_tri >= 0)) if self.per else 0
+        self.ndir = solver.ndir
+        self.nper = self.ndir * self.npf * self.nd if self.per else 0
+        self.n = self.nmom + self.nwall + self.nper
+        self.sweeps = 0
+        self.sweeps64 = 0
+        solver.ctx.use_pbuf = self.per
+        self._temp = dev_zeros(self.nt)
+        self._qx = dev_zeros(self.nt)
+        self._qy = dev_zeros(self.nt)
+
+    def _blocks(self, u):
+        k = self.nd * self.nt
+        ts = u[:k]
+        qxs = u[k:2 * k]
+        qys = u[2 * k:3 * k]
+        fw = u[self.nmom:self.nmom + self.nwall].view(-1, self.nd) if self.wall else None
+        pb = u[self.nmom + self.nwall:] if self.per else None
+        return ts, qxs, qys, fw, pb
+
+    def apply(self, u, out, precision: str = "fp64", homogeneous: bool = False) -> None:
+        """``out = F(u)`` (``homogeneous``: without the wall sources, ``T u``)."""
+        s, ctx = self.s, self.ctx
+        vdf = s.vdf_d if precision == "fp64" else s.vdf32_d
+        code = _native.PBTE_FP64 if precision == "fp64" else _native.PBTE_FP32
+        ts, qxs, qys, fw, pb = self._blocks(u)
+        if self.per:
+            buf = ctx.ensure_pbuf(vdf.dtype)
+            _native.check(ctx.handle, _native.lib().pbte_periodic_scatter(
+                ctx.handle, code, ptr(pb), ptr(buf)), "periodic_scatter")
+        ctx.sweep(None, vdf, homogeneous=homogeneous, flux_wall=fw, src=(ts, qxs, qys))
+        self.sweeps += 1
+        if precision == "fp64":
+            self.sweeps64 += 1
+        ots, oqxs, oqys, ofw, opb = self._blocks(out)
+        ctx.moments(vdf, None, out=(ots, oqxs, oqys, self._temp, self._qx, self._qy))
+        if self.wall:
+            ctx.wall_flux(vdf, out=ofw)
+        if self.per:
+            _native.check(ctx.handle, _native.lib().pbte_periodic_gather(
+                ctx.handle, code, ptr(vdf), ptr(opb)), "periodic_gather")
+
+    def publish(self, u) -> None:
+        """Copy a state's moment block (and emission) into the solver."""
+        s = self.s
+        ts, qxs, qys, fw, _ = self._blocks(u)
+        s.mom.ts_d.copy_(ts.view(self.nd, self.nt))
+        s.mom.qxs_d.copy_(qxs.view(self.nd, self.nt))
+        s.mom.qys_d.copy_(qys.view(self.nd, self.nt))
+        s.mom.temp_d.copy_(self._temp)
+        s.mom.qx_d.copy_(self._qx)
+        s.mom.qy_d.copy_(self._qy)
+        if self.wall:
+            s.ctx.flux_wall.copy_(fw)
+
+    def state(self):
+        """The state of the solver's current distribution, without sweeping."""
+        s = self.s
+        u = dev_zeros(self.n)
+        ts, qxs, qys, fw, pb = self._blocks(u)
+        s.ctx.moments(s.vdf_d, None, out=(ts, qxs, qys, self._temp, self._qx, self._qy))
+        if self.wall:
+            s.ctx.wall_flux(s.vdf_d, out=fw)
+        if self.per:
+            _native.check(s.ctx.handle, _native.lib().pbte_periodic_gather(
+                s.ctx.handle, _native.PBTE_FP64, ptr(s.vdf_d), ptr(pb)), "periodic_gather")
+        return u
+
+
+def _back_substitute(r, y):
+    out = np.array(y, dtype=float)
+    for i in range(out.size - 1, -1, -1):
+        s = out[i] - r[i, i + 1:] @ out[i + 1:]
+        out[i] = s / r[i, i] if r[i, i] != 0.0 else 0.0
+    return out
+
+
+def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None, callback=None):
+    """GMRES with a device basis; ``matvec(v, out)`` writes ``A v``.
+
+    Follows ``krylov.py:150-246``: restarts at ``max_basis``, CGS2, Givens
+    rotations, ``callback(res)`` once per product (true stops).
+    Returns ``(x, converged, n_matvec)``; ``x`` is updated in place."""
+    n = b.numel()
+    x = dev_zeros(n) if x is None else x
+    maxiter = int(n if maxiter is None else maxiter)
+    m_cap = maxiter if max_basis is None else max(1, min(int(max_basis), maxiter))
+    r = dev_zeros(n)
+    w = dev_zeros(n)
+    Q = None
+    nmv = 0
+    converged = halted = False
+    xnz = ctx.norm(x) > 0.0
+    while nmv < maxiter and not converged and not halted:
+        if xnz:
+            matvec(x, r)
+            ctx.axpby(1.0, b, -1.0, r)          # r = b - A x
+            nmv += 1
+            if callback is not None and callback(ctx.norm(r)):
+                halted = True
+                break
+        else:
+            r.copy_(b)
+        beta = ctx.norm(r)
+        if beta <= tol_abs:
+            converged = True
+            break
+        m = min(m_cap, maxiter - nmv)
+        if m <= 0:
+            break
+        if Q is None or Q.shape[0] < m + 1:
+            Q = dev_zeros((m + 1, n))
+        ctx.axpby(1.0 / beta, r, 0.0, Q[0])
+        cols, cs, sn, gam = [], [], [], [beta]
+        for j in range(m):
+            matvec(Q[j], w)
+            nmv += 1
+            c1 = ctx.multidot(Q, j + 1, w)
+            ctx.update(Q, j + 1, c1, w)
+            c2 = ctx.multidot(Q, j + 1, w)
+            ctx.update(Q, j + 1, c2, w)
+            col = np.empty(j + 2)
+            col[:j + 1] = c1 + c2
+            hn = ctx.norm(w)
+            col[j + 1] = hn
+            for i in range(j):
+                t = cs[i] * col[i] + sn[i] * col[i + 1]
+                col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1]
+                col[i] = t
+            d = float(np.hypot(col[j], col[j + 1]))
+            c_, s_ = (1.0, 0.0) if d == 0.0 else (col[j] / d, col[j + 1] / d)
+            cs.append(c_)
+            sn.append(s_)
+            col[j] = d
+            cols.append(col[:j + 1].copy())
+            gam.append(-s_ * gam[j])
+            gam[j] = c_ * gam[j]
+            res = abs(gam[j + 1])
+            stop = bool(callback(res)) if callback is not None else False
+            if res <= tol_abs:
+                converged = True
+                break
+            if stop:
+                halted = True
+                break
+            if hn == 0.0:
+                break
+            if j + 1 < m:
+                ctx.axpby(1.0 / hn, w, 0.0, Q[j + 1])
+        k = len(cols)
+        if k:
+            rmat = np.zeros((k, k))
+            for j, col in enumerate(cols):
+                rmat[:j + 1, j] = col
+            y = _back_substitute(rmat, np.asarray(gam[:k]))
+            ctx.update(Q, k, -y, x)                # x += Q[:k]^T y
+            xnz = True
+    return x, converged, nmv
+
+
+def krylov_basis_cap(size: int, tmax: int, max_bytes: float) -> int:
+    per = max(1.0, 8.0 * float(size))
+    return int(max(32, min(int(tmax), int(max_bytes / per))))
+
+
+def run_krylov(solver, callback=None, progress_every: int = 0):
+    ok, why = applicable(solver)
+    if not ok:
+        raise ValueError(f"Krylov acceleration does not apply: {why}")
+    op = FixedPointOperator(solver)
+    try:
+        if solver.precision == "fp32":
+            return _run_mixed(solver, op, callback, progress_every)
+        return _run_fp64(solver, op, callback, progress_every)
+    finally:
+        solver.ctx.use_pbuf = False
+
+
+class _History:
+    def __init__(self, solver, callback, progress_every, want_true):
+        self.s, self.cb, self.every = solver, callback, progress_every
+        self.residuals, self.masses, self.true_res = [], [], []
+        self.want_true = want_true
+
+    def sweep_done(self, op):
+        mass = op.ctx.residual_iterate(op._temp, op._temp)[1]
+        self.masses.append(mass)
+        self.residuals.append(np.nan)
+        if self.want_true:
+            self.true_res.append(np.nan)
+
+    def close(self, res):
+        self.residuals[-1] = res
+        step = len(self.residuals)
+        if self.every and step % self.every == 0:
+            print(f"  step {step:8d}  residual {res:.6e}  mass {self.masses[-1]:.10f}", flush=True)
+        if self.cb is not None:
+            self.cb(step, res, self.s)
+
+
+def _cap(solver, op):
+    c = solver.case
+    if c.scheme.krylov_restart:
+        return int(c.scheme.krylov_restart)
+    return krylov_basis_cap(op.n, c.iteration.tmax, c.scheme.krylov_max_bytes)
+
+
+def _run_fp64(solver, op, callback, progress_every):
+    """The reference driver (``krylov.py:285-363``) on device vectors."""
+    c = solver.case
+    tol, tmax = c.iteration.tol, c.iteration.tmax
+    ctx = solver.ctx
+    h = _History(solver, callback, progress_every, c.iteration.true_residual)
+    t0 = time.perf_counter()
+    x = op.state() if solver.restart_info.get("enabled") else dev_zeros(op.n)
+    zero = dev_zeros(op.n)
+    g = dev_zeros(op.n)
+    op.apply(zero, g)
+    h.sweep_done(op)
+    gnorm = ctx.norm(g)
+    scale = gnorm or 1.0
+    h.close(1.0)
+    tmp = dev_zeros(op.n)
+
+    def matvec(v, out):
+        op.apply(v, tmp)
+        h.sweep_done(op)
+        # out = v - (F(v) - g)
+        out.copy_(v)
+        ctx.axpby(-1.0, tmp, 1.0, out)
+        ctx.axpby(1.0, g, 1.0, out)
+
+    enough = max(tol * _ENOUGH_FACTOR, _FLOOR) * scale
+    target = max(tol * c.scheme.krylov_tol_factor, _FLOOR) * scale
+    cap = _cap(solver, op)
+    reached = -1
+
+    def on_residual(res):
+        nonlocal reached
+        h.close(res / scale)
+        n = len(h.residuals)
+        if reached < 0 and res <= enough:
+            reached = n
+        return reached >= 0 and n - reached > max(64, reached // 4)
+
+    converged = False
+    final = float("nan")
+    fx = dev_zeros(op.n)
+    for _ in range(c.scheme.krylov_max_rounds):
+        budget = tmax - len(h.residuals) - 1
+        if budget <= 0:
+            break
+        reached = -1
+        x, _, _ = gmres_full(ctx, matvec, g, x=x, tol_abs=target, maxiter=budget, max_basis=cap,
+                             callback=on_residual)
+        op.apply(x, fx)                                     # closing sweep
+        h.sweep_done(op)
+        op.publish(fx)
+        final = float(ctx.true_residual(solver.mom, solver.vdf_d))
+        if h.want_true:
+            h.true_res[-1] = final
+        h.close(final)
+        converged = bool(final < tol) if np.isfinite(final) else gnorm == 0.0
+        if converged or len(h.residuals) >= tmax:
+            break
+        target = max(target * 1.0e-3, _FLOOR * scale)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return solver._record(len(h.residuals), converged, h.residuals, h.masses,
+                          h.true_res if h.want_true else None, wall, scheme="KRYLOV",
+                          extra={"krylov_size": op.n, "krylov_periodic_state": op.nper,
+                                 "krylov_basis_cap": cap, "krylov_precond": False,
+                                 "transport_residual": final, "sweeps_fp64": op.sweeps64,
+                                 "sweeps_fp32": op.sweeps - op.sweeps64})
+
+
+def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.0e-5,
+               max_refine: int = 12):
+    """FP64 defect correction around FP32-sweep GMRES (module docstring)."""
+    c = solver.case
+    tol, tmax = c.iteration.tol, c.iteration.tmax
+    ctx = solver.ctx
+    h = _History(solver, callback, progress_every, c.iteration.true_residual)
+    t0 = time.perf_counter()
+    x = op.state() if solver.restart_info.get("enabled") else dev_zeros(op.n)
+    fx = dev_zeros(op.n)
+    r = dev_zeros(op.n)
+    tmp = dev_zeros(op.n)
+
+    def residual():
+        op.apply(x, fx, "fp64")                 # F64(x) = T x + g
+        h.sweep_done(op)
+        r.copy_(fx)
+        ctx.axpby(-1.0, x, 1.0, r)              # r = F(x) - x
+        return ctx.norm(r)
+
+    rnorm = residual()
+    scale = rnorm or 1.0                        # = ||g|| when x starts at 0
+    h.close(1.0)
+
+    def matvec(v, out):                         # (I - T32) v, homogeneous
+        op.apply(v, tmp, "fp32", homogeneous=True)
+        h.sweep_done(op)
+        out.copy_(v)
+        ctx.axpby(-1.0, tmp, 1.0, out)
+
+    # the defect the refinement drives to: three digits below the tolerance
+    target = max(tol * 1.0e-3, _FLOOR) * scale
+    cap = _cap(solver, op)
+    converged = False
+    final = float("nan")
+    refinements = 0
+    e = dev_zeros(op.n)
+    while True:
+        op.publish(fx)
+        final = float(ctx.true_residual(solver.mom, solver.vdf_d))
+        if h.want_true:
+            h.true_res[-1] = final
+        h.close(final if refinements else 1.0)
+        converged = bool(final < tol) and rnorm <= target
+        if converged or refinements >= max_refine or len(h.residuals) >= tmax:
+            break
+        budget = tmax - len(h.residuals) - 1
+        if budget <= 0:
+            break
+        e.zero_()
+        gmres_full(ctx, matvec, r, x=e, tol_abs=max(inner_reduction * rnorm, 0.1 * target),
+                   maxiter=budget, max_basis=cap, callback=lambda res: h.close(res / scale))
+        ctx.axpby(1.0, e, 1.0, x)
+        refinements += 1
+        rnorm = residual()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return solver._record(len(h.residuals), converged, h.residuals, h.masses,
+                          h.true_res if h.want_true else None, wall, scheme="KRYLOV",
+                          extra={"krylov_size": op.n, "krylov_periodic_state": op.nper,
+                                 "krylov_basis_cap": cap, "krylov_precond": False,
+                                 "transport_residual": final, "refinements": refinements,
+                                 "final_defect": rnorm / scale, "sweeps_fp64": op.sweeps64,
+                                 "sweeps_fp32": op.sweeps - op.sweeps64})

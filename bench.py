@@ -1,0 +1,193 @@
+"""Benchmark of the sweep hot path on one MI355X.
+
+Workload (default): the Krylov configuration -- unit square, 128 x 128
+structured grid split into 32 768 P2 triangles (Np = 6), 256 ordinates
+(8 Gauss-Legendre polar x 32 azimuth), Kn_R = 1, Kn_N = 0.1, diffuse
+isothermal walls with the north wall at dT = +1.  A step is one transport
+sweep over all 256 directions (FP32 element LU and distribution, the
+FP32-sweep leg of the mixed-precision solve), with the moments already
+resident in HBM.
+
+Prints one JSON line: sweep DOF-updates/s, the sweep kernel's roofline
+(algorithmic bytes per launch / mean kernel time from HIP events on the
+launch stream), a CPU baseline (the C restatement of the reference's sweep,
+timed on this host) and, with --solve, the time-to-solution of the FP64 and
+FP32+FP64-refinement Krylov solves.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12
+
+
+def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitter=0.0, seed=0):
+    from pbte_amd.mesh import write_square_mesh
+    mesh = Path(workdir) / f"square_{n}_{jitter}_{seed}.msh"
+    if not mesh.exists():
+        write_square_mesh(mesh, n, jitter=jitter, seed=seed)
+    return pkg.Case.from_dict({
+        "iteration": {"tol": 1e-8, "tmax": 20000},
+        "scheme": {"accflag": 0, "method": "krylov", "krylov_restart": 30},
+        "velmesh": {"npole": npole, "nazim": nazim},
+        "dg": {"deg": deg},
+        "flow": {"cv": 1.0, "vg": 1.0, "tau_r": tau_r, "tau_n": tau_n},
+        "mesh": {"file": str(mesh)},
+        "boundaries": [
+            {"name": "SWall", "phyid": 11, "type": "thermalising", "temp": 0.0},
+            {"name": "NWall", "phyid": 12, "type": "thermalising", "temp": 1.0},
+            {"name": "EWall", "phyid": 13, "type": "thermalising", "temp": 0.0},
+            {"name": "WWall", "phyid": 14, "type": "thermalising", "temp": 0.0},
+        ],
+        "restart": {"enabled": False, "error_on_stale": False},
+        "output": {"field": False, "runtime_log": False, "run_record": False},
+        "performance": {"precision": precision},
+    }, basedir=str(workdir))
+
+
+def b_alg(D, N, P, s_lu, s_f):
+    """Algorithmic HBM bytes of one precomputed-LU sweep: per (cell, dir) the
+    LU (P^2 s_lu), pivots (P bytes), the distribution write and one upwind
+    read (2 P s_f); per cell the direction-independent mass, face, source
+    and moment data (8 (4 P^2 + 3 P + 6) bytes)."""
+    return D * N * (P * P * s_lu + P + 2 * P * s_f) + N * 8 * (4 * P * P + 3 * P + 6)
+
+
+def cpu_baseline(pkg, case, seconds=15.0):
+    """The C restatement of the reference sweep (OpenMP over directions,
+    cells in the reference order, precomputed LU) on this host."""
+    from oracle import cpu
+    from pbte_amd.problem import build_problem
+    cpu.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = cpu.Oracle(build_problem(case))
+    o.factorise()
+    rng = np.random.default_rng(0)
+    o.ts[:] = rng.random(o.ts.shape)
+    o.sweep()                                   # warm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.sweep()
+        n += 1
+        el = time.perf_counter() - t0
+        if el > seconds or n >= 20:
+            break
+    dof = o.D * o.N * o.P
+    return {"value": dof * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} FP64 sweeps of the same workload after factorisation "
+                      f"({el:.1f} s), OMP_NUM_THREADS={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--deg", type=int, default=2)
+    ap.add_argument("--npole", type=int, default=8)
+    ap.add_argument("--nazim", type=int, default=32)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--bundle", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--jitter", type=float, default=0.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--solve", action="store_true", help="also time the Krylov solves")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    from __graft_entry__ import load_package
+    pkg = load_package()
+    import torch
+
+    work = Path(tempfile.mkdtemp(prefix="pbte_bench_"))
+    case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, 1.0, 0.1, args.precision,
+                     work, jitter=args.jitter)
+    t0 = time.perf_counter()
+    s = pkg.Solver(case, max_bundle=args.bundle, block=args.block)
+    setup_s = time.perf_counter() - t0
+    D, N, P = s.ndir, s.n_tris, s.ndof
+    vdf = s.sweep_vdf
+    rng = np.random.default_rng(0)
+    s.mom.set_dofs(rng.random((P, N)), 0.1 * rng.random((P, N)), 0.1 * rng.random((P, N)))
+    for _ in range(args.warmup):
+        s.ctx.sweep(s.mom, vdf)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        s.ctx.sweep(s.mom, vdf)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = [a.elapsed_time(b) for a, b in ev]
+    t_sweep = float(np.mean(ms)) * 1e-3
+    dof = D * N * P
+    s_lu = 4 if args.precision == "fp32" else 8
+    bytes_alg = b_alg(D, N, P, s_lu, s_lu)
+    out = {
+        "metric": "sweep DOF-updates/s",
+        "value": dof * args.steps / wall,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f64",
+        "data": "synthetic moments on the workload's mesh and ordinates",
+        "config": {"workload": f"square {args.n}x{args.n} -> {N} P{args.deg} triangles, "
+                               f"{D} ordinates ({args.npole}x{args.nazim}), "
+                               f"{args.precision} sweep",
+                   "n_tris": N, "ndof": P, "ndir": D, "dof_per_sweep": dof,
+                   "jitter": args.jitter},
+        "roofline": {"bound": "hbm", "achieved": bytes_alg / t_sweep / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": bytes_alg / t_sweep / HBM_PEAK, "traffic": None},
+        "sweep_ms_event_mean": t_sweep * 1e3,
+        "sweep_ms_event_min": float(np.min(ms)),
+        "schedule": s.ctx.schedule_info(),
+        "setup_s": setup_s,
+    }
+    if args.solve:
+        out["solve"] = solve_times(pkg, args, work)
+    if not args.no_cpu:
+        c64 = make_case(pkg, args.n, args.deg, args.npole, args.nazim, 1.0, 0.1, "fp64", work,
+                        jitter=args.jitter)
+        out["cpu_baseline"] = cpu_baseline(pkg, c64, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def solve_times(pkg, args, work):
+    res = {}
+    for prec in ("fp64", "fp32"):
+        case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, 1.0, 0.1, prec, work,
+                         jitter=args.jitter)
+        t0 = time.perf_counter()
+        s = pkg.Solver(case)
+        rec = s.run()
+        tts = time.perf_counter() - t0
+        res[prec] = {"time_to_solution_s": tts, "setup_s": s.setup_seconds,
+                     "iterations": rec.iterations, "converged": rec.converged,
+                     "solves_per_hour": 3600.0 / tts, "temp": rec.temp}
+    t64, t32 = res["fp64"].pop("temp"), res["fp32"].pop("temp")
+    res["fp32"]["rel_l2_T_vs_fp64"] = float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))
+    return res
+
+
+if __name__ == "__main__":
+    main()

@@ -133,6 +133,14 @@ int pbte_transport_residual(pbte_ctx *ctx, const double *ts, const double *qxs,
 int pbte_residual_iterate(pbte_ctx *ctx, const double *temp, const double *t_old,
                           double *res_host, double *mass_host);
 
+/* Periodic partner DOFs of the Krylov state (krylov.py:64-126): gather
+ * vdf[d][periodic_tri[f]][:] for every periodic face f (ascending) into
+ * out[D][K][P] (binary64), or scatter such a block into pbuf[D][F][P]
+ * (precision of the sweep).  Replaces FixedPointOperator.gather/scatter's
+ * periodic blocks (krylov.py:99-110). */
+int pbte_periodic_gather(pbte_ctx *ctx, int32_t precision, const void *vdf, double *out);
+int pbte_periodic_scatter(pbte_ctx *ctx, int32_t precision, const double *in, void *pbuf);
+
 /* ---- device vector kernels of the Krylov solver (krylov.py:150-246) --- */
 
 /* c[k] = sum_i Q[k][i] w[i], k < m (Q device [m][n], row stride ld);
@@ -146,6 +154,59 @@ int pbte_vec_update(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m,
 
 /* y[i] = a x[i] + b y[i] */
 int pbte_vec_axpby(pbte_ctx *ctx, double a, const double *x, double b, double *y, int64_t n);
+
+/* ---- GSIS with the HDG synthetic system (acceleration/*.py) ---------- */
+
+/* Host arrays read once by pbte_gsis_setup (index orders as the
+ * reference's Acceleration, acceleration/__init__.py:37-92). */
+typedef struct pbte_gsis_data {
+    const double *cxv, *cyv, *domega;     /* [D]                            */
+    const double *inv_aa_sol;             /* [N][7P][7P]                    */
+    const double *aa_trace;               /* [N][3][7P][3Q]                 */
+    const double *ba_sol;                 /* [N][3][3Q][7P]                 */
+    const double *tri_fc_fc;              /* int_tri_fc_fc [P][Q][N][3]     */
+    const double *tri_hmin;               /* [N]                            */
+    const int32_t *tri_faces;             /* [N][3]                         */
+    const int32_t *t1, *l1, *t2, *l2;     /* [F] contributing element sides */
+    const uint8_t *flip;                  /* [F] partner rows mirrored      */
+    int32_t n_wall, n_wall_th;            /* wall faces: thermalising first */
+    const int32_t *wall_face, *wall_tri, *wall_lfc;
+    const double *wall_nx, *wall_ny;
+    double tc5, tau_thr, drift;           /* tau_c*scale/5, TAU_THR,
+                                             tau_c/tau_n*3/(4 pi vg^2)     */
+    int32_t variant_b, use_hmin;
+} pbte_gsis_data;
+
+/* Uploads the GSIS operators.  Replaces Acceleration.__init__'s device-side
+ * state (acceleration/__init__.py:37-92). */
+int pbte_gsis_setup(pbte_ctx *ctx, const pbte_gsis_data *g);
+
+/* Seven angular moments per (dof, cell) in binary64, out [7][P][N]: the
+ * CIS moments (1, cx, cy) and the four third-order weights of the HoT
+ * source.  Replaces the direction loops of hot_source._hot_kernel and
+ * correction._correct_kernel (hot_source.py:29-58, correction.py:40-52). */
+int pbte_gsis_moments(pbte_ctx *ctx, int32_t precision, const void *vdf, double *mom7);
+
+/* AA_SRC [7P][N] = inv(AA_SOL) (stress rows of the HoT source).  Replaces
+ * hot_source.hot_source (hot_source.py:61-75). */
+int pbte_gsis_source(pbte_ctx *ctx, const double *mom7, double *aa_src);
+
+/* Trace right-hand side ffa [F][3Q] (two-sided, periodic mirror, wall rows
+ * from the kinetic traces).  Replaces GlobalSolver.rhs
+ * (global_solve.py:128-186). */
+int pbte_gsis_rhs(pbte_ctx *ctx, int32_t precision, const void *vdf, const double *aa_src,
+                  double *ffa);
+
+/* UQ [7P][N] = AA_SRC + sum AA_TRACE U_TRACE (u_trace [F][3Q]).  Replaces
+ * local_solve.local_solve (local_solve.py:16-26). */
+int pbte_gsis_local(pbte_ctx *ctx, const double *aa_src, const double *u_trace, double *uq);
+
+/* Damped kinetic correction of vdf and the moments; defect and mkin
+ * ([3][P][N]) may be NULL.  Replaces correction.correct_vdf
+ * (correction.py:29-98). */
+int pbte_gsis_correct(pbte_ctx *ctx, int32_t precision, void *vdf, const double *mom7,
+                      const double *uq, const double *defect, double *ts, double *qxs,
+                      double *qys, double *temp, double *qx, double *qy, double *mkin);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,72 @@
+"""GPU GSIS (HDG synthetic acceleration) against the reference's FROZEN
+regression rows (tests/integration/test_regression.py:29-48): iteration
+count exact and converged mass to 1e-12, on the shipped 200-triangle cavity
+(thermalising walls, north wall T = 1, tol 1e-8)."""
+import numpy as np
+import pytest
+
+from conftest import CASES
+
+pytestmark = pytest.mark.gpu
+
+FROZEN_GSIS = {
+    (1.0, 1e5, 3, 20, 40): (22, 0.2496873776739861),
+    (1e-1, 1e5, 3, 20, 40): (30, 0.24999647389562388),
+    (1e-2, 1e5, 1, 20, 40): (35, 0.24995968584823433),
+    (1e-2, 1e5, 2, 20, 40): (29, 0.24995682731416324),
+    (1e-2, 1e5, 3, 10, 20): (26, 0.24996286908141993),
+    (1e-3, 1e5, 3, 20, 40): (50, 0.24999752787448726),
+}
+
+
+def _shipped(pkg, tau_r, tau_n, deg, npole, nazim, precision="fp64"):
+    c = pkg.Case.from_yaml(CASES / "F1_1_1.yaml")
+    c.scheme.accflag = 1
+    c.flow.tau_r, c.flow.tau_n = tau_r, tau_n
+    c.dg.deg = deg
+    c.velmesh.npole, c.velmesh.nazim = npole, nazim
+    c.iteration.tol, c.iteration.tmax = 1e-8, 200000
+    c.performance.precision = precision
+    return c
+
+
+@pytest.mark.parametrize("key", list(FROZEN_GSIS), ids=lambda k: f"tau{k[0]:g}_deg{k[2]}_{k[3]}x{k[4]}")
+def test_frozen_gsis(pkg, key):
+    want_it, want_mass = FROZEN_GSIS[key]
+    print(f"case frozen GSIS {key}", flush=True)
+    rec = pkg.Solver(_shipped(pkg, *key)).run()
+    assert rec.converged
+    assert rec.iterations == want_it
+    assert abs(rec.mass - want_mass) <= 1e-12
+
+
+def test_gsis_adiabatic_cavity_conserves_energy(pkg):
+    """Adiabatic side walls: the converged cavity holds int T dA = 0.5
+    (test_deviations.py:63-65; global_solve.py comment: 30 iterations)."""
+    c = pkg.Case.from_yaml(CASES / "F2_1_1.yaml")
+    c.scheme.accflag = 1
+    c.flow.tau_r, c.flow.tau_n = 1e-2, 1e5
+    print("case GSIS adiabatic cavity", flush=True)
+    rec = pkg.Solver(c).run()
+    assert rec.converged
+    assert abs(rec.mass - 0.5) < 1e-6
+
+
+def test_gsis_periodic_channel_is_x_independent(pkg):
+    """As test_deviations.py:69-87: x-spread of T below 1e-3 of its range,
+    int T dA = 1/2 within 1e-5, a mirrored periodic pair present."""
+    c = pkg.Case.from_yaml(CASES / "F3_1_1.yaml")
+    c.scheme.accflag = 1
+    c.flow.tau_r, c.flow.tau_n = 1e-2, 1e5
+    print("case GSIS periodic channel", flush=True)
+    s = pkg.Solver(c)
+    rec = s.run()
+    assert rec.converged
+    # element mean temperature depends on y only: group elements by centroid y
+    xy = s.mesh.nodes[s.mesh.tri_nodes].mean(axis=1)
+    tbar = rec.temp / s.mesh.tri_area
+    key = np.round(xy[:, 1], 9)
+    spread = max(np.ptp(tbar[key == k]) for k in np.unique(key))
+    assert int(s.bcdata.periodic_flip.sum()) > 0
+    assert spread < 1e-3 * np.ptp(tbar)
+    assert abs(rec.mass - 0.5) < 1e-5

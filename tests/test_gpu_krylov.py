@@ -1,0 +1,52 @@
+"""GPU Krylov solves against the reference's certified fixed points
+(tasks/t01-square/reference/*.npz: transport residual ~1e-16) and sweep
+counts (F1/F2/F3_1_1: 21 / 23 / 27 sweeps, numerical_baseline.json)."""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import CASES, GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+SWEEPS = {"F1_1_1": 21, "F2_1_1": 23, "F3_1_1": 27}
+
+
+def _run(pkg, tag, precision="fp64", **kw):
+    c = pkg.Case.from_yaml(CASES / f"{tag}.yaml")
+    c.scheme.method = "krylov"
+    c.performance.precision = precision
+    for k, v in kw.items():
+        obj, attr = k.split("__")
+        setattr(getattr(c, obj), attr, v)
+    print(f"case krylov {tag} {precision} {kw}", flush=True)
+    return pkg.Solver(c).run()
+
+
+@pytest.mark.parametrize("tag", ["F1_1_1", "F2_1_1", "F3_1_1", "F1_1e-1"])
+def test_krylov_fp64_reaches_certified_fixed_point(pkg, tag):
+    rec = _run(pkg, tag)
+    z = np.load(GOLDEN / "krylov" / f"{tag}.npz")
+    idx = json.loads((GOLDEN / "krylov" / "index.json").read_text())[tag]
+    assert rec.converged
+    assert rec.diagnostics["transport_residual"] < 1e-8
+    assert abs(rec.iterations - idx["iterations"]) <= 1
+    np.testing.assert_allclose(rec.temp_dofs, z["temp_dofs"], rtol=0, atol=1e-9)
+    assert abs(rec.mass - float(z["mass"])) <= 1e-9
+
+
+@pytest.mark.parametrize("tag", ["F1_1_1", "F2_1_1", "F3_1_1", "F1_1e-2"])
+def test_krylov_fp32_refined_matches_fp64(pkg, tag):
+    """FP32 sweeps inside the FP64 defect correction reach the FP64 fixed
+    point within 1e-6 relative L2 on T and on the heat flux."""
+    rec = _run(pkg, tag, "fp32")
+    z = np.load(GOLDEN / "krylov" / f"{tag}.npz")
+    assert rec.converged
+    rel_t = np.linalg.norm(rec.temp - z["temp"]) / np.linalg.norm(z["temp"])
+    q = np.hypot(rec.qx, rec.qy)
+    qz = np.hypot(z["qx"], z["qy"])
+    rel_q = np.linalg.norm(q - qz) / np.linalg.norm(qz)
+    print(f"rel L2 T {rel_t:.2e} q {rel_q:.2e} sweeps fp64 {rec.diagnostics['sweeps_fp64']} "
+          f"fp32 {rec.diagnostics['sweeps_fp32']}", flush=True)
+    assert rel_t <= 1e-6 and rel_q <= 1e-6
