@@ -20,7 +20,9 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=fast",
+    # contraction only within one source expression: identical expressions give
+    # identical arithmetic in every kernel variant
+    "-ffp-contract=on",
     "-pthread",
 ]
 

@@ -57,6 +57,23 @@ template <> struct Deg<6> { static constexpr int v = 2; };
 template <> struct Deg<10> { static constexpr int v = 3; };
 template <> struct Deg<15> { static constexpr int v = 4; };
 
+// padded per-item / per-cell sizes so that every stream is read with
+// 16-byte vector loads
+__host__ __device__ constexpr int pad4(int n) { return (n + 3) & ~3; }
+template <int P> struct Lay {
+    static constexpr int Q = Deg<P>::v + 1;
+    static constexpr int LU = pad4(P * P);   // LU entries per (slot, direction)
+    static constexpr int SRC = pad4(3 * P);  // source values per cell
+    static constexpr int KQ = pad4(Q * Q);   // upwind face block per (cell, face)
+};
+// LU entry k of item (slot s, direction b of a bundle of B) in the
+// [k / V][slot][dir][V] layout, V = 16 / sizeof(T)
+template <typename T>
+__host__ __device__ constexpr size_t lu_at(int k, int s, int B, int b, int N)
+{
+    return (((size_t)(k / (16 / (int)sizeof(T))) * N + s) * B + b) * (16 / sizeof(T)) + (k % (16 / (int)sizeof(T)));
+}
+
 // face codes of CellGeo::info
 enum { FACE_INTERIOR = 0, FACE_PERIODIC = 1, FACE_ADIABATIC = 2, FACE_THERMAL = 3 };
 
@@ -118,6 +135,14 @@ struct pbte_ctx {
     int32_t *block_bundle = nullptr, *bundle_ndir = nullptr, *bundle_dir_off = nullptr,
             *bundle_dirs = nullptr, *slot_cell = nullptr, *lev_off = nullptr, *lev_ptr = nullptr;
     int64_t *lu_off = nullptr, *piv_off = nullptr;
+    int4* rec = nullptr;        // per-bundle per-slot records
+    int32_t* wstash = nullptr;  // per-bundle per-slot stash writes
+    uint8_t* bundle_global = nullptr;
+    int ring_slots = 0, stash_slots = 0;
+    int64_t global_bundles = 0;
+    int variant = 0;            // 0 pipelined level sweep, 1 plain level sweep
+    int max_bundle_dirs = 1;
+    int64_t ring_fallback = 0;  // upwind reads beyond the default LDS ring
     std::vector<int64_t> h_lu_off;
     int64_t lu_elems = 0, piv_elems = 0;
     // factors
@@ -128,6 +153,8 @@ struct pbte_ctx {
     // scratch
     double* src64 = nullptr;
     float* src32 = nullptr;
+    float* fw32 = nullptr;      // adiabatic emission, rows padded to 16 bytes, binary32
+    double* fw64 = nullptr;     // the same in binary64
     double* ssraw = nullptr;
     double* red = nullptr;     // reduction partials
     size_t red_len = 0;
@@ -271,9 +298,8 @@ __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
         for (int r = 0; r < P; ++r)
 #pragma unroll
             for (int c = 0; c < P; ++c) {
-                const size_t off = (size_t)base + ((size_t)(r * P + c) * N + s) * B + b;
-                if (a.lu64) a.lu64[off] = A[r][c];
-                if (a.lu32) a.lu32[off] = (float)A[r][c];
+                if (a.lu64) a.lu64[base + lu_at<double>(r * P + c, s, B, b, N)] = A[r][c];
+                if (a.lu32) a.lu32[base + lu_at<float>(r * P + c, s, B, b, N)] = (float)A[r][c];
             }
         a.piv[a.piv_off[bundle] + (size_t)s * B + b] = pv;
     }
@@ -306,7 +332,7 @@ __global__ void __launch_bounds__(256) source_kernel(const double* __restrict__ 
         vx += mlm * bx;
         vy += mlm * by;
     }
-    T* o = src + (size_t)i * 3 * P;
+    T* o = src + (size_t)i * Lay<P>::SRC;
     o[m] = (T)u;
     o[P + m] = (T)vx;
     o[2 * P + m] = (T)vy;
@@ -333,7 +359,36 @@ template <typename T> struct SweepArgs {
     const T* pbuf;
     T* vdf;
     int N, F, use_pbuf, homogeneous;
+    // pipelined variant: per-slot records {cell, face word x3}, LDS ring size
+    const int4* rec;
+    const T* flux_wall_t;       // adiabatic emission in the sweep precision
+    int lane_zero;              // 0 at run time; makes a uniform load per-lane
+    const int32_t* wstash;      // per slot: stash slot this item also writes, or -1
+    const uint8_t* bundle_global;
+    int ring, stash;            // LDS ring and stash slots (each slot B * P values)
 };
+
+// face word of a per-slot record: code in bits 28-30, value in bits 0-27
+// Face word of a per-slot record: code in bits 28-31, value in bits 0-27.
+// What an inflow face reads during the pipelined sweep:
+//   UP_RING / UP_STASH    upwind value of this sweep from the LDS ring (by
+//                         slot) or the LDS stash (by stash slot)
+//   OLD                   previous-sweep value of an element the reference
+//                         order visits later (broken cycle), global memory
+//   PER_RING/STASH/OLD    same for a periodic partner; with an explicit
+//                         periodic buffer the face reads that instead
+//   UP_GLOBAL/PER_GLOBAL  this sweep's value from global memory: only in a
+//                         bundle whose far reads overflow the stash, which
+//                         then runs with full memory barriers
+enum {
+    REC_UP_RING = 0, REC_UP_STASH = 1, REC_ADIABATIC = 2, REC_THERMAL = 3, REC_OLD = 4,
+    REC_PER_RING = 5, REC_PER_STASH = 6, REC_PER_OLD = 7, REC_UP_GLOBAL = 8, REC_PER_GLOBAL = 9,
+    REC_NONE = 15
+};
+__host__ __device__ constexpr bool rec_reads_neighbour(int code)
+{
+    return code == REC_UP_RING || code == REC_UP_STASH || (code >= REC_OLD && code <= REC_PER_GLOBAL);
+}
 
 template <int P, typename T>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
@@ -359,10 +414,28 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
         if (s < a.lev_ptr[lv + 1]) {
             const int i = cells[s];
             T x[P];
-            const T* sv = a.src + (size_t)i * 3 * P;
+            const T* sv = a.src + (size_t)i * Lay<P>::SRC;
 #pragma unroll
             for (int m = 0; m < P; ++m) x[m] = sv[m] + cx * sv[P + m] + cy * sv[2 * P + m];
             const CellGeo<T> g = a.geo[i];
+            // wall inflow first (face order), then the upwind neighbours (face
+            // order); the pipelined kernel applies the same sequence
+#pragma unroll
+            for (int il = 0; il < 3; ++il) {
+                const T sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
+                const T w = T(0.5) * (sp - fabs(sp));
+                if (w == T(0)) continue;
+                const int code = g.info[il] & 3;
+                if (code == FACE_ADIABATIC) {
+                    const double* fw = a.flux_wall + (size_t)g.other[il] * P;
+#pragma unroll
+                    for (int m = 0; m < P; ++m) x[m] += w * (T)fw[m];
+                } else if (code == FACE_THERMAL && !a.homogeneous) {
+                    const T* bv = a.bvec + (size_t)g.other[il] * pad4(P);
+#pragma unroll
+                    for (int m = 0; m < P; ++m) x[m] -= w * bv[m];
+                }
+            }
 #pragma unroll
             for (int il = 0; il < 3; ++il) {
                 const T sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
@@ -378,7 +451,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
                     T fb[Q];
 #pragma unroll
                     for (int q = 0; q < Q; ++q) fb[q] = fe[face_node(DEG, ilp, q)];
-                    const T* Kf = a.K + ((size_t)i * 3 + il) * Q * Q;
+                    const T* Kf = a.K + ((size_t)i * 3 + il) * Lay<P>::KQ;
 #pragma unroll
                     for (int r = 0; r < Q; ++r) {
                         T acc = x[face_node(DEG, il, r)];
@@ -386,14 +459,6 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
                         for (int q = 0; q < Q; ++q) acc -= (w * Kf[r * Q + q]) * fb[q];
                         x[face_node(DEG, il, r)] = acc;
                     }
-                } else if (code == FACE_ADIABATIC) {
-                    const double* fw = a.flux_wall + (size_t)g.other[il] * P;
-#pragma unroll
-                    for (int m = 0; m < P; ++m) x[m] += w * (T)fw[m];
-                } else if (!a.homogeneous) {
-                    const T* bv = a.bvec + (size_t)g.other[il] * P;
-#pragma unroll
-                    for (int m = 0; m < P; ++m) x[m] -= w * bv[m];
                 }
             }
             // P b: DGETRS row interchanges
@@ -405,28 +470,322 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
                 for (int r = k + 1; r < P; ++r)
                     if (p == r) { T tmp = x[k]; x[k] = x[r]; x[r] = tmp; }
             }
-            const size_t ld = (size_t)N * B;
-            const T* e = lu + (size_t)s * B + b;
             // unit lower solve
 #pragma unroll
             for (int k = 0; k < P; ++k) {
                 const T xk = x[k];
 #pragma unroll
-                for (int r = k + 1; r < P; ++r) x[r] -= xk * e[(size_t)(r * P + k) * ld];
+                for (int r = k + 1; r < P; ++r) x[r] -= xk * lu[lu_at<T>(r * P + k, s, B, b, N)];
             }
             // upper solve, accumulated downwards (DTRSM column order)
 #pragma unroll
             for (int k = P - 1; k >= 0; --k) {
-                x[k] = x[k] * e[(size_t)(k * P + k) * ld];
+                x[k] = x[k] * lu[lu_at<T>(k * P + k, s, B, b, N)];
                 const T xk = x[k];
 #pragma unroll
-                for (int r = 0; r < k; ++r) x[r] -= xk * e[(size_t)(r * P + k) * ld];
+                for (int r = 0; r < k; ++r) x[r] -= xk * lu[lu_at<T>(r * P + k, s, B, b, N)];
             }
             T* o = vdf_d + (size_t)i * P;
 #pragma unroll
             for (int m = 0; m < P; ++m) o[m] = x[m];
         }
         __syncthreads();
+    }
+}
+
+
+// --------------------------------------------------------------------------
+// software-pipelined level sweep.  Same per-item arithmetic as sweep_kernel
+// (the results are bitwise identical); what changes is when data arrives:
+//   * records {cell, face words} of level lv+2 are loaded while level lv
+//     is solved, the cell data, LU and pivots of level lv+1 likewise;
+//   * upwind traces written during the last `ring` slots are read from an
+//     LDS ring instead of global memory (global fallback beyond it).
+// --------------------------------------------------------------------------
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+
+// n contiguous values (n a multiple of 16 / sizeof(T), 16-byte aligned)
+// with 16-byte loads
+template <typename T, int n>
+__device__ __forceinline__ void load16(T* dst, const T* src)
+{
+    using V = typename Vec16<T>::type;
+    constexpr int w = 16 / sizeof(T);
+    const V* s = reinterpret_cast<const V*>(src);
+#pragma unroll
+    for (int k = 0; k < n / w; ++k) {
+        const V v = s[k];
+        const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+        for (int j = 0; j < w; ++j) dst[k * w + j] = e[j];
+    }
+}
+
+// One level's worth of a thread's item.  The gather stage only issues
+// loads whose addresses come from the record loaded a level earlier; every
+// use of the loaded data happens in the solve stage a level later.  (The
+// in-order vmcnt counter makes any earlier use wait for all prefetches in
+// flight.)  The three per-face sources are chosen by pointer and read with
+// one vector load into one destination, so no destination is picked at run
+// time (a pointer select between local arrays would put them in scratch).
+// Wall rows are pad4(P) long and read KQ >= pad4(P) values; their buffers
+// carry KQ values of tail padding.
+template <int P, typename T, bool PFLU> struct PipeData {
+    using L = Lay<P>;
+    bool valid;
+    int slot, cell, wst;
+    int fw[3];
+    int info[3];
+    T nrm[6];
+    T src[L::SRC];
+    T fd[3][L::KQ];     // upwind face block, or the wall vector of a wall face
+    T lu[PFLU ? L::LU : 4];
+    uint64_t piv;
+};
+struct PipeRec {
+    bool valid;
+    int slot, wst;
+    int4 r;
+};
+
+template <typename T>
+__device__ __forceinline__ void pipe_record(const int4* rec, const int32_t* wst, int2 span, int c, PipeRec& o)
+{
+    o.slot = span.x + c;
+    o.valid = o.slot < span.y;
+    if (o.valid) {
+        o.r = rec[o.slot];
+        o.wst = wst[o.slot];
+    }
+}
+
+template <int P, typename T, bool PFLU>
+__device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec& r, int B, int b, const T* lu,
+                                            const uint64_t* piv, PipeData<P, T, PFLU>& o)
+{
+    using L = Lay<P>;
+    o.valid = r.valid;
+    if (!r.valid) return;
+    const int i = r.r.x;
+    o.slot = r.slot;
+    o.cell = i;
+    o.wst = r.wst;
+    o.fw[0] = r.r.y;
+    o.fw[1] = r.r.z;
+    o.fw[2] = r.r.w;
+    load16<T, L::SRC>(o.src, a.src + (size_t)i * L::SRC);
+    const CellGeo<T> g = a.geo[i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o.nrm[k] = g.n[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.info[k] = g.info[k];
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        const int code = (int)(((unsigned)o.fw[il]) >> 28);
+        const int val = o.fw[il] & 0x0fffffff;
+        const T* srcp = nullptr;
+        if (rec_reads_neighbour(code))
+            srcp = a.K + ((size_t)i * 3 + il) * L::KQ;
+        else if (code == REC_ADIABATIC)
+            srcp = a.flux_wall_t + (size_t)val * pad4(P);
+        else if (code == REC_THERMAL && !a.homogeneous)
+            srcp = a.bvec + (size_t)val * pad4(P);
+        if (srcp) load16<T, L::KQ>(o.fd[il], srcp);
+    }
+    o.piv = piv[(size_t)r.slot * B + b];
+    if (PFLU) {
+        using V = typename Vec16<T>::type;
+        constexpr int w = 16 / sizeof(T);
+        const V* e = reinterpret_cast<const V*>(lu) + (size_t)r.slot * B + b;
+        const size_t ld = (size_t)a.N * B;
+#pragma unroll
+        for (int k = 0; k < L::LU / w; ++k) {
+            const V v = e[(size_t)k * ld];
+            const T* t = reinterpret_cast<const T*>(&v);
+#pragma unroll
+            for (int j = 0; j < w; ++j) o.lu[k * w + j] = t[j];
+        }
+    }
+}
+
+template <int P, typename T, bool PFLU>
+__device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData<P, T, PFLU>& dC, T cx, T cy, int d,
+                                           int B, int b, const T* lu, T* ring, int RM, T* vdf_d)
+{
+    constexpr int DEG = Deg<P>::v;
+    constexpr int Q = DEG + 1;
+    if (!dC.valid) return;
+    const int N = a.N;
+    T x[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) x[m] = dC.src[m] + cx * dC.src[P + m] + cy * dC.src[2 * P + m];
+    T w[3];
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        const T sp = cx * dC.nrm[2 * il] + cy * dC.nrm[2 * il + 1];
+        w[il] = T(0.5) * (sp - fabs(sp));
+    }
+    // wall inflow in face order
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        const int code = (int)(((unsigned)dC.fw[il]) >> 28);
+        if (w[il] == T(0)) continue;
+        if (code == REC_ADIABATIC) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) x[m] += w[il] * dC.fd[il][m];
+        } else if (code == REC_THERMAL && !a.homogeneous) {
+#pragma unroll
+            for (int m = 0; m < P; ++m) x[m] -= w[il] * dC.fd[il][m];
+        }
+    }
+    // upwind neighbours in face order
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        const int code = (int)(((unsigned)dC.fw[il]) >> 28);
+        if (!rec_reads_neighbour(code) || w[il] == T(0)) continue;
+        const int val = dC.fw[il] & 0x0fffffff;
+        const int ilp = (dC.info[il] >> 2) & 3;
+        const bool periodic = code >= REC_PER_RING && code != REC_UP_GLOBAL;
+        // the trace is applied inside each branch: merging an LDS-loaded
+        // and a global-loaded value would make the use wait on both counters
+        if ((periodic && a.use_pbuf) || code == REC_OLD || code == REC_PER_OLD || code == REC_UP_GLOBAL ||
+            code == REC_PER_GLOBAL) {
+            // global memory: explicit periodic buffer, a previous-sweep value
+            // (broken cycle) or a bundle running with full barriers
+            const T* fe = (periodic && a.use_pbuf) ? a.pbuf + ((size_t)d * a.F + (dC.info[il] >> 4)) * P
+                                                   : vdf_d + (size_t)val * P;
+            T fb[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fb[q] = fe[face_node(DEG, ilp, q)];
+#pragma unroll
+            for (int r = 0; r < Q; ++r) {
+                T acc = x[face_node(DEG, il, r)];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc -= (w[il] * dC.fd[il][r * Q + q]) * fb[q];
+                x[face_node(DEG, il, r)] = acc;
+            }
+        } else {
+            const int slot = (code == REC_UP_RING || code == REC_PER_RING) ? (val & RM) : a.ring + val;
+            const T* fe = ring + ((size_t)slot * B + b) * P;     // the stash follows the ring
+            T fb[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fb[q] = fe[face_node(DEG, ilp, q)];
+#pragma unroll
+            for (int r = 0; r < Q; ++r) {
+                T acc = x[face_node(DEG, il, r)];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc -= (w[il] * dC.fd[il][r * Q + q]) * fb[q];
+                x[face_node(DEG, il, r)] = acc;
+            }
+        }
+    }
+    const uint64_t pv = dC.piv;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int p = (int)((pv >> (4 * k)) & 15u);
+#pragma unroll
+        for (int r = k + 1; r < P; ++r)
+            if (p == r) { T tmp = x[k]; x[k] = x[r]; x[r] = tmp; }
+    }
+    const int s = dC.slot;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const T xk = x[k];
+#pragma unroll
+        for (int r = k + 1; r < P; ++r)
+            x[r] -= xk * (PFLU ? dC.lu[r * P + k] : lu[lu_at<T>(r * P + k, s, B, b, N)]);
+    }
+#pragma unroll
+    for (int k = P - 1; k >= 0; --k) {
+        x[k] = x[k] * (PFLU ? dC.lu[k * P + k] : lu[lu_at<T>(k * P + k, s, B, b, N)]);
+        const T xk = x[k];
+#pragma unroll
+        for (int r = 0; r < k; ++r) x[r] -= xk * (PFLU ? dC.lu[r * P + k] : lu[lu_at<T>(r * P + k, s, B, b, N)]);
+    }
+    T* o = vdf_d + (size_t)dC.cell * P;
+    T* rs = ring + ((size_t)(s & RM) * B + b) * P;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        o[m] = x[m];
+        rs[m] = x[m];
+    }
+    if (dC.wst >= 0) {
+        T* st = ring + ((size_t)(a.ring + dC.wst) * B + b) * P;
+#pragma unroll
+        for (int m = 0; m < P; ++m) st[m] = x[m];
+    }
+}
+
+__device__ __forceinline__ void level_barrier(bool global_mode)
+{
+    if (global_mode) {
+        __syncthreads();
+    } else {
+        // every value exchanged within the sweep lives in LDS: order LDS
+        // traffic only and leave the prefetch loads in flight
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
+}
+
+// --------------------------------------------------------------------------
+// software-pipelined level sweep.  Same per-item arithmetic as sweep_kernel
+// (bitwise identical results).  While level lv is solved, the records of
+// level lv+2 and the cell data, LU and pivots of level lv+1 are in flight;
+// the loop is unrolled by two over two named register sets so that no
+// register copy forces an early wait.  Upwind values of this sweep come from
+// the LDS ring / stash; the barrier orders LDS only.
+// --------------------------------------------------------------------------
+template <int P, typename T, bool PFLU>
+__global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int bundle = a.block_bundle[blockIdx.x];
+    const int B = a.bundle_ndir[bundle];
+    T* ring = reinterpret_cast<T*>(lds_raw);        // ring slots, then stash slots
+    const int RM = a.ring - 1;
+    const bool global_mode = a.bundle_global[bundle] != 0;
+    const int t = threadIdx.x;
+    const int b = t % B;
+    const int c = t / B;
+    const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
+    const T cx = a.cxv[d], cy = a.cyv[d];
+    const int N = a.N;
+    const int4* rec = a.rec + (size_t)bundle * N;
+    const int32_t* wst = a.wstash + (size_t)bundle * N;
+    const T* lu = a.lu + a.lu_off[bundle];
+    const uint64_t* piv = a.piv + a.piv_off[bundle];
+    T* vdf_d = a.vdf + (size_t)d * N * P;
+    const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
+
+    // level spans travel two levels ahead of their use, the records one
+    // level ahead of the gather; every stage alternates between two named
+    // register sets so that no copy forces a wait.  The spans are read with
+    // per-lane (vector) loads: a scalar load would share lgkmcnt with the
+    // LDS traffic that every level waits for.
+    const int32_t* lp = a.lev_ptr + (t & a.lane_zero);
+    auto span = [&](int lv) { return lv < l1 ? make_int2(lp[lv], lp[lv + 1]) : make_int2(0, 0); };
+    int2 S0 = span(l0 + 2), S1 = span(l0 + 3);
+    PipeRec R0, R1;
+    PipeData<P, T, PFLU> D0, D1;
+    pipe_record<T>(rec, wst, span(l0), c, R0);
+    pipe_gather<P, T, PFLU>(a, R0, B, b, lu, piv, D0);
+    pipe_record<T>(rec, wst, span(l0 + 1), c, R1);
+    for (int lv = l0; lv < l1; lv += 2) {
+        pipe_record<T>(rec, wst, S0, c, R0);                             // level lv+2
+        S0 = span(lv + 4);
+        pipe_gather<P, T, PFLU>(a, R1, B, b, lu, piv, D1);               // level lv+1
+        pipe_solve<P, T, PFLU>(a, D0, cx, cy, d, B, b, lu, ring, RM, vdf_d);   // level lv
+        level_barrier(global_mode);
+        if (lv + 1 >= l1) break;
+        pipe_record<T>(rec, wst, S1, c, R1);                             // level lv+3
+        S1 = span(lv + 5);
+        pipe_gather<P, T, PFLU>(a, R0, B, b, lu, piv, D0);               // level lv+2
+        pipe_solve<P, T, PFLU>(a, D1, cx, cy, d, B, b, lu, ring, RM, vdf_d);   // level lv+1
+        level_barrier(global_mode);
     }
 }
 
@@ -532,7 +891,7 @@ __global__ void __launch_bounds__(256) residual_kernel(
         const double cx = cxv[d], cy = cyv[d];
         const double* vd = vdf + (size_t)d * N * P;
         double b[P], f[P];
-        const double* sv = src + (size_t)i * 3 * P;
+        const double* sv = src + (size_t)i * Lay<P>::SRC;
 #pragma unroll
         for (int m = 0; m < P; ++m) {
             b[m] = sv[m] + cx * sv[P + m] + cy * sv[2 * P + m];
@@ -551,7 +910,7 @@ __global__ void __launch_bounds__(256) residual_kernel(
             if (code <= FACE_PERIODIC) {
                 const int ilp = (info >> 2) & 3;
                 const double* fe = vd + (size_t)g.other[il] * P;
-                const double* Kf = K + ((size_t)i * 3 + il) * Q * Q;
+                const double* Kf = K + ((size_t)i * 3 + il) * Lay<P>::KQ;
 #pragma unroll
                 for (int r = 0; r < Q; ++r) {
                     double acc = b[face_node(DEG, il, r)];
@@ -564,7 +923,7 @@ __global__ void __launch_bounds__(256) residual_kernel(
                 for (int m = 0; m < P; ++m) b[m] += w * flux_wall[(size_t)g.other[il] * P + m];
             } else {
 #pragma unroll
-                for (int m = 0; m < P; ++m) b[m] -= w * bvec[(size_t)g.other[il] * P + m];
+                for (int m = 0; m < P; ++m) b[m] -= w * bvec[(size_t)g.other[il] * pad4(P) + m];
             }
         }
         const double* fc = fcm + (size_t)i * 3 * P * P;
@@ -945,6 +1304,17 @@ __global__ void __launch_bounds__(256) dof_integrals_kernel(const double* ts, co
     qy[i] = y;
 }
 
+// [rows][P] binary64 -> [rows][pad4(P)] in T
+template <typename T>
+__global__ void __launch_bounds__(256) pad_rows_kernel(const double* in, T* out, int rows, int P)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)rows * P) return;
+    const int64_t r = g / P;
+    const int m = (int)(g - r * P);
+    out[r * pad4(P) + m] = (T)in[g];
+}
+
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <typename F>
@@ -1022,8 +1392,9 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         // --- compact cell data ------------------------------------------------
         std::vector<CellGeo<double>> g64(N);
         std::vector<CellGeo<float>> g32(N);
-        std::vector<double> K((size_t)N * 3 * Q * Q, 0.0);
-        std::vector<double> bvec((size_t)std::max(F, 1) * P, 0.0);
+        const int KQ = pad4(Q * Q);
+        std::vector<double> K((size_t)N * 3 * KQ, 0.0);
+        std::vector<double> bvec((size_t)std::max(F, 1) * pad4(P) + 32, 0.0);
         double worst = 0.0;
         for (int i = 0; i < N; ++i) {
             for (int il = 0; il < 3; ++il) {
@@ -1050,7 +1421,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                         other = fc;
                         const double tw = su->cv / 4.0 / su->pi * su->bc_temp[bc];
                         const double* tt = su->ttfc + ((size_t)i * 3 + il) * P * P;
-                        for (int m = 0; m < P; ++m) bvec[(size_t)fc * P + m] = tw * tt[m * P + 0];
+                        for (int m = 0; m < P; ++m) bvec[(size_t)fc * pad4(P) + m] = tw * tt[m * P + 0];
                     } else {
                         throw std::string("boundary type not supported by the sweep");
                     }
@@ -1073,7 +1444,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                     if (bmax > 0) worst = std::max(worst, omax / bmax);
                     for (int a = 0; a < Q; ++a)
                         for (int q = 0; q < Q; ++q)
-                            K[(((size_t)i * 3 + il) * Q + a) * Q + q] =
+                            K[((size_t)i * 3 + il) * KQ + a * Q + q] =
                                 tt[face_node(deg, il, a) * P + face_node(deg, ilp, q)];
                 }
                 g64[i].other[il] = other;
@@ -1171,6 +1542,24 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         std::vector<int64_t> h_lu_off(nb + 1, 0), h_piv_off(nb + 1, 0);
         std::vector<std::vector<int32_t>> slots(nb);
         std::vector<std::vector<int32_t>> lptr(nb);
+        std::vector<int4> h_rec((size_t)nb * N);
+        std::vector<int32_t> h_wst((size_t)nb * N, -1);
+        std::vector<uint8_t> global_flag(nb, 0);
+        std::vector<int64_t> far_reads(nb, 0);
+        {
+            // LDS budget per workgroup: 32 KiB ring + 32 KiB stash at binary64
+            int bm = 1;
+            for (auto& bb : bundles) bm = std::max(bm, (int)bb.size());
+            const int slot_bytes = bm * P * 8;
+            int r = 1;
+            while ((r * 2) * slot_bytes <= 32768) r *= 2;
+            ctx->ring_slots = r;
+            ctx->stash_slots = std::max(1, 32768 / slot_bytes);
+            if (su->lds_slots_cap > 0) {
+                while (ctx->ring_slots > su->lds_slots_cap) ctx->ring_slots /= 2;
+                ctx->stash_slots = std::min(ctx->stash_slots, su->lds_slots_cap);
+            }
+        }
         std::atomic_int next{0};
         std::vector<std::string> errs(nb);
         auto work = [&]() {
@@ -1199,6 +1588,100 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 for (int l = 0; l < nlev; ++l)
                     for (int s = cnt[l]; s < cnt[l + 1]; s += width) lp.push_back(s);
                 lp.push_back(N);
+                // sublevel index and end slot of every slot
+                std::vector<int32_t> sub_of((size_t)N), end_of((size_t)N);
+                for (size_t j = 0; j + 1 < lp.size(); ++j)
+                    for (int q = lp[j]; q < lp[j + 1]; ++q) {
+                        sub_of[q] = (int32_t)j;
+                        end_of[q] = lp[j + 1];
+                    }
+                // per-slot records: the element and, per local face, what the
+                // sweep reads through it (REC_*).  Upwind values of this sweep
+                // come from the LDS ring when written within the last `ring`
+                // slots, else from an LDS stash slot reused once its last
+                // reader's sublevel has passed; a bundle whose stash would
+                // overflow reads them from global memory instead.
+                std::vector<int32_t> pos((size_t)N), slot_of((size_t)N);
+                for (int p = 0; p < N; ++p) pos[order[p]] = p;
+                for (int q = 0; q < N; ++q) slot_of[sl[q]] = q;
+                const double cxd = su->cxv[d], cyd = su->cyv[d];
+                struct Read { int code, val; };
+                std::vector<Read> rd((size_t)N * 3);
+                std::vector<int32_t> last_reader((size_t)N, -1);
+                for (int q = 0; q < N; ++q) {
+                    const int i = sl[q];
+                    for (int il = 0; il < 3; ++il) {
+                        Read r{REC_NONE, 0};
+                        const double sp = cxd * su->nx[i * 3 + il] + cyd * su->ny[i * 3 + il];
+                        if (sp < 0.0) {
+                            const int fc = su->tri_faces[i * 3 + il];
+                            const int bc = su->face_bc[fc];
+                            const int typ = bc < 0 ? 0 : su->bc_type[bc];
+                            if (bc < 0 || typ == 3) {
+                                const int j = bc < 0 ? su->neighbour[i * 3 + il] : su->periodic_tri[fc];
+                                const bool per = bc >= 0;
+                                if (pos[j] < pos[i]) {
+                                    const int u = slot_of[j];
+                                    if (u >= end_of[q] - ctx->ring_slots) {
+                                        r = {per ? REC_PER_RING : REC_UP_RING, u};
+                                    } else {
+                                        r = {per ? REC_PER_STASH : REC_UP_STASH, u};   // slot for now
+                                        last_reader[u] = std::max(last_reader[u], sub_of[q]);
+                                        ++far_reads[k];
+                                    }
+                                } else {
+                                    r = {per ? REC_PER_OLD : REC_OLD, j};
+                                }
+                            } else {
+                                r = {typ == 2 ? REC_ADIABATIC : REC_THERMAL, fc};
+                            }
+                        }
+                        rd[(size_t)q * 3 + il] = r;
+                    }
+                }
+                // stash slots: interval reuse in slot order
+                std::vector<int32_t> sid((size_t)N, -1);
+                bool overflow = false;
+                {
+                    std::vector<std::pair<int, int>> busy;     // (last reader sublevel, slot)
+                    std::vector<int> freelist;
+                    int used = 0;
+                    for (int u = 0; u < N && !overflow; ++u) {
+                        if (last_reader[u] < 0) continue;
+                        for (size_t e = 0; e < busy.size();) {
+                            if (busy[e].first < sub_of[u]) {
+                                freelist.push_back(busy[e].second);
+                                busy[e] = busy.back();
+                                busy.pop_back();
+                            } else {
+                                ++e;
+                            }
+                        }
+                        int id;
+                        if (!freelist.empty()) { id = freelist.back(); freelist.pop_back(); }
+                        else if (used < ctx->stash_slots) id = used++;
+                        else { overflow = true; break; }
+                        sid[u] = id;
+                        busy.push_back({last_reader[u], id});
+                    }
+                }
+                global_flag[k] = overflow ? 1 : 0;
+                for (int q = 0; q < N; ++q) {
+                    int w[3];
+                    for (int il = 0; il < 3; ++il) {
+                        Read r = rd[(size_t)q * 3 + il];
+                        if (overflow) {
+                            // everything of this sweep through global memory
+                            if (r.code == REC_UP_RING || r.code == REC_UP_STASH) r = {REC_UP_GLOBAL, sl[r.val]};
+                            else if (r.code == REC_PER_RING || r.code == REC_PER_STASH) r = {REC_PER_GLOBAL, sl[r.val]};
+                        } else if (r.code == REC_UP_STASH || r.code == REC_PER_STASH) {
+                            r.val = sid[r.val];
+                        }
+                        w[il] = (int)(((unsigned)r.code << 28) | (unsigned)r.val);
+                    }
+                    h_rec[(size_t)k * N + q] = make_int4(sl[q], w[0], w[1], w[2]);
+                    h_wst[(size_t)k * N + q] = overflow ? -1 : sid[q];
+                }
             }
         };
         {
@@ -1208,6 +1691,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             for (auto& x : th) x.join();
         }
         int64_t maxw = 0;
+        for (int k = 0; k < nb; ++k) ctx->max_bundle_dirs = std::max(ctx->max_bundle_dirs, (int)bundles[k].size());
         for (int k = 0; k < nb; ++k) {
             const int B = (int)bundles[k].size();
             h_ndir[k] = B;
@@ -1222,7 +1706,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 maxw = std::max<int64_t>(maxw, (int64_t)(lptr[k][j + 1] - lptr[k][j]) * B);
             }
             h_lev_ptr.push_back(N);
-            h_lu_off[k + 1] = h_lu_off[k] + (int64_t)P * P * N * B;
+            h_lu_off[k + 1] = h_lu_off[k] + (int64_t)pad4(P * P) * N * B;
             h_piv_off[k + 1] = h_piv_off[k] + (int64_t)N * B;
         }
         // lev_off[k]..lev_off[k]+nlev_k-1 index level starts; lev_ptr[j+1] is
@@ -1263,15 +1747,28 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->lev_off = ctx->upload(h_lev_se.data(), h_lev_se.size());
         ctx->lev_ptr = ctx->upload(h_lev_ptr.data(), h_lev_ptr.size());
         ctx->lu_off = ctx->upload(h_lu_off.data(), nb + 1);
+        ctx->rec = ctx->upload(h_rec.data(), h_rec.size());
+        ctx->wstash = ctx->upload(h_wst.data(), h_wst.size());
+        ctx->bundle_global = ctx->upload(global_flag.data(), global_flag.size());
+        for (int k = 0; k < nb; ++k) ctx->global_bundles += global_flag[k];
+        for (int k = 0; k < nb; ++k) ctx->ring_fallback += far_reads[k];
+        ctx->variant = su->sweep_variant;
+        if (N >= (1 << 28) || F >= (1 << 28)) throw std::string("mesh too large for 28-bit record fields");
         ctx->piv_off = ctx->upload(h_piv_off.data(), nb + 1);
         ctx->h_lu_off = h_lu_off;
         ctx->lu_elems = h_lu_off[nb];
         ctx->piv_elems = h_piv_off[nb];
 
         // --- scratch -----------------------------------------------------------
-        ctx->src64 = ctx->alloc<double>((size_t)N * 3 * P);
-        ctx->src32 = ctx->alloc<float>((size_t)N * 3 * P);
+        ctx->src64 = ctx->alloc<double>((size_t)N * pad4(3 * P));
+        ctx->src32 = ctx->alloc<float>((size_t)N * pad4(3 * P));
+        HIP_TRY(hipMemset(ctx->src64, 0, sizeof(double) * (size_t)N * pad4(3 * P)));
+        HIP_TRY(hipMemset(ctx->src32, 0, sizeof(float) * (size_t)N * pad4(3 * P)));
         ctx->ssraw = ctx->alloc<double>((size_t)N * P);
+        ctx->fw32 = ctx->alloc<float>((size_t)std::max(F, 1) * pad4(P) + 32);
+        ctx->fw64 = ctx->alloc<double>((size_t)std::max(F, 1) * pad4(P) + 32);
+        HIP_TRY(hipMemset(ctx->fw32, 0, sizeof(float) * ((size_t)std::max(F, 1) * pad4(P) + 32)));
+        HIP_TRY(hipMemset(ctx->fw64, 0, sizeof(double) * ((size_t)std::max(F, 1) * pad4(P) + 32)));
         ctx->red_len = std::max<size_t>((size_t)D * nblocks(N, 256) * 2 + 2 * (size_t)D,
                                         (size_t)kMaxBasis * kDotBlocks + kMaxBasis) + 64;
         ctx->red = ctx->alloc<double>(ctx->red_len);
@@ -1288,13 +1785,15 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
     return ctx;
 }
 
-extern "C" int pbte_schedule_info(pbte_ctx* ctx, int64_t* out4)
+extern "C" int pbte_schedule_info(pbte_ctx* ctx, int64_t* out6)
 {
     return guarded(ctx, [&]() {
-        out4[0] = ctx->total_levels;
-        out4[1] = ctx->nb;
-        out4[2] = ctx->max_width;
-        out4[3] = ctx->lu_elems;
+        out6[0] = ctx->total_levels;
+        out6[1] = ctx->nb;
+        out6[2] = ctx->max_width;
+        out6[3] = ctx->ring_fallback;
+        out6[4] = ctx->global_bundles;
+        out6[5] = ctx->ring_slots;
         return 0;
     });
 }
@@ -1316,8 +1815,8 @@ void launch_factor(pbte_ctx* ctx, double* lu64, float* lu32)
 
 template <int P, typename T>
 void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const CellGeo<T>* geo, const T* K,
-                  const T* bvec, const T* src, const double* flux_wall, const T* pbuf, T* vdf, int use_pbuf,
-                  int homogeneous)
+                  const T* bvec, const T* src, const double* flux_wall, const T* flux_wall_t, const T* pbuf, T* vdf,
+                  int use_pbuf, int homogeneous)
 {
     SweepArgs<T> a{};
     a.block_bundle = ctx->block_bundle; a.bundle_ndir = ctx->bundle_ndir; a.bundle_dir_off = ctx->bundle_dir_off;
@@ -1325,7 +1824,20 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
     a.lev_ptr = ctx->lev_ptr; a.lu_off = ctx->lu_off; a.piv_off = ctx->piv_off; a.lu = lu; a.piv = ctx->piv;
     a.cxv = cx; a.cyv = cy; a.geo = geo; a.K = K; a.bvec = bvec; a.src = src; a.flux_wall = flux_wall;
     a.pbuf = pbuf; a.vdf = vdf; a.N = ctx->N; a.F = ctx->F; a.use_pbuf = use_pbuf; a.homogeneous = homogeneous;
-    hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
+    a.rec = ctx->rec;
+    a.flux_wall_t = flux_wall_t;
+    a.lane_zero = 0;
+    if (ctx->variant == 1) {
+        hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
+    } else {
+        a.ring = ctx->ring_slots;
+        a.stash = ctx->stash_slots;
+        a.wstash = ctx->wstash;
+        a.bundle_global = ctx->bundle_global;
+        const size_t lds = (size_t)(ctx->ring_slots + ctx->stash_slots) * ctx->max_bundle_dirs * P * sizeof(T);
+        constexpr bool pf = P * P * sizeof(T) <= 400;
+        hipLaunchKernelGGL((sweep_pipe_kernel<P, T, pf>), dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
+    }
     HIP_TRY(hipGetLastError());
 }
 
@@ -1384,18 +1896,30 @@ extern "C" int pbte_sweep(pbte_ctx* ctx, int32_t precision, const double* ts, co
         if (precision == PBTE_FP64) {
             if (!ctx->have64) throw std::string("FP64 factors not built; call pbte_factorise");
             launch_source<double>(ctx, ts, qxs, qys, ctx->src64);
+            if (ctx->n_wall) {
+                hipLaunchKernelGGL(pad_rows_kernel<double>, dim3(nblocks((int64_t)ctx->F * ctx->P, 256)), dim3(256), 0,
+                                   ctx->stream, flux_wall, ctx->fw64, ctx->F, ctx->P);
+                HIP_TRY(hipGetLastError());
+            }
             switch (ctx->P) {
 #define SW(PP) case PP: launch_sweep<PP, double>(ctx, ctx->lu64, ctx->cxv, ctx->cyv, ctx->geo64, ctx->K64, \
-                    ctx->bvec64, ctx->src64, flux_wall, (const double*)pbuf, (double*)vdf, use_pbuf, homogeneous); break;
+                    ctx->bvec64, ctx->src64, flux_wall, ctx->fw64, (const double*)pbuf, (double*)vdf, use_pbuf, \
+                    homogeneous); break;
             SW(3) SW(6) SW(10) SW(15)
 #undef SW
             }
         } else if (precision == PBTE_FP32) {
             if (!ctx->have32) throw std::string("FP32 factors not built; call pbte_factorise");
             launch_source<float>(ctx, ts, qxs, qys, ctx->src32);
+            if (ctx->n_wall) {
+                hipLaunchKernelGGL(pad_rows_kernel<float>, dim3(nblocks((int64_t)ctx->F * ctx->P, 256)), dim3(256), 0,
+                                   ctx->stream, flux_wall, ctx->fw32, ctx->F, ctx->P);
+                HIP_TRY(hipGetLastError());
+            }
             switch (ctx->P) {
 #define SW(PP) case PP: launch_sweep<PP, float>(ctx, ctx->lu32, ctx->cxv32, ctx->cyv32, ctx->geo32, ctx->K32, \
-                    ctx->bvec32, ctx->src32, flux_wall, (const float*)pbuf, (float*)vdf, use_pbuf, homogeneous); break;
+                    ctx->bvec32, ctx->src32, flux_wall, ctx->fw32, (const float*)pbuf, (float*)vdf, use_pbuf, \
+                    homogeneous); break;
             SW(3) SW(6) SW(10) SW(15)
 #undef SW
             }

@@ -63,7 +63,8 @@ class SweepContext:
     # -- construction ------------------------------------------------------
     @classmethod
     def build(cls, case, mesh, integrals, vel, sweep_order, bcdata, precompute=None,
-              max_bytes: float | None = None, max_bundle: int = 0, block: int = 0):
+              max_bytes: float | None = None, max_bundle: int = 0, block: int = 0,
+              variant: int = 0, lds_slots_cap: int = 0):
         self = cls()
         cxv, cyv, domega = vel.flat()
         self.cxv, self.cyv, self.domega = cxv, cyv, domega
@@ -102,7 +103,8 @@ class SweepContext:
         su.bc_type, su.bc_temp = i32(bcdata.bc_type), f64(bcdata.bc_temp)
         su.face_tri, su.face_lfc = i32(mesh.face_tri), i32(mesh.face_lfc)
         su.order = i32(sweep_order.order)
-        su.max_bundle, su.block = int(max_bundle), int(block)
+        su.max_bundle, su.block, su.sweep_variant = int(max_bundle), int(block), int(variant)
+        su.lds_slots_cap = int(lds_slots_cap)
         err = C.create_string_buffer(512)
         torch.cuda.synchronize()
         h = _native.lib().pbte_ctx_create(C.byref(dims), C.byref(su), C.c_void_p(stream_handle()),
@@ -126,10 +128,11 @@ class SweepContext:
             self.handle = None
 
     def schedule_info(self) -> dict:
-        out = (C.c_int64 * 4)()
+        out = (C.c_int64 * 6)()
         _native.check(self.handle, _native.lib().pbte_schedule_info(self.handle, out), "schedule")
         return {"levels": int(out[0]), "bundles": int(out[1]), "max_width": int(out[2]),
-                "lu_elements": int(out[3])}
+                "stash_reads": int(out[3]), "global_bundles": int(out[4]),
+                "ring_slots": int(out[5])}
 
     def operator_bytes(self, n_tris: int, ndof: int) -> float:
         return float(self.ndir) * n_tris * ndof * ndof * 8.0

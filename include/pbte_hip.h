@@ -64,6 +64,11 @@ typedef struct pbte_setup {
                                              0 = automatic                 */
     int32_t block;                        /* threads per sweep workgroup,
                                              0 = automatic                 */
+    int32_t sweep_variant;                /* 0 pipelined (LDS ring), 1 plain
+                                             level sweep (same results)    */
+    int32_t lds_slots_cap;                /* > 0 caps the LDS ring and stash
+                                             slots (exercises the global
+                                             fallback); 0 = automatic      */
 } pbte_setup;
 
 /* ---- setup ------------------------------------------------------------ */
@@ -88,8 +93,10 @@ void pbte_ctx_destroy(pbte_ctx *ctx);
 
 const char *pbte_last_error(pbte_ctx *ctx);
 
-/* Schedule statistics: levels per bundle summed, bundles, max level width. */
-int pbte_schedule_info(pbte_ctx *ctx, int64_t *out4);
+/* Schedule statistics: levels summed over bundles, bundles, widest level
+ * (items), upwind reads served by the LDS stash, bundles that read this
+ * sweep's values from global memory, LDS ring slots. */
+int pbte_schedule_info(pbte_ctx *ctx, int64_t *out6);
 
 /* Factorise A_SOL for every (direction, element) in the given precision.
  * Replaces SweepContext.factorise / _kernels.build_operators

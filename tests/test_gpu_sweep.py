@@ -166,3 +166,75 @@ def test_jittered_mesh_sweep(pkg, tmp_path):
     s.ctx.sweep(s.mom, s.vdf_d)
     o.sweep()
     assert _rel(s.vdf, o.vdf) <= 1e-12
+
+
+@pytest.mark.parametrize("tag,bundle,cap", [("F1_1_1", 1, 0), ("F2_1_1", 2, 0), ("F3_1_1", 4, 0),
+                                           ("F1_1_1", 1, 16), ("F3_1_1", 2, 2)])
+def test_pipelined_sweep_is_bitwise_plain_sweep(pkg, tag, bundle, cap):
+    """The pipelined kernel (LDS ring + stash, or the global fallback when a
+    capped stash overflows) computes the plain level kernel's values bit for
+    bit, in both precisions."""
+    import torch
+    for prec in ("fp64", "fp32"):
+        c = _case(pkg, tag, performance__precision=prec)
+        outs = []
+        for variant in (0, 1):
+            print(f"case {tag} {prec} variant {variant} bundle {bundle} cap {cap}", flush=True)
+            s = pkg.Solver(c, max_bundle=bundle, sweep_variant=variant, lds_slots_cap=cap)
+            if variant == 0 and cap:
+                info = s.ctx.schedule_info()
+                print(info, flush=True)
+                assert info["stash_reads"] > 0 or info["global_bundles"] > 0
+            rng = np.random.default_rng(11)
+            vdf = rng.standard_normal(s.sweep_vdf.shape)
+            s.vdf = vdf
+            s.mom.set_dofs(*(rng.standard_normal((s.ndof, s.n_tris)) for _ in range(3)))
+            s.ctx.flux_wall.copy_(torch.from_numpy(rng.standard_normal(s.ctx.flux_wall.shape)))
+            for _ in range(2):
+                s.ctx.sweep(s.mom, s.sweep_vdf)
+            outs.append(s.vdf)
+        assert np.array_equal(outs[0], outs[1])
+
+
+def test_broken_cycle_sweep_matches_oracle(pkg):
+    """Inflow lagged by a broken cycle reads the previous sweep's value (the
+    reference order emits the element before its upwind neighbour)."""
+    import torch
+    from oracle.cpu import Oracle
+
+    from pbte_amd.moments import Moments
+    from pbte_amd.ordering import build_sweep_order
+    from pbte_amd.problem import build_problem
+    from pbte_amd.sweep import SweepContext
+    c = _case(pkg, "F1_1_1", velmesh__npole=2, velmesh__nazim=4)
+    c.mesh.file = "../meshes/A1_Nx6_Ny6.msh"
+    pb = build_problem(c)
+    cx, cy, _ = pb.vel.flat()
+    m = pb.mesh
+    a = 10
+    b = int(m.tri_neighbour[a][m.tri_neighbour[a] >= 0][0])
+    la = int(np.flatnonzero(m.tri_neighbour[a] == b)[0])
+    lb = int(np.flatnonzero(m.tri_neighbour[b] == a)[0])
+    n = -np.array([cx[0], cy[0]]) / np.hypot(cx[0], cy[0])
+    m.tri_normal[a, la] = n
+    m.tri_normal[b, lb] = n
+    pb.order = build_sweep_order(m, pb.vel, on_cycle="break")
+    assert pb.order.cycles_broken.sum() >= 1
+    o = Oracle(pb)
+    o.factorise()
+    rng = np.random.default_rng(12)
+    o.vdf[:] = rng.standard_normal(o.vdf.shape)
+    o.ts[:], o.qxs[:], o.qys[:] = (rng.standard_normal(o.ts.shape) for _ in range(3))
+    for variant in (0, 1):
+        print(f"case broken cycle variant {variant}", flush=True)
+        ctx = SweepContext.build(c, m, pb.integrals, pb.vel, pb.order, pb.bcdata, variant=variant)
+        mom = Moments.zeros(o.P, o.N)
+        mom.set_dofs(o.ts, o.qxs, o.qys)
+        vdf = torch.from_numpy(o.vdf.copy()).cuda()
+        ctx.sweep(mom, vdf)
+        got = vdf.cpu().numpy()
+        ref = Oracle(pb)
+        ref.factorise()
+        ref.vdf[:], ref.ts[:], ref.qxs[:], ref.qys[:] = o.vdf, o.ts, o.qxs, o.qys
+        ref.sweep()
+        assert _rel(got, ref.vdf) <= 1e-12
