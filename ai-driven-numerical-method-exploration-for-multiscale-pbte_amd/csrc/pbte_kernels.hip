@@ -62,17 +62,10 @@ template <> struct Deg<15> { static constexpr int v = 4; };
 __host__ __device__ constexpr int pad4(int n) { return (n + 3) & ~3; }
 template <int P> struct Lay {
     static constexpr int Q = Deg<P>::v + 1;
-    static constexpr int LU = pad4(P * P);   // LU entries per (slot, direction)
     static constexpr int SRC = pad4(3 * P);  // source values per cell
     static constexpr int KQ = pad4(Q * Q);   // upwind face block per (cell, face)
+    static constexpr int LU = pad4(P * P);   // factors per (slot, direction), row-major
 };
-// LU entry k of item (slot s, direction b of a bundle of B) in the
-// [k / V][slot][dir][V] layout, V = 16 / sizeof(T)
-template <typename T>
-__host__ __device__ constexpr size_t lu_at(int k, int s, int B, int b, int N)
-{
-    return (((size_t)(k / (16 / (int)sizeof(T))) * N + s) * B + b) * (16 / sizeof(T)) + (k % (16 / (int)sizeof(T)));
-}
 
 // face codes of CellGeo::info
 enum { FACE_INTERIOR = 0, FACE_PERIODIC = 1, FACE_ADIABATIC = 2, FACE_THERMAL = 3 };
@@ -107,7 +100,8 @@ struct pbte_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     double cv = 1, vg = 1, tau_r = 1, tau_n = 1, tau_c = 1, pi = M_PI;
-    int block = 256;
+    int block = 256;            // workgroup size of the plain level kernel
+    int items = 256;            // (cell, direction) items per workgroup and level
 
     // direction tables
     double *cxv = nullptr, *cyv = nullptr, *domega = nullptr;
@@ -143,6 +137,8 @@ struct pbte_ctx {
     int variant = 0;            // 0 pipelined level sweep, 1 plain level sweep
     int max_bundle_dirs = 1;
     int64_t ring_fallback = 0;  // upwind reads beyond the default LDS ring
+    int64_t old_reads = 0;      // previous-sweep reads (broken cycles)
+    int64_t periodic_reads = 0; // upwind reads across periodic faces
     std::vector<int64_t> h_lu_off;
     int64_t lu_elems = 0, piv_elems = 0;
     // factors
@@ -293,15 +289,29 @@ __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
             A[k][k] = rcp;   // the solve multiplies by the reciprocal
         }
         if (info) atomicAdd(a.singular, 1);
-        const int64_t base = a.lu_off[bundle];
+        // the row interchanges composed into one permutation: (P b)[k] = b[perm[k]]
+        int idx[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) idx[k] = k;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const int p = (int)((pv >> (4 * k)) & 15u);
+#pragma unroll
+            for (int r = k + 1; r < P; ++r)
+                if (p == r) { int t = idx[k]; idx[k] = idx[r]; idx[r] = t; }
+        }
+        uint64_t perm = 0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) perm |= (uint64_t)idx[k] << (4 * k);
+        const int64_t base = a.lu_off[bundle] + ((int64_t)s * B + b) * Lay<P>::LU;   // row-major per item
 #pragma unroll
         for (int r = 0; r < P; ++r)
 #pragma unroll
             for (int c = 0; c < P; ++c) {
-                if (a.lu64) a.lu64[base + lu_at<double>(r * P + c, s, B, b, N)] = A[r][c];
-                if (a.lu32) a.lu32[base + lu_at<float>(r * P + c, s, B, b, N)] = (float)A[r][c];
+                if (a.lu64) a.lu64[base + r * P + c] = A[r][c];
+                if (a.lu32) a.lu32[base + r * P + c] = (float)A[r][c];
             }
-        a.piv[a.piv_off[bundle] + (size_t)s * B + b] = pv;
+        a.piv[a.piv_off[bundle] + (size_t)s * B + b] = perm;
     }
 }
 
@@ -461,29 +471,36 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
                     }
                 }
             }
-            // P b: DGETRS row interchanges
+            // P b (the composed DGETRS row interchanges)
             const uint64_t pv = piv[(size_t)s * B + b];
+            {
+                T y[P];
 #pragma unroll
-            for (int k = 0; k < P; ++k) {
-                const int p = (int)((pv >> (4 * k)) & 15u);
+                for (int k = 0; k < P; ++k) {
+                    const int m = (int)((pv >> (4 * k)) & 15u);
+                    T v = x[0];
 #pragma unroll
-                for (int r = k + 1; r < P; ++r)
-                    if (p == r) { T tmp = x[k]; x[k] = x[r]; x[r] = tmp; }
+                    for (int j = 1; j < P; ++j) v = (m == j) ? x[j] : v;
+                    y[k] = v;
+                }
+#pragma unroll
+                for (int k = 0; k < P; ++k) x[k] = y[k];
             }
+            const T* e = lu + ((size_t)s * B + b) * Lay<P>::LU;
             // unit lower solve
 #pragma unroll
             for (int k = 0; k < P; ++k) {
                 const T xk = x[k];
 #pragma unroll
-                for (int r = k + 1; r < P; ++r) x[r] -= xk * lu[lu_at<T>(r * P + k, s, B, b, N)];
+                for (int r = k + 1; r < P; ++r) x[r] -= xk * e[r * P + k];
             }
             // upper solve, accumulated downwards (DTRSM column order)
 #pragma unroll
             for (int k = P - 1; k >= 0; --k) {
-                x[k] = x[k] * lu[lu_at<T>(k * P + k, s, B, b, N)];
+                x[k] = x[k] * e[k * P + k];
                 const T xk = x[k];
 #pragma unroll
-                for (int r = 0; r < k; ++r) x[r] -= xk * lu[lu_at<T>(r * P + k, s, B, b, N)];
+                for (int r = 0; r < k; ++r) x[r] -= xk * e[r * P + k];
             }
             T* o = vdf_d + (size_t)i * P;
 #pragma unroll
@@ -523,58 +540,157 @@ __device__ __forceinline__ void load16(T* dst, const T* src)
     }
 }
 
-// One level's worth of a thread's item.  The gather stage only issues
-// loads whose addresses come from the record loaded a level earlier; every
-// use of the loaded data happens in the solve stage a level later.  (The
-// in-order vmcnt counter makes any earlier use wait for all prefetches in
-// flight.)  The three per-face sources are chosen by pointer and read with
-// one vector load into one destination, so no destination is picked at run
-// time (a pointer select between local arrays would put them in scratch).
-// Wall rows are pad4(P) long and read KQ >= pad4(P) values; their buffers
-// carry KQ values of tail padding.
+// --------------------------------------------------------------------------
+// software-pipelined level sweep.  Same per-item arithmetic as sweep_kernel
+// (the values agree bit for bit); the instruction stream per level is kept
+// short and free of divergent branches on the common path:
+//   * records {cell, face words} of level lv+2 and the cell data, face
+//     blocks, factors and permutation of level lv+1 are loaded while level
+//     lv is solved (two named register sets, no copies);
+//   * an element writes its values on each of its three faces, face-major,
+//     into an LDS ring slot (and a stash slot when a far reader needs it);
+//     an upwind face is one vector LDS read of the neighbour's trace;
+//   * every face runs the same code: a face without an upwind read takes
+//     weight 0 and reads an all-zero LDS slot, a face without wall inflow
+//     takes wall weight 0; only the rare global reads (previous-sweep value
+//     of a broken cycle, explicit periodic buffer, overflow bundles) branch.
+// --------------------------------------------------------------------------
+template <int P> struct Tr {
+    static constexpr int Q = Deg<P>::v + 1;
+    static constexpr int TQ = Q == 2 ? 2 : pad4(Q);     // padded trace of one face
+    static constexpr int W = 3 * TQ;                      // values per (slot, direction)
+};
+
+template <typename T, int n>
+__device__ __forceinline__ void lds_load(T* dst, const T* src)
+{
+    if constexpr (n * sizeof(T) == 8) {
+        using V = typename std::conditional<sizeof(T) == 4, float2, double>::type;
+        const V v = *reinterpret_cast<const V*>(src);
+        const T* e = reinterpret_cast<const T*>(&v);
+#pragma unroll
+        for (int j = 0; j < n; ++j) dst[j] = e[j];
+    } else {
+        load16<T, n>(dst, src);
+    }
+}
+
+template <typename T, int n>
+__device__ __forceinline__ void lds_store(T* dst, const T* src)
+{
+    if constexpr (n * sizeof(T) == 8) {
+        using V = typename std::conditional<sizeof(T) == 4, float2, double>::type;
+        V v;
+        T* e = reinterpret_cast<T*>(&v);
+#pragma unroll
+        for (int j = 0; j < n; ++j) e[j] = src[j];
+        *reinterpret_cast<V*>(dst) = v;
+    } else {
+        using V = typename Vec16<T>::type;
+        constexpr int w = 16 / sizeof(T);
+#pragma unroll
+        for (int k = 0; k < n / w; ++k) {
+            V v;
+            T* e = reinterpret_cast<T*>(&v);
+#pragma unroll
+            for (int j = 0; j < w; ++j) e[j] = src[k * w + j];
+            reinterpret_cast<V*>(dst)[k] = v;
+        }
+    }
+}
+
+struct PipeRec {
+    bool valid;
+    int slot, wst;
+    int4 r;
+    uint64_t perm;
+};
+
 template <int P, typename T, bool PFLU> struct PipeData {
     using L = Lay<P>;
     bool valid;
-    int slot, cell, wst;
-    int fw[3];
+    int cell, fw[3];
+    int lds[3];         // LDS element offset of the upwind slot of each face (zero slot when none)
+    int wr[2];          // ring and stash (or spill) element offsets this item writes
+    uint64_t perm;
     int info[3];
     T nrm[6];
     T src[L::SRC];
     T fd[3][L::KQ];     // upwind face block, or the wall vector of a wall face
     T lu[PFLU ? L::LU : 4];
-    uint64_t piv;
-};
-struct PipeRec {
-    bool valid;
-    int slot, wst;
-    int4 r;
+    int slot;
 };
 
 template <typename T>
-__device__ __forceinline__ void pipe_record(const int4* rec, const int32_t* wst, int2 span, int c, PipeRec& o)
+__device__ __forceinline__ void pipe_record(const int4* rec, const int32_t* wst, const uint64_t* piv, int2 span,
+                                            int c, int B, int b, PipeRec& o)
 {
-    o.slot = span.x + c;
-    o.valid = o.slot < span.y;
-    if (o.valid) {
-        o.r = rec[o.slot];
-        o.wst = wst[o.slot];
-    }
+    // an idle thread reads slot 0: the loads of every stage are then
+    // unconditional, so their results are always consumed by the next stage
+    // (a load left pending on one path would make a later overwrite of its
+    // registers wait for every prefetch in flight)
+    const int sl = span.x + c;
+    o.valid = sl < span.y;
+    o.slot = o.valid ? sl : 0;
+    o.r = rec[o.slot];
+    o.wst = wst[o.slot];
+    o.perm = piv[(size_t)o.slot * B + b];
 }
 
+// record-code classes as bit masks over the code: one shift and one test,
+// no short-circuit control flow
+__host__ __device__ constexpr int code_bit(int c) { return 1 << c; }
+constexpr int kRecRing = code_bit(REC_UP_RING) | code_bit(REC_PER_RING);
+constexpr int kRecWall = code_bit(REC_ADIABATIC) | code_bit(REC_THERMAL);
+constexpr int kRecNeighbour = code_bit(REC_UP_RING) | code_bit(REC_UP_STASH) | code_bit(REC_OLD) |
+                              code_bit(REC_PER_RING) | code_bit(REC_PER_STASH) | code_bit(REC_PER_OLD) |
+                              code_bit(REC_UP_GLOBAL) | code_bit(REC_PER_GLOBAL);
+constexpr int kRecPeriodic = code_bit(REC_PER_RING) | code_bit(REC_PER_STASH) | code_bit(REC_PER_OLD) |
+                             code_bit(REC_PER_GLOBAL);
+__device__ __forceinline__ bool code_in(int mask, int code) { return ((mask >> code) & 1) != 0; }
+// faces whose upwind trace comes from LDS (ring or stash)
+__device__ __forceinline__ int rec_lds_mask(int use_pbuf)
+{
+    const int up = code_bit(REC_UP_RING) | code_bit(REC_UP_STASH);
+    const int per = code_bit(REC_PER_RING) | code_bit(REC_PER_STASH);
+    return use_pbuf ? up : (up | per);
+}
+
+// gather: only loads whose addresses come from the record of a level
+// earlier; every use of the loaded data waits for the solve stage (the
+// in-order vmcnt counter would make an earlier use wait for every
+// prefetch in flight)
 template <int P, typename T, bool PFLU>
 __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec& r, int B, int b, const T* lu,
-                                            const uint64_t* piv, PipeData<P, T, PFLU>& o)
+                                            PipeData<P, T, PFLU>& o)
 {
     using L = Lay<P>;
+    constexpr int W = Tr<P>::W;
     o.valid = r.valid;
-    if (!r.valid) return;
     const int i = r.r.x;
     o.slot = r.slot;
     o.cell = i;
-    o.wst = r.wst;
     o.fw[0] = r.r.y;
     o.fw[1] = r.r.z;
     o.fw[2] = r.r.w;
+    o.perm = r.perm;
+    const int RM = a.ring - 1;
+    const int zero = a.ring + a.stash;
+    const int lds_mask = rec_lds_mask(a.use_pbuf);
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        // every choice below is a select between values computed beforehand
+        const int code = (int)(((unsigned)o.fw[il]) >> 28);
+        const int val = o.fw[il] & 0x0fffffff;
+        const int ring_sl = val & RM;
+        const int stash_sl = a.ring + val;
+        int sl = code_in(kRecRing, code) ? ring_sl : stash_sl;
+        sl = code_in(lds_mask, code) ? sl : zero;
+        o.lds[il] = (sl * B + b) * W;
+    }
+    o.wr[0] = ((r.slot & RM) * B + b) * W;
+    const int st_sl = a.ring + r.wst;
+    o.wr[1] = ((r.wst >= 0 ? st_sl : zero + 1) * B + b) * W;
     load16<T, L::SRC>(o.src, a.src + (size_t)i * L::SRC);
     const CellGeo<T> g = a.geo[i];
 #pragma unroll
@@ -585,39 +701,58 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
     for (int il = 0; il < 3; ++il) {
         const int code = (int)(((unsigned)o.fw[il]) >> 28);
         const int val = o.fw[il] & 0x0fffffff;
-        const T* srcp = nullptr;
-        if (rec_reads_neighbour(code))
-            srcp = a.K + ((size_t)i * 3 + il) * L::KQ;
-        else if (code == REC_ADIABATIC)
-            srcp = a.flux_wall_t + (size_t)val * pad4(P);
-        else if (code == REC_THERMAL && !a.homogeneous)
-            srcp = a.bvec + (size_t)val * pad4(P);
-        if (srcp) load16<T, L::KQ>(o.fd[il], srcp);
+        const T* pk = a.K + ((size_t)i * 3 + il) * L::KQ;
+        const T* pa = a.flux_wall_t + (size_t)val * pad4(P);
+        const T* pt = a.bvec + (size_t)val * pad4(P);
+        const T* pw = code == REC_ADIABATIC ? pa : pt;
+        const T* srcp = code_in(kRecWall, code) ? pw : pk;
+        load16<T, L::KQ>(o.fd[il], srcp);
     }
-    o.piv = piv[(size_t)r.slot * B + b];
-    if (PFLU) {
-        using V = typename Vec16<T>::type;
-        constexpr int w = 16 / sizeof(T);
-        const V* e = reinterpret_cast<const V*>(lu) + (size_t)r.slot * B + b;
-        const size_t ld = (size_t)a.N * B;
-#pragma unroll
-        for (int k = 0; k < L::LU / w; ++k) {
-            const V v = e[(size_t)k * ld];
-            const T* t = reinterpret_cast<const T*>(&v);
-#pragma unroll
-            for (int j = 0; j < w; ++j) o.lu[k * w + j] = t[j];
-        }
-    }
+    if (PFLU) load16<T, L::LU>(o.lu, lu + ((size_t)r.slot * B + b) * L::LU);
 }
 
-template <int P, typename T, bool PFLU>
+template <int P, typename T, bool PFLU, bool GREAD>
 __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData<P, T, PFLU>& dC, T cx, T cy, int d,
-                                           int B, int b, const T* lu, T* ring, int RM, T* vdf_d)
+                                           int B, int b, const T* lu, T* lds, T* vdf_d)
 {
+    using L = Lay<P>;
     constexpr int DEG = Deg<P>::v;
     constexpr int Q = DEG + 1;
-    if (!dC.valid) return;
-    const int N = a.N;
+    constexpr int TQ = Tr<P>::TQ;
+    // idle threads compute on slot 0's data and store nothing: every loaded
+    // register is consumed on every path
+    // upwind traces from LDS, the three reads in flight together
+    const int lds_mask = rec_lds_mask(a.use_pbuf);
+    int code[3], ilp[3];
+    T u[3][TQ];
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        code[il] = (int)(((unsigned)dC.fw[il]) >> 28);
+        ilp[il] = (dC.info[il] >> 2) & 3;
+        lds_load<T, TQ>(u[il], lds + dC.lds[il] + ilp[il] * TQ);
+    }
+    // upwind traces from global memory: explicit periodic buffer, a
+    // previous-sweep value (broken cycle) or a bundle running with full
+    // barriers.  Only in the kernel instance launched when such reads exist;
+    // there behind a wave-uniform branch with the wait inside it.
+    if constexpr (GREAD) {
+        const int gmask = dC.valid ? kRecNeighbour & ~lds_mask : 0;
+        const bool g0 = code_in(gmask, code[0]), g1 = code_in(gmask, code[1]), g2 = code_in(gmask, code[2]);
+        if (__builtin_amdgcn_ballot_w64(g0 || g1 || g2) != 0) {
+#pragma unroll
+            for (int il = 0; il < 3; ++il) {
+                if (code_in(gmask, code[il])) {
+                    const int val = dC.fw[il] & 0x0fffffff;
+                    const T* fe = (code_in(kRecPeriodic, code[il]) && a.use_pbuf)
+                                      ? a.pbuf + ((size_t)d * a.F + (dC.info[il] >> 4)) * P
+                                      : vdf_d + (size_t)val * P;
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) u[il][q] = fe[face_node(DEG, ilp[il], q)];
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0x0f70);       // vmcnt(0)
+        }
+    }
     T x[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) x[m] = dC.src[m] + cx * dC.src[P + m] + cy * dC.src[2 * P + m];
@@ -627,94 +762,69 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
         const T sp = cx * dC.nrm[2 * il] + cy * dC.nrm[2 * il + 1];
         w[il] = T(0.5) * (sp - fabs(sp));
     }
-    // wall inflow in face order
+    // wall inflow in face order; weight 0 on every other face
+    const T thermal_on = a.homogeneous ? T(0) : T(1);
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
-        const int code = (int)(((unsigned)dC.fw[il]) >> 28);
-        if (w[il] == T(0)) continue;
-        if (code == REC_ADIABATIC) {
+        const T wt = -w[il] * thermal_on;
+        const T wa = code[il] == REC_ADIABATIC ? w[il] : wt;
+        const T ww = code_in(kRecWall, code[il]) ? wa : T(0);
 #pragma unroll
-            for (int m = 0; m < P; ++m) x[m] += w[il] * dC.fd[il][m];
-        } else if (code == REC_THERMAL && !a.homogeneous) {
-#pragma unroll
-            for (int m = 0; m < P; ++m) x[m] -= w[il] * dC.fd[il][m];
-        }
+        for (int m = 0; m < P; ++m) x[m] += ww * dC.fd[il][m];
     }
-    // upwind neighbours in face order
+    // upwind neighbours in face order; weight 0 on every other face (whose
+    // trace is read from the all-zero slot)
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
-        const int code = (int)(((unsigned)dC.fw[il]) >> 28);
-        if (!rec_reads_neighbour(code) || w[il] == T(0)) continue;
-        const int val = dC.fw[il] & 0x0fffffff;
-        const int ilp = (dC.info[il] >> 2) & 3;
-        const bool periodic = code >= REC_PER_RING && code != REC_UP_GLOBAL;
-        // the trace is applied inside each branch: merging an LDS-loaded
-        // and a global-loaded value would make the use wait on both counters
-        if ((periodic && a.use_pbuf) || code == REC_OLD || code == REC_PER_OLD || code == REC_UP_GLOBAL ||
-            code == REC_PER_GLOBAL) {
-            // global memory: explicit periodic buffer, a previous-sweep value
-            // (broken cycle) or a bundle running with full barriers
-            const T* fe = (periodic && a.use_pbuf) ? a.pbuf + ((size_t)d * a.F + (dC.info[il] >> 4)) * P
-                                                   : vdf_d + (size_t)val * P;
-            T fb[Q];
+        const T wn = code_in(kRecNeighbour, code[il]) ? w[il] : T(0);
 #pragma unroll
-            for (int q = 0; q < Q; ++q) fb[q] = fe[face_node(DEG, ilp, q)];
+        for (int r = 0; r < Q; ++r) {
+            T acc = x[face_node(DEG, il, r)];
 #pragma unroll
-            for (int r = 0; r < Q; ++r) {
-                T acc = x[face_node(DEG, il, r)];
-#pragma unroll
-                for (int q = 0; q < Q; ++q) acc -= (w[il] * dC.fd[il][r * Q + q]) * fb[q];
-                x[face_node(DEG, il, r)] = acc;
-            }
-        } else {
-            const int slot = (code == REC_UP_RING || code == REC_PER_RING) ? (val & RM) : a.ring + val;
-            const T* fe = ring + ((size_t)slot * B + b) * P;     // the stash follows the ring
-            T fb[Q];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) fb[q] = fe[face_node(DEG, ilp, q)];
-#pragma unroll
-            for (int r = 0; r < Q; ++r) {
-                T acc = x[face_node(DEG, il, r)];
-#pragma unroll
-                for (int q = 0; q < Q; ++q) acc -= (w[il] * dC.fd[il][r * Q + q]) * fb[q];
-                x[face_node(DEG, il, r)] = acc;
-            }
+            for (int q = 0; q < Q; ++q) acc -= (wn * dC.fd[il][r * Q + q]) * u[il][q];
+            x[face_node(DEG, il, r)] = acc;
         }
     }
-    const uint64_t pv = dC.piv;
+    // P b (the composed DGETRS row interchanges)
+    {
+        T y[P];
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-        const int p = (int)((pv >> (4 * k)) & 15u);
+        for (int k = 0; k < P; ++k) {
+            const int m = (int)((dC.perm >> (4 * k)) & 15u);
+            T v = x[0];
 #pragma unroll
-        for (int r = k + 1; r < P; ++r)
-            if (p == r) { T tmp = x[k]; x[k] = x[r]; x[r] = tmp; }
+            for (int j = 1; j < P; ++j) v = (m == j) ? x[j] : v;
+            y[k] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) x[k] = y[k];
     }
-    const int s = dC.slot;
+    const T* e = PFLU ? dC.lu : lu + ((size_t)dC.slot * B + b) * L::LU;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const T xk = x[k];
 #pragma unroll
-        for (int r = k + 1; r < P; ++r)
-            x[r] -= xk * (PFLU ? dC.lu[r * P + k] : lu[lu_at<T>(r * P + k, s, B, b, N)]);
+        for (int r = k + 1; r < P; ++r) x[r] -= xk * e[r * P + k];
     }
 #pragma unroll
     for (int k = P - 1; k >= 0; --k) {
-        x[k] = x[k] * (PFLU ? dC.lu[k * P + k] : lu[lu_at<T>(k * P + k, s, B, b, N)]);
+        x[k] = x[k] * e[k * P + k];
         const T xk = x[k];
 #pragma unroll
-        for (int r = 0; r < k; ++r) x[r] -= xk * (PFLU ? dC.lu[r * P + k] : lu[lu_at<T>(r * P + k, s, B, b, N)]);
+        for (int r = 0; r < k; ++r) x[r] -= xk * e[r * P + k];
     }
-    T* o = vdf_d + (size_t)dC.cell * P;
-    T* rs = ring + ((size_t)(s & RM) * B + b) * P;
+    if (dC.valid) {
+        T* o = vdf_d + (size_t)dC.cell * P;
 #pragma unroll
-    for (int m = 0; m < P; ++m) {
-        o[m] = x[m];
-        rs[m] = x[m];
-    }
-    if (dC.wst >= 0) {
-        T* st = ring + ((size_t)(a.ring + dC.wst) * B + b) * P;
+        for (int m = 0; m < P; ++m) o[m] = x[m];
 #pragma unroll
-        for (int m = 0; m < P; ++m) st[m] = x[m];
+        for (int f = 0; f < 3; ++f) {
+            T t[TQ];
+#pragma unroll
+            for (int q = 0; q < TQ; ++q) t[q] = q < Q ? x[face_node(DEG, f, q < Q ? q : 0)] : T(0);
+            lds_store<T, TQ>(lds + dC.wr[0] + f * TQ, t);
+            lds_store<T, TQ>(lds + dC.wr[1] + f * TQ, t);
+        }
     }
 }
 
@@ -731,22 +841,15 @@ __device__ __forceinline__ void level_barrier(bool global_mode)
     }
 }
 
-// --------------------------------------------------------------------------
-// software-pipelined level sweep.  Same per-item arithmetic as sweep_kernel
-// (bitwise identical results).  While level lv is solved, the records of
-// level lv+2 and the cell data, LU and pivots of level lv+1 are in flight;
-// the loop is unrolled by two over two named register sets so that no
-// register copy forces an early wait.  Upwind values of this sweep come from
-// the LDS ring / stash; the barrier orders LDS only.
-// --------------------------------------------------------------------------
-template <int P, typename T, bool PFLU>
+template <int P, typename T, bool PFLU, bool GREAD>
 __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
 {
+    constexpr int W = Tr<P>::W;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int bundle = a.block_bundle[blockIdx.x];
     const int B = a.bundle_ndir[bundle];
-    T* ring = reinterpret_cast<T*>(lds_raw);        // ring slots, then stash slots
-    const int RM = a.ring - 1;
+    // LDS: ring slots, stash slots, one all-zero slot, one spill slot
+    T* lds = reinterpret_cast<T*>(lds_raw);
     const bool global_mode = a.bundle_global[bundle] != 0;
     const int t = threadIdx.x;
     const int b = t % B;
@@ -760,6 +863,11 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     const uint64_t* piv = a.piv + a.piv_off[bundle];
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
+    {
+        T* z = lds + (size_t)(a.ring + a.stash) * B * W;
+        for (int k = t; k < B * W; k += blockDim.x) z[k] = T(0);
+    }
+    __syncthreads();
 
     // level spans travel two levels ahead of their use, the records one
     // level ahead of the gather; every stage alternates between two named
@@ -771,20 +879,23 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     int2 S0 = span(l0 + 2), S1 = span(l0 + 3);
     PipeRec R0, R1;
     PipeData<P, T, PFLU> D0, D1;
-    pipe_record<T>(rec, wst, span(l0), c, R0);
-    pipe_gather<P, T, PFLU>(a, R0, B, b, lu, piv, D0);
-    pipe_record<T>(rec, wst, span(l0 + 1), c, R1);
+    // the prologue issues its loads in the steady-state order (records of
+    // the level after next before the gather of the next level): vmcnt is
+    // in order, and the loop-entry state must not ask for more waiting
+    pipe_record<T>(rec, wst, piv, span(l0), c, B, b, R0);
+    pipe_record<T>(rec, wst, piv, span(l0 + 1), c, B, b, R1);
+    pipe_gather<P, T, PFLU>(a, R0, B, b, lu, D0);
     for (int lv = l0; lv < l1; lv += 2) {
-        pipe_record<T>(rec, wst, S0, c, R0);                             // level lv+2
+        pipe_record<T>(rec, wst, piv, S0, c, B, b, R0);                // level lv+2
         S0 = span(lv + 4);
-        pipe_gather<P, T, PFLU>(a, R1, B, b, lu, piv, D1);               // level lv+1
-        pipe_solve<P, T, PFLU>(a, D0, cx, cy, d, B, b, lu, ring, RM, vdf_d);   // level lv
+        pipe_gather<P, T, PFLU>(a, R1, B, b, lu, D1);                  // level lv+1
+        pipe_solve<P, T, PFLU, GREAD>(a, D0, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv
         level_barrier(global_mode);
         if (lv + 1 >= l1) break;
-        pipe_record<T>(rec, wst, S1, c, R1);                             // level lv+3
+        pipe_record<T>(rec, wst, piv, S1, c, B, b, R1);                // level lv+3
         S1 = span(lv + 5);
-        pipe_gather<P, T, PFLU>(a, R0, B, b, lu, piv, D0);               // level lv+2
-        pipe_solve<P, T, PFLU>(a, D1, cx, cy, d, B, b, lu, ring, RM, vdf_d);   // level lv+1
+        pipe_gather<P, T, PFLU>(a, R0, B, b, lu, D0);                  // level lv+2
+        pipe_solve<P, T, PFLU, GREAD>(a, D1, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+1
         level_barrier(global_mode);
     }
 }
@@ -1362,6 +1473,9 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->tau_c = su->tau_c; ctx->pi = su->pi;
         ctx->block = su->block > 0 ? su->block : 256;
         if (ctx->block % 64 || ctx->block > 256) throw std::string("block must be a multiple of 64, <= 256");
+        ctx->variant = su->sweep_variant;
+        if (ctx->variant != 0 && ctx->variant != 1) throw std::string("sweep_variant must be 0 or 1");
+        ctx->items = ctx->block;
         const int deg = Q - 1;
 
         // --- direction tables -------------------------------------------------
@@ -1506,7 +1620,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         }
         int bmax = su->max_bundle > 0 ? su->max_bundle : 1;
         bmax = std::min(bmax, 8);
-        while (ctx->block % bmax) --bmax;
+        while (ctx->items % bmax) --bmax;
         std::vector<std::vector<int>> groups;
         {
             std::vector<char> used(D, 0);
@@ -1545,16 +1659,16 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         std::vector<int4> h_rec((size_t)nb * N);
         std::vector<int32_t> h_wst((size_t)nb * N, -1);
         std::vector<uint8_t> global_flag(nb, 0);
-        std::vector<int64_t> far_reads(nb, 0);
+        std::vector<int64_t> far_reads(nb, 0), old_reads(nb, 0), per_reads(nb, 0);
         {
             // LDS budget per workgroup: 32 KiB ring + 32 KiB stash at binary64
             int bm = 1;
             for (auto& bb : bundles) bm = std::max(bm, (int)bb.size());
-            const int slot_bytes = bm * P * 8;
+            const int slot_bytes = bm * 3 * (Q == 2 ? 2 : pad4(Q)) * 8;
             int r = 1;
             while ((r * 2) * slot_bytes <= 32768) r *= 2;
             ctx->ring_slots = r;
-            ctx->stash_slots = std::max(1, 32768 / slot_bytes);
+            ctx->stash_slots = std::max(1, 32768 / slot_bytes - 2);
             if (su->lds_slots_cap > 0) {
                 while (ctx->ring_slots > su->lds_slots_cap) ctx->ring_slots /= 2;
                 ctx->stash_slots = std::min(ctx->stash_slots, su->lds_slots_cap);
@@ -1582,7 +1696,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                     sl[start[level_of[i]]++] = i;
                 }
                 const int B = (int)bundles[k].size();
-                const int width = ctx->block / B;
+                const int width = ctx->items / B;
                 auto& lp = lptr[k];
                 lp.clear();
                 for (int l = 0; l < nlev; ++l)
@@ -1678,6 +1792,10 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                             r.val = sid[r.val];
                         }
                         w[il] = (int)(((unsigned)r.code << 28) | (unsigned)r.val);
+                        if (r.code == REC_OLD || r.code == REC_PER_OLD) ++old_reads[k];
+                        if (r.code == REC_PER_RING || r.code == REC_PER_STASH || r.code == REC_PER_OLD ||
+                            r.code == REC_PER_GLOBAL)
+                            ++per_reads[k];
                     }
                     h_rec[(size_t)k * N + q] = make_int4(sl[q], w[0], w[1], w[2]);
                     h_wst[(size_t)k * N + q] = overflow ? -1 : sid[q];
@@ -1751,8 +1869,11 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->wstash = ctx->upload(h_wst.data(), h_wst.size());
         ctx->bundle_global = ctx->upload(global_flag.data(), global_flag.size());
         for (int k = 0; k < nb; ++k) ctx->global_bundles += global_flag[k];
-        for (int k = 0; k < nb; ++k) ctx->ring_fallback += far_reads[k];
-        ctx->variant = su->sweep_variant;
+        for (int k = 0; k < nb; ++k) {
+            ctx->ring_fallback += far_reads[k];
+            ctx->old_reads += old_reads[k];
+            ctx->periodic_reads += per_reads[k];
+        }
         if (N >= (1 << 28) || F >= (1 << 28)) throw std::string("mesh too large for 28-bit record fields");
         ctx->piv_off = ctx->upload(h_piv_off.data(), nb + 1);
         ctx->h_lu_off = h_lu_off;
@@ -1834,9 +1955,13 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
         a.stash = ctx->stash_slots;
         a.wstash = ctx->wstash;
         a.bundle_global = ctx->bundle_global;
-        const size_t lds = (size_t)(ctx->ring_slots + ctx->stash_slots) * ctx->max_bundle_dirs * P * sizeof(T);
-        constexpr bool pf = P * P * sizeof(T) <= 400;
-        hipLaunchKernelGGL((sweep_pipe_kernel<P, T, pf>), dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
+        const size_t lds = (size_t)(ctx->ring_slots + ctx->stash_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W * sizeof(T);
+        constexpr bool pf = P * P * sizeof(T) <= 160;
+        const bool gread = ctx->global_bundles > 0 || ctx->old_reads > 0 || (use_pbuf && ctx->periodic_reads > 0);
+        if (gread)
+            hipLaunchKernelGGL((sweep_pipe_kernel<P, T, pf, true>), dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
+        else
+            hipLaunchKernelGGL((sweep_pipe_kernel<P, T, pf, false>), dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
     }
     HIP_TRY(hipGetLastError());
 }
