@@ -64,8 +64,22 @@ template <int P> struct Lay {
     static constexpr int Q = Deg<P>::v + 1;
     static constexpr int SRC = pad4(3 * P);  // source values per cell
     static constexpr int KQ = pad4(Q * Q);   // upwind face block per (cell, face)
-    static constexpr int LU = pad4(P * P);   // factors per (slot, direction), row-major
+    static constexpr int LU = pad4(P * P);   // factors per (slot, direction)
+    // cell record of the pipelined sweep, one contiguous row per cell:
+    // source (U, VX, VY), normals, face words, then per face the Q x Q upwind
+    // block (neighbour faces) or the wall vector (wall faces, first P values)
+    static constexpr int R_N = 3 * P, R_INFO = 3 * P + 6, R_FD = 3 * P + 9;
+    static constexpr int CR = pad4(3 * P + 9 + 3 * Q * Q);
 };
+
+// LU entry k of item `item` (slot * B + b) of a bundle with NB items, laid
+// out [k / V][item][V] with V = 16 / sizeof(T): lanes holding consecutive
+// items read consecutive 16-byte words
+template <typename T>
+__host__ __device__ constexpr size_t lu_idx(int k, size_t item, size_t NB)
+{
+    return ((size_t)(k / (16 / (int)sizeof(T))) * NB + item) * (16 / sizeof(T)) + (size_t)(k % (16 / (int)sizeof(T)));
+}
 
 // face codes of CellGeo::info
 enum { FACE_INTERIOR = 0, FACE_PERIODIC = 1, FACE_ADIABATIC = 2, FACE_THERMAL = 3 };
@@ -129,8 +143,9 @@ struct pbte_ctx {
     int32_t *block_bundle = nullptr, *bundle_ndir = nullptr, *bundle_dir_off = nullptr,
             *bundle_dirs = nullptr, *slot_cell = nullptr, *lev_off = nullptr, *lev_ptr = nullptr;
     int64_t *lu_off = nullptr, *piv_off = nullptr;
-    int4* rec = nullptr;        // per-bundle per-slot records
-    int32_t* wstash = nullptr;  // per-bundle per-slot stash writes
+    int4* irec = nullptr;       // per item (bundle, slot, direction): {cell, face words}, {stash, perm}
+    double* crec64 = nullptr;   // cell records of the pipelined sweep
+    float* crec32 = nullptr;
     uint8_t* bundle_global = nullptr;
     int ring_slots = 0, stash_slots = 0;
     int64_t global_bundles = 0;
@@ -149,8 +164,6 @@ struct pbte_ctx {
     // scratch
     double* src64 = nullptr;
     float* src32 = nullptr;
-    float* fw32 = nullptr;      // adiabatic emission, rows padded to 16 bytes, binary32
-    double* fw64 = nullptr;     // the same in binary64
     double* ssraw = nullptr;
     double* red = nullptr;     // reduction partials
     size_t red_len = 0;
@@ -217,6 +230,7 @@ struct FactorArgs {
     double* lu64;
     float* lu32;
     uint64_t* piv;
+    int4* irec;
     int32_t* singular;
 };
 
@@ -303,15 +317,21 @@ __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
         uint64_t perm = 0;
 #pragma unroll
         for (int k = 0; k < P; ++k) perm |= (uint64_t)idx[k] << (4 * k);
-        const int64_t base = a.lu_off[bundle] + ((int64_t)s * B + b) * Lay<P>::LU;   // row-major per item
+        const int64_t base = a.lu_off[bundle];
+        const size_t item = (size_t)s * B + b, NB = (size_t)N * B;
 #pragma unroll
         for (int r = 0; r < P; ++r)
 #pragma unroll
             for (int c = 0; c < P; ++c) {
-                if (a.lu64) a.lu64[base + r * P + c] = A[r][c];
-                if (a.lu32) a.lu32[base + r * P + c] = (float)A[r][c];
+                if (a.lu64) a.lu64[base + lu_idx<double>(r * P + c, item, NB)] = A[r][c];
+                if (a.lu32) a.lu32[base + lu_idx<float>(r * P + c, item, NB)] = (float)A[r][c];
             }
-        a.piv[a.piv_off[bundle] + (size_t)s * B + b] = perm;
+        a.piv[a.piv_off[bundle] + item] = perm;
+        if (a.irec) {
+            int4* ir = a.irec + 2 * (a.piv_off[bundle] + item) + 1;
+            ir->y = (int)(uint32_t)(perm & 0xffffffffu);
+            ir->z = (int)(uint32_t)(perm >> 32);
+        }
     }
 }
 
@@ -324,7 +344,7 @@ __global__ void __launch_bounds__(256) source_kernel(const double* __restrict__ 
                                                      const double* __restrict__ ts,
                                                      const double* __restrict__ qxs,
                                                      const double* __restrict__ qys, T* __restrict__ src,
-                                                     int N, double cv, double pi, double tau_r,
+                                                     int stride, int N, double cv, double pi, double tau_r,
                                                      double tau_n, double vg)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -342,7 +362,7 @@ __global__ void __launch_bounds__(256) source_kernel(const double* __restrict__ 
         vx += mlm * bx;
         vy += mlm * by;
     }
-    T* o = src + (size_t)i * Lay<P>::SRC;
+    T* o = src + (size_t)i * stride;
     o[m] = (T)u;
     o[P + m] = (T)vx;
     o[2 * P + m] = (T)vy;
@@ -369,16 +389,14 @@ template <typename T> struct SweepArgs {
     const T* pbuf;
     T* vdf;
     int N, F, use_pbuf, homogeneous;
-    // pipelined variant: per-slot records {cell, face word x3}, LDS ring size
-    const int4* rec;
-    const T* flux_wall_t;       // adiabatic emission in the sweep precision
+    // pipelined variant: per-item records, cell records, LDS ring and stash
+    const int4* irec;
+    const T* crec;
     int lane_zero;              // 0 at run time; makes a uniform load per-lane
-    const int32_t* wstash;      // per slot: stash slot this item also writes, or -1
     const uint8_t* bundle_global;
-    int ring, stash;            // LDS ring and stash slots (each slot B * P values)
+    int ring, stash;            // LDS ring and stash slots
 };
 
-// face word of a per-slot record: code in bits 28-30, value in bits 0-27
 // Face word of a per-slot record: code in bits 28-31, value in bits 0-27.
 // What an inflow face reads during the pipelined sweep:
 //   UP_RING / UP_STASH    upwind value of this sweep from the LDS ring (by
@@ -486,21 +504,22 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
 #pragma unroll
                 for (int k = 0; k < P; ++k) x[k] = y[k];
             }
-            const T* e = lu + ((size_t)s * B + b) * Lay<P>::LU;
+            const size_t item = (size_t)s * B + b, NB = (size_t)N * B;
+            auto e = [&](int k) { return lu[lu_idx<T>(k, item, NB)]; };
             // unit lower solve
 #pragma unroll
             for (int k = 0; k < P; ++k) {
                 const T xk = x[k];
 #pragma unroll
-                for (int r = k + 1; r < P; ++r) x[r] -= xk * e[r * P + k];
+                for (int r = k + 1; r < P; ++r) x[r] -= xk * e(r * P + k);
             }
             // upper solve, accumulated downwards (DTRSM column order)
 #pragma unroll
             for (int k = P - 1; k >= 0; --k) {
-                x[k] = x[k] * e[k * P + k];
+                x[k] = x[k] * e(k * P + k);
                 const T xk = x[k];
 #pragma unroll
-                for (int r = 0; r < k; ++r) x[r] -= xk * e[r * P + k];
+                for (int r = 0; r < k; ++r) x[r] -= xk * e(r * P + k);
             }
             T* o = vdf_d + (size_t)i * P;
 #pragma unroll
@@ -601,40 +620,32 @@ __device__ __forceinline__ void lds_store(T* dst, const T* src)
 
 struct PipeRec {
     bool valid;
-    int slot, wst;
-    int4 r;
-    uint64_t perm;
+    int slot;
+    int4 r;                 // {cell, face words}
 };
 
 template <int P, typename T, bool PFLU> struct PipeData {
     using L = Lay<P>;
     bool valid;
-    int cell, fw[3];
-    int lds[3];         // LDS element offset of the upwind slot of each face (zero slot when none)
-    int wr[2];          // ring and stash (or spill) element offsets this item writes
-    uint64_t perm;
-    int info[3];
-    T nrm[6];
-    T src[L::SRC];
-    T fd[3][L::KQ];     // upwind face block, or the wall vector of a wall face
+    int cell, slot, fw[3];
+    int lds[3];         // LDS element offset of the upwind trace slot of each face (zero slot when none)
+    int4 s;             // {stash slot, permutation}
+    T cr[L::CR];        // the cell record
     T lu[PFLU ? L::LU : 4];
-    int slot;
 };
 
-template <typename T>
-__device__ __forceinline__ void pipe_record(const int4* rec, const int32_t* wst, const uint64_t* piv, int2 span,
-                                            int c, int B, int b, PipeRec& o)
+// records of one level: one coalesced 16-byte word per item.  Idle
+// threads read the level's first slot (slot 0 for an empty span): every
+// load of every stage is unconditional and always consumed by the next
+// stage, so the waits the compiler places are exact (a load issued on one
+// path only, or left pending on one path, makes a later use or overwrite of
+// its registers wait for every prefetch in flight)
+__device__ __forceinline__ void pipe_record(const int4* irec, int2 span, int c, int B, int b, PipeRec& o)
 {
-    // an idle thread reads slot 0: the loads of every stage are then
-    // unconditional, so their results are always consumed by the next stage
-    // (a load left pending on one path would make a later overwrite of its
-    // registers wait for every prefetch in flight)
     const int sl = span.x + c;
     o.valid = sl < span.y;
-    o.slot = o.valid ? sl : 0;
-    o.r = rec[o.slot];
-    o.wst = wst[o.slot];
-    o.perm = piv[(size_t)o.slot * B + b];
+    o.slot = o.valid ? sl : span.x;
+    o.r = irec[2 * ((size_t)o.slot * B + b)];
 }
 
 // record-code classes as bit masks over the code: one shift and one test,
@@ -656,24 +667,26 @@ __device__ __forceinline__ int rec_lds_mask(int use_pbuf)
     return use_pbuf ? up : (up | per);
 }
 
+__device__ __forceinline__ int bits_of(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int bits_of(double v) { return (int)(uint32_t)(uint64_t)__double_as_longlong(v); }
+
 // gather: only loads whose addresses come from the record of a level
 // earlier; every use of the loaded data waits for the solve stage (the
 // in-order vmcnt counter would make an earlier use wait for every
 // prefetch in flight)
 template <int P, typename T, bool PFLU>
-__device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec& r, int B, int b, const T* lu,
-                                            PipeData<P, T, PFLU>& o)
+__device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int4* irec, const PipeRec& r, int B, int b,
+                                            const T* lu, PipeData<P, T, PFLU>& o)
 {
     using L = Lay<P>;
     constexpr int W = Tr<P>::W;
     o.valid = r.valid;
     const int i = r.r.x;
-    o.slot = r.slot;
     o.cell = i;
+    o.slot = r.slot;
     o.fw[0] = r.r.y;
     o.fw[1] = r.r.z;
     o.fw[2] = r.r.w;
-    o.perm = r.perm;
     const int RM = a.ring - 1;
     const int zero = a.ring + a.stash;
     const int lds_mask = rec_lds_mask(a.use_pbuf);
@@ -688,27 +701,23 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
         sl = code_in(lds_mask, code) ? sl : zero;
         o.lds[il] = (sl * B + b) * W;
     }
-    o.wr[0] = ((r.slot & RM) * B + b) * W;
-    const int st_sl = a.ring + r.wst;
-    o.wr[1] = ((r.wst >= 0 ? st_sl : zero + 1) * B + b) * W;
-    load16<T, L::SRC>(o.src, a.src + (size_t)i * L::SRC);
-    const CellGeo<T> g = a.geo[i];
+    // the second record word (stash slot, permutation) is only used by the
+    // solve: it travels with the gathered data
+    o.s = irec[2 * ((size_t)r.slot * B + b) + 1];
+    load16<T, L::CR>(o.cr, a.crec + (size_t)i * L::CR);
+    if constexpr (PFLU) {
+        using V = typename Vec16<T>::type;
+        constexpr int w = 16 / sizeof(T);
+        const size_t NB = (size_t)a.N * B;
+        const V* e = reinterpret_cast<const V*>(lu) + (size_t)r.slot * B + b;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) o.nrm[k] = g.n[k];
+        for (int g = 0; g < L::LU / w; ++g) {
+            const V v = e[(size_t)g * NB];
+            const T* t = reinterpret_cast<const T*>(&v);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) o.info[k] = g.info[k];
-#pragma unroll
-    for (int il = 0; il < 3; ++il) {
-        const int code = (int)(((unsigned)o.fw[il]) >> 28);
-        const int val = o.fw[il] & 0x0fffffff;
-        const T* pk = a.K + ((size_t)i * 3 + il) * L::KQ;
-        const T* pa = a.flux_wall_t + (size_t)val * pad4(P);
-        const T* pt = a.bvec + (size_t)val * pad4(P);
-        const T* pw = code == REC_ADIABATIC ? pa : pt;
-        const T* srcp = code_in(kRecWall, code) ? pw : pk;
-        load16<T, L::KQ>(o.fd[il], srcp);
+            for (int j = 0; j < w; ++j) o.lu[g * w + j] = t[j];
+        }
     }
-    if (PFLU) load16<T, L::LU>(o.lu, lu + ((size_t)r.slot * B + b) * L::LU);
 }
 
 template <int P, typename T, bool PFLU, bool GREAD>
@@ -718,17 +727,20 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
     using L = Lay<P>;
     constexpr int DEG = Deg<P>::v;
     constexpr int Q = DEG + 1;
+    constexpr int QQ = Q * Q;
     constexpr int TQ = Tr<P>::TQ;
-    // idle threads compute on slot 0's data and store nothing: every loaded
-    // register is consumed on every path
+    const T* cr = dC.cr;
+    // idle threads of a busy wave compute on the first slot's data and
+    // store nothing
     // upwind traces from LDS, the three reads in flight together
     const int lds_mask = rec_lds_mask(a.use_pbuf);
-    int code[3], ilp[3];
+    int code[3], ilp[3], info[3];
     T u[3][TQ];
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
         code[il] = (int)(((unsigned)dC.fw[il]) >> 28);
-        ilp[il] = (dC.info[il] >> 2) & 3;
+        info[il] = bits_of(cr[L::R_INFO + il]);
+        ilp[il] = (info[il] >> 2) & 3;
         lds_load<T, TQ>(u[il], lds + dC.lds[il] + ilp[il] * TQ);
     }
     // upwind traces from global memory: explicit periodic buffer, a
@@ -744,7 +756,7 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
                 if (code_in(gmask, code[il])) {
                     const int val = dC.fw[il] & 0x0fffffff;
                     const T* fe = (code_in(kRecPeriodic, code[il]) && a.use_pbuf)
-                                      ? a.pbuf + ((size_t)d * a.F + (dC.info[il] >> 4)) * P
+                                      ? a.pbuf + ((size_t)d * a.F + (info[il] >> 4)) * P
                                       : vdf_d + (size_t)val * P;
 #pragma unroll
                     for (int q = 0; q < Q; ++q) u[il][q] = fe[face_node(DEG, ilp[il], q)];
@@ -755,11 +767,11 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
     }
     T x[P];
 #pragma unroll
-    for (int m = 0; m < P; ++m) x[m] = dC.src[m] + cx * dC.src[P + m] + cy * dC.src[2 * P + m];
+    for (int m = 0; m < P; ++m) x[m] = cr[m] + cx * cr[P + m] + cy * cr[2 * P + m];
     T w[3];
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
-        const T sp = cx * dC.nrm[2 * il] + cy * dC.nrm[2 * il + 1];
+        const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
         w[il] = T(0.5) * (sp - fabs(sp));
     }
     // wall inflow in face order; weight 0 on every other face
@@ -770,7 +782,7 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
         const T wa = code[il] == REC_ADIABATIC ? w[il] : wt;
         const T ww = code_in(kRecWall, code[il]) ? wa : T(0);
 #pragma unroll
-        for (int m = 0; m < P; ++m) x[m] += ww * dC.fd[il][m];
+        for (int m = 0; m < P; ++m) x[m] += ww * cr[L::R_FD + il * QQ + m];
     }
     // upwind neighbours in face order; weight 0 on every other face (whose
     // trace is read from the all-zero slot)
@@ -781,7 +793,7 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
         for (int r = 0; r < Q; ++r) {
             T acc = x[face_node(DEG, il, r)];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) acc -= (wn * dC.fd[il][r * Q + q]) * u[il][q];
+            for (int q = 0; q < Q; ++q) acc -= (wn * cr[L::R_FD + il * QQ + r * Q + q]) * u[il][q];
             x[face_node(DEG, il, r)] = acc;
         }
     }
@@ -790,7 +802,7 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
         T y[P];
 #pragma unroll
         for (int k = 0; k < P; ++k) {
-            const int m = (int)((dC.perm >> (4 * k)) & 15u);
+            const int m = (int)((((unsigned)(k < 8 ? dC.s.y : dC.s.z)) >> (4 * (k & 7))) & 15u);
             T v = x[0];
 #pragma unroll
             for (int j = 1; j < P; ++j) v = (m == j) ? x[j] : v;
@@ -799,21 +811,27 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
 #pragma unroll
         for (int k = 0; k < P; ++k) x[k] = y[k];
     }
-    const T* e = PFLU ? dC.lu : lu + ((size_t)dC.slot * B + b) * L::LU;
+    const size_t item = (size_t)dC.slot * B + b, NB = (size_t)a.N * B;
+    auto e = [&](int k) { return PFLU ? dC.lu[k] : lu[lu_idx<T>(k, item, NB)]; };
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const T xk = x[k];
 #pragma unroll
-        for (int r = k + 1; r < P; ++r) x[r] -= xk * e[r * P + k];
+        for (int r = k + 1; r < P; ++r) x[r] -= xk * e(r * P + k);
     }
 #pragma unroll
     for (int k = P - 1; k >= 0; --k) {
-        x[k] = x[k] * e[k * P + k];
+        x[k] = x[k] * e(k * P + k);
         const T xk = x[k];
 #pragma unroll
-        for (int r = 0; r < k; ++r) x[r] -= xk * e[r * P + k];
+        for (int r = 0; r < k; ++r) x[r] -= xk * e(r * P + k);
     }
     if (dC.valid) {
+        constexpr int W = Tr<P>::W;
+        const int RM = a.ring - 1;
+        const int wr0 = ((dC.slot & RM) * B + b) * W;
+        const int st_sl = dC.s.x >= 0 ? a.ring + dC.s.x : a.ring + a.stash + 1;
+        const int wr1 = (st_sl * B + b) * W;
         T* o = vdf_d + (size_t)dC.cell * P;
 #pragma unroll
         for (int m = 0; m < P; ++m) o[m] = x[m];
@@ -822,8 +840,8 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
             T t[TQ];
 #pragma unroll
             for (int q = 0; q < TQ; ++q) t[q] = q < Q ? x[face_node(DEG, f, q < Q ? q : 0)] : T(0);
-            lds_store<T, TQ>(lds + dC.wr[0] + f * TQ, t);
-            lds_store<T, TQ>(lds + dC.wr[1] + f * TQ, t);
+            lds_store<T, TQ>(lds + wr0 + f * TQ, t);
+            lds_store<T, TQ>(lds + wr1 + f * TQ, t);
         }
     }
 }
@@ -857,10 +875,8 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
     const T cx = a.cxv[d], cy = a.cyv[d];
     const int N = a.N;
-    const int4* rec = a.rec + (size_t)bundle * N;
-    const int32_t* wst = a.wstash + (size_t)bundle * N;
+    const int4* irec = a.irec + 2 * a.piv_off[bundle];
     const T* lu = a.lu + a.lu_off[bundle];
-    const uint64_t* piv = a.piv + a.piv_off[bundle];
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
     {
@@ -882,22 +898,38 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     // the prologue issues its loads in the steady-state order (records of
     // the level after next before the gather of the next level): vmcnt is
     // in order, and the loop-entry state must not ask for more waiting
-    pipe_record<T>(rec, wst, piv, span(l0), c, B, b, R0);
-    pipe_record<T>(rec, wst, piv, span(l0 + 1), c, B, b, R1);
-    pipe_gather<P, T, PFLU>(a, R0, B, b, lu, D0);
+    pipe_record(irec, span(l0), c, B, b, R0);
+    pipe_record(irec, span(l0 + 1), c, B, b, R1);
+    pipe_gather<P, T, PFLU>(a, irec, R0, B, b, lu, D0);
     for (int lv = l0; lv < l1; lv += 2) {
-        pipe_record<T>(rec, wst, piv, S0, c, B, b, R0);                // level lv+2
+        pipe_record(irec, S0, c, B, b, R0);                            // level lv+2
         S0 = span(lv + 4);
-        pipe_gather<P, T, PFLU>(a, R1, B, b, lu, D1);                  // level lv+1
+        pipe_gather<P, T, PFLU>(a, irec, R1, B, b, lu, D1);                  // level lv+1
         pipe_solve<P, T, PFLU, GREAD>(a, D0, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv
         level_barrier(global_mode);
         if (lv + 1 >= l1) break;
-        pipe_record<T>(rec, wst, piv, S1, c, B, b, R1);                // level lv+3
+        pipe_record(irec, S1, c, B, b, R1);                            // level lv+3
         S1 = span(lv + 5);
-        pipe_gather<P, T, PFLU>(a, R0, B, b, lu, D0);                  // level lv+2
+        pipe_gather<P, T, PFLU>(a, irec, R0, B, b, lu, D0);                  // level lv+2
         pipe_solve<P, T, PFLU, GREAD>(a, D1, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+1
         level_barrier(global_mode);
     }
+}
+
+// adiabatic emission of this sweep into the wall faces' slots of the cell
+// records (the wall vector occupies the first P values of the face block)
+template <int P, typename T>
+__global__ void __launch_bounds__(256) wall_rec_kernel(const double* __restrict__ flux_wall,
+                                                       const int32_t* __restrict__ wface,
+                                                       const int32_t* __restrict__ wtri,
+                                                       const int32_t* __restrict__ wlfc, int n, T* crec)
+{
+    using L = Lay<P>;
+    constexpr int Q = Deg<P>::v + 1;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * P) return;
+    const int k = g / P, m = g % P;
+    crec[(size_t)wtri[k] * L::CR + L::R_FD + wlfc[k] * Q * Q + m] = (T)flux_wall[(size_t)wface[k] * P + m];
 }
 
 // ==========================================================================
@@ -1581,6 +1613,36 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->bvec64 = ctx->upload(bvec.data(), bvec.size());
         std::vector<float> bv32(bvec.begin(), bvec.end());
         ctx->bvec32 = ctx->upload(bv32.data(), bv32.size());
+        {
+            // cell records of the pipelined sweep (the source part is written
+            // by every sweep, the adiabatic wall vectors likewise)
+            const int CR = pad4(3 * P + 9 + 3 * Q * Q), RN = 3 * P, RI = 3 * P + 6, RF = 3 * P + 9;
+            std::vector<double> c64((size_t)N * CR, 0.0);
+            std::vector<float> c32((size_t)N * CR, 0.0f);
+            for (int i = 0; i < N; ++i) {
+                double* r = c64.data() + (size_t)i * CR;
+                float* r32 = c32.data() + (size_t)i * CR;
+                for (int k = 0; k < 6; ++k) r[RN + k] = g64[i].n[k];
+                for (int il = 0; il < 3; ++il) {
+                    const uint32_t bits = (uint32_t)g64[i].info[il];
+                    const uint64_t b64 = bits;
+                    std::memcpy(r + RI + il, &b64, sizeof(double));
+                    std::memcpy(r32 + RI + il, &bits, sizeof(float));
+                    const int code = g64[i].info[il] & 3;
+                    double* fd = r + RF + il * Q * Q;
+                    if (code <= FACE_PERIODIC) {
+                        for (int k = 0; k < Q * Q; ++k) fd[k] = K[((size_t)i * 3 + il) * KQ + k];
+                    } else if (code == FACE_THERMAL) {
+                        const int fc = g64[i].other[il];
+                        for (int m = 0; m < P; ++m) fd[m] = bvec[(size_t)fc * pad4(P) + m];
+                    }
+                }
+                for (int k = 0; k < CR; ++k)
+                    if (k < RI || k >= RI + 3) r32[k] = (float)r[k];
+            }
+            ctx->crec64 = ctx->upload(c64.data(), c64.size());
+            ctx->crec32 = ctx->upload(c32.data(), c32.size());
+        }
 
         // --- adiabatic walls --------------------------------------------------
         std::vector<int32_t> wf, wt, wl;
@@ -1674,27 +1736,59 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 ctx->stash_slots = std::min(ctx->stash_slots, su->lds_slots_cap);
             }
         }
+        // levels of every bundle: slots in level order (the reference order
+        // within a level) and the level boundaries
+        std::vector<std::vector<int32_t>> lcnt(nb);
+        auto run_threads = [&](auto&& fn) {
+            unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < hw; ++t) th.emplace_back(fn);
+            for (auto& x : th) x.join();
+        };
+        {
+            std::atomic_int next{0};
+            run_threads([&]() {
+                std::vector<int32_t> level_of;
+                for (int k = next++; k < nb; k = next++) {
+                    const int d = bundles[k][0];
+                    const int32_t* order = su->order + (size_t)d * N;
+                    int nlev = pbte_detail_levels(N, order, su->neighbour, su->tri_faces, su->face_bc,
+                                                  su->bc_type, su->periodic_tri, su->nx, su->ny, su->cxv[d],
+                                                  su->cyv[d], level_of);
+                    std::vector<int32_t> cnt(nlev + 1, 0);
+                    for (int i = 0; i < N; ++i) ++cnt[level_of[i] + 1];
+                    for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
+                    std::vector<int32_t> start(cnt.begin(), cnt.end() - 1);
+                    auto& sl = slots[k];
+                    sl.assign(N, -1);
+                    for (int p = 0; p < N; ++p) {
+                        const int i = order[p];
+                        sl[start[level_of[i]]++] = i;
+                    }
+                    lcnt[k] = std::move(cnt);
+                }
+            });
+        }
+        if (ctx->variant == 0 && su->block <= 0) {
+            // the pipelined kernel runs one wave per 64 items of the widest
+            // level (every wave issues the full load stream of a level)
+            int64_t widest = 1;
+            for (int k = 0; k < nb; ++k)
+                for (size_t l = 0; l + 1 < lcnt[k].size(); ++l)
+                    widest = std::max<int64_t>(widest, (int64_t)(lcnt[k][l + 1] - lcnt[k][l]) * bundles[k].size());
+            ctx->block = ctx->items = (int)std::min<int64_t>(256, (widest + 63) / 64 * 64);
+            for (auto& bb : bundles)
+                if (ctx->items % (int)bb.size()) ctx->block = ctx->items = 256;
+        }
         std::atomic_int next{0};
         std::vector<std::string> errs(nb);
         auto work = [&]() {
-            std::vector<int32_t> level_of;
             for (int k = next++; k < nb; k = next++) {
                 const int d = bundles[k][0];
                 const int32_t* order = su->order + (size_t)d * N;
-                int nlev = pbte_detail_levels(N, order, su->neighbour, su->tri_faces, su->face_bc,
-                                              su->bc_type, su->periodic_tri, su->nx, su->ny, su->cxv[d],
-                                              su->cyv[d], level_of);
-                std::vector<int32_t> cnt(nlev + 1, 0);
-                for (int i = 0; i < N; ++i) ++cnt[level_of[i] + 1];
-                for (int l = 0; l < nlev; ++l) cnt[l + 1] += cnt[l];
-                std::vector<int32_t> start(cnt.begin(), cnt.end() - 1);
+                const auto& cnt = lcnt[k];
+                const int nlev = (int)cnt.size() - 1;
                 auto& sl = slots[k];
-                sl.assign(N, -1);
-                // within a level keep the reference order
-                for (int p = 0; p < N; ++p) {
-                    const int i = order[p];
-                    sl[start[level_of[i]]++] = i;
-                }
                 const int B = (int)bundles[k].size();
                 const int width = ctx->items / B;
                 auto& lp = lptr[k];
@@ -1802,12 +1896,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 }
             }
         };
-        {
-            unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-            std::vector<std::thread> th;
-            for (unsigned t = 0; t < hw; ++t) th.emplace_back(work);
-            for (auto& x : th) x.join();
-        }
+        run_threads(work);
         int64_t maxw = 0;
         for (int k = 0; k < nb; ++k) ctx->max_bundle_dirs = std::max(ctx->max_bundle_dirs, (int)bundles[k].size());
         for (int k = 0; k < nb; ++k) {
@@ -1865,8 +1954,21 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->lev_off = ctx->upload(h_lev_se.data(), h_lev_se.size());
         ctx->lev_ptr = ctx->upload(h_lev_ptr.data(), h_lev_ptr.size());
         ctx->lu_off = ctx->upload(h_lu_off.data(), nb + 1);
-        ctx->rec = ctx->upload(h_rec.data(), h_rec.size());
-        ctx->wstash = ctx->upload(h_wst.data(), h_wst.size());
+        {
+            // per item (bundle, slot, direction of the bundle): {cell, face
+            // words}, {stash slot, permutation (written by the factorisation)}
+            std::vector<int4> h_irec((size_t)2 * h_piv_off[nb]);
+            for (int k = 0; k < nb; ++k) {
+                const int B = (int)bundles[k].size();
+                for (int q = 0; q < N; ++q)
+                    for (int b = 0; b < B; ++b) {
+                        const size_t it = (size_t)h_piv_off[k] + (size_t)q * B + b;
+                        h_irec[2 * it] = h_rec[(size_t)k * N + q];
+                        h_irec[2 * it + 1] = make_int4(h_wst[(size_t)k * N + q], 0, 0, 0);
+                    }
+            }
+            ctx->irec = ctx->upload(h_irec.data(), h_irec.size());
+        }
         ctx->bundle_global = ctx->upload(global_flag.data(), global_flag.size());
         for (int k = 0; k < nb; ++k) ctx->global_bundles += global_flag[k];
         for (int k = 0; k < nb; ++k) {
@@ -1886,10 +1988,6 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         HIP_TRY(hipMemset(ctx->src64, 0, sizeof(double) * (size_t)N * pad4(3 * P)));
         HIP_TRY(hipMemset(ctx->src32, 0, sizeof(float) * (size_t)N * pad4(3 * P)));
         ctx->ssraw = ctx->alloc<double>((size_t)N * P);
-        ctx->fw32 = ctx->alloc<float>((size_t)std::max(F, 1) * pad4(P) + 32);
-        ctx->fw64 = ctx->alloc<double>((size_t)std::max(F, 1) * pad4(P) + 32);
-        HIP_TRY(hipMemset(ctx->fw32, 0, sizeof(float) * ((size_t)std::max(F, 1) * pad4(P) + 32)));
-        HIP_TRY(hipMemset(ctx->fw64, 0, sizeof(double) * ((size_t)std::max(F, 1) * pad4(P) + 32)));
         ctx->red_len = std::max<size_t>((size_t)D * nblocks(N, 256) * 2 + 2 * (size_t)D,
                                         (size_t)kMaxBasis * kDotBlocks + kMaxBasis) + 64;
         ctx->red = ctx->alloc<double>(ctx->red_len);
@@ -1928,7 +2026,7 @@ void launch_factor(pbte_ctx* ctx, double* lu64, float* lu32)
     a.bundle_dirs = ctx->bundle_dirs; a.lu_off = ctx->lu_off; a.piv_off = ctx->piv_off;
     a.cxv = ctx->cxv; a.cyv = ctx->cyv; a.mass = ctx->mass; a.gx = ctx->gx; a.gy = ctx->gy; a.fcm = ctx->fcm;
     a.geo = ctx->geo64; a.inv_tau_c = 1.0 / ctx->tau_c; a.N = ctx->N;
-    a.lu64 = lu64; a.lu32 = lu32; a.piv = ctx->piv; a.singular = ctx->counter;
+    a.lu64 = lu64; a.lu32 = lu32; a.piv = ctx->piv; a.irec = ctx->irec; a.singular = ctx->counter;
     dim3 grid(nblocks(ctx->N, 128), ctx->nb);
     hipLaunchKernelGGL(factor_kernel<P>, grid, dim3(128), 0, ctx->stream, a);
     HIP_TRY(hipGetLastError());
@@ -1936,7 +2034,7 @@ void launch_factor(pbte_ctx* ctx, double* lu64, float* lu32)
 
 template <int P, typename T>
 void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const CellGeo<T>* geo, const T* K,
-                  const T* bvec, const T* src, const double* flux_wall, const T* flux_wall_t, const T* pbuf, T* vdf,
+                  const T* bvec, const T* src, const double* flux_wall, const T* crec, const T* pbuf, T* vdf,
                   int use_pbuf, int homogeneous)
 {
     SweepArgs<T> a{};
@@ -1945,15 +2043,14 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
     a.lev_ptr = ctx->lev_ptr; a.lu_off = ctx->lu_off; a.piv_off = ctx->piv_off; a.lu = lu; a.piv = ctx->piv;
     a.cxv = cx; a.cyv = cy; a.geo = geo; a.K = K; a.bvec = bvec; a.src = src; a.flux_wall = flux_wall;
     a.pbuf = pbuf; a.vdf = vdf; a.N = ctx->N; a.F = ctx->F; a.use_pbuf = use_pbuf; a.homogeneous = homogeneous;
-    a.rec = ctx->rec;
-    a.flux_wall_t = flux_wall_t;
+    a.irec = ctx->irec;
+    a.crec = crec;
     a.lane_zero = 0;
     if (ctx->variant == 1) {
         hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
     } else {
         a.ring = ctx->ring_slots;
         a.stash = ctx->stash_slots;
-        a.wstash = ctx->wstash;
         a.bundle_global = ctx->bundle_global;
         const size_t lds = (size_t)(ctx->ring_slots + ctx->stash_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W * sizeof(T);
         constexpr bool pf = P * P * sizeof(T) <= 160;
@@ -1967,14 +2064,47 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
 }
 
 template <typename T>
-void launch_source(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, T* out)
+void launch_source(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, T* out, int stride);
+
+// one sweep: the source (and, for the pipelined kernel, the adiabatic wall
+// vectors) into the layout the sweep kernel reads, then the sweep
+template <typename T>
+void run_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const CellGeo<T>* geo, const T* K, const T* bvec,
+               T* src, T* crec, const double* ts, const double* qxs, const double* qys, const double* flux_wall,
+               const T* pbuf, T* vdf, int use_pbuf, int homogeneous)
+{
+    const int P = ctx->P;
+    if (ctx->variant == 1) {
+        launch_source<T>(ctx, ts, qxs, qys, src, pad4(3 * P));
+    } else {
+        const int CR = pad4(3 * P + 9 + 3 * ctx->Q * ctx->Q);
+        launch_source<T>(ctx, ts, qxs, qys, crec, CR);
+        if (ctx->n_wall) {
+            const unsigned g = nblocks((int64_t)ctx->n_wall * P, 256);
+#define WR(PP) case PP: hipLaunchKernelGGL((wall_rec_kernel<PP, T>), dim3(g), dim3(256), 0, ctx->stream, flux_wall, \
+                                           ctx->wall_face, ctx->wall_tri, ctx->wall_lfc, ctx->n_wall, crec); break;
+            switch (P) { WR(3) WR(6) WR(10) WR(15) }
+#undef WR
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    switch (P) {
+#define SW(PP) case PP: launch_sweep<PP, T>(ctx, lu, cx, cy, geo, K, bvec, src, flux_wall, crec, pbuf, vdf, use_pbuf, \
+                                        homogeneous); break;
+    SW(3) SW(6) SW(10) SW(15)
+#undef SW
+    }
+}
+
+template <typename T>
+void launch_source(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, T* out, int stride)
 {
     const int P = ctx->P;
     const unsigned g = nblocks((int64_t)ctx->N * P, 256);
 #define SRC_CASE(PP)                                                                                 \
     case PP:                                                                                         \
         hipLaunchKernelGGL((source_kernel<PP, T>), dim3(g), dim3(256), 0, ctx->stream, ctx->mass, ts, qxs, \
-                           qys, out, ctx->N, ctx->cv, ctx->pi, ctx->tau_r, ctx->tau_n, ctx->vg);    \
+                           qys, out, stride, ctx->N, ctx->cv, ctx->pi, ctx->tau_r, ctx->tau_n, ctx->vg); \
         break;
     switch (P) { SRC_CASE(3) SRC_CASE(6) SRC_CASE(10) SRC_CASE(15) }
 #undef SRC_CASE
@@ -2020,34 +2150,14 @@ extern "C" int pbte_sweep(pbte_ctx* ctx, int32_t precision, const double* ts, co
         if (ctx->n_wall && !flux_wall) throw std::string("adiabatic walls need flux_wall");
         if (precision == PBTE_FP64) {
             if (!ctx->have64) throw std::string("FP64 factors not built; call pbte_factorise");
-            launch_source<double>(ctx, ts, qxs, qys, ctx->src64);
-            if (ctx->n_wall) {
-                hipLaunchKernelGGL(pad_rows_kernel<double>, dim3(nblocks((int64_t)ctx->F * ctx->P, 256)), dim3(256), 0,
-                                   ctx->stream, flux_wall, ctx->fw64, ctx->F, ctx->P);
-                HIP_TRY(hipGetLastError());
-            }
-            switch (ctx->P) {
-#define SW(PP) case PP: launch_sweep<PP, double>(ctx, ctx->lu64, ctx->cxv, ctx->cyv, ctx->geo64, ctx->K64, \
-                    ctx->bvec64, ctx->src64, flux_wall, ctx->fw64, (const double*)pbuf, (double*)vdf, use_pbuf, \
-                    homogeneous); break;
-            SW(3) SW(6) SW(10) SW(15)
-#undef SW
-            }
+            run_sweep<double>(ctx, ctx->lu64, ctx->cxv, ctx->cyv, ctx->geo64, ctx->K64, ctx->bvec64, ctx->src64,
+                              ctx->crec64, ts, qxs, qys, flux_wall, (const double*)pbuf, (double*)vdf, use_pbuf,
+                              homogeneous);
         } else if (precision == PBTE_FP32) {
             if (!ctx->have32) throw std::string("FP32 factors not built; call pbte_factorise");
-            launch_source<float>(ctx, ts, qxs, qys, ctx->src32);
-            if (ctx->n_wall) {
-                hipLaunchKernelGGL(pad_rows_kernel<float>, dim3(nblocks((int64_t)ctx->F * ctx->P, 256)), dim3(256), 0,
-                                   ctx->stream, flux_wall, ctx->fw32, ctx->F, ctx->P);
-                HIP_TRY(hipGetLastError());
-            }
-            switch (ctx->P) {
-#define SW(PP) case PP: launch_sweep<PP, float>(ctx, ctx->lu32, ctx->cxv32, ctx->cyv32, ctx->geo32, ctx->K32, \
-                    ctx->bvec32, ctx->src32, flux_wall, ctx->fw32, (const float*)pbuf, (float*)vdf, use_pbuf, \
-                    homogeneous); break;
-            SW(3) SW(6) SW(10) SW(15)
-#undef SW
-            }
+            run_sweep<float>(ctx, ctx->lu32, ctx->cxv32, ctx->cyv32, ctx->geo32, ctx->K32, ctx->bvec32, ctx->src32,
+                             ctx->crec32, ts, qxs, qys, flux_wall, (const float*)pbuf, (float*)vdf, use_pbuf,
+                             homogeneous);
         } else {
             throw std::string("bad precision");
         }
@@ -2099,7 +2209,7 @@ extern "C" int pbte_transport_residual(pbte_ctx* ctx, const double* ts, const do
 {
     return guarded(ctx, [&]() {
         if (ctx->n_wall && !flux_wall) throw std::string("adiabatic walls need flux_wall");
-        launch_source<double>(ctx, ts, qxs, qys, ctx->src64);
+        launch_source<double>(ctx, ts, qxs, qys, ctx->src64, pad4(3 * ctx->P));
         const unsigned gx = nblocks(ctx->N, 256);
         dim3 grid(gx, ctx->D);
         double* part = ctx->red;

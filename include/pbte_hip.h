@@ -162,7 +162,7 @@ int pbte_vec_update(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m,
 /* y[i] = a x[i] + b y[i] */
 int pbte_vec_axpby(pbte_ctx *ctx, double a, const double *x, double b, double *y, int64_t n);
 
-/* ---- GSIS with the HDG synthetic system (acceleration/*.py) ---------- */
+/* ---- GSIS with the HDG synthetic system (acceleration package) -------- */
 
 /* Host arrays read once by pbte_gsis_setup (index orders as the
  * reference's Acceleration, acceleration/__init__.py:37-92). */
