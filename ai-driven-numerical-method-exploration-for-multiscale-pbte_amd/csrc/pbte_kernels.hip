@@ -142,8 +142,9 @@ struct pbte_ctx {
     int64_t total_levels = 0, max_width = 0;
     int32_t *block_bundle = nullptr, *bundle_ndir = nullptr, *bundle_dir_off = nullptr,
             *bundle_dirs = nullptr, *slot_cell = nullptr, *lev_off = nullptr, *lev_ptr = nullptr;
-    int64_t *lu_off = nullptr, *piv_off = nullptr;
-    int4* irec = nullptr;       // per item (bundle, slot, direction): {cell, face words}, {stash, perm}
+    int64_t *lu_off = nullptr, *item_off = nullptr;
+    int4* irec = nullptr;       // per item (bundle, slot, direction): {cell, face words}
+    int32_t* iwst = nullptr;    // per item: stash slot the item also writes, or -1
     double* crec64 = nullptr;   // cell records of the pipelined sweep
     float* crec32 = nullptr;
     uint8_t* bundle_global = nullptr;
@@ -155,11 +156,10 @@ struct pbte_ctx {
     int64_t old_reads = 0;      // previous-sweep reads (broken cycles)
     int64_t periodic_reads = 0; // upwind reads across periodic faces
     std::vector<int64_t> h_lu_off;
-    int64_t lu_elems = 0, piv_elems = 0;
+    int64_t lu_elems = 0, item_elems = 0;
     // factors
     double* lu64 = nullptr;
     float* lu32 = nullptr;
-    uint64_t* piv = nullptr;
     bool have64 = false, have32 = false;
     // scratch
     double* src64 = nullptr;
@@ -216,21 +216,19 @@ struct pbte_ctx {
 namespace {
 
 // ==========================================================================
-// batched element LU: build A_SOL (_kernels.py:106-125) and factorise it
-// with DGETF2's pivoting (_kernels.py:43-76), both in binary64; store in the
-// sweep layout with the reciprocal of U's diagonal in the diagonal slot.
+// element operators: build A_SOL (_kernels.py:106-125), factorise it with
+// DGETF2's pivoting (_kernels.py:43-76) and form its inverse from the
+// factors, all in binary64; the sweep layout stores the inverse.
 // ==========================================================================
 struct FactorArgs {
     const int32_t *slot_cell, *bundle_ndir, *bundle_dir_off, *bundle_dirs;
-    const int64_t *lu_off, *piv_off;
+    const int64_t *lu_off, *item_off;
     const double *cxv, *cyv, *mass, *gx, *gy, *fcm;
     const CellGeo<double>* geo;
     double inv_tau_c;
     int N;
     double* lu64;
     float* lu32;
-    uint64_t* piv;
-    int4* irec;
     int32_t* singular;
 };
 
@@ -303,35 +301,43 @@ __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
             A[k][k] = rcp;   // the solve multiplies by the reciprocal
         }
         if (info) atomicAdd(a.singular, 1);
-        // the row interchanges composed into one permutation: (P b)[k] = b[perm[k]]
-        int idx[P];
+        // the inverse, column by column from the pivoted factors (DGETRS on
+        // the unit vectors): the sweep applies x = inv(A) b
+        double Ai[P][P];
 #pragma unroll
-        for (int k = 0; k < P; ++k) idx[k] = k;
+        for (int j = 0; j < P; ++j) {
+            double v[P];
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const int p = (int)((pv >> (4 * k)) & 15u);
+            for (int k = 0; k < P; ++k) v[k] = k == j ? 1.0 : 0.0;
 #pragma unroll
-            for (int r = k + 1; r < P; ++r)
-                if (p == r) { int t = idx[k]; idx[k] = idx[r]; idx[r] = t; }
+            for (int k = 0; k < P; ++k) {
+                const int p = (int)((pv >> (4 * k)) & 15u);
+#pragma unroll
+                for (int r = k + 1; r < P; ++r)
+                    if (p == r) { double t = v[k]; v[k] = v[r]; v[r] = t; }
+            }
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+#pragma unroll
+                for (int r = k + 1; r < P; ++r) v[r] -= v[k] * A[r][k];
+#pragma unroll
+            for (int k = P - 1; k >= 0; --k) {
+                v[k] = v[k] * A[k][k];
+#pragma unroll
+                for (int r = 0; r < k; ++r) v[r] -= v[k] * A[r][k];
+            }
+#pragma unroll
+            for (int r = 0; r < P; ++r) Ai[r][j] = v[r];
         }
-        uint64_t perm = 0;
-#pragma unroll
-        for (int k = 0; k < P; ++k) perm |= (uint64_t)idx[k] << (4 * k);
         const int64_t base = a.lu_off[bundle];
         const size_t item = (size_t)s * B + b, NB = (size_t)N * B;
 #pragma unroll
         for (int r = 0; r < P; ++r)
 #pragma unroll
             for (int c = 0; c < P; ++c) {
-                if (a.lu64) a.lu64[base + lu_idx<double>(r * P + c, item, NB)] = A[r][c];
-                if (a.lu32) a.lu32[base + lu_idx<float>(r * P + c, item, NB)] = (float)A[r][c];
+                if (a.lu64) a.lu64[base + lu_idx<double>(r * P + c, item, NB)] = Ai[r][c];
+                if (a.lu32) a.lu32[base + lu_idx<float>(r * P + c, item, NB)] = (float)Ai[r][c];
             }
-        a.piv[a.piv_off[bundle] + item] = perm;
-        if (a.irec) {
-            int4* ir = a.irec + 2 * (a.piv_off[bundle] + item) + 1;
-            ir->y = (int)(uint32_t)(perm & 0xffffffffu);
-            ir->z = (int)(uint32_t)(perm >> 32);
-        }
     }
 }
 
@@ -377,9 +383,8 @@ __global__ void __launch_bounds__(256) source_kernel(const double* __restrict__ 
 template <typename T> struct SweepArgs {
     const int32_t *block_bundle, *bundle_ndir, *bundle_dir_off, *bundle_dirs, *slot_cell,
         *lev_off, *lev_ptr;
-    const int64_t *lu_off, *piv_off;
+    const int64_t *lu_off, *item_off;
     const T *lu;
-    const uint64_t* piv;
     const T *cxv, *cyv;
     const CellGeo<T>* geo;
     const T* K;
@@ -391,6 +396,7 @@ template <typename T> struct SweepArgs {
     int N, F, use_pbuf, homogeneous;
     // pipelined variant: per-item records, cell records, LDS ring and stash
     const int4* irec;
+    const int32_t* iwst;
     const T* crec;
     int lane_zero;              // 0 at run time; makes a uniform load per-lane
     const uint8_t* bundle_global;
@@ -418,6 +424,38 @@ __host__ __device__ constexpr bool rec_reads_neighbour(int code)
     return code == REC_UP_RING || code == REC_UP_STASH || (code >= REC_OLD && code <= REC_PER_GLOBAL);
 }
 
+// the face coupling of one upwind face: b[node(il, r)] -= w (K u)[r], the
+// product K u accumulated first (the order every sweep kernel uses)
+template <int P, typename T>
+__device__ __forceinline__ void face_update(T (&x)[P], int il, T w, const T* K, const T* u)
+{
+    constexpr int DEG = Deg<P>::v;
+    constexpr int Q = DEG + 1;
+#pragma unroll
+    for (int r = 0; r < Q; ++r) {
+        T t = K[r * Q] * u[0];
+#pragma unroll
+        for (int q = 1; q < Q; ++q) t += K[r * Q + q] * u[q];
+        x[face_node(DEG, il, r)] -= w * t;
+    }
+}
+
+// x = inv(A) b, each row accumulated in column order
+template <int P, typename T, class E>
+__device__ __forceinline__ void apply_inverse(T (&x)[P], E&& e)
+{
+    T y[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        T v = e(r * P) * x[0];
+#pragma unroll
+        for (int c = 1; c < P; ++c) v += e(r * P + c) * x[c];
+        y[r] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < P; ++r) x[r] = y[r];
+}
+
 template <int P, typename T>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
 {
@@ -433,7 +471,6 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
     const int N = a.N;
     const int32_t* cells = a.slot_cell + (size_t)bundle * N;
     const T* lu = a.lu + a.lu_off[bundle];
-    const uint64_t* piv = a.piv + a.piv_off[bundle];
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
 
@@ -460,8 +497,9 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
                     for (int m = 0; m < P; ++m) x[m] += w * (T)fw[m];
                 } else if (code == FACE_THERMAL && !a.homogeneous) {
                     const T* bv = a.bvec + (size_t)g.other[il] * pad4(P);
+                    const T mw = -w;
 #pragma unroll
-                    for (int m = 0; m < P; ++m) x[m] -= w * bv[m];
+                    for (int m = 0; m < P; ++m) x[m] += mw * bv[m];
                 }
             }
 #pragma unroll
@@ -479,48 +517,11 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
                     T fb[Q];
 #pragma unroll
                     for (int q = 0; q < Q; ++q) fb[q] = fe[face_node(DEG, ilp, q)];
-                    const T* Kf = a.K + ((size_t)i * 3 + il) * Lay<P>::KQ;
-#pragma unroll
-                    for (int r = 0; r < Q; ++r) {
-                        T acc = x[face_node(DEG, il, r)];
-#pragma unroll
-                        for (int q = 0; q < Q; ++q) acc -= (w * Kf[r * Q + q]) * fb[q];
-                        x[face_node(DEG, il, r)] = acc;
-                    }
+                    face_update<P, T>(x, il, w, a.K + ((size_t)i * 3 + il) * Lay<P>::KQ, fb);
                 }
-            }
-            // P b (the composed DGETRS row interchanges)
-            const uint64_t pv = piv[(size_t)s * B + b];
-            {
-                T y[P];
-#pragma unroll
-                for (int k = 0; k < P; ++k) {
-                    const int m = (int)((pv >> (4 * k)) & 15u);
-                    T v = x[0];
-#pragma unroll
-                    for (int j = 1; j < P; ++j) v = (m == j) ? x[j] : v;
-                    y[k] = v;
-                }
-#pragma unroll
-                for (int k = 0; k < P; ++k) x[k] = y[k];
             }
             const size_t item = (size_t)s * B + b, NB = (size_t)N * B;
-            auto e = [&](int k) { return lu[lu_idx<T>(k, item, NB)]; };
-            // unit lower solve
-#pragma unroll
-            for (int k = 0; k < P; ++k) {
-                const T xk = x[k];
-#pragma unroll
-                for (int r = k + 1; r < P; ++r) x[r] -= xk * e(r * P + k);
-            }
-            // upper solve, accumulated downwards (DTRSM column order)
-#pragma unroll
-            for (int k = P - 1; k >= 0; --k) {
-                x[k] = x[k] * e(k * P + k);
-                const T xk = x[k];
-#pragma unroll
-                for (int r = 0; r < k; ++r) x[r] -= xk * e(r * P + k);
-            }
+            apply_inverse<P, T>(x, [&](int k) { return lu[lu_idx<T>(k, item, NB)]; });
             T* o = vdf_d + (size_t)i * P;
 #pragma unroll
             for (int m = 0; m < P; ++m) o[m] = x[m];
@@ -629,7 +630,7 @@ template <int P, typename T, bool PFLU> struct PipeData {
     bool valid;
     int cell, slot, fw[3];
     int lds[3];         // LDS element offset of the upwind trace slot of each face (zero slot when none)
-    int4 s;             // {stash slot, permutation}
+    int wst;            // stash slot this item also writes, or -1
     T cr[L::CR];        // the cell record
     T lu[PFLU ? L::LU : 4];
 };
@@ -645,7 +646,7 @@ __device__ __forceinline__ void pipe_record(const int4* irec, int2 span, int c, 
     const int sl = span.x + c;
     o.valid = sl < span.y;
     o.slot = o.valid ? sl : span.x;
-    o.r = irec[2 * ((size_t)o.slot * B + b)];
+    o.r = irec[(size_t)o.slot * B + b];
 }
 
 // record-code classes as bit masks over the code: one shift and one test,
@@ -675,7 +676,7 @@ __device__ __forceinline__ int bits_of(double v) { return (int)(uint32_t)(uint64
 // in-order vmcnt counter would make an earlier use wait for every
 // prefetch in flight)
 template <int P, typename T, bool PFLU>
-__device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int4* irec, const PipeRec& r, int B, int b,
+__device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int32_t* iwst, const PipeRec& r, int B, int b,
                                             const T* lu, PipeData<P, T, PFLU>& o)
 {
     using L = Lay<P>;
@@ -687,23 +688,20 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int4* i
     o.fw[0] = r.r.y;
     o.fw[1] = r.r.z;
     o.fw[2] = r.r.w;
-    const int RM = a.ring - 1;
     const int zero = a.ring + a.stash;
     const int lds_mask = rec_lds_mask(a.use_pbuf);
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
-        // every choice below is a select between values computed beforehand
+        // the value of an LDS face is its LDS slot; every other face reads
+        // the all-zero slot (a select, no control flow)
         const int code = (int)(((unsigned)o.fw[il]) >> 28);
         const int val = o.fw[il] & 0x0fffffff;
-        const int ring_sl = val & RM;
-        const int stash_sl = a.ring + val;
-        int sl = code_in(kRecRing, code) ? ring_sl : stash_sl;
-        sl = code_in(lds_mask, code) ? sl : zero;
+        const int sl = code_in(lds_mask, code) ? val : zero;
         o.lds[il] = (sl * B + b) * W;
     }
-    // the second record word (stash slot, permutation) is only used by the
-    // solve: it travels with the gathered data
-    o.s = irec[2 * ((size_t)r.slot * B + b) + 1];
+    // the stash slot is only used by the solve: it travels with the
+    // gathered data
+    o.wst = iwst[(size_t)r.slot * B + b];
     load16<T, L::CR>(o.cr, a.crec + (size_t)i * L::CR);
     if constexpr (PFLU) {
         using V = typename Vec16<T>::type;
@@ -789,48 +787,15 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
         const T wn = code_in(kRecNeighbour, code[il]) ? w[il] : T(0);
-#pragma unroll
-        for (int r = 0; r < Q; ++r) {
-            T acc = x[face_node(DEG, il, r)];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) acc -= (wn * cr[L::R_FD + il * QQ + r * Q + q]) * u[il][q];
-            x[face_node(DEG, il, r)] = acc;
-        }
-    }
-    // P b (the composed DGETRS row interchanges)
-    {
-        T y[P];
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            const int m = (int)((((unsigned)(k < 8 ? dC.s.y : dC.s.z)) >> (4 * (k & 7))) & 15u);
-            T v = x[0];
-#pragma unroll
-            for (int j = 1; j < P; ++j) v = (m == j) ? x[j] : v;
-            y[k] = v;
-        }
-#pragma unroll
-        for (int k = 0; k < P; ++k) x[k] = y[k];
+        face_update<P, T>(x, il, wn, cr + L::R_FD + il * QQ, u[il]);
     }
     const size_t item = (size_t)dC.slot * B + b, NB = (size_t)a.N * B;
-    auto e = [&](int k) { return PFLU ? dC.lu[k] : lu[lu_idx<T>(k, item, NB)]; };
-#pragma unroll
-    for (int k = 0; k < P; ++k) {
-        const T xk = x[k];
-#pragma unroll
-        for (int r = k + 1; r < P; ++r) x[r] -= xk * e(r * P + k);
-    }
-#pragma unroll
-    for (int k = P - 1; k >= 0; --k) {
-        x[k] = x[k] * e(k * P + k);
-        const T xk = x[k];
-#pragma unroll
-        for (int r = 0; r < k; ++r) x[r] -= xk * e(r * P + k);
-    }
+    apply_inverse<P, T>(x, [&](int k) { return PFLU ? dC.lu[k] : lu[lu_idx<T>(k, item, NB)]; });
     if (dC.valid) {
         constexpr int W = Tr<P>::W;
         const int RM = a.ring - 1;
         const int wr0 = ((dC.slot & RM) * B + b) * W;
-        const int st_sl = dC.s.x >= 0 ? a.ring + dC.s.x : a.ring + a.stash + 1;
+        const int st_sl = dC.wst >= 0 ? a.ring + dC.wst : a.ring + a.stash + 1;
         const int wr1 = (st_sl * B + b) * W;
         T* o = vdf_d + (size_t)dC.cell * P;
 #pragma unroll
@@ -875,7 +840,8 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
     const T cx = a.cxv[d], cy = a.cyv[d];
     const int N = a.N;
-    const int4* irec = a.irec + 2 * a.piv_off[bundle];
+    const int4* irec = a.irec + a.item_off[bundle];
+    const int32_t* iwst = a.iwst + a.item_off[bundle];
     const T* lu = a.lu + a.lu_off[bundle];
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
@@ -900,17 +866,17 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     // in order, and the loop-entry state must not ask for more waiting
     pipe_record(irec, span(l0), c, B, b, R0);
     pipe_record(irec, span(l0 + 1), c, B, b, R1);
-    pipe_gather<P, T, PFLU>(a, irec, R0, B, b, lu, D0);
+    pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);
     for (int lv = l0; lv < l1; lv += 2) {
         pipe_record(irec, S0, c, B, b, R0);                            // level lv+2
         S0 = span(lv + 4);
-        pipe_gather<P, T, PFLU>(a, irec, R1, B, b, lu, D1);                  // level lv+1
+        pipe_gather<P, T, PFLU>(a, iwst, R1, B, b, lu, D1);                  // level lv+1
         pipe_solve<P, T, PFLU, GREAD>(a, D0, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv
         level_barrier(global_mode);
         if (lv + 1 >= l1) break;
         pipe_record(irec, S1, c, B, b, R1);                            // level lv+3
         S1 = span(lv + 5);
-        pipe_gather<P, T, PFLU>(a, irec, R0, B, b, lu, D0);                  // level lv+2
+        pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);                  // level lv+2
         pipe_solve<P, T, PFLU, GREAD>(a, D1, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+1
         level_barrier(global_mode);
     }
@@ -1715,7 +1681,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         // levels per bundle (all directions of a bundle share them)
         std::vector<int32_t> h_slot((size_t)nb * N), h_lev_off(nb + 1, 0), h_lev_ptr;
         std::vector<int32_t> h_ndir(nb), h_dir_off(nb + 1, 0), h_dirs;
-        std::vector<int64_t> h_lu_off(nb + 1, 0), h_piv_off(nb + 1, 0);
+        std::vector<int64_t> h_lu_off(nb + 1, 0), h_item_off(nb + 1, 0);
         std::vector<std::vector<int32_t>> slots(nb);
         std::vector<std::vector<int32_t>> lptr(nb);
         std::vector<int4> h_rec((size_t)nb * N);
@@ -1883,7 +1849,9 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                             if (r.code == REC_UP_RING || r.code == REC_UP_STASH) r = {REC_UP_GLOBAL, sl[r.val]};
                             else if (r.code == REC_PER_RING || r.code == REC_PER_STASH) r = {REC_PER_GLOBAL, sl[r.val]};
                         } else if (r.code == REC_UP_STASH || r.code == REC_PER_STASH) {
-                            r.val = sid[r.val];
+                            r.val = ctx->ring_slots + sid[r.val];     // the LDS slot
+                        } else if (r.code == REC_UP_RING || r.code == REC_PER_RING) {
+                            r.val = r.val & (ctx->ring_slots - 1);    // the LDS slot
                         }
                         w[il] = (int)(((unsigned)r.code << 28) | (unsigned)r.val);
                         if (r.code == REC_OLD || r.code == REC_PER_OLD) ++old_reads[k];
@@ -1914,7 +1882,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             }
             h_lev_ptr.push_back(N);
             h_lu_off[k + 1] = h_lu_off[k] + (int64_t)pad4(P * P) * N * B;
-            h_piv_off[k + 1] = h_piv_off[k] + (int64_t)N * B;
+            h_item_off[k + 1] = h_item_off[k] + (int64_t)N * B;
         }
         // lev_off[k]..lev_off[k]+nlev_k-1 index level starts; lev_ptr[j+1] is
         // the end of level j; the sentinel N closes every bundle's last level.
@@ -1956,18 +1924,20 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->lu_off = ctx->upload(h_lu_off.data(), nb + 1);
         {
             // per item (bundle, slot, direction of the bundle): {cell, face
-            // words}, {stash slot, permutation (written by the factorisation)}
-            std::vector<int4> h_irec((size_t)2 * h_piv_off[nb]);
+            // words} and the stash slot
+            std::vector<int4> h_irec((size_t)h_item_off[nb]);
+            std::vector<int32_t> h_iwst((size_t)h_item_off[nb]);
             for (int k = 0; k < nb; ++k) {
                 const int B = (int)bundles[k].size();
                 for (int q = 0; q < N; ++q)
                     for (int b = 0; b < B; ++b) {
-                        const size_t it = (size_t)h_piv_off[k] + (size_t)q * B + b;
-                        h_irec[2 * it] = h_rec[(size_t)k * N + q];
-                        h_irec[2 * it + 1] = make_int4(h_wst[(size_t)k * N + q], 0, 0, 0);
+                        const size_t it = (size_t)h_item_off[k] + (size_t)q * B + b;
+                        h_irec[it] = h_rec[(size_t)k * N + q];
+                        h_iwst[it] = h_wst[(size_t)k * N + q];
                     }
             }
             ctx->irec = ctx->upload(h_irec.data(), h_irec.size());
+            ctx->iwst = ctx->upload(h_iwst.data(), h_iwst.size());
         }
         ctx->bundle_global = ctx->upload(global_flag.data(), global_flag.size());
         for (int k = 0; k < nb; ++k) ctx->global_bundles += global_flag[k];
@@ -1977,10 +1947,10 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             ctx->periodic_reads += per_reads[k];
         }
         if (N >= (1 << 28) || F >= (1 << 28)) throw std::string("mesh too large for 28-bit record fields");
-        ctx->piv_off = ctx->upload(h_piv_off.data(), nb + 1);
+        ctx->item_off = ctx->upload(h_item_off.data(), nb + 1);
         ctx->h_lu_off = h_lu_off;
         ctx->lu_elems = h_lu_off[nb];
-        ctx->piv_elems = h_piv_off[nb];
+        ctx->item_elems = h_item_off[nb];
 
         // --- scratch -----------------------------------------------------------
         ctx->src64 = ctx->alloc<double>((size_t)N * pad4(3 * P));
@@ -2023,10 +1993,10 @@ void launch_factor(pbte_ctx* ctx, double* lu64, float* lu32)
 {
     FactorArgs a{};
     a.slot_cell = ctx->slot_cell; a.bundle_ndir = ctx->bundle_ndir; a.bundle_dir_off = ctx->bundle_dir_off;
-    a.bundle_dirs = ctx->bundle_dirs; a.lu_off = ctx->lu_off; a.piv_off = ctx->piv_off;
+    a.bundle_dirs = ctx->bundle_dirs; a.lu_off = ctx->lu_off; a.item_off = ctx->item_off;
     a.cxv = ctx->cxv; a.cyv = ctx->cyv; a.mass = ctx->mass; a.gx = ctx->gx; a.gy = ctx->gy; a.fcm = ctx->fcm;
     a.geo = ctx->geo64; a.inv_tau_c = 1.0 / ctx->tau_c; a.N = ctx->N;
-    a.lu64 = lu64; a.lu32 = lu32; a.piv = ctx->piv; a.irec = ctx->irec; a.singular = ctx->counter;
+    a.lu64 = lu64; a.lu32 = lu32; a.singular = ctx->counter;
     dim3 grid(nblocks(ctx->N, 128), ctx->nb);
     hipLaunchKernelGGL(factor_kernel<P>, grid, dim3(128), 0, ctx->stream, a);
     HIP_TRY(hipGetLastError());
@@ -2040,10 +2010,11 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
     SweepArgs<T> a{};
     a.block_bundle = ctx->block_bundle; a.bundle_ndir = ctx->bundle_ndir; a.bundle_dir_off = ctx->bundle_dir_off;
     a.bundle_dirs = ctx->bundle_dirs; a.slot_cell = ctx->slot_cell; a.lev_off = ctx->lev_off;
-    a.lev_ptr = ctx->lev_ptr; a.lu_off = ctx->lu_off; a.piv_off = ctx->piv_off; a.lu = lu; a.piv = ctx->piv;
+    a.lev_ptr = ctx->lev_ptr; a.lu_off = ctx->lu_off; a.item_off = ctx->item_off; a.lu = lu;
     a.cxv = cx; a.cyv = cy; a.geo = geo; a.K = K; a.bvec = bvec; a.src = src; a.flux_wall = flux_wall;
     a.pbuf = pbuf; a.vdf = vdf; a.N = ctx->N; a.F = ctx->F; a.use_pbuf = use_pbuf; a.homogeneous = homogeneous;
     a.irec = ctx->irec;
+    a.iwst = ctx->iwst;
     a.crec = crec;
     a.lane_zero = 0;
     if (ctx->variant == 1) {
@@ -2053,7 +2024,7 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
         a.stash = ctx->stash_slots;
         a.bundle_global = ctx->bundle_global;
         const size_t lds = (size_t)(ctx->ring_slots + ctx->stash_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W * sizeof(T);
-        constexpr bool pf = P * P * sizeof(T) <= 160;
+        constexpr bool pf = P * P * sizeof(T) <= 288;
         const bool gread = ctx->global_bundles > 0 || ctx->old_reads > 0 || (use_pbuf && ctx->periodic_reads > 0);
         if (gread)
             hipLaunchKernelGGL((sweep_pipe_kernel<P, T, pf, true>), dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
@@ -2116,7 +2087,6 @@ extern "C" int pbte_factorise(pbte_ctx* ctx, int32_t precision)
 {
     return guarded(ctx, [&]() {
         if (precision != PBTE_FP64 && precision != PBTE_FP32) throw std::string("bad precision");
-        if (!ctx->piv) ctx->piv = ctx->alloc<uint64_t>((size_t)ctx->piv_elems);
         double* l64 = nullptr;
         float* l32 = nullptr;
         if (precision == PBTE_FP64) {
