@@ -169,6 +169,7 @@ struct pbte_ctx {
     size_t red_len = 0;
     double* small = nullptr;   // small device vector (GMRES coefficients)
     int32_t* counter = nullptr;
+    double* spill_row = nullptr;   // 16 values written by idle sweep threads
     double* hbuf = nullptr;    // pinned host staging
     size_t hbuf_len = 0;
 
@@ -398,6 +399,7 @@ template <typename T> struct SweepArgs {
     const int4* irec;
     const int32_t* iwst;
     const T* crec;
+    T* spill_row;               // write target of idle threads (P values)
     int lane_zero;              // 0 at run time; makes a uniform load per-lane
     const uint8_t* bundle_global;
     int ring, stash;            // LDS ring and stash slots
@@ -791,13 +793,18 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
     }
     const size_t item = (size_t)dC.slot * B + b, NB = (size_t)a.N * B;
     apply_inverse<P, T>(x, [&](int k) { return PFLU ? dC.lu[k] : lu[lu_idx<T>(k, item, NB)]; });
-    if (dC.valid) {
+    // stores without a branch: an idle thread writes its values to the
+    // spill slot and to a scratch row (a branch would let the compiler sink
+    // the whole solve into it, leaving the loads of idle threads unconsumed)
+    {
         constexpr int W = Tr<P>::W;
         const int RM = a.ring - 1;
-        const int wr0 = ((dC.slot & RM) * B + b) * W;
-        const int st_sl = dC.wst >= 0 ? a.ring + dC.wst : a.ring + a.stash + 1;
+        const int spill = a.ring + a.stash + 1;
+        const int ring_sl = dC.valid ? (dC.slot & RM) : spill;
+        const int st_sl = (dC.valid && dC.wst >= 0) ? a.ring + dC.wst : spill;
+        const int wr0 = (ring_sl * B + b) * W;
         const int wr1 = (st_sl * B + b) * W;
-        T* o = vdf_d + (size_t)dC.cell * P;
+        T* o = dC.valid ? vdf_d + (size_t)dC.cell * P : a.spill_row;
 #pragma unroll
         for (int m = 0; m < P; ++m) o[m] = x[m];
 #pragma unroll
@@ -867,13 +874,16 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     pipe_record(irec, span(l0), c, B, b, R0);
     pipe_record(irec, span(l0 + 1), c, B, b, R1);
     pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);
+    // two levels per trip, with no exit between them: an odd level count
+    // runs one empty level (span (0, 0)) at the end.  A mid-trip exit would
+    // make the compiler rotate the register sets with copies, and a copy of
+    // a loaded value waits for it.
     for (int lv = l0; lv < l1; lv += 2) {
         pipe_record(irec, S0, c, B, b, R0);                            // level lv+2
         S0 = span(lv + 4);
         pipe_gather<P, T, PFLU>(a, iwst, R1, B, b, lu, D1);                  // level lv+1
         pipe_solve<P, T, PFLU, GREAD>(a, D0, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv
         level_barrier(global_mode);
-        if (lv + 1 >= l1) break;
         pipe_record(irec, S1, c, B, b, R1);                            // level lv+3
         S1 = span(lv + 5);
         pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);                  // level lv+2
@@ -1963,6 +1973,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->red = ctx->alloc<double>(ctx->red_len);
         ctx->small = ctx->alloc<double>(4096);
         ctx->counter = ctx->alloc<int32_t>(16);
+        ctx->spill_row = ctx->alloc<double>(16);
         HIP_TRY(hipDeviceSynchronize());
     } catch (const std::string& e) {
         delete ctx;
@@ -2015,6 +2026,7 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
     a.pbuf = pbuf; a.vdf = vdf; a.N = ctx->N; a.F = ctx->F; a.use_pbuf = use_pbuf; a.homogeneous = homogeneous;
     a.irec = ctx->irec;
     a.iwst = ctx->iwst;
+    a.spill_row = reinterpret_cast<T*>(ctx->spill_row);
     a.crec = crec;
     a.lane_zero = 0;
     if (ctx->variant == 1) {
