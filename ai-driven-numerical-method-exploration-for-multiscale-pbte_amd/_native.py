@@ -58,6 +58,8 @@ _SIGS = {
     "pbte_ctx_destroy": (None, [C.c_void_p]),
     "pbte_last_error": (C.c_char_p, [C.c_void_p]),
     "pbte_schedule_info": (C.c_int, [C.c_void_p, _i64p]),
+    "pbte_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pbte_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int32]),
     "pbte_factorise": (C.c_int, [C.c_void_p, C.c_int32]),
     "pbte_sweep": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]),

@@ -170,6 +170,9 @@ struct pbte_ctx {
     double* small = nullptr;   // small device vector (GMRES coefficients)
     int32_t* counter = nullptr;
     double* spill_row = nullptr;   // 16 values written by idle sweep threads
+    // sweep-kernel timing events (armed by pbte_timing)
+    std::vector<hipEvent_t> events;
+    int ev_used = 0;
     double* hbuf = nullptr;    // pinned host staging
     size_t hbuf_len = 0;
 
@@ -199,6 +202,7 @@ struct pbte_ctx {
     }
     ~pbte_ctx()
     {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (void* p : owned) (void)hipFree(p);
         if (hbuf) (void)hipHostFree(hbuf);
     }
@@ -1985,6 +1989,34 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
     return ctx;
 }
 
+extern "C" int pbte_timing(pbte_ctx* ctx, int32_t capacity)
+{
+    return guarded(ctx, [&]() {
+        if (capacity < 0 || capacity > (1 << 20)) throw std::string("bad timing capacity");
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (hipEvent_t e : ctx->events) HIP_TRY(hipEventDestroy(e));
+        ctx->events.clear();
+        ctx->ev_used = 0;
+        for (int k = 0; k < 2 * capacity; ++k) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            ctx->events.push_back(e);
+        }
+        return 0;
+    });
+}
+
+extern "C" int pbte_timing_read(pbte_ctx* ctx, float* ms, int32_t n)
+{
+    return guarded(ctx, [&]() {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        const int m = std::min(n, ctx->ev_used);
+        for (int k = 0; k < m; ++k)
+            HIP_TRY(hipEventElapsedTime(ms + k, ctx->events[2 * k], ctx->events[2 * k + 1]));
+        return m;
+    });
+}
+
 extern "C" int pbte_schedule_info(pbte_ctx* ctx, int64_t* out6)
 {
     return guarded(ctx, [&]() {
@@ -2029,6 +2061,8 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
     a.spill_row = reinterpret_cast<T*>(ctx->spill_row);
     a.crec = crec;
     a.lane_zero = 0;
+    const bool timed = 2 * (size_t)ctx->ev_used + 1 < ctx->events.size();
+    if (timed) HIP_TRY(hipEventRecord(ctx->events[2 * ctx->ev_used], ctx->stream));
     if (ctx->variant == 1) {
         hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
     } else {
@@ -2044,6 +2078,10 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
             hipLaunchKernelGGL((sweep_pipe_kernel<P, T, pf, false>), dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
     }
     HIP_TRY(hipGetLastError());
+    if (timed) {
+        HIP_TRY(hipEventRecord(ctx->events[2 * ctx->ev_used + 1], ctx->stream));
+        ++ctx->ev_used;
+    }
 }
 
 template <typename T>

@@ -134,6 +134,17 @@ class SweepContext:
                 "stash_reads": int(out[3]), "global_bundles": int(out[4]),
                 "ring_slots": int(out[5])}
 
+    def timing(self, capacity: int) -> None:
+        """Arm HIP events around the next ``capacity`` sweep-kernel launches."""
+        _native.check(self.handle, _native.lib().pbte_timing(self.handle, int(capacity)), "timing")
+
+    def timing_read(self, n: int) -> np.ndarray:
+        """Durations (ms) of the recorded sweep-kernel launches."""
+        out = (C.c_float * max(int(n), 1))()
+        m = _native.lib().pbte_timing_read(self.handle, out, int(n))
+        _native.check(self.handle, m, "timing_read")
+        return np.array(out[:m], dtype=np.float64)
+
     def operator_bytes(self, n_tris: int, ndof: int) -> float:
         return float(self.ndir) * n_tris * ndof * ndof * 8.0
 

@@ -4,15 +4,22 @@ Workload (default): the Krylov configuration -- unit square, 128 x 128
 structured grid split into 32 768 P2 triangles (Np = 6), 256 ordinates
 (8 Gauss-Legendre polar x 32 azimuth), Kn_R = 1, Kn_N = 0.1, diffuse
 isothermal walls with the north wall at dT = +1.  A step is one transport
-sweep over all 256 directions (FP32 element LU and distribution, the
+sweep over all 256 directions (FP32 element operators and distribution, the
 FP32-sweep leg of the mixed-precision solve), with the moments already
 resident in HBM.
 
-Prints one JSON line: sweep DOF-updates/s, the sweep kernel's roofline
-(algorithmic bytes per launch / mean kernel time from HIP events on the
-launch stream), a CPU baseline (the C restatement of the reference's sweep,
-timed on this host) and, with --solve, the time-to-solution of the FP64 and
-FP32+FP64-refinement Krylov solves.
+Prints one JSON line:
+  value          sweep DOF-updates/s over the timed steps (wall clock);
+  roofline       algorithmic bytes per sweep / mean sweep-kernel time, the
+                 kernel timed with HIP events recorded on its launch stream
+                 around each launch; `traffic` is the HBM bytes per launch
+                 from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+                 (profiles/sweep_traffic.json), null for other workloads;
+  solve          time-to-solution and solves/hour of the FP64 and the
+                 FP32-sweep + FP64-refinement Krylov solves, and the relative
+                 L2 differences of T and heat flux between them (--no-solve
+                 skips it);
+  cpu_baseline   the C restatement of the reference's sweep on this host.
 """
 from __future__ import annotations
 
@@ -30,6 +37,19 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12
+METRIC = "solves/hour + sweep DOF-updates/s, FP32+refine vs FP64 GPU at 1e-6 rel L2"
+TRAFFIC_FILE = ROOT / "profiles" / "sweep_traffic.json"
+
+
+def committed_traffic(workload):
+    """HBM bytes per sweep launch measured by rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration) for this workload."""
+    try:
+        table = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    row = table.get("workloads", {}).get(workload)
+    return None if row is None else row.get("hbm_bytes_per_launch")
 
 
 def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitter=0.0, seed=0):
@@ -104,8 +124,13 @@ def main():
     ap.add_argument("--jitter", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--solve", action="store_true", help="also time the Krylov solves")
+    ap.add_argument("--no-solve", action="store_true", help="skip the Krylov solve leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy, max(args.steps, 1))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
@@ -126,24 +151,23 @@ def main():
     s.mom.set_dofs(rng.random((P, N)), 0.1 * rng.random((P, N)), 0.1 * rng.random((P, N)))
     for _ in range(args.warmup):
         s.ctx.sweep(s.mom, vdf)
+    s.ctx.timing(args.steps)
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
     t0 = time.perf_counter()
-    for a, b in ev:
-        a.record(stream)
+    for _ in range(args.steps):
         s.ctx.sweep(s.mom, vdf)
-        b.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = [a.elapsed_time(b) for a, b in ev]
+    ms = s.ctx.timing_read(args.steps)
+    s.ctx.timing(0)
     t_sweep = float(np.mean(ms)) * 1e-3
     dof = D * N * P
     s_lu = 4 if args.precision == "fp32" else 8
     bytes_alg = b_alg(D, N, P, s_lu, s_lu)
+    workload = (f"square {args.n}x{args.n} -> {N} P{args.deg} triangles, "
+                f"{D} ordinates ({args.npole}x{args.nazim}), {args.precision} sweep")
     out = {
-        "metric": "sweep DOF-updates/s",
+        "metric": METRIC,
         "value": dof * args.steps / wall,
         "unit": "DOF-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -151,25 +175,41 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic moments on the workload's mesh and ordinates",
-        "config": {"workload": f"square {args.n}x{args.n} -> {N} P{args.deg} triangles, "
-                               f"{D} ordinates ({args.npole}x{args.nazim}), "
-                               f"{args.precision} sweep",
-                   "n_tris": N, "ndof": P, "ndir": D, "dof_per_sweep": dof,
+        "config": {"workload": workload, "n_tris": N, "ndof": P, "ndir": D, "dof_per_sweep": dof,
                    "jitter": args.jitter},
         "roofline": {"bound": "hbm", "achieved": bytes_alg / t_sweep / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": bytes_alg / t_sweep / HBM_PEAK, "traffic": None},
-        "sweep_ms_event_mean": t_sweep * 1e3,
-        "sweep_ms_event_min": float(np.min(ms)),
+                     "unit": "GB/s", "frac": bytes_alg / t_sweep / HBM_PEAK,
+                     "traffic": committed_traffic(workload),
+                     "bytes_alg_per_launch": bytes_alg},
+        "sweep_kernel_ms_mean": t_sweep * 1e3,
+        "sweep_kernel_ms_min": float(np.min(ms)),
         "schedule": s.ctx.schedule_info(),
         "setup_s": setup_s,
     }
-    if args.solve:
+    if not args.no_solve:
         out["solve"] = solve_times(pkg, args, work)
     if not args.no_cpu:
         c64 = make_case(pkg, args.n, args.deg, args.npole, args.nazim, 1.0, 0.1, "fp64", work,
                         jitter=args.jitter)
         out["cpu_baseline"] = cpu_baseline(pkg, c64, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def calibrate_copy(nbytes, reps):
+    """A plain device-to-device copy of nbytes (reads nbytes, writes nbytes):
+    the reference point that scales rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = nbytes // 4
+    src = torch.rand(n, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"copy_bytes": 4 * n, "reps": reps, "GB_per_s": 2 * 4 * n * reps / el / 1e9}),
+          flush=True)
 
 
 def solve_times(pkg, args, work):
@@ -183,9 +223,13 @@ def solve_times(pkg, args, work):
         tts = time.perf_counter() - t0
         res[prec] = {"time_to_solution_s": tts, "setup_s": s.setup_seconds,
                      "iterations": rec.iterations, "converged": rec.converged,
-                     "solves_per_hour": 3600.0 / tts, "temp": rec.temp}
+                     "solves_per_hour": 3600.0 / tts, "temp": rec.temp,
+                     "q": np.concatenate([rec.qx, rec.qy])}
     t64, t32 = res["fp64"].pop("temp"), res["fp32"].pop("temp")
+    q64, q32 = res["fp64"].pop("q"), res["fp32"].pop("q")
     res["fp32"]["rel_l2_T_vs_fp64"] = float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))
+    res["fp32"]["rel_l2_q_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
+    res["speedup_fp32_refine_vs_fp64"] = res["fp64"]["time_to_solution_s"] / res["fp32"]["time_to_solution_s"]
     return res
 
 

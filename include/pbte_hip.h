@@ -98,6 +98,16 @@ const char *pbte_last_error(pbte_ctx *ctx);
  * sweep's values from global memory, LDS ring slots. */
 int pbte_schedule_info(pbte_ctx *ctx, int64_t *out6);
 
+/* Sweep-kernel timing with HIP events recorded on the context's stream
+ * around each sweep-kernel launch (source and wall-vector kernels
+ * excluded).  No counterpart in the reference: instrumentation for
+ * benchmarks.  pbte_timing(ctx, n) arms n event pairs (0 disarms) and
+ * clears earlier records; pbte_timing_read synchronises the stream and
+ * writes the durations (ms) of up to n recorded launches, returning how
+ * many were written. */
+int pbte_timing(pbte_ctx *ctx, int32_t capacity);
+int pbte_timing_read(pbte_ctx *ctx, float *ms, int32_t n);
+
 /* Factorise A_SOL for every (direction, element) in the given precision.
  * Replaces SweepContext.factorise / _kernels.build_operators
  * (sweep.py:116-129, _kernels.py:43-148).  Returns 0, or the number of
