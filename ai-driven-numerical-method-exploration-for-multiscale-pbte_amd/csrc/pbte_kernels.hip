@@ -1083,31 +1083,60 @@ __global__ void __launch_bounds__(256) residual_kernel(
     }
 }
 
-// sum partials per row in index order: out[r] = sum_k in[r][k]
-__global__ void rowsum_kernel(const double* in, int nrow, int ncol, int stride_k, int comps,
-                              double* out)
+// fixed-order block reduction: thread t sums elements t, t + 256, ... in
+// index order, then a pairwise tree over the 256 partials.  The order never
+// depends on timing, so every run gives the same bits.
+template <int NV>
+__device__ __forceinline__ void block_sum256(double (&v)[NV], double (*sh)[256])
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nrow * comps) return;
-    const int r = g / comps, c = g - (g / comps) * comps;
-    double s = 0.0;
-    for (int k = 0; k < ncol; ++k) s += in[((size_t)r * ncol + k) * stride_k + c];
-    out[g] = s;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) sh[j][t] = v[j];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+            for (int j = 0; j < NV; ++j) sh[j][t] = sh[j][t] + sh[j][t + w];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = sh[j][0];
 }
 
-// sequential index-order sums of the iterate residual (_kernels.py:340-361)
-__global__ void iterate_residual_kernel(const double* temp, const double* t_old, int n, double* out)
+// out[g] = sum over k of in[(r * ncol + k) * stride_k + c], g = r * comps + c:
+// one workgroup per output
+__global__ void __launch_bounds__(256) rowsum_kernel(const double* in, int nrow, int ncol, int stride_k, int comps,
+                                                     double* out)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double s1 = 0.0, s2 = 0.0, m = 0.0;
-    for (int i = 0; i < n; ++i) {
+    __shared__ double sh[1][256];
+    const int g = blockIdx.x;
+    const int r = g / comps, c = g - (g / comps) * comps;
+    double v[1] = {0.0};
+    for (int k = threadIdx.x; k < ncol; k += 256) v[0] += in[((size_t)r * ncol + k) * stride_k + c];
+    block_sum256<1>(v, sh);
+    if (threadIdx.x == 0) out[g] = v[0];
+}
+
+// residual and mass of an iterate (_kernels.py:340-361), one workgroup; the
+// sums run in the fixed block order above instead of the reference's
+// sequential index order
+__global__ void __launch_bounds__(256) iterate_residual_kernel(const double* temp, const double* t_old, int n,
+                                                               double* out)
+{
+    __shared__ double sh[3][256];
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += 256) {
         const double dlt = temp[i] - t_old[i];
-        s1 += dlt * dlt;
-        s2 += temp[i] * temp[i];
-        m += temp[i];
+        v[0] += dlt * dlt;
+        v[1] += temp[i] * temp[i];
+        v[2] += temp[i];
     }
-    out[0] = sqrt(s1 / s2);
-    out[1] = m;
+    block_sum256<3>(v, sh);
+    if (threadIdx.x == 0) {
+        out[0] = sqrt(v[0] / v[1]);
+        out[1] = v[2];
+    }
 }
 
 // ==========================================================================
@@ -2241,8 +2270,8 @@ extern "C" int pbte_transport_residual(pbte_ctx* ctx, const double* ts, const do
         switch (ctx->P) { RS(3) RS(6) RS(10) RS(15) }
 #undef RS
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(rowsum_kernel, dim3(nblocks((int64_t)ctx->D * 2, 256)), dim3(256), 0, ctx->stream,
-                           part, ctx->D, (int)gx, 2, 2, out);
+        hipLaunchKernelGGL(rowsum_kernel, dim3(ctx->D * 2), dim3(256), 0, ctx->stream, part, ctx->D, (int)gx, 2, 2,
+                           out);
         HIP_TRY(hipGetLastError());
         double* h = ctx->host_staging((size_t)2 * ctx->D);
         HIP_TRY(hipMemcpyAsync(h, out, sizeof(double) * 2 * ctx->D, hipMemcpyDeviceToHost, ctx->stream));
@@ -2259,7 +2288,7 @@ extern "C" int pbte_residual_iterate(pbte_ctx* ctx, const double* temp, const do
                                      double* mass_host)
 {
     return guarded(ctx, [&]() {
-        hipLaunchKernelGGL(iterate_residual_kernel, dim3(1), dim3(64), 0, ctx->stream, temp, t_old, ctx->N,
+        hipLaunchKernelGGL(iterate_residual_kernel, dim3(1), dim3(256), 0, ctx->stream, temp, t_old, ctx->N,
                            ctx->small);
         HIP_TRY(hipGetLastError());
         double* h = ctx->host_staging(2);
@@ -2282,8 +2311,7 @@ extern "C" int pbte_vec_multidot(pbte_ctx* ctx, const double* Q, int64_t ld, int
         hipLaunchKernelGGL(multidot_kernel, grid, dim3(256), 0, ctx->stream, Q, ld, m, w, n, ctx->red);
         HIP_TRY(hipGetLastError());
         double* out = ctx->red + (size_t)m * nbk;
-        hipLaunchKernelGGL(rowsum_kernel, dim3(nblocks(m, 256)), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1,
-                           out);
+        hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, out);
         HIP_TRY(hipGetLastError());
         double* h = ctx->host_staging((size_t)m);
         HIP_TRY(hipMemcpyAsync(h, out, sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));

@@ -25,6 +25,8 @@ which changes no arithmetic.  Monomial powers use ``numpy.power`` (libm
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 
 import numpy as np
@@ -124,7 +126,7 @@ class Integrals:
 
 def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
                     bc_codes=None, bc_xoff=None, bc_yoff=None,
-                    chunk: int = 16384) -> Integrals:
+                    chunk: int = 2048) -> Integrals:
     P = basis.ndof_tri
     Q = basis.ndof_fc
     deg = basis.deg
@@ -219,7 +221,8 @@ def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
     adj_tri, _ = mesh.adjacent_arrays()
     int_tri_fc_fc = np.zeros((P, Q, n_tris, 3))
     int_tri_tri_fc = np.zeros((n_tris, 3, P, P))
-    for j2 in range(3):
+
+    def face_slot(j2):
         fcid = mesh.tri_faces[:, j2].astype(np.int64)
         fbc = mesh.face_bc[fcid].astype(np.int64)
         per = np.zeros(n_tris, dtype=bool)
@@ -237,50 +240,69 @@ def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
             bcp = mesh.face_bc[pair].astype(np.int64)
             ox[per] = bc_xoff[bcp]
             oy[per] = bc_yoff[bcp]
-        for s0 in range(0, n_tris, chunk):
-            sl = slice(s0, min(n_tris, s0 + chunk))
-            f = fcid[sl]
-            ln = mesh.face_len[f]
-            absc, weig = face_rule(ln)
-            fx1 = nodes[mesh.face_nodes[f, 0], 0][:, None]
-            fy1 = nodes[mesh.face_nodes[f, 0], 1][:, None]
-            fx2 = nodes[mesh.face_nodes[f, 1], 0][:, None]
-            fy2 = nodes[mesh.face_nodes[f, 1], 1][:, None]
-            lnc = ln[:, None]
-            xp = (fx2 - fx1) / lnc * absc + fx1
-            yp = (fy2 - fy1) / lnc * absc + fy1
-            (bA, cA, dA), (fE, gE, hE) = _affine(x[sl], y[sl])
-            xi = bA[:, None] * xp + cA[:, None] * yp + dA[:, None]
-            eta = fE[:, None] * xp + gE[:, None] * yp + hE[:, None]
-            pm = _eval_mono(C, mono, xi, eta)                  # (c, q, P)
-            t = 2.0 * absc / lnc - 1.0
-            pl = np.zeros(t.shape + (Q,))
-            for k in range(Q):
-                pl += Cf[:, k] * _pow(t, k)[..., None]
-            acc = np.zeros((pm.shape[0], P, Q))
-            for q in range(np_fc):
-                acc += (pm[:, q, :, None] * pl[:, q, None, :]) * weig[:, q, None, None]
-            int_tri_fc_fc[:, :, sl, j2] = np.transpose(acc, (1, 2, 0))
+        return fcid, per, interior, other, ox, oy
 
-            # element x neighbour (or wall) tensor
-            two = (interior | per)[sl]
-            acc2 = np.zeros((pm.shape[0], P, P))
-            if two.any():
-                o = other[sl]
-                xo = nodes[mesh.tri_nodes[o], 0] + ox[sl][:, None]
-                yo = nodes[mesh.tri_nodes[o], 1] + oy[sl][:, None]
-                (bB, cB, dB), (fF, gF, hF) = _affine(xo, yo)
-                xi2 = bB[:, None] * xp + cB[:, None] * yp + dB[:, None]
-                eta2 = fF[:, None] * xp + gF[:, None] * yp + hF[:, None]
-                pn = _eval_mono(C, mono, xi2, eta2)
-                for q in range(np_fc):
-                    acc2 += (pm[:, q, :, None] * pn[:, q, None, :]) * weig[:, q, None, None]
-            if (~two).any():
-                accw = np.zeros((pm.shape[0], P))
-                for q in range(np_fc):
-                    accw += pm[:, q, :] * weig[:, q, None]
-                acc2 = np.where(two[:, None, None], acc2, accw[:, :, None])
-            int_tri_tri_fc[sl, j2] = acc2
+    slots = [face_slot(j2) for j2 in range(3)]
+
+    def face_chunk(j2, sl):
+        # element-wise arithmetic only: the result of a chunk does not
+        # depend on how the cells are split into chunks or threads
+        fcid, per, interior, other, ox, oy = slots[j2]
+        f = fcid[sl]
+        ln = mesh.face_len[f]
+        absc, weig = face_rule(ln)
+        fx1 = nodes[mesh.face_nodes[f, 0], 0][:, None]
+        fy1 = nodes[mesh.face_nodes[f, 0], 1][:, None]
+        fx2 = nodes[mesh.face_nodes[f, 1], 0][:, None]
+        fy2 = nodes[mesh.face_nodes[f, 1], 1][:, None]
+        lnc = ln[:, None]
+        xp = (fx2 - fx1) / lnc * absc + fx1
+        yp = (fy2 - fy1) / lnc * absc + fy1
+        (bA, cA, dA), (fE, gE, hE) = _affine(x[sl], y[sl])
+        xi = bA[:, None] * xp + cA[:, None] * yp + dA[:, None]
+        eta = fE[:, None] * xp + gE[:, None] * yp + hE[:, None]
+        pm = _eval_mono(C, mono, xi, eta)                  # (c, q, P)
+        t = 2.0 * absc / lnc - 1.0
+        pl = np.zeros(t.shape + (Q,))
+        for k in range(Q):
+            pl += Cf[:, k] * _pow(t, k)[..., None]
+        acc = np.zeros((pm.shape[0], P, Q))
+        for q in range(np_fc):
+            acc += (pm[:, q, :, None] * pl[:, q, None, :]) * weig[:, q, None, None]
+        int_tri_fc_fc[:, :, sl, j2] = np.transpose(acc, (1, 2, 0))
+
+        # element x neighbour (or wall) tensor
+        two = (interior | per)[sl]
+        acc2 = np.zeros((pm.shape[0], P, P))
+        if two.any():
+            o = other[sl]
+            xo = nodes[mesh.tri_nodes[o], 0] + ox[sl][:, None]
+            yo = nodes[mesh.tri_nodes[o], 1] + oy[sl][:, None]
+            (bB, cB, dB), (fF, gF, hF) = _affine(xo, yo)
+            xi2 = bB[:, None] * xp + cB[:, None] * yp + dB[:, None]
+            eta2 = fF[:, None] * xp + gF[:, None] * yp + hF[:, None]
+            pn = _eval_mono(C, mono, xi2, eta2)
+            for q in range(np_fc):
+                acc2 += (pm[:, q, :, None] * pn[:, q, None, :]) * weig[:, q, None, None]
+        if (~two).any():
+            accw = np.zeros((pm.shape[0], P))
+            for q in range(np_fc):
+                accw += pm[:, q, :] * weig[:, q, None]
+            acc2 = np.where(two[:, None, None], acc2, accw[:, :, None])
+        int_tri_tri_fc[sl, j2] = acc2
+
+    # numpy releases the GIL inside its array loops: chunks of cells run on
+    # a thread pool, each writing its own slice
+    tasks = [(j2, slice(s0, min(n_tris, s0 + chunk))) for j2 in range(3)
+             for s0 in range(0, n_tris, chunk)]
+    workers = max(1, min(len(tasks), os.cpu_count() or 1, 16))
+    if workers == 1:
+        for j2, sl in tasks:
+            face_chunk(j2, sl)
+    else:
+        with ThreadPoolExecutor(max_workers=workers) as pool:
+            for fut in [pool.submit(face_chunk, j2, sl) for j2, sl in tasks]:
+                fut.result()
 
     qax, qay, qw = tri_quadrature(np_tri)
     nodfun_qua_p = _eval_mono(C, mono, qax, qay).T

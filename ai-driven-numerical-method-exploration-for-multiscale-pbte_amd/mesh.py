@@ -182,39 +182,52 @@ def build_mesh(path, bc_phyid, bc_codes, bc_names=None, bc_xoff=None, bc_yoff=No
     bc_xoff = np.asarray(bc_xoff if bc_xoff is not None else np.zeros(nbc), dtype=float)
     bc_yoff = np.asarray(bc_yoff if bc_yoff is not None else np.zeros(nbc), dtype=float)
 
-    face_nodes: list = []
-    face_bc: list = []
-    key_to_face: dict = {}
-    tri_nodes: list = []
-    for et, phy, nds in zip(g.elem_type, g.elem_phyid, g.elem_nodes):
-        if et == LINE_ELEMENT:
-            a, b = nds
-            bc = -1
-            for j in range(nbc):
-                if phy == bc_phyid[j]:
-                    bc = j
-            key = (a, b) if a < b else (b, a)
-            if key in key_to_face:
-                raise ValueError(f"{path}: duplicate boundary edge {(a + 1, b + 1)}")
-            key_to_face[key] = len(face_nodes)
-            face_nodes.append((a, b))
-            face_bc.append(bc)
-        else:
-            tri_nodes.append(nds)
-            for j in range(3):
-                a, b = nds[j], nds[(j + 1) % 3]
-                key = (a, b) if a < b else (b, a)
-                if key not in key_to_face:
-                    key_to_face[key] = len(face_nodes)
-                    face_nodes.append((a, b))
-                    face_bc.append(-1)
-    n_tris = len(tri_nodes)
-    if n_tris == 0:
+    # faces in order of first appearance: boundary lines as read, then the
+    # edges (j, j+1) of every triangle; a face keeps the orientation of its
+    # first appearance
+    et = np.asarray(g.elem_type)
+    is_line = et == LINE_ELEMENT
+    if np.count_nonzero(~is_line) == 0:
         raise ValueError(f"{path}: no triangles")
-    n_faces = len(face_nodes)
-    tri_nodes_a = np.asarray(tri_nodes, dtype=np.int32)
-    face_nodes_a = np.asarray(face_nodes, dtype=np.int32)
-    face_bc_a = np.asarray(face_bc, dtype=np.int32)
+    line_idx = np.flatnonzero(is_line)
+    tri_idx = np.flatnonzero(~is_line)
+    tri_nodes_a = np.asarray([g.elem_nodes[e] for e in tri_idx], dtype=np.int64).reshape(-1, 3)
+    lines_a = (np.asarray([g.elem_nodes[e] for e in line_idx], dtype=np.int64).reshape(-1, 2)
+               if line_idx.size else np.zeros((0, 2), dtype=np.int64))
+    # occurrence stream in element order
+    pos_line = line_idx
+    pos_tri = tri_idx
+    ea = np.concatenate([lines_a[:, 0], tri_nodes_a[:, [0, 1, 2]].reshape(-1)])
+    eb = np.concatenate([lines_a[:, 1], tri_nodes_a[:, [1, 2, 0]].reshape(-1)])
+    order_key = np.concatenate([pos_line * 4, (pos_tri[:, None] * 4 + np.arange(3)[None, :]).reshape(-1)])
+    srt = np.argsort(order_key, kind="stable")
+    ea, eb = ea[srt], eb[srt]
+    is_l = np.concatenate([np.ones(len(lines_a), bool), np.zeros(3 * len(tri_nodes_a), bool)])[srt]
+    lo = np.minimum(ea, eb)
+    hi = np.maximum(ea, eb)
+    nmax = int(max(int(hi.max()) + 1, 1))
+    key = lo * nmax + hi
+    ukey, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    # a boundary line must be the first appearance of its edge
+    seen = first[inv] != np.arange(key.size)
+    if np.any(seen & is_l):
+        o = int(np.flatnonzero(seen & is_l)[0])
+        raise ValueError(f"{path}: duplicate boundary edge {(int(ea[o]) + 1, int(eb[o]) + 1)}")
+    rank = np.empty(ukey.size, dtype=np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(ukey.size)
+    face_of = rank[inv]                                  # face id of every occurrence
+    n_faces = int(ukey.size)
+    fo = np.sort(first)
+    face_nodes_a = np.stack([ea[fo], eb[fo]], axis=1).astype(np.int32)
+    face_bc_a = np.full(n_faces, -1, dtype=np.int32)
+    if line_idx.size:
+        phy = np.asarray(g.elem_phyid)[line_idx]
+        bcl = np.full(phy.size, -1, dtype=np.int32)
+        for j in range(nbc):
+            bcl = np.where(phy == bc_phyid[j], j, bcl)
+        face_bc_a[face_of[is_l]] = bcl                   # lines keep their file order
+    n_tris = len(tri_nodes_a)
+    tri_nodes_a = tri_nodes_a.astype(np.int32)
     face_pair = np.full(n_faces, -1, dtype=np.int32)
 
     # periodic pairing between the boundaries named 'Master' and 'Slave'
@@ -248,20 +261,16 @@ def build_mesh(path, bc_phyid, bc_codes, bc_names=None, bc_xoff=None, bc_yoff=No
     d = fn[:, 0, :] - fn[:, 1, :]
     face_len = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1])
 
-    tri_faces = np.full((n_tris, 3), -1, dtype=np.int32)
+    tri_faces = face_of[~is_l].reshape(n_tris, 3).astype(np.int32)
     face_tri = np.full((n_faces, 2), -1, dtype=np.int32)
     face_lfc = np.full((n_faces, 2), -1, dtype=np.int32)
-    for i, nds in enumerate(tri_nodes):
-        for j in range(3):
-            a, b = nds[j], nds[(j + 1) % 3]
-            k = key_to_face[(a, b) if a < b else (b, a)]
-            tri_faces[i, j] = k
-            if face_nodes_a[k, 0] == a and face_nodes_a[k, 1] == b:
-                face_tri[k, 0] = i
-                face_lfc[k, 0] = j
-            else:
-                face_tri[k, 1] = i
-                face_lfc[k, 1] = j
+    ta = tri_nodes_a[:, [0, 1, 2]]
+    same = face_nodes_a[tri_faces, 0] == ta                  # first-appearance orientation
+    tri_id = np.repeat(np.arange(n_tris, dtype=np.int32)[:, None], 3, axis=1)
+    lfc_id = np.repeat(np.arange(3, dtype=np.int32)[None, :], n_tris, axis=0)
+    for slot, m in ((0, same), (1, ~same)):
+        face_tri[tri_faces[m], slot] = tri_id[m]
+        face_lfc[tri_faces[m], slot] = lfc_id[m]
 
     # wall / symmetry queue index overwrites face_pair (topology.py:238-243)
     counter = 0

@@ -123,9 +123,13 @@ class SweepContext:
         self.close()
 
     def close(self):
-        if getattr(self, "handle", None):
-            _native.lib().pbte_ctx_destroy(self.handle)
-            self.handle = None
+        h = getattr(self, "handle", None)
+        self.handle = None
+        if h:
+            try:
+                _native.lib().pbte_ctx_destroy(h)
+            except Exception:  # interpreter shutdown: modules already torn down
+                pass
 
     def schedule_info(self) -> dict:
         out = (C.c_int64 * 6)()
