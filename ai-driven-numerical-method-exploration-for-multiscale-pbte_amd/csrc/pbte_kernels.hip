@@ -25,7 +25,9 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <functional>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <thread>
 #include <vector>
@@ -65,12 +67,25 @@ template <int P> struct Lay {
     static constexpr int SRC = pad4(3 * P);  // source values per cell
     static constexpr int KQ = pad4(Q * Q);   // upwind face block per (cell, face)
     static constexpr int LU = pad4(P * P);   // factors per (slot, direction)
-    // cell record of the pipelined sweep, one contiguous row per cell:
-    // source (U, VX, VY), normals, face words, then per face the Q x Q upwind
-    // block (neighbour faces) or the wall vector (wall faces, first P values)
-    static constexpr int R_N = 3 * P, R_INFO = 3 * P + 6, R_FD = 3 * P + 9;
-    static constexpr int CR = pad4(3 * P + 9 + 3 * Q * Q);
+    // cell record of the sweep, one row per cell: the source (U, VX, VY),
+    // the outward normals of faces 0-2 scaled by the face length, and the
+    // face info words.  Every interior, periodic and wall face block of a
+    // straight-sided element is its length times one reversed edge mass
+    // matrix, so no per-face block is stored.
+    static constexpr int R_N = 3 * P, R_INFO = 3 * P + 6;
+    static constexpr int CR = pad4(3 * P + 9);
 };
+
+// row stride of the cell records: a multiple of 128 bytes, so a record
+// spans the fewest cache lines (one for P <= 6 in binary32)
+template <int P, typename T> constexpr int rec_stride()
+{
+    return (int)(((size_t)Lay<P>::CR * sizeof(T) + 127) / 128 * 128 / sizeof(T));
+}
+__host__ inline int rec_stride_rt(int P, size_t sz)
+{
+    return (int)(((size_t)pad4(3 * P + 9) * sz + 127) / 128 * 128 / sz);
+}
 
 // LU entry k of item `item` (slot * B + b) of a bundle with NB items, laid
 // out [k / V][item][V] with V = 16 / sizeof(T): lanes holding consecutive
@@ -123,14 +138,22 @@ struct pbte_ctx {
     // full FP64 tensors (factorisation, residual)
     double *mass = nullptr, *gx = nullptr, *gy = nullptr, *fcm = nullptr, *int_tri = nullptr;
     double* nrm = nullptr;     // [N][3][2] outward normals
-    // compact sweep data
+    // element data of the factorisation and the transport residual
     CellGeo<double>* geo64 = nullptr;
-    CellGeo<float>* geo32 = nullptr;
-    double* K64 = nullptr;
-    float* K32 = nullptr;
+    double* K64 = nullptr;      // [N][3][KQ] face blocks on the shared face's nodes
     double* bvec64 = nullptr;   // [F][P] tw * ttfc[:, 0] on thermal walls
-    float* bvec32 = nullptr;
     int n_bc_th = 0;
+    // sweep data: the reversed unit-edge mass matrix Kr, the element across
+    // each face, the wall vectors
+    double kref64[25] = {};
+    float kref32[25] = {};
+    int32_t* other = nullptr;   // [N][3]
+    int n_wrow = 0;             // wall rows (thermal and adiabatic faces)
+    double* wtr64 = nullptr;    // [n_wrow][pad4(P)] wall vectors over the face length
+    float* wtr32 = nullptr;
+    int n_adw = 0;              // adiabatic wall rows, refreshed every sweep
+    int32_t *adw_row = nullptr, *adw_face = nullptr;
+    double* adw_rlen = nullptr;
     // adiabatic walls
     int n_wall = 0;
     int32_t *wall_face = nullptr, *wall_tri = nullptr, *wall_lfc = nullptr;
@@ -144,15 +167,19 @@ struct pbte_ctx {
             *bundle_dirs = nullptr, *slot_cell = nullptr, *lev_off = nullptr, *lev_ptr = nullptr;
     int64_t *lu_off = nullptr, *item_off = nullptr;
     int4* irec = nullptr;       // per item (bundle, slot, direction): {cell, face words}
-    int32_t* iwst = nullptr;    // per item: stash slot the item also writes, or -1
-    double* crec64 = nullptr;   // cell records of the pipelined sweep
+    int32_t* iwst = nullptr;    // per item: LDS pool slot the item writes, or -1
+    int4* wcell = nullptr;      // per bundle: inflow wall elements {cell, wall row of faces 0-2}
+    int32_t* wc_off = nullptr;
+    int max_wall_in = 0;        // most inflow wall elements of one bundle
+    double* crec64 = nullptr;   // cell records of the sweep
     float* crec32 = nullptr;
+    int rs64 = 0, rs32 = 0;     // their row strides
     uint8_t* bundle_global = nullptr;
-    int ring_slots = 0, stash_slots = 0;
+    int pool_slots = 0;         // LDS trace slots per workgroup
+    int64_t pool_peak = 0;      // most slots one bundle holds at once
     int64_t global_bundles = 0;
     int variant = 0;            // 0 pipelined level sweep, 1 plain level sweep
     int max_bundle_dirs = 1;
-    int64_t ring_fallback = 0;  // upwind reads beyond the default LDS ring
     int64_t old_reads = 0;      // previous-sweep reads (broken cycles)
     int64_t periodic_reads = 0; // upwind reads across periodic faces
     std::vector<int64_t> h_lu_off;
@@ -162,8 +189,7 @@ struct pbte_ctx {
     float* lu32 = nullptr;
     bool have64 = false, have32 = false;
     // scratch
-    double* src64 = nullptr;
-    float* src32 = nullptr;
+    double* src64 = nullptr;   // source of the transport residual
     double* ssraw = nullptr;
     double* red = nullptr;     // reduction partials
     size_t red_len = 0;
@@ -384,54 +410,126 @@ __global__ void __launch_bounds__(256) source_kernel(const double* __restrict__ 
 // the triangular solve of 78-101).  One workgroup per direction bundle;
 // thread t owns direction t % B of the bundle for the whole sweep and slot
 // t / B of every level; a barrier separates levels.
+//
+// Per (cell, direction) item:
+//   x = U + cx VX + cy VY + sum over inflow wall faces of w v    the source
+//       and wall inflow (_kernels.py:153-167, 202-215)
+//   for each face il, in face order, with w = 0.5 (s - |s|), s = c . n L:
+//       x[node(il, r)] -= w (Kr u)[r]            the upwind inflow (_kernels.py:183-201)
+//   x = inv(A) x                                 (_kernels.py:78-101)
+// Kr is the unit-edge mass matrix with its columns reversed (the element
+// across numbers the shared edge's nodes the other way round), L the face
+// length, u the upwind element's values at its face nodes (zero on faces
+// without upwind inflow) and v a wall's P-vector over L: -tw ttfc[:, 0] for
+// a thermal wall, flux_wall for an adiabatic one.  L Kr is the reference's
+// face block int_tri_tri_fc to rounding (checked when the context is
+// created).
 // ==========================================================================
 template <typename T> struct SweepArgs {
-    const int32_t *block_bundle, *bundle_ndir, *bundle_dir_off, *bundle_dirs, *slot_cell,
-        *lev_off, *lev_ptr;
+    const int32_t *block_bundle, *bundle_ndir, *bundle_dir_off, *bundle_dirs, *lev_off, *lev_ptr;
     const int64_t *lu_off, *item_off;
     const T *lu;
     const T *cxv, *cyv;
-    const CellGeo<T>* geo;
-    const T* K;
-    const T* bvec;
-    const T* src;
-    const double* flux_wall;
+    const T* crec;              // cell records, row stride rs
+    const int32_t* other;       // [N][3] element across each face (plain kernel)
+    const T* wtr;               // wall vectors over L, [wall row][PV]
+    const int4* wcell;          // per bundle: its inflow wall elements {cell, row of faces 0-2}
+    const int32_t* wc_off;
     const T* pbuf;
     T* vdf;
-    int N, F, use_pbuf, homogeneous;
-    // pipelined variant: per-item records, cell records, LDS ring and stash
-    const int4* irec;
-    const int32_t* iwst;
-    const T* crec;
+    const int4* irec;           // per item (bundle, slot, direction): {cell, face words}
+    const int32_t* iwst;        // per item: the LDS pool slot it writes, or -1
     T* spill_row;               // write target of idle threads (P values)
-    int lane_zero;              // 0 at run time; makes a uniform load per-lane
     const uint8_t* bundle_global;
-    int ring, stash;            // LDS ring and stash slots
+    int N, F, rs, use_pbuf, homogeneous;
+    int lane_zero;              // 0 at run time; makes a uniform load per-lane
+    int pool;                   // LDS pool slots
+    int wbase;                  // LDS offset (in values) of the wall inflow vectors
+    T kref[25];                 // Kr, Q x Q, row major
 };
 
-// Face word of a per-slot record: code in bits 28-31, value in bits 0-27.
-// What an inflow face reads during the pipelined sweep:
-//   UP_RING / UP_STASH    upwind value of this sweep from the LDS ring (by
-//                         slot) or the LDS stash (by stash slot)
-//   OLD                   previous-sweep value of an element the reference
+// Face word of an item record: code in bits 28-31, the local face of the
+// element across in bits 26-27, a value in bits 0-25.  What an inflow face
+// reads:
+//   UP_LDS / PER_LDS      this sweep's trace of the upwind element (periodic
+//                         partner) from its LDS pool slot (value)
+//   ADIABATIC / THERMAL   a wall: the element's summed wall inflow (value:
+//                         the element's index in the bundle's wall list)
+//   OLD / PER_OLD         previous-sweep values of an element the reference
 //                         order visits later (broken cycle), global memory
-//   PER_RING/STASH/OLD    same for a periodic partner; with an explicit
-//                         periodic buffer the face reads that instead
-//   UP_GLOBAL/PER_GLOBAL  this sweep's value from global memory: only in a
-//                         bundle whose far reads overflow the stash, which
-//                         then runs with full memory barriers
+//                         (value: the element)
+//   UP_GLOBAL/PER_GLOBAL  this sweep's values from global memory (value: the
+//                         element), only in a bundle whose LDS pool would
+//                         overflow, which then runs with full barriers
+// With an explicit periodic buffer every periodic face reads that instead.
 enum {
-    REC_UP_RING = 0, REC_UP_STASH = 1, REC_ADIABATIC = 2, REC_THERMAL = 3, REC_OLD = 4,
-    REC_PER_RING = 5, REC_PER_STASH = 6, REC_PER_OLD = 7, REC_UP_GLOBAL = 8, REC_PER_GLOBAL = 9,
-    REC_NONE = 15
+    REC_UP_LDS = 0, REC_ADIABATIC = 2, REC_THERMAL = 3, REC_OLD = 4, REC_PER_LDS = 5, REC_PER_OLD = 7,
+    REC_UP_GLOBAL = 8, REC_PER_GLOBAL = 9, REC_NONE = 15
 };
-__host__ __device__ constexpr bool rec_reads_neighbour(int code)
+constexpr int kFaceValBits = 26;
+constexpr int kFaceValMask = (1 << kFaceValBits) - 1;
+__host__ __device__ constexpr int face_word(int code, int ilp, int val)
 {
-    return code == REC_UP_RING || code == REC_UP_STASH || (code >= REC_OLD && code <= REC_PER_GLOBAL);
+    return (int)(((unsigned)code << 28) | ((unsigned)ilp << kFaceValBits) | (unsigned)val);
+}
+__host__ __device__ constexpr int fw_code(int w) { return (int)((unsigned)w >> 28); }
+__host__ __device__ constexpr int fw_ilp(int w) { return (w >> kFaceValBits) & 3; }
+__host__ __device__ constexpr int fw_val(int w) { return w & kFaceValMask; }
+
+// record-code classes as bit masks over the code: one shift and one test,
+// no short-circuit control flow
+__host__ __device__ constexpr int code_bit(int c) { return 1 << c; }
+constexpr int kRecWall = code_bit(REC_ADIABATIC) | code_bit(REC_THERMAL);
+constexpr int kRecNeighbour = code_bit(REC_UP_LDS) | code_bit(REC_OLD) | code_bit(REC_PER_LDS) |
+                              code_bit(REC_PER_OLD) | code_bit(REC_UP_GLOBAL) | code_bit(REC_PER_GLOBAL);
+constexpr int kRecPeriodic = code_bit(REC_PER_LDS) | code_bit(REC_PER_OLD) | code_bit(REC_PER_GLOBAL);
+__device__ __forceinline__ bool code_in(int mask, int code) { return ((mask >> code) & 1) != 0; }
+
+// value m of the direction's source plus the wall inflow v
+template <int P, typename T>
+__device__ __forceinline__ T source_plus(const T* cr, T cx, T cy, int m, T v)
+{
+    return cr[m] + cx * cr[P + m] + cy * cr[2 * P + m] + v;
 }
 
-// the face coupling of one upwind face: b[node(il, r)] -= w (K u)[r], the
-// product K u accumulated first (the order every sweep kernel uses)
+// LDS trace layout: per (slot, direction of the bundle) the element's
+// values on its three faces, face-major, each face padded to TQ values
+template <int P> struct Tr {
+    static constexpr int Q = Deg<P>::v + 1;
+    static constexpr int TQ = Q == 2 ? 2 : pad4(Q);     // padded trace of one face
+    // values per (slot, direction), a multiple of 4 so that every slot and
+    // the wall vectors (read from the zero slot) stay 16-byte aligned
+    static constexpr int W = pad4(3 * TQ);
+};
+__host__ inline int trace_pad(int Q) { return Q == 2 ? 2 : pad4(Q); }
+
+// wall entries of the bundle's wall list: the wall row of each face, -1
+// when the face is no inflow wall; bit 30 marks a thermal wall (it emits
+// nothing in a homogeneous sweep)
+constexpr int kWallThermal = 1 << 30;
+
+// the summed wall inflow of element i for one direction, value m:
+// sum over its inflow wall faces, in face order, of w v[m]
+template <int P, typename T>
+__device__ __forceinline__ T wall_inflow(const T* cr, int4 e, T cx, T cy, const T* wtr, T thermal_on, int m)
+{
+    using L = Lay<P>;
+    constexpr int PV = pad4(P);
+    const int row[3] = {e.y, e.z, e.w};
+    T v = T(0);
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        if (row[il] < 0) continue;
+        const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
+        const T w = T(0.5) * (sp - fabs(sp));
+        const T on = (row[il] & kWallThermal) ? thermal_on : T(1);
+        v += w * (wtr[(size_t)(row[il] & (kWallThermal - 1)) * PV + m] * on);
+    }
+    return v;
+}
+
+// the face coupling of one inflow face: x[node(il, r)] -= w (Kr u)[r], the
+// product Kr u accumulated first
 template <int P, typename T>
 __device__ __forceinline__ void face_update(T (&x)[P], int il, T w, const T* K, const T* u)
 {
@@ -462,11 +560,19 @@ __device__ __forceinline__ void apply_inverse(T (&x)[P], E&& e)
     for (int r = 0; r < P; ++r) x[r] = y[r];
 }
 
+__device__ __forceinline__ int bits_of(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int bits_of(double v) { return (int)(uint32_t)(uint64_t)__double_as_longlong(v); }
+
+// the plain level kernel: every value from global memory, one level at a
+// time.  Same per-item arithmetic as the pipelined kernel below (the two
+// agree bit for bit); kept as the reference schedule for tests.
 template <int P, typename T>
 __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
 {
+    using L = Lay<P>;
     constexpr int DEG = Deg<P>::v;
     constexpr int Q = DEG + 1;
+    constexpr int TQ = Tr<P>::TQ;
     const int bundle = a.block_bundle[blockIdx.x];
     const int B = a.bundle_ndir[bundle];
     const int t = threadIdx.x;
@@ -475,56 +581,51 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
     const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
     const T cx = a.cxv[d], cy = a.cyv[d];
     const int N = a.N;
-    const int32_t* cells = a.slot_cell + (size_t)bundle * N;
+    const int4* irec = a.irec + a.item_off[bundle];
+    const int4* wc = a.wcell + a.wc_off[bundle];
     const T* lu = a.lu + a.lu_off[bundle];
     T* vdf_d = a.vdf + (size_t)d * N * P;
+    const T thermal_on = a.homogeneous ? T(0) : T(1);
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
 
     for (int lv = l0; lv < l1; ++lv) {
         const int s = a.lev_ptr[lv] + c;
         if (s < a.lev_ptr[lv + 1]) {
-            const int i = cells[s];
+            const int4 r = irec[(size_t)s * B + b];
+            const int i = r.x;
+            const int fw[3] = {r.y, r.z, r.w};
+            const T* cr = a.crec + (size_t)i * a.rs;
+            int wk = -1;
+#pragma unroll
+            for (int il = 0; il < 3; ++il)
+                if (code_in(kRecWall, fw_code(fw[il]))) wk = fw_val(fw[il]);
             T x[P];
-            const T* sv = a.src + (size_t)i * Lay<P>::SRC;
+            if (wk >= 0) {
+                const int4 e = wc[wk];
 #pragma unroll
-            for (int m = 0; m < P; ++m) x[m] = sv[m] + cx * sv[P + m] + cy * sv[2 * P + m];
-            const CellGeo<T> g = a.geo[i];
-            // wall inflow first (face order), then the upwind neighbours (face
-            // order); the pipelined kernel applies the same sequence
+                for (int m = 0; m < P; ++m)
+                    x[m] = source_plus<P, T>(cr, cx, cy, m, wall_inflow<P, T>(cr, e, cx, cy, a.wtr, thermal_on, m));
+            } else {
 #pragma unroll
-            for (int il = 0; il < 3; ++il) {
-                const T sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
-                const T w = T(0.5) * (sp - fabs(sp));
-                if (w == T(0)) continue;
-                const int code = g.info[il] & 3;
-                if (code == FACE_ADIABATIC) {
-                    const double* fw = a.flux_wall + (size_t)g.other[il] * P;
-#pragma unroll
-                    for (int m = 0; m < P; ++m) x[m] += w * (T)fw[m];
-                } else if (code == FACE_THERMAL && !a.homogeneous) {
-                    const T* bv = a.bvec + (size_t)g.other[il] * pad4(P);
-                    const T mw = -w;
-#pragma unroll
-                    for (int m = 0; m < P; ++m) x[m] += mw * bv[m];
-                }
+                for (int m = 0; m < P; ++m) x[m] = source_plus<P, T>(cr, cx, cy, m, T(0));
             }
 #pragma unroll
             for (int il = 0; il < 3; ++il) {
-                const T sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
+                const int code = fw_code(fw[il]);
+                if (!code_in(kRecNeighbour, code)) continue;
+                const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
                 const T w = T(0.5) * (sp - fabs(sp));
-                if (w == T(0)) continue;
-                const int info = g.info[il];
-                const int code = info & 3;
-                if (code <= FACE_PERIODIC) {
-                    const int ilp = (info >> 2) & 3;
-                    const T* fe = (code == FACE_PERIODIC && a.use_pbuf)
-                                      ? a.pbuf + ((size_t)d * a.F + (info >> 4)) * P
-                                      : vdf_d + (size_t)g.other[il] * P;
-                    T fb[Q];
+                T u[TQ];
+                {
+                    const bool per = code == REC_PER_LDS || code == REC_PER_OLD || code == REC_PER_GLOBAL;
+                    const T* fe = (per && a.use_pbuf)
+                                      ? a.pbuf + ((size_t)d * a.F + (bits_of(cr[L::R_INFO + il]) >> 4)) * P
+                                      : vdf_d + (size_t)a.other[(size_t)i * 3 + il] * P;
+                    const int ilp = fw_ilp(fw[il]);
 #pragma unroll
-                    for (int q = 0; q < Q; ++q) fb[q] = fe[face_node(DEG, ilp, q)];
-                    face_update<P, T>(x, il, w, a.K + ((size_t)i * 3 + il) * Lay<P>::KQ, fb);
+                    for (int q = 0; q < TQ; ++q) u[q] = q < Q ? fe[face_node(DEG, ilp, q < Q ? q : 0)] : T(0);
                 }
+                face_update<P, T>(x, il, w, a.kref, u);
             }
             const size_t item = (size_t)s * B + b, NB = (size_t)N * B;
             apply_inverse<P, T>(x, [&](int k) { return lu[lu_idx<T>(k, item, NB)]; });
@@ -539,11 +640,18 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
 
 // --------------------------------------------------------------------------
 // software-pipelined level sweep.  Same per-item arithmetic as sweep_kernel
-// (the results are bitwise identical); what changes is when data arrives:
-//   * records {cell, face words} of level lv+2 are loaded while level lv
-//     is solved, the cell data, LU and pivots of level lv+1 likewise;
-//   * upwind traces written during the last `ring` slots are read from an
-//     LDS ring instead of global memory (global fallback beyond it).
+// (the values agree bit for bit); the instruction stream per level is kept
+// short and free of divergent branches on the common path:
+//   * records {cell, face words} of level lv+2 and the cell record and
+//     inverse of level lv+1 are loaded while level lv is solved (two named
+//     register sets, no copies);
+//   * an element with downwind readers writes its values on its three
+//     faces, face-major, into an LDS pool slot held until its last reader's
+//     level has passed; an inflow face is one vector LDS read;
+//   * the bundle's inflow wall traces sit in LDS from the start; a face
+//     without inflow reads an all-zero slot, so every face runs the same
+//     code; only the rare global reads (previous-sweep value of a broken
+//     cycle, explicit periodic buffer, overflow bundles) branch.
 // --------------------------------------------------------------------------
 template <typename T> struct Vec16;
 template <> struct Vec16<float> { using type = float4; };
@@ -565,27 +673,6 @@ __device__ __forceinline__ void load16(T* dst, const T* src)
         for (int j = 0; j < w; ++j) dst[k * w + j] = e[j];
     }
 }
-
-// --------------------------------------------------------------------------
-// software-pipelined level sweep.  Same per-item arithmetic as sweep_kernel
-// (the values agree bit for bit); the instruction stream per level is kept
-// short and free of divergent branches on the common path:
-//   * records {cell, face words} of level lv+2 and the cell data, face
-//     blocks, factors and permutation of level lv+1 are loaded while level
-//     lv is solved (two named register sets, no copies);
-//   * an element writes its values on each of its three faces, face-major,
-//     into an LDS ring slot (and a stash slot when a far reader needs it);
-//     an upwind face is one vector LDS read of the neighbour's trace;
-//   * every face runs the same code: a face without an upwind read takes
-//     weight 0 and reads an all-zero LDS slot, a face without wall inflow
-//     takes wall weight 0; only the rare global reads (previous-sweep value
-//     of a broken cycle, explicit periodic buffer, overflow bundles) branch.
-// --------------------------------------------------------------------------
-template <int P> struct Tr {
-    static constexpr int Q = Deg<P>::v + 1;
-    static constexpr int TQ = Q == 2 ? 2 : pad4(Q);     // padded trace of one face
-    static constexpr int W = 3 * TQ;                      // values per (slot, direction)
-};
 
 template <typename T, int n>
 __device__ __forceinline__ void lds_load(T* dst, const T* src)
@@ -635,8 +722,9 @@ template <int P, typename T, bool PFLU> struct PipeData {
     using L = Lay<P>;
     bool valid;
     int cell, slot, fw[3];
-    int lds[3];         // LDS element offset of the upwind trace slot of each face (zero slot when none)
-    int wst;            // stash slot this item also writes, or -1
+    int lds[3];         // LDS offset of each face's upwind trace (the zero slot when none)
+    int wv;             // LDS offset of the element's wall inflow (the zero slot when none)
+    int wst;            // LDS pool slot this item writes, or -1
     T cr[L::CR];        // the cell record
     T lu[PFLU ? L::LU : 4];
 };
@@ -655,27 +743,11 @@ __device__ __forceinline__ void pipe_record(const int4* irec, int2 span, int c, 
     o.r = irec[(size_t)o.slot * B + b];
 }
 
-// record-code classes as bit masks over the code: one shift and one test,
-// no short-circuit control flow
-__host__ __device__ constexpr int code_bit(int c) { return 1 << c; }
-constexpr int kRecRing = code_bit(REC_UP_RING) | code_bit(REC_PER_RING);
-constexpr int kRecWall = code_bit(REC_ADIABATIC) | code_bit(REC_THERMAL);
-constexpr int kRecNeighbour = code_bit(REC_UP_RING) | code_bit(REC_UP_STASH) | code_bit(REC_OLD) |
-                              code_bit(REC_PER_RING) | code_bit(REC_PER_STASH) | code_bit(REC_PER_OLD) |
-                              code_bit(REC_UP_GLOBAL) | code_bit(REC_PER_GLOBAL);
-constexpr int kRecPeriodic = code_bit(REC_PER_RING) | code_bit(REC_PER_STASH) | code_bit(REC_PER_OLD) |
-                             code_bit(REC_PER_GLOBAL);
-__device__ __forceinline__ bool code_in(int mask, int code) { return ((mask >> code) & 1) != 0; }
-// faces whose upwind trace comes from LDS (ring or stash)
+// faces whose inflow trace comes from the LDS pool
 __device__ __forceinline__ int rec_lds_mask(int use_pbuf)
 {
-    const int up = code_bit(REC_UP_RING) | code_bit(REC_UP_STASH);
-    const int per = code_bit(REC_PER_RING) | code_bit(REC_PER_STASH);
-    return use_pbuf ? up : (up | per);
+    return use_pbuf ? code_bit(REC_UP_LDS) : (code_bit(REC_UP_LDS) | code_bit(REC_PER_LDS));
 }
-
-__device__ __forceinline__ int bits_of(float v) { return __float_as_int(v); }
-__device__ __forceinline__ int bits_of(double v) { return (int)(uint32_t)(uint64_t)__double_as_longlong(v); }
 
 // gather: only loads whose addresses come from the record of a level
 // earlier; every use of the loaded data waits for the solve stage (the
@@ -687,6 +759,7 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int32_t
 {
     using L = Lay<P>;
     constexpr int W = Tr<P>::W;
+    constexpr int TQ = Tr<P>::TQ;
     o.valid = r.valid;
     const int i = r.r.x;
     o.cell = i;
@@ -694,21 +767,26 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int32_t
     o.fw[0] = r.r.y;
     o.fw[1] = r.r.z;
     o.fw[2] = r.r.w;
-    const int zero = a.ring + a.stash;
+    constexpr int PV = pad4(P);
+    const unsigned zero = ((unsigned)a.pool * B + b) * W;
     const int lds_mask = rec_lds_mask(a.use_pbuf);
+    unsigned wv = zero;
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
-        // the value of an LDS face is its LDS slot; every other face reads
-        // the all-zero slot (a select, no control flow)
-        const int code = (int)(((unsigned)o.fw[il]) >> 28);
-        const int val = o.fw[il] & 0x0fffffff;
-        const int sl = code_in(lds_mask, code) ? val : zero;
-        o.lds[il] = (sl * B + b) * W;
+        // a pool face reads its upwind slot, every other face the all-zero
+        // slot; a wall face names the element's wall inflow (selects, no
+        // control flow)
+        const int code = fw_code(o.fw[il]);
+        const unsigned val = (unsigned)fw_val(o.fw[il]);
+        const unsigned nbr = (val * B + b) * W + (unsigned)fw_ilp(o.fw[il]) * TQ;
+        o.lds[il] = (int)(code_in(lds_mask, code) ? nbr : zero);
+        wv = code_in(kRecWall, code) ? (unsigned)a.wbase + (val * B + b) * PV : wv;
     }
-    // the stash slot is only used by the solve: it travels with the
+    o.wv = (int)wv;
+    // the pool slot is only used by the solve: it travels with the
     // gathered data
     o.wst = iwst[(size_t)r.slot * B + b];
-    load16<T, L::CR>(o.cr, a.crec + (size_t)i * L::CR);
+    load16<T, L::CR>(o.cr, a.crec + (size_t)i * a.rs);
     if constexpr (PFLU) {
         using V = typename Vec16<T>::type;
         constexpr int w = 16 / sizeof(T);
@@ -731,39 +809,39 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
     using L = Lay<P>;
     constexpr int DEG = Deg<P>::v;
     constexpr int Q = DEG + 1;
-    constexpr int QQ = Q * Q;
     constexpr int TQ = Tr<P>::TQ;
     const T* cr = dC.cr;
     // idle threads of a busy wave compute on the first slot's data and
-    // store nothing
-    // upwind traces from LDS, the three reads in flight together
-    const int lds_mask = rec_lds_mask(a.use_pbuf);
-    int code[3], ilp[3], info[3];
+    // store nothing that is read
+    // inflow traces from LDS, the three reads in flight together
+    constexpr int PV = pad4(P);
+    int code[3];
     T u[3][TQ];
+    T v[PV];
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
-        code[il] = (int)(((unsigned)dC.fw[il]) >> 28);
-        info[il] = bits_of(cr[L::R_INFO + il]);
-        ilp[il] = (info[il] >> 2) & 3;
-        lds_load<T, TQ>(u[il], lds + dC.lds[il] + ilp[il] * TQ);
+        code[il] = fw_code(dC.fw[il]);
+        lds_load<T, TQ>(u[il], lds + dC.lds[il]);
     }
-    // upwind traces from global memory: explicit periodic buffer, a
+    lds_load<T, PV>(v, lds + dC.wv);
+    // inflow traces from global memory: explicit periodic buffer, a
     // previous-sweep value (broken cycle) or a bundle running with full
     // barriers.  Only in the kernel instance launched when such reads exist;
     // there behind a wave-uniform branch with the wait inside it.
     if constexpr (GREAD) {
+        const int lds_mask = rec_lds_mask(a.use_pbuf);
         const int gmask = dC.valid ? kRecNeighbour & ~lds_mask : 0;
         const bool g0 = code_in(gmask, code[0]), g1 = code_in(gmask, code[1]), g2 = code_in(gmask, code[2]);
         if (__builtin_amdgcn_ballot_w64(g0 || g1 || g2) != 0) {
 #pragma unroll
             for (int il = 0; il < 3; ++il) {
                 if (code_in(gmask, code[il])) {
-                    const int val = dC.fw[il] & 0x0fffffff;
+                    const int ilp = fw_ilp(dC.fw[il]);
                     const T* fe = (code_in(kRecPeriodic, code[il]) && a.use_pbuf)
-                                      ? a.pbuf + ((size_t)d * a.F + (info[il] >> 4)) * P
-                                      : vdf_d + (size_t)val * P;
+                                      ? a.pbuf + ((size_t)d * a.F + (bits_of(cr[L::R_INFO + il]) >> 4)) * P
+                                      : vdf_d + (size_t)fw_val(dC.fw[il]) * P;
 #pragma unroll
-                    for (int q = 0; q < Q; ++q) u[il][q] = fe[face_node(DEG, ilp[il], q)];
+                    for (int q = 0; q < Q; ++q) u[il][q] = fe[face_node(DEG, ilp, q)];
                 }
             }
             __builtin_amdgcn_s_waitcnt(0x0f70);       // vmcnt(0)
@@ -771,43 +849,25 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
     }
     T x[P];
 #pragma unroll
-    for (int m = 0; m < P; ++m) x[m] = cr[m] + cx * cr[P + m] + cy * cr[2 * P + m];
-    T w[3];
+    for (int m = 0; m < P; ++m) x[m] = source_plus<P, T>(cr, cx, cy, m, v[m]);
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
         const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
-        w[il] = T(0.5) * (sp - fabs(sp));
-    }
-    // wall inflow in face order; weight 0 on every other face
-    const T thermal_on = a.homogeneous ? T(0) : T(1);
-#pragma unroll
-    for (int il = 0; il < 3; ++il) {
-        const T wt = -w[il] * thermal_on;
-        const T wa = code[il] == REC_ADIABATIC ? w[il] : wt;
-        const T ww = code_in(kRecWall, code[il]) ? wa : T(0);
-#pragma unroll
-        for (int m = 0; m < P; ++m) x[m] += ww * cr[L::R_FD + il * QQ + m];
-    }
-    // upwind neighbours in face order; weight 0 on every other face (whose
-    // trace is read from the all-zero slot)
-#pragma unroll
-    for (int il = 0; il < 3; ++il) {
-        const T wn = code_in(kRecNeighbour, code[il]) ? w[il] : T(0);
-        face_update<P, T>(x, il, wn, cr + L::R_FD + il * QQ, u[il]);
+        const T w = T(0.5) * (sp - fabs(sp));
+        face_update<P, T>(x, il, w, a.kref, u[il]);
     }
     const size_t item = (size_t)dC.slot * B + b, NB = (size_t)a.N * B;
     apply_inverse<P, T>(x, [&](int k) { return PFLU ? dC.lu[k] : lu[lu_idx<T>(k, item, NB)]; });
-    // stores without a branch: an idle thread writes its values to the
-    // spill slot and to a scratch row (a branch would let the compiler sink
-    // the whole solve into it, leaving the loads of idle threads unconsumed)
+    // stores without a branch: an idle thread, and an element nobody reads
+    // through LDS, write their traces to the spill slot; an idle thread
+    // writes its values to a scratch row (a branch would let the compiler
+    // sink the whole solve into it, leaving the loads of idle threads
+    // unconsumed)
     {
         constexpr int W = Tr<P>::W;
-        const int RM = a.ring - 1;
-        const int spill = a.ring + a.stash + 1;
-        const int ring_sl = dC.valid ? (dC.slot & RM) : spill;
-        const int st_sl = (dC.valid && dC.wst >= 0) ? a.ring + dC.wst : spill;
-        const int wr0 = (ring_sl * B + b) * W;
-        const int wr1 = (st_sl * B + b) * W;
+        const int spill = a.pool + 1;
+        const int sl = (dC.valid && dC.wst >= 0) ? dC.wst : spill;
+        const int wr = (sl * B + b) * W;
         T* o = dC.valid ? vdf_d + (size_t)dC.cell * P : a.spill_row;
 #pragma unroll
         for (int m = 0; m < P; ++m) o[m] = x[m];
@@ -816,8 +876,7 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
             T t[TQ];
 #pragma unroll
             for (int q = 0; q < TQ; ++q) t[q] = q < Q ? x[face_node(DEG, f, q < Q ? q : 0)] : T(0);
-            lds_store<T, TQ>(lds + wr0 + f * TQ, t);
-            lds_store<T, TQ>(lds + wr1 + f * TQ, t);
+            lds_store<T, TQ>(lds + wr + f * TQ, t);
         }
     }
 }
@@ -839,10 +898,11 @@ template <int P, typename T, bool PFLU, bool GREAD>
 __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
 {
     constexpr int W = Tr<P>::W;
+    constexpr int TQ = Tr<P>::TQ;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int bundle = a.block_bundle[blockIdx.x];
     const int B = a.bundle_ndir[bundle];
-    // LDS: ring slots, stash slots, one all-zero slot, one spill slot
+    // LDS: pool slots, one all-zero slot, one spill slot, the wall traces
     T* lds = reinterpret_cast<T*>(lds_raw);
     const bool global_mode = a.bundle_global[bundle] != 0;
     const int t = threadIdx.x;
@@ -857,8 +917,20 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
     {
-        T* z = lds + (size_t)(a.ring + a.stash) * B * W;
+        T* z = lds + (size_t)a.pool * B * W;
         for (int k = t; k < B * W; k += blockDim.x) z[k] = T(0);
+        // the summed wall inflow of every inflow wall element of the bundle,
+        // per direction (a thermal wall emits nothing in a homogeneous sweep)
+        constexpr int PV = pad4(P);
+        const int w0 = a.wc_off[bundle], nw = a.wc_off[bundle + 1] - w0;
+        const T thermal_on = a.homogeneous ? T(0) : T(1);
+        for (int k = t; k < nw * B * PV; k += blockDim.x) {
+            const int e = k / (B * PV), bb = (k / PV) % B, m = k % PV;
+            const int4 ent = a.wcell[w0 + e];
+            const int db = a.bundle_dirs[a.bundle_dir_off[bundle] + bb];
+            const T* cr = a.crec + (size_t)ent.x * a.rs;
+            lds[a.wbase + k] = m < P ? wall_inflow<P, T>(cr, ent, a.cxv[db], a.cyv[db], a.wtr, thermal_on, m) : T(0);
+        }
     }
     __syncthreads();
 
@@ -896,20 +968,20 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     }
 }
 
-// adiabatic emission of this sweep into the wall faces' slots of the cell
-// records (the wall vector occupies the first P values of the face block)
+// the wall vector of each adiabatic wall face from this iterate's emission
+// flux_wall (bc.py:47-58, _kernels.py:364-397), over the face length
 template <int P, typename T>
-__global__ void __launch_bounds__(256) wall_rec_kernel(const double* __restrict__ flux_wall,
-                                                       const int32_t* __restrict__ wface,
-                                                       const int32_t* __restrict__ wtri,
-                                                       const int32_t* __restrict__ wlfc, int n, T* crec)
+__global__ void __launch_bounds__(256) wall_vec_kernel(const double* __restrict__ flux_wall,
+                                                       const int32_t* __restrict__ row,
+                                                       const int32_t* __restrict__ face,
+                                                       const double* __restrict__ rlen, int n,
+                                                       T* __restrict__ wtr)
 {
-    using L = Lay<P>;
-    constexpr int Q = Deg<P>::v + 1;
+    constexpr int PV = pad4(P);
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n * P) return;
     const int k = g / P, m = g % P;
-    crec[(size_t)wtri[k] * L::CR + L::R_FD + wlfc[k] * Q * Q + m] = (T)flux_wall[(size_t)wface[k] * P + m];
+    wtr[(size_t)row[k] * PV + m] = (T)(flux_wall[(size_t)face[k] * P + m] * rlen[k]);
 }
 
 // ==========================================================================
@@ -1486,6 +1558,43 @@ int guarded(pbte_ctx* ctx, F&& f)
 // ==========================================================================
 // C-ABI
 // ==========================================================================
+namespace {
+// Kr[a][q] = integral over the unit edge of l_a(s) l_{deg-q}(s), l_k the
+// Lagrange polynomials on the equispaced edge nodes k / deg: the element
+// across lists the shared edge's nodes in the opposite order
+std::vector<double> reversed_edge_matrix(int deg)
+{
+    const int Q = deg + 1;
+    std::vector<std::vector<double>> c(Q, std::vector<double>(Q, 0.0));    // coefficients of l_k
+    for (int k = 0; k < Q; ++k) {
+        std::vector<double> p{1.0};
+        for (int j = 0; j < Q; ++j) {
+            if (j == k) continue;
+            const double sj = (double)j / deg, den = (double)(k - j) / deg;
+            std::vector<double> np(p.size() + 1, 0.0);
+            for (size_t e = 0; e < p.size(); ++e) {
+                np[e + 1] += p[e] / den;
+                np[e] -= p[e] * sj / den;
+            }
+            p = np;
+        }
+        c[k] = p;
+    }
+    std::vector<double> kr((size_t)Q * Q, 0.0);
+    for (int a = 0; a < Q; ++a)
+        for (int q = 0; q < Q; ++q) {
+            const auto& x = c[a];
+            const auto& y = c[deg - q];
+            double v = 0.0;
+            for (size_t e = 0; e < x.size(); ++e)
+                for (size_t f = 0; f < y.size(); ++f) v += x[e] * y[f] / (double)(e + f + 1);
+            kr[(size_t)a * Q + q] = v;
+        }
+    return kr;
+}
+
+}  // namespace
+
 extern "C" const char* pbte_last_error(pbte_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 extern "C" void pbte_ctx_destroy(pbte_ctx* ctx) { delete ctx; }
@@ -1544,19 +1653,31 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             ctx->nrm = ctx->upload(nrm.data(), nrm.size());
         }
 
-        // --- compact cell data ------------------------------------------------
+        // --- element data -----------------------------------------------------
+        // CellGeo (factorisation), the compact face blocks and thermal wall
+        // vectors (transport residual), the face lengths and the elements
+        // across
         std::vector<CellGeo<double>> g64(N);
-        std::vector<CellGeo<float>> g32(N);
         const int KQ = pad4(Q * Q);
         std::vector<double> K((size_t)N * 3 * KQ, 0.0);
         std::vector<double> bvec((size_t)std::max(F, 1) * pad4(P) + 32, 0.0);
-        double worst = 0.0;
+        std::vector<double> flen((size_t)N * 3, 0.0);
+        std::vector<int32_t> h_other((size_t)N * 3, -1);
+        // Kr from the first interior or periodic face block over its length:
+        // the reference integrates every face with the same edge rule, so the
+        // blocks are proportional to rounding (the exact edge matrix differs
+        // from them by that rule's own error, ~1e-13); the exact matrix only
+        // when there is no such face
+        std::vector<double> kr = reversed_edge_matrix(deg);
+        bool have_kr = false;
+        double worst = 0.0, kworst = 0.0;
         for (int i = 0; i < N; ++i) {
             for (int il = 0; il < 3; ++il) {
                 g64[i].n[2 * il] = su->nx[i * 3 + il];
                 g64[i].n[2 * il + 1] = su->ny[i * 3 + il];
                 const int fc = su->tri_faces[i * 3 + il];
                 const int bc = su->face_bc[fc];
+                const double* tt = su->ttfc + ((size_t)i * 3 + il) * P * P;
                 int code, other, ilp = 0;
                 if (bc < 0) {
                     code = FACE_INTERIOR;
@@ -1575,21 +1696,19 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                         code = FACE_THERMAL;
                         other = fc;
                         const double tw = su->cv / 4.0 / su->pi * su->bc_temp[bc];
-                        const double* tt = su->ttfc + ((size_t)i * 3 + il) * P * P;
                         for (int m = 0; m < P; ++m) bvec[(size_t)fc * pad4(P) + m] = tw * tt[m * P + 0];
                     } else {
                         throw std::string("boundary type not supported by the sweep");
                     }
                 }
+                std::vector<char> onr(P, 0);
+                for (int a = 0; a < Q; ++a) onr[face_node(deg, il, a)] = 1;
                 if (code <= FACE_PERIODIC) {
                     if (other < 0 || ilp < 0 || ilp > 2) throw std::string("bad across-face element");
-                    const double* tt = su->ttfc + ((size_t)i * 3 + il) * P * P;
-                    double bmax = 0.0, omax = 0.0;
-                    std::vector<char> onr(P, 0), onc(P, 0);
-                    for (int a = 0; a < Q; ++a) {
-                        onr[face_node(deg, il, a)] = 1;
-                        onc[face_node(deg, ilp, a)] = 1;
-                    }
+                    h_other[(size_t)i * 3 + il] = other;
+                    double bmax = 0.0, omax = 0.0, len = 0.0;
+                    std::vector<char> onc(P, 0);
+                    for (int a = 0; a < Q; ++a) onc[face_node(deg, ilp, a)] = 1;
                     for (int m = 0; m < P; ++m)
                         for (int l = 0; l < P; ++l) {
                             const double v = std::fabs(tt[m * P + l]);
@@ -1598,60 +1717,148 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                         }
                     if (bmax > 0) worst = std::max(worst, omax / bmax);
                     for (int a = 0; a < Q; ++a)
-                        for (int q = 0; q < Q; ++q)
-                            K[((size_t)i * 3 + il) * KQ + a * Q + q] =
-                                tt[face_node(deg, il, a) * P + face_node(deg, ilp, q)];
+                        for (int q = 0; q < Q; ++q) {
+                            const double v = tt[face_node(deg, il, a) * P + face_node(deg, ilp, q)];
+                            K[((size_t)i * 3 + il) * KQ + a * Q + q] = v;
+                            len += v;
+                        }
+                    if (!have_kr && len > 0.0) {
+                        for (int k = 0; k < Q * Q; ++k)
+                            kr[k] = K[((size_t)i * 3 + il) * KQ + k] / len;
+                        have_kr = true;
+                    }
+                    flen[(size_t)i * 3 + il] = len;
+                } else {
+                    // a wall face: int_tri_tri_fc holds the integral of each
+                    // basis function on the face in every column
+                    double len = 0.0, bmax = 0.0, omax = 0.0;
+                    for (int a = 0; a < Q; ++a) len += tt[face_node(deg, il, a) * P];
+                    for (int m = 0; m < P; ++m) {
+                        const double v = std::fabs(tt[m * P]);
+                        if (onr[m]) bmax = std::max(bmax, v); else omax = std::max(omax, v);
+                    }
+                    if (bmax > 0) worst = std::max(worst, omax / bmax);
+                    // the adiabatic emission goes through the face mass matrix
+                    const double* fm = su->fcm + ((size_t)i * 3 + il) * P * P;
+                    double fmax = 0.0, fout = 0.0;
+                    for (int m = 0; m < P; ++m)
+                        for (int l = 0; l < P; ++l) {
+                            const double v = std::fabs(fm[m * P + l]);
+                            if (onr[m] && onr[l]) fmax = std::max(fmax, v); else fout = std::max(fout, v);
+                        }
+                    if (fmax > 0) worst = std::max(worst, fout / fmax);
+                    flen[(size_t)i * 3 + il] = len;
                 }
                 g64[i].other[il] = other;
                 g64[i].info[il] = code | (ilp << 2) | (fc << 4);
-            }
-            for (int k = 0; k < 6; ++k) g32[i].n[k] = (float)g64[i].n[k];
-            for (int k = 0; k < 3; ++k) {
-                g32[i].other[k] = g64[i].other[k];
-                g32[i].info[k] = g64[i].info[k];
             }
         }
         if (worst > 1e-9)
             throw std::string("element x neighbour face tensor is not confined to the shared face "
                               "(non-conforming mesh?)");
+        // every face block must be its length times Kr; a wall face's column
+        // its length times the row sums of Kr
+        for (int i = 0; i < N; ++i)
+            for (int il = 0; il < 3; ++il) {
+                const double len = flen[(size_t)i * 3 + il];
+                const int code = g64[i].info[il] & 3;
+                if (code <= FACE_PERIODIC) {
+                    const double* Kf = K.data() + ((size_t)i * 3 + il) * KQ;
+                    double bmax = 0.0, e = 0.0;
+                    for (int k = 0; k < Q * Q; ++k) {
+                        bmax = std::max(bmax, std::fabs(Kf[k]));
+                        e = std::max(e, std::fabs(Kf[k] - len * kr[k]));
+                    }
+                    if (bmax > 0) kworst = std::max(kworst, e / bmax);
+                } else {
+                    const double* tt = su->ttfc + ((size_t)i * 3 + il) * P * P;
+                    double bmax = 0.0, e = 0.0;
+                    for (int a = 0; a < Q; ++a) {
+                        double rsum = 0.0;
+                        for (int q = 0; q < Q; ++q) rsum += kr[a * Q + q];
+                        const double v = tt[face_node(deg, il, a) * P];
+                        bmax = std::max(bmax, std::fabs(v));
+                        e = std::max(e, std::fabs(v - len * rsum));
+                    }
+                    if (bmax > 0) kworst = std::max(kworst, e / bmax);
+                }
+            }
+        if (kworst > 1e-11)
+            throw std::string("face blocks are not a length times the unit-edge mass matrix "
+                              "(curved or non-affine faces?)");
         ctx->geo64 = ctx->upload(g64.data(), N);
-        ctx->geo32 = ctx->upload(g32.data(), N);
         ctx->K64 = ctx->upload(K.data(), K.size());
-        std::vector<float> K32(K.begin(), K.end());
-        ctx->K32 = ctx->upload(K32.data(), K32.size());
         ctx->bvec64 = ctx->upload(bvec.data(), bvec.size());
-        std::vector<float> bv32(bvec.begin(), bvec.end());
-        ctx->bvec32 = ctx->upload(bv32.data(), bv32.size());
+        ctx->other = ctx->upload(h_other.data(), h_other.size());
+        for (int k = 0; k < Q * Q; ++k) {
+            ctx->kref64[k] = kr[k];
+            ctx->kref32[k] = (float)kr[k];
+        }
         {
-            // cell records of the pipelined sweep (the source part is written
-            // by every sweep, the adiabatic wall vectors likewise)
-            const int CR = pad4(3 * P + 9 + 3 * Q * Q), RN = 3 * P, RI = 3 * P + 6, RF = 3 * P + 9;
-            std::vector<double> c64((size_t)N * CR, 0.0);
-            std::vector<float> c32((size_t)N * CR, 0.0f);
+            // cell records: source (written by every sweep), length-scaled
+            // outward normals, info words (bit patterns)
+            const int rs64 = rec_stride_rt(P, 8), rs32 = rec_stride_rt(P, 4);
+            const int RN = 3 * P, RI = 3 * P + 6;
+            std::vector<double> c64((size_t)N * rs64, 0.0);
+            std::vector<float> c32((size_t)N * rs32, 0.0f);
             for (int i = 0; i < N; ++i) {
-                double* r = c64.data() + (size_t)i * CR;
-                float* r32 = c32.data() + (size_t)i * CR;
-                for (int k = 0; k < 6; ++k) r[RN + k] = g64[i].n[k];
+                double* r = c64.data() + (size_t)i * rs64;
+                float* r32 = c32.data() + (size_t)i * rs32;
                 for (int il = 0; il < 3; ++il) {
+                    const double len = flen[(size_t)i * 3 + il];
+                    r[RN + 2 * il] = g64[i].n[2 * il] * len;
+                    r[RN + 2 * il + 1] = g64[i].n[2 * il + 1] * len;
+                    r32[RN + 2 * il] = (float)r[RN + 2 * il];
+                    r32[RN + 2 * il + 1] = (float)r[RN + 2 * il + 1];
                     const uint32_t bits = (uint32_t)g64[i].info[il];
                     const uint64_t b64 = bits;
                     std::memcpy(r + RI + il, &b64, sizeof(double));
                     std::memcpy(r32 + RI + il, &bits, sizeof(float));
-                    const int code = g64[i].info[il] & 3;
-                    double* fd = r + RF + il * Q * Q;
-                    if (code <= FACE_PERIODIC) {
-                        for (int k = 0; k < Q * Q; ++k) fd[k] = K[((size_t)i * 3 + il) * KQ + k];
-                    } else if (code == FACE_THERMAL) {
-                        const int fc = g64[i].other[il];
-                        for (int m = 0; m < P; ++m) fd[m] = bvec[(size_t)fc * pad4(P) + m];
-                    }
                 }
-                for (int k = 0; k < CR; ++k)
-                    if (k < RI || k >= RI + 3) r32[k] = (float)r[k];
             }
+            ctx->rs64 = rs64;
+            ctx->rs32 = rs32;
             ctx->crec64 = ctx->upload(c64.data(), c64.size());
             ctx->crec32 = ctx->upload(c32.data(), c32.size());
         }
+        // wall vectors over the face length, one row per thermal or
+        // adiabatic wall face: -tw ttfc[:, 0] / L for a thermal wall (the
+        // inflow term of _kernels.py:213-215), flux_wall / L for an adiabatic
+        // one, refreshed by every sweep (_kernels.py:210-212)
+        const int TQ = trace_pad(Q);
+        const int PV = pad4(P);
+        std::vector<int32_t> wrow((size_t)std::max(F, 1), -1);
+        {
+            std::vector<double> w64;
+            std::vector<int32_t> ar, af;
+            std::vector<double> arl;
+            for (int f = 0; f < F; ++f) {
+                const int bc = su->face_bc[f];
+                if (bc < 0) continue;
+                const int typ = su->bc_type[bc];
+                if (typ != 1 && typ != 2) continue;
+                int tri = su->face_tri[f * 2], lfc = su->face_lfc[f * 2];
+                if (tri < 0) { tri = su->face_tri[f * 2 + 1]; lfc = su->face_lfc[f * 2 + 1]; }
+                wrow[f] = ctx->n_wrow++;
+                const double len = flen[(size_t)tri * 3 + lfc];
+                const double tw = typ == 1 ? su->cv / 4.0 / su->pi * su->bc_temp[bc] : 0.0;
+                const double* tt = su->ttfc + ((size_t)tri * 3 + lfc) * P * P;
+                for (int m = 0; m < PV; ++m) w64.push_back(typ == 1 && m < P ? -(tw * tt[m * P]) / len : 0.0);
+                if (typ == 2) {
+                    ar.push_back(wrow[f]); af.push_back(f);
+                    arl.push_back(1.0 / len);
+                }
+            }
+            w64.resize(std::max<size_t>(w64.size(), 1), 0.0);
+            std::vector<float> w32(w64.begin(), w64.end());
+            ctx->wtr64 = ctx->upload(w64.data(), w64.size());
+            ctx->wtr32 = ctx->upload(w32.data(), w32.size());
+            ctx->n_adw = (int)ar.size();
+            ctx->adw_row = ctx->upload(ar.data(), ar.size());
+            ctx->adw_face = ctx->upload(af.data(), af.size());
+            ctx->adw_rlen = ctx->upload(arl.data(), arl.size());
+        }
+        if (ctx->n_wrow >= kWallThermal) throw std::string("too many wall faces");
 
         // --- adiabatic walls --------------------------------------------------
         std::vector<int32_t> wf, wt, wl;
@@ -1672,6 +1879,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->n_per = (int)pf.size();
         ctx->per_face = ctx->upload(pf.data(), pf.size());
         ctx->per_tri = ctx->upload(pt.data(), pt.size());
+        if (N > kFaceValMask || F > kFaceValMask) throw std::string("mesh too large for 26-bit record fields");
 
         // --- bundles: directions with identical inflow signs ------------------
         std::vector<uint64_t> hsh(D);
@@ -1721,30 +1929,57 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         }
         const int nb = (int)bundles.size();
         ctx->nb = nb;
+        int bm = 1;
+        for (auto& bb : bundles) bm = std::max(bm, (int)bb.size());
+        // inflow wall elements per bundle (the same for every direction of it)
+        {
+            int most = 0;
+            std::vector<int32_t> mark((size_t)N, -1);
+            for (int k = 0; k < nb; ++k) {
+                const int d = bundles[k][0];
+                int n = 0;
+                for (int f = 0; f < F; ++f) {
+                    if (wrow[f] < 0) continue;
+                    int tri = su->face_tri[f * 2], lfc = su->face_lfc[f * 2];
+                    if (tri < 0) { tri = su->face_tri[f * 2 + 1]; lfc = su->face_lfc[f * 2 + 1]; }
+                    const double sp = su->cxv[d] * su->nx[tri * 3 + lfc] + su->cyv[d] * su->ny[tri * 3 + lfc];
+                    if (sp < 0.0 && mark[tri] != k) { mark[tri] = k; ++n; }
+                }
+                most = std::max(most, n);
+            }
+            ctx->max_wall_in = most;
+        }
+        // LDS pool: what one workgroup may hold when every CU runs its share
+        // of the bundles at once, sized at binary64 (a binary32 sweep uses
+        // half of it)
+        {
+            int dev = 0, ncu = 1, lds_blk = 65536, lds_cu = 0;
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            HIP_TRY(hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+            if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+                    hipSuccess || lds_cu <= 0)
+                lds_cu = lds_blk;
+            const int per_cu = std::max(1, (nb + std::max(ncu, 1) - 1) / std::max(ncu, 1));
+            const size_t budget = std::min<size_t>((size_t)lds_blk, (size_t)lds_cu / per_cu);
+            const size_t slot_bytes = (size_t)bm * pad4(3 * TQ) * 8;
+            const size_t wall_bytes = (size_t)ctx->max_wall_in * bm * PV * 8;
+            if (budget < wall_bytes + 4 * slot_bytes) throw std::string("LDS too small for the wall traces");
+            int pool = (int)((budget - wall_bytes) / slot_bytes) - 2;
+            if (su->lds_slots_cap > 0) pool = std::min(pool, su->lds_slots_cap);
+            ctx->pool_slots = std::max(pool, 1);
+        }
         // levels per bundle (all directions of a bundle share them)
-        std::vector<int32_t> h_slot((size_t)nb * N), h_lev_off(nb + 1, 0), h_lev_ptr;
+        std::vector<int32_t> h_slot((size_t)nb * N), h_lev_ptr;
         std::vector<int32_t> h_ndir(nb), h_dir_off(nb + 1, 0), h_dirs;
         std::vector<int64_t> h_lu_off(nb + 1, 0), h_item_off(nb + 1, 0);
         std::vector<std::vector<int32_t>> slots(nb);
         std::vector<std::vector<int32_t>> lptr(nb);
+        std::vector<std::vector<int4>> wlists(nb);
         std::vector<int4> h_rec((size_t)nb * N);
         std::vector<int32_t> h_wst((size_t)nb * N, -1);
         std::vector<uint8_t> global_flag(nb, 0);
-        std::vector<int64_t> far_reads(nb, 0), old_reads(nb, 0), per_reads(nb, 0);
-        {
-            // LDS budget per workgroup: 32 KiB ring + 32 KiB stash at binary64
-            int bm = 1;
-            for (auto& bb : bundles) bm = std::max(bm, (int)bb.size());
-            const int slot_bytes = bm * 3 * (Q == 2 ? 2 : pad4(Q)) * 8;
-            int r = 1;
-            while ((r * 2) * slot_bytes <= 32768) r *= 2;
-            ctx->ring_slots = r;
-            ctx->stash_slots = std::max(1, 32768 / slot_bytes - 2);
-            if (su->lds_slots_cap > 0) {
-                while (ctx->ring_slots > su->lds_slots_cap) ctx->ring_slots /= 2;
-                ctx->stash_slots = std::min(ctx->stash_slots, su->lds_slots_cap);
-            }
-        }
+        std::vector<int64_t> peak(nb, 0), old_reads(nb, 0), per_reads(nb, 0);
         // levels of every bundle: slots in level order (the reference order
         // within a level) and the level boundaries
         std::vector<std::vector<int32_t>> lcnt(nb);
@@ -1790,7 +2025,6 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 if (ctx->items % (int)bb.size()) ctx->block = ctx->items = 256;
         }
         std::atomic_int next{0};
-        std::vector<std::string> errs(nb);
         auto work = [&]() {
             for (int k = next++; k < nb; k = next++) {
                 const int d = bundles[k][0];
@@ -1805,30 +2039,28 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 for (int l = 0; l < nlev; ++l)
                     for (int s = cnt[l]; s < cnt[l + 1]; s += width) lp.push_back(s);
                 lp.push_back(N);
-                // sublevel index and end slot of every slot
-                std::vector<int32_t> sub_of((size_t)N), end_of((size_t)N);
+                // sublevel of every slot
+                std::vector<int32_t> sub_of((size_t)N);
                 for (size_t j = 0; j + 1 < lp.size(); ++j)
-                    for (int q = lp[j]; q < lp[j + 1]; ++q) {
-                        sub_of[q] = (int32_t)j;
-                        end_of[q] = lp[j + 1];
-                    }
+                    for (int q = lp[j]; q < lp[j + 1]; ++q) sub_of[q] = (int32_t)j;
                 // per-slot records: the element and, per local face, what the
-                // sweep reads through it (REC_*).  Upwind values of this sweep
-                // come from the LDS ring when written within the last `ring`
-                // slots, else from an LDS stash slot reused once its last
-                // reader's sublevel has passed; a bundle whose stash would
-                // overflow reads them from global memory instead.
+                // sweep reads through it (REC_*).  This sweep's upwind values
+                // come from the LDS pool slot of the upwind element; a bundle
+                // whose pool would overflow reads them from global memory.
                 std::vector<int32_t> pos((size_t)N), slot_of((size_t)N);
                 for (int p = 0; p < N; ++p) pos[order[p]] = p;
                 for (int q = 0; q < N; ++q) slot_of[sl[q]] = q;
                 const double cxd = su->cxv[d], cyd = su->cyv[d];
-                struct Read { int code, val; };
+                struct Read { int code, val, ilp; };
                 std::vector<Read> rd((size_t)N * 3);
                 std::vector<int32_t> last_reader((size_t)N, -1);
+                auto& wlst = wlists[k];
+                wlst.clear();
                 for (int q = 0; q < N; ++q) {
                     const int i = sl[q];
+                    int wk = -1;
                     for (int il = 0; il < 3; ++il) {
-                        Read r{REC_NONE, 0};
+                        Read r{REC_NONE, 0, 0};
                         const double sp = cxd * su->nx[i * 3 + il] + cyd * su->ny[i * 3 + il];
                         if (sp < 0.0) {
                             const int fc = su->tri_faces[i * 3 + il];
@@ -1837,49 +2069,50 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                             if (bc < 0 || typ == 3) {
                                 const int j = bc < 0 ? su->neighbour[i * 3 + il] : su->periodic_tri[fc];
                                 const bool per = bc >= 0;
+                                const int ilp = su->other_lfc[i * 3 + il];
                                 if (pos[j] < pos[i]) {
                                     const int u = slot_of[j];
-                                    if (u >= end_of[q] - ctx->ring_slots) {
-                                        r = {per ? REC_PER_RING : REC_UP_RING, u};
-                                    } else {
-                                        r = {per ? REC_PER_STASH : REC_UP_STASH, u};   // slot for now
-                                        last_reader[u] = std::max(last_reader[u], sub_of[q]);
-                                        ++far_reads[k];
-                                    }
+                                    r = {per ? REC_PER_LDS : REC_UP_LDS, u, ilp};
+                                    last_reader[u] = std::max(last_reader[u], sub_of[q]);
                                 } else {
-                                    r = {per ? REC_PER_OLD : REC_OLD, j};
+                                    r = {per ? REC_PER_OLD : REC_OLD, j, ilp};
                                 }
                             } else {
-                                r = {typ == 2 ? REC_ADIABATIC : REC_THERMAL, fc};
+                                if (wk < 0) {
+                                    wk = (int)wlst.size();
+                                    wlst.push_back(make_int4(i, -1, -1, -1));
+                                }
+                                const int ent = wrow[fc] | (typ == 1 ? kWallThermal : 0);
+                                int4& e = wlst[wk];
+                                (il == 0 ? e.y : (il == 1 ? e.z : e.w)) = ent;
+                                r = {typ == 2 ? REC_ADIABATIC : REC_THERMAL, wk, 0};
                             }
                         }
                         rd[(size_t)q * 3 + il] = r;
                     }
                 }
-                // stash slots: interval reuse in slot order
+                // pool slots: interval allocation in slot order; a slot is free
+                // again once the sublevel of its last reader has passed
                 std::vector<int32_t> sid((size_t)N, -1);
                 bool overflow = false;
                 {
-                    std::vector<std::pair<int, int>> busy;     // (last reader sublevel, slot)
-                    std::vector<int> freelist;
+                    using Busy = std::pair<int32_t, int32_t>;     // (last reader sublevel, slot)
+                    std::priority_queue<Busy, std::vector<Busy>, std::greater<Busy>> busy;
+                    std::vector<int32_t> freelist;
                     int used = 0;
                     for (int u = 0; u < N && !overflow; ++u) {
                         if (last_reader[u] < 0) continue;
-                        for (size_t e = 0; e < busy.size();) {
-                            if (busy[e].first < sub_of[u]) {
-                                freelist.push_back(busy[e].second);
-                                busy[e] = busy.back();
-                                busy.pop_back();
-                            } else {
-                                ++e;
-                            }
+                        while (!busy.empty() && busy.top().first < sub_of[u]) {
+                            freelist.push_back(busy.top().second);
+                            busy.pop();
                         }
                         int id;
                         if (!freelist.empty()) { id = freelist.back(); freelist.pop_back(); }
-                        else if (used < ctx->stash_slots) id = used++;
+                        else if (used < ctx->pool_slots) id = used++;
                         else { overflow = true; break; }
                         sid[u] = id;
-                        busy.push_back({last_reader[u], id});
+                        busy.push({last_reader[u], id});
+                        peak[k] = std::max<int64_t>(peak[k], (int64_t)busy.size());
                     }
                 }
                 global_flag[k] = overflow ? 1 : 0;
@@ -1887,20 +2120,13 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                     int w[3];
                     for (int il = 0; il < 3; ++il) {
                         Read r = rd[(size_t)q * 3 + il];
-                        if (overflow) {
-                            // everything of this sweep through global memory
-                            if (r.code == REC_UP_RING || r.code == REC_UP_STASH) r = {REC_UP_GLOBAL, sl[r.val]};
-                            else if (r.code == REC_PER_RING || r.code == REC_PER_STASH) r = {REC_PER_GLOBAL, sl[r.val]};
-                        } else if (r.code == REC_UP_STASH || r.code == REC_PER_STASH) {
-                            r.val = ctx->ring_slots + sid[r.val];     // the LDS slot
-                        } else if (r.code == REC_UP_RING || r.code == REC_PER_RING) {
-                            r.val = r.val & (ctx->ring_slots - 1);    // the LDS slot
+                        if (r.code == REC_UP_LDS || r.code == REC_PER_LDS) {
+                            if (overflow) r = {r.code == REC_UP_LDS ? REC_UP_GLOBAL : REC_PER_GLOBAL, sl[r.val], r.ilp};
+                            else r.val = sid[r.val];
                         }
-                        w[il] = (int)(((unsigned)r.code << 28) | (unsigned)r.val);
+                        w[il] = face_word(r.code, r.ilp, r.val);
                         if (r.code == REC_OLD || r.code == REC_PER_OLD) ++old_reads[k];
-                        if (r.code == REC_PER_RING || r.code == REC_PER_STASH || r.code == REC_PER_OLD ||
-                            r.code == REC_PER_GLOBAL)
-                            ++per_reads[k];
+                        if (r.code == REC_PER_LDS || r.code == REC_PER_OLD || r.code == REC_PER_GLOBAL) ++per_reads[k];
                     }
                     h_rec[(size_t)k * N + q] = make_int4(sl[q], w[0], w[1], w[2]);
                     h_wst[(size_t)k * N + q] = overflow ? -1 : sid[q];
@@ -1909,7 +2135,10 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         };
         run_threads(work);
         int64_t maxw = 0;
+        std::vector<int4> h_wcell;
+        std::vector<int32_t> h_wc_off(nb + 1, 0);
         for (int k = 0; k < nb; ++k) ctx->max_bundle_dirs = std::max(ctx->max_bundle_dirs, (int)bundles[k].size());
+        std::vector<int32_t> h_lev_off(nb + 1, 0);
         for (int k = 0; k < nb; ++k) {
             const int B = (int)bundles[k].size();
             h_ndir[k] = B;
@@ -1926,6 +2155,8 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             h_lev_ptr.push_back(N);
             h_lu_off[k + 1] = h_lu_off[k] + (int64_t)pad4(P * P) * N * B;
             h_item_off[k + 1] = h_item_off[k] + (int64_t)N * B;
+            h_wc_off[k + 1] = h_wc_off[k] + (int32_t)wlists[k].size();
+            h_wcell.insert(h_wcell.end(), wlists[k].begin(), wlists[k].end());
         }
         // lev_off[k]..lev_off[k]+nlev_k-1 index level starts; lev_ptr[j+1] is
         // the end of level j; the sentinel N closes every bundle's last level.
@@ -1933,8 +2164,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         for (int k = 0; k < nb; ++k) tot += (int64_t)lptr[k].size() - 1;
         ctx->total_levels = tot;
         ctx->max_width = maxw;
-        // kernel loops lv in [lev_off[k], lev_end[k]); pass the end through
-        // lev_off[k+1] by storing per-bundle (start, end) pairs instead
+        // the kernels loop lv over [start, end): per-bundle (start, end) pairs
         std::vector<int32_t> h_lev_se(2 * nb);
         for (int k = 0; k < nb; ++k) {
             h_lev_se[2 * k] = h_lev_off[k];
@@ -1965,9 +2195,12 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->lev_off = ctx->upload(h_lev_se.data(), h_lev_se.size());
         ctx->lev_ptr = ctx->upload(h_lev_ptr.data(), h_lev_ptr.size());
         ctx->lu_off = ctx->upload(h_lu_off.data(), nb + 1);
+        h_wcell.resize(std::max<size_t>(h_wcell.size(), 1), make_int4(0, -1, -1, -1));
+        ctx->wcell = ctx->upload(h_wcell.data(), h_wcell.size());
+        ctx->wc_off = ctx->upload(h_wc_off.data(), h_wc_off.size());
         {
             // per item (bundle, slot, direction of the bundle): {cell, face
-            // words} and the stash slot
+            // words} and the pool slot
             std::vector<int4> h_irec((size_t)h_item_off[nb]);
             std::vector<int32_t> h_iwst((size_t)h_item_off[nb]);
             for (int k = 0; k < nb; ++k) {
@@ -1983,13 +2216,12 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             ctx->iwst = ctx->upload(h_iwst.data(), h_iwst.size());
         }
         ctx->bundle_global = ctx->upload(global_flag.data(), global_flag.size());
-        for (int k = 0; k < nb; ++k) ctx->global_bundles += global_flag[k];
         for (int k = 0; k < nb; ++k) {
-            ctx->ring_fallback += far_reads[k];
+            ctx->global_bundles += global_flag[k];
+            ctx->pool_peak = std::max(ctx->pool_peak, peak[k]);
             ctx->old_reads += old_reads[k];
             ctx->periodic_reads += per_reads[k];
         }
-        if (N >= (1 << 28) || F >= (1 << 28)) throw std::string("mesh too large for 28-bit record fields");
         ctx->item_off = ctx->upload(h_item_off.data(), nb + 1);
         ctx->h_lu_off = h_lu_off;
         ctx->lu_elems = h_lu_off[nb];
@@ -1997,9 +2229,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
 
         // --- scratch -----------------------------------------------------------
         ctx->src64 = ctx->alloc<double>((size_t)N * pad4(3 * P));
-        ctx->src32 = ctx->alloc<float>((size_t)N * pad4(3 * P));
         HIP_TRY(hipMemset(ctx->src64, 0, sizeof(double) * (size_t)N * pad4(3 * P)));
-        HIP_TRY(hipMemset(ctx->src32, 0, sizeof(float) * (size_t)N * pad4(3 * P)));
         ctx->ssraw = ctx->alloc<double>((size_t)N * P);
         ctx->red_len = std::max<size_t>((size_t)D * nblocks(N, 256) * 2 + 2 * (size_t)D,
                                         (size_t)kMaxBasis * kDotBlocks + kMaxBasis) + 64;
@@ -2052,9 +2282,9 @@ extern "C" int pbte_schedule_info(pbte_ctx* ctx, int64_t* out6)
         out6[0] = ctx->total_levels;
         out6[1] = ctx->nb;
         out6[2] = ctx->max_width;
-        out6[3] = ctx->ring_fallback;
+        out6[3] = ctx->pool_peak;
         out6[4] = ctx->global_bundles;
-        out6[5] = ctx->ring_slots;
+        out6[5] = ctx->pool_slots;
         return 0;
     });
 }
@@ -2074,37 +2304,75 @@ void launch_factor(pbte_ctx* ctx, double* lu64, float* lu32)
     HIP_TRY(hipGetLastError());
 }
 
-template <int P, typename T>
-void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const CellGeo<T>* geo, const T* K,
-                  const T* bvec, const T* src, const double* flux_wall, const T* crec, const T* pbuf, T* vdf,
-                  int use_pbuf, int homogeneous)
+// the per-precision sweep data of a context
+template <typename T> struct SweepData;
+template <> struct SweepData<double> {
+    static const double* lu(const pbte_ctx* c) { return c->lu64; }
+    static const double* cx(const pbte_ctx* c) { return c->cxv; }
+    static const double* cy(const pbte_ctx* c) { return c->cyv; }
+    static double* crec(pbte_ctx* c) { return c->crec64; }
+    static int rs(const pbte_ctx* c) { return c->rs64; }
+    static double* wtr(pbte_ctx* c) { return c->wtr64; }
+    static const double* kref(const pbte_ctx* c) { return c->kref64; }
+};
+template <> struct SweepData<float> {
+    static const float* lu(const pbte_ctx* c) { return c->lu32; }
+    static const float* cx(const pbte_ctx* c) { return c->cxv32; }
+    static const float* cy(const pbte_ctx* c) { return c->cyv32; }
+    static float* crec(pbte_ctx* c) { return c->crec32; }
+    static int rs(const pbte_ctx* c) { return c->rs32; }
+    static float* wtr(pbte_ctx* c) { return c->wtr32; }
+    static const float* kref(const pbte_ctx* c) { return c->kref32; }
+};
+
+// LDS bytes of the pipelined kernel: pool, zero and spill slots, wall traces
+template <int P, typename T> size_t sweep_lds_bytes(const pbte_ctx* ctx)
 {
+    return ((size_t)(ctx->pool_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W +
+            (size_t)ctx->max_wall_in * ctx->max_bundle_dirs * pad4(P)) *
+           sizeof(T);
+}
+
+template <int P, typename T, bool PF, bool GR>
+void launch_pipe(pbte_ctx* ctx, const SweepArgs<T>& a, size_t lds)
+{
+    auto* fn = sweep_pipe_kernel<P, T, PF, GR>;
+    if (lds > 65536)
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    hipLaunchKernelGGL(fn, dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
+}
+
+template <int P, typename T>
+void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous)
+{
+    using S = SweepData<T>;
     SweepArgs<T> a{};
     a.block_bundle = ctx->block_bundle; a.bundle_ndir = ctx->bundle_ndir; a.bundle_dir_off = ctx->bundle_dir_off;
-    a.bundle_dirs = ctx->bundle_dirs; a.slot_cell = ctx->slot_cell; a.lev_off = ctx->lev_off;
-    a.lev_ptr = ctx->lev_ptr; a.lu_off = ctx->lu_off; a.item_off = ctx->item_off; a.lu = lu;
-    a.cxv = cx; a.cyv = cy; a.geo = geo; a.K = K; a.bvec = bvec; a.src = src; a.flux_wall = flux_wall;
+    a.bundle_dirs = ctx->bundle_dirs; a.lev_off = ctx->lev_off; a.lev_ptr = ctx->lev_ptr;
+    a.lu_off = ctx->lu_off; a.item_off = ctx->item_off;
+    a.lu = S::lu(ctx); a.cxv = S::cx(ctx); a.cyv = S::cy(ctx);
+    a.crec = S::crec(ctx); a.rs = S::rs(ctx); a.other = ctx->other; a.wtr = S::wtr(ctx);
+    a.wcell = ctx->wcell; a.wc_off = ctx->wc_off;
     a.pbuf = pbuf; a.vdf = vdf; a.N = ctx->N; a.F = ctx->F; a.use_pbuf = use_pbuf; a.homogeneous = homogeneous;
     a.irec = ctx->irec;
     a.iwst = ctx->iwst;
     a.spill_row = reinterpret_cast<T*>(ctx->spill_row);
-    a.crec = crec;
+    a.bundle_global = ctx->bundle_global;
     a.lane_zero = 0;
+    a.pool = ctx->pool_slots;
+    a.wbase = (ctx->pool_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W;
+    for (int k = 0; k < ctx->Q * ctx->Q; ++k) a.kref[k] = S::kref(ctx)[k];
     const bool timed = 2 * (size_t)ctx->ev_used + 1 < ctx->events.size();
     if (timed) HIP_TRY(hipEventRecord(ctx->events[2 * ctx->ev_used], ctx->stream));
     if (ctx->variant == 1) {
         hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
     } else {
-        a.ring = ctx->ring_slots;
-        a.stash = ctx->stash_slots;
-        a.bundle_global = ctx->bundle_global;
-        const size_t lds = (size_t)(ctx->ring_slots + ctx->stash_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W * sizeof(T);
+        const size_t lds = sweep_lds_bytes<P, T>(ctx);
         constexpr bool pf = P * P * sizeof(T) <= 288;
         const bool gread = ctx->global_bundles > 0 || ctx->old_reads > 0 || (use_pbuf && ctx->periodic_reads > 0);
-        if (gread)
-            hipLaunchKernelGGL((sweep_pipe_kernel<P, T, pf, true>), dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
-        else
-            hipLaunchKernelGGL((sweep_pipe_kernel<P, T, pf, false>), dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
+        if (gread) launch_pipe<P, T, pf, true>(ctx, a, lds);
+        else launch_pipe<P, T, pf, false>(ctx, a, lds);
     }
     HIP_TRY(hipGetLastError());
     if (timed) {
@@ -2116,31 +2384,26 @@ void launch_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const Ce
 template <typename T>
 void launch_source(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, T* out, int stride);
 
-// one sweep: the source (and, for the pipelined kernel, the adiabatic wall
-// vectors) into the layout the sweep kernel reads, then the sweep
+// one sweep: the source into the cell records, the adiabatic wall vectors
+// from flux_wall, then the sweep
 template <typename T>
-void run_sweep(pbte_ctx* ctx, const T* lu, const T* cx, const T* cy, const CellGeo<T>* geo, const T* K, const T* bvec,
-               T* src, T* crec, const double* ts, const double* qxs, const double* qys, const double* flux_wall,
+void run_sweep(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, const double* flux_wall,
                const T* pbuf, T* vdf, int use_pbuf, int homogeneous)
 {
+    using S = SweepData<T>;
     const int P = ctx->P;
-    if (ctx->variant == 1) {
-        launch_source<T>(ctx, ts, qxs, qys, src, pad4(3 * P));
-    } else {
-        const int CR = pad4(3 * P + 9 + 3 * ctx->Q * ctx->Q);
-        launch_source<T>(ctx, ts, qxs, qys, crec, CR);
-        if (ctx->n_wall) {
-            const unsigned g = nblocks((int64_t)ctx->n_wall * P, 256);
-#define WR(PP) case PP: hipLaunchKernelGGL((wall_rec_kernel<PP, T>), dim3(g), dim3(256), 0, ctx->stream, flux_wall, \
-                                           ctx->wall_face, ctx->wall_tri, ctx->wall_lfc, ctx->n_wall, crec); break;
-            switch (P) { WR(3) WR(6) WR(10) WR(15) }
-#undef WR
-            HIP_TRY(hipGetLastError());
-        }
+    launch_source<T>(ctx, ts, qxs, qys, S::crec(ctx), S::rs(ctx));
+    if (ctx->n_adw) {
+        const unsigned g = nblocks((int64_t)ctx->n_adw * P, 256);
+#define WT(PP) case PP: hipLaunchKernelGGL((wall_vec_kernel<PP, T>), dim3(g), dim3(256), 0, ctx->stream, \
+                                           flux_wall, ctx->adw_row, ctx->adw_face, ctx->adw_rlen, ctx->n_adw, \
+                                           S::wtr(ctx)); break;
+        switch (P) { WT(3) WT(6) WT(10) WT(15) }
+#undef WT
+        HIP_TRY(hipGetLastError());
     }
     switch (P) {
-#define SW(PP) case PP: launch_sweep<PP, T>(ctx, lu, cx, cy, geo, K, bvec, src, flux_wall, crec, pbuf, vdf, use_pbuf, \
-                                        homogeneous); break;
+#define SW(PP) case PP: launch_sweep<PP, T>(ctx, pbuf, vdf, use_pbuf, homogeneous); break;
     SW(3) SW(6) SW(10) SW(15)
 #undef SW
     }
@@ -2199,13 +2462,11 @@ extern "C" int pbte_sweep(pbte_ctx* ctx, int32_t precision, const double* ts, co
         if (ctx->n_wall && !flux_wall) throw std::string("adiabatic walls need flux_wall");
         if (precision == PBTE_FP64) {
             if (!ctx->have64) throw std::string("FP64 factors not built; call pbte_factorise");
-            run_sweep<double>(ctx, ctx->lu64, ctx->cxv, ctx->cyv, ctx->geo64, ctx->K64, ctx->bvec64, ctx->src64,
-                              ctx->crec64, ts, qxs, qys, flux_wall, (const double*)pbuf, (double*)vdf, use_pbuf,
+            run_sweep<double>(ctx, ts, qxs, qys, flux_wall, (const double*)pbuf, (double*)vdf, use_pbuf,
                               homogeneous);
         } else if (precision == PBTE_FP32) {
             if (!ctx->have32) throw std::string("FP32 factors not built; call pbte_factorise");
-            run_sweep<float>(ctx, ctx->lu32, ctx->cxv32, ctx->cyv32, ctx->geo32, ctx->K32, ctx->bvec32, ctx->src32,
-                             ctx->crec32, ts, qxs, qys, flux_wall, (const float*)pbuf, (float*)vdf, use_pbuf,
+            run_sweep<float>(ctx, ts, qxs, qys, flux_wall, (const float*)pbuf, (float*)vdf, use_pbuf,
                              homogeneous);
         } else {
             throw std::string("bad precision");

@@ -135,8 +135,8 @@ class SweepContext:
         out = (C.c_int64 * 6)()
         _native.check(self.handle, _native.lib().pbte_schedule_info(self.handle, out), "schedule")
         return {"levels": int(out[0]), "bundles": int(out[1]), "max_width": int(out[2]),
-                "stash_reads": int(out[3]), "global_bundles": int(out[4]),
-                "ring_slots": int(out[5])}
+                "pool_peak": int(out[3]), "global_bundles": int(out[4]),
+                "pool_slots": int(out[5])}
 
     def timing(self, capacity: int) -> None:
         """Arm HIP events around the next ``capacity`` sweep-kernel launches."""

@@ -64,9 +64,10 @@ typedef struct pbte_setup {
                                              0 = automatic                 */
     int32_t block;                        /* threads per sweep workgroup,
                                              0 = automatic                 */
-    int32_t sweep_variant;                /* 0 pipelined (LDS ring), 1 plain
-                                             level sweep (same results)    */
-    int32_t lds_slots_cap;                /* > 0 caps the LDS ring and stash
+    int32_t sweep_variant;                /* 0 pipelined (LDS trace pool),
+                                             1 plain level sweep (same
+                                             results)                      */
+    int32_t lds_slots_cap;                /* > 0 caps the LDS trace pool
                                              slots (exercises the global
                                              fallback); 0 = automatic      */
 } pbte_setup;
@@ -94,8 +95,9 @@ void pbte_ctx_destroy(pbte_ctx *ctx);
 const char *pbte_last_error(pbte_ctx *ctx);
 
 /* Schedule statistics: levels summed over bundles, bundles, widest level
- * (items), upwind reads served by the LDS stash, bundles that read this
- * sweep's values from global memory, LDS ring slots. */
+ * (items), most LDS trace slots one bundle holds at once, bundles that read
+ * this sweep's values from global memory (their pool would overflow), LDS
+ * trace slots per workgroup. */
 int pbte_schedule_info(pbte_ctx *ctx, int64_t *out6);
 
 /* Sweep-kernel timing with HIP events recorded on the context's stream
