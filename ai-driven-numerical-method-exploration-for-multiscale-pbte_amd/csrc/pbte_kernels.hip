@@ -74,6 +74,7 @@ template <int P> struct Lay {
     // matrix, so no per-face block is stored.
     static constexpr int R_N = 3 * P, R_INFO = 3 * P + 6;
     static constexpr int CR = pad4(3 * P + 9);
+    static constexpr int CRL = pad4(3 * P + 6);  // the part every item loads
 };
 
 // row stride of the cell records: a multiple of 128 bytes, so a record
@@ -176,6 +177,7 @@ struct pbte_ctx {
     int rs64 = 0, rs32 = 0;     // their row strides
     uint8_t* bundle_global = nullptr;
     int pool_slots = 0;         // LDS trace slots per workgroup
+    int depth = 3;              // levels between a gather and its solve, plus one
     int64_t pool_peak = 0;      // most slots one bundle holds at once
     int64_t global_bundles = 0;
     int variant = 0;            // 0 pipelined level sweep, 1 plain level sweep
@@ -725,7 +727,7 @@ template <int P, typename T, bool PFLU> struct PipeData {
     int lds[3];         // LDS offset of each face's upwind trace (the zero slot when none)
     int wv;             // LDS offset of the element's wall inflow (the zero slot when none)
     int wst;            // LDS pool slot this item writes, or -1
-    T cr[L::CR];        // the cell record
+    T cr[L::CRL];       // the cell record without the info words
     T lu[PFLU ? L::LU : 4];
 };
 
@@ -786,7 +788,7 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int32_t
     // the pool slot is only used by the solve: it travels with the
     // gathered data
     o.wst = iwst[(size_t)r.slot * B + b];
-    load16<T, L::CR>(o.cr, a.crec + (size_t)i * a.rs);
+    load16<T, L::CRL>(o.cr, a.crec + (size_t)i * a.rs);
     if constexpr (PFLU) {
         using V = typename Vec16<T>::type;
         constexpr int w = 16 / sizeof(T);
@@ -838,7 +840,8 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
                 if (code_in(gmask, code[il])) {
                     const int ilp = fw_ilp(dC.fw[il]);
                     const T* fe = (code_in(kRecPeriodic, code[il]) && a.use_pbuf)
-                                      ? a.pbuf + ((size_t)d * a.F + (bits_of(cr[L::R_INFO + il]) >> 4)) * P
+                                      ? a.pbuf + ((size_t)d * a.F +
+                                                  (bits_of(a.crec[(size_t)dC.cell * a.rs + L::R_INFO + il]) >> 4)) * P
                                       : vdf_d + (size_t)fw_val(dC.fw[il]) * P;
 #pragma unroll
                     for (int q = 0; q < Q; ++q) u[il][q] = fe[face_node(DEG, ilp, q)];
@@ -894,7 +897,7 @@ __device__ __forceinline__ void level_barrier(bool global_mode)
     }
 }
 
-template <int P, typename T, bool PFLU, bool GREAD>
+template <int P, typename T, bool PFLU, bool GREAD, int DEPTH>
 __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
 {
     constexpr int W = Tr<P>::W;
@@ -934,37 +937,80 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     }
     __syncthreads();
 
-    // level spans travel two levels ahead of their use, the records one
-    // level ahead of the gather; every stage alternates between two named
-    // register sets so that no copy forces a wait.  The spans are read with
-    // per-lane (vector) loads: a scalar load would share lgkmcnt with the
-    // LDS traffic that every level waits for.
+    // The spans are read with per-lane (vector) loads: a scalar load would
+    // share lgkmcnt with the LDS traffic that every level waits for.
+    // A level past the end reads the last level's span and takes (0, 0):
+    // the load is unconditional (a load on one path only leaves the
+    // compiler unsure of the in-order vmcnt and it then waits for every
+    // prefetch in flight).
     const int32_t* lp = a.lev_ptr + (t & a.lane_zero);
-    auto span = [&](int lv) { return lv < l1 ? make_int2(lp[lv], lp[lv + 1]) : make_int2(0, 0); };
-    int2 S0 = span(l0 + 2), S1 = span(l0 + 3);
-    PipeRec R0, R1;
-    PipeData<P, T, PFLU> D0, D1;
-    // the prologue issues its loads in the steady-state order (records of
-    // the level after next before the gather of the next level): vmcnt is
-    // in order, and the loop-entry state must not ask for more waiting
-    pipe_record(irec, span(l0), c, B, b, R0);
-    pipe_record(irec, span(l0 + 1), c, B, b, R1);
-    pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);
-    // two levels per trip, with no exit between them: an odd level count
-    // runs one empty level (span (0, 0)) at the end.  A mid-trip exit would
-    // make the compiler rotate the register sets with copies, and a copy of
-    // a loaded value waits for it.
-    for (int lv = l0; lv < l1; lv += 2) {
-        pipe_record(irec, S0, c, B, b, R0);                            // level lv+2
-        S0 = span(lv + 4);
-        pipe_gather<P, T, PFLU>(a, iwst, R1, B, b, lu, D1);                  // level lv+1
-        pipe_solve<P, T, PFLU, GREAD>(a, D0, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv
-        level_barrier(global_mode);
-        pipe_record(irec, S1, c, B, b, R1);                            // level lv+3
-        S1 = span(lv + 5);
-        pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);                  // level lv+2
-        pipe_solve<P, T, PFLU, GREAD>(a, D1, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+1
-        level_barrier(global_mode);
+    auto span = [&](int lv) {
+        const int k = min(lv, l1 - 1);
+        const int2 v = make_int2(lp[k], lp[k + 1]);
+        return lv < l1 ? v : make_int2(0, 0);
+    };
+    if constexpr (DEPTH == 2) {
+        // level spans travel two levels ahead of their use, the records one
+        // level ahead of the gather, the gather one level ahead of the
+        // solve; every stage alternates between two named register sets so
+        // that no copy forces a wait
+        int2 S0 = span(l0 + 2), S1 = span(l0 + 3);
+        PipeRec R0, R1;
+        PipeData<P, T, PFLU> D0, D1;
+        // the prologue issues its loads in the steady-state order (records
+        // of the level after next before the gather of the next level):
+        // vmcnt is in order, and the loop-entry state must not ask for more
+        // waiting
+        pipe_record(irec, span(l0), c, B, b, R0);
+        pipe_record(irec, span(l0 + 1), c, B, b, R1);
+        pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);
+        // two levels per trip, with no exit between them: an odd level count
+        // runs one empty level (span (0, 0)) at the end.  A mid-trip exit
+        // would make the compiler rotate the register sets with copies, and a
+        // copy of a loaded value waits for it.
+        for (int lv = l0; lv < l1; lv += 2) {
+            pipe_record(irec, S0, c, B, b, R0);                            // level lv+2
+            S0 = span(lv + 4);
+            pipe_gather<P, T, PFLU>(a, iwst, R1, B, b, lu, D1);                  // level lv+1
+            pipe_solve<P, T, PFLU, GREAD>(a, D0, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv
+            level_barrier(global_mode);
+            pipe_record(irec, S1, c, B, b, R1);                            // level lv+3
+            S1 = span(lv + 5);
+            pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);                  // level lv+2
+            pipe_solve<P, T, PFLU, GREAD>(a, D1, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+1
+            level_barrier(global_mode);
+        }
+    } else {
+        // the gather two levels ahead of the solve and the records two levels
+        // ahead of the gather (three register sets each, three levels per
+        // trip): every load has two level times to arrive.  Set k holds
+        // level l0 + k mod 3.
+        int2 S0 = span(l0 + 6), S1 = span(l0 + 4), S2 = span(l0 + 5);
+        PipeRec R0, R1, R2;
+        PipeData<P, T, PFLU> D0, D1, D2;
+        pipe_record(irec, span(l0), c, B, b, R0);
+        pipe_record(irec, span(l0 + 1), c, B, b, R1);
+        pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);
+        pipe_record(irec, span(l0 + 2), c, B, b, R2);
+        pipe_gather<P, T, PFLU>(a, iwst, R1, B, b, lu, D1);
+        pipe_record(irec, span(l0 + 3), c, B, b, R0);
+        for (int lv = l0; lv < l1; lv += 3) {
+            pipe_record(irec, S1, c, B, b, R1);                            // level lv+4
+            S1 = span(lv + 7);
+            pipe_gather<P, T, PFLU>(a, iwst, R2, B, b, lu, D2);                  // level lv+2
+            pipe_solve<P, T, PFLU, GREAD>(a, D0, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv
+            level_barrier(global_mode);
+            pipe_record(irec, S2, c, B, b, R2);                            // level lv+5
+            S2 = span(lv + 8);
+            pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);                  // level lv+3
+            pipe_solve<P, T, PFLU, GREAD>(a, D1, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+1
+            level_barrier(global_mode);
+            pipe_record(irec, S0, c, B, b, R0);                            // level lv+6
+            S0 = span(lv + 9);
+            pipe_gather<P, T, PFLU>(a, iwst, R1, B, b, lu, D1);                  // level lv+4
+            pipe_solve<P, T, PFLU, GREAD>(a, D2, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+2
+            level_barrier(global_mode);
+        }
     }
 }
 
@@ -1624,7 +1670,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->block = su->block > 0 ? su->block : 256;
         if (ctx->block % 64 || ctx->block > 256) throw std::string("block must be a multiple of 64, <= 256");
         ctx->variant = su->sweep_variant;
-        if (ctx->variant != 0 && ctx->variant != 1) throw std::string("sweep_variant must be 0 or 1");
+        if (ctx->variant < 0 || ctx->variant > 3) throw std::string("sweep_variant must be 0-3");
         ctx->items = ctx->block;
         const int deg = Q - 1;
 
@@ -1950,8 +1996,9 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             ctx->max_wall_in = most;
         }
         // LDS pool: what one workgroup may hold when every CU runs its share
-        // of the bundles at once, sized at binary64 (a binary32 sweep uses
-        // half of it)
+        // of the bundles at once, sized at binary32; a binary64 sweep takes
+        // twice the bytes (and by its register use runs one workgroup of up
+        // to 256 threads per CU in any case)
         {
             int dev = 0, ncu = 1, lds_blk = 65536, lds_cu = 0;
             HIP_TRY(hipGetDevice(&dev));
@@ -1961,9 +2008,10 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                     hipSuccess || lds_cu <= 0)
                 lds_cu = lds_blk;
             const int per_cu = std::max(1, (nb + std::max(ncu, 1) - 1) / std::max(ncu, 1));
-            const size_t budget = std::min<size_t>((size_t)lds_blk, (size_t)lds_cu / per_cu);
-            const size_t slot_bytes = (size_t)bm * pad4(3 * TQ) * 8;
-            const size_t wall_bytes = (size_t)ctx->max_wall_in * bm * PV * 8;
+            const size_t budget = std::min<size_t>((size_t)lds_blk / 2, (size_t)lds_cu / per_cu);
+            const size_t slot_bytes = (size_t)bm * pad4(3 * TQ) * 4;
+            const size_t wall_bytes = (size_t)ctx->max_wall_in * bm * PV * 4;
+            ctx->depth = ctx->variant >= 2 ? ctx->variant : 3;
             if (budget < wall_bytes + 4 * slot_bytes) throw std::string("LDS too small for the wall traces");
             int pool = (int)((budget - wall_bytes) / slot_bytes) - 2;
             if (su->lds_slots_cap > 0) pool = std::min(pool, su->lds_slots_cap);
@@ -2013,7 +2061,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 }
             });
         }
-        if (ctx->variant == 0 && su->block <= 0) {
+        if (ctx->variant != 1 && su->block <= 0) {
             // the pipelined kernel runs one wave per 64 items of the widest
             // level (every wave issues the full load stream of a level)
             int64_t widest = 1;
@@ -2333,10 +2381,10 @@ template <int P, typename T> size_t sweep_lds_bytes(const pbte_ctx* ctx)
            sizeof(T);
 }
 
-template <int P, typename T, bool PF, bool GR>
+template <int P, typename T, bool PF, bool GR, int DEPTH>
 void launch_pipe(pbte_ctx* ctx, const SweepArgs<T>& a, size_t lds)
 {
-    auto* fn = sweep_pipe_kernel<P, T, PF, GR>;
+    auto* fn = sweep_pipe_kernel<P, T, PF, GR, DEPTH>;
     if (lds > 65536)
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
@@ -2371,8 +2419,13 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
         const size_t lds = sweep_lds_bytes<P, T>(ctx);
         constexpr bool pf = P * P * sizeof(T) <= 288;
         const bool gread = ctx->global_bundles > 0 || ctx->old_reads > 0 || (use_pbuf && ctx->periodic_reads > 0);
-        if (gread) launch_pipe<P, T, pf, true>(ctx, a, lds);
-        else launch_pipe<P, T, pf, false>(ctx, a, lds);
+        if (ctx->depth == 3) {
+            if (gread) launch_pipe<P, T, pf, true, 3>(ctx, a, lds);
+            else launch_pipe<P, T, pf, false, 3>(ctx, a, lds);
+        } else {
+            if (gread) launch_pipe<P, T, pf, true, 2>(ctx, a, lds);
+            else launch_pipe<P, T, pf, false, 2>(ctx, a, lds);
+        }
     }
     HIP_TRY(hipGetLastError());
     if (timed) {

@@ -121,6 +121,9 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--bundle", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="sweep kernel: 0 default, 1 plain level sweep, 2 / 3 pipelined with the "
+                         "gather 1 / 2 levels ahead")
     ap.add_argument("--jitter", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -143,7 +146,7 @@ def main():
     case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, 1.0, 0.1, args.precision,
                      work, jitter=args.jitter)
     t0 = time.perf_counter()
-    s = pkg.Solver(case, max_bundle=args.bundle, block=args.block)
+    s = pkg.Solver(case, max_bundle=args.bundle, block=args.block, sweep_variant=args.variant)
     setup_s = time.perf_counter() - t0
     D, N, P = s.ndir, s.n_tris, s.ndof
     vdf = s.sweep_vdf

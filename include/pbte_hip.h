@@ -64,9 +64,12 @@ typedef struct pbte_setup {
                                              0 = automatic                 */
     int32_t block;                        /* threads per sweep workgroup,
                                              0 = automatic                 */
-    int32_t sweep_variant;                /* 0 pipelined (LDS trace pool),
-                                             1 plain level sweep (same
-                                             results)                      */
+    int32_t sweep_variant;                /* 0 pipelined (LDS trace pool,
+                                             gather 2 levels ahead), 1
+                                             plain level sweep, 2 / 3
+                                             pipelined with the gather 1 /
+                                             2 levels ahead (all give the
+                                             same bits)                    */
     int32_t lds_slots_cap;                /* > 0 caps the LDS trace pool
                                              slots (exercises the global
                                              fallback); 0 = automatic      */

@@ -168,9 +168,10 @@ def test_jittered_mesh_sweep(pkg, tmp_path):
     assert _rel(s.vdf, o.vdf) <= 1e-12
 
 
-@pytest.mark.parametrize("tag,bundle,cap", [("F1_1_1", 1, 0), ("F2_1_1", 2, 0), ("F3_1_1", 4, 0),
-                                           ("F1_1_1", 1, 16), ("F3_1_1", 2, 2)])
-def test_pipelined_sweep_is_bitwise_plain_sweep(pkg, tag, bundle, cap):
+@pytest.mark.parametrize("tag,bundle,cap,depth", [("F1_1_1", 1, 0, 3), ("F2_1_1", 2, 0, 3), ("F3_1_1", 4, 0, 3),
+                                                 ("F2_1_1", 1, 0, 2), ("F3_1_1", 2, 0, 2),
+                                                 ("F1_1_1", 1, 16, 3), ("F3_1_1", 2, 2, 2)])
+def test_pipelined_sweep_is_bitwise_plain_sweep(pkg, tag, bundle, cap, depth):
     """The pipelined kernel (LDS trace pool, or the global fallback when a
     capped pool overflows) computes the plain level kernel's values bit for
     bit, in both precisions."""
@@ -178,10 +179,10 @@ def test_pipelined_sweep_is_bitwise_plain_sweep(pkg, tag, bundle, cap):
     for prec in ("fp64", "fp32"):
         c = _case(pkg, tag, performance__precision=prec)
         outs = []
-        for variant in (0, 1):
+        for variant in (depth, 1):
             print(f"case {tag} {prec} variant {variant} bundle {bundle} cap {cap}", flush=True)
             s = pkg.Solver(c, max_bundle=bundle, sweep_variant=variant, lds_slots_cap=cap)
-            if variant == 0 and cap:
+            if variant != 1 and cap:
                 info = s.ctx.schedule_info()
                 print(info, flush=True)
                 assert info["pool_slots"] <= cap and info["global_bundles"] > 0
