@@ -186,6 +186,10 @@ class Performance:
     kernel: str = "hip"
     threads: int = 0
     precision: str = "fp64"
+    # fp32 source iteration (CIS / GSIS): binary32 sweeps until the iterate
+    # step falls below this (or stalls), then binary64 sweeps to the
+    # tolerance; an addition of this build
+    refine_switch: float = 1.0e-6
 
     def __post_init__(self):
         self.precompute_inverse = bool(self.precompute_inverse)
@@ -200,6 +204,9 @@ class Performance:
         self.precision = str(self.precision).lower()
         if self.precision not in ("fp64", "fp32"):
             raise ValueError("performance.precision must be 'fp64' or 'fp32'")
+        self.refine_switch = float(self.refine_switch)
+        if not self.refine_switch > 0.0:
+            raise ValueError("performance.refine_switch must be > 0")
 
 
 @dataclass

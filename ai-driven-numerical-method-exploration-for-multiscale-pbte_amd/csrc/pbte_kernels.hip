@@ -265,6 +265,97 @@ struct FactorArgs {
     int32_t* singular;
 };
 
+// A_SOL of element i for the direction (cx, cy) (_kernels.py:106-125) and
+// its DGETF2 factorisation with the reference's pivoting (_kernels.py:43-76),
+// in binary64; the diagonal holds reciprocals (the solve multiplies by
+// them).  Returns 0 or the first zero pivot (1-based), like INFO.
+template <int P>
+__device__ __forceinline__ int element_lu(const double* __restrict__ mass, const double* __restrict__ gx,
+                                          const double* __restrict__ gy, const double* __restrict__ fcm,
+                                          const CellGeo<double>& g, double inv_tau_c, int N, int i, double cx,
+                                          double cy, double (&A)[P][P], uint64_t& pv)
+{
+    const double* fc = fcm + (size_t)i * 3 * P * P;
+    double w[3];
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        double sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
+        w[il] = 0.5 * (sp + fabs(sp));
+    }
+#pragma unroll
+    for (int l = 0; l < P; ++l)
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            double v = (inv_tau_c * mass[((size_t)m * P + l) * N + i] -
+                        cx * gx[((size_t)i * P + m) * P + l] - cy * gy[((size_t)i * P + m) * P + l]);
+            v = v + w[0] * fc[(0 * P + m) * P + l];
+            v = v + w[1] * fc[(1 * P + m) * P + l];
+            v = v + w[2] * fc[(2 * P + m) * P + l];
+            A[m][l] = v;
+        }
+    pv = 0;
+    int info = 0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        int p = k;
+        double big = fabs(A[k][k]);
+#pragma unroll
+        for (int r = k + 1; r < P; ++r) {
+            double v = fabs(A[r][k]);
+            if (v > big) { big = v; p = r; }
+        }
+        pv |= (uint64_t)p << (4 * k);
+#pragma unroll
+        for (int r = k + 1; r < P; ++r) {
+            if (p == r) {
+#pragma unroll
+                for (int j = 0; j < P; ++j) {
+                    double t = A[k][j]; A[k][j] = A[r][j]; A[r][j] = t;
+                }
+            }
+        }
+        if (A[k][k] == 0.0) {
+            if (info == 0) info = k + 1;
+            continue;
+        }
+        const double rcp = 1.0 / A[k][k];
+#pragma unroll
+        for (int r = k + 1; r < P; ++r) A[r][k] = A[r][k] * rcp;
+#pragma unroll
+        for (int r = k + 1; r < P; ++r) {
+            const double ark = A[r][k];
+#pragma unroll
+            for (int j = k + 1; j < P; ++j) A[r][j] -= ark * A[k][j];
+        }
+        A[k][k] = rcp;
+    }
+    return info;
+}
+
+// v = inv(A) v from element_lu's factors, DGETRS's loop order
+// (_kernels.py:78-101): row interchanges, unit lower, then upper
+template <int P>
+__device__ __forceinline__ void element_lu_solve(const double (&A)[P][P], uint64_t pv, double (&v)[P])
+{
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        const int p = (int)((pv >> (4 * k)) & 15u);
+#pragma unroll
+        for (int r = k + 1; r < P; ++r)
+            if (p == r) { double t = v[k]; v[k] = v[r]; v[r] = t; }
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+#pragma unroll
+        for (int r = k + 1; r < P; ++r) v[r] -= v[k] * A[r][k];
+#pragma unroll
+    for (int k = P - 1; k >= 0; --k) {
+        v[k] = v[k] * A[k][k];
+#pragma unroll
+        for (int r = 0; r < k; ++r) v[r] -= v[k] * A[r][k];
+    }
+}
+
 template <int P>
 __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
 {
@@ -275,65 +366,12 @@ __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
     const int i = a.slot_cell[(size_t)bundle * a.N + s];
     const int N = a.N;
     const CellGeo<double> g = a.geo[i];
-    const double* fc = a.fcm + (size_t)i * 3 * P * P;
     for (int b = 0; b < B; ++b) {
         const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
-        const double cx = a.cxv[d], cy = a.cyv[d];
-        double w[3];
-#pragma unroll
-        for (int il = 0; il < 3; ++il) {
-            double sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
-            w[il] = 0.5 * (sp + fabs(sp));
-        }
         double A[P][P];
-#pragma unroll
-        for (int l = 0; l < P; ++l)
-#pragma unroll
-            for (int m = 0; m < P; ++m) {
-                double v = (a.inv_tau_c * a.mass[((size_t)m * P + l) * N + i] -
-                            cx * a.gx[((size_t)i * P + m) * P + l] - cy * a.gy[((size_t)i * P + m) * P + l]);
-                v = v + w[0] * fc[(0 * P + m) * P + l];
-                v = v + w[1] * fc[(1 * P + m) * P + l];
-                v = v + w[2] * fc[(2 * P + m) * P + l];
-                A[m][l] = v;
-            }
-        uint64_t pv = 0;
-        int info = 0;
-#pragma unroll
-        for (int k = 0; k < P; ++k) {
-            int p = k;
-            double big = fabs(A[k][k]);
-#pragma unroll
-            for (int r = k + 1; r < P; ++r) {
-                double v = fabs(A[r][k]);
-                if (v > big) { big = v; p = r; }
-            }
-            pv |= (uint64_t)p << (4 * k);
-#pragma unroll
-            for (int r = k + 1; r < P; ++r) {
-                if (p == r) {
-#pragma unroll
-                    for (int j = 0; j < P; ++j) {
-                        double t = A[k][j]; A[k][j] = A[r][j]; A[r][j] = t;
-                    }
-                }
-            }
-            if (A[k][k] == 0.0) {
-                if (info == 0) info = k + 1;
-                continue;
-            }
-            const double rcp = 1.0 / A[k][k];
-#pragma unroll
-            for (int r = k + 1; r < P; ++r) A[r][k] = A[r][k] * rcp;
-#pragma unroll
-            for (int r = k + 1; r < P; ++r) {
-                const double ark = A[r][k];
-#pragma unroll
-                for (int j = k + 1; j < P; ++j) A[r][j] -= ark * A[k][j];
-            }
-            A[k][k] = rcp;   // the solve multiplies by the reciprocal
-        }
-        if (info) atomicAdd(a.singular, 1);
+        uint64_t pv;
+        if (element_lu<P>(a.mass, a.gx, a.gy, a.fcm, g, a.inv_tau_c, N, i, a.cxv[d], a.cyv[d], A, pv))
+            atomicAdd(a.singular, 1);
         // the inverse, column by column from the pivoted factors (DGETRS on
         // the unit vectors): the sweep applies x = inv(A) b
         double Ai[P][P];
@@ -342,23 +380,7 @@ __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
             double v[P];
 #pragma unroll
             for (int k = 0; k < P; ++k) v[k] = k == j ? 1.0 : 0.0;
-#pragma unroll
-            for (int k = 0; k < P; ++k) {
-                const int p = (int)((pv >> (4 * k)) & 15u);
-#pragma unroll
-                for (int r = k + 1; r < P; ++r)
-                    if (p == r) { double t = v[k]; v[k] = v[r]; v[r] = t; }
-            }
-#pragma unroll
-            for (int k = 0; k < P; ++k)
-#pragma unroll
-                for (int r = k + 1; r < P; ++r) v[r] -= v[k] * A[r][k];
-#pragma unroll
-            for (int k = P - 1; k >= 0; --k) {
-                v[k] = v[k] * A[k][k];
-#pragma unroll
-                for (int r = 0; r < k; ++r) v[r] -= v[k] * A[r][k];
-            }
+            element_lu_solve<P>(A, pv, v);
 #pragma unroll
             for (int r = 0; r < P; ++r) Ai[r][j] = v[r];
         }
@@ -448,6 +470,10 @@ template <typename T> struct SweepArgs {
     int pool;                   // LDS pool slots
     int wbase;                  // LDS offset (in values) of the wall inflow vectors
     T kref[25];                 // Kr, Q x Q, row major
+    // the on-the-fly sweep: element tensors and binary64 directions
+    const double *mass, *gx, *gy, *fcm, *cxv64, *cyv64;
+    const CellGeo<double>* geo;
+    double inv_tau_c;
 };
 
 // Face word of an item record: code in bits 28-31, the local face of the
@@ -634,6 +660,86 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
             T* o = vdf_d + (size_t)i * P;
 #pragma unroll
             for (int m = 0; m < P; ++m) o[m] = x[m];
+        }
+        __syncthreads();
+    }
+}
+
+
+// the on-the-fly sweep (_kernels.py:242-269): the plain level schedule
+// with every element operator built and factorised in binary64 where it is
+// used (element_lu), so no factors are stored; the inflow right-hand side
+// is assembled exactly as in the other sweep kernels, the solve follows
+// DGETRS's loop order
+template <int P, typename T>
+__global__ void __launch_bounds__(256) sweep_otf_kernel(SweepArgs<T> a)
+{
+    using L = Lay<P>;
+    constexpr int DEG = Deg<P>::v;
+    constexpr int Q = DEG + 1;
+    constexpr int TQ = Tr<P>::TQ;
+    const int bundle = a.block_bundle[blockIdx.x];
+    const int B = a.bundle_ndir[bundle];
+    const int t = threadIdx.x;
+    const int b = t % B;
+    const int c = t / B;
+    const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
+    const T cx = a.cxv[d], cy = a.cyv[d];
+    const double cx64 = a.cxv64[d], cy64 = a.cyv64[d];
+    const int N = a.N;
+    const int4* irec = a.irec + a.item_off[bundle];
+    const int4* wc = a.wcell + a.wc_off[bundle];
+    T* vdf_d = a.vdf + (size_t)d * N * P;
+    const T thermal_on = a.homogeneous ? T(0) : T(1);
+    const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
+
+    for (int lv = l0; lv < l1; ++lv) {
+        const int s = a.lev_ptr[lv] + c;
+        if (s < a.lev_ptr[lv + 1]) {
+            const int4 r = irec[(size_t)s * B + b];
+            const int i = r.x;
+            const int fw[3] = {r.y, r.z, r.w};
+            const T* cr = a.crec + (size_t)i * a.rs;
+            double A[P][P];
+            uint64_t pv;
+            (void)element_lu<P>(a.mass, a.gx, a.gy, a.fcm, a.geo[i], a.inv_tau_c, N, i, cx64, cy64, A, pv);
+            int wk = -1;
+#pragma unroll
+            for (int il = 0; il < 3; ++il)
+                if (code_in(kRecWall, fw_code(fw[il]))) wk = fw_val(fw[il]);
+            T x[P];
+            if (wk >= 0) {
+                const int4 e = wc[wk];
+#pragma unroll
+                for (int m = 0; m < P; ++m)
+                    x[m] = source_plus<P, T>(cr, cx, cy, m, wall_inflow<P, T>(cr, e, cx, cy, a.wtr, thermal_on, m));
+            } else {
+#pragma unroll
+                for (int m = 0; m < P; ++m) x[m] = source_plus<P, T>(cr, cx, cy, m, T(0));
+            }
+#pragma unroll
+            for (int il = 0; il < 3; ++il) {
+                const int code = fw_code(fw[il]);
+                if (!code_in(kRecNeighbour, code)) continue;
+                const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
+                const T w = T(0.5) * (sp - fabs(sp));
+                const bool per = code_in(kRecPeriodic, code);
+                const T* fe = (per && a.use_pbuf)
+                                  ? a.pbuf + ((size_t)d * a.F + (bits_of(cr[L::R_INFO + il]) >> 4)) * P
+                                  : vdf_d + (size_t)a.other[(size_t)i * 3 + il] * P;
+                const int ilp = fw_ilp(fw[il]);
+                T u[TQ];
+#pragma unroll
+                for (int q = 0; q < TQ; ++q) u[q] = q < Q ? fe[face_node(DEG, ilp, q < Q ? q : 0)] : T(0);
+                face_update<P, T>(x, il, w, a.kref, u);
+            }
+            double v[P];
+#pragma unroll
+            for (int m = 0; m < P; ++m) v[m] = (double)x[m];
+            element_lu_solve<P>(A, pv, v);
+            T* o = vdf_d + (size_t)i * P;
+#pragma unroll
+            for (int m = 0; m < P; ++m) o[m] = (T)v[m];
         }
         __syncthreads();
     }
@@ -2392,7 +2498,7 @@ void launch_pipe(pbte_ctx* ctx, const SweepArgs<T>& a, size_t lds)
 }
 
 template <int P, typename T>
-void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous)
+void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous, bool onthefly)
 {
     using S = SweepData<T>;
     SweepArgs<T> a{};
@@ -2411,9 +2517,13 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
     a.pool = ctx->pool_slots;
     a.wbase = (ctx->pool_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W;
     for (int k = 0; k < ctx->Q * ctx->Q; ++k) a.kref[k] = S::kref(ctx)[k];
+    a.mass = ctx->mass; a.gx = ctx->gx; a.gy = ctx->gy; a.fcm = ctx->fcm; a.geo = ctx->geo64;
+    a.cxv64 = ctx->cxv; a.cyv64 = ctx->cyv; a.inv_tau_c = 1.0 / ctx->tau_c;
     const bool timed = 2 * (size_t)ctx->ev_used + 1 < ctx->events.size();
     if (timed) HIP_TRY(hipEventRecord(ctx->events[2 * ctx->ev_used], ctx->stream));
-    if (ctx->variant == 1) {
+    if (onthefly) {
+        hipLaunchKernelGGL((sweep_otf_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
+    } else if (ctx->variant == 1) {
         hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
     } else {
         const size_t lds = sweep_lds_bytes<P, T>(ctx);
@@ -2441,7 +2551,7 @@ void launch_source(pbte_ctx* ctx, const double* ts, const double* qxs, const dou
 // from flux_wall, then the sweep
 template <typename T>
 void run_sweep(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, const double* flux_wall,
-               const T* pbuf, T* vdf, int use_pbuf, int homogeneous)
+               const T* pbuf, T* vdf, int use_pbuf, int homogeneous, bool onthefly = false)
 {
     using S = SweepData<T>;
     const int P = ctx->P;
@@ -2456,7 +2566,7 @@ void run_sweep(pbte_ctx* ctx, const double* ts, const double* qxs, const double*
         HIP_TRY(hipGetLastError());
     }
     switch (P) {
-#define SW(PP) case PP: launch_sweep<PP, T>(ctx, pbuf, vdf, use_pbuf, homogeneous); break;
+#define SW(PP) case PP: launch_sweep<PP, T>(ctx, pbuf, vdf, use_pbuf, homogeneous, onthefly); break;
     SW(3) SW(6) SW(10) SW(15)
 #undef SW
     }
@@ -2524,6 +2634,25 @@ extern "C" int pbte_sweep(pbte_ctx* ctx, int32_t precision, const double* ts, co
         } else {
             throw std::string("bad precision");
         }
+        return 0;
+    });
+}
+
+extern "C" int pbte_sweep_onthefly(pbte_ctx* ctx, int32_t precision, const double* ts, const double* qxs,
+                                   const double* qys, void* vdf, const double* flux_wall, const void* pbuf,
+                                   int32_t use_pbuf, int32_t homogeneous)
+{
+    return guarded(ctx, [&]() {
+        if (use_pbuf && !pbuf) throw std::string("use_pbuf without a buffer");
+        if (ctx->n_wall && !flux_wall) throw std::string("adiabatic walls need flux_wall");
+        if (precision == PBTE_FP64)
+            run_sweep<double>(ctx, ts, qxs, qys, flux_wall, (const double*)pbuf, (double*)vdf, use_pbuf,
+                              homogeneous, true);
+        else if (precision == PBTE_FP32)
+            run_sweep<float>(ctx, ts, qxs, qys, flux_wall, (const float*)pbuf, (float*)vdf, use_pbuf,
+                             homogeneous, true);
+        else
+            throw std::string("bad precision");
         return 0;
     });
 }

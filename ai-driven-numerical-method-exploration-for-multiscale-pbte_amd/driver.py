@@ -50,6 +50,7 @@ class Solver:
                                       variant=sweep_variant, lds_slots_cap=lds_slots_cap)
         self.factorisation_count = self.ndir * self.n_tris
         self.precision = c.performance.precision
+        self.sweep_precision = self.precision     # fp32 runs finish in fp64
         self.vdf_d = dev_zeros((self.ndir, self.n_tris, self.ndof))
         self.vdf32_d = (dev_zeros((self.ndir, self.n_tris, self.ndof), dtype=torch.float32)
                         if self.precision == "fp32" else None)
@@ -67,8 +68,14 @@ class Solver:
     # -- state views ---------------------------------------------------------
     @property
     def sweep_vdf(self):
-        """The device tensor the sweeps run on (binary32 in fp32 mode)."""
-        return self.vdf32_d if self.precision == "fp32" else self.vdf_d
+        """The device tensor the sweeps run on (binary32 in the fp32 phase)."""
+        return self.vdf32_d if self.sweep_precision == "fp32" else self.vdf_d
+
+    def promote(self) -> None:
+        """Continue with binary64 sweeps from the binary32 distribution."""
+        if self.sweep_precision == "fp32":
+            self.vdf_d.copy_(self.vdf32_d)
+            self.sweep_precision = "fp64"
 
     @property
     def vdf(self) -> np.ndarray:
@@ -145,6 +152,11 @@ class Solver:
         t0 = time.perf_counter()
         step = 0
         converged = False
+        # fp32: binary32 sweeps until the iterate step is below the switch
+        # level or has stopped falling (it floors at the binary32 rounding
+        # of the sweep), then binary64 sweeps to the tolerance
+        switch = max(tol, c.performance.refine_switch)
+        step32 = 0
         while True:
             step += 1
             self.step()
@@ -159,6 +171,17 @@ class Solver:
                 print(f"  step {step:8d}  residual {res:.6e}  mass {mass:.10f}", flush=True)
             if callback is not None:
                 callback(step, res, self)
+            if self.sweep_precision == "fp32":
+                step32 += 1
+                # no new minimum over 8 steps: at the rounding floor (a slow
+                # but steady decrease keeps making new minima)
+                stalled = step32 > 8 and res >= min(residuals[-9:-1])
+                if res < switch or stalled:
+                    self.promote()
+                    step32 = step
+                if step >= tmax:
+                    break
+                continue
             if res < tol:
                 if (self.acc is not None and self.acc.omega and not self.acc.defect_every
                         and self.acc.refresh_defect() > tol):
@@ -169,9 +192,13 @@ class Solver:
                 break
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        extra = None
+        if self.precision == "fp32":
+            extra = {"steps_fp32": step32 if self.sweep_precision == "fp64" else step,
+                     "refine_switch": switch}
         return self._record(step, converged, residuals, masses,
                             true_res if want_true else None, wall,
-                            scheme="GSIS" if c.scheme.accflag == 1 else "CIS")
+                            scheme="GSIS" if c.scheme.accflag == 1 else "CIS", extra=extra)
 
     def _record(self, iterations, converged, residuals, masses, true_res, wall, scheme,
                 extra=None) -> RunRecord:

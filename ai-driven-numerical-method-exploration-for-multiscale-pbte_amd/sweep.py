@@ -65,6 +65,11 @@ class SweepContext:
     def build(cls, case, mesh, integrals, vel, sweep_order, bcdata, precompute=None,
               max_bytes: float | None = None, max_bundle: int = 0, block: int = 0,
               variant: int = 0, lds_slots_cap: int = 0):
+        """``precompute`` (default ``performance.precompute_inverse``) and
+        ``max_bytes`` choose the mode as the reference does (sweep.py:104-110):
+        precomputed factors when wanted and their bytes fit, else "onthefly".
+        The default cap is half of the device memory (the reference's 8e9
+        is sized for host RAM)."""
         self = cls()
         cxv, cyv, domega = vel.flat()
         self.cxv, self.cyv, self.domega = cxv, cyv, domega
@@ -114,9 +119,19 @@ class SweepContext:
         self.handle = h
         self.flux_wall = dev_zeros((max(self.n_faces, 1), self.nd))
         self.precision = case.performance.precision
-        self.factorise("fp64")
+        if precompute is None:
+            precompute = case.performance.precompute_inverse
+        if max_bytes is None:
+            max_bytes = 0.5 * torch.cuda.get_device_properties(0).total_memory
+        nbytes = self.operator_bytes(self.n_tris, self.nd)
         if self.precision == "fp32":
-            self.factorise("fp32")
+            nbytes *= 1.5                       # binary64 and binary32 factors
+        if precompute and nbytes <= max_bytes:
+            self.factorise("fp64")
+            if self.precision == "fp32":
+                self.factorise("fp32")
+        else:
+            self.mode = "onthefly"
         return self
 
     def __del__(self):
@@ -177,9 +192,9 @@ class SweepContext:
         pb = None
         if self.use_pbuf:
             pb = self.ensure_pbuf(vdf.dtype)
-        st = _native.lib().pbte_sweep(self.handle, prec, ptr(ts), ptr(qxs), ptr(qys), ptr(vdf),
-                                      ptr(fw), ptr(pb), 1 if self.use_pbuf else 0,
-                                      1 if homogeneous else 0)
+        fn = _native.lib().pbte_sweep if self.mode == "precomputed" else _native.lib().pbte_sweep_onthefly
+        st = fn(self.handle, prec, ptr(ts), ptr(qxs), ptr(qys), ptr(vdf), ptr(fw), ptr(pb),
+                1 if self.use_pbuf else 0, 1 if homogeneous else 0)
         _native.check(self.handle, st, "sweep")
         self.sweep_count += 1
 

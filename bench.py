@@ -15,11 +15,21 @@ Prints one JSON line:
                  around each launch; `traffic` is the HBM bytes per launch
                  from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
                  (profiles/sweep_traffic.json), null for other workloads;
-  solve          time-to-solution and solves/hour of the FP64 and the
-                 FP32-sweep + FP64-refinement Krylov solves, and the relative
-                 L2 differences of T and heat flux between them (--no-solve
+  solve          per Kn_R in {0.01, 0.1, 1, 10} (Kn_N = 0.1 Kn_R): time-to-
+                 solution and solves/hour of the FP64 and the FP32-sweep +
+                 FP64-refinement Krylov solves, and the relative L2
+                 differences of T and heat flux between them (--no-solve
                  skips it);
   cpu_baseline   the C restatement of the reference's sweep on this host.
+
+--workload gsis runs the GSIS configuration instead: jittered triangulated
+square (interior vertices moved by U(-0.25h, 0.25h), seed 0), P2, 512
+ordinates (16 x 32), Kn_R = 1e-3, Kn_N = 1e-4, west wall dT = +1, east wall
+0, north and south walls diffuse adiabatic; one FP64 and one FP32-sweep
+(binary64 finish) GSIS solve to the iterate step 1e-8.  value is the
+solves/hour of the FP32 solve; roofline is the sweep kernel inside those
+solves.  The default mesh is 128 x 128: the 256 x 256 trace system (1.77 M
+unknowns) does not fit the host SuperLU factorisation.
 """
 from __future__ import annotations
 
@@ -76,6 +86,89 @@ def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitte
     }, basedir=str(workdir))
 
 
+def gsis_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitter=0.25, seed=0):
+    from pbte_amd.mesh import write_square_mesh
+    mesh = Path(workdir) / f"square_{n}_{jitter}_{seed}.msh"
+    if not mesh.exists():
+        write_square_mesh(mesh, n, jitter=jitter, seed=seed)
+    return pkg.Case.from_dict({
+        "iteration": {"tol": 1e-8, "tmax": 5000},
+        "scheme": {"accflag": 1, "method": "source"},
+        "velmesh": {"npole": npole, "nazim": nazim},
+        "dg": {"deg": deg},
+        "flow": {"cv": 1.0, "vg": 1.0, "tau_r": tau_r, "tau_n": tau_n},
+        "mesh": {"file": str(mesh)},
+        "boundaries": [
+            {"name": "SWall", "phyid": 11, "type": "adiabatic"},
+            {"name": "NWall", "phyid": 12, "type": "adiabatic"},
+            {"name": "EWall", "phyid": 13, "type": "thermalising", "temp": 0.0},
+            {"name": "WWall", "phyid": 14, "type": "thermalising", "temp": 1.0},
+        ],
+        "restart": {"enabled": False, "error_on_stale": False},
+        "output": {"field": False, "runtime_log": False, "run_record": False},
+        "performance": {"precision": precision},
+    }, basedir=str(workdir))
+
+
+def gsis_workload(pkg, args, work):
+    """FP64 and FP32-sweep GSIS solves of the same case; the sweep kernel
+    is timed with HIP events over every sweep of each solve."""
+    import torch
+    n = args.n
+    res, fields = {}, {}
+    for prec in ("fp64", "fp32"):
+        case = gsis_case(pkg, n, args.deg, args.npole, args.nazim, 1e-3, 1e-4, prec, work,
+                         jitter=args.jitter if args.jitter else 0.25)
+        print(f"gsis {prec}: setup", flush=True)
+        t0 = time.perf_counter()
+        s = pkg.Solver(case)
+        setup = time.perf_counter() - t0
+        s.ctx.timing(case.iteration.tmax)
+        print(f"gsis {prec}: run", flush=True)
+        t1 = time.perf_counter()
+        rec = s.run(progress_every=10)
+        torch.cuda.synchronize()
+        run = time.perf_counter() - t1
+        ms = s.ctx.timing_read(case.iteration.tmax)
+        s.ctx.timing(0)
+        fields[prec] = (rec.temp, np.concatenate([rec.qx, rec.qy]))
+        D, N, P = s.ndir, s.n_tris, s.ndof
+        k32 = int(rec.diagnostics.get("steps_fp32", 0))
+        res[prec] = {"time_to_solution_s": setup + run, "setup_s": setup, "run_s": run,
+                     "solves_per_hour": 3600.0 / (setup + run), "iterations": rec.iterations,
+                     "converged": rec.converged, "mass": rec.mass, "steps_fp32": k32,
+                     "sweeps": len(ms),
+                     "sweep_kernel_ms_fp32_mean": float(np.mean(ms[:k32])) if k32 else None,
+                     "sweep_kernel_ms_fp64_mean": float(np.mean(ms[k32:])) if len(ms) > k32 else None,
+                     "sweep_kernel_s_total": float(np.sum(ms)) * 1e-3,
+                     "trace_unknowns": rec.diagnostics.get("global_matrix_n"),
+                     "trace_ordering": rec.diagnostics.get("trace_ordering")}
+        del s
+        torch.cuda.empty_cache()
+    (t64, q64), (t32, q32) = fields["fp64"], fields["fp32"]
+    res["fp32"]["rel_l2_T_vs_fp64"] = float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))
+    res["fp32"]["rel_l2_q_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
+    res["speedup_fp32_vs_fp64"] = res["fp64"]["time_to_solution_s"] / res["fp32"]["time_to_solution_s"]
+    res["speedup_fp32_vs_fp64_run"] = res["fp64"]["run_s"] / res["fp32"]["run_s"]
+    t32 = res["fp32"]["sweep_kernel_ms_fp32_mean"] or res["fp32"]["sweep_kernel_ms_fp64_mean"]
+    bytes_alg = b_alg(D, N, P, 4, 4)
+    workload = (f"gsis jittered square {n}x{n} -> {N} P{args.deg} triangles, {D} ordinates "
+                f"({args.npole}x{args.nazim}), Kn_R 1e-3, Kn_N 1e-4")
+    out = {
+        "metric": METRIC, "value": res["fp32"]["solves_per_hour"], "unit": "solves/hour",
+        "n_gpus": 1, "steps": res["fp32"]["iterations"], "warmup": 0,
+        "ms_per_step": 1e3 * res["fp32"]["run_s"] / max(res["fp32"]["iterations"], 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: the workload's mesh, ordinates and boundary data",
+        "config": {"workload": workload, "n_tris": N, "ndof": P, "ndir": D},
+        "roofline": {"bound": "hbm", "achieved": bytes_alg / (t32 * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": bytes_alg / (t32 * 1e-3) / HBM_PEAK, "traffic": None,
+                     "bytes_alg_per_launch": bytes_alg},
+        "gsis": res, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def b_alg(D, N, P, s_lu, s_f):
     """Algorithmic HBM bytes of one precomputed-LU sweep: per (cell, dir) the
     LU (P^2 s_lu), pivots (P bytes), the distribution write and one upwind
@@ -115,6 +208,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "gsis"])
     ap.add_argument("--deg", type=int, default=2)
     ap.add_argument("--npole", type=int, default=8)
     ap.add_argument("--nazim", type=int, default=32)
@@ -143,6 +237,11 @@ def main():
     import torch
 
     work = Path(tempfile.mkdtemp(prefix="pbte_bench_"))
+    if args.workload == "gsis":
+        if args.npole == 8 and args.nazim == 32:
+            args.npole, args.nazim = 16, 32
+        gsis_workload(pkg, args, work)
+        return
     case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, 1.0, 0.1, args.precision,
                      work, jitter=args.jitter)
     t0 = time.perf_counter()
@@ -216,24 +315,30 @@ def calibrate_copy(nbytes, reps):
 
 
 def solve_times(pkg, args, work):
-    res = {}
-    for prec in ("fp64", "fp32"):
-        case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, 1.0, 0.1, prec, work,
-                         jitter=args.jitter)
-        t0 = time.perf_counter()
-        s = pkg.Solver(case)
-        rec = s.run()
-        tts = time.perf_counter() - t0
-        res[prec] = {"time_to_solution_s": tts, "setup_s": s.setup_seconds,
-                     "iterations": rec.iterations, "converged": rec.converged,
-                     "solves_per_hour": 3600.0 / tts, "temp": rec.temp,
-                     "q": np.concatenate([rec.qx, rec.qy])}
-    t64, t32 = res["fp64"].pop("temp"), res["fp32"].pop("temp")
-    q64, q32 = res["fp64"].pop("q"), res["fp32"].pop("q")
-    res["fp32"]["rel_l2_T_vs_fp64"] = float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))
-    res["fp32"]["rel_l2_q_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
-    res["speedup_fp32_refine_vs_fp64"] = res["fp64"]["time_to_solution_s"] / res["fp32"]["time_to_solution_s"]
-    return res
+    """Krylov solves of the workload at Kn_R in {0.01, 0.1, 1, 10}."""
+    out = {}
+    for tau_r in (0.01, 0.1, 1.0, 10.0):
+        res = {}
+        for prec in ("fp64", "fp32"):
+            case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, tau_r, 0.1 * tau_r, prec,
+                             work, jitter=args.jitter)
+            t0 = time.perf_counter()
+            s = pkg.Solver(case)
+            rec = s.run()
+            tts = time.perf_counter() - t0
+            res[prec] = {"time_to_solution_s": tts, "setup_s": s.setup_seconds,
+                         "iterations": rec.iterations, "converged": rec.converged,
+                         "solves_per_hour": 3600.0 / tts, "temp": rec.temp,
+                         "q": np.concatenate([rec.qx, rec.qy])}
+            del s
+        t64, t32 = res["fp64"].pop("temp"), res["fp32"].pop("temp")
+        q64, q32 = res["fp64"].pop("q"), res["fp32"].pop("q")
+        res["fp32"]["rel_l2_T_vs_fp64"] = float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))
+        res["fp32"]["rel_l2_q_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
+        res["speedup_fp32_refine_vs_fp64"] = (res["fp64"]["time_to_solution_s"] /
+                                              res["fp32"]["time_to_solution_s"])
+        out[f"kn_r={tau_r:g}"] = res
+    return out
 
 
 if __name__ == "__main__":

@@ -130,6 +130,15 @@ int pbte_sweep(pbte_ctx *ctx, int32_t precision, const double *ts, const double 
                const double *qys, void *vdf, const double *flux_wall, const void *pbuf,
                int32_t use_pbuf, int32_t homogeneous);
 
+/* The same sweep with every element operator built and LU-factorised in
+ * binary64 where it is used, so no factors are stored (pbte_factorise is
+ * not needed); arguments as pbte_sweep.  Replaces the "onthefly" mode of
+ * SweepContext.sweep / _kernels.sweep_onthefly (sweep.py:104-110, 147-156,
+ * _kernels.py:242-269). */
+int pbte_sweep_onthefly(pbte_ctx *ctx, int32_t precision, const double *ts, const double *qxs,
+                        const double *qys, void *vdf, const double *flux_wall, const void *pbuf,
+                        int32_t use_pbuf, int32_t homogeneous);
+
 /* Angular moments of vdf (precision as in pbte_sweep) in binary64,
  * directions summed in index order.  Replaces moments.compute_moments /
  * _kernels.moments_kernel (moments.py:51-55, _kernels.py:311-338). */

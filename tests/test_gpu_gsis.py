@@ -70,3 +70,22 @@ def test_gsis_periodic_channel_is_x_independent(pkg):
     assert int(s.bcdata.periodic_flip.sum()) > 0
     assert spread < 1e-3 * np.ptp(tbar)
     assert abs(rec.mass - 0.5) < 1e-5
+
+
+@pytest.mark.parametrize("key", [(1e-2, 1e5, 2, 20, 40), (1e-3, 1e5, 3, 20, 40)],
+                         ids=lambda k: f"tau{k[0]:g}_deg{k[2]}")
+def test_gsis_fp32_sweeps_then_fp64_reach_fp64_fixed_point(pkg, key):
+    """fp32 GSIS: binary32 sweeps until the iterate step is below
+    performance.refine_switch, then binary64 sweeps to the tolerance; the
+    result is the binary64 GSIS fixed point within the tolerance."""
+    want_it, want_mass = FROZEN_GSIS[key]
+    print(f"case GSIS fp32 then fp64 {key}", flush=True)
+    r32 = pkg.Solver(_shipped(pkg, *key, precision="fp32")).run()
+    assert r32.converged
+    assert r32.diagnostics["steps_fp32"] > 0
+    assert abs(r32.mass - want_mass) <= 1e-7
+    r64 = pkg.Solver(_shipped(pkg, *key)).run()
+    assert np.linalg.norm(r32.temp - r64.temp) / np.linalg.norm(r64.temp) <= 1e-6
+    q32 = np.concatenate([r32.qx, r32.qy])
+    q64 = np.concatenate([r64.qx, r64.qy])
+    assert np.linalg.norm(q32 - q64) / np.linalg.norm(q64) <= 1e-6

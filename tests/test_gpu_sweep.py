@@ -239,3 +239,41 @@ def test_broken_cycle_sweep_matches_oracle(pkg):
         ref.vdf[:], ref.ts[:], ref.qxs[:], ref.qys[:] = o.vdf, o.ts, o.qxs, o.qys
         ref.sweep()
         assert _rel(got, ref.vdf) <= 1e-12
+
+
+@pytest.mark.parametrize("tag,deg", [("F1_1_1", 2), ("F2_1_1", 1), ("F3_1_1", 3)])
+def test_onthefly_sweep_matches_oracle(pkg, tag, deg):
+    """performance.precompute_inverse: false selects the reference's
+    on-the-fly mode (sweep.py:104-110, _kernels.py:242-269): every element
+    operator is rebuilt and factorised where it is used, nothing stored."""
+    c = _case(pkg, tag, performance__precompute_inverse=False, dg__deg=deg)
+    s, o = _pair(pkg, c)
+    assert s.ctx.mode == "onthefly"
+    _seed_state(s, o, seed=13)
+    s.ctx.sweep(s.mom, s.vdf_d)
+    o.sweep()
+    assert _rel(s.vdf, o.vdf) <= 1e-12
+
+
+def test_onthefly_sweep_fp32(pkg):
+    c = _case(pkg, "F2_1_1", performance__precompute_inverse=False, performance__precision="fp32")
+    s, o = _pair(pkg, c)
+    assert s.ctx.mode == "onthefly"
+    _seed_state(s, o, seed=14)
+    s.ctx.sweep(s.mom, s.vdf32_d)
+    o.sweep()
+    assert _rel(s.vdf, o.vdf) <= 2e-5
+
+
+def test_onthefly_cis_reproduces_golden_history(pkg):
+    """The on-the-fly mode reaches the reference's converged source
+    iteration (adiabatic walls: the wall emission is refreshed every step)."""
+    c = _case(pkg, "F2_1_1", performance__precompute_inverse=False)
+    print("case F2_1_1 cis onthefly", flush=True)
+    s = pkg.Solver(c)
+    assert s.ctx.mode == "onthefly"
+    rec = s.run()
+    g = np.load(GOLDEN / "cis_F2_1_1.npz")
+    assert rec.iterations == len(g["residual_history"])
+    np.testing.assert_allclose(rec.residual_history, g["residual_history"], rtol=1e-8, atol=0)
+    np.testing.assert_allclose(rec.temp, g["temp"], rtol=0, atol=1e-12)
