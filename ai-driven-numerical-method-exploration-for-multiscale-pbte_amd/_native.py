@@ -84,6 +84,8 @@ _SIGS = {
     "pbte_gsis_source": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbte_gsis_rhs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbte_gsis_local": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_precond_setup": (C.c_int, [C.c_void_p, _f64p]),
+    "pbte_gsis_precond_src": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "pbte_gsis_correct": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),

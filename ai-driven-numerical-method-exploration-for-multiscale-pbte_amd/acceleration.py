@@ -49,6 +49,8 @@ class Acceleration:
         bc_type = s.bcdata.bc_type
         inv_aa_sol, aa_trace = build_local_operators(mesh, ig, self.nd, self.nf, c.flow.cv,
                                                      c.flow.tau_r, c.tau_c, st, self.variant)
+        # kept on the host for the Krylov moment preconditioner only
+        self.inv_aa_sol = inv_aa_sol if c.scheme.krylov_precond else None
         ba_sol = build_ba_sol(mesh, ig, self.nd, self.nf, st, bc_type, self.variant)
         drop = 1e-30 if self.variant == "A" else 0.0
         self.K, self.nnz = assemble_global_matrix(mesh, ig, ba_sol, aa_trace, self.nf, bc_type,
@@ -146,11 +148,14 @@ class Acceleration:
         if self.omega and self.defect_every and self._ncall % self.defect_every == 0:
             self.refresh_defect()
 
-    def trace_solve(self) -> None:
-        """``K u = ffa`` with the SuperLU factors (``global_solve.py:190-195``)."""
-        self._host_ffa.copy_(self.ffa_d)
+    def trace_solve(self, ffa=None, out=None) -> None:
+        """``K u = ffa`` with the SuperLU factors (``global_solve.py:190-195``);
+        by default from ``ffa_d`` into ``u_d``."""
+        ffa = self.ffa_d if ffa is None else ffa
+        out = self.u_d if out is None else out
+        self._host_ffa.copy_(ffa)
         x = self._lu.solve(self._host_ffa.numpy().reshape(-1))
-        self.u_d.copy_(torch.from_numpy(x.reshape(self.u_d.shape)))
+        out.copy_(torch.from_numpy(x.reshape(out.shape)))
         self.solve_count += 1
 
     # -- the fixed-point repair (acceleration/__init__.py:118-172) ----------

@@ -208,6 +208,7 @@ struct pbte_ctx {
     bool gsis = false;
     int gs_nw = 0, gs_nth = 0, gs_variant_b = 0, gs_use_hmin = 1;
     double gs_tc5 = 0, gs_tau_thr = 1, gs_drift = 0;
+    double* gs_inv3 = nullptr;  // T, qx, qy columns of inv(AA_SOL), moment preconditioner
     double *gs_hw = nullptr, *gs_inv4 = nullptr, *gs_aa_trace = nullptr, *gs_ba_sol = nullptr,
            *gs_wtfc = nullptr, *gs_wnx = nullptr, *gs_wny = nullptr, *gs_hmin = nullptr,
            *gs_work = nullptr, *gs_wmom = nullptr;
@@ -1606,6 +1607,30 @@ __global__ void __launch_bounds__(256) trace_wall_kernel(const double* __restric
     }
 }
 
+// y[0:n] = v (a fill that a captured graph can hold, unlike a memset)
+__global__ void __launch_bounds__(256) fill_kernel(double* y, int64_t n, double v)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) y[g] = v;
+}
+
+// the moment preconditioner's element source (krylov.py:388-402): the T,
+// qx, qy blocks of r (each [P][N]) times sign through the matching columns
+// of inv(AA_SOL): aa_src[p][i] = sum_q inv(AA_SOL)[i][p][q] sign r[q][i]
+__global__ void __launch_bounds__(256) precond_src_kernel(const double* __restrict__ inv3,
+                                                          const double* __restrict__ r, double sign, int N, int P,
+                                                          double* __restrict__ aa_src)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int R = 7 * P;
+    if (g >= N * R) return;
+    const int i = g / R, p = g - (g / R) * R;
+    const double* a = inv3 + ((size_t)i * R + p) * 3 * P;
+    double s = 0.0;
+    for (int q = 0; q < 3 * P; ++q) s += a[q] * (sign * r[(size_t)q * N + i]);
+    aa_src[(size_t)p * N + i] = s;
+}
+
 // UQ = AA_SRC + sum_il AA_TRACE[i][il] U_TRACE[face(i, il)]  (local_solve.py:16-26)
 __global__ void __launch_bounds__(256) local_solve_kernel(const double* __restrict__ aa_trace, const double* __restrict__ aa_src,
                                                           const double* __restrict__ u, const int32_t* tri_faces,
@@ -2920,7 +2945,10 @@ extern "C" int pbte_gsis_rhs(pbte_ctx* ctx, int32_t precision, const void* vdf, 
         HIP_TRY(hipGetLastError());
         if (ctx->gs_nw) {
             const unsigned g = nblocks((int64_t)ctx->gs_nw * Q, 256);
-            if (precision == PBTE_FP64)
+            if (!vdf)
+                hipLaunchKernelGGL(fill_kernel, dim3(nblocks((int64_t)ctx->gs_nw * 3 * Q, 256)), dim3(256), 0,
+                                   ctx->stream, ctx->gs_wmom, (int64_t)ctx->gs_nw * 3 * Q, 0.0);
+            else if (precision == PBTE_FP64)
                 hipLaunchKernelGGL(wall_trace_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
                                    ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_wtfc, ctx->gs_wtri, ctx->gs_nw, ctx->D,
                                    N, P, Q, ctx->gs_wmom);
@@ -2933,6 +2961,33 @@ extern "C" int pbte_gsis_rhs(pbte_ctx* ctx, int32_t precision, const void* vdf, 
                                ctx->gs_wnx, ctx->gs_wny, ctx->gs_nw, ctx->gs_nth, Q, ctx->cv, ctx->gs_variant_b, ffa);
             HIP_TRY(hipGetLastError());
         }
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_precond_setup(pbte_ctx* ctx, const double* inv_aa_sol)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
+        const int P = ctx->P, N = ctx->N;
+        std::vector<double> inv3((size_t)N * 7 * P * 3 * P);
+        for (int i = 0; i < N; ++i)
+            for (int p = 0; p < 7 * P; ++p)
+                std::memcpy(&inv3[((size_t)i * 7 * P + p) * 3 * P], inv_aa_sol + ((size_t)i * 7 * P + p) * 7 * P,
+                            sizeof(double) * 3 * P);
+        if (!ctx->gs_inv3) ctx->gs_inv3 = ctx->alloc<double>(inv3.size());
+        HIP_TRY(hipMemcpy(ctx->gs_inv3, inv3.data(), inv3.size() * sizeof(double), hipMemcpyHostToDevice));
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_precond_src(pbte_ctx* ctx, const double* r, double sign, double* aa_src)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gs_inv3) throw std::string("pbte_gsis_precond_setup first");
+        hipLaunchKernelGGL(precond_src_kernel, dim3(nblocks((int64_t)ctx->N * 7 * ctx->P, 256)), dim3(256), 0,
+                           ctx->stream, ctx->gs_inv3, r, sign, ctx->N, ctx->P, aa_src);
+        HIP_TRY(hipGetLastError());
         return 0;
     });
 }

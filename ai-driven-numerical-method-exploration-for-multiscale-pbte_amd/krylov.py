@@ -32,7 +32,8 @@ import numpy as np
 from . import _native
 from .device import dev_zeros, ptr, torch
 
-__all__ = ["FixedPointOperator", "applicable", "gmres_full", "run_krylov", "krylov_basis_cap"]
+__all__ = ["FixedPointOperator", "MomentPreconditioner", "applicable", "gmres_full", "run_krylov",
+           "krylov_basis_cap"]
 
 _ENOUGH_FACTOR = 1.0e-2
 _FLOOR = 1.0e-14
@@ -133,12 +134,15 @@ def _back_substitute(r, y):
     return out
 
 
-def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None, callback=None):
+def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None, callback=None,
+               precond=None):
     """GMRES with a device basis; ``matvec(v, out)`` writes ``A v``.
 
     Follows ``krylov.py:150-246``: restarts at ``max_basis``, CGS2, Givens
-    rotations, ``callback(res)`` once per product (true stops).
-    Returns ``(x, converged, n_matvec)``; ``x`` is updated in place."""
+    rotations, ``callback(res)`` once per product (true stops), and the
+    optional left preconditioner ``precond(v)`` (in place) applied to every
+    residual and product.  Returns ``(x, converged, n_matvec)``; ``x`` is
+    updated in place."""
     n = b.numel()
     x = dev_zeros(n) if x is None else x
     maxiter = int(n if maxiter is None else maxiter)
@@ -153,12 +157,16 @@ def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None
         if xnz:
             matvec(x, r)
             ctx.axpby(1.0, b, -1.0, r)          # r = b - A x
+            if precond is not None:
+                precond(r)
             nmv += 1
             if callback is not None and callback(ctx.norm(r)):
                 halted = True
                 break
         else:
             r.copy_(b)
+            if precond is not None:
+                precond(r)
         beta = ctx.norm(r)
         if beta <= tol_abs:
             converged = True
@@ -172,6 +180,8 @@ def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None
         cols, cs, sn, gam = [], [], [], [beta]
         for j in range(m):
             matvec(Q[j], w)
+            if precond is not None:
+                precond(w)
             nmv += 1
             c1 = ctx.multidot(Q, j + 1, w)
             ctx.update(Q, j + 1, c1, w)
@@ -214,6 +224,50 @@ def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None
             ctx.update(Q, k, -y, x)                # x += Q[:k]^T y
             xnz = True
     return x, converged, nmv
+
+
+class MomentPreconditioner:
+    """Approximate ``(I - T)^-1`` by the macroscopic moment system
+    (``krylov.py:369-414``; experimental there and here, switched on by
+    ``scheme.krylov_precond`` with ``scheme.accflag: 1``).
+
+    For a state vector ``r``: the T, qx, qy blocks times ``sign`` become the
+    element source through ``inv(AA_SOL)``, the trace system is solved with
+    a zero distribution on the walls, and the local recovery's T, qx, qy
+    replace those blocks; wall and periodic parts pass through."""
+
+    def __init__(self, solver, acc, sign: float = -1.0):
+        self.s, self.acc, self.sign = solver, acc, float(sign)
+        self.nd, self.nt = solver.ndof, solver.n_tris
+        self.nmom = 3 * self.nd * self.nt
+        ctx = solver.ctx
+        inv = np.ascontiguousarray(acc.inv_aa_sol, dtype=np.float64)
+        _native.check(ctx.handle, _native.lib().pbte_gsis_precond_setup(ctx.handle, _native.f64(inv)),
+                      "gsis_precond_setup")
+        self.aa_src = dev_zeros((7 * self.nd, self.nt))
+        self.ffa = dev_zeros(acc.ffa_d.shape)
+        self.u = dev_zeros(acc.u_d.shape)
+        self.uq = dev_zeros((7 * self.nd, self.nt))
+        self.calls = 0
+
+    def __call__(self, v) -> None:
+        h = self.s.ctx.handle
+        lib = _native.lib()
+        _native.check(h, lib.pbte_gsis_precond_src(h, ptr(v), self.sign, ptr(self.aa_src)),
+                      "gsis_precond_src")
+        _native.check(h, lib.pbte_gsis_rhs(h, _native.PBTE_FP64, None, ptr(self.aa_src), ptr(self.ffa)),
+                      "gsis_rhs")
+        self.acc.trace_solve(self.ffa, self.u)
+        _native.check(h, lib.pbte_gsis_local(h, ptr(self.aa_src), ptr(self.u), ptr(self.uq)), "gsis_local")
+        v[:self.nmom].copy_(self.uq[:3 * self.nd].reshape(-1))
+        self.calls += 1
+
+
+def _precond(solver):
+    c = solver.case
+    if c.scheme.krylov_precond and solver.acc is not None:
+        return MomentPreconditioner(solver, solver.acc)
+    return None
 
 
 def krylov_basis_cap(size: int, tmax: int, max_bytes: float) -> int:
@@ -304,13 +358,14 @@ def _run_fp64(solver, op, callback, progress_every):
     converged = False
     final = float("nan")
     fx = dev_zeros(op.n)
+    precond = _precond(solver)
     for _ in range(c.scheme.krylov_max_rounds):
         budget = tmax - len(h.residuals) - 1
         if budget <= 0:
             break
         reached = -1
         x, _, _ = gmres_full(ctx, matvec, g, x=x, tol_abs=target, maxiter=budget, max_basis=cap,
-                             callback=on_residual)
+                             callback=on_residual, precond=precond)
         op.apply(x, fx)                                     # closing sweep
         h.sweep_done(op)
         op.publish(fx)
@@ -327,7 +382,8 @@ def _run_fp64(solver, op, callback, progress_every):
     return solver._record(len(h.residuals), converged, h.residuals, h.masses,
                           h.true_res if h.want_true else None, wall, scheme="KRYLOV",
                           extra={"krylov_size": op.n, "krylov_periodic_state": op.nper,
-                                 "krylov_basis_cap": cap, "krylov_precond": False,
+                                 "krylov_basis_cap": cap, "krylov_precond": precond is not None,
+                                 "precond_calls": precond.calls if precond is not None else 0,
                                  "transport_residual": final, "sweeps_fp64": op.sweeps64,
                                  "sweeps_fp32": op.sweeps - op.sweeps64})
 
@@ -369,6 +425,7 @@ def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.
     final = float("nan")
     refinements = 0
     e = dev_zeros(op.n)
+    precond = _precond(solver)
     while True:
         op.publish(fx)
         final = float(ctx.true_residual(solver.mom, solver.vdf_d))
@@ -383,7 +440,8 @@ def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.
             break
         e.zero_()
         gmres_full(ctx, matvec, r, x=e, tol_abs=max(inner_reduction * rnorm, 0.1 * target),
-                   maxiter=budget, max_basis=cap, callback=lambda res: h.close(res / scale))
+                   maxiter=budget, max_basis=cap, callback=lambda res: h.close(res / scale),
+                   precond=precond)
         ctx.axpby(1.0, e, 1.0, x)
         refinements += 1
         rnorm = residual()
@@ -392,7 +450,8 @@ def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.
     return solver._record(len(h.residuals), converged, h.residuals, h.masses,
                           h.true_res if h.want_true else None, wall, scheme="KRYLOV",
                           extra={"krylov_size": op.n, "krylov_periodic_state": op.nper,
-                                 "krylov_basis_cap": cap, "krylov_precond": False,
+                                 "krylov_basis_cap": cap, "krylov_precond": precond is not None,
+                                 "precond_calls": precond.calls if precond is not None else 0,
                                  "transport_residual": final, "refinements": refinements,
                                  "final_defect": rnorm / scale, "sweeps_fp64": op.sweeps64,
                                  "sweeps_fp32": op.sweeps - op.sweeps64})

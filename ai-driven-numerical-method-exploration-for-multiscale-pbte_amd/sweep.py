@@ -240,19 +240,28 @@ class SweepContext:
         return self.residual_iterate(mom.temp_d, mom.temp_d)[1]
 
     # -- Krylov vector kernels --------------------------------------------
+    # one device call takes at most _ROWS basis rows; larger bases go in
+    # chunks (every row's dot product is still one call's)
+    _ROWS = 512
+
     def multidot(self, Q, m: int, w) -> np.ndarray:
         out = np.zeros(max(m, 1))
-        st = _native.lib().pbte_vec_multidot(self.handle, ptr(Q), Q.stride(0), m, ptr(w),
-                                             w.numel(), out.ctypes.data_as(C.POINTER(C.c_double)))
-        _native.check(self.handle, st, "multidot")
+        for k0 in range(0, m, self._ROWS):
+            k = min(self._ROWS, m - k0)
+            st = _native.lib().pbte_vec_multidot(
+                self.handle, ptr(Q[k0]) if Q.dim() > 1 else ptr(Q), Q.stride(0), k, ptr(w), w.numel(),
+                out[k0:].ctypes.data_as(C.POINTER(C.c_double)))
+            _native.check(self.handle, st, "multidot")
         return out[:m]
 
     def update(self, Q, m: int, c: np.ndarray, w) -> None:
         c = np.ascontiguousarray(c, dtype=np.float64)
-        st = _native.lib().pbte_vec_update(self.handle, ptr(Q), Q.stride(0), m,
-                                           c.ctypes.data_as(C.POINTER(C.c_double)), ptr(w),
-                                           w.numel())
-        _native.check(self.handle, st, "update")
+        for k0 in range(0, m, self._ROWS):
+            k = min(self._ROWS, m - k0)
+            st = _native.lib().pbte_vec_update(
+                self.handle, ptr(Q[k0]) if Q.dim() > 1 else ptr(Q), Q.stride(0), k,
+                c[k0:].ctypes.data_as(C.POINTER(C.c_double)), ptr(w), w.numel())
+            _native.check(self.handle, st, "update")
 
     def axpby(self, a: float, x, b: float, y) -> None:
         st = _native.lib().pbte_vec_axpby(self.handle, float(a), ptr(x), float(b), ptr(y), y.numel())

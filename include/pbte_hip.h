@@ -223,10 +223,20 @@ int pbte_gsis_moments(pbte_ctx *ctx, int32_t precision, const void *vdf, double 
 int pbte_gsis_source(pbte_ctx *ctx, const double *mom7, double *aa_src);
 
 /* Trace right-hand side ffa [F][3Q] (two-sided, periodic mirror, wall rows
- * from the kinetic traces).  Replaces GlobalSolver.rhs
- * (global_solve.py:128-186). */
+ * from the kinetic traces; vdf == NULL stands for a zero distribution, wall
+ * rows zero).  Replaces GlobalSolver.rhs (global_solve.py:128-186). */
 int pbte_gsis_rhs(pbte_ctx *ctx, int32_t precision, const void *vdf, const double *aa_src,
                   double *ffa);
+
+/* Uploads the T, qx, qy columns of inv_aa_sol [N][7P][7P] (host) for the
+ * moment preconditioner.  Part of MomentPreconditioner.__init__
+ * (krylov.py:379-390). */
+int pbte_gsis_precond_setup(pbte_ctx *ctx, const double *inv_aa_sol);
+
+/* aa_src [7P][N] = inv(AA_SOL) applied to sign * the T, qx, qy blocks of r
+ * (three [P][N] blocks, device).  Replaces the element source of
+ * MomentPreconditioner._apply (krylov.py:392-402). */
+int pbte_gsis_precond_src(pbte_ctx *ctx, const double *r, double sign, double *aa_src);
 
 /* UQ [7P][N] = AA_SRC + sum AA_TRACE U_TRACE (u_trace [F][3Q]).  Replaces
  * local_solve.local_solve (local_solve.py:16-26). */

@@ -50,3 +50,17 @@ def test_krylov_fp32_refined_matches_fp64(pkg, tag):
     print(f"rel L2 T {rel_t:.2e} q {rel_q:.2e} sweeps fp64 {rec.diagnostics['sweeps_fp64']} "
           f"fp32 {rec.diagnostics['sweeps_fp32']}", flush=True)
     assert rel_t <= 1e-6 and rel_q <= 1e-6
+
+
+@pytest.mark.parametrize("tag", ["F1_1e-2", "F1_1e-3"])
+def test_krylov_moment_preconditioner_reaches_certified_fixed_point(pkg, tag):
+    """scheme.krylov_precond with accflag 1: GMRES left-preconditioned by
+    the HDG moment solve (krylov.py:369-414) converges to the same
+    certified transport fixed point."""
+    rec = _run(pkg, tag, scheme__accflag=1, scheme__krylov_precond=True)
+    z = np.load(GOLDEN / "krylov" / f"{tag}.npz")
+    assert rec.diagnostics["krylov_precond"] and rec.diagnostics["precond_calls"] > 0
+    assert rec.converged
+    assert rec.diagnostics["transport_residual"] < 1e-8
+    np.testing.assert_allclose(rec.temp_dofs, z["temp_dofs"], rtol=0, atol=1e-9)
+    assert abs(rec.mass - float(z["mass"])) <= 1e-9
