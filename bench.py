@@ -22,10 +22,17 @@ Prints one JSON line:
                  skips it);
   cpu_baseline   the C restatement of the reference's sweep on this host.
 
+--workload cis runs the plain-CIS configuration: unit square 16 x 16 split
+into 512 P1 triangles, 64 ordinates (4 x 16), Kn_R = Kn_N = 1, diffuse
+isothermal walls (north dT = +1), FP64 source iteration to 1e-8; value is
+the GPU solves/hour and cpu_baseline the C restatement's solve on the host.
+
 --workload gsis runs the GSIS configuration instead: jittered triangulated
 square (interior vertices moved by U(-0.25h, 0.25h), seed 0), P2, 512
-ordinates (16 x 32), Kn_R = 1e-3, Kn_N = 1e-4, west wall dT = +1, east wall
-0, north and south walls diffuse adiabatic; one FP64 and one FP32-sweep
+ordinates (16 x 32), Kn_R = 1e-3, Kn_N = 1e-4, one wall at dT = +1, the
+opposite wall at 0, the other two sides a periodic pair (in place of the
+specular walls, which the reference does not have; for this layout both
+give the same one-dimensional solution); one FP64 and one FP32-sweep
 (binary64 finish) GSIS solve to the iterate step 1e-8.  value is the
 solves/hour of the FP32 solve; roofline is the sweep kernel inside those
 solves.  The default mesh is 128 x 128: the 256 x 256 trace system (1.77 M
@@ -87,10 +94,15 @@ def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitte
 
 
 def gsis_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitter=0.25, seed=0):
+    """Hot wall (dT = +1) opposite a cold wall (0), the other two sides a
+    periodic pair: for this layout a specular pair and a periodic pair give
+    the same one-dimensional solution, and the reference has no specular
+    wall.  The walls are north and south of the mesh writer's channel
+    layout (the problem turned by 90 degrees)."""
     from pbte_amd.mesh import write_square_mesh
-    mesh = Path(workdir) / f"square_{n}_{jitter}_{seed}.msh"
+    mesh = Path(workdir) / f"channel_{n}_{jitter}_{seed}.msh"
     if not mesh.exists():
-        write_square_mesh(mesh, n, jitter=jitter, seed=seed)
+        write_square_mesh(mesh, n, layout="channel", jitter=jitter, seed=seed)
     return pkg.Case.from_dict({
         "iteration": {"tol": 1e-8, "tmax": 5000},
         "scheme": {"accflag": 1, "method": "source"},
@@ -99,10 +111,10 @@ def gsis_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitte
         "flow": {"cv": 1.0, "vg": 1.0, "tau_r": tau_r, "tau_n": tau_n},
         "mesh": {"file": str(mesh)},
         "boundaries": [
-            {"name": "SWall", "phyid": 11, "type": "adiabatic"},
-            {"name": "NWall", "phyid": 12, "type": "adiabatic"},
-            {"name": "EWall", "phyid": 13, "type": "thermalising", "temp": 0.0},
-            {"name": "WWall", "phyid": 14, "type": "thermalising", "temp": 1.0},
+            {"name": "SWall", "phyid": 11, "type": "thermalising", "temp": 0.0},
+            {"name": "NWall", "phyid": 12, "type": "thermalising", "temp": 1.0},
+            {"name": "Master", "phyid": 13, "type": "periodic", "xoff": -1.0},
+            {"name": "Slave", "phyid": 14, "type": "periodic", "xoff": 1.0},
         ],
         "restart": {"enabled": False, "error_on_stale": False},
         "output": {"field": False, "runtime_log": False, "run_record": False},
@@ -152,8 +164,8 @@ def gsis_workload(pkg, args, work):
     res["speedup_fp32_vs_fp64_run"] = res["fp64"]["run_s"] / res["fp32"]["run_s"]
     t32 = res["fp32"]["sweep_kernel_ms_fp32_mean"] or res["fp32"]["sweep_kernel_ms_fp64_mean"]
     bytes_alg = b_alg(D, N, P, 4, 4)
-    workload = (f"gsis jittered square {n}x{n} -> {N} P{args.deg} triangles, {D} ordinates "
-                f"({args.npole}x{args.nazim}), Kn_R 1e-3, Kn_N 1e-4")
+    workload = (f"gsis jittered channel {n}x{n} -> {N} P{args.deg} triangles, {D} ordinates "
+                f"({args.npole}x{args.nazim}), Kn_R 1e-3, Kn_N 1e-4, hot/cold walls, periodic sides")
     out = {
         "metric": METRIC, "value": res["fp32"]["solves_per_hour"], "unit": "solves/hour",
         "n_gpus": 1, "steps": res["fp32"]["iterations"], "warmup": 0,
@@ -165,6 +177,59 @@ def gsis_workload(pkg, args, work):
                      "unit": "GB/s", "frac": bytes_alg / (t32 * 1e-3) / HBM_PEAK, "traffic": None,
                      "bytes_alg_per_launch": bytes_alg},
         "gsis": res, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cis_workload(pkg, args, work):
+    """FP64 CIS to the iterate step 1e-8 on the GPU and with the C
+    restatement of the reference on the host (same case, same schedule of
+    operations), both timed from the built problem to the converged moments."""
+    import torch
+    from oracle import cpu
+    from pbte_amd.problem import build_problem
+    n, npole, nazim = 16, 4, 16
+    case = make_case(pkg, n, 1, npole, nazim, 1.0, 1.0, "fp64", work)
+    case.scheme.method = "source"
+    t0 = time.perf_counter()
+    s = pkg.Solver(case)
+    setup = time.perf_counter() - t0
+    s.ctx.timing(case.iteration.tmax)
+    t1 = time.perf_counter()
+    rec = s.run()
+    torch.cuda.synchronize()
+    run = time.perf_counter() - t1
+    ms = s.ctx.timing_read(case.iteration.tmax)
+    s.ctx.timing(0)
+    cpu.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t2 = time.perf_counter()
+    o = cpu.Oracle(build_problem(case))
+    o.factorise()
+    res, mass = o.run_cis(tol=case.iteration.tol, tmax=case.iteration.tmax)
+    cpu_s = time.perf_counter() - t2
+    D, N, P = s.ndir, s.n_tris, s.ndof
+    t_sweep = float(np.mean(ms)) * 1e-3
+    bytes_alg = b_alg(D, N, P, 8, 8)
+    out = {
+        "metric": METRIC, "value": 3600.0 / (setup + run), "unit": "solves/hour",
+        "n_gpus": 1, "steps": rec.iterations, "warmup": 0,
+        "ms_per_step": 1e3 * run / max(rec.iterations, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: the workload's mesh, ordinates and boundary data",
+        "config": {"workload": f"cis square {n}x{n} -> {N} P1 triangles, {D} ordinates "
+                               f"({npole}x{nazim}), Kn_R 1, Kn_N 1", "n_tris": N, "ndof": P, "ndir": D},
+        "roofline": {"bound": "hbm", "achieved": bytes_alg / t_sweep / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": bytes_alg / t_sweep / HBM_PEAK, "traffic": None,
+                     "bytes_alg_per_launch": bytes_alg, "note": "latency bound: 2 MB per sweep"},
+        "cis": {"setup_s": setup, "run_s": run, "iterations": rec.iterations,
+                "converged": rec.converged, "mass": rec.mass, "sweep_kernel_ms_mean": t_sweep * 1e3,
+                "cpu_iterations": len(res), "cpu_mass": float(mass[-1]),
+                "mass_abs_diff_vs_cpu": abs(rec.mass - float(mass[-1]))},
+        "cpu_baseline": {"value": 3600.0 / cpu_s, "unit": "solves/hour", "cores": threads,
+                         "kind": "port", "sample": f"one FP64 CIS solve of the same case "
+                                                   f"({cpu_s:.2f} s incl. factorisation), "
+                                                   f"OMP_NUM_THREADS={threads}"},
     }
     print(json.dumps(out), flush=True)
 
@@ -208,7 +273,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "gsis"])
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "cis", "gsis"])
     ap.add_argument("--deg", type=int, default=2)
     ap.add_argument("--npole", type=int, default=8)
     ap.add_argument("--nazim", type=int, default=32)
@@ -237,6 +302,9 @@ def main():
     import torch
 
     work = Path(tempfile.mkdtemp(prefix="pbte_bench_"))
+    if args.workload == "cis":
+        cis_workload(pkg, args, work)
+        return
     if args.workload == "gsis":
         if args.npole == 8 and args.nazim == 32:
             args.npole, args.nazim = 16, 32
