@@ -167,8 +167,7 @@ struct pbte_ctx {
     int32_t *block_bundle = nullptr, *bundle_ndir = nullptr, *bundle_dir_off = nullptr,
             *bundle_dirs = nullptr, *slot_cell = nullptr, *lev_off = nullptr, *lev_ptr = nullptr;
     int64_t *lu_off = nullptr, *item_off = nullptr;
-    int4* irec = nullptr;       // per item (bundle, slot, direction): {cell, face words}
-    int32_t* iwst = nullptr;    // per item: LDS pool slot the item writes, or -1
+    int4* irec = nullptr;       // per item (bundle, slot, direction): {cell | pool slot, face words}
     int4* wcell = nullptr;      // per bundle: inflow wall elements {cell, wall row of faces 0-2}
     int32_t* wc_off = nullptr;
     int max_wall_in = 0;        // most inflow wall elements of one bundle
@@ -462,8 +461,7 @@ template <typename T> struct SweepArgs {
     const int32_t* wc_off;
     const T* pbuf;
     T* vdf;
-    const int4* irec;           // per item (bundle, slot, direction): {cell, face words}
-    const int32_t* iwst;        // per item: the LDS pool slot it writes, or -1
+    const int4* irec;           // per item (bundle, slot, direction): {cell | pool slot, face words}
     T* spill_row;               // write target of idle threads (P values)
     const uint8_t* bundle_global;
     int N, F, rs, use_pbuf, homogeneous;
@@ -504,6 +502,14 @@ __host__ __device__ constexpr int face_word(int code, int ilp, int val)
 __host__ __device__ constexpr int fw_code(int w) { return (int)((unsigned)w >> 28); }
 __host__ __device__ constexpr int fw_ilp(int w) { return (w >> kFaceValBits) & 3; }
 __host__ __device__ constexpr int fw_val(int w) { return w & kFaceValMask; }
+
+// item record word x: the element in bits 0-20, the LDS pool slot it writes
+// plus one in bits 21-31 (0: no downwind reader reads it through LDS)
+constexpr int kCellBits = 21;
+constexpr int kCellMask = (1 << kCellBits) - 1;
+constexpr int kPoolMax = (1 << (32 - kCellBits)) - 2;
+__host__ __device__ constexpr int rec_cell(int x) { return x & kCellMask; }
+__host__ __device__ constexpr int rec_pool(int x) { return (int)((unsigned)x >> kCellBits) - 1; }
 
 // record-code classes as bit masks over the code: one shift and one test,
 // no short-circuit control flow
@@ -621,7 +627,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
         const int s = a.lev_ptr[lv] + c;
         if (s < a.lev_ptr[lv + 1]) {
             const int4 r = irec[(size_t)s * B + b];
-            const int i = r.x;
+            const int i = rec_cell(r.x);
             const int fw[3] = {r.y, r.z, r.w};
             const T* cr = a.crec + (size_t)i * a.rs;
             int wk = -1;
@@ -698,7 +704,7 @@ __global__ void __launch_bounds__(256) sweep_otf_kernel(SweepArgs<T> a)
         const int s = a.lev_ptr[lv] + c;
         if (s < a.lev_ptr[lv + 1]) {
             const int4 r = irec[(size_t)s * B + b];
-            const int i = r.x;
+            const int i = rec_cell(r.x);
             const int fw[3] = {r.y, r.z, r.w};
             const T* cr = a.crec + (size_t)i * a.rs;
             double A[P][P];
@@ -824,7 +830,7 @@ __device__ __forceinline__ void lds_store(T* dst, const T* src)
 struct PipeRec {
     bool valid;
     int slot;
-    int4 r;                 // {cell, face words}
+    int4 r;                 // {cell | pool slot, face words}
 };
 
 template <int P, typename T, bool PFLU> struct PipeData {
@@ -838,45 +844,71 @@ template <int P, typename T, bool PFLU> struct PipeData {
     T lu[PFLU ? L::LU : 4];
 };
 
+// what the solve of a level reads from LDS (and, rarely, global memory)
+template <int P, typename T> struct PipeIn {
+    int code[3];
+    T u[3][Tr<P>::TQ];
+    T v[pad4(P)];
+};
+
+// buffer descriptors of one bundle's streams (raw, bounds-checked: an
+// offset past num_records reads 0)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bundle_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, bytes, 0x00020000);
+}
+// 16 loaded bytes (the builtin's native 4 x u32 vector) as another 16-byte type
+template <typename R, typename V> __device__ __forceinline__ R bits16(const V& v)
+{
+    static_assert(sizeof(R) == 16 && sizeof(V) == 16, "16-byte words");
+    R r;
+    __builtin_memcpy(&r, &v, 16);
+    return r;
+}
+
 // records of one level: one coalesced 16-byte word per item.  Idle
 // threads read the level's first slot (slot 0 for an empty span): every
 // load of every stage is unconditional and always consumed by the next
 // stage, so the waits the compiler places are exact (a load issued on one
 // path only, or left pending on one path, makes a later use or overwrite of
 // its registers wait for every prefetch in flight)
-__device__ __forceinline__ void pipe_record(const int4* irec, int2 span, int c, int B, int b, PipeRec& o)
+__device__ __forceinline__ void pipe_record(__amdgpu_buffer_rsrc_t irec, int2 span, int c, int B, int b, PipeRec& o)
 {
     const int sl = span.x + c;
     o.valid = sl < span.y;
     o.slot = o.valid ? sl : span.x;
-    o.r = irec[(size_t)o.slot * B + b];
+    o.r = bits16<int4>(__builtin_amdgcn_raw_buffer_load_b128(irec, (uint32_t)(o.slot * B + b) * 16u, 0, 0));
 }
 
+// record-code classes as bit masks over the code: one shift and one test,
+// no short-circuit control flow
 // faces whose inflow trace comes from the LDS pool
 __device__ __forceinline__ int rec_lds_mask(int use_pbuf)
 {
     return use_pbuf ? code_bit(REC_UP_LDS) : (code_bit(REC_UP_LDS) | code_bit(REC_PER_LDS));
 }
 
-// gather: only loads whose addresses come from the record of a level
+// gather: only loads whose addresses come from the record of two levels
 // earlier; every use of the loaded data waits for the solve stage (the
 // in-order vmcnt counter would make an earlier use wait for every
-// prefetch in flight)
+// prefetch in flight).  The inverse streams through a buffer descriptor:
+// one lane offset, the k-groups a scalar offset apart.
 template <int P, typename T, bool PFLU>
-__device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int32_t* iwst, const PipeRec& r, int B, int b,
-                                            const T* lu, PipeData<P, T, PFLU>& o)
+__device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec& r, int B, int b,
+                                            __amdgpu_buffer_rsrc_t lu, PipeData<P, T, PFLU>& o)
 {
     using L = Lay<P>;
     constexpr int W = Tr<P>::W;
     constexpr int TQ = Tr<P>::TQ;
+    constexpr int PV = pad4(P);
     o.valid = r.valid;
-    const int i = r.r.x;
+    const int i = rec_cell(r.r.x);
     o.cell = i;
     o.slot = r.slot;
+    o.wst = rec_pool(r.r.x);
     o.fw[0] = r.r.y;
     o.fw[1] = r.r.z;
     o.fw[2] = r.r.w;
-    constexpr int PV = pad4(P);
     const unsigned zero = ((unsigned)a.pool * B + b) * W;
     const int lds_mask = rec_lds_mask(a.use_pbuf);
     unsigned wv = zero;
@@ -892,18 +924,15 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int32_t
         wv = code_in(kRecWall, code) ? (unsigned)a.wbase + (val * B + b) * PV : wv;
     }
     o.wv = (int)wv;
-    // the pool slot is only used by the solve: it travels with the
-    // gathered data
-    o.wst = iwst[(size_t)r.slot * B + b];
     load16<T, L::CRL>(o.cr, a.crec + (size_t)i * a.rs);
     if constexpr (PFLU) {
-        using V = typename Vec16<T>::type;
         constexpr int w = 16 / sizeof(T);
-        const size_t NB = (size_t)a.N * B;
-        const V* e = reinterpret_cast<const V*>(lu) + (size_t)r.slot * B + b;
+        const uint32_t voff = (uint32_t)(r.slot * B + b) * 16u;
+        const uint32_t gstride = (uint32_t)a.N * (uint32_t)B * 16u;
 #pragma unroll
         for (int g = 0; g < L::LU / w; ++g) {
-            const V v = e[(size_t)g * NB];
+            const auto v = bits16<typename Vec16<T>::type>(
+                __builtin_amdgcn_raw_buffer_load_b128(lu, voff, (uint32_t)g * gstride, 0));
             const T* t = reinterpret_cast<const T*>(&v);
 #pragma unroll
             for (int j = 0; j < w; ++j) o.lu[g * w + j] = t[j];
@@ -911,9 +940,54 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const int32_t
     }
 }
 
+// first half of the solve: the upwind traces and the wall inflow from LDS
+// (issued before the next gather, so their latency passes while it issues)
+// and, in the kernel instance launched when such reads exist, the rare
+// global reads (explicit periodic buffer, a previous-sweep value of a
+// broken cycle, a bundle running with full barriers) behind a wave-uniform
+// branch with the wait inside it
 template <int P, typename T, bool PFLU, bool GREAD>
-__device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData<P, T, PFLU>& dC, T cx, T cy, int d,
-                                           int B, int b, const T* lu, T* lds, T* vdf_d)
+__device__ __forceinline__ void pipe_read(const SweepArgs<T>& a, const PipeData<P, T, PFLU>& dC, int d, const T* lds,
+                                          const T* vdf_d, PipeIn<P, T>& in)
+{
+    using L = Lay<P>;
+    constexpr int DEG = Deg<P>::v;
+    constexpr int Q = DEG + 1;
+    constexpr int TQ = Tr<P>::TQ;
+    constexpr int PV = pad4(P);
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        in.code[il] = fw_code(dC.fw[il]);
+        lds_load<T, TQ>(in.u[il], lds + dC.lds[il]);
+    }
+    lds_load<T, PV>(in.v, lds + dC.wv);
+    if constexpr (GREAD) {
+        const int lds_mask = rec_lds_mask(a.use_pbuf);
+        const int gmask = dC.valid ? kRecNeighbour & ~lds_mask : 0;
+        const bool g0 = code_in(gmask, in.code[0]), g1 = code_in(gmask, in.code[1]),
+                   g2 = code_in(gmask, in.code[2]);
+        if (__builtin_amdgcn_ballot_w64(g0 || g1 || g2) != 0) {
+#pragma unroll
+            for (int il = 0; il < 3; ++il) {
+                if (code_in(gmask, in.code[il])) {
+                    const int ilp = fw_ilp(dC.fw[il]);
+                    const T* fe = (code_in(kRecPeriodic, in.code[il]) && a.use_pbuf)
+                                      ? a.pbuf + ((size_t)d * a.F +
+                                                  (bits_of(a.crec[(size_t)dC.cell * a.rs + L::R_INFO + il]) >> 4)) * P
+                                      : vdf_d + (size_t)fw_val(dC.fw[il]) * P;
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) in.u[il][q] = fe[face_node(DEG, ilp, q)];
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0x0f70);       // vmcnt(0)
+        }
+    }
+}
+
+// second half: source, wall inflow, upwind faces, the inverse, the stores
+template <int P, typename T, bool PFLU>
+__device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeData<P, T, PFLU>& dC,
+                                             const PipeIn<P, T>& in, T cx, T cy, int B, int b, T* lds, T* vdf_d)
 {
     using L = Lay<P>;
     constexpr int DEG = Deg<P>::v;
@@ -922,52 +996,22 @@ __device__ __forceinline__ void pipe_solve(const SweepArgs<T>& a, const PipeData
     const T* cr = dC.cr;
     // idle threads of a busy wave compute on the first slot's data and
     // store nothing that is read
-    // inflow traces from LDS, the three reads in flight together
-    constexpr int PV = pad4(P);
-    int code[3];
-    T u[3][TQ];
-    T v[PV];
-#pragma unroll
-    for (int il = 0; il < 3; ++il) {
-        code[il] = fw_code(dC.fw[il]);
-        lds_load<T, TQ>(u[il], lds + dC.lds[il]);
-    }
-    lds_load<T, PV>(v, lds + dC.wv);
-    // inflow traces from global memory: explicit periodic buffer, a
-    // previous-sweep value (broken cycle) or a bundle running with full
-    // barriers.  Only in the kernel instance launched when such reads exist;
-    // there behind a wave-uniform branch with the wait inside it.
-    if constexpr (GREAD) {
-        const int lds_mask = rec_lds_mask(a.use_pbuf);
-        const int gmask = dC.valid ? kRecNeighbour & ~lds_mask : 0;
-        const bool g0 = code_in(gmask, code[0]), g1 = code_in(gmask, code[1]), g2 = code_in(gmask, code[2]);
-        if (__builtin_amdgcn_ballot_w64(g0 || g1 || g2) != 0) {
-#pragma unroll
-            for (int il = 0; il < 3; ++il) {
-                if (code_in(gmask, code[il])) {
-                    const int ilp = fw_ilp(dC.fw[il]);
-                    const T* fe = (code_in(kRecPeriodic, code[il]) && a.use_pbuf)
-                                      ? a.pbuf + ((size_t)d * a.F +
-                                                  (bits_of(a.crec[(size_t)dC.cell * a.rs + L::R_INFO + il]) >> 4)) * P
-                                      : vdf_d + (size_t)fw_val(dC.fw[il]) * P;
-#pragma unroll
-                    for (int q = 0; q < Q; ++q) u[il][q] = fe[face_node(DEG, ilp, q)];
-                }
-            }
-            __builtin_amdgcn_s_waitcnt(0x0f70);       // vmcnt(0)
-        }
-    }
     T x[P];
 #pragma unroll
-    for (int m = 0; m < P; ++m) x[m] = source_plus<P, T>(cr, cx, cy, m, v[m]);
+    for (int m = 0; m < P; ++m) x[m] = source_plus<P, T>(cr, cx, cy, m, in.v[m]);
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
         const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
         const T w = T(0.5) * (sp - fabs(sp));
-        face_update<P, T>(x, il, w, a.kref, u[il]);
+        face_update<P, T>(x, il, w, a.kref, in.u[il]);
     }
-    const size_t item = (size_t)dC.slot * B + b, NB = (size_t)a.N * B;
-    apply_inverse<P, T>(x, [&](int k) { return PFLU ? dC.lu[k] : lu[lu_idx<T>(k, item, NB)]; });
+    if constexpr (PFLU) {
+        apply_inverse<P, T>(x, [&](int k) { return dC.lu[k]; });
+    } else {
+        const T* lu = a.lu + a.lu_off[a.block_bundle[blockIdx.x]];
+        const size_t item = (size_t)dC.slot * B + b, NB = (size_t)a.N * B;
+        apply_inverse<P, T>(x, [&](int k) { return lu[lu_idx<T>(k, item, NB)]; });
+    }
     // stores without a branch: an idle thread, and an element nobody reads
     // through LDS, write their traces to the spill slot; an idle thread
     // writes its values to a scratch row (a branch would let the compiler
@@ -1004,26 +1048,28 @@ __device__ __forceinline__ void level_barrier(bool global_mode)
     }
 }
 
-template <int P, typename T, bool PFLU, bool GREAD, int DEPTH>
+// BF: 1 when every bundle holds one direction (the default schedule; the
+// slot and LDS arithmetic then drops its bundle factors), 0 for bundles
+template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
 __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
 {
     constexpr int W = Tr<P>::W;
-    constexpr int TQ = Tr<P>::TQ;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int bundle = a.block_bundle[blockIdx.x];
-    const int B = a.bundle_ndir[bundle];
-    // LDS: pool slots, one all-zero slot, one spill slot, the wall traces
+    const int B = BF ? 1 : a.bundle_ndir[bundle];
+    // LDS: pool slots, one all-zero slot, one spill slot, the wall inflow
     T* lds = reinterpret_cast<T*>(lds_raw);
     const bool global_mode = a.bundle_global[bundle] != 0;
     const int t = threadIdx.x;
-    const int b = t % B;
-    const int c = t / B;
+    const int b = BF ? 0 : t % B;
+    const int c = BF ? t : t / B;
     const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
     const T cx = a.cxv[d], cy = a.cyv[d];
     const int N = a.N;
-    const int4* irec = a.irec + a.item_off[bundle];
-    const int32_t* iwst = a.iwst + a.item_off[bundle];
-    const T* lu = a.lu + a.lu_off[bundle];
+    const int64_t it0 = a.item_off[bundle];
+    const __amdgpu_buffer_rsrc_t irec = bundle_rsrc(a.irec + it0, (uint32_t)(a.item_off[bundle + 1] - it0) * 16u);
+    const int64_t lu0 = a.lu_off[bundle];
+    const __amdgpu_buffer_rsrc_t lu = bundle_rsrc(a.lu + lu0, (uint32_t)((a.lu_off[bundle + 1] - lu0) * sizeof(T)));
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
     {
@@ -1045,16 +1091,32 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
     __syncthreads();
 
     // The spans are read with per-lane (vector) loads: a scalar load would
-    // share lgkmcnt with the LDS traffic that every level waits for.
-    // A level past the end reads the last level's span and takes (0, 0):
-    // the load is unconditional (a load on one path only leaves the
-    // compiler unsure of the in-order vmcnt and it then waits for every
-    // prefetch in flight).
+    // share lgkmcnt with the LDS traffic that every level waits for.  A
+    // level past the end reads the last level's span and takes (0, 0): the
+    // load is unconditional (a load on one path only leaves the compiler
+    // unsure of the in-order vmcnt and it then waits for every prefetch in
+    // flight).
     const int32_t* lp = a.lev_ptr + (t & a.lane_zero);
     auto span = [&](int lv) {
         const int k = min(lv, l1 - 1);
         const int2 v = make_int2(lp[k], lp[k + 1]);
         return lv < l1 ? v : make_int2(0, 0);
+    };
+    // one level: in binary32 the solve's LDS reads, the loads of later
+    // levels, then the solve's arithmetic and stores (the LDS latency passes
+    // while the loads issue); in binary64 the loads first (the read values
+    // held across the gather would push the binary64 register sets into
+    // the accumulation registers)
+    auto level = [&](const PipeData<P, T, PFLU>& dS, PipeRec& rN, int2& sN, int lvN, PipeData<P, T, PFLU>& dG,
+                     const PipeRec& rG) {
+        PipeIn<P, T> in;
+        if constexpr (sizeof(T) == 4) pipe_read<P, T, PFLU, GREAD>(a, dS, d, lds, vdf_d, in);
+        pipe_record(irec, sN, c, B, b, rN);
+        sN = span(lvN);
+        pipe_gather<P, T, PFLU>(a, rG, B, b, lu, dG);
+        if constexpr (sizeof(T) != 4) pipe_read<P, T, PFLU, GREAD>(a, dS, d, lds, vdf_d, in);
+        pipe_compute<P, T, PFLU>(a, dS, in, cx, cy, B, b, lds, vdf_d);
+        level_barrier(global_mode);
     };
     if constexpr (DEPTH == 2) {
         // level spans travel two levels ahead of their use, the records one
@@ -1070,22 +1132,14 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
         // waiting
         pipe_record(irec, span(l0), c, B, b, R0);
         pipe_record(irec, span(l0 + 1), c, B, b, R1);
-        pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);
+        pipe_gather<P, T, PFLU>(a, R0, B, b, lu, D0);
         // two levels per trip, with no exit between them: an odd level count
         // runs one empty level (span (0, 0)) at the end.  A mid-trip exit
         // would make the compiler rotate the register sets with copies, and a
         // copy of a loaded value waits for it.
         for (int lv = l0; lv < l1; lv += 2) {
-            pipe_record(irec, S0, c, B, b, R0);                            // level lv+2
-            S0 = span(lv + 4);
-            pipe_gather<P, T, PFLU>(a, iwst, R1, B, b, lu, D1);                  // level lv+1
-            pipe_solve<P, T, PFLU, GREAD>(a, D0, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv
-            level_barrier(global_mode);
-            pipe_record(irec, S1, c, B, b, R1);                            // level lv+3
-            S1 = span(lv + 5);
-            pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);                  // level lv+2
-            pipe_solve<P, T, PFLU, GREAD>(a, D1, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+1
-            level_barrier(global_mode);
+            level(D0, R0, S0, lv + 4, D1, R1);     // solve lv; record lv+2; gather lv+1
+            level(D1, R1, S1, lv + 5, D0, R0);     // solve lv+1; record lv+3; gather lv+2
         }
     } else {
         // the gather two levels ahead of the solve and the records two levels
@@ -1097,26 +1151,14 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
         PipeData<P, T, PFLU> D0, D1, D2;
         pipe_record(irec, span(l0), c, B, b, R0);
         pipe_record(irec, span(l0 + 1), c, B, b, R1);
-        pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);
+        pipe_gather<P, T, PFLU>(a, R0, B, b, lu, D0);
         pipe_record(irec, span(l0 + 2), c, B, b, R2);
-        pipe_gather<P, T, PFLU>(a, iwst, R1, B, b, lu, D1);
+        pipe_gather<P, T, PFLU>(a, R1, B, b, lu, D1);
         pipe_record(irec, span(l0 + 3), c, B, b, R0);
         for (int lv = l0; lv < l1; lv += 3) {
-            pipe_record(irec, S1, c, B, b, R1);                            // level lv+4
-            S1 = span(lv + 7);
-            pipe_gather<P, T, PFLU>(a, iwst, R2, B, b, lu, D2);                  // level lv+2
-            pipe_solve<P, T, PFLU, GREAD>(a, D0, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv
-            level_barrier(global_mode);
-            pipe_record(irec, S2, c, B, b, R2);                            // level lv+5
-            S2 = span(lv + 8);
-            pipe_gather<P, T, PFLU>(a, iwst, R0, B, b, lu, D0);                  // level lv+3
-            pipe_solve<P, T, PFLU, GREAD>(a, D1, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+1
-            level_barrier(global_mode);
-            pipe_record(irec, S0, c, B, b, R0);                            // level lv+6
-            S0 = span(lv + 9);
-            pipe_gather<P, T, PFLU>(a, iwst, R1, B, b, lu, D1);                  // level lv+4
-            pipe_solve<P, T, PFLU, GREAD>(a, D2, cx, cy, d, B, b, lu, lds, vdf_d);   // level lv+2
-            level_barrier(global_mode);
+            level(D0, R1, S1, lv + 7, D2, R2);     // solve lv; record lv+4; gather lv+2
+            level(D1, R2, S2, lv + 8, D0, R0);     // solve lv+1; record lv+5; gather lv+3
+            level(D2, R0, S0, lv + 9, D1, R1);     // solve lv+2; record lv+6; gather lv+4
         }
     }
 }
@@ -2056,7 +2098,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->n_per = (int)pf.size();
         ctx->per_face = ctx->upload(pf.data(), pf.size());
         ctx->per_tri = ctx->upload(pt.data(), pt.size());
-        if (N > kFaceValMask || F > kFaceValMask) throw std::string("mesh too large for 26-bit record fields");
+        if (N > kCellMask || F > kFaceValMask) throw std::string("mesh too large for the record fields");
 
         // --- bundles: directions with identical inflow signs ------------------
         std::vector<uint64_t> hsh(D);
@@ -2144,7 +2186,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             const size_t wall_bytes = (size_t)ctx->max_wall_in * bm * PV * 4;
             ctx->depth = ctx->variant >= 2 ? ctx->variant : 3;
             if (budget < wall_bytes + 4 * slot_bytes) throw std::string("LDS too small for the wall traces");
-            int pool = (int)((budget - wall_bytes) / slot_bytes) - 2;
+            int pool = std::min((int)((budget - wall_bytes) / slot_bytes) - 2, kPoolMax);
             if (su->lds_slots_cap > 0) pool = std::min(pool, su->lds_slots_cap);
             ctx->pool_slots = std::max(pool, 1);
         }
@@ -2156,7 +2198,6 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         std::vector<std::vector<int32_t>> lptr(nb);
         std::vector<std::vector<int4>> wlists(nb);
         std::vector<int4> h_rec((size_t)nb * N);
-        std::vector<int32_t> h_wst((size_t)nb * N, -1);
         std::vector<uint8_t> global_flag(nb, 0);
         std::vector<int64_t> peak(nb, 0), old_reads(nb, 0), per_reads(nb, 0);
         // levels of every bundle: slots in level order (the reference order
@@ -2307,8 +2348,9 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                         if (r.code == REC_OLD || r.code == REC_PER_OLD) ++old_reads[k];
                         if (r.code == REC_PER_LDS || r.code == REC_PER_OLD || r.code == REC_PER_GLOBAL) ++per_reads[k];
                     }
-                    h_rec[(size_t)k * N + q] = make_int4(sl[q], w[0], w[1], w[2]);
-                    h_wst[(size_t)k * N + q] = overflow ? -1 : sid[q];
+                    const int wst = overflow ? -1 : sid[q];
+                    h_rec[(size_t)k * N + q] =
+                        make_int4((int)((uint32_t)sl[q] | ((uint32_t)(wst + 1) << kCellBits)), w[0], w[1], w[2]);
                 }
             }
         };
@@ -2378,21 +2420,16 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->wcell = ctx->upload(h_wcell.data(), h_wcell.size());
         ctx->wc_off = ctx->upload(h_wc_off.data(), h_wc_off.size());
         {
-            // per item (bundle, slot, direction of the bundle): {cell, face
-            // words} and the pool slot
+            // per item (bundle, slot, direction of the bundle): {cell | pool
+            // slot, face words}
             std::vector<int4> h_irec((size_t)h_item_off[nb]);
-            std::vector<int32_t> h_iwst((size_t)h_item_off[nb]);
             for (int k = 0; k < nb; ++k) {
                 const int B = (int)bundles[k].size();
                 for (int q = 0; q < N; ++q)
-                    for (int b = 0; b < B; ++b) {
-                        const size_t it = (size_t)h_item_off[k] + (size_t)q * B + b;
-                        h_irec[it] = h_rec[(size_t)k * N + q];
-                        h_iwst[it] = h_wst[(size_t)k * N + q];
-                    }
+                    for (int b = 0; b < B; ++b)
+                        h_irec[(size_t)h_item_off[k] + (size_t)q * B + b] = h_rec[(size_t)k * N + q];
             }
             ctx->irec = ctx->upload(h_irec.data(), h_irec.size());
-            ctx->iwst = ctx->upload(h_iwst.data(), h_iwst.size());
         }
         ctx->bundle_global = ctx->upload(global_flag.data(), global_flag.size());
         for (int k = 0; k < nb; ++k) {
@@ -2515,7 +2552,8 @@ template <int P, typename T> size_t sweep_lds_bytes(const pbte_ctx* ctx)
 template <int P, typename T, bool PF, bool GR, int DEPTH>
 void launch_pipe(pbte_ctx* ctx, const SweepArgs<T>& a, size_t lds)
 {
-    auto* fn = sweep_pipe_kernel<P, T, PF, GR, DEPTH>;
+    auto* fn = ctx->max_bundle_dirs == 1 ? sweep_pipe_kernel<P, T, PF, GR, DEPTH, 1>
+                                         : sweep_pipe_kernel<P, T, PF, GR, DEPTH, 0>;
     if (lds > 65536)
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
@@ -2535,7 +2573,6 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
     a.wcell = ctx->wcell; a.wc_off = ctx->wc_off;
     a.pbuf = pbuf; a.vdf = vdf; a.N = ctx->N; a.F = ctx->F; a.use_pbuf = use_pbuf; a.homogeneous = homogeneous;
     a.irec = ctx->irec;
-    a.iwst = ctx->iwst;
     a.spill_row = reinterpret_cast<T*>(ctx->spill_row);
     a.bundle_global = ctx->bundle_global;
     a.lane_zero = 0;
