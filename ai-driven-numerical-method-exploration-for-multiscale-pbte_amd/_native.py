@@ -71,12 +71,15 @@ _SIGS = {
     "pbte_transport_residual": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, _f64p]),
     "pbte_residual_iterate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _f64p, _f64p]),
+    "pbte_residual_iterate_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbte_periodic_gather": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "pbte_periodic_scatter": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "pbte_vec_multidot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                     C.c_int64, _f64p]),
     "pbte_vec_update": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, _f64p,
                                   C.c_void_p, C.c_int64]),
+    "pbte_vec_cgs2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
+                                C.c_int64, _f64p]),
     "pbte_vec_axpby": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.c_void_p,
                                  C.c_int64]),
     "pbte_gsis_setup": (C.c_int, [C.c_void_p, C.POINTER(PbteGsisData)]),

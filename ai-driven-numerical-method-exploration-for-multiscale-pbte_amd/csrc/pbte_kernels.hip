@@ -1210,6 +1210,59 @@ __global__ void __launch_bounds__(256) moments_kernel(const T* __restrict__ vdf,
     ssraw[(size_t)l * N + i] = ss;
 }
 
+// the same sums, V consecutive (cell, dof) values per thread read as one
+// 16-byte word per direction, eight directions' words in flight before the
+// in-order accumulation (each value's sum keeps the order and expressions
+// of moments_kernel).  Needs N*P % V == 0 and a 16-byte aligned vdf.
+template <typename T, int V>
+__global__ void __launch_bounds__(256) moments_vec_kernel(const T* __restrict__ vdf,
+                                                          const double* __restrict__ cxv,
+                                                          const double* __restrict__ cyv,
+                                                          const double* __restrict__ dw, int D, int N,
+                                                          int P, double cv, double* ts, double* qxs,
+                                                          double* qys, double* ssraw)
+{
+    static_assert(V * sizeof(T) == 16, "one 16-byte word per direction");
+    using W = typename Vec16<T>::type;
+    constexpr int U = 8;
+    const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * V;
+    if (g0 >= (int64_t)N * P) return;
+    const size_t stride = (size_t)N * P / V;
+    const W* f = reinterpret_cast<const W*>(vdf) + g0 / V;
+    double ss[V], ssx[V], ssy[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) ss[e] = ssx[e] = ssy[e] = 0.0;
+    auto acc = [&](const W& x, int d) {
+        const T* t = reinterpret_cast<const T*>(&x);
+        const double w = dw[d], cx = cxv[d], cy = cyv[d];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const double v = (double)t[e];
+            ss[e] += v * w;
+            ssx[e] += cx * v * w;
+            ssy[e] += cy * v * w;
+        }
+    };
+    int d = 0;
+    for (; d + U <= D; d += U) {
+        W x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = f[(size_t)(d + u) * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc(x[u], d + u);
+    }
+    for (; d < D; ++d) acc(f[(size_t)d * stride], d);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int64_t g = g0 + e;
+        const int i = (int)(g / P), l = (int)(g - (int64_t)i * P);
+        ts[(size_t)l * N + i] = ss[e] / cv;
+        qxs[(size_t)l * N + i] = ssx[e];
+        qys[(size_t)l * N + i] = ssy[e];
+        ssraw[(size_t)l * N + i] = ss[e];
+    }
+}
+
 __global__ void __launch_bounds__(256) integrals_kernel(const double* ssraw, const double* qxs,
                                                         const double* qys, const double* int_tri,
                                                         int N, int P, double cv, double* temp,
@@ -1385,19 +1438,37 @@ __global__ void __launch_bounds__(256) rowsum_kernel(const double* in, int nrow,
     if (threadIdx.x == 0) out[g] = v[0];
 }
 
-// residual and mass of an iterate (_kernels.py:340-361), one workgroup; the
-// sums run in the fixed block order above instead of the reference's
-// sequential index order
-__global__ void __launch_bounds__(256) iterate_residual_kernel(const double* temp, const double* t_old, int n,
-                                                               double* out)
+// residual and mass of an iterate (_kernels.py:340-361) in two passes: per
+// workgroup partial sums over a grid-stride partition, then one workgroup
+// sums the partials in block order.  The order is fixed (a run repeats
+// bitwise), but it is not the reference's sequential index order.
+constexpr int kIterBlocks = 64;
+
+__global__ void __launch_bounds__(256) iterate_partial_kernel(const double* temp, const double* t_old, int n,
+                                                              double* partial)
 {
     __shared__ double sh[3][256];
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < n; i += 256) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const double dlt = temp[i] - t_old[i];
         v[0] += dlt * dlt;
         v[1] += temp[i] * temp[i];
         v[2] += temp[i];
+    }
+    block_sum256<3>(v, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) partial[k * gridDim.x + blockIdx.x] = v[k];
+    }
+}
+
+__global__ void __launch_bounds__(256) iterate_residual_kernel(const double* partial, int nbk, double* out)
+{
+    __shared__ double sh[3][256];
+    double v[3] = {0.0, 0.0, 0.0};
+    if ((int)threadIdx.x < nbk) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[k] = partial[k * nbk + threadIdx.x];
     }
     block_sum256<3>(v, sh);
     if (threadIdx.x == 0) {
@@ -2723,15 +2794,28 @@ extern "C" int pbte_moments(pbte_ctx* ctx, int32_t precision, const void* vdf, d
                             double* qys, double* temp, double* qx, double* qy)
 {
     return guarded(ctx, [&]() {
-        const unsigned g = nblocks((int64_t)ctx->N * ctx->P, 256);
-        if (precision == PBTE_FP64)
-            hipLaunchKernelGGL(moments_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
-                               ctx->cxv, ctx->cyv, ctx->domega, ctx->D, ctx->N, ctx->P, ctx->cv, ts, qxs, qys,
-                               ctx->ssraw);
-        else
-            hipLaunchKernelGGL(moments_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (const float*)vdf,
-                               ctx->cxv, ctx->cyv, ctx->domega, ctx->D, ctx->N, ctx->P, ctx->cv, ts, qxs, qys,
-                               ctx->ssraw);
+        const int64_t np = (int64_t)ctx->N * ctx->P;
+        const bool aligned = (reinterpret_cast<uintptr_t>(vdf) & 15u) == 0;
+        const unsigned g = nblocks(np, 256);
+        if (precision == PBTE_FP64) {
+            if (aligned && np % 2 == 0)
+                hipLaunchKernelGGL((moments_vec_kernel<double, 2>), dim3(nblocks(np / 2, 256)), dim3(256), 0,
+                                   ctx->stream, (const double*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->D, ctx->N,
+                                   ctx->P, ctx->cv, ts, qxs, qys, ctx->ssraw);
+            else
+                hipLaunchKernelGGL(moments_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
+                                   ctx->cxv, ctx->cyv, ctx->domega, ctx->D, ctx->N, ctx->P, ctx->cv, ts, qxs, qys,
+                                   ctx->ssraw);
+        } else {
+            if (aligned && np % 4 == 0)
+                hipLaunchKernelGGL((moments_vec_kernel<float, 4>), dim3(nblocks(np / 4, 256)), dim3(256), 0,
+                                   ctx->stream, (const float*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->D, ctx->N,
+                                   ctx->P, ctx->cv, ts, qxs, qys, ctx->ssraw);
+            else
+                hipLaunchKernelGGL(moments_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (const float*)vdf,
+                                   ctx->cxv, ctx->cyv, ctx->domega, ctx->D, ctx->N, ctx->P, ctx->cv, ts, qxs, qys,
+                                   ctx->ssraw);
+        }
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(integrals_kernel, dim3(nblocks(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->ssraw,
                            qxs, qys, ctx->int_tri, ctx->N, ctx->P, ctx->cv, temp, qx, qy);
@@ -2789,13 +2873,31 @@ extern "C" int pbte_transport_residual(pbte_ctx* ctx, const double* ts, const do
     });
 }
 
+// residual and mass into out[0], out[1] (device memory), stream-ordered
+static void launch_iterate_residual(pbte_ctx* ctx, const double* temp, const double* t_old, double* out)
+{
+    const int nbk = (int)std::min<int64_t>(kIterBlocks, std::max<int64_t>(1, ((int64_t)ctx->N + 255) / 256));
+    if ((size_t)3 * nbk > ctx->red_len) throw std::string("reduction buffer too small");
+    hipLaunchKernelGGL(iterate_partial_kernel, dim3(nbk), dim3(256), 0, ctx->stream, temp, t_old, ctx->N, ctx->red);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(iterate_residual_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->red, nbk, out);
+    HIP_TRY(hipGetLastError());
+}
+
+extern "C" int pbte_residual_iterate_async(pbte_ctx* ctx, const double* temp, const double* t_old, double* out_dev)
+{
+    return guarded(ctx, [&]() {
+        if (!out_dev) throw std::string("residual_iterate_async needs an output");
+        launch_iterate_residual(ctx, temp, t_old, out_dev);
+        return 0;
+    });
+}
+
 extern "C" int pbte_residual_iterate(pbte_ctx* ctx, const double* temp, const double* t_old, double* res_host,
                                      double* mass_host)
 {
     return guarded(ctx, [&]() {
-        hipLaunchKernelGGL(iterate_residual_kernel, dim3(1), dim3(256), 0, ctx->stream, temp, t_old, ctx->N,
-                           ctx->small);
-        HIP_TRY(hipGetLastError());
+        launch_iterate_residual(ctx, temp, t_old, ctx->small);
         double* h = ctx->host_staging(2);
         HIP_TRY(hipMemcpyAsync(h, ctx->small, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -2840,6 +2942,43 @@ extern "C" int pbte_vec_update(pbte_ctx* ctx, const double* Q, int64_t ld, int32
                            w, n);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+// one classical Gram-Schmidt pass applied twice (krylov.py:203-207) on the
+// device: c1 = Q w, w -= Q^T c1, c2 = Q w, w -= Q^T c2, then |w|^2; the
+// coefficients never leave the device until the single copy of
+// (c1, c2, |w|^2) at the end.  The same kernels and reduction order as
+// pbte_vec_multidot / pbte_vec_update / a multidot of w with itself, so
+// the results are those of that sequence bit for bit.
+extern "C" int pbte_vec_cgs2(pbte_ctx* ctx, const double* Q, int64_t ld, int32_t m, double* w, int64_t n,
+                             double* out_host)
+{
+    return guarded(ctx, [&]() {
+        if (m <= 0) throw std::string("cgs2 needs at least one basis vector");
+        if (m > kMaxBasis) throw std::string("too many basis vectors");
+        const int nbk = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + 255) / 256));
+        if ((size_t)m * nbk + 2 * (size_t)m + 1 > ctx->red_len) throw std::string("too many basis vectors");
+        double* coef = ctx->red + (size_t)m * nbk;          // c1 [m], c2 [m], |w|^2
+        const dim3 grid(nbk, (m + kRows - 1) / kRows);
+        for (int pass = 0; pass < 2; ++pass) {
+            double* c = coef + (size_t)pass * m;
+            hipLaunchKernelGGL(multidot_kernel, grid, dim3(256), 0, ctx->stream, Q, ld, m, w, n, ctx->red);
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, c);
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(update_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, Q, ld, m, c, w, n);
+            HIP_TRY(hipGetLastError());
+        }
+        hipLaunchKernelGGL(multidot_kernel, dim3(nbk, 1), dim3(256), 0, ctx->stream, w, n, 1, w, n, ctx->red);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(rowsum_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->red, 1, nbk, 1, 1, coef + 2 * m);
+        HIP_TRY(hipGetLastError());
+        double* h = ctx->host_staging((size_t)2 * m + 1);
+        HIP_TRY(hipMemcpyAsync(h, coef, sizeof(double) * (2 * (size_t)m + 1), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::memcpy(out_host, h, sizeof(double) * (2 * (size_t)m + 1));
         return 0;
     });
 }

@@ -183,13 +183,9 @@ def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None
             if precond is not None:
                 precond(w)
             nmv += 1
-            c1 = ctx.multidot(Q, j + 1, w)
-            ctx.update(Q, j + 1, c1, w)
-            c2 = ctx.multidot(Q, j + 1, w)
-            ctx.update(Q, j + 1, c2, w)
+            hcol, hn = ctx.cgs2(Q, j + 1, w)
             col = np.empty(j + 2)
-            col[:j + 1] = c1 + c2
-            hn = ctx.norm(w)
+            col[:j + 1] = hcol
             col[j + 1] = hn
             for i in range(j):
                 t = cs[i] * col[i] + sn[i] * col[i + 1]
@@ -289,25 +285,52 @@ def run_krylov(solver, callback=None, progress_every: int = 0):
 
 
 class _History:
+    """Residual and mass histories (``krylov.py:289-308``).  The mass of
+    every sweep is summed on the device into a slot of a device buffer
+    without waiting for it; ``finish`` reads the buffer back once."""
+
+    _CHUNK = 4096
+
     def __init__(self, solver, callback, progress_every, want_true):
         self.s, self.cb, self.every = solver, callback, progress_every
         self.residuals, self.masses, self.true_res = [], [], []
         self.want_true = want_true
+        self._chunks = []
+        self._ctx = None
+
+    def _slot(self, k):
+        return self._chunks[k // self._CHUNK][k % self._CHUNK]
 
     def sweep_done(self, op):
-        mass = op.ctx.residual_iterate(op._temp, op._temp)[1]
-        self.masses.append(mass)
+        k = len(self.masses)
+        if k // self._CHUNK >= len(self._chunks):
+            self._chunks.append(dev_zeros((self._CHUNK, 2)))
+        self._ctx = op.ctx
+        op.ctx.residual_iterate_async(op._temp, op._temp, self._slot(k))
+        self.masses.append(np.nan)
         self.residuals.append(np.nan)
         if self.want_true:
             self.true_res.append(np.nan)
+
+    def _mass(self, k):
+        return float(self._slot(k)[1].item())
 
     def close(self, res):
         self.residuals[-1] = res
         step = len(self.residuals)
         if self.every and step % self.every == 0:
-            print(f"  step {step:8d}  residual {res:.6e}  mass {self.masses[-1]:.10f}", flush=True)
+            print(f"  step {step:8d}  residual {res:.6e}  mass {self._mass(len(self.masses) - 1):.10f}",
+                  flush=True)
         if self.cb is not None:
             self.cb(step, res, self.s)
+
+    def finish(self):
+        """Read every sweep's mass back into ``masses``."""
+        n = len(self.masses)
+        if n:
+            host = torch.cat(self._chunks).cpu().numpy()
+            self.masses = [float(v) for v in host[:n, 1]]
+        return self.masses
 
 
 def _cap(solver, op):
@@ -379,6 +402,7 @@ def _run_fp64(solver, op, callback, progress_every):
         target = max(target * 1.0e-3, _FLOOR * scale)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    h.finish()
     return solver._record(len(h.residuals), converged, h.residuals, h.masses,
                           h.true_res if h.want_true else None, wall, scheme="KRYLOV",
                           extra={"krylov_size": op.n, "krylov_periodic_state": op.nper,
@@ -462,6 +486,7 @@ def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.
         rnorm = residual()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    h.finish()
     return solver._record(len(h.residuals), converged, h.residuals, h.masses,
                           h.true_res if h.want_true else None, wall, scheme="KRYLOV",
                           extra={"krylov_size": op.n, "krylov_periodic_state": op.nper,

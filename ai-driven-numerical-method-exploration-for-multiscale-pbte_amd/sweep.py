@@ -263,6 +263,27 @@ class SweepContext:
                 c[k0:].ctypes.data_as(C.POINTER(C.c_double)), ptr(w), w.numel())
             _native.check(self.handle, st, "update")
 
+    def cgs2(self, Q, m: int, w):
+        """Gram-Schmidt of ``w`` against ``Q[:m]`` applied twice, on the device
+        (``krylov.py:203-207``); returns ``(c1 + c2, |w|)`` with ``w``
+        orthogonalised in place.  Bitwise the multidot/update sequence."""
+        if m > self._ROWS:
+            c1 = self.multidot(Q, m, w)
+            self.update(Q, m, c1, w)
+            c2 = self.multidot(Q, m, w)
+            self.update(Q, m, c2, w)
+            return c1 + c2, self.norm(w)
+        out = np.zeros(2 * m + 1)
+        st = _native.lib().pbte_vec_cgs2(self.handle, ptr(Q), Q.stride(0), m, ptr(w), w.numel(),
+                                         out.ctypes.data_as(C.POINTER(C.c_double)))
+        _native.check(self.handle, st, "cgs2")
+        return out[:m] + out[m:2 * m], float(np.sqrt(max(out[2 * m], 0.0)))
+
+    def residual_iterate_async(self, temp_d, t_old_d, out_d) -> None:
+        """Residual and mass into the device pair ``out_d`` without waiting."""
+        st = _native.lib().pbte_residual_iterate_async(self.handle, ptr(temp_d), ptr(t_old_d), ptr(out_d))
+        _native.check(self.handle, st, "residual_iterate_async")
+
     def axpby(self, a: float, x, b: float, y) -> None:
         st = _native.lib().pbte_vec_axpby(self.handle, float(a), ptr(x), float(b), ptr(y), y.numel())
         _native.check(self.handle, st, "axpby")

@@ -164,6 +164,12 @@ int pbte_transport_residual(pbte_ctx *ctx, const double *ts, const double *qxs,
 int pbte_residual_iterate(pbte_ctx *ctx, const double *temp, const double *t_old,
                           double *res_host, double *mass_host);
 
+/* The same two values into out_dev[0], out_dev[1] (device memory),
+ * stream-ordered, without waiting: for histories read back once at the
+ * end of a run (the Krylov driver's per-sweep mass, krylov.py:300-305). */
+int pbte_residual_iterate_async(pbte_ctx *ctx, const double *temp, const double *t_old,
+                                double *out_dev);
+
 /* Periodic partner DOFs of the Krylov state (krylov.py:64-126): gather
  * vdf[d][periodic_tri[f]][:] for every periodic face f (ascending) into
  * out[D][K][P] (binary64), or scatter such a block into pbuf[D][F][P]
@@ -182,6 +188,14 @@ int pbte_vec_multidot(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m,
 /* w[i] -= sum_k c[k] Q[k][i] (c from the host). */
 int pbte_vec_update(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m,
                     const double *c_host, double *w, int64_t n);
+
+/* Classical Gram-Schmidt applied twice to w against the m rows of Q
+ * (krylov.py:203-207): out_host[0..m) = c1, out_host[m..2m) = c2 (w is
+ * left as w - Q^T c1 - Q^T c2), out_host[2m] = |w|^2 after both passes.
+ * Same kernels and reduction order as two multidot/update pairs and a
+ * multidot of w with itself, with one device-to-host copy; m <= 512. */
+int pbte_vec_cgs2(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m, double *w, int64_t n,
+                  double *out_host);
 
 /* y[i] = a x[i] + b y[i] */
 int pbte_vec_axpby(pbte_ctx *ctx, double a, const double *x, double b, double *y, int64_t n);
