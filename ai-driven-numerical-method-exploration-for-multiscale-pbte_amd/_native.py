@@ -80,6 +80,11 @@ _SIGS = {
                                   C.c_void_p, C.c_int64]),
     "pbte_vec_cgs2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                 C.c_int64, _f64p]),
+    "pbte_vec_cgs2_begin": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
+                                      C.c_int64, C.c_void_p]),
+    "pbte_vec_cgs2_end": (C.c_int, [C.c_void_p, _f64p]),
+    "pbte_vec_defect": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_int64]),
     "pbte_vec_axpby": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.c_void_p,
                                  C.c_int64]),
     "pbte_gsis_setup": (C.c_int, [C.c_void_p, C.POINTER(PbteGsisData)]),

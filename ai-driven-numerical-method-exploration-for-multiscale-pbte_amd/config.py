@@ -190,6 +190,12 @@ class Performance:
     # step falls below this (or stalls), then binary64 sweeps to the
     # tolerance; an addition of this build
     refine_switch: float = 1.0e-6
+    # fp32 Krylov: 0 decides convergence as the binary64 driver does (the
+    # transport residual below iteration.tol); > 0 stops the refinement
+    # once the estimated relative L2 error of the temperature and heat-flux
+    # moments is below this, the accuracy asked of a mixed-precision solve;
+    # an addition of this build
+    refine_tol: float = 0.0
 
     def __post_init__(self):
         self.precompute_inverse = bool(self.precompute_inverse)
@@ -207,6 +213,9 @@ class Performance:
         self.refine_switch = float(self.refine_switch)
         if not self.refine_switch > 0.0:
             raise ValueError("performance.refine_switch must be > 0")
+        self.refine_tol = float(self.refine_tol)
+        if not self.refine_tol >= 0.0:
+            raise ValueError("performance.refine_tol must be >= 0")
 
 
 @dataclass

@@ -202,6 +202,10 @@ struct pbte_ctx {
     int ev_used = 0;
     double* hbuf = nullptr;    // pinned host staging
     size_t hbuf_len = 0;
+    // Gram-Schmidt coefficients read back behind an event (pbte_vec_cgs2_begin/end)
+    double* cg_host = nullptr;
+    hipEvent_t cg_event = nullptr;
+    int cg_m = 0;
 
     // GSIS data
     bool gsis = false;
@@ -231,8 +235,10 @@ struct pbte_ctx {
     ~pbte_ctx()
     {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (cg_event) (void)hipEventDestroy(cg_event);
         for (void* p : owned) (void)hipFree(p);
         if (hbuf) (void)hipHostFree(hbuf);
+        if (cg_host) (void)hipHostFree(cg_host);
     }
     double* host_staging(size_t n)
     {
@@ -1315,24 +1321,33 @@ __global__ void __launch_bounds__(256) wall_flux_kernel(const T* __restrict__ vd
 }
 
 // ==========================================================================
-// transport residual ||A f - b||^2 (_kernels.py:274-306), binary64: one
-// thread per (cell, direction), workgroup partials per direction
+// transport residual ||A f - b||^2 (_kernels.py:274-306), binary64, per
+// direction.  A workgroup covers 256 cells and 64 directions, each wave 64
+// consecutive cells in turn with one lane per direction, so the element
+// tensors are wave-uniform (scalar) loads, read once per 64 directions, and
+// only the distribution rows are per lane.  Per direction a lane sums its
+// 64 cells in index order, then the four waves are added in wave order:
+// partial[d][block][2] (numerator, denominator).
 // ==========================================================================
 template <int P>
-__global__ void __launch_bounds__(256) residual_kernel(
-    const double* __restrict__ vdf, const double* __restrict__ src, const double* cxv, const double* cyv,
-    const double* dw, const double* mass, const double* gx, const double* gy, const double* fcm,
-    const CellGeo<double>* geo, const double* K, const double* bvec, const double* flux_wall,
-    double inv_tau_c, int N, double* partial)
+__global__ void __launch_bounds__(256) residual_cells_kernel(
+    const double* __restrict__ vdf, const double* __restrict__ src, const double* __restrict__ cxv,
+    const double* __restrict__ cyv, const double* __restrict__ dw, const double* __restrict__ mass,
+    const double* __restrict__ gx, const double* __restrict__ gy, const double* __restrict__ fcm,
+    const CellGeo<double>* __restrict__ geo, const double* __restrict__ K, const double* __restrict__ bvec,
+    const double* __restrict__ flux_wall, double inv_tau_c, int D, int N, double* __restrict__ partial)
 {
     constexpr int DEG = Deg<P>::v;
     constexpr int Q = DEG + 1;
-    const int d = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int d = blockIdx.y * 64 + lane;
+    const int dd = d < D ? d : D - 1;
+    const double cx = cxv[dd], cy = cyv[dd], wd = dw[dd];
+    const double* vd = vdf + (size_t)dd * N * P;
     double num = 0.0, den = 0.0;
-    if (i < N) {
-        const double cx = cxv[d], cy = cyv[d];
-        const double* vd = vdf + (size_t)d * N * P;
+    const int c0 = __builtin_amdgcn_readfirstlane(blockIdx.x * 256 + wave * 64);
+    const int c1 = min(c0 + 64, N);
+    for (int i = c0; i < c1; ++i) {
         double b[P], f[P];
         const double* sv = src + (size_t)i * Lay<P>::SRC;
 #pragma unroll
@@ -1347,24 +1362,27 @@ __global__ void __launch_bounds__(256) residual_kernel(
             const double sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
             wout[il] = 0.5 * (sp + fabs(sp));
             const double w = 0.5 * (sp - fabs(sp));
-            if (w == 0.0) continue;
             const int info = g.info[il];
             const int code = info & 3;
             if (code <= FACE_PERIODIC) {
                 const int ilp = (info >> 2) & 3;
                 const double* fe = vd + (size_t)g.other[il] * P;
                 const double* Kf = K + ((size_t)i * 3 + il) * Lay<P>::KQ;
+                if (w != 0.0) {
 #pragma unroll
-                for (int r = 0; r < Q; ++r) {
-                    double acc = b[face_node(DEG, il, r)];
+                    for (int r = 0; r < Q; ++r) {
+                        double acc = b[face_node(DEG, il, r)];
 #pragma unroll
-                    for (int q = 0; q < Q; ++q) acc -= (w * Kf[r * Q + q]) * fe[face_node(DEG, ilp, q)];
-                    b[face_node(DEG, il, r)] = acc;
+                        for (int q = 0; q < Q; ++q) acc -= (w * Kf[r * Q + q]) * fe[face_node(DEG, ilp, q)];
+                        b[face_node(DEG, il, r)] = acc;
+                    }
                 }
             } else if (code == FACE_ADIABATIC) {
+                if (w != 0.0) {
 #pragma unroll
-                for (int m = 0; m < P; ++m) b[m] += w * flux_wall[(size_t)g.other[il] * P + m];
-            } else {
+                    for (int m = 0; m < P; ++m) b[m] += w * flux_wall[(size_t)g.other[il] * P + m];
+                }
+            } else if (w != 0.0) {
 #pragma unroll
                 for (int m = 0; m < P; ++m) b[m] -= w * bvec[(size_t)g.other[il] * pad4(P) + m];
             }
@@ -1382,24 +1400,23 @@ __global__ void __launch_bounds__(256) residual_kernel(
                 a = a + wout[2] * fc[(2 * P + m) * P + l];
                 r += a * f[l];
             }
-            num += r * r * dw[d];
-            den += b[m] * b[m] * dw[d];
+            num += r * r * wd;
+            den += b[m] * b[m] * wd;
         }
     }
-    __shared__ double sn[256], sd[256];
-    sn[threadIdx.x] = num;
-    sd[threadIdx.x] = den;
+    __shared__ double sn[4][64], sd[4][64];
+    sn[wave][lane] = num;
+    sd[wave][lane] = den;
     __syncthreads();
-    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-            sn[threadIdx.x] += sn[threadIdx.x + h];
-            sd[threadIdx.x] += sd[threadIdx.x + h];
+    if (wave == 0 && d < D) {
+        double a = sn[0][lane], c = sd[0][lane];
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            a += sn[k][lane];
+            c += sd[k][lane];
         }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        partial[((size_t)d * gridDim.x + blockIdx.x) * 2 + 0] = sn[0];
-        partial[((size_t)d * gridDim.x + blockIdx.x) * 2 + 1] = sd[0];
+        partial[((size_t)d * gridDim.x + blockIdx.x) * 2 + 0] = a;
+        partial[((size_t)d * gridDim.x + blockIdx.x) * 2 + 1] = c;
     }
 }
 
@@ -1528,6 +1545,29 @@ __global__ void __launch_bounds__(256) axpby_kernel(double a, const double* x, d
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     y[i] = a * x[i] + b * y[i];
+}
+
+// y = (1 / |w|) w with |w|^2 read from the device: axpby_kernel's
+// expression with a = 1 / sqrt(max(|w|^2, 0)) and b = 0
+__global__ void __launch_bounds__(256) normalize_kernel(const double* __restrict__ n2, const double* x, double* y,
+                                                        int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = n2[0];
+    const double a = 1.0 / sqrt(v > 0.0 ? v : 0.0), b = 0.0;
+    y[i] = a * x[i] + b * y[i];
+}
+
+// out = (v - t) + g, the order of out = v; out += -t; out += g
+__global__ void __launch_bounds__(256) fixed_point_defect_kernel(const double* v, const double* t, const double* g,
+                                                                 double* out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double o = -1.0 * t[i] + 1.0 * v[i];
+    if (g) o = 1.0 * g[i] + 1.0 * o;
+    out[i] = o;
 }
 
 // periodic partner DOFs as Krylov state: gather vdf[d][ptri[k]][:] into a
@@ -2849,13 +2889,13 @@ extern "C" int pbte_transport_residual(pbte_ctx* ctx, const double* ts, const do
         if (ctx->n_wall && !flux_wall) throw std::string("adiabatic walls need flux_wall");
         launch_source<double>(ctx, ts, qxs, qys, ctx->src64, pad4(3 * ctx->P));
         const unsigned gx = nblocks(ctx->N, 256);
-        dim3 grid(gx, ctx->D);
         double* part = ctx->red;
         double* out = ctx->red + (size_t)ctx->D * gx * 2;
         if ((size_t)ctx->D * gx * 2 + 2 * (size_t)ctx->D > ctx->red_len) throw std::string("reduction buffer too small");
-#define RS(PP) case PP: hipLaunchKernelGGL(residual_kernel<PP>, grid, dim3(256), 0, ctx->stream, vdf, ctx->src64, \
-        ctx->cxv, ctx->cyv, ctx->domega, ctx->mass, ctx->gx, ctx->gy, ctx->fcm, ctx->geo64, ctx->K64, ctx->bvec64, \
-        flux_wall, 1.0 / ctx->tau_c, ctx->N, part); break;
+        const dim3 cgrid(gx, (ctx->D + 63) / 64);
+#define RS(PP) case PP: hipLaunchKernelGGL(residual_cells_kernel<PP>, cgrid, dim3(256), 0, ctx->stream, vdf, \
+        ctx->src64, ctx->cxv, ctx->cyv, ctx->domega, ctx->mass, ctx->gx, ctx->gy, ctx->fcm, ctx->geo64, ctx->K64, \
+        ctx->bvec64, flux_wall, 1.0 / ctx->tau_c, ctx->D, ctx->N, part); break;
         switch (ctx->P) { RS(3) RS(6) RS(10) RS(15) }
 #undef RS
         HIP_TRY(hipGetLastError());
@@ -2979,6 +3019,76 @@ extern "C" int pbte_vec_cgs2(pbte_ctx* ctx, const double* Q, int64_t ld, int32_t
         HIP_TRY(hipMemcpyAsync(h, coef, sizeof(double) * (2 * (size_t)m + 1), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         std::memcpy(out_host, h, sizeof(double) * (2 * (size_t)m + 1));
+        return 0;
+    });
+}
+
+// The same Gram-Schmidt, returning at once: the coefficients are copied to
+// pinned host memory behind an event, and (next != NULL) w / |w| is written
+// into next on the device, so the caller can queue the next product before
+// it waits for the coefficients in pbte_vec_cgs2_end.
+extern "C" int pbte_vec_cgs2_begin(pbte_ctx* ctx, const double* Q, int64_t ld, int32_t m, double* w, int64_t n,
+                                   double* next)
+{
+    return guarded(ctx, [&]() {
+        if (m <= 0) throw std::string("cgs2 needs at least one basis vector");
+        if (m > kMaxBasis) throw std::string("too many basis vectors");
+        if (ctx->cg_m) throw std::string("cgs2_begin: the previous call was not ended");
+        const int nbk = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + 255) / 256));
+        if ((size_t)m * nbk + 2 * (size_t)m + 1 > ctx->red_len) throw std::string("too many basis vectors");
+        if (!ctx->cg_host) {
+            HIP_TRY(hipHostMalloc((void**)&ctx->cg_host, sizeof(double) * (2 * kMaxBasis + 1), hipHostMallocDefault));
+            HIP_TRY(hipEventCreateWithFlags(&ctx->cg_event, hipEventDisableTiming));
+        }
+        double* coef = ctx->red + (size_t)m * nbk;
+        const dim3 grid(nbk, (m + kRows - 1) / kRows);
+        for (int pass = 0; pass < 2; ++pass) {
+            double* c = coef + (size_t)pass * m;
+            hipLaunchKernelGGL(multidot_kernel, grid, dim3(256), 0, ctx->stream, Q, ld, m, w, n, ctx->red);
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, c);
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(update_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, Q, ld, m, c, w, n);
+            HIP_TRY(hipGetLastError());
+        }
+        hipLaunchKernelGGL(multidot_kernel, dim3(nbk, 1), dim3(256), 0, ctx->stream, w, n, 1, w, n, ctx->red);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(rowsum_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->red, 1, nbk, 1, 1, coef + 2 * m);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(ctx->cg_host, coef, sizeof(double) * (2 * (size_t)m + 1), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->cg_event, ctx->stream));
+        if (next) {
+            hipLaunchKernelGGL(normalize_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, coef + 2 * m, w,
+                               next, n);
+            HIP_TRY(hipGetLastError());
+        }
+        ctx->cg_m = m;
+        return 0;
+    });
+}
+
+extern "C" int pbte_vec_cgs2_end(pbte_ctx* ctx, double* out_host)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->cg_m) throw std::string("cgs2_end without cgs2_begin");
+        const int m = ctx->cg_m;
+        ctx->cg_m = 0;
+        HIP_TRY(hipEventSynchronize(ctx->cg_event));
+        std::memcpy(out_host, ctx->cg_host, sizeof(double) * (2 * (size_t)m + 1));
+        return 0;
+    });
+}
+
+// out = (v - t) + g (g may be NULL: out = v - t), the fixed-point defect of
+// a product in the same rounding as a copy and two axpby calls
+extern "C" int pbte_vec_defect(pbte_ctx* ctx, const double* v, const double* t, const double* g, double* out,
+                               int64_t n)
+{
+    return guarded(ctx, [&]() {
+        hipLaunchKernelGGL(fixed_point_defect_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, v, t, g, out,
+                           n);
+        HIP_TRY(hipGetLastError());
         return 0;
     });
 }

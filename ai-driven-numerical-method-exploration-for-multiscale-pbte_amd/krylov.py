@@ -21,7 +21,10 @@ Precision (``performance.precision``):
           solved by GMRES whose products are homogeneous binary32 sweeps
           (boundary sources removed, moments summed in binary64); ``x += e``.
           The last ``F64(x)`` is the closing sweep.  Convergence is decided
-          by the binary64 transport residual, as in the reference.
+          by the binary64 transport residual, as in the reference, or, with
+          ``performance.refine_tol > 0``, by the estimated relative error of
+          the temperature and heat-flux moments (the accuracy a
+          mixed-precision solve is asked for).
 """
 from __future__ import annotations
 
@@ -63,6 +66,8 @@ class FixedPointOperator:
         self.n = self.nmom + self.nwall + self.nper
         self.sweeps = 0
         self.sweeps64 = 0
+        self.discarded = 0
+        self._last = "fp64"
         solver.ctx.use_pbuf = self.per
         self._temp = dev_zeros(self.nt)
         self._qx = dev_zeros(self.nt)
@@ -89,6 +94,7 @@ class FixedPointOperator:
                 ctx.handle, code, ptr(pb), ptr(buf)), "periodic_scatter")
         ctx.sweep(None, vdf, homogeneous=homogeneous, flux_wall=fw, src=(ts, qxs, qys))
         self.sweeps += 1
+        self._last = precision
         if precision == "fp64":
             self.sweeps64 += 1
         ots, oqxs, oqys, ofw, opb = self._blocks(out)
@@ -98,6 +104,14 @@ class FixedPointOperator:
         if self.per:
             _native.check(ctx.handle, _native.lib().pbte_periodic_gather(
                 ctx.handle, code, ptr(vdf), ptr(opb)), "periodic_gather")
+
+    def discard(self) -> None:
+        """Uncount the last product: it was computed ahead and not used."""
+        self.sweeps -= 1
+        self.discarded += 1
+        self.ctx.sweep_count -= 1
+        if self._last == "fp64":
+            self.sweeps64 -= 1
 
     def publish(self, u) -> None:
         """Copy a state's moment block (and emission) into the solver."""
@@ -135,14 +149,22 @@ def _back_substitute(r, y):
 
 
 def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None, callback=None,
-               precond=None):
+               precond=None, overlap=False):
     """GMRES with a device basis; ``matvec(v, out)`` writes ``A v``.
 
     Follows ``krylov.py:150-246``: restarts at ``max_basis``, CGS2, Givens
     rotations, ``callback(res)`` once per product (true stops), and the
     optional left preconditioner ``precond(v)`` (in place) applied to every
     residual and product.  Returns ``(x, converged, n_matvec)``; ``x`` is
-    updated in place."""
+    updated in place.
+
+    ``overlap`` (no preconditioner; ``matvec`` takes ``record=False`` and
+    has ``matvec.commit()`` and ``matvec.discard()``): the next basis vector
+    is normalised on the device and its product queued before the host
+    waits for the Gram-Schmidt coefficients, so the Givens work and the
+    launches overlap the sweep.  A queued product is committed (counted,
+    its history entry made) when the loop reaches it, and discarded when
+    the loop stops first; the arithmetic is that of the plain loop."""
     n = b.numel()
     x = dev_zeros(n) if x is None else x
     maxiter = int(n if maxiter is None else maxiter)
@@ -178,12 +200,24 @@ def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None
             Q = dev_zeros((m + 1, n))
         ctx.axpby(1.0 / beta, r, 0.0, Q[0])
         cols, cs, sn, gam = [], [], [], [beta]
+        ahead = overlap and precond is None
+        pending = False                         # w holds the queued product of Q[j]
         for j in range(m):
-            matvec(Q[j], w)
+            if pending:
+                matvec.commit()
+                pending = False
+            else:
+                matvec(Q[j], w)
             if precond is not None:
                 precond(w)
             nmv += 1
-            hcol, hn = ctx.cgs2(Q, j + 1, w)
+            if ahead and j + 1 < m and j + 1 <= ctx._ROWS:
+                ctx.cgs2_begin(Q, j + 1, w, next_row=Q[j + 1])
+                matvec(Q[j + 1], w, record=False)
+                pending = True
+                hcol, hn = ctx.cgs2_end()
+            else:
+                hcol, hn = ctx.cgs2(Q, j + 1, w)
             col = np.empty(j + 2)
             col[:j + 1] = hcol
             col[j + 1] = hn
@@ -209,8 +243,11 @@ def gmres_full(ctx, matvec, b, x=None, tol_abs=0.0, maxiter=None, max_basis=None
                 break
             if hn == 0.0:
                 break
-            if j + 1 < m:
+            if j + 1 < m and not pending:
                 ctx.axpby(1.0 / hn, w, 0.0, Q[j + 1])
+        if pending:
+            matvec.discard()
+            pending = False
         k = len(cols)
         if k:
             rmat = np.zeros((k, k))
@@ -357,13 +394,14 @@ def _run_fp64(solver, op, callback, progress_every):
     h.close(1.0)
     tmp = dev_zeros(op.n)
 
-    def matvec(v, out):
+    def matvec(v, out, record=True):
         op.apply(v, tmp)
-        h.sweep_done(op)
-        # out = v - (F(v) - g)
-        out.copy_(v)
-        ctx.axpby(-1.0, tmp, 1.0, out)
-        ctx.axpby(1.0, g, 1.0, out)
+        if record:
+            h.sweep_done(op)
+        ctx.defect(v, tmp, g, out)              # out = v - (F(v) - g)
+
+    matvec.commit = lambda: h.sweep_done(op)
+    matvec.discard = op.discard
 
     enough = max(tol * _ENOUGH_FACTOR, _FLOOR) * scale
     target = max(tol * c.scheme.krylov_tol_factor, _FLOOR) * scale
@@ -388,7 +426,7 @@ def _run_fp64(solver, op, callback, progress_every):
             break
         reached = -1
         x, _, _ = gmres_full(ctx, matvec, g, x=x, tol_abs=target, maxiter=budget, max_basis=cap,
-                             callback=on_residual, precond=precond)
+                             callback=on_residual, precond=precond, overlap=callback is None)
         op.apply(x, fx)                                     # closing sweep
         h.sweep_done(op)
         op.publish(fx)
@@ -409,7 +447,8 @@ def _run_fp64(solver, op, callback, progress_every):
                                  "krylov_basis_cap": cap, "krylov_precond": precond is not None,
                                  "precond_calls": precond.calls if precond is not None else 0,
                                  "transport_residual": final, "sweeps_fp64": op.sweeps64,
-                                 "sweeps_fp32": op.sweeps - op.sweeps64})
+                                 "sweeps_fp32": op.sweeps - op.sweeps64,
+                                 "sweeps_discarded": op.discarded})
 
 
 def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.0e-2,
@@ -434,13 +473,15 @@ def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.
 
     rnorm = residual()
     scale = rnorm or 1.0                        # = ||g|| when x starts at 0
-    h.close(1.0)
 
-    def matvec(v, out):                         # (I - T32) v, homogeneous
+    def matvec(v, out, record=True):            # (I - T32) v, homogeneous
         op.apply(v, tmp, "fp32", homogeneous=True)
-        h.sweep_done(op)
-        out.copy_(v)
-        ctx.axpby(-1.0, tmp, 1.0, out)
+        if record:
+            h.sweep_done(op)
+        ctx.defect(v, tmp, None, out)
+
+    matvec.commit = lambda: h.sweep_done(op)
+    matvec.discard = op.discard
 
     # the defect the refinement drives to: the level at which the binary64
     # driver counts its GMRES residual as enough (two digits below the
@@ -452,13 +493,24 @@ def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.
     refinements = 0
     e = dev_zeros(op.n)
     precond = _precond(solver)
+    # performance.refine_tol > 0: the error of x after a refinement is about
+    # rho / (1 - rho) times that refinement's correction, rho the defect
+    # reduction it achieved (the next corrections shrink by rho each); the
+    # refinement stops when that estimate, relative to x, is a tenth of
+    # refine_tol for the temperature moments and for the heat-flux moments
+    refine_tol = c.performance.refine_tol
+    kt = op.nd * op.nt
+    est = np.inf
     while True:
         op.publish(fx)
         final = float(ctx.true_residual(solver.mom, solver.vdf_d))
         if h.want_true:
             h.true_res[-1] = final
         h.close(final if refinements else 1.0)
-        converged = bool(final < tol) and rnorm <= target
+        if refine_tol > 0.0:
+            converged = rnorm == 0.0 or est <= 0.1 * refine_tol
+        else:
+            converged = bool(final < tol) and rnorm <= target
         if converged or refinements >= max_refine or len(h.residuals) >= tmax:
             break
         budget = tmax - len(h.residuals) - 1
@@ -480,10 +532,17 @@ def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.
             return best[1] > stall_products
 
         gmres_full(ctx, matvec, r, x=e, tol_abs=max(inner_reduction * rnorm, 0.1 * target),
-                   maxiter=budget, max_basis=cap, callback=inner, precond=precond)
+                   maxiter=budget, max_basis=cap, callback=inner, precond=precond, overlap=callback is None)
         ctx.axpby(1.0, e, 1.0, x)
         refinements += 1
-        rnorm = residual()
+        rnorm_old, rnorm = rnorm, residual()
+        if refine_tol > 0.0:
+            rho = rnorm / rnorm_old if rnorm_old > 0.0 else 0.0
+            est = np.inf
+            if rho < 1.0:
+                grow = rho / (1.0 - rho)
+                parts = ((e[:kt], x[:kt]), (e[kt:3 * kt], x[kt:3 * kt]))
+                est = max(grow * ctx.norm(de) / max(ctx.norm(xx), np.finfo(float).tiny) for de, xx in parts)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     h.finish()
@@ -493,5 +552,7 @@ def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.
                                  "krylov_basis_cap": cap, "krylov_precond": precond is not None,
                                  "precond_calls": precond.calls if precond is not None else 0,
                                  "transport_residual": final, "refinements": refinements,
-                                 "final_defect": rnorm / scale, "sweeps_fp64": op.sweeps64,
-                                 "sweeps_fp32": op.sweeps - op.sweeps64})
+                                 "final_defect": rnorm / scale, "refine_tol": refine_tol,
+                                 "refine_error_estimate": est, "sweeps_fp64": op.sweeps64,
+                                 "sweeps_fp32": op.sweeps - op.sweeps64,
+                                 "sweeps_discarded": op.discarded})

@@ -279,6 +279,27 @@ class SweepContext:
         _native.check(self.handle, st, "cgs2")
         return out[:m] + out[m:2 * m], float(np.sqrt(max(out[2 * m], 0.0)))
 
+    def cgs2_begin(self, Q, m: int, w, next_row=None) -> None:
+        """Queue :meth:`cgs2` (``m <= 512``) and, with ``next_row``,
+        ``next_row = w / |w|`` on the device; :meth:`cgs2_end` collects."""
+        st = _native.lib().pbte_vec_cgs2_begin(self.handle, ptr(Q), Q.stride(0), m, ptr(w), w.numel(),
+                                               None if next_row is None else ptr(next_row))
+        _native.check(self.handle, st, "cgs2_begin")
+        self._cg_m = m
+
+    def cgs2_end(self):
+        m = self._cg_m
+        out = np.zeros(2 * m + 1)
+        st = _native.lib().pbte_vec_cgs2_end(self.handle, out.ctypes.data_as(C.POINTER(C.c_double)))
+        _native.check(self.handle, st, "cgs2_end")
+        return out[:m] + out[m:2 * m], float(np.sqrt(max(out[2 * m], 0.0)))
+
+    def defect(self, v, t, g, out) -> None:
+        """``out = (v - t) + g`` (``g`` None: ``v - t``)."""
+        st = _native.lib().pbte_vec_defect(self.handle, ptr(v), ptr(t), None if g is None else ptr(g),
+                                           ptr(out), out.numel())
+        _native.check(self.handle, st, "defect")
+
     def residual_iterate_async(self, temp_d, t_old_d, out_d) -> None:
         """Residual and mass into the device pair ``out_d`` without waiting."""
         st = _native.lib().pbte_residual_iterate_async(self.handle, ptr(temp_d), ptr(t_old_d), ptr(out_d))

@@ -89,7 +89,7 @@ def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitte
         ],
         "restart": {"enabled": False, "error_on_stale": False},
         "output": {"field": False, "runtime_log": False, "run_record": False},
-        "performance": {"precision": precision},
+        "performance": {"precision": precision, "refine_tol": 1e-6 if precision == "fp32" else 0.0},
     }, basedir=str(workdir))
 
 

@@ -197,6 +197,22 @@ int pbte_vec_update(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m,
 int pbte_vec_cgs2(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m, double *w, int64_t n,
                   double *out_host);
 
+/* pbte_vec_cgs2 split in two so that the next Krylov product can be queued
+ * while the coefficients travel: _begin queues the same kernels, a copy of
+ * (c1, c2, |w|^2) to pinned host memory behind an event and, when next is
+ * not NULL, next = w / |w| on the device (axpby's rounding); _end waits for
+ * the event and writes the 2m+1 values to out_host.  One pair in flight
+ * per context. */
+int pbte_vec_cgs2_begin(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m, double *w,
+                        int64_t n, double *next);
+int pbte_vec_cgs2_end(pbte_ctx *ctx, double *out_host);
+
+/* out = (v - t) + g, or v - t when g is NULL: the operator product of the
+ * Krylov drivers, A v = v - (F(v) - g) (krylov.py:332-337), in the
+ * rounding of a copy followed by two axpby calls. */
+int pbte_vec_defect(pbte_ctx *ctx, const double *v, const double *t, const double *g, double *out,
+                    int64_t n);
+
 /* y[i] = a x[i] + b y[i] */
 int pbte_vec_axpby(pbte_ctx *ctx, double a, const double *x, double b, double *y, int64_t n);
 

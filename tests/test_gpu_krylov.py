@@ -36,11 +36,14 @@ def test_krylov_fp64_reaches_certified_fixed_point(pkg, tag):
     assert abs(rec.mass - float(z["mass"])) <= 1e-9
 
 
-@pytest.mark.parametrize("tag", ["F1_1_1", "F2_1_1", "F3_1_1", "F1_1e-2"])
-def test_krylov_fp32_refined_matches_fp64(pkg, tag):
+@pytest.mark.parametrize("refine_tol", [0.0, 1e-6], ids=["residual", "tq1e-6"])
+@pytest.mark.parametrize("tag", ["F1_1_1", "F2_1_1", "F3_1_1", "F1_1e-2", "F1_1e-3"])
+def test_krylov_fp32_refined_matches_fp64(pkg, tag, refine_tol):
     """FP32 sweeps inside the FP64 defect correction reach the FP64 fixed
-    point within 1e-6 relative L2 on T and on the heat flux."""
-    rec = _run(pkg, tag, "fp32")
+    point within 1e-6 relative L2 on T and on the heat flux, whether the
+    refinement stops on the transport residual or on its own estimate of
+    the T and q error (performance.refine_tol)."""
+    rec = _run(pkg, tag, "fp32", performance__refine_tol=refine_tol)
     z = np.load(GOLDEN / "krylov" / f"{tag}.npz")
     assert rec.converged
     rel_t = np.linalg.norm(rec.temp - z["temp"]) / np.linalg.norm(z["temp"])
@@ -64,3 +67,26 @@ def test_krylov_moment_preconditioner_reaches_certified_fixed_point(pkg, tag):
     assert rec.diagnostics["transport_residual"] < 1e-8
     np.testing.assert_allclose(rec.temp_dofs, z["temp_dofs"], rtol=0, atol=1e-9)
     assert abs(rec.mass - float(z["mass"])) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_krylov_lookahead_product_changes_nothing(pkg, precision):
+    """With no user callback GMRES queues the next product before it reads
+    the Gram-Schmidt coefficients back; with a callback it does not.  Both
+    runs give the same iterates bit for bit, the same histories and the
+    same sweep count (a queued product the loop never reaches is
+    discarded and uncounted)."""
+    c = pkg.Case.from_yaml(CASES / "F1_1e-1.yaml")
+    c.scheme.method = "krylov"
+    c.performance.precision = precision
+    print(f"case krylov lookahead F1_1e-1 {precision}", flush=True)
+    fast = pkg.Solver(c).run()
+    seen = []
+    plain = pkg.Solver(c).run(callback=lambda step, res, s: seen.append(res))
+    assert fast.converged and plain.converged
+    assert fast.iterations == plain.iterations == len(seen)
+    assert fast.sweep_count == plain.sweep_count
+    np.testing.assert_array_equal(fast.temp_dofs, plain.temp_dofs)
+    np.testing.assert_array_equal(np.asarray(fast.residual_history), np.asarray(plain.residual_history))
+    np.testing.assert_array_equal(np.asarray(fast.mass_history), np.asarray(plain.mass_history))
+    assert fast.diagnostics["sweeps_discarded"] > 0 and plain.diagnostics["sweeps_discarded"] == 0
