@@ -35,7 +35,8 @@ class PbteSetup(C.Structure):
                 ("periodic_tri", _i32p), ("other_lfc", _i32p), ("bc_type", _i32p),
                 ("bc_temp", _f64p), ("face_tri", _i32p), ("face_lfc", _i32p),
                 ("order", _i32p), ("max_bundle", C.c_int32), ("block", C.c_int32),
-                ("sweep_variant", C.c_int32), ("lds_slots_cap", C.c_int32)]
+                ("sweep_variant", C.c_int32), ("lds_slots_cap", C.c_int32),
+                ("lds_value_bytes", C.c_int32)]
 
 
 class PbteGsisData(C.Structure):

@@ -176,6 +176,7 @@ struct pbte_ctx {
     int rs64 = 0, rs32 = 0;     // their row strides
     uint8_t* bundle_global = nullptr;
     int pool_slots = 0;         // LDS trace slots per workgroup
+    size_t lds_blk = 65536;     // LDS bytes one workgroup may take
     int depth = 3;              // levels between a gather and its solve, plus one
     int64_t pool_peak = 0;      // most slots one bundle holds at once
     int64_t global_bundles = 0;
@@ -1923,6 +1924,25 @@ std::vector<double> reversed_edge_matrix(int deg)
     return kr;
 }
 
+// workgroups of `block` threads of the binary32 pipelined sweep kernel that
+// one CU holds at once by their registers (LDS aside); 0 when unknown
+template <int P>
+int pipe_blocks_per_cu(int depth, bool one_dir, int block)
+{
+    constexpr bool pf = P * P * sizeof(float) <= 288;
+    const void* fn =
+        depth == 2 ? (one_dir ? reinterpret_cast<const void*>(sweep_pipe_kernel<P, float, pf, false, 2, 1>)
+                              : reinterpret_cast<const void*>(sweep_pipe_kernel<P, float, pf, false, 2, 0>))
+                   : (one_dir ? reinterpret_cast<const void*>(sweep_pipe_kernel<P, float, pf, false, 3, 1>)
+                              : reinterpret_cast<const void*>(sweep_pipe_kernel<P, float, pf, false, 3, 0>));
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, block, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
 }  // namespace
 
 extern "C" const char* pbte_last_error(pbte_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -2280,9 +2300,12 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             ctx->max_wall_in = most;
         }
         // LDS pool: what one workgroup may hold when every CU runs its share
-        // of the bundles at once, sized at binary32; a binary64 sweep takes
-        // twice the bytes (and by its register use runs one workgroup of up
-        // to 256 threads per CU in any case)
+        // of the bundles at once, counted at binary32.  For binary64 sweeps
+        // (lds_value_bytes 8) half of a workgroup's LDS: a binary64 sweep
+        // takes twice the bytes (and by its register use runs one workgroup
+        // of up to 256 threads per CU in any case).  For binary32 sweeps all
+        // of it; a binary64 sweep whose pool then does not fit runs the
+        // plain level kernel.
         {
             int dev = 0, ncu = 1, lds_blk = 65536, lds_cu = 0;
             HIP_TRY(hipGetDevice(&dev));
@@ -2291,11 +2314,23 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
                     hipSuccess || lds_cu <= 0)
                 lds_cu = lds_blk;
-            const int per_cu = std::max(1, (nb + std::max(ncu, 1) - 1) / std::max(ncu, 1));
-            const size_t budget = std::min<size_t>((size_t)lds_blk / 2, (size_t)lds_cu / per_cu);
+            ctx->depth = ctx->variant >= 2 ? ctx->variant : 3;
+            // bundles a CU runs at once: its share of them, but no more
+            // workgroups than its registers hold (the widest workgroup)
+            int per_cu = std::max(1, (nb + std::max(ncu, 1) - 1) / std::max(ncu, 1));
+            int held = 0;
+            switch (P) {
+            case 3: held = pipe_blocks_per_cu<3>(ctx->depth, bm == 1, 256); break;
+            case 6: held = pipe_blocks_per_cu<6>(ctx->depth, bm == 1, 256); break;
+            case 10: held = pipe_blocks_per_cu<10>(ctx->depth, bm == 1, 256); break;
+            case 15: held = pipe_blocks_per_cu<15>(ctx->depth, bm == 1, 256); break;
+            }
+            if (held > 0) per_cu = std::min(per_cu, held);
+            const size_t blk = su->lds_value_bytes == 4 ? (size_t)lds_blk : (size_t)lds_blk / 2;
+            const size_t budget = std::min<size_t>(blk, (size_t)lds_cu / per_cu);
+            ctx->lds_blk = (size_t)lds_blk;
             const size_t slot_bytes = (size_t)bm * pad4(3 * TQ) * 4;
             const size_t wall_bytes = (size_t)ctx->max_wall_in * bm * PV * 4;
-            ctx->depth = ctx->variant >= 2 ? ctx->variant : 3;
             if (budget < wall_bytes + 4 * slot_bytes) throw std::string("LDS too small for the wall traces");
             int pool = std::min((int)((budget - wall_bytes) / slot_bytes) - 2, kPoolMax);
             if (su->lds_slots_cap > 0) pool = std::min(pool, su->lds_slots_cap);
@@ -2310,7 +2345,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         std::vector<std::vector<int4>> wlists(nb);
         std::vector<int4> h_rec((size_t)nb * N);
         std::vector<uint8_t> global_flag(nb, 0);
-        std::vector<int64_t> peak(nb, 0), old_reads(nb, 0), per_reads(nb, 0);
+        std::vector<int64_t> peak(nb, 0), old_reads(nb, 0), per_reads(nb, 0), slots_used(nb, 0);
         // levels of every bundle: slots in level order (the reference order
         // within a level) and the level boundaries
         std::vector<std::vector<int32_t>> lcnt(nb);
@@ -2445,6 +2480,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                         busy.push({last_reader[u], id});
                         peak[k] = std::max<int64_t>(peak[k], (int64_t)busy.size());
                     }
+                    slots_used[k] = overflow ? 0 : used;
                 }
                 global_flag[k] = overflow ? 1 : 0;
                 for (int q = 0; q < N; ++q) {
@@ -2466,6 +2502,9 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             }
         };
         run_threads(work);
+        // the pool shrinks to the slots the bundles that kept it use (every
+        // slot id is below that count; the zero and spill slots follow it)
+        ctx->pool_slots = (int)std::max<int64_t>(1, *std::max_element(slots_used.begin(), slots_used.end()));
         int64_t maxw = 0;
         std::vector<int4> h_wcell;
         std::vector<int32_t> h_wc_off(nb + 1, 0);
@@ -2696,7 +2735,9 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
     if (timed) HIP_TRY(hipEventRecord(ctx->events[2 * ctx->ev_used], ctx->stream));
     if (onthefly) {
         hipLaunchKernelGGL((sweep_otf_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
-    } else if (ctx->variant == 1) {
+    } else if (ctx->variant == 1 || sweep_lds_bytes<P, T>(ctx) > ctx->lds_blk) {
+        // the plain level kernel: asked for, or a binary64 sweep whose pool,
+        // sized for binary32, exceeds a workgroup's LDS
         hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
     } else {
         const size_t lds = sweep_lds_bytes<P, T>(ctx);

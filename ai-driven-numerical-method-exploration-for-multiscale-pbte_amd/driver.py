@@ -36,7 +36,7 @@ __all__ = ["Solver"]
 class Solver:
     def __init__(self, case: Case, validate_order: bool = False, on_cycle: str | None = None,
                  max_bundle: int = 0, block: int = 0, sweep_variant: int = 0,
-                 lds_slots_cap: int = 0):
+                 lds_slots_cap: int = 0, max_bytes: float | None = None):
         t0 = time.perf_counter()
         require_gpu()
         self.case = c = case
@@ -47,7 +47,8 @@ class Solver:
         self.ndir, self.n_tris, self.ndof = self.vel.ndir, self.mesh.n_tris, c.ndof_tri
         self.ctx = SweepContext.build(c, self.mesh, self.integrals, self.vel, self.order,
                                       self.bcdata, max_bundle=max_bundle, block=block,
-                                      variant=sweep_variant, lds_slots_cap=lds_slots_cap)
+                                      variant=sweep_variant, lds_slots_cap=lds_slots_cap,
+                                      max_bytes=max_bytes)
         self.factorisation_count = self.ndir * self.n_tris
         self.precision = c.performance.precision
         self.sweep_precision = self.precision     # fp32 runs finish in fp64

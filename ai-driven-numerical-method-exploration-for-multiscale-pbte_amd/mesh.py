@@ -16,7 +16,9 @@ The writers follow the reference's mesh family convention
 (``solver-python/tools/make_meshes.py:60-140``): nodes numbered x-major
 (``id = i*(n+1) + j + 1``), boundary lines south (+x), north (-x), east (+y),
 west (-y) with physical ids 11..14, and per-quad diagonals chosen by
-``(2i >= n) == (2j >= n)`` (on ``j`` only for the periodic channel).
+``(2i >= n) == (2j >= n)`` (on ``j`` only for the periodic channel).  The
+square with a circular hole (physical id 16) is this build's own: the
+reference has no mesher for it.
 """
 from __future__ import annotations
 
@@ -315,13 +317,20 @@ _HEADER = ("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$PhysicalNames\n5\n"
 
 
 def square_mesh_text(n: int, layout: str = "cavity", distribution: str = "uniform",
-                     jitter: float = 0.0, seed: int = 0) -> str:
+                     jitter: float = 0.0, seed: int = 0, hole=None) -> str:
     """gmsh 2.2 text of an ``n x n`` triangulated unit square.
 
     ``jitter`` moves every interior node by ``U(-jitter*h, +jitter*h)`` in x and
     y, drawn from ``numpy.random.default_rng(seed)`` in node-id order (x draw
     then y draw per node); ``h = 1/n``.  Boundary nodes stay on the boundary
     lines so that wall physical ids are unchanged.
+
+    ``hole = (x0, y0, r)`` removes every triangle whose centroid lies within
+    ``r`` of ``(x0, y0)`` (after the jitter); the edges left with one triangle
+    inside the square become the boundary "Hole" (physical id 16), each line
+    oriented as its triangle's edge (the domain on its left, as the outer
+    walls), and the nodes no triangle uses are dropped (ids renumbered in
+    order).  The hole is a staircase of mesh edges around the circle.
     """
     if distribution not in ("uniform", "smootherstep"):
         raise ValueError(f"unknown distribution {distribution!r}")
@@ -348,12 +357,8 @@ def square_mesh_text(n: int, layout: str = "cavity", distribution: str = "unifor
     def nid(i, j):
         return i * nn + j + 1
 
-    out = [_HEADER.format(**names), "$Nodes\n", f"{nn * nn:6d}\n"]
-    ids = np.arange(1, nn * nn + 1)
     xs = X.reshape(-1)
     ys = Y.reshape(-1)
-    out.extend(f"{k:6d} {x:.16f} {y:.16f} 0\n" for k, x, y in zip(ids, xs, ys))
-    out.append("$EndNodes\n")
     lines = []
     lines += [(11, 5, nid(i, 0), nid(i + 1, 0)) for i in range(n)]
     lines += [(12, 6, nid(i, n), nid(i - 1, n)) for i in range(n, 0, -1)]
@@ -369,6 +374,41 @@ def square_mesh_text(n: int, layout: str = "cavity", distribution: str = "unifor
             else:
                 tris.append((nid(i, j), nid(i + 1, j), nid(i, j + 1)))
                 tris.append((nid(i, j + 1), nid(i + 1, j), nid(i + 1, j + 1)))
+    header = _HEADER.format(**names)
+    if hole is not None:
+        x0, y0, r = (float(v) for v in hole)
+        t = np.asarray(tris, dtype=np.int64) - 1
+        gx = xs[t].mean(axis=1)
+        gy = ys[t].mean(axis=1)
+        t = t[(gx - x0) ** 2 + (gy - y0) ** 2 > r * r]
+        # edges used by one kept triangle and not on the outer boundary
+        outer = {tuple(sorted((a - 1, b - 1))) for _, _, a, b in lines}
+        count = {}
+        for tri in t:
+            for k in range(3):
+                e = (int(tri[k]), int(tri[(k + 1) % 3]))
+                key = (min(e), max(e))
+                count[key] = count.get(key, 0) + 1
+        hole_lines = []
+        for tri in t:
+            for k in range(3):
+                a0, b0 = int(tri[k]), int(tri[(k + 1) % 3])
+                key = (min(a0, b0), max(a0, b0))
+                if count[key] == 1 and key not in outer:
+                    hole_lines.append((a0, b0))
+        used = np.zeros(nn * nn, dtype=bool)
+        used[t.reshape(-1)] = True
+        new = np.cumsum(used) * used                     # 1-based new ids, 0 = dropped
+        xs, ys = xs[used], ys[used]
+        lines = [(p, g, int(new[a - 1]), int(new[b - 1])) for p, g, a, b in lines]
+        lines += [(16, 9, int(new[a]), int(new[b])) for a, b in hole_lines]
+        tris = [tuple(int(new[v]) for v in tri) for tri in t]
+        header = header.replace("$PhysicalNames\n5\n", "$PhysicalNames\n6\n").replace(
+            "$EndPhysicalNames", '1 16 "Hole (physical id 16)"\n$EndPhysicalNames')
+    out = [header, "$Nodes\n", f"{xs.size:6d}\n"]
+    ids = np.arange(1, xs.size + 1)
+    out.extend(f"{k:6d} {x:.16f} {y:.16f} 0\n" for k, x, y in zip(ids, xs, ys))
+    out.append("$EndNodes\n")
     out.append("$Elements\n")
     out.append(f"{len(lines) + len(tris):6d}\n")
     k = 0

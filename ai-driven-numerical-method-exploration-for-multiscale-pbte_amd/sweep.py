@@ -56,6 +56,7 @@ class SweepContext:
         self.handle = None
         self.sweep_count = 0
         self.mode = "precomputed"
+        self._precomputed = set()               # precisions with stored factors
         self.kernel = "hip"
         self.use_pbuf = False
         self.pbuf = None
@@ -69,7 +70,10 @@ class SweepContext:
         ``max_bytes`` choose the mode as the reference does (sweep.py:104-110):
         precomputed factors when wanted and their bytes fit, else "onthefly".
         The default cap is half of the device memory (the reference's 8e9
-        is sized for host RAM)."""
+        is sized for host RAM).  An fp32 run stores factors in both
+        precisions; when only the binary32 ones fit, the mode is "mixed":
+        binary32 sweeps use stored factors, binary64 sweeps build theirs on
+        the fly."""
         self = cls()
         cxv, cyv, domega = vel.flat()
         self.cxv, self.cyv, self.domega = cxv, cyv, domega
@@ -110,6 +114,7 @@ class SweepContext:
         su.order = i32(sweep_order.order)
         su.max_bundle, su.block, su.sweep_variant = int(max_bundle), int(block), int(variant)
         su.lds_slots_cap = int(lds_slots_cap)
+        su.lds_value_bytes = 4 if case.performance.precision == "fp32" else 8
         err = C.create_string_buffer(512)
         torch.cuda.synchronize()
         h = _native.lib().pbte_ctx_create(C.byref(dims), C.byref(su), C.c_void_p(stream_handle()),
@@ -124,14 +129,18 @@ class SweepContext:
         if max_bytes is None:
             max_bytes = 0.5 * torch.cuda.get_device_properties(0).total_memory
         nbytes = self.operator_bytes(self.n_tris, self.nd)
-        if self.precision == "fp32":
-            nbytes *= 1.5                       # binary64 and binary32 factors
-        if precompute and nbytes <= max_bytes:
+        self.mode = "onthefly"
+        self._precomputed = set()
+        if precompute and self.precision == "fp32" and 1.5 * nbytes > max_bytes >= 0.5 * nbytes:
+            # binary32 factors fit, binary64 ones beside them do not: the
+            # binary32 sweeps use stored factors, the binary64 sweeps of the
+            # refinement build theirs on the fly
+            self.factorise("fp32")
+            self.mode = "mixed"
+        elif precompute and nbytes * (1.5 if self.precision == "fp32" else 1.0) <= max_bytes:
             self.factorise("fp64")
             if self.precision == "fp32":
                 self.factorise("fp32")
-        else:
-            self.mode = "onthefly"
         return self
 
     def __del__(self):
@@ -173,6 +182,7 @@ class SweepContext:
         if bad:
             raise np.linalg.LinAlgError(
                 f"{bad} element operators are singular; check TAU_C and the mesh")
+        self._precomputed.add(precision)
         self.mode = "precomputed"
 
     def ensure_pbuf(self, dtype):
@@ -192,7 +202,8 @@ class SweepContext:
         pb = None
         if self.use_pbuf:
             pb = self.ensure_pbuf(vdf.dtype)
-        fn = _native.lib().pbte_sweep if self.mode == "precomputed" else _native.lib().pbte_sweep_onthefly
+        stored = ("fp64" if prec == _native.PBTE_FP64 else "fp32") in self._precomputed
+        fn = _native.lib().pbte_sweep if stored else _native.lib().pbte_sweep_onthefly
         st = fn(self.handle, prec, ptr(ts), ptr(qxs), ptr(qys), ptr(vdf), ptr(fw), ptr(pb),
                 1 if self.use_pbuf else 0, 1 if homogeneous else 0)
         _native.check(self.handle, st, "sweep")

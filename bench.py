@@ -16,10 +16,12 @@ Prints one JSON line:
                  from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
                  (profiles/sweep_traffic.json), null for other workloads;
   solve          per Kn_R in {0.01, 0.1, 1, 10} (Kn_N = 0.1 Kn_R): time-to-
-                 solution and solves/hour of the FP64 and the FP32-sweep +
-                 FP64-refinement Krylov solves, and the relative L2
-                 differences of T and heat flux between them (--no-solve
-                 skips it);
+                 solution (setup + run, each also reported) and solves/hour
+                 of the FP64 solve (transport residual below 1e-8) and of the
+                 FP32-sweep + FP64-refinement solve (asked for 1e-6 relative
+                 L2 on T and heat flux, performance.refine_tol), and the
+                 relative L2 differences of T and heat flux between them
+                 (--no-solve skips it);
   cpu_baseline   the C restatement of the reference's sweep on this host.
 
 --workload cis runs the plain-CIS configuration: unit square 16 x 16 split
@@ -58,6 +60,11 @@ METRIC = "solves/hour + sweep DOF-updates/s, FP32+refine vs FP64 GPU at 1e-6 rel
 TRAFFIC_FILE = ROOT / "profiles" / "sweep_traffic.json"
 
 
+def log(msg):
+    """Progress on stderr (stdout carries the one JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def committed_traffic(workload):
     """HBM bytes per sweep launch measured by rocprofv3 (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy-kernel calibration) for this workload."""
@@ -69,11 +76,22 @@ def committed_traffic(workload):
     return None if row is None else row.get("hbm_bytes_per_launch")
 
 
-def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitter=0.0, seed=0):
+def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitter=0.0, seed=0, hole=0.0):
+    """Diffuse isothermal walls, the north one at dT = +1; with ``hole`` > 0
+    a central circular hole of that radius whose wall is at dT = +1 and
+    the four outer walls at 0."""
     from pbte_amd.mesh import write_square_mesh
-    mesh = Path(workdir) / f"square_{n}_{jitter}_{seed}.msh"
+    mesh = Path(workdir) / f"square_{n}_{jitter}_{seed}_{hole}.msh"
     if not mesh.exists():
-        write_square_mesh(mesh, n, jitter=jitter, seed=seed)
+        write_square_mesh(mesh, n, jitter=jitter, seed=seed, hole=(0.5, 0.5, hole) if hole > 0 else None)
+    walls = [
+        {"name": "SWall", "phyid": 11, "type": "thermalising", "temp": 0.0},
+        {"name": "NWall", "phyid": 12, "type": "thermalising", "temp": 0.0 if hole > 0 else 1.0},
+        {"name": "EWall", "phyid": 13, "type": "thermalising", "temp": 0.0},
+        {"name": "WWall", "phyid": 14, "type": "thermalising", "temp": 0.0},
+    ]
+    if hole > 0:
+        walls.append({"name": "Hole", "phyid": 16, "type": "thermalising", "temp": 1.0})
     return pkg.Case.from_dict({
         "iteration": {"tol": 1e-8, "tmax": 20000},
         "scheme": {"accflag": 0, "method": "krylov", "krylov_restart": 30},
@@ -81,12 +99,7 @@ def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitte
         "dg": {"deg": deg},
         "flow": {"cv": 1.0, "vg": 1.0, "tau_r": tau_r, "tau_n": tau_n},
         "mesh": {"file": str(mesh)},
-        "boundaries": [
-            {"name": "SWall", "phyid": 11, "type": "thermalising", "temp": 0.0},
-            {"name": "NWall", "phyid": 12, "type": "thermalising", "temp": 1.0},
-            {"name": "EWall", "phyid": 13, "type": "thermalising", "temp": 0.0},
-            {"name": "WWall", "phyid": 14, "type": "thermalising", "temp": 0.0},
-        ],
+        "boundaries": walls,
         "restart": {"enabled": False, "error_on_stale": False},
         "output": {"field": False, "runtime_log": False, "run_record": False},
         "performance": {"precision": precision, "refine_tol": 1e-6 if precision == "fp32" else 0.0},
@@ -284,6 +297,9 @@ def main():
                     help="sweep kernel: 0 default, 1 plain level sweep, 2 / 3 pipelined with the "
                          "gather 1 / 2 levels ahead")
     ap.add_argument("--jitter", type=float, default=0.0)
+    ap.add_argument("--hole", type=float, default=0.0,
+                    help="radius of a central circular hole (hot wall; the outer walls cold)")
+    ap.add_argument("--tau-r", type=float, default=1.0, help="Kn_R of the sweep workload (Kn_N = 0.1 Kn_R)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true", help="skip the Krylov solve leg")
@@ -310,11 +326,13 @@ def main():
             args.npole, args.nazim = 16, 32
         gsis_workload(pkg, args, work)
         return
-    case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, 1.0, 0.1, args.precision,
-                     work, jitter=args.jitter)
+    log(f"mesh {args.n}x{args.n} (hole {args.hole:g}), P{args.deg}, {args.npole}x{args.nazim} ordinates")
+    case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, args.tau_r, 0.1 * args.tau_r,
+                     args.precision, work, jitter=args.jitter, hole=args.hole)
     t0 = time.perf_counter()
     s = pkg.Solver(case, max_bundle=args.bundle, block=args.block, sweep_variant=args.variant)
     setup_s = time.perf_counter() - t0
+    log(f"solver set up in {setup_s:.1f} s ({s.n_tris} triangles, factor mode {s.ctx.mode}); sweeping")
     D, N, P = s.ndir, s.n_tris, s.ndof
     vdf = s.sweep_vdf
     rng = np.random.default_rng(0)
@@ -334,7 +352,8 @@ def main():
     dof = D * N * P
     s_lu = 4 if args.precision == "fp32" else 8
     bytes_alg = b_alg(D, N, P, s_lu, s_lu)
-    workload = (f"square {args.n}x{args.n} -> {N} P{args.deg} triangles, "
+    hole = f" with a central hole r={args.hole:g}" if args.hole > 0 else ""
+    workload = (f"square {args.n}x{args.n}{hole} -> {N} P{args.deg} triangles, "
                 f"{D} ordinates ({args.npole}x{args.nazim}), {args.precision} sweep")
     out = {
         "metric": METRIC,
@@ -355,12 +374,21 @@ def main():
         "sweep_kernel_ms_min": float(np.min(ms)),
         "schedule": s.ctx.schedule_info(),
         "setup_s": setup_s,
+        "factor_mode": s.ctx.mode,
     }
+    if s.ctx.mode == "mixed":
+        # binary64 sweeps of the refinement build their operators on the fly
+        s.ctx.timing(2)
+        for _ in range(2):
+            s.ctx.sweep(s.mom, s.vdf_d)
+        torch.cuda.synchronize()
+        out["fp64_onthefly_sweep_ms"] = float(np.mean(s.ctx.timing_read(2)))
+        s.ctx.timing(0)
     if not args.no_solve:
         out["solve"] = solve_times(pkg, args, work)
     if not args.no_cpu:
-        c64 = make_case(pkg, args.n, args.deg, args.npole, args.nazim, 1.0, 0.1, "fp64", work,
-                        jitter=args.jitter)
+        c64 = make_case(pkg, args.n, args.deg, args.npole, args.nazim, args.tau_r, 0.1 * args.tau_r, "fp64",
+                        work, jitter=args.jitter, hole=args.hole)
         out["cpu_baseline"] = cpu_baseline(pkg, c64, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
@@ -392,10 +420,14 @@ def solve_times(pkg, args, work):
                              work, jitter=args.jitter)
             t0 = time.perf_counter()
             s = pkg.Solver(case)
+            t1 = time.perf_counter()
             rec = s.run()
-            tts = time.perf_counter() - t0
-            res[prec] = {"time_to_solution_s": tts, "setup_s": s.setup_seconds,
+            t2 = time.perf_counter()
+            tts = t2 - t0
+            d = rec.diagnostics
+            res[prec] = {"time_to_solution_s": tts, "setup_s": s.setup_seconds, "run_s": t2 - t1,
                          "iterations": rec.iterations, "converged": rec.converged,
+                         "sweeps_fp64": d.get("sweeps_fp64"), "sweeps_fp32": d.get("sweeps_fp32"),
                          "solves_per_hour": 3600.0 / tts, "temp": rec.temp,
                          "q": np.concatenate([rec.qx, rec.qy])}
             del s
@@ -405,6 +437,7 @@ def solve_times(pkg, args, work):
         res["fp32"]["rel_l2_q_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
         res["speedup_fp32_refine_vs_fp64"] = (res["fp64"]["time_to_solution_s"] /
                                               res["fp32"]["time_to_solution_s"])
+        res["speedup_run_only"] = res["fp64"]["run_s"] / res["fp32"]["run_s"]
         out[f"kn_r={tau_r:g}"] = res
     return out
 

@@ -73,6 +73,12 @@ typedef struct pbte_setup {
     int32_t lds_slots_cap;                /* > 0 caps the LDS trace pool
                                              slots (exercises the global
                                              fallback); 0 = automatic      */
+    int32_t lds_value_bytes;              /* 4: size the LDS trace pool
+                                             for binary32 sweeps (a
+                                             binary64 sweep whose pool then
+                                             exceeds the LDS runs the plain
+                                             level kernel); 0 or 8: for
+                                             binary64 sweeps              */
 } pbte_setup;
 
 /* ---- setup ------------------------------------------------------------ */

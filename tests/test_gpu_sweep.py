@@ -277,3 +277,46 @@ def test_onthefly_cis_reproduces_golden_history(pkg):
     assert rec.iterations == len(g["residual_history"])
     np.testing.assert_allclose(rec.residual_history, g["residual_history"], rtol=1e-8, atol=0)
     np.testing.assert_allclose(rec.temp, g["temp"], rtol=0, atol=1e-12)
+
+
+def _hole_case(pkg, path, n=20, deg=3):
+    from pbte_amd.mesh import write_square_mesh
+    write_square_mesh(path, n, jitter=0.25, seed=0, hole=(0.5, 0.5, 0.2))
+    walls = [{"name": nm, "phyid": p, "type": "thermalising", "temp": 0.0}
+             for nm, p in (("SWall", 11), ("NWall", 12), ("EWall", 13), ("WWall", 14))]
+    walls.append({"name": "Hole", "phyid": 16, "type": "thermalising", "temp": 1.0})
+    return pkg.Case.from_dict({
+        "iteration": {"tol": 1e-8, "tmax": 100},
+        "scheme": {"accflag": 0, "method": "source"},
+        "velmesh": {"npole": 4, "nazim": 8},
+        "dg": {"deg": deg},
+        "flow": {"cv": 1.0, "vg": 1.0, "tau_r": 0.05, "tau_n": 0.005},
+        "mesh": {"file": str(path)},
+        "boundaries": walls,
+        "restart": {"enabled": False, "error_on_stale": False},
+        "output": {"field": False, "runtime_log": False, "run_record": False},
+        "performance": {"precision": "fp32"},
+    }, basedir=str(path.parent))
+
+
+def test_hole_mesh_mixed_factor_mode_matches_oracle(pkg, tmp_path):
+    """The square with a central hot hole (the 362^2 P3 1024-ordinate
+    layout, here 20^2 P3 with 32 ordinates) in the "mixed" factor mode
+    that layout needs: binary32 factors stored, binary64 operators built
+    and factorised on the fly.  Both sweeps against the oracle."""
+    import torch
+    c = _hole_case(pkg, tmp_path / "hole.msh")
+    from pbte_amd.problem import build_problem
+    pb = build_problem(c)
+    nbytes = float(pb.vel.ndir) * pb.mesh.n_tris * c.ndof_tri ** 2 * 8.0
+    s, o = _pair(pkg, c, max_bytes=nbytes)         # binary32 (0.5 nbytes) fits, both (1.5) do not
+    assert s.ctx.mode == "mixed"
+    assert int(np.count_nonzero(np.asarray(pb.bcdata.bc_type) >= 0)) >= 5
+    _seed_state(s, o, seed=21)
+    s.ctx.sweep(s.mom, s.vdf_d)
+    o.sweep()
+    assert _rel(s.vdf_d.cpu().numpy(), o.vdf) <= 1e-12
+    _seed_state(s, o, seed=22)
+    s.ctx.sweep(s.mom, s.vdf32_d)
+    o.sweep()
+    assert _rel(s.vdf32_d.to(torch.float64).cpu().numpy(), o.vdf) <= 2e-5
