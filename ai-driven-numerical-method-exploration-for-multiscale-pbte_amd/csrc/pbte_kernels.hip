@@ -4,19 +4,26 @@
 // Data layout in HBM
 //   vdf        [D][N][P]   the reference's layout (sweep.py:132-158); a
 //                          cell's P values are contiguous, so the sweep
-//                          writes and the upwind gathers are 12-120 B runs and
-//                          the moment reduction reads coalesced rows.
-//   LU factors per bundle [P*P][slot][dir-in-bundle]: element-major, the
-//                          slots of one level contiguous, so a level's LU
-//                          streams as full 64-lane coalesced loads.
-//   pivots     per bundle [slot][dir] uint64, 4 bits per row.
-//   cell data  CellGeo{normals, across-face element, face code} and the
-//                          compact Q x Q upwind face blocks, gathered by cell.
+//                          writes are 12-120 B runs and the moment reduction
+//                          reads coalesced rows.
+//   element inverse per bundle [k / V][item][V], V = 16 B / sizeof(T),
+//                          item = slot * B + direction-in-bundle: the items of
+//                          one level are consecutive, so a level's inverses
+//                          stream as 64-lane coalesced 16-byte loads.
+//   item record per bundle [item] int4: cell | (LDS pool slot + 1) << 21 and
+//                          three face words saying where each upwind trace
+//                          lives (LDS slot, global value, wall, nothing).
+//   cell record [N][rs]    U, VX, VY (the source, rewritten every sweep), the
+//                          face normals times the face length, info words;
+//                          one 128-byte line per cell in binary32 at P = 6.
+//   CellGeo, face blocks   the binary64 tensors of the residual, moment and
+//                          on-the-fly kernels.
 //
 // A bundle is a set of directions with the same inflow sign on every
 // interior and periodic face; such directions have the same reference
 // order and the same level schedule, so one workgroup sweeps all of them
-// together and loads each cell's direction-independent data once.
+// together and loads each cell's direction-independent data once (one
+// direction per bundle by default).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

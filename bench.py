@@ -303,6 +303,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true", help="skip the Krylov solve leg")
+    ap.add_argument("--solve-kn", default="0.01,0.1,1,10",
+                    help="comma-separated Kn_R values of the Krylov solve leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -413,7 +415,7 @@ def calibrate_copy(nbytes, reps):
 def solve_times(pkg, args, work):
     """Krylov solves of the workload at Kn_R in {0.01, 0.1, 1, 10}."""
     out = {}
-    for tau_r in (0.01, 0.1, 1.0, 10.0):
+    for tau_r in (float(v) for v in args.solve_kn.split(",")):
         res = {}
         for prec in ("fp64", "fp32"):
             case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, tau_r, 0.1 * tau_r, prec,
