@@ -865,6 +865,12 @@ template <int P, typename T> struct PipeIn {
     T v[pad4(P)];
 };
 
+// cache policy of the loads of data read once per sweep (item records,
+// element inverses): bit 1 of the gfx940-family buffer-load policy, "nt"
+// (non-temporal), so these streams do not push the cell records, which
+// every direction reads again, out of L2
+constexpr int kStream = 2;
+
 // buffer descriptors of one bundle's streams (raw, bounds-checked: an
 // offset past num_records reads 0)
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t bundle_rsrc(const void* base, uint32_t bytes)
@@ -891,7 +897,7 @@ __device__ __forceinline__ void pipe_record(__amdgpu_buffer_rsrc_t irec, int2 sp
     const int sl = span.x + c;
     o.valid = sl < span.y;
     o.slot = o.valid ? sl : span.x;
-    o.r = bits16<int4>(__builtin_amdgcn_raw_buffer_load_b128(irec, (uint32_t)(o.slot * B + b) * 16u, 0, 0));
+    o.r = bits16<int4>(__builtin_amdgcn_raw_buffer_load_b128(irec, (uint32_t)(o.slot * B + b) * 16u, 0, kStream));
 }
 
 // record-code classes as bit masks over the code: one shift and one test,
@@ -946,7 +952,7 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
 #pragma unroll
         for (int g = 0; g < L::LU / w; ++g) {
             const auto v = bits16<typename Vec16<T>::type>(
-                __builtin_amdgcn_raw_buffer_load_b128(lu, voff, (uint32_t)g * gstride, 0));
+                __builtin_amdgcn_raw_buffer_load_b128(lu, voff, (uint32_t)g * gstride, kStream));
             const T* t = reinterpret_cast<const T*>(&v);
 #pragma unroll
             for (int j = 0; j < w; ++j) o.lu[g * w + j] = t[j];
@@ -1024,7 +1030,7 @@ __device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeDa
     } else {
         const T* lu = a.lu + a.lu_off[a.block_bundle[blockIdx.x]];
         const size_t item = (size_t)dC.slot * B + b, NB = (size_t)a.N * B;
-        apply_inverse<P, T>(x, [&](int k) { return lu[lu_idx<T>(k, item, NB)]; });
+        apply_inverse<P, T>(x, [&](int k) { return __builtin_nontemporal_load(lu + lu_idx<T>(k, item, NB)); });
     }
     // stores without a branch: an idle thread, and an element nobody reads
     // through LDS, write their traces to the spill slot; an idle thread

@@ -112,15 +112,37 @@ class SweepContext:
         su.bc_type, su.bc_temp = i32(bcdata.bc_type), f64(bcdata.bc_temp)
         su.face_tri, su.face_lfc = i32(mesh.face_tri), i32(mesh.face_lfc)
         su.order = i32(sweep_order.order)
-        su.max_bundle, su.block, su.sweep_variant = int(max_bundle), int(block), int(variant)
+        su.block, su.sweep_variant = int(block), int(variant)
         su.lds_slots_cap = int(lds_slots_cap)
         su.lds_value_bytes = 4 if case.performance.precision == "fp32" else 8
+        # max_bundle 0: with at least two directions per CU, pairs of
+        # directions with the same order share a workgroup (and each cell
+        # record load) when their trace pool fits the LDS without any bundle
+        # falling back to global memory; else one direction per workgroup
+        tries = [int(max_bundle)]
+        if max_bundle == 0:
+            ncu = torch.cuda.get_device_properties(0).multi_processor_count
+            tries = [2, 1] if self.ndir >= 2 * ncu else [1]
         err = C.create_string_buffer(512)
         torch.cuda.synchronize()
-        h = _native.lib().pbte_ctx_create(C.byref(dims), C.byref(su), C.c_void_p(stream_handle()),
-                                          err, 512)
-        if not h:
-            raise _native.NativeError(f"pbte_ctx_create: {err.value.decode()}")
+        h = None
+        for k, mb in enumerate(tries):
+            su.max_bundle = mb
+            h = _native.lib().pbte_ctx_create(C.byref(dims), C.byref(su), C.c_void_p(stream_handle()),
+                                              err, 512)
+            last = k == len(tries) - 1
+            if not h:
+                if last:
+                    raise _native.NativeError(f"pbte_ctx_create: {err.value.decode()}")
+                continue
+            if not last:
+                out = (C.c_int64 * 6)()
+                _native.check(h, _native.lib().pbte_schedule_info(h, out), "schedule")
+                if out[4] > 0:                  # a bundle would sweep through global memory
+                    _native.lib().pbte_ctx_destroy(h)
+                    h = None
+                    continue
+            break
         self.handle = h
         self.flux_wall = dev_zeros((max(self.n_faces, 1), self.nd))
         self.precision = case.performance.precision
