@@ -1554,6 +1554,55 @@ __global__ void __launch_bounds__(256) update_kernel(const double* __restrict__ 
     w[i] -= s;
 }
 
+// w -= Q^T c with update_kernel's arithmetic per element, then, over the
+// same pass, the dot products of the updated w with the m rows of Q
+// (NORM: with itself) with multidot_kernel's partition, per-thread order and
+// tree, so the partials are those of an update followed by a multidot bit
+// for bit, for one read of Q instead of two.  Launched on multidot's
+// gridDim.x blocks; m <= MR.
+template <int MR, bool NORM>
+__global__ void __launch_bounds__(256) update_dot_kernel(const double* __restrict__ Qm, int64_t ld, int m,
+                                                         const double* __restrict__ c, double* __restrict__ w,
+                                                         int64_t n, double* __restrict__ partial)
+{
+    constexpr int NR = NORM ? 1 : MR;
+    double acc[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < m; ++k) s += c[k] * Qm[(size_t)k * ld + i];
+        const double wn = w[i] - s;
+        w[i] = wn;
+        if constexpr (NORM) {
+            acc[0] += wn * wn;
+        } else {
+#pragma unroll
+            for (int r = 0; r < MR; ++r)
+                if (r < m) acc[r] += Qm[(size_t)r * ld + i] * wn;
+        }
+    }
+    __shared__ double sh[kRows][256];
+    const int rows = NORM ? 1 : m;
+#pragma unroll
+    for (int g = 0; g < (NR + kRows - 1) / kRows; ++g) {
+        if (g * kRows >= rows) break;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) sh[r][threadIdx.x] = g * kRows + r < NR ? acc[g * kRows + r] : 0.0;
+        __syncthreads();
+        for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h)
+#pragma unroll
+                for (int r = 0; r < kRows; ++r) sh[r][threadIdx.x] += sh[r][threadIdx.x + h];
+            __syncthreads();
+        }
+        if ((int)threadIdx.x < kRows && g * kRows + (int)threadIdx.x < rows)
+            partial[(size_t)(g * kRows + threadIdx.x) * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+        __syncthreads();
+    }
+}
+
 __global__ void __launch_bounds__(256) axpby_kernel(double a, const double* x, double b, double* y, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -3040,6 +3089,46 @@ extern "C" int pbte_vec_update(pbte_ctx* ctx, const double* Q, int64_t ld, int32
     });
 }
 
+// CGS2 into coef = (c1 [m], c2 [m], |w|^2) on the stream: multidot + rowsum
+// for c1, then (m <= 32) update + multidot fused for c2 and update + norm
+// fused for |w|^2 (two reads of Q instead of four), each followed by its
+// rowsum; the same bits as the unfused multidot / update sequence
+static void launch_cgs2(pbte_ctx* ctx, const double* Q, int64_t ld, int m, double* w, int64_t n, int nbk,
+                        double* coef)
+{
+    constexpr int kFuseRows = 32;
+    const dim3 grid(nbk, (m + kRows - 1) / kRows);
+    double* c1 = coef;
+    double* c2 = coef + m;
+    hipLaunchKernelGGL(multidot_kernel, grid, dim3(256), 0, ctx->stream, Q, ld, m, w, n, ctx->red);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, c1);
+    HIP_TRY(hipGetLastError());
+    if (m <= kFuseRows) {
+        hipLaunchKernelGGL((update_dot_kernel<kFuseRows, false>), dim3(nbk), dim3(256), 0, ctx->stream, Q, ld, m, c1,
+                           w, n, ctx->red);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, c2);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((update_dot_kernel<kFuseRows, true>), dim3(nbk), dim3(256), 0, ctx->stream, Q, ld, m, c2,
+                           w, n, ctx->red);
+        HIP_TRY(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(update_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, Q, ld, m, c1, w, n);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(multidot_kernel, grid, dim3(256), 0, ctx->stream, Q, ld, m, w, n, ctx->red);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, c2);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(update_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, Q, ld, m, c2, w, n);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(multidot_kernel, dim3(nbk, 1), dim3(256), 0, ctx->stream, w, n, 1, w, n, ctx->red);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(rowsum_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->red, 1, nbk, 1, 1, coef + 2 * m);
+    HIP_TRY(hipGetLastError());
+}
+
 // one classical Gram-Schmidt pass applied twice (krylov.py:203-207) on the
 // device: c1 = Q w, w -= Q^T c1, c2 = Q w, w -= Q^T c2, then |w|^2; the
 // coefficients never leave the device until the single copy of
@@ -3055,20 +3144,7 @@ extern "C" int pbte_vec_cgs2(pbte_ctx* ctx, const double* Q, int64_t ld, int32_t
         const int nbk = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + 255) / 256));
         if ((size_t)m * nbk + 2 * (size_t)m + 1 > ctx->red_len) throw std::string("too many basis vectors");
         double* coef = ctx->red + (size_t)m * nbk;          // c1 [m], c2 [m], |w|^2
-        const dim3 grid(nbk, (m + kRows - 1) / kRows);
-        for (int pass = 0; pass < 2; ++pass) {
-            double* c = coef + (size_t)pass * m;
-            hipLaunchKernelGGL(multidot_kernel, grid, dim3(256), 0, ctx->stream, Q, ld, m, w, n, ctx->red);
-            HIP_TRY(hipGetLastError());
-            hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, c);
-            HIP_TRY(hipGetLastError());
-            hipLaunchKernelGGL(update_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, Q, ld, m, c, w, n);
-            HIP_TRY(hipGetLastError());
-        }
-        hipLaunchKernelGGL(multidot_kernel, dim3(nbk, 1), dim3(256), 0, ctx->stream, w, n, 1, w, n, ctx->red);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(rowsum_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->red, 1, nbk, 1, 1, coef + 2 * m);
-        HIP_TRY(hipGetLastError());
+        launch_cgs2(ctx, Q, ld, m, w, n, nbk, coef);
         double* h = ctx->host_staging((size_t)2 * m + 1);
         HIP_TRY(hipMemcpyAsync(h, coef, sizeof(double) * (2 * (size_t)m + 1), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -3095,20 +3171,7 @@ extern "C" int pbte_vec_cgs2_begin(pbte_ctx* ctx, const double* Q, int64_t ld, i
             HIP_TRY(hipEventCreateWithFlags(&ctx->cg_event, hipEventDisableTiming));
         }
         double* coef = ctx->red + (size_t)m * nbk;
-        const dim3 grid(nbk, (m + kRows - 1) / kRows);
-        for (int pass = 0; pass < 2; ++pass) {
-            double* c = coef + (size_t)pass * m;
-            hipLaunchKernelGGL(multidot_kernel, grid, dim3(256), 0, ctx->stream, Q, ld, m, w, n, ctx->red);
-            HIP_TRY(hipGetLastError());
-            hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, c);
-            HIP_TRY(hipGetLastError());
-            hipLaunchKernelGGL(update_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, Q, ld, m, c, w, n);
-            HIP_TRY(hipGetLastError());
-        }
-        hipLaunchKernelGGL(multidot_kernel, dim3(nbk, 1), dim3(256), 0, ctx->stream, w, n, 1, w, n, ctx->red);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(rowsum_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->red, 1, nbk, 1, 1, coef + 2 * m);
-        HIP_TRY(hipGetLastError());
+        launch_cgs2(ctx, Q, ld, m, w, n, nbk, coef);
         HIP_TRY(hipMemcpyAsync(ctx->cg_host, coef, sizeof(double) * (2 * (size_t)m + 1), hipMemcpyDeviceToHost,
                                ctx->stream));
         HIP_TRY(hipEventRecord(ctx->cg_event, ctx->stream));

@@ -90,3 +90,28 @@ def test_krylov_lookahead_product_changes_nothing(pkg, precision):
     np.testing.assert_array_equal(np.asarray(fast.residual_history), np.asarray(plain.residual_history))
     np.testing.assert_array_equal(np.asarray(fast.mass_history), np.asarray(plain.mass_history))
     assert fast.diagnostics["sweeps_discarded"] > 0 and plain.diagnostics["sweeps_discarded"] == 0
+
+
+def test_cgs2_matches_multidot_update_sequence(pkg):
+    """pbte_vec_cgs2 (update and next dot product fused in one pass over
+    the basis for up to 32 rows) gives the bits of two multidot / update
+    pairs and a norm on the same vectors."""
+    import torch
+    c = pkg.Case.from_yaml(CASES / "F1_1_1.yaml")
+    print("case cgs2 F1_1_1", flush=True)
+    ctx = pkg.Solver(c).ctx
+    rng = np.random.default_rng(5)
+    n = 100003
+    for m in (1, 7, 30, 33):
+        Q = torch.tensor(rng.standard_normal((m, n)), device="cuda")
+        w0 = torch.tensor(rng.standard_normal(n), device="cuda")
+        w1 = w0.clone()
+        h, hn = ctx.cgs2(Q, m, w1)
+        w2 = w0.clone()
+        c1 = ctx.multidot(Q, m, w2)
+        ctx.update(Q, m, c1, w2)
+        c2 = ctx.multidot(Q, m, w2)
+        ctx.update(Q, m, c2, w2)
+        assert np.array_equal(h, c1 + c2), m
+        assert hn == ctx.norm(w2), m
+        assert torch.equal(w1, w2), m
