@@ -1571,8 +1571,14 @@ __global__ void __launch_bounds__(256) update_dot_kernel(const double* __restric
     for (int r = 0; r < NR; ++r) acc[r] = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
+        // the element's column of Q, all loads issued before the sums
+        double q[MR];
+#pragma unroll
+        for (int k = 0; k < MR; ++k) q[k] = k < m ? Qm[(size_t)k * ld + i] : 0.0;
         double s = 0.0;
-        for (int k = 0; k < m; ++k) s += c[k] * Qm[(size_t)k * ld + i];
+#pragma unroll
+        for (int k = 0; k < MR; ++k)
+            if (k < m) s += c[k] * q[k];
         const double wn = w[i] - s;
         w[i] = wn;
         if constexpr (NORM) {
@@ -1580,7 +1586,7 @@ __global__ void __launch_bounds__(256) update_dot_kernel(const double* __restric
         } else {
 #pragma unroll
             for (int r = 0; r < MR; ++r)
-                if (r < m) acc[r] += Qm[(size_t)r * ld + i] * wn;
+                if (r < m) acc[r] += q[r] * wn;
         }
     }
     __shared__ double sh[kRows][256];
@@ -2296,18 +2302,28 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         // --- bundles: directions with identical inflow signs ------------------
         std::vector<uint64_t> hsh(D);
         std::vector<std::vector<uint8_t>> sig(D);
-        for (int d = 0; d < D; ++d) {
-            auto& sg = sig[d];
-            sg.resize((size_t)N * 3);
-            uint64_t h = 1469598103934665603ull;
-            for (int i = 0; i < N; ++i)
-                for (int il = 0; il < 3; ++il) {
-                    const double sp = su->cxv[d] * su->nx[i * 3 + il] + su->cyv[d] * su->ny[i * 3 + il];
-                    const uint8_t v = sp < 0.0 ? 1 : (sp > 0.0 ? 2 : 0);
-                    sg[(size_t)i * 3 + il] = v;
-                    h = (h ^ v) * 1099511628211ull;
+        {
+            // directions split over host threads, each writing its own rows
+            std::atomic_int next{0};
+            auto signs = [&]() {
+                for (int d = next++; d < D; d = next++) {
+                    auto& sg = sig[d];
+                    sg.resize((size_t)N * 3);
+                    uint64_t h = 1469598103934665603ull;
+                    for (int i = 0; i < N; ++i)
+                        for (int il = 0; il < 3; ++il) {
+                            const double sp = su->cxv[d] * su->nx[i * 3 + il] + su->cyv[d] * su->ny[i * 3 + il];
+                            const uint8_t v = sp < 0.0 ? 1 : (sp > 0.0 ? 2 : 0);
+                            sg[(size_t)i * 3 + il] = v;
+                            h = (h ^ v) * 1099511628211ull;
+                        }
+                    hsh[d] = h;
                 }
-            hsh[d] = h;
+            };
+            const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < hw; ++t) th.emplace_back(signs);
+            for (auto& x : th) x.join();
         }
         int bmax = su->max_bundle > 0 ? su->max_bundle : 1;
         bmax = std::min(bmax, 8);
@@ -3105,13 +3121,16 @@ static void launch_cgs2(pbte_ctx* ctx, const double* Q, int64_t ld, int m, doubl
     hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, c1);
     HIP_TRY(hipGetLastError());
     if (m <= kFuseRows) {
-        hipLaunchKernelGGL((update_dot_kernel<kFuseRows, false>), dim3(nbk), dim3(256), 0, ctx->stream, Q, ld, m, c1,
-                           w, n, ctx->red);
+        // accumulators for the rows present, in steps of eight
+        auto* dot = m <= 8 ? update_dot_kernel<8, false> : m <= 16 ? update_dot_kernel<16, false>
+                  : m <= 24 ? update_dot_kernel<24, false> : update_dot_kernel<kFuseRows, false>;
+        hipLaunchKernelGGL(dot, dim3(nbk), dim3(256), 0, ctx->stream, Q, ld, m, c1, w, n, ctx->red);
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(rowsum_kernel, dim3(m), dim3(256), 0, ctx->stream, ctx->red, m, nbk, 1, 1, c2);
         HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL((update_dot_kernel<kFuseRows, true>), dim3(nbk), dim3(256), 0, ctx->stream, Q, ld, m, c2,
-                           w, n, ctx->red);
+        auto* nrm = m <= 8 ? update_dot_kernel<8, true> : m <= 16 ? update_dot_kernel<16, true>
+                  : m <= 24 ? update_dot_kernel<24, true> : update_dot_kernel<kFuseRows, true>;
+        hipLaunchKernelGGL(nrm, dim3(nbk), dim3(256), 0, ctx->stream, Q, ld, m, c2, w, n, ctx->red);
         HIP_TRY(hipGetLastError());
     } else {
         hipLaunchKernelGGL(update_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, Q, ld, m, c1, w, n);

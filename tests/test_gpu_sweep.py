@@ -320,3 +320,25 @@ def test_hole_mesh_mixed_factor_mode_matches_oracle(pkg, tmp_path):
     s.ctx.sweep(s.mom, s.vdf32_d)
     o.sweep()
     assert _rel(s.vdf32_d.to(torch.float64).cpu().numpy(), o.vdf) <= 2e-5
+
+
+@pytest.mark.parametrize("deg", [1, 3])
+def test_pipelined_sweep_is_bitwise_plain_sweep_hole_mesh(pkg, tmp_path, deg):
+    """P = 3 and P = 10 (the inverse read in the solve, not prefetched) on the
+    square with a hole: the pipelined kernel gives the plain level kernel's
+    bits in both precisions."""
+    import torch
+    for prec in ("fp64", "fp32"):
+        c = _hole_case(pkg, tmp_path / f"hole{deg}.msh", n=16, deg=deg)
+        c.performance.precision = prec
+        outs = []
+        for variant in (3, 1):
+            print(f"case hole deg {deg} {prec} variant {variant}", flush=True)
+            s = pkg.Solver(c, sweep_variant=variant)
+            rng = np.random.default_rng(12)
+            s.vdf = rng.standard_normal(s.sweep_vdf.shape)
+            s.mom.set_dofs(*(rng.standard_normal((s.ndof, s.n_tris)) for _ in range(3)))
+            for _ in range(2):
+                s.ctx.sweep(s.mom, s.sweep_vdf)
+            outs.append(s.vdf)
+        assert np.array_equal(outs[0], outs[1])
