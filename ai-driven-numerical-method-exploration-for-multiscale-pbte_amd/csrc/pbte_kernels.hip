@@ -2409,10 +2409,17 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             ctx->lds_blk = (size_t)lds_blk;
             const size_t slot_bytes = (size_t)bm * pad4(3 * TQ) * 4;
             const size_t wall_bytes = (size_t)ctx->max_wall_in * bm * PV * 4;
-            if (budget < wall_bytes + 4 * slot_bytes) throw std::string("LDS too small for the wall traces");
-            int pool = std::min((int)((budget - wall_bytes) / slot_bytes) - 2, kPoolMax);
-            if (su->lds_slots_cap > 0) pool = std::min(pool, su->lds_slots_cap);
-            ctx->pool_slots = std::max(pool, 1);
+            if (budget < wall_bytes + 4 * slot_bytes) {
+                // the bundles' wall inflow does not fit beside a pool: the
+                // plain level kernel (wall rows and upwind values read from
+                // global memory) sweeps this context
+                ctx->variant = 1;
+                ctx->pool_slots = 1;
+            } else {
+                int pool = std::min((int)((budget - wall_bytes) / slot_bytes) - 2, kPoolMax);
+                if (su->lds_slots_cap > 0) pool = std::min(pool, su->lds_slots_cap);
+                ctx->pool_slots = std::max(pool, 1);
+            }
         }
         // levels per bundle (all directions of a bundle share them)
         std::vector<int32_t> h_slot((size_t)nb * N), h_lev_ptr;
@@ -3182,7 +3189,12 @@ extern "C" int pbte_vec_cgs2_begin(pbte_ctx* ctx, const double* Q, int64_t ld, i
     return guarded(ctx, [&]() {
         if (m <= 0) throw std::string("cgs2 needs at least one basis vector");
         if (m > kMaxBasis) throw std::string("too many basis vectors");
-        if (ctx->cg_m) throw std::string("cgs2_begin: the previous call was not ended");
+        if (ctx->cg_m) {
+            // a pair left open (the caller raised between begin and end):
+            // let its copy land, then drop it
+            HIP_TRY(hipEventSynchronize(ctx->cg_event));
+            ctx->cg_m = 0;
+        }
         const int nbk = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + 255) / 256));
         if ((size_t)m * nbk + 2 * (size_t)m + 1 > ctx->red_len) throw std::string("too many basis vectors");
         if (!ctx->cg_host) {

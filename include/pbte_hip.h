@@ -208,7 +208,8 @@ int pbte_vec_cgs2(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m, double 
  * (c1, c2, |w|^2) to pinned host memory behind an event and, when next is
  * not NULL, next = w / |w| on the device (axpby's rounding); _end waits for
  * the event and writes the 2m+1 values to out_host.  One pair in flight
- * per context. */
+ * per context: a _begin while one is open waits for that one's copy and
+ * drops it; _end without an open _begin fails. */
 int pbte_vec_cgs2_begin(pbte_ctx *ctx, const double *Q, int64_t ld, int32_t m, double *w,
                         int64_t n, double *next);
 int pbte_vec_cgs2_end(pbte_ctx *ctx, double *out_host);
