@@ -1231,9 +1231,10 @@ __global__ void __launch_bounds__(256) moments_kernel(const T* __restrict__ vdf,
 }
 
 // the same sums, V consecutive (cell, dof) values per thread read as one
-// 16-byte word per direction, eight directions' words in flight before the
-// in-order accumulation (each value's sum keeps the order and expressions
-// of moments_kernel).  Needs N*P % V == 0 and a 16-byte aligned vdf.
+// 16-byte word per direction, the words of 16 (binary32) or 8 (binary64)
+// directions in flight before the in-order accumulation (each value's sum
+// keeps the order and expressions of moments_kernel).  Needs N*P % V == 0
+// and a 16-byte aligned vdf.
 template <typename T, int V>
 __global__ void __launch_bounds__(256) moments_vec_kernel(const T* __restrict__ vdf,
                                                           const double* __restrict__ cxv,
@@ -1244,7 +1245,7 @@ __global__ void __launch_bounds__(256) moments_vec_kernel(const T* __restrict__ 
 {
     static_assert(V * sizeof(T) == 16, "one 16-byte word per direction");
     using W = typename Vec16<T>::type;
-    constexpr int U = 8;
+    constexpr int U = sizeof(T) == 4 ? 16 : 8;     // words in flight (measured best per type)
     const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * V;
     if (g0 >= (int64_t)N * P) return;
     const size_t stride = (size_t)N * P / V;
