@@ -57,6 +57,12 @@ _SIGS = {
     "pbte_ctx_create": (C.c_void_p, [C.POINTER(PbteDims), C.POINTER(PbteSetup), C.c_void_p,
                                      C.c_char_p, C.c_int32]),
     "pbte_ctx_destroy": (None, [C.c_void_p]),
+    "pbte_batch_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.c_int32, C.c_char_p, C.c_int32]),
+    "pbte_batch_destroy": (None, [C.c_void_p]),
+    "pbte_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "pbte_batch_sweep": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                   C.c_int32]),
     "pbte_last_error": (C.c_char_p, [C.c_void_p]),
     "pbte_schedule_info": (C.c_int, [C.c_void_p, _i64p]),
     "pbte_timing": (C.c_int, [C.c_void_p, C.c_int32]),

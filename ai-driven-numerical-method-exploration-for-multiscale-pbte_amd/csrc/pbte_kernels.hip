@@ -1007,7 +1007,8 @@ __device__ __forceinline__ void pipe_read(const SweepArgs<T>& a, const PipeData<
 // second half: source, wall inflow, upwind faces, the inverse, the stores
 template <int P, typename T, bool PFLU>
 __device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeData<P, T, PFLU>& dC,
-                                             const PipeIn<P, T>& in, T cx, T cy, int B, int b, T* lds, T* vdf_d)
+                                             const PipeIn<P, T>& in, T cx, T cy, int B, int b, T* lds, T* vdf_d,
+                                             const T* lub)
 {
     using L = Lay<P>;
     constexpr int DEG = Deg<P>::v;
@@ -1028,9 +1029,8 @@ __device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeDa
     if constexpr (PFLU) {
         apply_inverse<P, T>(x, [&](int k) { return dC.lu[k]; });
     } else {
-        const T* lu = a.lu + a.lu_off[a.block_bundle[blockIdx.x]];
         const size_t item = (size_t)dC.slot * B + b, NB = (size_t)a.N * B;
-        apply_inverse<P, T>(x, [&](int k) { return __builtin_nontemporal_load(lu + lu_idx<T>(k, item, NB)); });
+        apply_inverse<P, T>(x, [&](int k) { return __builtin_nontemporal_load(lub + lu_idx<T>(k, item, NB)); });
     }
     // stores without a branch: an idle thread, and an element nobody reads
     // through LDS, write their traces to the spill slot; an idle thread
@@ -1071,11 +1071,11 @@ __device__ __forceinline__ void level_barrier(bool global_mode)
 // BF: 1 when every bundle holds one direction (the default schedule; the
 // slot and LDS arithmetic then drops its bundle factors), 0 for bundles
 template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
-__global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
+__device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
 {
     constexpr int W = Tr<P>::W;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    const int bundle = a.block_bundle[blockIdx.x];
+    const int bundle = a.block_bundle[blk];
     const int B = BF ? 1 : a.bundle_ndir[bundle];
     // LDS: pool slots, one all-zero slot, one spill slot, the wall inflow
     T* lds = reinterpret_cast<T*>(lds_raw);
@@ -1135,7 +1135,7 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
         sN = span(lvN);
         pipe_gather<P, T, PFLU>(a, rG, B, b, lu, dG);
         if constexpr (sizeof(T) != 4) pipe_read<P, T, PFLU, GREAD>(a, dS, d, lds, vdf_d, in);
-        pipe_compute<P, T, PFLU>(a, dS, in, cx, cy, B, b, lds, vdf_d);
+        pipe_compute<P, T, PFLU>(a, dS, in, cx, cy, B, b, lds, vdf_d, a.lu + lu0);
         level_barrier(global_mode);
     };
     if constexpr (DEPTH == 2) {
@@ -1181,6 +1181,22 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
             level(D2, R0, S0, lv + 9, D1, R1);     // solve lv+2; record lv+6; gather lv+4
         }
     }
+}
+
+template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
+__global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
+{
+    sweep_pipe_body<P, T, PFLU, GREAD, DEPTH, BF>(a, blockIdx.x);
+}
+
+// several contexts' sweeps in one launch: workgroup k sweeps block
+// bmap[k].y of context bmap[k].x, whose arguments it reads from args
+template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
+__global__ void __launch_bounds__(256) sweep_pipe_batch_kernel(const SweepArgs<T>* __restrict__ args,
+                                                               const int2* __restrict__ bmap)
+{
+    const int2 m = bmap[blockIdx.x];
+    sweep_pipe_body<P, T, PFLU, GREAD, DEPTH, BF>(args[m.x], m.y);
 }
 
 // the wall vector of each adiabatic wall face from this iterate's emission
@@ -2796,8 +2812,9 @@ void launch_pipe(pbte_ctx* ctx, const SweepArgs<T>& a, size_t lds)
     hipLaunchKernelGGL(fn, dim3(ctx->nb), dim3(ctx->block), lds, ctx->stream, a);
 }
 
+// the kernel arguments of one context's sweep
 template <int P, typename T>
-void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous, bool onthefly)
+SweepArgs<T> sweep_args(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous)
 {
     using S = SweepData<T>;
     SweepArgs<T> a{};
@@ -2817,11 +2834,24 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
     for (int k = 0; k < ctx->Q * ctx->Q; ++k) a.kref[k] = S::kref(ctx)[k];
     a.mass = ctx->mass; a.gx = ctx->gx; a.gy = ctx->gy; a.fcm = ctx->fcm; a.geo = ctx->geo64;
     a.cxv64 = ctx->cxv; a.cyv64 = ctx->cyv; a.inv_tau_c = 1.0 / ctx->tau_c;
+    return a;
+}
+
+// whether a context's sweep of this precision runs the pipelined kernel
+template <int P, typename T> bool runs_pipelined(const pbte_ctx* ctx)
+{
+    return ctx->variant != 1 && sweep_lds_bytes<P, T>(ctx) <= ctx->lds_blk;
+}
+
+template <int P, typename T>
+void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous, bool onthefly)
+{
+    const SweepArgs<T> a = sweep_args<P, T>(ctx, pbuf, vdf, use_pbuf, homogeneous);
     const bool timed = 2 * (size_t)ctx->ev_used + 1 < ctx->events.size();
     if (timed) HIP_TRY(hipEventRecord(ctx->events[2 * ctx->ev_used], ctx->stream));
     if (onthefly) {
         hipLaunchKernelGGL((sweep_otf_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
-    } else if (ctx->variant == 1 || sweep_lds_bytes<P, T>(ctx) > ctx->lds_blk) {
+    } else if (!runs_pipelined<P, T>(ctx)) {
         // the plain level kernel: asked for, or a binary64 sweep whose pool,
         // sized for binary32, exceeds a workgroup's LDS
         hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
@@ -2850,8 +2880,7 @@ void launch_source(pbte_ctx* ctx, const double* ts, const double* qxs, const dou
 // one sweep: the source into the cell records, the adiabatic wall vectors
 // from flux_wall, then the sweep
 template <typename T>
-void run_sweep(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, const double* flux_wall,
-               const T* pbuf, T* vdf, int use_pbuf, int homogeneous, bool onthefly = false)
+void prep_sweep(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, const double* flux_wall)
 {
     using S = SweepData<T>;
     const int P = ctx->P;
@@ -2865,6 +2894,14 @@ void run_sweep(pbte_ctx* ctx, const double* ts, const double* qxs, const double*
 #undef WT
         HIP_TRY(hipGetLastError());
     }
+}
+
+template <typename T>
+void run_sweep(pbte_ctx* ctx, const double* ts, const double* qxs, const double* qys, const double* flux_wall,
+               const T* pbuf, T* vdf, int use_pbuf, int homogeneous, bool onthefly = false)
+{
+    const int P = ctx->P;
+    prep_sweep<T>(ctx, ts, qxs, qys, flux_wall);
     switch (P) {
 #define SW(PP) case PP: launch_sweep<PP, T>(ctx, pbuf, vdf, use_pbuf, homogeneous, onthefly); break;
     SW(3) SW(6) SW(10) SW(15)
@@ -2955,6 +2992,158 @@ extern "C" int pbte_sweep_onthefly(pbte_ctx* ctx, int32_t precision, const doubl
             throw std::string("bad precision");
         return 0;
     });
+}
+
+// ==========================================================================
+// batched sweeps: the bundles of several contexts (independent problems of
+// one element degree, built on one stream) in one launch of the pipelined
+// kernel; each workgroup finds its context and bundle in a block map
+// ==========================================================================
+struct pbte_batch {
+    std::vector<pbte_ctx*> ctxs;
+    int2* bmap = nullptr;           // device [blocks] {context, block}
+    void* args = nullptr;           // device SweepArgs<T>[n]
+    std::vector<unsigned char> host_args;
+    int nblk = 0, block = 0, P = 0;
+    std::string err;
+    ~pbte_batch()
+    {
+        if (bmap) (void)hipFree(bmap);
+        if (args) (void)hipFree(args);
+    }
+};
+
+extern "C" pbte_batch* pbte_batch_create(pbte_ctx* const* ctxs, int32_t n, char* err, int32_t errlen)
+{
+    auto fail = [&](const std::string& m) -> pbte_batch* {
+        if (err && errlen > 0) {
+            std::strncpy(err, m.c_str(), (size_t)errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        return nullptr;
+    };
+    if (!ctxs || n <= 0) return fail("empty batch");
+    auto* bt = new pbte_batch();
+    try {
+        std::vector<int2> h;
+        for (int k = 0; k < n; ++k) {
+            pbte_ctx* c = ctxs[k];
+            if (!c) throw std::string("null context in batch");
+            if (c->P != ctxs[0]->P) throw std::string("a batch needs one element degree");
+            if (c->stream != ctxs[0]->stream) throw std::string("a batch needs one stream");
+            if (c->depth != ctxs[0]->depth) throw std::string("a batch needs one pipeline depth");
+            bt->ctxs.push_back(c);
+            bt->block = std::max(bt->block, c->block);
+            for (int b = 0; b < c->nb; ++b) h.push_back(make_int2(k, b));
+        }
+        // contexts with narrower workgroups run in the widest one: their
+        // extra threads are idle items of every level
+        bt->P = ctxs[0]->P;
+        bt->nblk = (int)h.size();
+        HIP_TRY(hipMalloc((void**)&bt->bmap, h.size() * sizeof(int2)));
+        HIP_TRY(hipMemcpy(bt->bmap, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice));
+        const size_t abytes = (size_t)n * std::max(sizeof(SweepArgs<double>), sizeof(SweepArgs<float>));
+        HIP_TRY(hipMalloc(&bt->args, abytes));
+        return bt;
+    } catch (const std::string& m) {
+        delete bt;
+        return fail(m);
+    }
+}
+
+extern "C" void pbte_batch_destroy(pbte_batch* bt) { delete bt; }
+
+extern "C" const char* pbte_batch_last_error(pbte_batch* bt) { return bt ? bt->err.c_str() : "null batch"; }
+
+template <int P, typename T, bool PF, bool GR, int DEPTH>
+void launch_pipe_batch(pbte_batch* bt, bool one_dir, size_t lds, hipStream_t st)
+{
+    auto* fn = one_dir ? sweep_pipe_batch_kernel<P, T, PF, GR, DEPTH, 1> : sweep_pipe_batch_kernel<P, T, PF, GR, DEPTH, 0>;
+    if (lds > 65536)
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    hipLaunchKernelGGL(fn, dim3(bt->nblk), dim3(bt->block), lds, st, static_cast<const SweepArgs<T>*>(bt->args),
+                       bt->bmap);
+}
+
+template <int P, typename T>
+void batch_sweep(pbte_batch* bt, T* const* vdf, int homogeneous)
+{
+    const int n = (int)bt->ctxs.size();
+    std::vector<unsigned char> host((size_t)n * sizeof(SweepArgs<T>));
+    size_t lds = 0;
+    bool gread = false, one_dir = true;
+    for (int k = 0; k < n; ++k) {
+        pbte_ctx* c = bt->ctxs[k];
+        if (!runs_pipelined<P, T>(c))
+            throw std::string("batch: a context does not run the pipelined kernel in this precision");
+        const SweepArgs<T> a = sweep_args<P, T>(c, nullptr, vdf[k], 0, homogeneous);
+        std::memcpy(host.data() + (size_t)k * sizeof(SweepArgs<T>), &a, sizeof(a));
+        lds = std::max(lds, sweep_lds_bytes<P, T>(c));
+        gread = gread || c->global_bundles > 0 || c->old_reads > 0;
+        one_dir = one_dir && c->max_bundle_dirs == 1;
+    }
+    // the arguments change only with the distributions and the source flag:
+    // upload (and wait for the previous launch to have read them) only then
+    if (host != bt->host_args) {
+        HIP_TRY(hipStreamSynchronize(bt->ctxs[0]->stream));
+        HIP_TRY(hipMemcpy(bt->args, host.data(), host.size(), hipMemcpyHostToDevice));
+        bt->host_args.swap(host);
+    }
+    constexpr bool pf = P * P * sizeof(T) <= 288;
+    const hipStream_t st = bt->ctxs[0]->stream;
+    if (bt->ctxs[0]->depth == 3) {
+        if (gread) launch_pipe_batch<P, T, pf, true, 3>(bt, one_dir, lds, st);
+        else launch_pipe_batch<P, T, pf, false, 3>(bt, one_dir, lds, st);
+    } else {
+        if (gread) launch_pipe_batch<P, T, pf, true, 2>(bt, one_dir, lds, st);
+        else launch_pipe_batch<P, T, pf, false, 2>(bt, one_dir, lds, st);
+    }
+    HIP_TRY(hipGetLastError());
+}
+
+extern "C" int pbte_batch_sweep(pbte_batch* bt, int32_t precision, const double* const* ts, const double* const* qxs,
+                                const double* const* qys, void* const* vdf, const double* const* flux_wall,
+                                int32_t homogeneous)
+{
+    if (!bt) return -1;
+    try {
+        const int n = (int)bt->ctxs.size();
+        for (int k = 0; k < n; ++k) {
+            pbte_ctx* c = bt->ctxs[k];
+            if (c->n_wall && !flux_wall[k]) throw std::string("adiabatic walls need flux_wall");
+            if (precision == PBTE_FP64) {
+                if (!c->have64) throw std::string("FP64 factors not built in a batch context");
+                prep_sweep<double>(c, ts[k], qxs[k], qys[k], flux_wall[k]);
+            } else if (precision == PBTE_FP32) {
+                if (!c->have32) throw std::string("FP32 factors not built in a batch context");
+                prep_sweep<float>(c, ts[k], qxs[k], qys[k], flux_wall[k]);
+            } else {
+                throw std::string("bad precision");
+            }
+        }
+        if (precision == PBTE_FP64) {
+            auto v = reinterpret_cast<double* const*>(vdf);
+            switch (bt->P) {
+            case 3: batch_sweep<3, double>(bt, v, homogeneous); break;
+            case 6: batch_sweep<6, double>(bt, v, homogeneous); break;
+            case 10: batch_sweep<10, double>(bt, v, homogeneous); break;
+            case 15: batch_sweep<15, double>(bt, v, homogeneous); break;
+            }
+        } else {
+            auto v = reinterpret_cast<float* const*>(vdf);
+            switch (bt->P) {
+            case 3: batch_sweep<3, float>(bt, v, homogeneous); break;
+            case 6: batch_sweep<6, float>(bt, v, homogeneous); break;
+            case 10: batch_sweep<10, float>(bt, v, homogeneous); break;
+            case 15: batch_sweep<15, float>(bt, v, homogeneous); break;
+            }
+        }
+        return 0;
+    } catch (const std::string& m) {
+        bt->err = m;
+        return -1;
+    }
 }
 
 extern "C" int pbte_moments(pbte_ctx* ctx, int32_t precision, const void* vdf, double* ts, double* qxs,

@@ -184,6 +184,25 @@ int pbte_residual_iterate_async(pbte_ctx *ctx, const double *temp, const double 
 int pbte_periodic_gather(pbte_ctx *ctx, int32_t precision, const void *vdf, double *out);
 int pbte_periodic_scatter(pbte_ctx *ctx, int32_t precision, const double *in, void *pbuf);
 
+/* ---- batched sweeps (no counterpart in the reference) ------------------
+ * The sweeps of several independent problems in one launch of the
+ * pipelined sweep kernel: every workgroup finds its context and bundle in a
+ * block map.  The contexts must share the element degree, the stream and the
+ * pipeline depth and run the pipelined kernel in the precision swept; no
+ * explicit periodic buffer.  pbte_batch_sweep takes per context the moment
+ * sources, the distribution (of the precision) and flux_wall (NULL without
+ * adiabatic walls), runs each context's source and wall-vector kernels and
+ * then the one sweep launch; each context's sweep is the one pbte_sweep
+ * computes, bit for bit.  For many small problems, whose workgroups alone
+ * leave most of the CUs idle. */
+typedef struct pbte_batch pbte_batch;
+pbte_batch *pbte_batch_create(pbte_ctx *const *ctxs, int32_t n, char *err, int32_t errlen);
+void pbte_batch_destroy(pbte_batch *batch);
+const char *pbte_batch_last_error(pbte_batch *batch);
+int pbte_batch_sweep(pbte_batch *batch, int32_t precision, const double *const *ts,
+                     const double *const *qxs, const double *const *qys, void *const *vdf,
+                     const double *const *flux_wall, int32_t homogeneous);
+
 /* ---- device vector kernels of the Krylov solver (krylov.py:150-246) --- */
 
 /* c[k] = sum_i Q[k][i] w[i], k < m (Q device [m][n], row stride ld);

@@ -1,0 +1,88 @@
+"""Batched sweeps and lockstep source iteration of several problems: the
+same bits as each problem alone (the batch only merges the sweep launches)."""
+import numpy as np
+import pytest
+
+from conftest import CASES
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(pkg, tag, **kw):
+    c = pkg.Case.from_yaml(CASES / f"{tag}.yaml")
+    for k, v in kw.items():
+        obj, attr = k.split("__")
+        setattr(getattr(c, obj), attr, v)
+    return c
+
+
+def _cases(pkg, tmp_path, prec):
+    from pbte_amd.mesh import write_square_mesh
+    write_square_mesh(tmp_path / "j.msh", 12, jitter=0.25, seed=3)
+    jit = _case(pkg, "F1_1_1", velmesh__npole=4, velmesh__nazim=12, performance__precision=prec)
+    jit.mesh.file = str(tmp_path / "j.msh")
+    return [_case(pkg, "F1_1_1", performance__precision=prec),
+            _case(pkg, "F2_1_1", velmesh__npole=4, velmesh__nazim=16, performance__precision=prec),
+            _case(pkg, "F3_1_1", performance__precision=prec),
+            jit]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batched_sweep_is_bitwise_individual_sweeps(pkg, tmp_path, prec):
+    """Four problems (thermalising, adiabatic and periodic walls, a jittered
+    mesh; different ordinate sets) swept in one launch and one at a time."""
+    import torch
+
+    from pbte_amd.batch import SweepBatch
+    cases = _cases(pkg, tmp_path, prec)
+    print(f"case batch sweep {prec}", flush=True)
+    solvers = [pkg.Solver(c) for c in cases]
+    rng = np.random.default_rng(31)
+    start = []
+    for s in solvers:
+        v = rng.standard_normal(s.sweep_vdf.shape)
+        m = [rng.standard_normal((s.ndof, s.n_tris)) for _ in range(3)]
+        fw = rng.standard_normal(s.ctx.flux_wall.shape)
+        start.append((v, m, fw))
+
+    def load(k):
+        s = solvers[k]
+        v, m, fw = start[k]
+        s.vdf = v
+        s.mom.set_dofs(*m)
+        s.ctx.flux_wall.copy_(torch.from_numpy(fw))
+
+    alone = []
+    for k, s in enumerate(solvers):
+        load(k)
+        for _ in range(2):
+            s.ctx.sweep(s.mom, s.sweep_vdf)
+        alone.append(s.vdf)
+    for k in range(len(solvers)):
+        load(k)
+    b = SweepBatch([s.ctx for s in solvers])
+    for _ in range(2):
+        b.sweep([s.mom for s in solvers], [s.sweep_vdf for s in solvers])
+    b.close()
+    for k, s in enumerate(solvers):
+        assert np.array_equal(s.vdf, alone[k]), k
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_run_source_batch_is_each_solver_alone(pkg, tmp_path, prec):
+    """Lockstep CIS of four problems gives each problem's own run: the same
+    iteration count, residual and mass histories and moments, bit for bit
+    (fp32: with the switch to binary64 sweeps at its own step)."""
+    from pbte_amd.batch import run_source_batch
+    cases = _cases(pkg, tmp_path, prec)
+    for c in cases:
+        c.iteration.tmax = 60
+    print(f"case batch cis {prec}", flush=True)
+    alone = [pkg.Solver(c).run() for c in cases]
+    recs = run_source_batch([pkg.Solver(c) for c in cases])
+    for a, r in zip(alone, recs):
+        assert r.iterations == a.iterations and r.converged == a.converged
+        np.testing.assert_array_equal(r.residual_history, a.residual_history)
+        np.testing.assert_array_equal(r.mass_history, a.mass_history)
+        np.testing.assert_array_equal(r.temp_dofs, a.temp_dofs)
+        assert r.sweep_count == a.sweep_count
