@@ -92,9 +92,9 @@ def run_source_batch(solvers, callback=None):
     in lockstep; returns their ``RunRecord`` objects in order.
 
     Per step: the wall emission of each unfinished problem, one batched
-    sweep per precision group, each problem's moments (or GSIS correction),
-    the residual and mass of every problem summed on the device, one wait,
-    then the stop tests of ``Solver._run_source`` (the fp32 switch
+    sweep per group of one precision and one element degree, each
+    problem's moments (or GSIS correction), the residual and mass of every
+    problem summed on the device, one wait, then the stop tests of ``Solver._run_source`` (the fp32 switch
     included).  ``callback(k, step, res, solver)`` after each problem's
     step.  Options that need the transport residual per step
     (``iteration.true_residual``) are not supported here."""
@@ -122,11 +122,15 @@ def run_source_batch(solvers, callback=None):
                 s = tracks[k].s
                 if s.bcdata.has_nonthermalising:
                     s.ctx.wall_flux(s.sweep_vdf)
-            for prec in ("fp32", "fp64"):
-                group = [k for k in active if tracks[k].s.sweep_precision == prec]
-                if group:
-                    batch_of(group).sweep([tracks[k].s.mom for k in group],
-                                          [tracks[k].s.sweep_vdf for k in group])
+            # one launch per (precision, element degree): a launch runs one
+            # kernel instance
+            groups = {}
+            for k in active:
+                s = tracks[k].s
+                groups.setdefault((s.sweep_precision, s.ndof), []).append(k)
+            for group in groups.values():
+                batch_of(group).sweep([tracks[k].s.mom for k in group],
+                                      [tracks[k].s.sweep_vdf for k in group])
             for k in active:
                 t = tracks[k]
                 s = t.s

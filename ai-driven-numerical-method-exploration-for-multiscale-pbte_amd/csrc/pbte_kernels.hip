@@ -3026,6 +3026,7 @@ extern "C" pbte_batch* pbte_batch_create(pbte_ctx* const* ctxs, int32_t n, char*
     auto* bt = new pbte_batch();
     try {
         std::vector<int2> h;
+        std::vector<int> by_len(n);
         for (int k = 0; k < n; ++k) {
             pbte_ctx* c = ctxs[k];
             if (!c) throw std::string("null context in batch");
@@ -3034,8 +3035,16 @@ extern "C" pbte_batch* pbte_batch_create(pbte_ctx* const* ctxs, int32_t n, char*
             if (c->depth != ctxs[0]->depth) throw std::string("a batch needs one pipeline depth");
             bt->ctxs.push_back(c);
             bt->block = std::max(bt->block, c->block);
-            for (int b = 0; b < c->nb; ++b) h.push_back(make_int2(k, b));
+            by_len[k] = k;
         }
+        // longest bundles first (levels per bundle, the workgroup's run
+        // time): workgroups start in block order, so the short ones fill the
+        // CUs the long ones leave; a context's blocks stay together, in its
+        // own XCD-dealt order
+        auto len = [&](int k) { return (double)ctxs[k]->total_levels / std::max(ctxs[k]->nb, 1); };
+        std::stable_sort(by_len.begin(), by_len.end(), [&](int a, int b) { return len(a) > len(b); });
+        for (int k : by_len)
+            for (int b = 0; b < ctxs[k]->nb; ++b) h.push_back(make_int2(k, b));
         // contexts with narrower workgroups run in the widest one: their
         // extra threads are idle items of every level
         bt->P = ctxs[0]->P;

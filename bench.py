@@ -39,6 +39,13 @@ give the same one-dimensional solution); one FP64 and one FP32-sweep
 solves/hour of the FP32 solve; roofline is the sweep kernel inside those
 solves.  The default mesh is 128 x 128: the 256 x 256 trace system (1.77 M
 unknowns) does not fit the host SuperLU factorisation.
+
+--workload batch runs the batched configuration: 64 independent problems
+drawn from default_rng(1) (2k-65k jittered P1/P2 triangles, 64-512
+ordinates, Kn_R log-uniform on [1e-3, 10]); a step is one FP32 sweep of
+every problem, one launch per element degree; value is the aggregate
+DOF-updates/s, with the problem-by-problem launches timed beside it, and a
+GSIS solve leg of the same draws on coarser meshes (batch_workload).
 """
 from __future__ import annotations
 
@@ -76,7 +83,8 @@ def committed_traffic(workload):
     return None if row is None else row.get("hbm_bytes_per_launch")
 
 
-def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitter=0.0, seed=0, hole=0.0):
+def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitter=0.0, seed=0, hole=0.0,
+              method="krylov", accflag=0, tmax=20000):
     """Diffuse isothermal walls, the north one at dT = +1; with ``hole`` > 0
     a central circular hole of that radius whose wall is at dT = +1 and
     the four outer walls at 0."""
@@ -93,8 +101,8 @@ def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitte
     if hole > 0:
         walls.append({"name": "Hole", "phyid": 16, "type": "thermalising", "temp": 1.0})
     return pkg.Case.from_dict({
-        "iteration": {"tol": 1e-8, "tmax": 20000},
-        "scheme": {"accflag": 0, "method": "krylov", "krylov_restart": 30},
+        "iteration": {"tol": 1e-8, "tmax": tmax},
+        "scheme": {"accflag": accflag, "method": method, "krylov_restart": 30},
         "velmesh": {"npole": npole, "nazim": nazim},
         "dg": {"deg": deg},
         "flow": {"cv": 1.0, "vg": 1.0, "tau_r": tau_r, "tau_n": tau_n},
@@ -247,6 +255,172 @@ def cis_workload(pkg, args, work):
     print(json.dumps(out), flush=True)
 
 
+BATCH_CELLS = (2048, 8192, 32768, 65536)
+BATCH_DIRS = {64: (4, 16), 128: (8, 16), 256: (8, 32), 512: (16, 32)}
+
+
+def batch_draws(count=64, seed=1):
+    """The batched workload's problems, drawn from default_rng(seed) in this
+    order: cell counts from BATCH_CELLS, element degrees from {1, 2},
+    ordinate counts from {64, 128, 256, 512} (polar x azimuth as in
+    BATCH_DIRS), log10 Kn_R uniform on [-3, 1], Kn_N / Kn_R uniform on
+    [0.01, 1].  A square of n x n cells split in two triangles each, with
+    n = round(sqrt(cells / 2)) (65 536 cells: 181 x 181 = 65 522)."""
+    rng = np.random.default_rng(seed)
+    cells = rng.choice(BATCH_CELLS, count)
+    degs = rng.choice([1, 2], count)
+    dirs = rng.choice(sorted(BATCH_DIRS), count)
+    kn_r = 10.0 ** rng.uniform(-3.0, 1.0, count)
+    kn_n = kn_r * rng.uniform(0.01, 1.0, count)
+    return [{"n": int(round(np.sqrt(c / 2))), "deg": int(g), "ndir": int(d), "tau_r": float(a),
+             "tau_n": float(b)} for c, g, d, a, b in zip(cells, degs, dirs, kn_r, kn_n)]
+
+
+def batch_workload(pkg, args, work):
+    """The batched configuration: 64 independent problems (batch_draws),
+    jittered meshes (interior vertices moved by U(-0.25h, 0.25h), seed 1),
+    diffuse isothermal walls with the north one at dT = +1.
+
+    Sweep leg, at the drawn sizes: one step is one FP32 sweep of every
+    problem; the problems of one element degree go in one launch of the
+    pipelined kernel (pbte_batch_sweep), so a step is two launches (plus
+    each problem's source kernel).  value = total DOF-updates per step /
+    wall time per step; the same sweeps launched problem by problem are
+    timed beside it.
+
+    Solve leg (--batch-solve-div k, default 4; 0 skips it): the same 64
+    draws on meshes k times coarser in each direction (n // k cells a
+    side, at least 4), GSIS with FP32 sweeps and the binary64 finish to the
+    iterate step 1e-8, all 64 in lockstep (batch.run_source_batch); the
+    aggregate solves/hour and each problem's iterations and time to
+    tolerance.  The full-size GSIS solves are not run: each GSIS iteration
+    solves the HDG trace system with SuperLU on the host."""
+    import torch
+    from pbte_amd.batch import SweepBatch
+    draws = batch_draws(args.batch_count, 1)
+    log(f"batch: building {len(draws)} problems (sweep leg)")
+    t0 = time.perf_counter()
+    solvers = []
+    rng = np.random.default_rng(0)
+    for k, d in enumerate(draws):
+        npole, nazim = BATCH_DIRS[d["ndir"]]
+        case = make_case(pkg, d["n"], d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp32", work,
+                         jitter=0.25, seed=1, method="source")
+        s = pkg.Solver(case)
+        P, N = s.ndof, s.n_tris
+        s.mom.set_dofs(rng.random((P, N)), 0.1 * rng.random((P, N)), 0.1 * rng.random((P, N)))
+        solvers.append(s)
+        if k % 8 == 7:
+            log(f"batch: {k + 1} problems set up ({time.perf_counter() - t0:.1f} s)")
+    setup_s = time.perf_counter() - t0
+    groups = {}
+    for k, s in enumerate(solvers):
+        groups.setdefault(s.ndof, []).append(k)
+    batches = [(SweepBatch([solvers[k].ctx for k in g]), g) for g in groups.values()]
+
+    def step_batched():
+        for b, g in batches:
+            b.sweep([solvers[k].mom for k in g], [solvers[k].sweep_vdf for k in g])
+
+    def step_alone():
+        for s in solvers:
+            s.ctx.sweep(s.mom, s.sweep_vdf)
+
+    def timed(step):
+        for _ in range(args.warmup):
+            step()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ev0.record()
+        for _ in range(args.steps):
+            step()
+        ev1.record()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, ev0.elapsed_time(ev1) * 1e-3
+
+    log("batch: batched sweeps")
+    wall_b, dev_b = timed(step_batched)
+    log("batch: the same sweeps problem by problem")
+    wall_a, dev_a = timed(step_alone)
+    for b, _ in batches:
+        b.close()
+    dof = sum(s.ndir * s.n_tris * s.ndof for s in solvers)
+    bytes_alg = sum(b_alg(s.ndir, s.n_tris, s.ndof, 4, 4) for s in solvers)
+    t_step = dev_b / args.steps
+    cpu_case = None
+    if not args.no_cpu:
+        # the median problem by DOF count
+        small = sorted(range(len(draws)), key=lambda k: solvers[k].ndir * solvers[k].n_tris * solvers[k].ndof)
+        small = small[len(small) // 2]
+        d = draws[small]
+        npole, nazim = BATCH_DIRS[d["ndir"]]
+        cpu_case = (small, make_case(pkg, d["n"], d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp64",
+                                     work, jitter=0.25, seed=1, method="source"))
+    shapes = [{"n_tris": s.n_tris, "ndof": s.ndof, "ndir": s.ndir, "tau_r": d["tau_r"], "tau_n": d["tau_n"]}
+              for s, d in zip(solvers, draws)]
+    del solvers, batches
+    torch.cuda.empty_cache()
+    out = {
+        "metric": METRIC, "value": dof * args.steps / wall_b, "unit": "DOF-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall_b / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic moments on the drawn problems' meshes and ordinates",
+        "config": {"workload": f"batch of {len(draws)} jittered squares (seed 1), "
+                               "2k-65k P1/P2 triangles, 64-512 ordinates, fp32 sweep",
+                   "problems": len(draws), "dof_per_step": dof, "launches_per_step": len(groups)},
+        "roofline": {"bound": "hbm", "achieved": bytes_alg / t_step / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": bytes_alg / t_step / HBM_PEAK, "traffic": None,
+                     "bytes_alg_per_launch": bytes_alg / len(groups),
+                     "note": "one step's launches together (sweeps and source kernels), HIP events"},
+        "batch": {"setup_s": setup_s, "step_ms_batched": 1e3 * t_step,
+                  "step_ms_one_by_one": 1e3 * dev_a / args.steps,
+                  "speedup_batched_vs_one_by_one": dev_a / dev_b,
+                  "wall_ms_batched": 1e3 * wall_b / args.steps,
+                  "wall_ms_one_by_one": 1e3 * wall_a / args.steps, "problems": shapes},
+    }
+    if args.batch_solve_div > 0:
+        out["batch"]["solve"] = batch_solves(pkg, args, work, draws)
+    if cpu_case is not None:
+        k, case = cpu_case
+        out["cpu_baseline"] = cpu_baseline(pkg, case, args.cpu_seconds, max_sweeps=1000)
+        out["cpu_baseline"]["sample"] += f" (problem {k}, the median of the batch by DOF count)"
+    print(json.dumps(out), flush=True)
+
+
+def batch_solves(pkg, args, work, draws):
+    """GSIS FP32 + binary64 finish of every drawn problem, in lockstep, on
+    meshes args.batch_solve_div times coarser a side."""
+    import torch
+    from pbte_amd.batch import run_source_batch
+    div = args.batch_solve_div
+    log(f"batch: building {len(draws)} GSIS problems (meshes / {div} a side)")
+    t0 = time.perf_counter()
+    solvers = []
+    for d in draws:
+        npole, nazim = BATCH_DIRS[d["ndir"]]
+        case = make_case(pkg, max(4, d["n"] // div), d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp32",
+                         work, jitter=0.25, seed=1, method="source", accflag=1, tmax=args.batch_tmax)
+        solvers.append(pkg.Solver(case))
+    setup = time.perf_counter() - t0
+    log(f"batch: {len(solvers)} GSIS problems set up in {setup:.1f} s; solving")
+    def progress(k, step, res, s):
+        if step % 200 == 0 and k == 0:
+            log(f"batch: step {step}")
+
+    t1 = time.perf_counter()
+    recs = run_source_batch(solvers, callback=progress)
+    torch.cuda.synchronize()
+    run = time.perf_counter() - t1
+    per = [{"n_tris": s.n_tris, "ndof": s.ndof, "ndir": s.ndir, "iterations": r.iterations,
+            "converged": r.converged, "time_to_tolerance_s": r.wall_clock,
+            "steps_fp32": r.diagnostics.get("steps_fp32")} for s, r in zip(solvers, recs)]
+    return {"mesh_div": div, "setup_s": setup, "run_s": run,
+            "solves_per_hour": 3600.0 * len(recs) / (setup + run),
+            "converged": int(sum(r.converged for r in recs)), "problems": per}
+
+
 def b_alg(D, N, P, s_lu, s_f):
     """Algorithmic HBM bytes of one precomputed-LU sweep: per (cell, dir) the
     LU (P^2 s_lu), pivots (P bytes), the distribution write and one upwind
@@ -255,7 +429,7 @@ def b_alg(D, N, P, s_lu, s_f):
     return D * N * (P * P * s_lu + P + 2 * P * s_f) + N * 8 * (4 * P * P + 3 * P + 6)
 
 
-def cpu_baseline(pkg, case, seconds=15.0):
+def cpu_baseline(pkg, case, seconds=15.0, max_sweeps=20):
     """The C restatement of the reference sweep (OpenMP over directions,
     cells in the reference order, precomputed LU) on this host."""
     from oracle import cpu
@@ -272,7 +446,7 @@ def cpu_baseline(pkg, case, seconds=15.0):
         o.sweep()
         n += 1
         el = time.perf_counter() - t0
-        if el > seconds or n >= 20:
+        if el > seconds or n >= max_sweeps:
             break
     dof = o.D * o.N * o.P
     return {"value": dof * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
@@ -286,7 +460,11 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "cis", "gsis"])
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "cis", "gsis", "batch"])
+    ap.add_argument("--batch-count", type=int, default=64, help="problems of the batch workload")
+    ap.add_argument("--batch-solve-div", type=int, default=4,
+                    help="batch workload: solve leg on meshes this many times coarser a side (0 skips)")
+    ap.add_argument("--batch-tmax", type=int, default=5000, help="batch workload: GSIS iteration cap")
     ap.add_argument("--deg", type=int, default=2)
     ap.add_argument("--npole", type=int, default=8)
     ap.add_argument("--nazim", type=int, default=32)
@@ -322,6 +500,9 @@ def main():
     work = Path(tempfile.mkdtemp(prefix="pbte_bench_"))
     if args.workload == "cis":
         cis_workload(pkg, args, work)
+        return
+    if args.workload == "batch":
+        batch_workload(pkg, args, work)
         return
     if args.workload == "gsis":
         if args.npole == 8 and args.nazim == 32:

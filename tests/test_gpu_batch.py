@@ -86,3 +86,26 @@ def test_run_source_batch_is_each_solver_alone(pkg, tmp_path, prec):
         np.testing.assert_array_equal(r.mass_history, a.mass_history)
         np.testing.assert_array_equal(r.temp_dofs, a.temp_dofs)
         assert r.sweep_count == a.sweep_count
+
+
+def test_run_source_batch_mixed_degrees_and_gsis(pkg, tmp_path):
+    """Lockstep source iteration over problems of degrees 1 and 2 (one sweep
+    launch per degree), two of them GSIS (``accflag: 1``), fp32 with the
+    binary64 finish: each problem's run alone, bit for bit."""
+    from pbte_amd.batch import run_source_batch
+    cases = _cases(pkg, tmp_path, "fp32")
+    cases[1].dg.deg = 1
+    cases[3].dg.deg = 1
+    for k in (0, 1):
+        cases[k].scheme.accflag = 1
+    for c in cases:
+        c.iteration.tmax = 40
+    print("case batch mixed degrees and gsis", flush=True)
+    alone = [pkg.Solver(c).run() for c in cases]
+    recs = run_source_batch([pkg.Solver(c) for c in cases])
+    for a, r in zip(alone, recs):
+        assert r.scheme == a.scheme and r.deg == a.deg
+        assert r.iterations == a.iterations and r.converged == a.converged
+        np.testing.assert_array_equal(r.residual_history, a.residual_history)
+        np.testing.assert_array_equal(r.mass_history, a.mass_history)
+        np.testing.assert_array_equal(r.temp_dofs, a.temp_dofs)
