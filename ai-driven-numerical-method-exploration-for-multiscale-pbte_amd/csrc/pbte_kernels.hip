@@ -482,6 +482,8 @@ template <typename T> struct SweepArgs {
     int lane_zero;              // 0 at run time; makes a uniform load per-lane
     int pool;                   // LDS pool slots
     int wbase;                  // LDS offset (in values) of the wall inflow vectors
+    int nthr;                   // threads that sweep (the context's workgroup size; a batch
+                                // launches the widest of its contexts)
     T kref[25];                 // Kr, Q x Q, row major
     // the on-the-fly sweep: element tensors and binary64 directions
     const double *mass, *gx, *gy, *fcm, *cxv64, *cyv64;
@@ -1092,15 +1094,22 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
     const __amdgpu_buffer_rsrc_t lu = bundle_rsrc(a.lu + lu0, (uint32_t)((a.lu_off[bundle + 1] - lu0) * sizeof(T)));
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
-    {
+    // waves past the context's own workgroup (a batch launches the widest
+    // one) sweep nothing: they take the prologue barrier and then as many
+    // level barriers as the sweeping waves.  nthr is a multiple of 64, and
+    // the test reads lane 0's index, so the branch is uniform per wave and
+    // scalar (no barrier of the other path runs under an empty exec mask).
+    const int nthr = a.nthr;
+    const bool sweeps = __builtin_amdgcn_readfirstlane(t) < nthr;
+    if (sweeps) {
         T* z = lds + (size_t)a.pool * B * W;
-        for (int k = t; k < B * W; k += blockDim.x) z[k] = T(0);
+        for (int k = t; k < B * W; k += nthr) z[k] = T(0);
         // the summed wall inflow of every inflow wall element of the bundle,
         // per direction (a thermal wall emits nothing in a homogeneous sweep)
         constexpr int PV = pad4(P);
         const int w0 = a.wc_off[bundle], nw = a.wc_off[bundle + 1] - w0;
         const T thermal_on = a.homogeneous ? T(0) : T(1);
-        for (int k = t; k < nw * B * PV; k += blockDim.x) {
+        for (int k = t; k < nw * B * PV; k += nthr) {
             const int e = k / (B * PV), bb = (k / PV) % B, m = k % PV;
             const int4 ent = a.wcell[w0 + e];
             const int db = a.bundle_dirs[a.bundle_dir_off[bundle] + bb];
@@ -1109,6 +1118,13 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         }
     }
     __syncthreads();
+    if (!sweeps) {
+        // the loops below run whole trips of DEPTH levels, one barrier each
+        const int nlev = l1 > l0 ? l1 - l0 : 0;
+        const int nbar = (nlev + DEPTH - 1) / DEPTH * DEPTH;
+        for (int k = 0; k < nbar; ++k) level_barrier(global_mode);
+        return;
+    }
 
     // The spans are read with per-lane (vector) loads: a scalar load would
     // share lgkmcnt with the LDS traffic that every level waits for.  A
@@ -2831,6 +2847,7 @@ SweepArgs<T> sweep_args(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int 
     a.lane_zero = 0;
     a.pool = ctx->pool_slots;
     a.wbase = (ctx->pool_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W;
+    a.nthr = ctx->block;
     for (int k = 0; k < ctx->Q * ctx->Q; ++k) a.kref[k] = S::kref(ctx)[k];
     a.mass = ctx->mass; a.gx = ctx->gx; a.gy = ctx->gy; a.fcm = ctx->fcm; a.geo = ctx->geo64;
     a.cxv64 = ctx->cxv; a.cyv64 = ctx->cyv; a.inv_tau_c = 1.0 / ctx->tau_c;
