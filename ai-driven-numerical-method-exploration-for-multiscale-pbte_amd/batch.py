@@ -92,10 +92,10 @@ def run_source_batch(solvers, callback=None):
     in lockstep; returns their ``RunRecord`` objects in order.
 
     Per step: the wall emission of each unfinished problem, one batched
-    sweep per group of one precision and one element degree, each
-    problem's moments (or GSIS correction), the residual and mass of every
-    problem summed on the device, one wait, then the stop tests of ``Solver._run_source`` (the fp32 switch
-    included).  ``callback(k, step, res, solver)`` after each problem's
+    sweep per group of one precision, element degree and pipeline depth,
+    each problem's moments (or GSIS correction), the residual and mass of
+    every problem summed on the device, one wait, then the stop tests of
+    ``Solver._run_source`` (the fp32 switch included).  ``callback(k, step, res, solver)`` after each problem's
     step.  Options that need the transport residual per step
     (``iteration.true_residual``) are not supported here."""
     tracks = [_Track(s) for s in solvers]
@@ -122,12 +122,13 @@ def run_source_batch(solvers, callback=None):
                 s = tracks[k].s
                 if s.bcdata.has_nonthermalising:
                     s.ctx.wall_flux(s.sweep_vdf)
-            # one launch per (precision, element degree): a launch runs one
-            # kernel instance
+            # one launch per (precision, element degree, pipeline depth): a
+            # launch runs one kernel instance
             groups = {}
             for k in active:
                 s = tracks[k].s
-                groups.setdefault((s.sweep_precision, s.ndof), []).append(k)
+                depth = 2 if s.ctx.sweep_variant == 2 else 3
+                groups.setdefault((s.sweep_precision, s.ndof, depth), []).append(k)
             for group in groups.values():
                 batch_of(group).sweep([tracks[k].s.mom for k in group],
                                       [tracks[k].s.sweep_vdf for k in group])

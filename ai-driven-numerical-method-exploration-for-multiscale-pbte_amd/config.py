@@ -196,9 +196,17 @@ class Performance:
     # moments is below this, the accuracy asked of a mixed-precision solve;
     # an addition of this build
     refine_tol: float = 0.0
+    # time the sweep's tile shapes (directions per workgroup, workgroup
+    # width, pipeline depth) on the case's own mesh and ordinates and sweep
+    # with the fastest; results kept per process and, when autotune_cache
+    # names a JSON file, across runs; an addition of this build
+    autotune: bool = False
+    autotune_cache: str = ""
 
     def __post_init__(self):
         self.precompute_inverse = bool(self.precompute_inverse)
+        self.autotune = bool(self.autotune)
+        self.autotune_cache = str(self.autotune_cache or "")
         key = str(self.kernel).lower()
         if key not in _KERNEL_ALIASES:
             raise ValueError("performance.kernel must be 'hip' "

@@ -45,6 +45,11 @@ class Solver:
         self.mesh, self.basis, self.vel = pb.mesh, pb.basis, pb.vel
         self.integrals, self.order, self.bcdata = pb.integrals, pb.order, pb.bcdata
         self.ndir, self.n_tris, self.ndof = self.vel.ndir, self.mesh.n_tris, c.ndof_tri
+        self.tile_shape = None
+        if c.performance.autotune and not (max_bundle or block or sweep_variant):
+            from .autotune import tune
+            self.tile_shape = tune(c, pb, max_bytes=max_bytes)
+            max_bundle, block, sweep_variant = (self.tile_shape[k] for k in ("max_bundle", "block", "variant"))
         self.ctx = SweepContext.build(c, self.mesh, self.integrals, self.vel, self.order,
                                       self.bcdata, max_bundle=max_bundle, block=block,
                                       variant=sweep_variant, lds_slots_cap=lds_slots_cap,
@@ -223,6 +228,8 @@ class Solver:
                          "hmin": float(self.mesh.hmin), "n_faces_b": int(self.mesh.n_faces_b),
                          "boundary_types": [b.type for b in c.boundaries],
                          **{f"schedule_{k}": v for k, v in self.ctx.schedule_info().items()}})
+        if self.tile_shape is not None:
+            rec.diagnostics["tile_shape"] = {k: self.tile_shape[k] for k in ("max_bundle", "block", "variant", "ms")}
         if self.acc is not None:
             rec.diagnostics.update(self.acc.diagnostics())
         if extra:

@@ -60,6 +60,7 @@ class SweepContext:
         self.kernel = "hip"
         self.use_pbuf = False
         self.pbuf = None
+        self.sweep_variant = 0
 
     # -- construction ------------------------------------------------------
     @classmethod
@@ -113,6 +114,7 @@ class SweepContext:
         su.face_tri, su.face_lfc = i32(mesh.face_tri), i32(mesh.face_lfc)
         su.order = i32(sweep_order.order)
         su.block, su.sweep_variant = int(block), int(variant)
+        self.sweep_variant = int(variant)
         su.lds_slots_cap = int(lds_slots_cap)
         su.lds_value_bytes = 4 if case.performance.precision == "fp32" else 8
         # max_bundle 0: with at least two directions per CU, pairs of

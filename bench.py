@@ -43,7 +43,7 @@ unknowns) does not fit the host SuperLU factorisation.
 --workload batch runs the batched configuration: 64 independent problems
 drawn from default_rng(1) (2k-65k jittered P1/P2 triangles, 64-512
 ordinates, Kn_R log-uniform on [1e-3, 10]); a step is one FP32 sweep of
-every problem, one launch per element degree; value is the aggregate
+every problem, one launch per element degree and pipeline depth; value is the aggregate
 DOF-updates/s, with the problem-by-problem launches timed beside it, and a
 GSIS solve leg of the same draws on coarser meshes (batch_workload).
 """
@@ -306,6 +306,7 @@ def batch_workload(pkg, args, work):
         npole, nazim = BATCH_DIRS[d["ndir"]]
         case = make_case(pkg, d["n"], d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp32", work,
                          jitter=0.25, seed=1, method="source")
+        case.performance.autotune = args.autotune
         s = pkg.Solver(case)
         P, N = s.ndof, s.n_tris
         s.mom.set_dofs(rng.random((P, N)), 0.1 * rng.random((P, N)), 0.1 * rng.random((P, N)))
@@ -315,7 +316,7 @@ def batch_workload(pkg, args, work):
     setup_s = time.perf_counter() - t0
     groups = {}
     for k, s in enumerate(solvers):
-        groups.setdefault(s.ndof, []).append(k)
+        groups.setdefault((s.ndof, 2 if s.ctx.sweep_variant == 2 else 3), []).append(k)
     batches = [(SweepBatch([solvers[k].ctx for k in g]), g) for g in groups.values()]
 
     def step_batched():
@@ -357,7 +358,9 @@ def batch_workload(pkg, args, work):
         npole, nazim = BATCH_DIRS[d["ndir"]]
         cpu_case = (small, make_case(pkg, d["n"], d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp64",
                                      work, jitter=0.25, seed=1, method="source"))
-    shapes = [{"n_tris": s.n_tris, "ndof": s.ndof, "ndir": s.ndir, "tau_r": d["tau_r"], "tau_n": d["tau_n"]}
+    shapes = [{"n_tris": s.n_tris, "ndof": s.ndof, "ndir": s.ndir, "tau_r": d["tau_r"], "tau_n": d["tau_n"],
+               "tile": None if s.tile_shape is None else
+               [s.tile_shape[k] for k in ("max_bundle", "block", "variant")]}
               for s, d in zip(solvers, draws)]
     del solvers, batches
     torch.cuda.empty_cache()
@@ -369,7 +372,8 @@ def batch_workload(pkg, args, work):
         "data": "synthetic moments on the drawn problems' meshes and ordinates",
         "config": {"workload": f"batch of {len(draws)} jittered squares (seed 1), "
                                "2k-65k P1/P2 triangles, 64-512 ordinates, fp32 sweep",
-                   "problems": len(draws), "dof_per_step": dof, "launches_per_step": len(groups)},
+                   "problems": len(draws), "dof_per_step": dof, "launches_per_step": len(groups),
+                   "autotune": bool(args.autotune)},
         "roofline": {"bound": "hbm", "achieved": bytes_alg / t_step / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": bytes_alg / t_step / HBM_PEAK, "traffic": None,
                      "bytes_alg_per_launch": bytes_alg / len(groups),
@@ -402,6 +406,7 @@ def batch_solves(pkg, args, work, draws):
         npole, nazim = BATCH_DIRS[d["ndir"]]
         case = make_case(pkg, max(4, d["n"] // div), d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp32",
                          work, jitter=0.25, seed=1, method="source", accflag=1, tmax=args.batch_tmax)
+        case.performance.autotune = args.autotune
         solvers.append(pkg.Solver(case))
     setup = time.perf_counter() - t0
     log(f"batch: {len(solvers)} GSIS problems set up in {setup:.1f} s; solving")
@@ -478,6 +483,8 @@ def main():
     ap.add_argument("--hole", type=float, default=0.0,
                     help="radius of a central circular hole (hot wall; the outer walls cold)")
     ap.add_argument("--tau-r", type=float, default=1.0, help="Kn_R of the sweep workload (Kn_N = 0.1 Kn_R)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the sweep's tile shapes by timing them (performance.autotune)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true", help="skip the Krylov solve leg")
@@ -512,6 +519,7 @@ def main():
     log(f"mesh {args.n}x{args.n} (hole {args.hole:g}), P{args.deg}, {args.npole}x{args.nazim} ordinates")
     case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, args.tau_r, 0.1 * args.tau_r,
                      args.precision, work, jitter=args.jitter, hole=args.hole)
+    case.performance.autotune = args.autotune
     t0 = time.perf_counter()
     s = pkg.Solver(case, max_bundle=args.bundle, block=args.block, sweep_variant=args.variant)
     setup_s = time.perf_counter() - t0
@@ -556,6 +564,7 @@ def main():
         "sweep_kernel_ms_mean": t_sweep * 1e3,
         "sweep_kernel_ms_min": float(np.min(ms)),
         "schedule": s.ctx.schedule_info(),
+        "tile_shape": s.tile_shape,
         "setup_s": setup_s,
         "factor_mode": s.ctx.mode,
     }
