@@ -16,7 +16,7 @@ import time
 from . import _native
 from .device import dev_zeros, ptr, torch
 
-__all__ = ["SweepBatch", "run_source_batch"]
+__all__ = ["SweepBatch", "launch_key", "run_source_batch"]
 
 _PREC = {torch.float64: _native.PBTE_FP64, torch.float32: _native.PBTE_FP32}
 
@@ -70,6 +70,13 @@ class SweepBatch:
             c.sweep_count += 1
 
 
+def launch_key(ctx) -> tuple:
+    """Contexts with equal keys can share a launch: one kernel instance
+    (element degree, pipeline depth, and whether every bundle holds one
+    direction, whose instance drops the bundle arithmetic)."""
+    return (ctx.nd, 2 if ctx.sweep_variant == 2 else 3, ctx.one_dir)
+
+
 class _Track:
     """One problem's histories and switch state (``Solver._run_source``)."""
 
@@ -92,11 +99,11 @@ def run_source_batch(solvers, callback=None):
     in lockstep; returns their ``RunRecord`` objects in order.
 
     Per step: the wall emission of each unfinished problem, one batched
-    sweep per group of one precision, element degree and pipeline depth,
-    each problem's moments (or GSIS correction), the residual and mass of
-    every problem summed on the device, one wait, then the stop tests of
-    ``Solver._run_source`` (the fp32 switch included).  ``callback(k, step, res, solver)`` after each problem's
-    step.  Options that need the transport residual per step
+    sweep per group of one precision and one ``launch_key``, each
+    problem's moments (or GSIS correction), the residual and mass of every
+    problem summed on the device, one wait, then the stop tests of
+    ``Solver._run_source`` (the fp32 switch included).
+    ``callback(k, step, res, solver)`` after each problem's step.  Options that need the transport residual per step
     (``iteration.true_residual``) are not supported here."""
     tracks = [_Track(s) for s in solvers]
     for t in tracks:
@@ -122,13 +129,10 @@ def run_source_batch(solvers, callback=None):
                 s = tracks[k].s
                 if s.bcdata.has_nonthermalising:
                     s.ctx.wall_flux(s.sweep_vdf)
-            # one launch per (precision, element degree, pipeline depth): a
-            # launch runs one kernel instance
             groups = {}
             for k in active:
                 s = tracks[k].s
-                depth = 2 if s.ctx.sweep_variant == 2 else 3
-                groups.setdefault((s.sweep_precision, s.ndof, depth), []).append(k)
+                groups.setdefault((s.sweep_precision,) + launch_key(s.ctx), []).append(k)
             for group in groups.values():
                 batch_of(group).sweep([tracks[k].s.mom for k in group],
                                       [tracks[k].s.sweep_vdf for k in group])

@@ -61,6 +61,7 @@ class SweepContext:
         self.use_pbuf = False
         self.pbuf = None
         self.sweep_variant = 0
+        self.one_dir = True
 
     # -- construction ------------------------------------------------------
     @classmethod
@@ -146,6 +147,9 @@ class SweepContext:
                     continue
             break
         self.handle = h
+        # every bundle one direction: the sweep runs the kernel instance
+        # without bundle arithmetic (a batched launch runs one instance)
+        self.one_dir = self.schedule_info()["bundles"] == self.ndir
         self.flux_wall = dev_zeros((max(self.n_faces, 1), self.nd))
         self.precision = case.performance.precision
         if precompute is None:

@@ -43,7 +43,7 @@ unknowns) does not fit the host SuperLU factorisation.
 --workload batch runs the batched configuration: 64 independent problems
 drawn from default_rng(1) (2k-65k jittered P1/P2 triangles, 64-512
 ordinates, Kn_R log-uniform on [1e-3, 10]); a step is one FP32 sweep of
-every problem, one launch per element degree and pipeline depth; value is the aggregate
+every problem, one launch per kernel instance (batch.launch_key); value is the aggregate
 DOF-updates/s, with the problem-by-problem launches timed beside it, and a
 GSIS solve leg of the same draws on coarser meshes (batch_workload).
 """
@@ -282,9 +282,10 @@ def batch_workload(pkg, args, work):
     diffuse isothermal walls with the north one at dT = +1.
 
     Sweep leg, at the drawn sizes: one step is one FP32 sweep of every
-    problem; the problems of one element degree go in one launch of the
-    pipelined kernel (pbte_batch_sweep), so a step is two launches (plus
-    each problem's source kernel).  value = total DOF-updates per step /
+    problem; the problems that run one kernel instance (element degree,
+    pipeline depth, one or two directions per workgroup: batch.launch_key)
+    go in one launch of the pipelined kernel (pbte_batch_sweep), plus each
+    problem's source kernel.  value = total DOF-updates per step /
     wall time per step; the same sweeps launched problem by problem are
     timed beside it.
 
@@ -296,7 +297,7 @@ def batch_workload(pkg, args, work):
     tolerance.  The full-size GSIS solves are not run: each GSIS iteration
     solves the HDG trace system with SuperLU on the host."""
     import torch
-    from pbte_amd.batch import SweepBatch
+    from pbte_amd.batch import SweepBatch, launch_key
     draws = batch_draws(args.batch_count, 1)
     log(f"batch: building {len(draws)} problems (sweep leg)")
     t0 = time.perf_counter()
@@ -316,7 +317,7 @@ def batch_workload(pkg, args, work):
     setup_s = time.perf_counter() - t0
     groups = {}
     for k, s in enumerate(solvers):
-        groups.setdefault((s.ndof, 2 if s.ctx.sweep_variant == 2 else 3), []).append(k)
+        groups.setdefault(launch_key(s.ctx), []).append(k)
     batches = [(SweepBatch([solvers[k].ctx for k in g]), g) for g in groups.values()]
 
     def step_batched():
