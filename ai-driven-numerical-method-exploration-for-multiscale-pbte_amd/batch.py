@@ -71,7 +71,7 @@ class SweepBatch:
 
 
 def launch_key(ctx) -> tuple:
-    """Contexts with equal keys can share a launch: one kernel instance
+    """Contexts with equal keys share a launch: one kernel instance
     (element degree, pipeline depth, and whether every bundle holds one
     direction, whose instance drops the bundle arithmetic)."""
     return (ctx.nd, 2 if ctx.sweep_variant == 2 else 3, ctx.one_dir)
@@ -103,8 +103,9 @@ def run_source_batch(solvers, callback=None):
     problem's moments (or GSIS correction), the residual and mass of every
     problem summed on the device, one wait, then the stop tests of
     ``Solver._run_source`` (the fp32 switch included).
-    ``callback(k, step, res, solver)`` after each problem's step.  Options that need the transport residual per step
-    (``iteration.true_residual``) are not supported here."""
+    ``callback(k, step, res, solver)`` after each problem's step.  Options
+    that need the transport residual per step (``iteration.true_residual``)
+    are not supported here."""
     tracks = [_Track(s) for s in solvers]
     for t in tracks:
         if t.s.case.iteration.true_residual:
