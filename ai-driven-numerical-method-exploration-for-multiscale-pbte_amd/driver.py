@@ -19,13 +19,14 @@ from __future__ import annotations
 
 import hashlib
 import time
+import warnings
 from pathlib import Path
 
 import numpy as np
 
 from .config import Case
 from .device import dev_zeros, require_gpu, torch
-from .io_output import RunRecord
+from .io_output import RunRecord, _fields, append_runtime_log, locate_points, sampling_grid, write_tecplot
 from .moments import Moments
 from .problem import build_problem
 from .sweep import SweepContext
@@ -235,3 +236,39 @@ class Solver:
         if extra:
             rec.diagnostics.update(extra)
         return rec
+
+    # -- output (driver.py:256-295) --------------------------------------------
+    def write_outputs(self, rec: RunRecord) -> dict:
+        """The run's files under ``output.dir``, as the reference writes
+        them: ``run_record.json``, a ``RunTime.txt`` line, and the fields
+        T, qx, qy (and the GSIS stress trio Nxx, Nxy, Nyy) sampled on the
+        output grid into ``field.npz`` and/or ``field.dat`` (Tecplot).
+        The fields come from the moment DOFs in HBM (one copy of ``3 P N``
+        values, not of the distribution)."""
+        c = self.case
+        out = c.output_path()
+        out.mkdir(parents=True, exist_ok=True)
+        written = {}
+        if c.output.run_record:
+            written["run_record"] = str(rec.to_json(out / "run_record.json"))
+        if c.output.runtime_log:
+            written["runtime_log"] = str(append_runtime_log(
+                out / "RunTime.txt", rec.scheme, c.flow.tau_r, c.flow.tau_n, self.n_tris,
+                c.velmesh.npole, c.velmesh.nazim, rec.iterations, rec.wall_clock))
+        if c.output.field:
+            px, py = sampling_grid(c.output.field_nx, c.output.field_ny)
+            triid = locate_points(self.mesh, px, py)
+            missing = int(np.count_nonzero(triid < 0))
+            if missing:
+                warnings.warn(f"{missing} sampling points fell outside the mesh")
+            uq = self.acc.uq_d.cpu().numpy() if self.acc is not None else None
+            m = self.mom
+            fields = _fields(self.mesh, self.basis, m.ts, m.qxs, m.qys, px, py, triid, uq, self.ndof)
+            names = ["T", "qx", "qy", "Nxx", "Nxy", "Nyy"]
+            if c.output.npz:
+                np.savez_compressed(out / "field.npz", px=px, py=py, triid=triid,
+                                    **dict(zip(names, fields)))
+                written["field_npz"] = str(out / "field.npz")
+            if c.output.tecplot:
+                written["field_dat"] = str(write_tecplot(out / "field.dat", px, py, fields, names))
+        return written

@@ -1,0 +1,42 @@
+"""Solver.write_outputs after a device run: the files the reference writes,
+with the fields sampled from the moment DOFs in HBM equal to the fields
+sampled from the host distribution (the reference's route)."""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import CASES
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("accflag", [0, 1])
+def test_write_outputs_after_a_run(pkg, tmp_path, accflag):
+    from pbte_amd.io_output import locate_points, sample_fields, sampling_grid
+    c = pkg.Case.from_yaml(CASES / "F1_1_1.yaml")
+    c.scheme.accflag = accflag
+    c.iteration.tmax = 40
+    c.output.dir = str(tmp_path / "out")
+    c.output.field = c.output.run_record = c.output.runtime_log = True
+    c.output.npz = c.output.tecplot = True
+    print(f"case write_outputs F1_1_1 accflag {accflag}", flush=True)
+    s = pkg.Solver(c)
+    rec = s.run()
+    written = s.write_outputs(rec)
+    assert set(written) == {"run_record", "runtime_log", "field_npz", "field_dat"}
+    rr = json.loads((tmp_path / "out" / "run_record.json").read_text())
+    assert rr["iterations"] == rec.iterations and rr["mass"] == rec.mass
+    line = (tmp_path / "out" / "RunTime.txt").read_text().splitlines()[-1]
+    assert line.startswith(" GSIS" if accflag else "  CIS") and int(line[47:55]) == rec.iterations
+    f = np.load(tmp_path / "out" / "field.npz")
+    px, py = sampling_grid()
+    assert np.array_equal(f["px"], px) and np.array_equal(f["triid"], locate_points(s.mesh, px, py))
+    uq = s.acc.uq_d.cpu().numpy() if accflag else None
+    cx, cy, w = s.vel.flat()
+    ref = sample_fields(s.mesh, s.basis, s.vdf, cx, cy, w, c.flow.cv, px, py, f["triid"], uq=uq)
+    for k, name in enumerate(["T", "qx", "qy", "Nxx", "Nxy", "Nyy"]):
+        np.testing.assert_allclose(f[name], ref[k], rtol=0, atol=1e-12 * max(1.0, np.abs(ref[k]).max()))
+    assert np.abs(f["T"]).max() > 0.1 and (np.abs(f["Nxx"]).max() > 0) == bool(accflag)
+    dat = (tmp_path / "out" / "field.dat").read_text().splitlines()
+    assert len(dat) == 2 + 109 * 109
