@@ -243,8 +243,11 @@ class Solver:
         them: ``run_record.json``, a ``RunTime.txt`` line, and the fields
         T, qx, qy (and the GSIS stress trio Nxx, Nxy, Nyy) sampled on the
         output grid into ``field.npz`` and/or ``field.dat`` (Tecplot).
-        The fields come from the moment DOFs in HBM (one copy of ``3 P N``
-        values, not of the distribution)."""
+        The fields are the angular moments of the distribution, as the
+        reference samples them (after a GSIS correction the run's moments
+        are not exactly those of the corrected distribution), summed in
+        HBM by the moment kernel: one copy of ``3 P N`` values to the
+        host, not of the distribution."""
         c = self.case
         out = c.output_path()
         out.mkdir(parents=True, exist_ok=True)
@@ -262,7 +265,8 @@ class Solver:
             if missing:
                 warnings.warn(f"{missing} sampling points fell outside the mesh")
             uq = self.acc.uq_d.cpu().numpy() if self.acc is not None else None
-            m = self.mom
+            m = Moments.zeros(self.ndof, self.n_tris)
+            self.ctx.moments(self.sweep_vdf, m)
             fields = _fields(self.mesh, self.basis, m.ts, m.qxs, m.qys, px, py, triid, uq, self.ndof)
             names = ["T", "qx", "qy", "Nxx", "Nxy", "Nyy"]
             if c.output.npz:
