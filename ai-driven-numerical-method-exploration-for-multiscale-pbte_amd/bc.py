@@ -26,6 +26,25 @@ class BoundaryData:
     has_periodic: bool
     n_faces: int
     ndof: int
+    _ctx: object = None          # the owning solver's SweepContext
+
+    @property
+    def flux_wall(self) -> np.ndarray:
+        """Host copy of the adiabatic emission ``flux_wall[F, P]``."""
+        if self._ctx is None:
+            return np.zeros((self.n_faces, self.ndof))
+        return self._ctx.flux_wall.detach().cpu().numpy()
+
+    def update_wall_flux(self, vdf, cxv=None, cyv=None, domega=None, int_tri_fc=None, mesh=None) -> None:
+        """Diffuse emission of the adiabatic walls from ``vdf``
+        (``bc.py:47-58``, ``_kernels.py:364-397``), into the device
+        ``flux_wall`` the sweeps read.  ``vdf`` is the device tensor or the
+        solver's host array; the ordinates, face integrals and mesh are the
+        solver's own (the reference's argument list is accepted)."""
+        if self._ctx is None:
+            raise TypeError("update_wall_flux needs the boundary data of a Solver")
+        if self.has_nonthermalising:
+            self._ctx.wall_flux(vdf)
 
 
 def build_boundary_data(mesh, boundaries, ndof_tri: int) -> BoundaryData:

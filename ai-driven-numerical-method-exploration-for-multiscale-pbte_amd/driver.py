@@ -31,7 +31,67 @@ from .moments import Moments
 from .problem import build_problem
 from .sweep import SweepContext
 
-__all__ = ["Solver"]
+__all__ = ["Solver", "HostVdf"]
+
+
+class _Mirror:
+    """Which copy of a solver's distribution is newer: the host array
+    (``dirty``, written through ``Solver.vdf``) or the device tensor."""
+
+    def __init__(self, solver):
+        self.solver = solver
+        self.array = None
+        self.dirty = False
+
+    def push(self) -> None:
+        """Host writes to the device (when there are any)."""
+        if self.dirty:
+            s = self.solver
+            s.sweep_vdf.copy_(torch.from_numpy(np.asarray(self.array)).to(s.sweep_vdf.dtype))
+            self.dirty = False
+
+    def pull(self) -> None:
+        """The device state into the host array, in place."""
+        np.asarray(self.array)[...] = self.solver.sweep_vdf.detach().to(torch.float64).cpu().numpy()
+        self.dirty = False
+
+
+class HostVdf(np.ndarray):
+    """``Solver.vdf``: the host ``[D, N, P]`` binary64 mirror of the
+    distribution in HBM, as callers of the reference use ``solver.vdf``
+    (``run_agent_case.py``, ``evaluate.py``): element writes
+    (``solver.vdf[:] = f``, also through views and in-place operators) make
+    it newer than the device copy; the solver's device operations given it
+    (``compute_moments``, ``update_wall_flux``, ``ctx.true_residual``,
+    ``ctx.sweep``) upload it first, and ``ctx.sweep`` writes its result
+    back into it, in place."""
+
+    def __array_finalize__(self, obj):
+        shared = obj is not None and np.may_share_memory(self, obj)
+        self._mirror = getattr(obj, "_mirror", None) if shared else None
+
+    def _touch(self):
+        if self._mirror is not None:
+            self._mirror.dirty = True
+
+    def __setitem__(self, key, value):
+        super().__setitem__(key, value)
+        self._touch()
+
+
+def _inplace(name):
+    base = getattr(np.ndarray, name)
+
+    def op(self, other):
+        out = base(self, other)
+        self._touch()
+        return out
+    op.__name__ = name
+    return op
+
+
+for _name in ("__iadd__", "__isub__", "__imul__", "__itruediv__", "__ipow__"):
+    setattr(HostVdf, _name, _inplace(_name))
 
 
 class Solver:
@@ -63,6 +123,9 @@ class Solver:
                         if self.precision == "fp32" else None)
         self.mom = Moments.zeros(self.ndof, self.n_tris)
         self.mom._ctx = self.ctx
+        self._mirror = _Mirror(self)
+        self.ctx._solver = self
+        self.bcdata._ctx = self.ctx
         self.t_old_d = dev_zeros(self.n_tris)
         self.restart_info = self._apply_restart()
         self.acc = None
@@ -86,12 +149,47 @@ class Solver:
 
     @property
     def vdf(self) -> np.ndarray:
-        """Host copy of the distribution ``[D, N, P]`` (binary64)."""
-        return self.sweep_vdf.detach().to(torch.float64).cpu().numpy()
+        """The distribution ``[D, N, P]`` (binary64) as a host array
+        (``HostVdf``): refreshed from HBM on access, or, when it was written
+        on the host since, uploaded to HBM instead."""
+        m = self._mirror
+        if m.array is None:
+            m.array = np.empty((self.ndir, self.n_tris, self.ndof)).view(HostVdf)
+            m.array._mirror = m
+            m.pull()
+        elif m.dirty:
+            m.push()
+        else:
+            m.pull()
+        return m.array
 
     @vdf.setter
     def vdf(self, value) -> None:
         self.sweep_vdf.copy_(torch.as_tensor(np.ascontiguousarray(value)).to(self.sweep_vdf.dtype))
+        self._mirror.dirty = False
+
+    def device_vdf(self, vdf):
+        """The device tensor a device operation runs on for ``vdf``, and
+        the function that hands a changed distribution back (None when
+        there is nothing to hand back): a device tensor is itself; this
+        solver's host mirror is the solver's distribution (host writes
+        uploaded first); any other host array ``[D, N, P]`` is copied to a
+        scratch binary64 tensor and copied back."""
+        if isinstance(vdf, torch.Tensor):
+            return vdf, None
+        m = getattr(vdf, "_mirror", None)
+        shape = (self.ndir, self.n_tris, self.ndof)
+        if m is self._mirror and vdf.shape == shape and np.shares_memory(vdf, m.array):
+            m.push()
+            return self.sweep_vdf, m.pull
+        arr = np.asarray(vdf)
+        if arr.shape != shape:
+            raise ValueError(f"vdf shape {arr.shape} does not match the discretisation {shape}")
+        dev = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(self.vdf_d.device)
+
+        def back():
+            arr[...] = dev.cpu().numpy()
+        return dev, back
 
     def vdf_fortran(self) -> np.ndarray:
         npole, nazim = self.case.velmesh.npole, self.case.velmesh.nazim

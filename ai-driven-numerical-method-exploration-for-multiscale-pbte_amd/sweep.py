@@ -62,6 +62,7 @@ class SweepContext:
         self.pbuf = None
         self.sweep_variant = 0
         self.one_dir = True
+        self._solver = None                     # the Solver that owns this context
 
     # -- construction ------------------------------------------------------
     @classmethod
@@ -219,11 +220,28 @@ class SweepContext:
         return self.pbuf
 
     # -- operators ---------------------------------------------------------
+    def _device(self, vdf):
+        """``(device tensor, hand-back)`` for a distribution argument: a
+        device tensor as it is; a host array (the owning solver's ``vdf``
+        mirror, or any ``[D, N, P]`` array) through ``Solver.device_vdf``."""
+        if isinstance(vdf, torch.Tensor):
+            return vdf, None
+        if self._solver is None:
+            raise TypeError("a host distribution needs the Solver that owns this context")
+        return self._solver.device_vdf(vdf)
+
     def sweep(self, mom, vdf, homogeneous: bool = False, flux_wall=None, src=None) -> None:
-        """One transport sweep in place on the device tensor ``vdf``.
+        """One transport sweep of ``vdf`` in place: the device tensor, or a
+        host array (``solver.vdf``) written back after the sweep.
 
         ``src`` optionally overrides the moment source with a tuple of device
         ``[P, N]`` binary64 tensors ``(ts, qxs, qys)``."""
+        vdf, back = self._device(vdf)
+        self._sweep(mom, vdf, homogeneous, flux_wall, src)
+        if back is not None:
+            back()
+
+    def _sweep(self, mom, vdf, homogeneous, flux_wall, src) -> None:
         prec = _PREC[vdf.dtype]
         ts, qxs, qys = src if src is not None else (mom.ts_d, mom.qxs_d, mom.qys_d)
         fw = self.flux_wall if flux_wall is None else flux_wall
@@ -239,18 +257,21 @@ class SweepContext:
 
     def moments(self, vdf, mom, out=None) -> None:
         """Moments of ``vdf`` into ``mom`` (or into ``out`` = 6 device tensors)."""
+        vdf, _ = self._device(vdf)
         prec = _PREC[vdf.dtype]
         o = out if out is not None else (mom.ts_d, mom.qxs_d, mom.qys_d, mom.temp_d, mom.qx_d, mom.qy_d)
         st = _native.lib().pbte_moments(self.handle, prec, ptr(vdf), *(ptr(t) for t in o))
         _native.check(self.handle, st, "moments")
 
     def wall_flux(self, vdf, out=None) -> None:
+        vdf, _ = self._device(vdf)
         prec = _PREC[vdf.dtype]
         st = _native.lib().pbte_wall_flux(self.handle, prec, ptr(vdf),
                                           ptr(self.flux_wall if out is None else out))
         _native.check(self.handle, st, "wall_flux")
 
     def transport_residual_parts(self, mom, vdf):
+        vdf, _ = self._device(vdf)
         if vdf.dtype != torch.float64:
             vdf = vdf.to(torch.float64)
         out = np.zeros(2 * self.ndir)

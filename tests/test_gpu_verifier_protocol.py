@@ -1,0 +1,75 @@
+"""The legacy verifier's protocol against the device solver, with the
+reference's own calls (tasks/t01-square/verifier/run_agent_case.py and
+evaluate.py): run a case, dump ``solver.vdf`` and ``rec.temp``; in a fresh
+solver built with ``tmax = 0`` write ``solver.vdf[:] = vdf``, then
+``compute_moments``, ``update_wall_flux`` (adiabatic walls) and
+``ctx.true_residual`` on ``solver.vdf``; gates: transport residual below
+1e-7 and the temperature within 1e-5 (relative to its largest value) of the
+certified fixed point (cells.json: transport_residual_max, field_gap_max)."""
+import numpy as np
+import pytest
+
+from conftest import CASES, GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(pkg, tag):
+    c = pkg.Case.from_yaml(CASES / f"{tag}.yaml")
+    c.output.field = c.output.run_record = c.output.runtime_log = False
+    c.scheme.method = "krylov"
+    return c
+
+
+@pytest.mark.parametrize("tag", ["F1_1e-1", "F2_1e-1", "F3_1e-1"])
+def test_verifier_gates_through_the_host_distribution(pkg, tag):
+    from pbte_amd.moments import compute_moments
+    print(f"case verifier protocol {tag}", flush=True)
+    solver = pkg.Solver(_case(pkg, tag))
+    rec = solver.run()
+    vdf, temp = np.asarray(solver.vdf), np.asarray(rec.temp)
+
+    case = _case(pkg, tag)
+    case.iteration.tmax = 0
+    s = pkg.Solver(case)
+    assert vdf.shape == s.vdf.shape
+    s.vdf[:] = vdf
+    compute_moments(s.vdf, s.ctx.cxv, s.ctx.cyv, s.ctx.domega, s.integrals.int_tri, case.flow.cv, s.mom)
+    if s.bcdata.has_nonthermalising:
+        s.bcdata.update_wall_flux(s.vdf, s.ctx.cxv, s.ctx.cyv, s.ctx.domega, s.integrals.int_tri_fc, s.mesh)
+    tr = float(s.ctx.true_residual(s.mom, s.vdf))
+    ref = np.load(GOLDEN / "krylov" / f"{tag}.npz")
+    scale = float(np.abs(ref["temp"]).max())
+    assert tr < 1e-7
+    assert np.abs(temp - ref["temp"]).max() / scale < 1e-5
+    assert np.abs(s.mom.temp - ref["temp"]).max() / scale < 1e-5
+    # the same residual from the device state the host array was written to
+    assert tr == float(s.ctx.true_residual(s.mom, s.vdf_d))
+
+
+def test_host_array_sweep_is_in_place_and_equals_the_device_sweep(pkg):
+    """``ctx.sweep(mom, solver.vdf)`` updates the host array in place with
+    the device sweep's result; writes through a view reach the device with
+    the next device operation given the array."""
+    c = _case(pkg, "F2_1e-1")
+    c.iteration.tmax = 0
+    s = pkg.Solver(c)
+    rng = np.random.default_rng(4)
+    f0 = rng.standard_normal(s.vdf.shape)
+    s.mom.set_dofs(*(rng.standard_normal((s.ndof, s.n_tris)) for _ in range(3)))
+    h = s.vdf
+    h[:] = f0
+    s.ctx.wall_flux(h)
+    s.ctx.sweep(s.mom, h)
+    got = np.array(h)
+    assert not np.array_equal(got, f0)
+    s.vdf = f0
+    s.ctx.wall_flux(s.vdf_d)
+    s.ctx.sweep(s.mom, s.vdf_d)
+    assert np.array_equal(s.vdf, got)
+    v = s.vdf
+    v[3] *= 2.0                                     # a view of one direction, in place
+    # host writes reach HBM with the next device operation given the array
+    s.ctx.moments(v, s.mom)
+    assert np.array_equal(s.vdf_d[3].cpu().numpy(), 2.0 * got[3])
+    assert np.array_equal(s.vdf_d[2].cpu().numpy(), got[2])
