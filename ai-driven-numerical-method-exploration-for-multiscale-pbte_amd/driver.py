@@ -20,6 +20,7 @@ from __future__ import annotations
 import hashlib
 import time
 import warnings
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -39,9 +40,13 @@ class _Mirror:
     (``dirty``, written through ``Solver.vdf``) or the device tensor."""
 
     def __init__(self, solver):
-        self.solver = solver
+        self._solver = weakref.ref(solver)     # no cycle: the solver frees its HBM on del
         self.array = None
         self.dirty = False
+
+    @property
+    def solver(self):
+        return self._solver()
 
     def push(self) -> None:
         """Host writes to the device (when there are any)."""
@@ -124,7 +129,7 @@ class Solver:
         self.mom = Moments.zeros(self.ndof, self.n_tris)
         self.mom._ctx = self.ctx
         self._mirror = _Mirror(self)
-        self.ctx._solver = self
+        self.ctx._solver = weakref.ref(self)
         self.bcdata._ctx = self.ctx
         self.t_old_d = dev_zeros(self.n_tris)
         self.restart_info = self._apply_restart()

@@ -62,7 +62,7 @@ class SweepContext:
         self.pbuf = None
         self.sweep_variant = 0
         self.one_dir = True
-        self._solver = None                     # the Solver that owns this context
+        self._solver = None                     # weak reference to the Solver that owns this context
 
     # -- construction ------------------------------------------------------
     @classmethod
@@ -226,9 +226,10 @@ class SweepContext:
         mirror, or any ``[D, N, P]`` array) through ``Solver.device_vdf``."""
         if isinstance(vdf, torch.Tensor):
             return vdf, None
-        if self._solver is None:
+        solver = self._solver() if self._solver is not None else None
+        if solver is None:
             raise TypeError("a host distribution needs the Solver that owns this context")
-        return self._solver.device_vdf(vdf)
+        return solver.device_vdf(vdf)
 
     def sweep(self, mom, vdf, homogeneous: bool = False, flux_wall=None, src=None) -> None:
         """One transport sweep of ``vdf`` in place: the device tensor, or a

@@ -435,7 +435,7 @@ def b_alg(D, N, P, s_lu, s_f):
     return D * N * (P * P * s_lu + P + 2 * P * s_f) + N * 8 * (4 * P * P + 3 * P + 6)
 
 
-def cpu_baseline(pkg, case, seconds=15.0, max_sweeps=20):
+def cpu_baseline(pkg, case, seconds=15.0, max_sweeps=1000):
     """The C restatement of the reference sweep (OpenMP over directions,
     cells in the reference order, precomputed LU) on this host."""
     from oracle import cpu
@@ -491,6 +491,12 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the Krylov solve leg")
     ap.add_argument("--solve-kn", default="0.01,0.1,1,10",
                     help="comma-separated Kn_R values of the Krylov solve leg")
+    ap.add_argument("--solve-precisions", default="fp64,fp32",
+                    help="precisions of the Krylov solve leg, in order")
+    ap.add_argument("--restart", type=int, default=30, help="GMRES restart length of the solve leg")
+    ap.add_argument("--solve-tmax", type=int, default=20000, help="product cap of the solve leg")
+    ap.add_argument("--solve-max-bytes", type=float, default=0.0,
+                    help="factor byte cap of the solve leg (0: half the device memory)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -577,6 +583,10 @@ def main():
         torch.cuda.synchronize()
         out["fp64_onthefly_sweep_ms"] = float(np.mean(s.ctx.timing_read(2)))
         s.ctx.timing(0)
+    del s, vdf
+    import gc
+    gc.collect()
+    torch.cuda.empty_cache()
     if not args.no_solve:
         out["solve"] = solve_times(pkg, args, work)
     if not args.no_cpu:
@@ -604,33 +614,51 @@ def calibrate_copy(nbytes, reps):
 
 
 def solve_times(pkg, args, work):
-    """Krylov solves of the workload at Kn_R in {0.01, 0.1, 1, 10}."""
+    """Krylov solves of the workload at each Kn_R of --solve-kn (Kn_N =
+    0.1 Kn_R), GMRES(--restart), in each precision of --solve-precisions;
+    each solve's result is also logged as it finishes."""
     out = {}
+    precs = [p.strip() for p in args.solve_precisions.split(",") if p.strip()]
+    max_bytes = args.solve_max_bytes if args.solve_max_bytes > 0 else None
     for tau_r in (float(v) for v in args.solve_kn.split(",")):
         res = {}
-        for prec in ("fp64", "fp32"):
+        for prec in precs:
             case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, tau_r, 0.1 * tau_r, prec,
-                             work, jitter=args.jitter)
+                             work, jitter=args.jitter, hole=args.hole, tmax=args.solve_tmax)
+            case.scheme.krylov_restart = args.restart
+
+            def progress(step, r, solver, prec=prec, tau_r=tau_r):
+                if step % 200 == 0:
+                    log(f"solve Kn_R {tau_r:g} {prec}: step {step}, residual {r:.3e}")
+
             t0 = time.perf_counter()
-            s = pkg.Solver(case)
+            s = pkg.Solver(case, max_bytes=max_bytes)
             t1 = time.perf_counter()
-            rec = s.run()
+            rec = s.run(callback=progress)
             t2 = time.perf_counter()
             tts = t2 - t0
             d = rec.diagnostics
             res[prec] = {"time_to_solution_s": tts, "setup_s": s.setup_seconds, "run_s": t2 - t1,
                          "iterations": rec.iterations, "converged": rec.converged,
                          "sweeps_fp64": d.get("sweeps_fp64"), "sweeps_fp32": d.get("sweeps_fp32"),
+                         "factor_mode": s.ctx.mode, "restart": args.restart,
                          "solves_per_hour": 3600.0 / tts, "temp": rec.temp,
                          "q": np.concatenate([rec.qx, rec.qy])}
-            del s
-        t64, t32 = res["fp64"].pop("temp"), res["fp32"].pop("temp")
-        q64, q32 = res["fp64"].pop("q"), res["fp32"].pop("q")
-        res["fp32"]["rel_l2_T_vs_fp64"] = float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))
-        res["fp32"]["rel_l2_q_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
-        res["speedup_fp32_refine_vs_fp64"] = (res["fp64"]["time_to_solution_s"] /
-                                              res["fp32"]["time_to_solution_s"])
-        res["speedup_run_only"] = res["fp64"]["run_s"] / res["fp32"]["run_s"]
+            log(f"solve Kn_R {tau_r:g} {prec}: " + json.dumps(
+                {k: v for k, v in res[prec].items() if k not in ("temp", "q")}))
+            del s, rec
+            import gc
+            import torch
+            gc.collect()
+            torch.cuda.empty_cache()
+        fields = {p: (res[p].pop("temp"), res[p].pop("q")) for p in precs}
+        if "fp64" in res and "fp32" in res:
+            (t64, q64), (t32, q32) = fields["fp64"], fields["fp32"]
+            res["fp32"]["rel_l2_T_vs_fp64"] = float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))
+            res["fp32"]["rel_l2_q_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
+            res["speedup_fp32_refine_vs_fp64"] = (res["fp64"]["time_to_solution_s"] /
+                                                  res["fp32"]["time_to_solution_s"])
+            res["speedup_run_only"] = res["fp64"]["run_s"] / res["fp32"]["run_s"]
         out[f"kn_r={tau_r:g}"] = res
     return out
 
