@@ -36,7 +36,7 @@ class PbteSetup(C.Structure):
                 ("bc_temp", _f64p), ("face_tri", _i32p), ("face_lfc", _i32p),
                 ("order", _i32p), ("max_bundle", C.c_int32), ("block", C.c_int32),
                 ("sweep_variant", C.c_int32), ("lds_slots_cap", C.c_int32),
-                ("lds_value_bytes", C.c_int32)]
+                ("lds_value_bytes", C.c_int32), ("pairs_or_fail", C.c_int32)]
 
 
 class PbteGsisData(C.Structure):

@@ -28,6 +28,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -2066,6 +2069,16 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         return fail("unsupported (ndof_tri, ndof_fc): degrees 1-4 only");
     if (N <= 0 || D <= 0) return fail("empty mesh or velocity set");
     pbte_ctx* ctx = new pbte_ctx();
+    // PBTE_SETUP_TIMES set: the wall time of each phase on stderr
+    const bool say = std::getenv("PBTE_SETUP_TIMES") != nullptr;
+    auto t_last = std::chrono::steady_clock::now();
+    auto phase = [&](const char* name) {
+        if (!say) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "pbte_ctx_create: %-24s %8.3f s\n", name,
+                     std::chrono::duration<double>(now - t_last).count());
+        t_last = now;
+    };
     try {
         ctx->dm = *dims;
         ctx->P = P; ctx->Q = Q; ctx->N = N; ctx->D = D; ctx->F = F; ctx->deg = Q - 1;
@@ -2104,6 +2117,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             ctx->nrm = ctx->upload(nrm.data(), nrm.size());
         }
 
+        phase("direction tables");
         // --- element data -----------------------------------------------------
         // CellGeo (factorisation), the compact face blocks and thermal wall
         // vectors (transport residual), the face lengths and the elements
@@ -2332,6 +2346,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->per_tri = ctx->upload(pt.data(), pt.size());
         if (N > kCellMask || F > kFaceValMask) throw std::string("mesh too large for the record fields");
 
+        phase("element data");
         // --- bundles: directions with identical inflow signs ------------------
         std::vector<uint64_t> hsh(D);
         std::vector<std::vector<uint8_t>> sig(D);
@@ -2392,6 +2407,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->nb = nb;
         int bm = 1;
         for (auto& bb : bundles) bm = std::max(bm, (int)bb.size());
+        phase("bundles");
         // inflow wall elements per bundle (the same for every direction of it)
         {
             int most = 0;
@@ -2454,6 +2470,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 ctx->pool_slots = std::max(pool, 1);
             }
         }
+        phase("wall lists, LDS budget");
         // levels per bundle (all directions of a bundle share them)
         std::vector<int32_t> h_slot((size_t)nb * N), h_lev_ptr;
         std::vector<int32_t> h_ndir(nb), h_dir_off(nb + 1, 0), h_dirs;
@@ -2509,8 +2526,13 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 if (ctx->items % (int)bb.size()) ctx->block = ctx->items = 256;
         }
         std::atomic_int next{0};
+        // a paired schedule asked to fail on overflow (pairs_or_fail) stops
+        // at the first bundle whose pool overflows: the caller builds the
+        // one-direction schedule instead
+        std::atomic<bool> pair_overflow{false};
         auto work = [&]() {
             for (int k = next++; k < nb; k = next++) {
+                if (pair_overflow.load(std::memory_order_relaxed)) break;
                 const int d = bundles[k][0];
                 const int32_t* order = su->order + (size_t)d * N;
                 const auto& cnt = lcnt[k];
@@ -2601,6 +2623,10 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                     slots_used[k] = overflow ? 0 : used;
                 }
                 global_flag[k] = overflow ? 1 : 0;
+                if (overflow && su->pairs_or_fail && bundles[k].size() > 1) {
+                    pair_overflow = true;
+                    break;
+                }
                 for (int q = 0; q < N; ++q) {
                     int w[3];
                     for (int il = 0; il < 3; ++il) {
@@ -2620,6 +2646,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             }
         };
         run_threads(work);
+        if (pair_overflow) throw std::string("paired bundles overflow the LDS trace pool (pairs_or_fail)");
         // the pool shrinks to the slots the bundles that kept it use (every
         // slot id is below that count; the zero and spill slots follow it)
         ctx->pool_slots = (int)std::max<int64_t>(1, *std::max_element(slots_used.begin(), slots_used.end()));
@@ -2659,6 +2686,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             h_lev_se[2 * k] = h_lev_off[k];
             h_lev_se[2 * k + 1] = h_lev_off[k] + (int32_t)lptr[k].size() - 1;
         }
+        phase("levels, records, pools");
         // XCD-aware block -> bundle map: blocks are dispatched round robin over
         // 8 XCDs; give each XCD a contiguous range of bundles sorted by the
         // in-plane angle so its L2 sees similar sweep fronts.
@@ -2690,15 +2718,25 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         {
             // per item (bundle, slot, direction of the bundle): {cell | pool
             // slot, face words}
-            std::vector<int4> h_irec((size_t)h_item_off[nb]);
-            for (int k = 0; k < nb; ++k) {
-                const int B = (int)bundles[k].size();
-                for (int q = 0; q < N; ++q)
-                    for (int b = 0; b < B; ++b)
-                        h_irec[(size_t)h_item_off[k] + (size_t)q * B + b] = h_rec[(size_t)k * N + q];
+            phase("schedule upload");
+            if (ctx->max_bundle_dirs == 1) {
+                // one direction per bundle: the items are the bundles' slots
+                ctx->irec = ctx->upload(h_rec.data(), h_rec.size());
+            } else {
+                std::vector<int4> h_irec((size_t)h_item_off[nb]);
+                std::atomic<int> nk{0};
+                run_threads([&]() {
+                    for (int k = nk++; k < nb; k = nk++) {
+                        const int B = (int)bundles[k].size();
+                        for (int q = 0; q < N; ++q)
+                            for (int b = 0; b < B; ++b)
+                                h_irec[(size_t)h_item_off[k] + (size_t)q * B + b] = h_rec[(size_t)k * N + q];
+                    }
+                });
+                ctx->irec = ctx->upload(h_irec.data(), h_irec.size());
             }
-            ctx->irec = ctx->upload(h_irec.data(), h_irec.size());
         }
+        phase("item records");
         ctx->bundle_global = ctx->upload(global_flag.data(), global_flag.size());
         for (int k = 0; k < nb; ++k) {
             ctx->global_bundles += global_flag[k];
@@ -2722,6 +2760,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->counter = ctx->alloc<int32_t>(16);
         ctx->spill_row = ctx->alloc<double>(16);
         HIP_TRY(hipDeviceSynchronize());
+        phase("scratch");
     } catch (const std::string& e) {
         delete ctx;
         return fail(e);

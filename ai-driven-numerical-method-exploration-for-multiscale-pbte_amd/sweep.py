@@ -132,6 +132,7 @@ class SweepContext:
         h = None
         for k, mb in enumerate(tries):
             su.max_bundle = mb
+            su.pairs_or_fail = 1 if k < len(tries) - 1 else 0
             h = _native.lib().pbte_ctx_create(C.byref(dims), C.byref(su), C.c_void_p(stream_handle()),
                                               err, 512)
             last = k == len(tries) - 1

@@ -79,6 +79,14 @@ typedef struct pbte_setup {
                                              exceeds the LDS runs the plain
                                              level kernel); 0 or 8: for
                                              binary64 sweeps              */
+    int32_t pairs_or_fail;                /* 1: a schedule that pairs
+                                             directions fails (NULL) when a
+                                             bundle's trace pool would
+                                             overflow the LDS, instead of
+                                             sweeping that bundle through
+                                             global memory; the caller
+                                             then builds one direction per
+                                             workgroup                     */
 } pbte_setup;
 
 /* ---- setup ------------------------------------------------------------ */
