@@ -40,6 +40,7 @@
 #include <queue>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "pbte_hip.h"
@@ -200,6 +201,9 @@ struct pbte_ctx {
     double* lu64 = nullptr;
     float* lu32 = nullptr;
     bool have64 = false, have32 = false;
+    // matrix-free element operators (the refined binary64 sweep)
+    double* refop = nullptr;    // [6][P][P]
+    double* cellop = nullptr;   // [N][8]
     // scratch
     double* src64 = nullptr;   // source of the transport residual
     double* ssraw = nullptr;
@@ -492,6 +496,13 @@ template <typename T> struct SweepArgs {
     const double *mass, *gx, *gy, *fcm, *cxv64, *cyv64;
     const CellGeo<double>* geo;
     double inv_tau_c;
+    // the refined binary64 sweep: reference-element matrices [6][P][P]
+    // (mass, d/dxi, d/deta, the three edge matrices), per-cell scalars
+    // [N][8] (twice the area, the four gradient coefficients, the three
+    // edge lengths as the edge tensors scale them), the stored binary32
+    // inverse of the same schedule
+    const double *refop, *cellop;
+    const float* lu32;
 };
 
 // Face word of an item record: code in bits 28-31, the local face of the
@@ -692,6 +703,147 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
 }
 
 
+// the right-hand side of item (slot s, direction b of the bundle) of the
+// plain level kernels: source, summed wall inflow, upwind faces, exactly as
+// the other sweep kernels assemble it
+template <int P, typename T>
+__device__ __forceinline__ int otf_rhs(const SweepArgs<T>& a, const int4* irec, const int4* wc, int s, int B, int b,
+                                       int d, T cx, T cy, T thermal_on, const T* vdf_d, T (&x)[P])
+{
+    using L = Lay<P>;
+    constexpr int DEG = Deg<P>::v;
+    constexpr int Q = DEG + 1;
+    constexpr int TQ = Tr<P>::TQ;
+    const int4 r = irec[(size_t)s * B + b];
+    const int i = rec_cell(r.x);
+    const int fw[3] = {r.y, r.z, r.w};
+    const T* cr = a.crec + (size_t)i * a.rs;
+    int wk = -1;
+#pragma unroll
+    for (int il = 0; il < 3; ++il)
+        if (code_in(kRecWall, fw_code(fw[il]))) wk = fw_val(fw[il]);
+    if (wk >= 0) {
+        const int4 e = wc[wk];
+#pragma unroll
+        for (int m = 0; m < P; ++m)
+            x[m] = source_plus<P, T>(cr, cx, cy, m, wall_inflow<P, T>(cr, e, cx, cy, a.wtr, thermal_on, m));
+    } else {
+#pragma unroll
+        for (int m = 0; m < P; ++m) x[m] = source_plus<P, T>(cr, cx, cy, m, T(0));
+    }
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        const int code = fw_code(fw[il]);
+        if (!code_in(kRecNeighbour, code)) continue;
+        const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
+        const T w = T(0.5) * (sp - fabs(sp));
+        const bool per = code_in(kRecPeriodic, code);
+        const T* fe = (per && a.use_pbuf)
+                          ? a.pbuf + ((size_t)d * a.F + (bits_of(cr[L::R_INFO + il]) >> 4)) * P
+                          : vdf_d + (size_t)a.other[(size_t)i * 3 + il] * P;
+        const int ilp = fw_ilp(fw[il]);
+        T u[TQ];
+#pragma unroll
+        for (int q = 0; q < TQ; ++q) u[q] = q < Q ? fe[face_node(DEG, ilp, q < Q ? q : 0)] : T(0);
+        face_update<P, T>(x, il, w, a.kref, u);
+    }
+    return i;
+}
+
+// the binary64 sweep of the "mixed" factor mode (binary32 inverses stored,
+// binary64 ones not): per item, x = inv32 b refined twice by
+// x += inv32 (b - A x), with A x formed in binary64 without A: A is
+// area2 (inv_tau_c M - (cx ax + cy ay) Dx - (cx bx + cy by) Dy)
+// + sum_faces w len R_face over the reference-element matrices in LDS
+// (_kernels.py:106-125 assembles the same operator from the element
+// tensors, which are these products).  The inverse's binary32 error
+// (about cond(A) 6e-8) shrinks by that factor per refinement: after two,
+// x is A^-1 b to binary64 rounding, with no factorisation and no P x P
+// matrix in registers
+template <int P>
+__global__ void __launch_bounds__(256) sweep_refine_kernel(SweepArgs<double> a)
+{
+    using T = double;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    double* ref = reinterpret_cast<double*>(lds_raw);            // [6][P][P]
+    const int bundle = a.block_bundle[blockIdx.x];
+    const int B = a.bundle_ndir[bundle];
+    const int t = threadIdx.x;
+    const int b = t % B;
+    const int c = t / B;
+    const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
+    const T cx = a.cxv[d], cy = a.cyv[d];
+    const int N = a.N;
+    const int4* irec = a.irec + a.item_off[bundle];
+    const int4* wc = a.wcell + a.wc_off[bundle];
+    T* vdf_d = a.vdf + (size_t)d * N * P;
+    const T thermal_on = a.homogeneous ? T(0) : T(1);
+    const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
+    const float* inv = a.lu32 + a.lu_off[bundle];
+    const size_t NB = (size_t)N * B;
+    for (int k = t; k < 6 * P * P; k += blockDim.x) ref[k] = a.refop[k];
+    __syncthreads();
+
+    for (int lv = l0; lv < l1; ++lv) {
+        const int s = a.lev_ptr[lv] + c;
+        if (s < a.lev_ptr[lv + 1]) {
+            T rhs[P];
+            const int i = otf_rhs<P, T>(a, irec, wc, s, B, b, d, cx, cy, thermal_on, vdf_d, rhs);
+            const size_t item = (size_t)s * B + b;
+            float iv[P * P];
+#pragma unroll
+            for (int k = 0; k < P * P; ++k) iv[k] = inv[lu_idx<float>(k, item, NB)];
+            const double* co = a.cellop + (size_t)i * 8;
+            const CellGeo<double> g = a.geo[i];
+            double sk[6];
+            sk[0] = co[0] * a.inv_tau_c;
+            sk[1] = -co[0] * (cx * co[1] + cy * co[3]);
+            sk[2] = -co[0] * (cx * co[2] + cy * co[4]);
+#pragma unroll
+            for (int il = 0; il < 3; ++il) {
+                const double sp = cx * g.n[2 * il] + cy * g.n[2 * il + 1];
+                sk[3 + il] = 0.5 * (sp + fabs(sp)) * co[5 + il];
+            }
+            double x[P];
+#pragma unroll
+            for (int r = 0; r < P; ++r) {
+                double v = 0.0;
+#pragma unroll
+                for (int q = 0; q < P; ++q) v += (double)iv[r * P + q] * rhs[q];
+                x[r] = v;
+            }
+#pragma unroll 1
+            for (int it = 0; it < 2; ++it) {
+                double res[P];
+#pragma unroll
+                for (int m = 0; m < P; ++m) {
+                    double ax = 0.0;
+#pragma unroll
+                    for (int l = 0; l < P; ++l) {
+                        const int e = m * P + l;
+                        const double am = sk[0] * ref[e] + sk[1] * ref[P * P + e] + sk[2] * ref[2 * P * P + e] +
+                                          sk[3] * ref[3 * P * P + e] + sk[4] * ref[4 * P * P + e] +
+                                          sk[5] * ref[5 * P * P + e];
+                        ax += am * x[l];
+                    }
+                    res[m] = rhs[m] - ax;
+                }
+#pragma unroll
+                for (int r = 0; r < P; ++r) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int q = 0; q < P; ++q) v += (double)iv[r * P + q] * res[q];
+                    x[r] += v;
+                }
+            }
+            T* o = vdf_d + (size_t)i * P;
+#pragma unroll
+            for (int m = 0; m < P; ++m) o[m] = x[m];
+        }
+        __syncthreads();
+    }
+}
+
 // the on-the-fly sweep (_kernels.py:242-269): the plain level schedule
 // with every element operator built and factorised in binary64 where it is
 // used (element_lu), so no factors are stored; the inflow right-hand side
@@ -700,10 +852,6 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
 template <int P, typename T>
 __global__ void __launch_bounds__(256) sweep_otf_kernel(SweepArgs<T> a)
 {
-    using L = Lay<P>;
-    constexpr int DEG = Deg<P>::v;
-    constexpr int Q = DEG + 1;
-    constexpr int TQ = Tr<P>::TQ;
     const int bundle = a.block_bundle[blockIdx.x];
     const int B = a.bundle_ndir[bundle];
     const int t = threadIdx.x;
@@ -722,43 +870,11 @@ __global__ void __launch_bounds__(256) sweep_otf_kernel(SweepArgs<T> a)
     for (int lv = l0; lv < l1; ++lv) {
         const int s = a.lev_ptr[lv] + c;
         if (s < a.lev_ptr[lv + 1]) {
-            const int4 r = irec[(size_t)s * B + b];
-            const int i = rec_cell(r.x);
-            const int fw[3] = {r.y, r.z, r.w};
-            const T* cr = a.crec + (size_t)i * a.rs;
+            T x[P];
+            const int i = otf_rhs<P, T>(a, irec, wc, s, B, b, d, cx, cy, thermal_on, vdf_d, x);
             double A[P][P];
             uint64_t pv;
             (void)element_lu<P>(a.mass, a.gx, a.gy, a.fcm, a.geo[i], a.inv_tau_c, N, i, cx64, cy64, A, pv);
-            int wk = -1;
-#pragma unroll
-            for (int il = 0; il < 3; ++il)
-                if (code_in(kRecWall, fw_code(fw[il]))) wk = fw_val(fw[il]);
-            T x[P];
-            if (wk >= 0) {
-                const int4 e = wc[wk];
-#pragma unroll
-                for (int m = 0; m < P; ++m)
-                    x[m] = source_plus<P, T>(cr, cx, cy, m, wall_inflow<P, T>(cr, e, cx, cy, a.wtr, thermal_on, m));
-            } else {
-#pragma unroll
-                for (int m = 0; m < P; ++m) x[m] = source_plus<P, T>(cr, cx, cy, m, T(0));
-            }
-#pragma unroll
-            for (int il = 0; il < 3; ++il) {
-                const int code = fw_code(fw[il]);
-                if (!code_in(kRecNeighbour, code)) continue;
-                const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
-                const T w = T(0.5) * (sp - fabs(sp));
-                const bool per = code_in(kRecPeriodic, code);
-                const T* fe = (per && a.use_pbuf)
-                                  ? a.pbuf + ((size_t)d * a.F + (bits_of(cr[L::R_INFO + il]) >> 4)) * P
-                                  : vdf_d + (size_t)a.other[(size_t)i * 3 + il] * P;
-                const int ilp = fw_ilp(fw[il]);
-                T u[TQ];
-#pragma unroll
-                for (int q = 0; q < TQ; ++q) u[q] = q < Q ? fe[face_node(DEG, ilp, q < Q ? q : 0)] : T(0);
-                face_update<P, T>(x, il, w, a.kref, u);
-            }
             double v[P];
 #pragma unroll
             for (int m = 0; m < P; ++m) v[m] = (double)x[m];
@@ -2890,6 +3006,7 @@ SweepArgs<T> sweep_args(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int 
     for (int k = 0; k < ctx->Q * ctx->Q; ++k) a.kref[k] = S::kref(ctx)[k];
     a.mass = ctx->mass; a.gx = ctx->gx; a.gy = ctx->gy; a.fcm = ctx->fcm; a.geo = ctx->geo64;
     a.cxv64 = ctx->cxv; a.cyv64 = ctx->cyv; a.inv_tau_c = 1.0 / ctx->tau_c;
+    a.refop = ctx->refop; a.cellop = ctx->cellop; a.lu32 = ctx->lu32;
     return a;
 }
 
@@ -2906,7 +3023,18 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
     const bool timed = 2 * (size_t)ctx->ev_used + 1 < ctx->events.size();
     if (timed) HIP_TRY(hipEventRecord(ctx->events[2 * ctx->ev_used], ctx->stream));
     if (onthefly) {
-        hipLaunchKernelGGL((sweep_otf_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
+        bool refined = false;
+        if constexpr (std::is_same<T, double>::value && P <= 10) {
+            // binary32 inverses stored (the "mixed" mode): refine from them
+            // instead of building and factorising every operator
+            if (ctx->have32 && ctx->refop) {
+                hipLaunchKernelGGL((sweep_refine_kernel<P>), dim3(ctx->nb), dim3(ctx->block),
+                                   6 * P * P * sizeof(double), ctx->stream, a);
+                refined = true;
+            }
+        }
+        if (!refined)
+            hipLaunchKernelGGL((sweep_otf_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
     } else if (!runs_pipelined<P, T>(ctx)) {
         // the plain level kernel: asked for, or a binary64 sweep whose pool,
         // sized for binary32, exceeds a workgroup's LDS
@@ -3027,6 +3155,19 @@ extern "C" int pbte_sweep(pbte_ctx* ctx, int32_t precision, const double* ts, co
         } else {
             throw std::string("bad precision");
         }
+        return 0;
+    });
+}
+
+extern "C" int pbte_set_cell_operators(pbte_ctx* ctx, const double* ref, const double* cell)
+{
+    return guarded(ctx, [&]() {
+        if (!ref || !cell) throw std::string("null operator arrays");
+        const size_t nr = (size_t)6 * ctx->P * ctx->P, nc = (size_t)ctx->N * 8;
+        if (!ctx->refop) ctx->refop = ctx->alloc<double>(nr);
+        if (!ctx->cellop) ctx->cellop = ctx->alloc<double>(nc);
+        HIP_TRY(hipMemcpy(ctx->refop, ref, nr * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ctx->cellop, cell, nc * sizeof(double), hipMemcpyHostToDevice));
         return 0;
     });
 }

@@ -122,6 +122,11 @@ class Integrals:
     nodfun_qua_p: np.ndarray
     ref_tri: np.ndarray
     ref_tri_tri: np.ndarray
+    # the element operators' factors: [6][P][P] reference matrices (mass,
+    # d/dxi, d/deta, edges 1-3) and [N][8] cell scalars (2 area, gradient
+    # coefficients, edge lengths as the edge tensors scale them)
+    ref_ops: np.ndarray | None = None
+    cell_ops: np.ndarray | None = None
 
 
 def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
@@ -311,4 +316,8 @@ def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
                      int_tri_fc=int_tri_fc, int_tri_fc_fc=int_tri_fc_fc,
                      int_fc_fc=int_fc_fc, int_fc=int_fc, int_tri_tri_fc=int_tri_tri_fc,
                      qua_abs_x=qax, qua_abs_y=qay, qua_wei=qw, nodfun_qua_p=nodfun_qua_p,
-                     ref_tri=ref_tri, ref_tri_tri=ref_tri_tri)
+                     ref_tri=ref_tri, ref_tri_tri=ref_tri_tri,
+                     ref_ops=np.ascontiguousarray(np.stack([ref_tri_tri, ref_dx, ref_dy, r1, r2, r3])),
+                     cell_ops=np.ascontiguousarray(np.stack(
+                         [area2, (y3 - y1) / ss, (y1 - y2) / ss, (x1 - x3) / ss, (x2 - x1) / ss,
+                          len12, len23 / sqrt2, len31], axis=1)))

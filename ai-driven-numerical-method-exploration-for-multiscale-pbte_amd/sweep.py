@@ -149,6 +149,9 @@ class SweepContext:
                     continue
             break
         self.handle = h
+        if getattr(integrals, "ref_ops", None) is not None:
+            _native.check(h, _native.lib().pbte_set_cell_operators(
+                h, f64(integrals.ref_ops), f64(integrals.cell_ops)), "set_cell_operators")
         # every bundle one direction: the sweep runs the kernel instance
         # without bundle arithmetic (a batched launch runs one instance)
         self.one_dir = self.schedule_info()["bundles"] == self.ndir

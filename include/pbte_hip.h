@@ -192,6 +192,18 @@ int pbte_residual_iterate_async(pbte_ctx *ctx, const double *temp, const double 
 int pbte_periodic_gather(pbte_ctx *ctx, int32_t precision, const void *vdf, double *out);
 int pbte_periodic_scatter(pbte_ctx *ctx, int32_t precision, const double *in, void *pbuf);
 
+/* The element operators in matrix-free form, for the binary64 sweeps of
+ * the "mixed" mode (binary32 inverses stored, binary64 ones not; the
+ * reference's sweep_onthefly, _kernels.py:242-269, builds and factorises
+ * them instead): ref [6][P][P] = the reference-element mass matrix, the
+ * two reference derivative matrices and the three edge matrices of
+ * integration.py:199-260; cell [N][8] = twice the area, the gradient
+ * coefficients (y3-y1)/s, (y1-y2)/s, (x1-x3)/s, (x2-x1)/s, and the edge
+ * lengths len12, len23/sqrt(2), len31.  With both set and binary32 factors
+ * built, pbte_sweep_onthefly in binary64 applies inv32 refined twice
+ * against A formed from these. */
+int pbte_set_cell_operators(pbte_ctx *ctx, const double *ref, const double *cell);
+
 /* ---- batched sweeps (no counterpart in the reference) ------------------
  * The sweeps of several independent problems in one launch of the
  * pipelined sweep kernel: every workgroup finds its context and bundle in a

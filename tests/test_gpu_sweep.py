@@ -255,6 +255,28 @@ def test_onthefly_sweep_matches_oracle(pkg, tag, deg):
     assert _rel(s.vdf, o.vdf) <= 1e-12
 
 
+@pytest.mark.parametrize("tag,deg", [("F1_1_1", 3), ("F2_1_1", 2), ("F3_1_1", 1)])
+def test_mixed_mode_refined_sweep_matches_oracle(pkg, tag, deg):
+    """The "mixed" factor mode (fp32 run whose binary64 factors do not fit
+    beside the binary32 ones): the binary64 sweep refines the stored
+    binary32 inverse against the matrix-free operator; thermalising,
+    adiabatic and periodic walls, degrees 1-3, against the oracle."""
+    import torch
+    c = _case(pkg, tag, performance__precision="fp32", dg__deg=deg)
+    from pbte_amd.problem import build_problem
+    pb = build_problem(c)
+    nbytes = float(pb.vel.ndir) * pb.mesh.n_tris * c.ndof_tri ** 2 * 8.0
+    s, o = _pair(pkg, c, max_bytes=nbytes)
+    assert s.ctx.mode == "mixed"
+    _seed_state(s, o, seed=17)
+    # an fp32 solver's vdf is its binary32 tensor: seed the binary64 one
+    # too (the periodic mesh's broken cycles read the previous sweep)
+    s.vdf_d.copy_(torch.from_numpy(o.vdf))
+    s.ctx.sweep(s.mom, s.vdf_d)
+    o.sweep()
+    assert _rel(s.vdf_d.cpu().numpy(), o.vdf) <= 1e-12
+
+
 def test_onthefly_sweep_fp32(pkg):
     c = _case(pkg, "F2_1_1", performance__precompute_inverse=False, performance__precision="fp32")
     s, o = _pair(pkg, c)
@@ -299,13 +321,15 @@ def _hole_case(pkg, path, n=20, deg=3):
     }, basedir=str(path.parent))
 
 
-def test_hole_mesh_mixed_factor_mode_matches_oracle(pkg, tmp_path):
+@pytest.mark.parametrize("deg", [1, 2, 3])
+def test_hole_mesh_mixed_factor_mode_matches_oracle(pkg, tmp_path, deg):
     """The square with a central hot hole (the 362^2 P3 1024-ordinate
-    layout, here 20^2 P3 with 32 ordinates) in the "mixed" factor mode
-    that layout needs: binary32 factors stored, binary64 operators built
-    and factorised on the fly.  Both sweeps against the oracle."""
+    layout, here 20^2 with 32 ordinates) in the "mixed" factor mode that
+    layout needs: binary32 factors stored, the binary64 sweep refining
+    the binary32 inverse against the matrix-free binary64 operator.  Both
+    sweeps against the oracle (which factorises every operator)."""
     import torch
-    c = _hole_case(pkg, tmp_path / "hole.msh")
+    c = _hole_case(pkg, tmp_path / f"hole{deg}.msh", deg=deg)
     from pbte_amd.problem import build_problem
     pb = build_problem(c)
     nbytes = float(pb.vel.ndir) * pb.mesh.n_tris * c.ndof_tri ** 2 * 8.0
@@ -313,6 +337,7 @@ def test_hole_mesh_mixed_factor_mode_matches_oracle(pkg, tmp_path):
     assert s.ctx.mode == "mixed"
     assert int(np.count_nonzero(np.asarray(pb.bcdata.bc_type) >= 0)) >= 5
     _seed_state(s, o, seed=21)
+    s.vdf_d.copy_(torch.from_numpy(o.vdf))
     s.ctx.sweep(s.mom, s.vdf_d)
     o.sweep()
     assert _rel(s.vdf_d.cpu().numpy(), o.vdf) <= 1e-12
