@@ -152,14 +152,19 @@ def gsis_workload(pkg, args, work):
     for prec in ("fp64", "fp32"):
         case = gsis_case(pkg, n, args.deg, args.npole, args.nazim, 1e-3, 1e-4, prec, work,
                          jitter=args.jitter if args.jitter else 0.25)
-        print(f"gsis {prec}: setup", flush=True)
+        log(f"gsis {prec}: setup")
         t0 = time.perf_counter()
         s = pkg.Solver(case)
         setup = time.perf_counter() - t0
         s.ctx.timing(case.iteration.tmax)
-        print(f"gsis {prec}: run", flush=True)
+        log(f"gsis {prec}: run")
         t1 = time.perf_counter()
-        rec = s.run(progress_every=10)
+
+        def progress(k, r, _s, prec=prec):
+            if k % 50 == 0:
+                log(f"gsis {prec}: step {k}, residual {r:.3e}")
+
+        rec = s.run(callback=progress)
         torch.cuda.synchronize()
         run = time.perf_counter() - t1
         ms = s.ctx.timing_read(case.iteration.tmax)
