@@ -408,12 +408,14 @@ def batch_solves(pkg, args, work, draws):
     log(f"batch: building {len(draws)} GSIS problems (meshes / {div} a side)")
     t0 = time.perf_counter()
     solvers = []
-    for d in draws:
+    for k, d in enumerate(draws):
         npole, nazim = BATCH_DIRS[d["ndir"]]
         case = make_case(pkg, max(4, d["n"] // div), d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp32",
                          work, jitter=0.25, seed=1, method="source", accflag=1, tmax=args.batch_tmax)
         case.performance.autotune = args.autotune
         solvers.append(pkg.Solver(case))
+        if k % 8 == 7:
+            log(f"batch: {k + 1} GSIS problems set up ({time.perf_counter() - t0:.1f} s)")
     setup = time.perf_counter() - t0
     log(f"batch: {len(solvers)} GSIS problems set up in {setup:.1f} s; solving")
     def progress(k, step, res, s):
