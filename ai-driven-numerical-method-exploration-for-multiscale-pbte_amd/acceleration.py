@@ -33,10 +33,32 @@ __all__ = ["Acceleration"]
 
 # trace unknowns up to which the reference's ordering (COLAMD) is kept: every
 # fixture and FROZEN case (at most 29 520 unknowns, 20 x 40 at degree 3);
-# beyond, the minimum-degree ordering of K + K^T (64^2 channel, 111 744
-# unknowns: 2.7x less fill, solves 2.1x and factorisation 3.2x faster, the
-# same solution to 3e-14)
+# beyond, the minimum-degree ordering of K + K^T with diagonal pivots
+# (factorise_trace; 64^2 channel, 111 744 unknowns: 2.7x less fill, solves
+# 2.1x and factorisation 3.4x faster, the same solution to 3e-14)
 _COLAMD_MAX = 40_000
+
+
+def factorise_trace(K):
+    """SuperLU factors of the trace matrix and the ordering used.  Up to
+    _COLAMD_MAX unknowns the reference's call (``global_solve.py:120-124``,
+    column ordering COLAMD, partial pivoting).  Beyond, the minimum-degree
+    ordering of K + K^T with diagonal pivots (SuperLU's symmetric mode,
+    no row interchanges): its fill is fixed by the ordering, while partial
+    pivoting under that ordering can run away (a 55 485-unknown trace matrix
+    of the batched workload did not factorise in minutes).  The diagonal
+    factors are checked on a seeded right-hand side; a relative residual
+    above 1e-10 falls back to the reference's call."""
+    from scipy.sparse.linalg import splu
+    Kc = K.tocsc()
+    if K.shape[0] > _COLAMD_MAX:
+        lu = splu(Kc, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0, options={"SymmetricMode": True})
+        b = np.random.default_rng(0).standard_normal(K.shape[0])
+        x = lu.solve(b)
+        res = np.abs(Kc @ x - b).max() / np.abs(b).max()
+        if np.isfinite(res) and res <= 1e-10:
+            return lu, "MMD_AT_PLUS_A"
+    return splu(Kc, permc_spec="COLAMD"), "COLAMD"
 
 
 class Acceleration:
@@ -61,11 +83,7 @@ class Acceleration:
         self.K, self.nnz = assemble_global_matrix(mesh, ig, ba_sol, aa_trace, self.nf, bc_type,
                                                   drop_tol=drop,
                                                   periodic_flip=s.bcdata.periodic_flip)
-        from scipy.sparse.linalg import splu
-        # the reference's SuperLU call (column ordering COLAMD); for larger
-        # trace systems the minimum-degree ordering of K + K^T
-        self.trace_ordering = "COLAMD" if self.K.shape[0] <= _COLAMD_MAX else "MMD_AT_PLUS_A"
-        self._lu = splu(self.K.tocsc(), permc_spec=self.trace_ordering)
+        self._lu, self.trace_ordering = factorise_trace(self.K)
         self.solve_count = 0
 
         t1, l1, t2, l2, periodic = face_sides(mesh, bc_type)
