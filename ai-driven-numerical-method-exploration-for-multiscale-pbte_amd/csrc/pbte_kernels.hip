@@ -394,26 +394,23 @@ __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
         if (element_lu<P>(a.mass, a.gx, a.gy, a.fcm, g, a.inv_tau_c, N, i, a.cxv[d], a.cyv[d], A, pv))
             atomicAdd(a.singular, 1);
         // the inverse, column by column from the pivoted factors (DGETRS on
-        // the unit vectors): the sweep applies x = inv(A) b
-        double Ai[P][P];
-#pragma unroll
+        // the unit vectors), each column stored as it is solved (no P x P
+        // array beside the factors: it spilled at P = 10): the sweep
+        // applies x = inv(A) b
+        const int64_t base = a.lu_off[bundle];
+        const size_t item = (size_t)s * B + b, NB = (size_t)N * B;
+#pragma unroll 1
         for (int j = 0; j < P; ++j) {
             double v[P];
 #pragma unroll
             for (int k = 0; k < P; ++k) v[k] = k == j ? 1.0 : 0.0;
             element_lu_solve<P>(A, pv, v);
 #pragma unroll
-            for (int r = 0; r < P; ++r) Ai[r][j] = v[r];
-        }
-        const int64_t base = a.lu_off[bundle];
-        const size_t item = (size_t)s * B + b, NB = (size_t)N * B;
-#pragma unroll
-        for (int r = 0; r < P; ++r)
-#pragma unroll
-            for (int c = 0; c < P; ++c) {
-                if (a.lu64) a.lu64[base + lu_idx<double>(r * P + c, item, NB)] = Ai[r][c];
-                if (a.lu32) a.lu32[base + lu_idx<float>(r * P + c, item, NB)] = (float)Ai[r][c];
+            for (int r = 0; r < P; ++r) {
+                if (a.lu64) a.lu64[base + lu_idx<double>(r * P + j, item, NB)] = v[r];
+                if (a.lu32) a.lu32[base + lu_idx<float>(r * P + j, item, NB)] = (float)v[r];
             }
+        }
     }
 }
 
