@@ -451,19 +451,18 @@ def _run_fp64(solver, op, callback, progress_every):
                                  "sweeps_discarded": op.discarded})
 
 
-def _run_mixed(solver, op, callback, progress_every, inner_reduction: float | None = None,
+def _run_mixed(solver, op, callback, progress_every, inner_reduction: float = 1.0e-2,
                max_refine: int = 64, stall_products: int = 300):
     """FP64 defect correction around FP32-sweep GMRES (module docstring).
 
-    ``inner_reduction`` (default by factor mode): each inner solve reduces
-    the defect by this factor, 1e-2 when the binary64 factors are stored
-    (a binary64 refinement sweep costs about two binary32 products), 1e-4
-    in the "mixed" mode, where every binary64 sweep builds and factorises
-    its operators (config 4: 0.81 s against 27 ms for a binary32 product),
-    so fewer refinements pay for longer inner solves."""
+    ``inner_reduction``: each inner solve reduces the defect by this
+    factor.  Two digits: a binary64 refinement sweep costs two binary32
+    products with stored binary64 factors and about four when they are
+    refined from the binary32 inverse (the "mixed" mode), so deeper inner
+    solves do not pay (on a 362^2 P3 square with a hole and 1 024
+    ordinates, inner solves to 1e-4 took 36 more binary32 products to save
+    two binary64 sweeps)."""
     c = solver.case
-    if inner_reduction is None:
-        inner_reduction = 1.0e-4 if solver.ctx.mode == "mixed" else 1.0e-2
     tol, tmax = c.iteration.tol, c.iteration.tmax
     ctx = solver.ctx
     h = _History(solver, callback, progress_every, c.iteration.true_residual)

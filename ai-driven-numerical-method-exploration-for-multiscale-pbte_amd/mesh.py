@@ -17,7 +17,7 @@ The writers follow the reference's mesh family convention
 (``id = i*(n+1) + j + 1``), boundary lines south (+x), north (-x), east (+y),
 west (-y) with physical ids 11..14, and per-quad diagonals chosen by
 ``(2i >= n) == (2j >= n)`` (on ``j`` only for the periodic channel).  The
-square with a circular hole (physical id 16) is this build's own: the
+square with a circular hole (physical id 16) is not in the reference: the
 reference has no mesher for it.
 """
 from __future__ import annotations
