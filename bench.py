@@ -313,7 +313,7 @@ def batch_workload(pkg, args, work):
         case = make_case(pkg, d["n"], d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp32", work,
                          jitter=0.25, seed=1, method="source")
         case.performance.autotune = args.autotune
-        s = pkg.Solver(case)
+        s = pkg.Solver(case, max_bundle=args.bundle)
         P, N = s.ndof, s.n_tris
         s.mom.set_dofs(rng.random((P, N)), 0.1 * rng.random((P, N)), 0.1 * rng.random((P, N)))
         solvers.append(s)
