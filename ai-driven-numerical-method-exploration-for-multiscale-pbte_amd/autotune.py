@@ -68,11 +68,13 @@ def _store(path: str, key: str, entry: dict) -> None:
     os.replace(tmp, path)
 
 
-def tune(case, problem, candidates=CANDIDATES, reps: int = 4, max_bytes: float | None = None) -> dict:
+def tune(case, problem, candidates=CANDIDATES, reps: int = 4, max_bytes: float | None = None) -> dict | None:
     """The fastest tile shape of the case's sweep: ``{"max_bundle",
-    "block", "variant", "ms", "timed": [...]}``.  Candidates that do not
-    build, that fall back to global-memory bundles or to another kernel,
-    or whose bundles would not pair, are skipped."""
+    "block", "variant", "ms", "timed": [...]}``, or None when no candidate
+    can be timed (the caller keeps the default schedule).  Candidates that
+    do not build, that store no factors of the run's precision, that fall
+    back to global-memory bundles, or whose bundles would not pair, are
+    skipped."""
     from .moments import Moments
     from .sweep import SweepContext
     key = signature(case, problem)
@@ -101,7 +103,10 @@ def tune(case, problem, candidates=CANDIDATES, reps: int = 4, max_bytes: float |
             continue
         try:
             info = ctx.schedule_info()
-            if ctx.mode != "precomputed" or info["global_bundles"] > 0:
+            # the timed sweeps need stored factors of the run's precision
+            # (binary32 ones are stored in the "mixed" mode too)
+            stored = ctx.mode == "precomputed" or (fp32 and ctx.mode == "mixed")
+            if not stored or info["global_bundles"] > 0:
                 continue
             if mb > 1 and info["bundles"] == D:
                 continue                        # nothing paired: the same as one direction
@@ -120,7 +125,7 @@ def tune(case, problem, candidates=CANDIDATES, reps: int = 4, max_bytes: float |
         finally:
             ctx.close()
     if not timed:
-        raise _native.NativeError("autotune: no candidate tile shape builds for this case")
+        return None
     best = min(timed, key=lambda r: r["ms"])
     entry = {**best, "timed": timed, "candidates": [list(c) for c in cands]}
     _memo[memo_key] = entry

@@ -115,7 +115,11 @@ class Solver:
         if c.performance.autotune and not (max_bundle or block or sweep_variant):
             from .autotune import tune
             self.tile_shape = tune(c, pb, max_bytes=max_bytes)
-            max_bundle, block, sweep_variant = (self.tile_shape[k] for k in ("max_bundle", "block", "variant"))
+            if self.tile_shape is None:
+                warnings.warn("performance.autotune: no tile shape could be timed (no stored factors of "
+                              "the run's precision); the default schedule is used")
+            else:
+                max_bundle, block, sweep_variant = (self.tile_shape[k] for k in ("max_bundle", "block", "variant"))
         self.ctx = SweepContext.build(c, self.mesh, self.integrals, self.vel, self.order,
                                       self.bcdata, max_bundle=max_bundle, block=block,
                                       variant=sweep_variant, lds_slots_cap=lds_slots_cap,
