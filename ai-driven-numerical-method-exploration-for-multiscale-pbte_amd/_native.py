@@ -51,6 +51,12 @@ class PbteGsisData(C.Structure):
                 ("variant_b", C.c_int32), ("use_hmin", C.c_int32)]
 
 
+class PbteTraceDesc(C.Structure):
+    _fields_ = [("n_faces", C.c_int32), ("n_cells", C.c_int32), ("blk", C.c_int32),
+                ("leaf_cells", C.c_int32), ("face_t1", _i32p), ("face_t2", _i32p),
+                ("tri_faces", _i32p), ("cell_xy", _f64p), ("bsr_ptr", _i32p), ("bsr_col", _i32p)]
+
+
 _SIGS = {
     "pbte_build_order": (C.c_int, [C.c_int32, C.c_int32, _i32p, _f64p, _f64p, _f64p, _f64p,
                                    C.c_int32, _i32p, _i32p]),
@@ -105,6 +111,12 @@ _SIGS = {
     "pbte_gsis_correct": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_trace_create": (C.c_void_p, [C.POINTER(PbteTraceDesc), C.c_void_p, C.c_char_p, C.c_int32]),
+    "pbte_trace_destroy": (None, [C.c_void_p]),
+    "pbte_trace_last_error": (C.c_char_p, [C.c_void_p]),
+    "pbte_trace_info": (C.c_int, [C.c_void_p, _i64p]),
+    "pbte_trace_factor": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pbte_trace_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -8,9 +8,9 @@ After each sweep, on the device:
 2. ``pbte_gsis_source``   HoT stress rows and ``AA_SRC = inv(AA_SOL) work``;
 3. ``pbte_gsis_rhs``      trace right-hand side, wall rows from the kinetic
    traces (adiabatic flux projected on the tangent);
-4. trace solve            ``K u = ffa`` with ``K`` factorised once
-   (SuperLU, as ``global_solve.py:120-124``); the solve runs on the host
-   in binary64 and moves ``3Q F`` values each way;
+4. trace solve            ``K u = ffa`` with ``K`` factorised once on the
+   device (``trace_solve.TraceSolver``: nested-dissection multifrontal, in
+   place of the reference's SuperLU, ``global_solve.py:120-124, 190-195``);
 5. ``pbte_gsis_local``    ``UQ = AA_SRC + AA_TRACE U_TRACE``;
 6. ``pbte_gsis_correct``  damped blend of ``UQ`` into ``vdf`` and the moments.
 
@@ -25,41 +25,11 @@ import numpy as np
 
 from . import _native
 from .constants import BC_NONTHERMALISING, BC_THERMALISING, PI
-from .device import dev_zeros, ptr, to_dev, torch
-from .hdg import (assemble_global_matrix, build_ba_sol, build_local_operators, face_sides,
-                  mirror_permutation)
+from .device import dev_zeros, ptr, stream_handle, torch
+from .hdg import assemble_global_matrix, build_ba_sol, build_local_operators, face_sides
+from .trace_solve import TraceSolver, block_pattern
 
 __all__ = ["Acceleration"]
-
-# trace unknowns up to which the reference's ordering (COLAMD) is kept: every
-# fixture and FROZEN case (at most 29 520 unknowns, 20 x 40 at degree 3);
-# beyond, the minimum-degree ordering of K + K^T with diagonal pivots
-# (factorise_trace; 64^2 channel, 111 744 unknowns: 2.7x less fill, solves
-# 2.1x and factorisation 3.4x faster, the same solution to 3e-14)
-_COLAMD_MAX = 40_000
-
-
-def factorise_trace(K):
-    """SuperLU factors of the trace matrix and the ordering used.  Up to
-    _COLAMD_MAX unknowns the reference's call (``global_solve.py:120-124``,
-    column ordering COLAMD, partial pivoting).  Beyond, the minimum-degree
-    ordering of K + K^T with diagonal pivots (SuperLU's symmetric mode,
-    no row interchanges): its fill is fixed by the ordering, while partial
-    pivoting under that ordering can run away (a 55 485-unknown trace matrix
-    of the batched workload did not factorise in minutes).  The diagonal
-    factors are checked on a seeded right-hand side; a relative residual
-    above 1e-10 falls back to the reference's call."""
-    from scipy.sparse.linalg import splu
-    Kc = K.tocsc()
-    if K.shape[0] > _COLAMD_MAX:
-        lu = splu(Kc, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0, options={"SymmetricMode": True})
-        b = np.random.default_rng(0).standard_normal(K.shape[0])
-        x = lu.solve(b)
-        res = np.abs(Kc @ x - b).max() / np.abs(b).max()
-        if np.isfinite(res) and res <= 1e-10:
-            return lu, "MMD_AT_PLUS_A"
-    return splu(Kc, permc_spec="COLAMD"), "COLAMD"
-
 
 class Acceleration:
     def __init__(self, solver):
@@ -83,10 +53,13 @@ class Acceleration:
         self.K, self.nnz = assemble_global_matrix(mesh, ig, ba_sol, aa_trace, self.nf, bc_type,
                                                   drop_tol=drop,
                                                   periodic_flip=s.bcdata.periodic_flip)
-        self._lu, self.trace_ordering = factorise_trace(self.K)
-        self.solve_count = 0
-
         t1, l1, t2, l2, periodic = face_sides(mesh, bc_type)
+        bptr, bcol, bval = block_pattern(self.K, 3 * self.nf)
+        self.trace = TraceSolver(mesh, t1, t2, bptr, bcol, 3 * self.nf, stream=C.c_void_p(stream_handle()),
+                                 leaf_cells=c.performance.trace_leaf)
+        self.trace.factor(bval)
+        self.trace_info = self.trace.info()
+        del bval
         flip = (s.bcdata.periodic_flip & periodic).astype(np.uint8)
         code = np.where(mesh.face_bc >= 0, bc_type[np.clip(mesh.face_bc, 0, None)], 0)
         wall_th = np.flatnonzero((mesh.face_bc >= 0) & (code == BC_THERMALISING))
@@ -136,7 +109,6 @@ class Acceleration:
         self.uq_d = dev_zeros((7 * P, N))
         self.mkin_d = dev_zeros((3, P, N))
         self.defect_d = dev_zeros((3, P, N))
-        self._host_ffa = torch.empty((F, 3 * Q), dtype=torch.float64, pin_memory=True)
         self.omega = c.scheme.defect_omega
         self.defect_every = c.scheme.defect_every
         self.anderson = c.scheme.defect_anderson
@@ -171,14 +143,15 @@ class Acceleration:
             self.refresh_defect()
 
     def trace_solve(self, ffa=None, out=None) -> None:
-        """``K u = ffa`` with the SuperLU factors (``global_solve.py:190-195``);
-        by default from ``ffa_d`` into ``u_d``."""
+        """``K u = ffa`` on the device (``global_solve.py:190-195``); by
+        default from ``ffa_d`` into ``u_d``."""
         ffa = self.ffa_d if ffa is None else ffa
         out = self.u_d if out is None else out
-        self._host_ffa.copy_(ffa)
-        x = self._lu.solve(self._host_ffa.numpy().reshape(-1))
-        out.copy_(torch.from_numpy(x.reshape(out.shape)))
-        self.solve_count += 1
+        self.trace.solve(ffa, out)
+
+    @property
+    def solve_count(self) -> int:
+        return self.trace.solve_count
 
     # -- the fixed-point repair (acceleration/__init__.py:118-172) ----------
     def refresh_defect(self) -> float:
@@ -221,7 +194,8 @@ class Acceleration:
                 "defect_every": int(self.defect_every), "defect_anderson": int(self.anderson),
                 "defect_outer": len(self.defect_steps),
                 "defect_norm": float(self.defect_d.abs().max()) if self.omega else 0.0,
-                "trace_solve": "host-superlu", "trace_ordering": self.trace_ordering}
+                "trace_solve": "device-multifrontal", "trace_ordering": "nested-dissection",
+                "trace_factor_entries": self.trace_info["factor_entries"]}
 
     def global_matrix(self):
         return self.K

@@ -1,24 +1,29 @@
-"""YAML case description: the same schema as the reference's ``Case``.
+"""YAML case description (``Case.from_yaml``), the reference's schema.
 
-Field names, defaults and validation follow ``solver-python/pybte/config.py:35-355``
-so that an existing case file loads unchanged.  Two ``performance`` fields are
-new and change only how the device computes, never the discrete problem:
+A case file of the reference (``solver-python/pybte/config.py:35-355``) loads
+unchanged: the same sections, keys, defaults, accepted values and
+``ValueError`` on an invalid one.  The structure here is a small schema
+layer: every section is a dataclass whose fields carry their coercion
+(YAML 1.1 reads ``1.0e5`` as a string, so every scalar is converted to its
+declared type) and whose rules are a table of (test, message) pairs checked
+after construction.
 
-* ``performance.kernel`` -- ``"hip"``; the reference's ``"numba"``/``"numpy"``
-  are accepted as aliases so that shipped YAML keeps working.
-* ``performance.precision`` -- ``"fp64"`` (default: every sweep in binary64) or
-  ``"fp32"`` (sweeps and element LU stored and accumulated in binary32 inside
-  an FP64 defect correction).
+Additions of this build, none of which changes the discrete problem:
 
-``scheme.krylov_restart`` exposes the GMRES basis size directly (the
-reference derives it from a byte budget and floors it at 32,
-``krylov.py:249-251``).
+* ``performance.kernel`` -- ``"hip"``; the reference's ``"numba"`` and
+  ``"numpy"`` are accepted as aliases so that shipped files keep working;
+* ``performance.precision`` -- ``"fp64"`` (every sweep in binary64) or
+  ``"fp32"`` (binary32 sweeps inside a binary64 correction);
+* ``performance.refine_switch``, ``refine_tol``, ``autotune``,
+  ``autotune_cache``, ``trace_leaf`` -- documented at their fields;
+* ``scheme.krylov_restart`` -- the GMRES basis size (the reference derives
+  it from a byte budget with a floor of 32, ``krylov.py:249-251``).
 """
 from __future__ import annotations
 
 import hashlib
 import json
-from dataclasses import asdict, dataclass, field, replace
+from dataclasses import asdict, dataclass, field, fields, replace
 from pathlib import Path
 
 from .constants import BC_TYPE_NAMES, NP_FC_DEFAULT, NP_TRI_BY_DEG
@@ -27,203 +32,209 @@ __all__ = ["Case", "Boundary", "Iteration", "Scheme", "VelMesh", "DG", "Flow",
            "MeshCfg", "Restart", "Output", "Performance"]
 
 
-@dataclass
-class Iteration:
-    tol: float = 1.0e-8
-    tmax: int = 10_000
-    true_residual: bool = False
-    true_residual_every: int = 1
-
-    def __post_init__(self):
-        self.tol = float(self.tol)
-        self.tmax = int(self.tmax)
-        self.true_residual = bool(self.true_residual)
-        self.true_residual_every = int(self.true_residual_every)
+# ---------------------------------------------------------------------------
+# schema helpers
+# ---------------------------------------------------------------------------
+def _key(default, conv, factory=None):
+    """A section field with its coercion."""
+    if factory is not None:
+        return field(default_factory=factory, metadata={"conv": conv})
+    return field(default=default, metadata={"conv": conv})
 
 
-@dataclass
-class Scheme:
-    accflag: int = 0                     # 0 = CIS, 1 = GSIS
-    acc_variant: str = "A"
-    stabilisation: list = field(default_factory=lambda: [1.0, 1.0, 1.0])
-    scale_by_hmin: bool = False
-    on_cycle: str = "raise"
-    defect_omega: float = 0.0
-    defect_every: int = 1
-    defect_anderson: int = 0
-    method: str = "source"
-    krylov_tol_factor: float = 1.0e-6
-    krylov_max_bytes: float = 5.0e8
-    krylov_max_rounds: int = 3
-    krylov_precond: bool = False
-    #: GMRES basis size; 0 keeps the reference's byte-budget rule
-    krylov_restart: int = 0
-
-    def __post_init__(self):
-        self.method = str(self.method).lower()
-        if self.method not in ("source", "krylov"):
-            raise ValueError("scheme.method must be 'source' or 'krylov'")
-        self.krylov_tol_factor = float(self.krylov_tol_factor)
-        self.krylov_max_bytes = float(self.krylov_max_bytes)
-        self.krylov_max_rounds = int(self.krylov_max_rounds)
-        self.krylov_precond = bool(self.krylov_precond)
-        self.krylov_restart = int(self.krylov_restart)
-        if self.krylov_max_bytes <= 0 or self.krylov_max_rounds < 1:
-            raise ValueError("scheme.krylov_max_bytes must be > 0 and "
-                             "krylov_max_rounds >= 1")
-        if self.krylov_restart < 0:
-            raise ValueError("scheme.krylov_restart must be >= 0")
-        self.accflag = int(self.accflag)
-        self.acc_variant = str(self.acc_variant).upper()
-        self.stabilisation = [float(v) for v in self.stabilisation]
-        self.scale_by_hmin = bool(self.scale_by_hmin)
-        self.defect_omega = float(self.defect_omega)
-        self.defect_every = int(self.defect_every)
-        if self.defect_every < 0:
-            raise ValueError("scheme.defect_every must be >= 0")
-        self.defect_anderson = int(self.defect_anderson)
-        if self.defect_anderson < 0:
-            raise ValueError("scheme.defect_anderson must be >= 0")
-        if not 0.0 <= self.defect_omega <= 1.0:
-            raise ValueError("scheme.defect_omega must be in [0, 1]")
-        if self.on_cycle not in ("raise", "break"):
-            raise ValueError("scheme.on_cycle must be 'raise' or 'break'")
+def _lower(v) -> str:
+    return str(v).lower()
 
 
-@dataclass
-class VelMesh:
-    npole: int = 20
-    nazim: int = 40
-
-    def __post_init__(self):
-        self.npole = int(self.npole)
-        self.nazim = int(self.nazim)
+def _upper(v) -> str:
+    return str(v).upper()
 
 
-@dataclass
-class DG:
-    deg: int = 3
-
-    def __post_init__(self):
-        self.deg = int(self.deg)
+def _floats(v) -> list:
+    return [float(x) for x in v]
 
 
-@dataclass
-class Flow:
-    cv: float = 1.0
-    vg: float = 1.0
-    tau_r: float = 1.0e-3
-    tau_n: float = 1.0e5
-    tau_thr: float = 1.0
-
-    def __post_init__(self):
-        self.cv = float(self.cv)
-        self.vg = float(self.vg)
-        self.tau_r = float(self.tau_r)
-        self.tau_n = float(self.tau_n)
-        self.tau_thr = float(self.tau_thr)
-        if self.tau_r <= 0.0 or self.tau_n <= 0.0:
-            raise ValueError("tau_r and tau_n must be positive")
-        if self.vg <= 0.0:
-            raise ValueError("vg must be positive")
-
-
-@dataclass
-class MeshCfg:
-    file: str = "meshes/A1_Nx11_Ny11.msh"
-
-
-@dataclass
-class Boundary:
-    name: str
-    phyid: int
-    type: str = "thermalising"
-    temp: float = 0.0
-    xoff: float = 0.0
-    yoff: float = 0.0
-
-    def __post_init__(self):
-        self.phyid = int(self.phyid)
-        self.temp = float(self.temp)
-        self.xoff = float(self.xoff)
-        self.yoff = float(self.yoff)
-
-    @property
-    def code(self) -> int:
-        key = str(self.type).strip().lower()
-        if key not in BC_TYPE_NAMES:
-            raise ValueError(f"unknown boundary type {self.type!r}; "
-                             f"expected one of {sorted(set(BC_TYPE_NAMES))}")
-        return BC_TYPE_NAMES[key]
-
-
-@dataclass
-class Restart:
-    enabled: bool = False
-    path: str | None = None
-    error_on_stale: bool = True
-
-
-@dataclass
-class Output:
-    dir: str = "out"
-    field: bool = True
-    runtime_log: bool = True
-    run_record: bool = True
-    tecplot: bool = False
-    npz: bool = True
-    field_nx: int = 109
-    field_ny: int = 109
+def _opt_str(v):
+    return None if v is None else str(v)
 
 
 _KERNEL_ALIASES = {"hip": "hip", "numba": "hip", "numpy": "hip"}
 
 
-@dataclass
-class Performance:
-    """Storage and kernel choices; none of them changes the discrete problem."""
-    precompute_inverse: bool = True
-    kernel: str = "hip"
-    threads: int = 0
-    precision: str = "fp64"
-    # fp32 source iteration (CIS / GSIS): binary32 sweeps until the iterate
-    # step falls below this (or stalls), then binary64 sweeps to the
-    # tolerance; an addition of this build
-    refine_switch: float = 1.0e-6
-    # fp32 Krylov: 0 decides convergence as the binary64 driver does (the
-    # transport residual below iteration.tol); > 0 stops the refinement
-    # once the estimated relative L2 error of the temperature and heat-flux
-    # moments is below this, the accuracy asked of a mixed-precision solve;
-    # an addition of this build
-    refine_tol: float = 0.0
-    # time the sweep's tile shapes (directions per workgroup, workgroup
-    # width, pipeline depth) on the case's own mesh and ordinates and sweep
-    # with the fastest; results kept per process and, when autotune_cache
-    # names a JSON file, across runs; an addition of this build
-    autotune: bool = False
-    autotune_cache: str = ""
+def _kernel(v) -> str:
+    k = str(v).lower()
+    if k not in _KERNEL_ALIASES:
+        raise ValueError(f"performance.kernel {v!r} is not one of 'hip', 'numba', 'numpy'")
+    return _KERNEL_ALIASES[k]
+
+
+class _Section:
+    """Coerces every field, then checks the section's rules in order."""
+
+    RULES: tuple = ()
 
     def __post_init__(self):
-        self.precompute_inverse = bool(self.precompute_inverse)
-        self.autotune = bool(self.autotune)
-        self.autotune_cache = str(self.autotune_cache or "")
-        key = str(self.kernel).lower()
-        if key not in _KERNEL_ALIASES:
-            raise ValueError("performance.kernel must be 'hip' "
-                             "(or the reference names 'numba'/'numpy')")
-        self.kernel = _KERNEL_ALIASES[key]
-        self.threads = int(self.threads)
-        if self.threads < 0:
-            raise ValueError("performance.threads must be >= 0")
-        self.precision = str(self.precision).lower()
-        if self.precision not in ("fp64", "fp32"):
-            raise ValueError("performance.precision must be 'fp64' or 'fp32'")
-        self.refine_switch = float(self.refine_switch)
-        if not self.refine_switch > 0.0:
-            raise ValueError("performance.refine_switch must be > 0")
-        self.refine_tol = float(self.refine_tol)
-        if not self.refine_tol >= 0.0:
-            raise ValueError("performance.refine_tol must be >= 0")
+        for f in fields(self):
+            conv = f.metadata.get("conv")
+            if conv is not None:
+                setattr(self, f.name, conv(getattr(self, f.name)))
+        for test, message in self.RULES:
+            if not test(self):
+                raise ValueError(message)
+
+
+# ---------------------------------------------------------------------------
+# sections
+# ---------------------------------------------------------------------------
+@dataclass
+class Iteration(_Section):
+    tol: float = _key(1.0e-8, float)
+    tmax: int = _key(10_000, int)
+    # also evaluate the transport residual of the discrete system (never the
+    # stopping test), every true_residual_every iterations
+    true_residual: bool = _key(False, bool)
+    true_residual_every: int = _key(1, int)
+
+
+@dataclass
+class Scheme(_Section):
+    accflag: int = _key(0, int)                          # 0 CIS, 1 GSIS
+    acc_variant: str = _key("A", _upper)
+    stabilisation: list = _key(None, _floats, factory=lambda: [1.0, 1.0, 1.0])
+    scale_by_hmin: bool = _key(False, bool)
+    on_cycle: str = _key("raise", str)
+    defect_omega: float = _key(0.0, float)
+    defect_every: int = _key(1, int)
+    defect_anderson: int = _key(0, int)
+    method: str = _key("source", _lower)
+    krylov_tol_factor: float = _key(1.0e-6, float)
+    krylov_max_bytes: float = _key(5.0e8, float)
+    krylov_max_rounds: int = _key(3, int)
+    krylov_precond: bool = _key(False, bool)
+    # GMRES basis size; 0 keeps the reference's byte-budget rule
+    krylov_restart: int = _key(0, int)
+
+    RULES = (
+        (lambda s: s.method in ("source", "krylov"), "scheme.method: expected 'source' or 'krylov'"),
+        (lambda s: s.krylov_max_bytes > 0 and s.krylov_max_rounds >= 1,
+         "scheme.krylov_max_bytes must be positive and scheme.krylov_max_rounds at least 1"),
+        (lambda s: s.krylov_restart >= 0, "scheme.krylov_restart must not be negative"),
+        (lambda s: s.defect_every >= 0, "scheme.defect_every must not be negative"),
+        (lambda s: s.defect_anderson >= 0, "scheme.defect_anderson must not be negative"),
+        (lambda s: 0.0 <= s.defect_omega <= 1.0, "scheme.defect_omega lies outside [0, 1]"),
+        (lambda s: s.on_cycle in ("raise", "break"), "scheme.on_cycle: expected 'raise' or 'break'"),
+    )
+
+
+@dataclass
+class VelMesh(_Section):
+    npole: int = _key(20, int)
+    nazim: int = _key(40, int)
+
+
+@dataclass
+class DG(_Section):
+    deg: int = _key(3, int)
+
+
+@dataclass
+class Flow(_Section):
+    cv: float = _key(1.0, float)
+    vg: float = _key(1.0, float)
+    tau_r: float = _key(1.0e-3, float)
+    tau_n: float = _key(1.0e5, float)
+    tau_thr: float = _key(1.0, float)
+
+    RULES = (
+        (lambda s: s.tau_r > 0.0 and s.tau_n > 0.0, "flow.tau_r and flow.tau_n must be positive"),
+        (lambda s: s.vg > 0.0, "flow.vg must be positive"),
+    )
+
+
+@dataclass
+class MeshCfg(_Section):
+    file: str = _key("meshes/A1_Nx11_Ny11.msh", str)
+
+
+@dataclass
+class Boundary(_Section):
+    name: str
+    phyid: int = field(metadata={"conv": int})
+    type: str = "thermalising"
+    temp: float = _key(0.0, float)
+    xoff: float = _key(0.0, float)
+    yoff: float = _key(0.0, float)
+
+    @property
+    def code(self) -> int:
+        """The boundary-condition code of ``type`` (constants.py)."""
+        code = BC_TYPE_NAMES.get(str(self.type).strip().lower())
+        if code is None:
+            raise ValueError(f"boundary {self.name!r}: type {self.type!r} is not one of "
+                             f"{sorted(set(BC_TYPE_NAMES))}")
+        return code
+
+
+@dataclass
+class Restart(_Section):
+    enabled: bool = _key(False, bool)
+    path: str | None = _key(None, _opt_str)
+    # with restart off, refuse to run over a VDF*.out left in the output dir
+    error_on_stale: bool = _key(True, bool)
+
+
+@dataclass
+class Output(_Section):
+    dir: str = _key("out", str)
+    field: bool = _key(True, bool)
+    runtime_log: bool = _key(True, bool)
+    run_record: bool = _key(True, bool)
+    tecplot: bool = _key(False, bool)
+    npz: bool = _key(True, bool)
+    field_nx: int = _key(109, int)
+    field_ny: int = _key(109, int)
+
+
+@dataclass
+class Performance(_Section):
+    """How the device computes; nothing here changes the discrete problem."""
+    precompute_inverse: bool = _key(True, bool)
+    kernel: str = _key("hip", _kernel)
+    threads: int = _key(0, int)
+    precision: str = _key("fp64", _lower)
+    # fp32 source iteration (CIS / GSIS): binary32 sweeps until the iterate
+    # step falls below this (or stalls), then binary64 sweeps to the tolerance
+    refine_switch: float = _key(1.0e-6, float)
+    # fp32 Krylov: 0 decides convergence as the binary64 driver does (the
+    # transport residual below iteration.tol); > 0 stops the refinement once
+    # the estimated relative L2 error of the temperature and heat-flux
+    # moments is below this
+    refine_tol: float = _key(0.0, float)
+    # time the sweep's tile shapes on the case's own mesh and ordinates and
+    # sweep with the fastest (kept per process and, with autotune_cache
+    # naming a JSON file, across runs)
+    autotune: bool = _key(False, bool)
+    autotune_cache: str = _key("", lambda v: str(v or ""))
+    # elements per leaf of the trace solve's nested dissection, 0 = 16
+    trace_leaf: int = _key(0, int)
+
+    RULES = (
+        (lambda s: s.threads >= 0, "performance.threads must not be negative"),
+        (lambda s: s.precision in ("fp64", "fp32"), "performance.precision: expected 'fp64' or 'fp32'"),
+        (lambda s: s.refine_switch > 0.0, "performance.refine_switch must be positive"),
+        (lambda s: s.refine_tol >= 0.0, "performance.refine_tol must not be negative"),
+        (lambda s: s.trace_leaf >= 0, "performance.trace_leaf must not be negative"),
+    )
+
+
+# section name in the YAML -> its class (``boundaries`` is a list)
+_SECTIONS = {"iteration": Iteration, "scheme": Scheme, "velmesh": VelMesh, "dg": DG,
+             "flow": Flow, "mesh": MeshCfg, "restart": Restart, "output": Output,
+             "performance": Performance}
+
+# what the numbers depend on: everything except where they are written and
+# how the device computes them
+_NOT_NUMERICS = ("output", "performance")
 
 
 @dataclass
@@ -238,23 +249,26 @@ class Case:
     restart: Restart = field(default_factory=Restart)
     output: Output = field(default_factory=Output)
     performance: Performance = field(default_factory=Performance)
-    basedir: str = "."
+    basedir: str = "."            # relative paths of the case resolve here
 
     def __post_init__(self):
-        # the reference forces an even azimuthal count (config.py:252-255)
-        self.velmesh.nazim = (self.velmesh.nazim // 2) * 2
-        if self.velmesh.nazim < 2:
-            raise ValueError("nazim must be >= 2 after the even-ing step")
-        if self.dg.deg not in NP_TRI_BY_DEG:
-            raise ValueError(f"deg must be one of {sorted(NP_TRI_BY_DEG)}")
-        if self.scheme.acc_variant not in ("A", "B"):
-            raise ValueError("scheme.acc_variant must be 'A' or 'B'")
-        if len(self.scheme.stabilisation) != 3:
-            raise ValueError("scheme.stabilisation must have three entries")
+        # the azimuthal count is made even (config.py:252-255 of the reference)
+        self.velmesh.nazim -= self.velmesh.nazim % 2
+        checks = (
+            (self.velmesh.nazim >= 2, "velmesh.nazim is below 2 once made even"),
+            (self.dg.deg in NP_TRI_BY_DEG, f"dg.deg must be one of {sorted(NP_TRI_BY_DEG)}"),
+            (self.scheme.acc_variant in ("A", "B"), "scheme.acc_variant: expected 'A' or 'B'"),
+            (len(self.scheme.stabilisation) == 3, "scheme.stabilisation needs exactly three values"),
+        )
+        for ok, message in checks:
+            if not ok:
+                raise ValueError(message)
 
+    # -- derived sizes and constants ------------------------------------------
     @property
     def ndof_tri(self) -> int:
-        return (self.dg.deg + 1) * (self.dg.deg + 2) // 2
+        d = self.dg.deg
+        return (d + 1) * (d + 2) // 2
 
     @property
     def ndof_fc(self) -> int:
@@ -270,6 +284,7 @@ class Case:
 
     @property
     def tau_c(self) -> float:
+        """Combined relaxation time, 1 / (1/tau_R + 1/tau_N)."""
         return 1.0 / (1.0 / self.flow.tau_r + 1.0 / self.flow.tau_n)
 
     @property
@@ -279,58 +294,53 @@ class Case:
     def bc_codes(self) -> list:
         return [b.code for b in self.boundaries]
 
+    def _resolve(self, p) -> Path:
+        p = Path(p)
+        return p if p.is_absolute() else Path(self.basedir) / p
+
     def mesh_path(self) -> Path:
-        p = Path(self.mesh.file)
-        return p if p.is_absolute() else (Path(self.basedir) / p)
+        return self._resolve(self.mesh.file)
 
     def output_path(self) -> Path:
-        p = Path(self.output.dir)
-        return p if p.is_absolute() else (Path(self.basedir) / p)
+        return self._resolve(self.output.dir)
 
+    # -- serialisation -------------------------------------------------------
     def to_dict(self) -> dict:
-        d = asdict(self)
-        d.pop("basedir", None)
-        return d
+        out = {name: asdict(getattr(self, name)) for name in _SECTIONS}
+        out["boundaries"] = [asdict(b) for b in self.boundaries]
+        return out
 
     def canonical_json(self) -> str:
-        """Everything that can change the numbers, and nothing else."""
-        d = self.to_dict()
-        d.pop("output", None)
-        d.pop("performance", None)
+        """The inputs that determine the discrete problem, as sorted compact
+        JSON."""
+        d = {k: v for k, v in self.to_dict().items() if k not in _NOT_NUMERICS}
         return json.dumps(d, sort_keys=True, separators=(",", ":"))
 
     @property
     def config_hash(self) -> str:
-        h = hashlib.sha256(self.canonical_json().encode()).hexdigest()
+        """SHA-256 over the canonical JSON followed by the mesh file's bytes
+        (when the file exists)."""
+        h = hashlib.sha256(self.canonical_json().encode())
         mesh = self.mesh_path()
         if mesh.exists():
-            h2 = hashlib.sha256(mesh.read_bytes()).hexdigest()
-            h = hashlib.sha256((h + h2).encode()).hexdigest()
-        return h
+            h.update(mesh.read_bytes())
+        return h.hexdigest()
 
     def replace(self, **kw) -> "Case":
         return replace(self, **kw)
 
     @classmethod
     def from_dict(cls, data: dict, basedir: str = ".") -> "Case":
-        def sub(klass, key):
-            return klass(**(data.get(key) or {}))
-
-        bnds = [Boundary(**b) for b in (data.get("boundaries") or [])]
-        return cls(iteration=sub(Iteration, "iteration"), scheme=sub(Scheme, "scheme"),
-                   velmesh=sub(VelMesh, "velmesh"), dg=sub(DG, "dg"),
-                   flow=sub(Flow, "flow"), mesh=sub(MeshCfg, "mesh"),
-                   boundaries=bnds, restart=sub(Restart, "restart"),
-                   output=sub(Output, "output"),
-                   performance=sub(Performance, "performance"), basedir=basedir)
+        kw = {name: klass(**(data.get(name) or {})) for name, klass in _SECTIONS.items()}
+        kw["boundaries"] = [Boundary(**b) for b in data.get("boundaries") or []]
+        return cls(basedir=basedir, **kw)
 
     @classmethod
     def from_yaml(cls, path) -> "Case":
         import yaml
 
         path = Path(path)
-        data = yaml.safe_load(path.read_text()) or {}
-        return cls.from_dict(data, basedir=str(path.parent))
+        return cls.from_dict(yaml.safe_load(path.read_text()) or {}, basedir=str(path.parent))
 
     def to_yaml(self, path) -> None:
         import yaml

@@ -1,0 +1,826 @@
+// Device sparse direct solve of the GSIS-HDG trace system K u = f.
+//
+// Replaces the reference's SuperLU factorisation and solve of the global
+// trace matrix (solver-python/pybte/acceleration/global_solve.py:120-124
+// factorise, :190-195 solve) with a nested-dissection multifrontal solver
+// that runs on the device:
+//
+// * symbolic phase (host, once per mesh): the cells are split by recursive
+//   coordinate bisection of their centroids into a binary tree; a face
+//   belongs to the lowest tree node whose cells hold both elements it
+//   couples (the separator faces of that node; interior faces of a leaf);
+//   a node's front is its own faces followed by the faces of its ancestors
+//   that touch its cells.  Every block K[f, g] is assembled into the front of
+//   the deeper of the owners of f and g.
+// * numeric phase (device, level by level from the leaves): the fronts of a
+//   level are assembled (original blocks, then the update matrices of child
+//   0, then child 1: a fixed summation order), and the fully summed block of
+//   each front is eliminated by the blocked Gauss-Jordan sweep with row
+//   interchanges among its fully summed rows (partial pivoting).  The sweep
+//   leaves [F11^-1 P^T, F11^-1 F12] in the fully summed rows, -F21 F11^-1 P^T
+//   below them and the Schur complement F22 - F21 F11^-1 F12 (the update
+//   matrix passed to the parent) in the rest.
+// * solve (device): forward from the leaves, c = b2 - W P b1 passed to the
+//   parent; backward from the root, x1 = [Z V] [P b1; -x2].  Every step is a
+//   dense matrix-vector product over a front's stored rows: no triangular
+//   recurrences, and each row is reduced in a fixed order (the result does
+//   not depend on the launch).
+//
+// All arithmetic is binary64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pbte_hip.h"
+
+namespace {
+
+#define TL_TRY(expr)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess) {                                             \
+            throw std::string(#expr " failed: ") + hipGetErrorString(e_);  \
+        }                                                                   \
+    } while (0)
+
+constexpr int KB = 32;        // pivot columns per sweep step
+constexpr int GEMV_ROWS = 16; // rows of a front per solve work item
+constexpr int TILE = 64;      // update tile (rows and columns)
+
+struct Front {                // one tree node, device and host
+    int64_t ws;               // front matrix offset in its level's workspace (doubles)
+    int64_t zv;               // [Z V] rows (nfs x n) in the factor storage
+    int64_t w;                // W rows (ncb x nfs) in the factor storage
+    int64_t pan;              // panel scratch (n x KB) and row copy (KB x n) offset
+    int32_t n, nfs, ncb;
+    int32_t faces;            // offset of the front's faces (own, then ancestors')
+    int32_t piv;              // offset of nfs pivot rows / forward values y
+    int32_t cvec;             // offset of the ncb forward contributions
+    int32_t child[2];         // node ids, -1 when absent
+    int32_t cmap[2];          // offsets of the child face -> front position maps
+};
+
+struct Level {
+    std::vector<int32_t> nodes;
+    int64_t ws_size = 0, pan_size = 0;
+    int32_t n_max = 0, nfs_max = 0, ncb_max = 0, ncb_child_max = 0;
+    int32_t node_off = 0;             // into the device node list
+    int32_t rec_off = 0, rec_n = 0;   // original-block records
+    int32_t fw_off = 0, fw_n = 0;     // forward work items (front, row0)
+    int32_t bw_off = 0, bw_n = 0;     // backward work items
+    bool has_children = false;
+};
+
+// ---------------------------------------------------------------------------
+// device kernels: numeric factorisation
+// ---------------------------------------------------------------------------
+__global__ void k_fill(double* __restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+__global__ void k_set_int(int32_t* __restrict__ p, int32_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// rec = {block, node, row position (scalars), column position (scalars)}
+__global__ void k_assemble(const int4* __restrict__ rec, int nrec, const double* __restrict__ val, int b,
+                           const Front* __restrict__ fr, double* __restrict__ ws)
+{
+    const int64_t bb = (int64_t)b * b;
+    const int64_t total = (int64_t)nrec * bb;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = g / bb;
+        const int e = (int)(g - r * bb);
+        const int i = e / b, j = e - (e / b) * b;
+        const int4 q = rec[r];
+        const Front f = fr[q.y];
+        ws[f.ws + (int64_t)(q.z + i) * f.n + q.w + j] = val[(int64_t)q.x * bb + e];
+    }
+}
+
+// adds the update matrix of child `slot` of every front of the level:
+// blockIdx.x = front, blockIdx.y = chunk of 8 rows of the child's matrix
+__global__ void k_extend(const int32_t* __restrict__ nodes, int slot, const Front* __restrict__ fr,
+                         const int32_t* __restrict__ fmap, int b, double* __restrict__ ws,
+                         const double* __restrict__ wsc)
+{
+    const Front v = fr[nodes[blockIdx.x]];
+    const int c = v.child[slot];
+    if (c < 0) return;
+    const Front ch = fr[c];
+    const double* S = wsc + ch.ws + (int64_t)ch.nfs * ch.n + ch.nfs;
+    const int32_t* map = fmap + v.cmap[slot];
+    double* M = ws + v.ws;
+    const int i0 = blockIdx.y * 8;
+    for (int i = i0; i < min(i0 + 8, ch.ncb); ++i) {
+        const int pi = map[i / b] * b + i % b;
+        for (int j = threadIdx.x; j < ch.ncb; j += blockDim.x) {
+            const int pj = map[j / b] * b + j % b;
+            M[(int64_t)pi * v.n + pj] += S[(int64_t)i * ch.n + j];
+        }
+    }
+}
+
+// Panel of KB pivot columns [j0, j0 + kbe) of each front: copies the columns
+// (all n rows) to the panel scratch, sweeps them pivot by pivot with the
+// pivot row chosen among the remaining fully summed rows (largest
+// magnitude, first on ties), and records the interchanges.
+__global__ void __launch_bounds__(1024) k_panel(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
+                                                const double* __restrict__ ws, double* __restrict__ pan, int j0,
+                                                int32_t* __restrict__ ipiv, int32_t* __restrict__ perm,
+                                                int32_t* __restrict__ status)
+{
+    __shared__ double s_val[1024];
+    __shared__ int s_idx[1024];
+    const Front f = fr[nodes[blockIdx.x]];
+    if (j0 >= f.nfs) return;
+    const int n = f.n, kbe = min(KB, f.nfs - j0), tid = threadIdx.x, nt = blockDim.x;
+    const double* M = ws + f.ws;
+    double* P = pan + f.pan;
+    if (j0 == 0)
+        for (int k = tid; k < f.nfs; k += nt) perm[f.piv + k] = k;
+    for (int64_t e = tid; e < (int64_t)n * KB; e += nt) {
+        const int i = (int)(e / KB), c = (int)(e - (int64_t)i * KB);
+        P[e] = c < kbe ? M[(int64_t)i * n + j0 + c] : 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < kbe; ++k) {
+        const int r0 = j0 + k;
+        double best = -1.0;
+        int bi = r0;
+        for (int i = r0 + tid; i < f.nfs; i += nt) {
+            const double a = fabs(P[(int64_t)i * KB + k]);
+            if (a > best) { best = a; bi = i; }
+        }
+        s_val[tid] = best;
+        s_idx[tid] = bi;
+        __syncthreads();
+        for (int s = nt / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                const double o = s_val[tid + s];
+                const int oi = s_idx[tid + s];
+                if (o > s_val[tid] || (o == s_val[tid] && oi < s_idx[tid])) {
+                    s_val[tid] = o;
+                    s_idx[tid] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        const int p = s_idx[0];
+        const bool singular = !(s_val[0] > 0.0);
+        __syncthreads();
+        if (singular) {
+            // no pivot: the rest of the panel is left unswept, with no row
+            // interchanges recorded (the later kernels read them)
+            for (int q = k + tid; q < kbe; q += nt) ipiv[f.piv + j0 + q] = j0 + q;
+            if (tid == 0) atomicMin(status, f.faces);   // a front's first face id as the tag
+            return;
+        }
+        if (p != r0) {
+            for (int c = tid; c < KB; c += nt) {
+                const double t = P[(int64_t)r0 * KB + c];
+                P[(int64_t)r0 * KB + c] = P[(int64_t)p * KB + c];
+                P[(int64_t)p * KB + c] = t;
+            }
+        }
+        if (tid == 0) {
+            ipiv[f.piv + r0] = p;
+            const int t = perm[f.piv + r0];
+            perm[f.piv + r0] = perm[f.piv + p];
+            perm[f.piv + p] = t;
+        }
+        __syncthreads();
+        const double inv = 1.0 / P[(int64_t)r0 * KB + k];
+        for (int64_t e = tid; e < (int64_t)n * kbe; e += nt) {
+            const int i = (int)(e / kbe), c = (int)(e - (int64_t)i * kbe);
+            if (i == r0 || c == k) continue;
+            const double t = P[(int64_t)i * KB + k] * inv;
+            P[(int64_t)i * KB + c] = fma(-t, P[(int64_t)r0 * KB + c], P[(int64_t)i * KB + c]);
+        }
+        __syncthreads();
+        for (int c = tid; c < kbe; c += nt)
+            if (c != k) P[(int64_t)r0 * KB + c] *= inv;
+        for (int i = tid; i < n; i += nt)
+            if (i != r0) P[(int64_t)i * KB + k] = -P[(int64_t)i * KB + k] * inv;
+        if (tid == 0) P[(int64_t)r0 * KB + k] = inv;
+        __syncthreads();
+    }
+}
+
+// Row interchanges of the step on the columns outside the panel, then the
+// pivot rows (outside the panel; zero inside it) copied to the row scratch.
+// blockIdx.x = front, blockIdx.y = chunk of 256 columns.
+__global__ void k_swap_rows(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
+                            double* __restrict__ ws, double* __restrict__ pan, int j0,
+                            const int32_t* __restrict__ ipiv)
+{
+    const Front f = fr[nodes[blockIdx.x]];
+    if (j0 >= f.nfs) return;
+    const int c = blockIdx.y * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    const int kbe = min(KB, f.nfs - j0), n = f.n;
+    double* M = ws + f.ws;
+    double* T = pan + f.pan + (int64_t)n * KB;
+    const bool in_panel = c >= j0 && c < j0 + kbe;
+    if (!in_panel) {
+        for (int k = 0; k < kbe; ++k) {
+            const int r0 = j0 + k, p = ipiv[f.piv + r0];
+            if (p != r0 && p > r0 && p < f.nfs) {
+                const double t = M[(int64_t)r0 * n + c];
+                M[(int64_t)r0 * n + c] = M[(int64_t)p * n + c];
+                M[(int64_t)p * n + c] = t;
+            }
+        }
+    }
+    for (int k = 0; k < KB; ++k)
+        T[(int64_t)k * n + c] = (in_panel || k >= kbe) ? 0.0 : M[(int64_t)(j0 + k) * n + c];
+}
+
+// The rank-KB update of the sweep step on 64 x 64 tiles of every front:
+// outside the panel columns M[i][c] = [i not a pivot row] M[i][c]
+// + sum_k P[i][k] T[k][c]; inside them, the swept panel is stored.
+// blockIdx.x = tile (row-major over the largest front of the level),
+// blockIdx.y = front.
+__global__ void __launch_bounds__(256) k_update(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
+                                                double* __restrict__ ws, const double* __restrict__ pan, int j0,
+                                                int tiles_per_row)
+{
+    __shared__ double sP[TILE][KB + 1];
+    __shared__ double sT[KB][TILE + 1];
+    const Front f = fr[nodes[blockIdx.y]];
+    if (j0 >= f.nfs) return;
+    const int n = f.n;
+    const int nt = (n + TILE - 1) / TILE;
+    const int ti = blockIdx.x / tiles_per_row, tj = blockIdx.x - (blockIdx.x / tiles_per_row) * tiles_per_row;
+    if (ti >= nt || tj >= nt) return;
+    const int kbe = min(KB, f.nfs - j0);
+    const int r0 = ti * TILE, c0 = tj * TILE;
+    const double* P = pan + f.pan;
+    const double* T = P + (int64_t)n * KB;
+    double* M = ws + f.ws;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < TILE * KB; e += 256) {
+        const int i = e / KB, k = e - (e / KB) * KB;
+        sP[i][k] = (r0 + i < n) ? P[(int64_t)(r0 + i) * KB + k] : 0.0;
+    }
+    for (int e = tid; e < TILE * KB; e += 256) {
+        const int k = e / TILE, c = e - (e / TILE) * TILE;
+        sT[k][c] = (c0 + c < n) ? T[(int64_t)k * n + c0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int tx = tid & 15, ty = tid >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[a][q] = 0.0;
+    for (int k = 0; k < KB; ++k) {
+        double pa[4], tb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) pa[a] = sP[ty * 4 + a][k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tb[q] = sT[k][tx * 4 + q];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[a][q] = fma(pa[a], tb[q], acc[a][q]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int i = r0 + ty * 4 + a;
+        if (i >= n) continue;
+        const bool pivot_row = i >= j0 && i < j0 + kbe;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = c0 + tx * 4 + q;
+            if (c >= n) continue;
+            double* m = M + (int64_t)i * n + c;
+            if (c >= j0 && c < j0 + kbe)
+                *m = P[(int64_t)i * KB + (c - j0)];
+            else
+                *m = (pivot_row ? 0.0 : *m) + acc[a][q];
+        }
+    }
+}
+
+// factors out of the front matrix: [Z V] rows and W = -(rows below);
+// blockIdx.x = front, blockIdx.y = chunk of 4 rows
+__global__ void k_extract(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
+                          const double* __restrict__ ws, double* __restrict__ fac)
+{
+    const Front f = fr[nodes[blockIdx.x]];
+    const double* M = ws + f.ws;
+    for (int i = blockIdx.y * 4; i < min(blockIdx.y * 4 + 4, f.n); ++i) {
+        if (i < f.nfs) {
+            for (int j = threadIdx.x; j < f.n; j += blockDim.x)
+                fac[f.zv + (int64_t)i * f.n + j] = M[(int64_t)i * f.n + j];
+        } else {
+            for (int j = threadIdx.x; j < f.nfs; j += blockDim.x)
+                fac[f.w + (int64_t)(i - f.nfs) * f.nfs + j] = -M[(int64_t)i * f.n + j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// device kernels: solve
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// forward assembly of one front: b1 from the right-hand side plus the
+// children's contributions, y = P b1, c = (contributions on the ancestors'
+// faces); one workgroup per front, the front vector in LDS
+__global__ void k_fwd_gather(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
+                             const int32_t* __restrict__ faces, const int32_t* __restrict__ fmap,
+                             const int32_t* __restrict__ perm, int b, const double* __restrict__ rhs,
+                             double* __restrict__ y, double* __restrict__ cv)
+{
+    extern __shared__ double bf[];
+    const Front f = fr[nodes[blockIdx.x]];
+    for (int i = threadIdx.x; i < f.n; i += blockDim.x)
+        bf[i] = i < f.nfs ? rhs[(int64_t)faces[f.faces + i / b] * b + i % b] : 0.0;
+    __syncthreads();
+    for (int s = 0; s < 2; ++s) {
+        const int c = f.child[s];
+        if (c < 0) continue;
+        const Front ch = fr[c];
+        const int32_t* map = fmap + f.cmap[s];
+        for (int j = threadIdx.x; j < ch.ncb; j += blockDim.x) bf[map[j / b] * b + j % b] += cv[ch.cvec + j];
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < f.nfs; k += blockDim.x) y[f.piv + k] = bf[perm[f.piv + k]];
+    for (int i = threadIdx.x; i < f.ncb; i += blockDim.x) cv[f.cvec + i] = bf[f.nfs + i];
+}
+
+// c[i] -= W[i] . y for GEMV_ROWS rows of a front per workgroup (4 waves,
+// one row per wave at a time, lanes across the row, fixed-order reduction)
+__global__ void __launch_bounds__(256) k_fwd_gemv(const int2* __restrict__ items, const Front* __restrict__ fr,
+                                                  const double* __restrict__ fac, const double* __restrict__ y,
+                                                  double* __restrict__ cv)
+{
+    const int2 it = items[blockIdx.x];
+    const Front f = fr[it.x];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const double* yv = y + f.piv;
+    for (int r = it.y + wv; r < min(it.y + GEMV_ROWS, f.ncb); r += 4) {
+        const double* row = fac + f.w + (int64_t)r * f.nfs;
+        double s = 0.0;
+        for (int k = lane; k < f.nfs; k += 64) s = fma(row[k], yv[k], s);
+        s = wave_sum(s);
+        if (lane == 0) cv[f.cvec + r] -= s;
+    }
+}
+
+// x[own faces] = [Z V] [y; -x(ancestors' faces)] for GEMV_ROWS rows of a
+// front per workgroup; the input vector is staged in LDS
+__global__ void __launch_bounds__(256) k_bwd(const int2* __restrict__ items, const Front* __restrict__ fr,
+                                             const int32_t* __restrict__ faces, int b,
+                                             const double* __restrict__ fac, const double* __restrict__ y,
+                                             double* __restrict__ x)
+{
+    extern __shared__ double u[];
+    const int2 it = items[blockIdx.x];
+    const Front f = fr[it.x];
+    for (int j = threadIdx.x; j < f.n; j += blockDim.x)
+        u[j] = j < f.nfs ? y[f.piv + j] : -x[(int64_t)faces[f.faces + j / b] * b + j % b];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int r = it.y + wv; r < min(it.y + GEMV_ROWS, f.nfs); r += 4) {
+        const double* row = fac + f.zv + (int64_t)r * f.n;
+        double s = 0.0;
+        for (int k = lane; k < f.n; k += 64) s = fma(row[k], u[k], s);
+        s = wave_sum(s);
+        if (lane == 0) x[(int64_t)faces[f.faces + r / b] * b + r % b] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host: symbolic phase
+// ---------------------------------------------------------------------------
+struct Symbolic {
+    int F = 0, N = 0, b = 0;
+    std::vector<Front> fronts;
+    std::vector<Level> levels;           // by depth, 0 = root
+    std::vector<int32_t> face_list, fmap, node_list;
+    std::vector<int4> rec;
+    std::vector<int2> fw_items, bw_items;
+    int64_t fac_size = 0, piv_size = 0, cvec_size = 0;
+    int64_t flops = 0;
+    int max_depth = 0;
+};
+
+template <typename T> T* dmalloc(size_t n)
+{
+    void* p = nullptr;
+    TL_TRY(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+void build_symbolic(const pbte_trace_desc* d, Symbolic& S)
+{
+    const int F = d->n_faces, N = d->n_cells, b = d->blk;
+    const int leaf = d->leaf_cells > 0 ? d->leaf_cells : 16;
+    if (F <= 0 || N <= 0 || b <= 0) throw std::string("trace solve: empty system");
+    S.F = F;
+    S.N = N;
+    S.b = b;
+    // tree by recursive coordinate bisection; cells permuted so that every
+    // node holds a contiguous range
+    std::vector<int32_t> cell(N);
+    std::iota(cell.begin(), cell.end(), 0);
+    struct Node { int parent, depth, lo, hi, child[2]; };
+    std::vector<Node> nodes;
+    nodes.push_back({-1, 0, 0, N, {-1, -1}});
+    for (size_t v = 0; v < nodes.size(); ++v) {
+        const int lo = nodes[v].lo, hi = nodes[v].hi;
+        if (hi - lo <= leaf) continue;
+        double mn[2] = {1e300, 1e300}, mx[2] = {-1e300, -1e300};
+        for (int k = lo; k < hi; ++k)
+            for (int a = 0; a < 2; ++a) {
+                mn[a] = std::min(mn[a], d->cell_xy[2 * cell[k] + a]);
+                mx[a] = std::max(mx[a], d->cell_xy[2 * cell[k] + a]);
+            }
+        const int ax = (mx[1] - mn[1] > mx[0] - mn[0]) ? 1 : 0;
+        const int mid = lo + (hi - lo) / 2;
+        auto key = [&](int c) { return d->cell_xy[2 * c + ax]; };
+        std::nth_element(cell.begin() + lo, cell.begin() + mid, cell.begin() + hi, [&](int p, int q) {
+            const double kp = key(p), kq = key(q);
+            return kp < kq || (kp == kq && p < q);
+        });
+        const int depth = nodes[v].depth + 1;
+        const int a = (int)nodes.size();
+        nodes.push_back({(int)v, depth, lo, mid, {-1, -1}});
+        nodes.push_back({(int)v, depth, mid, hi, {-1, -1}});
+        nodes[v].child[0] = a;
+        nodes[v].child[1] = a + 1;
+    }
+    const int V = (int)nodes.size();
+    std::vector<int32_t> leaf_of(N, -1);
+    for (int v = 0; v < V; ++v)
+        if (nodes[v].child[0] < 0)
+            for (int k = nodes[v].lo; k < nodes[v].hi; ++k) leaf_of[cell[k]] = v;
+    // owner of each face: the lowest node holding both cells it couples
+    std::vector<int32_t> owner(F);
+    for (int f = 0; f < F; ++f) {
+        const int c1 = d->face_t1[f], c2 = d->face_t2[f];
+        if (c1 < 0 || c1 >= N || c2 >= N) throw std::string("trace solve: face cell out of range");
+        int a = leaf_of[c1], q = c2 >= 0 ? leaf_of[c2] : a;
+        while (a != q) {
+            if (nodes[a].depth >= nodes[q].depth) a = nodes[a].parent;
+            else q = nodes[q].parent;
+        }
+        owner[f] = a;
+    }
+    // faces touching each cell: its own three and every face that couples it
+    std::vector<int32_t> tptr(N + 1, 0), tf;
+    {
+        std::vector<std::vector<int32_t>> t(N);
+        for (int c = 0; c < N; ++c)
+            for (int k = 0; k < 3; ++k) t[c].push_back(d->tri_faces[3 * c + k]);
+        for (int f = 0; f < F; ++f) {
+            t[d->face_t1[f]].push_back(f);
+            if (d->face_t2[f] >= 0) t[d->face_t2[f]].push_back(f);
+        }
+        for (int c = 0; c < N; ++c) {
+            std::sort(t[c].begin(), t[c].end());
+            t[c].erase(std::unique(t[c].begin(), t[c].end()), t[c].end());
+            tptr[c + 1] = tptr[c] + (int)t[c].size();
+            tf.insert(tf.end(), t[c].begin(), t[c].end());
+        }
+    }
+    // own faces of each node (ascending face id)
+    std::vector<std::vector<int32_t>> own(V), anc(V);
+    for (int f = 0; f < F; ++f) own[owner[f]].push_back(f);
+    std::vector<int32_t> stamp(F, -1);
+    for (int v = 0; v < V; ++v) {
+        const int dv = nodes[v].depth;
+        for (int k = nodes[v].lo; k < nodes[v].hi; ++k) {
+            const int c = cell[k];
+            for (int e = tptr[c]; e < tptr[c + 1]; ++e) {
+                const int f = tf[e];
+                if (nodes[owner[f]].depth < dv && stamp[f] != v) {
+                    stamp[f] = v;
+                    anc[v].push_back(f);
+                }
+            }
+        }
+        std::sort(anc[v].begin(), anc[v].end());
+    }
+    // fronts, levels, offsets
+    int max_depth = 0;
+    for (auto& nd : nodes) max_depth = std::max(max_depth, nd.depth);
+    S.max_depth = max_depth;
+    S.levels.assign(max_depth + 1, Level());
+    S.fronts.resize(V);
+    int64_t fac = 0, piv = 0, cvec = 0;
+    for (int v = 0; v < V; ++v) {
+        Front& f = S.fronts[v];
+        f.nfs = (int)own[v].size() * b;
+        f.ncb = (int)anc[v].size() * b;
+        f.n = f.nfs + f.ncb;
+        f.faces = (int32_t)S.face_list.size();
+        S.face_list.insert(S.face_list.end(), own[v].begin(), own[v].end());
+        S.face_list.insert(S.face_list.end(), anc[v].begin(), anc[v].end());
+        f.child[0] = nodes[v].child[0];
+        f.child[1] = nodes[v].child[1];
+        f.cmap[0] = f.cmap[1] = -1;
+        f.zv = fac;
+        fac += (int64_t)f.nfs * f.n;
+        f.w = fac;
+        fac += (int64_t)f.ncb * f.nfs;
+        f.piv = (int32_t)piv;
+        piv += f.nfs;
+        f.cvec = (int32_t)cvec;
+        cvec += f.ncb;
+        Level& L = S.levels[nodes[v].depth];
+        f.ws = L.ws_size;
+        L.ws_size += (int64_t)f.n * f.n;
+        f.pan = L.pan_size;
+        L.pan_size += 2 * (int64_t)f.n * KB;
+        L.nodes.push_back(v);
+        L.n_max = std::max(L.n_max, f.n);
+        L.nfs_max = std::max(L.nfs_max, f.nfs);
+        L.ncb_max = std::max(L.ncb_max, f.ncb);
+        S.flops += 2 * (int64_t)f.nfs * f.n * f.n;
+    }
+    if (piv != (int64_t)F * b) throw std::string("trace solve: faces not owned exactly once");
+    S.fac_size = fac;
+    S.piv_size = piv;
+    S.cvec_size = cvec;
+    // child maps: positions of the child's ancestor faces in the parent front
+    std::vector<int32_t> pos(F, -1);
+    for (int v = 0; v < V; ++v) {
+        Front& f = S.fronts[v];
+        const int nfc = f.n / b;
+        for (int k = 0; k < nfc; ++k) pos[S.face_list[f.faces + k]] = k;
+        for (int s = 0; s < 2; ++s) {
+            const int c = f.child[s];
+            if (c < 0) continue;
+            const Front& ch = S.fronts[c];
+            f.cmap[s] = (int32_t)S.fmap.size();
+            const int nown = ch.nfs / b;
+            for (int k = 0; k < ch.ncb / b; ++k) {
+                const int g = S.face_list[ch.faces + nown + k];
+                if (pos[g] < 0) throw std::string("trace solve: child face missing from parent front");
+                S.fmap.push_back(pos[g]);
+            }
+            S.levels[nodes[v].depth].has_children = true;
+            S.levels[nodes[v].depth].ncb_child_max = std::max(S.levels[nodes[v].depth].ncb_child_max, ch.ncb);
+        }
+        for (int k = 0; k < nfc; ++k) pos[S.face_list[f.faces + k]] = -1;
+    }
+    // original blocks, grouped by destination node and level
+    std::vector<std::vector<int4>> recs(max_depth + 1);
+    {
+        std::vector<std::vector<std::pair<int, int>>> by_node(V);   // (block, row face)
+        for (int f = 0; f < F; ++f)
+            for (int e = d->bsr_ptr[f]; e < d->bsr_ptr[f + 1]; ++e) {
+                const int g = d->bsr_col[e];
+                if (g < 0 || g >= F) throw std::string("trace solve: block column out of range");
+                const int of = owner[f], og = owner[g];
+                const int dst = nodes[of].depth >= nodes[og].depth ? of : og;
+                by_node[dst].push_back({e, f});
+            }
+        for (int v = 0; v < V; ++v) {
+            const Front& fv = S.fronts[v];
+            const int nfc = fv.n / b;
+            for (int k = 0; k < nfc; ++k) pos[S.face_list[fv.faces + k]] = k;
+            for (auto [e, f] : by_node[v]) {
+                const int g = d->bsr_col[e];
+                if (pos[f] < 0 || pos[g] < 0)
+                    throw std::string("trace solve: a block couples faces that share no cell");
+                recs[nodes[v].depth].push_back({e, v, pos[f] * b, pos[g] * b});
+            }
+            for (int k = 0; k < nfc; ++k) pos[S.face_list[fv.faces + k]] = -1;
+        }
+    }
+    for (int l = 0; l <= max_depth; ++l) {
+        Level& L = S.levels[l];
+        L.node_off = (int32_t)S.node_list.size();
+        S.node_list.insert(S.node_list.end(), L.nodes.begin(), L.nodes.end());
+        L.rec_off = (int32_t)S.rec.size();
+        L.rec_n = (int32_t)recs[l].size();
+        S.rec.insert(S.rec.end(), recs[l].begin(), recs[l].end());
+        L.fw_off = (int32_t)S.fw_items.size();
+        L.bw_off = (int32_t)S.bw_items.size();
+        for (int v : L.nodes) {
+            const Front& f = S.fronts[v];
+            for (int r = 0; r < f.ncb; r += GEMV_ROWS) S.fw_items.push_back({v, r});
+            for (int r = 0; r < f.nfs; r += GEMV_ROWS) S.bw_items.push_back({v, r});
+        }
+        L.fw_n = (int32_t)S.fw_items.size() - L.fw_off;
+        L.bw_n = (int32_t)S.bw_items.size() - L.bw_off;
+        if ((int64_t)L.n_max * 8 > 64 * 1024)
+            throw std::string("trace solve: a front exceeds the LDS vector of the solve (n > 8192)");
+    }
+}
+
+}  // namespace
+
+struct pbte_trace {
+    Symbolic S;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool factored = false;
+    // device
+    Front* d_fr = nullptr;
+    int32_t *d_faces = nullptr, *d_fmap = nullptr, *d_nodes = nullptr, *d_perm = nullptr, *d_ipiv = nullptr,
+            *d_status = nullptr;
+    int4* d_rec = nullptr;
+    int2 *d_fw = nullptr, *d_bw = nullptr;
+    double *d_fac = nullptr, *d_ws[2] = {nullptr, nullptr}, *d_pan = nullptr, *d_y = nullptr, *d_cv = nullptr;
+    int64_t ws_size[2] = {0, 0}, pan_size = 0;
+
+    ~pbte_trace()
+    {
+        void* ps[] = {d_fr, d_faces, d_fmap, d_nodes, d_perm, d_ipiv, d_status, d_rec, d_fw, d_bw,
+                      d_fac, d_ws[0], d_ws[1], d_pan, d_y, d_cv};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+    }
+
+    void alloc()
+    {
+        if (d_fr) return;
+        for (size_t l = 0; l < S.levels.size(); ++l) {
+            ws_size[l & 1] = std::max(ws_size[l & 1], S.levels[l].ws_size);
+            pan_size = std::max(pan_size, S.levels[l].pan_size);
+        }
+        auto up = [&](auto*& dst, const auto& v) {
+            using E = std::remove_reference_t<decltype(v[0])>;
+            dst = dmalloc<std::remove_const_t<E>>(v.size());
+            if (!v.empty())
+                TL_TRY(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice, stream));
+        };
+        up(d_fr, S.fronts);
+        up(d_faces, S.face_list);
+        up(d_fmap, S.fmap);
+        up(d_nodes, S.node_list);
+        up(d_rec, S.rec);
+        up(d_fw, S.fw_items);
+        up(d_bw, S.bw_items);
+        d_perm = dmalloc<int32_t>(S.piv_size);
+        d_ipiv = dmalloc<int32_t>(S.piv_size);
+        d_status = dmalloc<int32_t>(1);
+        d_fac = dmalloc<double>(S.fac_size);
+        d_ws[0] = dmalloc<double>(ws_size[0]);
+        d_ws[1] = dmalloc<double>(ws_size[1]);
+        d_pan = dmalloc<double>(pan_size);
+        d_y = dmalloc<double>(S.piv_size);
+        d_cv = dmalloc<double>(S.cvec_size);
+        TL_TRY(hipStreamSynchronize(stream));
+    }
+};
+
+namespace {
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+template <typename Fn>
+int tguard(pbte_trace* t, Fn&& fn)
+{
+    if (!t) return -1;
+    try {
+        return fn();
+    } catch (const std::string& e) {
+        t->err = e;
+        return -2;
+    }
+}
+}  // namespace
+
+extern "C" pbte_trace* pbte_trace_create(const pbte_trace_desc* desc, void* stream, char* err, int32_t errlen)
+{
+    auto* t = new pbte_trace();
+    try {
+        if (!desc) throw std::string("null descriptor");
+        t->stream = static_cast<hipStream_t>(stream);
+        build_symbolic(desc, t->S);
+        return t;
+    } catch (const std::string& e) {
+        if (err && errlen > 0) {
+            std::strncpy(err, e.c_str(), (size_t)errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        delete t;
+        return nullptr;
+    }
+}
+
+extern "C" void pbte_trace_destroy(pbte_trace* t) { delete t; }
+
+extern "C" const char* pbte_trace_last_error(pbte_trace* t) { return t ? t->err.c_str() : "null trace solver"; }
+
+extern "C" int pbte_trace_info(pbte_trace* t, int64_t* out8)
+{
+    return tguard(t, [&] {
+        const Symbolic& S = t->S;
+        int64_t ws = 0, nmax = 0;
+        for (size_t l = 0; l < S.levels.size(); ++l) nmax = std::max<int64_t>(nmax, S.levels[l].n_max);
+        int64_t wsz[2] = {0, 0}, pz = 0;
+        for (size_t l = 0; l < S.levels.size(); ++l) {
+            wsz[l & 1] = std::max(wsz[l & 1], S.levels[l].ws_size);
+            pz = std::max(pz, S.levels[l].pan_size);
+        }
+        ws = wsz[0] + wsz[1] + pz;
+        out8[0] = (int64_t)S.fronts.size();
+        out8[1] = (int64_t)S.levels.size();
+        out8[2] = nmax;
+        out8[3] = S.fac_size;
+        out8[4] = S.flops;
+        out8[5] = ws * 8;
+        out8[6] = (int64_t)S.F * S.b;
+        out8[7] = (int64_t)S.rec.size();
+        return 0;
+    });
+}
+
+extern "C" int pbte_trace_factor(pbte_trace* t, const double* bsr_val)
+{
+    return tguard(t, [&] {
+        t->alloc();
+        Symbolic& S = t->S;
+        hipStream_t st = t->stream;
+        const int big = 0x7fffffff;
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, t->d_status, big);
+        for (int l = S.max_depth; l >= 0; --l) {
+            const Level& L = S.levels[l];
+            const int nf = (int)L.nodes.size();
+            const int32_t* nodes = t->d_nodes + L.node_off;
+            double* ws = t->d_ws[l & 1];
+            double* wsc = t->d_ws[(l + 1) & 1];
+            hipLaunchKernelGGL(k_fill, dim3(std::min<int64_t>(cdiv(L.ws_size, 256), 65536)), dim3(256), 0, st, ws,
+                               L.ws_size);
+            if (L.rec_n > 0) {
+                const int64_t tot = (int64_t)L.rec_n * S.b * S.b;
+                hipLaunchKernelGGL(k_assemble, dim3(std::min<int64_t>(cdiv(tot, 256), 65536)), dim3(256), 0, st,
+                                   t->d_rec + L.rec_off, L.rec_n, bsr_val, S.b, t->d_fr, ws);
+            }
+            if (L.has_children)
+                for (int s = 0; s < 2; ++s)
+                    hipLaunchKernelGGL(k_extend, dim3(nf, cdiv(L.ncb_child_max, 8)), dim3(256), 0, st, nodes, s,
+                                       t->d_fr, t->d_fmap, S.b, ws, wsc);
+            const int tpr = cdiv(L.n_max, TILE);
+            for (int j0 = 0; j0 < L.nfs_max; j0 += KB) {
+                hipLaunchKernelGGL(k_panel, dim3(nf), dim3(1024), 0, st, nodes, t->d_fr, ws, t->d_pan, j0,
+                                   t->d_ipiv, t->d_perm, t->d_status);
+                hipLaunchKernelGGL(k_swap_rows, dim3(nf, cdiv(L.n_max, 256)), dim3(256), 0, st, nodes, t->d_fr, ws,
+                                   t->d_pan, j0, t->d_ipiv);
+                hipLaunchKernelGGL(k_update, dim3(tpr * tpr, nf), dim3(256), 0, st, nodes, t->d_fr, ws, t->d_pan, j0,
+                                   tpr);
+            }
+            hipLaunchKernelGGL(k_extract, dim3(nf, cdiv(L.n_max, 4)), dim3(256), 0, st, nodes, t->d_fr, ws,
+                               t->d_fac);
+        }
+        TL_TRY(hipGetLastError());
+        int status = big;
+        TL_TRY(hipMemcpyAsync(&status, t->d_status, sizeof(int), hipMemcpyDeviceToHost, st));
+        TL_TRY(hipStreamSynchronize(st));
+        if (status != big) {
+            t->err = "trace solve: the trace matrix is singular (a front without a nonzero pivot)";
+            t->factored = false;
+            return 1;
+        }
+        t->factored = true;
+        return 0;
+    });
+}
+
+extern "C" int pbte_trace_solve(pbte_trace* t, const double* rhs, double* x)
+{
+    return tguard(t, [&] {
+        if (!t->factored) throw std::string("trace solve: pbte_trace_factor has not succeeded");
+        Symbolic& S = t->S;
+        hipStream_t st = t->stream;
+        for (int l = S.max_depth; l >= 0; --l) {
+            const Level& L = S.levels[l];
+            const int nf = (int)L.nodes.size();
+            hipLaunchKernelGGL(k_fwd_gather, dim3(nf), dim3(256), (size_t)L.n_max * 8, st, t->d_nodes + L.node_off,
+                               t->d_fr, t->d_faces, t->d_fmap, t->d_perm, S.b, rhs, t->d_y, t->d_cv);
+            if (L.fw_n > 0)
+                hipLaunchKernelGGL(k_fwd_gemv, dim3(L.fw_n), dim3(256), 0, st, t->d_fw + L.fw_off, t->d_fr, t->d_fac,
+                                   t->d_y, t->d_cv);
+        }
+        for (int l = 0; l <= S.max_depth; ++l) {
+            const Level& L = S.levels[l];
+            if (L.bw_n > 0)
+                hipLaunchKernelGGL(k_bwd, dim3(L.bw_n), dim3(256), (size_t)L.n_max * 8, st, t->d_bw + L.bw_off,
+                                   t->d_fr, t->d_faces, S.b, t->d_fac, t->d_y, x);
+        }
+        TL_TRY(hipGetLastError());
+        return 0;
+    });
+}

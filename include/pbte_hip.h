@@ -325,6 +325,49 @@ int pbte_gsis_correct(pbte_ctx *ctx, int32_t precision, void *vdf, const double 
                       const double *uq, const double *defect, double *ts, double *qxs,
                       double *qys, double *temp, double *qx, double *qy, double *mkin);
 
+/* ---- HDG global trace solve on the device ------------------------------
+ * K u = f for the trace matrix of the HDG synthetic system, by a
+ * nested-dissection multifrontal elimination in binary64 (dense fronts,
+ * partial pivoting among each front's fully summed rows).  Replaces the
+ * SuperLU factorisation and solve of GlobalSolver
+ * (acceleration/global_solve.py:120-124 factorise, :190-195 solve). */
+typedef struct pbte_trace pbte_trace;
+
+typedef struct pbte_trace_desc {
+    int32_t n_faces;              /* F                                        */
+    int32_t n_cells;              /* N                                        */
+    int32_t blk;                  /* unknowns per face, 3Q                    */
+    int32_t leaf_cells;           /* cells per dissection leaf, 0 = 16        */
+    const int32_t *face_t1;       /* [F] first element a face couples         */
+    const int32_t *face_t2;       /* [F] second one (periodic partner's
+                                     element across a periodic pair), -1 on
+                                     walls                                    */
+    const int32_t *tri_faces;     /* [N][3]                                   */
+    const double *cell_xy;        /* [N][2] element centroids                 */
+    const int32_t *bsr_ptr;       /* [F+1] block rows of K (3Q x 3Q blocks)   */
+    const int32_t *bsr_col;       /* [nnzb] block columns (face ids)          */
+} pbte_trace_desc;
+
+/* Symbolic phase on the host (the dissection tree, fronts, assembly maps);
+ * device memory is taken by the first pbte_trace_factor.  Every later call
+ * is ordered on `stream`.  NULL on failure with the message in err. */
+pbte_trace *pbte_trace_create(const pbte_trace_desc *desc, void *stream, char *err,
+                              int32_t errlen);
+void pbte_trace_destroy(pbte_trace *t);
+const char *pbte_trace_last_error(pbte_trace *t);
+
+/* out8: tree nodes, levels, largest front (unknowns), factor entries,
+ * factorisation flops, device workspace bytes, unknowns, assembly blocks. */
+int pbte_trace_info(pbte_trace *t, int64_t *out8);
+
+/* Numeric factorisation from the block values bsr_val (device,
+ * [nnzb][3Q][3Q] row-major, the pattern given to pbte_trace_create).
+ * Returns 0, or 1 when a front has no nonzero pivot (singular K). */
+int pbte_trace_factor(pbte_trace *t, const double *bsr_val);
+
+/* x = K^-1 rhs (device, [F][3Q] each; x must not alias rhs). */
+int pbte_trace_solve(pbte_trace *t, const double *rhs, double *x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,56 @@
+"""Device trace solve (multifrontal, binary64) against SciPy's SuperLU with
+the reference's own call (global_solve.py:120-124, COLAMD) on the trace
+matrices of the test meshes: thermalising, adiabatic and periodic walls,
+degrees 1-3, a jittered mesh, two dissection leaf sizes."""
+import numpy as np
+import pytest
+
+from conftest import MESHES
+from trace_helpers import trace_system
+
+pytestmark = pytest.mark.gpu
+
+CASES_TL = [("F1_1_1.yaml", 1, None), ("F1_1_1.yaml", 2, None), ("F1_1_1.yaml", 3, None),
+            ("F2_1_1.yaml", 2, None), ("F3_1_1.yaml", 2, None), ("F3_1_1.yaml", 3, None),
+            ("F1_1_1.yaml", 2, MESHES / "J_Nx6_jit25_s0.msh")]
+
+
+@pytest.mark.parametrize("case,deg,mesh", CASES_TL,
+                         ids=lambda v: str(v if not hasattr(v, "name") else v.name))
+@pytest.mark.parametrize("leaf", [4, 16])
+def test_trace_solve_matches_superlu(pkg, case, deg, mesh, leaf):
+    import torch
+    from scipy.sparse.linalg import splu
+
+    from pbte_amd.trace_solve import TraceSolver, block_pattern
+    print(f"case trace solve {case} deg {deg} mesh {mesh} leaf {leaf}", flush=True)
+    pb, K, t1, t2, blk = trace_system(pkg, case, deg, mesh=mesh)
+    ptr, col, val = block_pattern(K, blk)
+    ts = TraceSolver(pb.mesh, t1, t2, ptr, col, blk, leaf_cells=leaf)
+    ts.factor(val)
+    lu = splu(K.tocsc(), permc_spec="COLAMD")
+    rng = np.random.default_rng(7)
+    for _ in range(3):
+        b = rng.standard_normal(K.shape[0])
+        bd = torch.from_numpy(b).cuda()
+        xd = torch.empty_like(bd)
+        ts.solve(bd, xd)
+        x = xd.cpu().numpy()
+        xr = lu.solve(b)
+        err = np.abs(x - xr).max() / np.abs(xr).max()
+        res = np.abs(K @ x - b).max() / np.abs(b).max()
+        print(f"  rel diff {err:.2e} residual {res:.2e}", flush=True)
+        assert err <= 1e-12
+        assert res <= 1e-13
+
+
+def test_trace_solve_singular_raises(pkg):
+    from pbte_amd.trace_solve import TraceSolver, block_pattern
+    pb, K, t1, t2, blk = trace_system(pkg, "F1_1_1.yaml", 1)
+    ptr, col, val = block_pattern(K, blk)
+    val = val.copy()
+    # zero the block row of one face: its unknowns have no pivot
+    val[ptr[5]:ptr[6]] = 0.0
+    ts = TraceSolver(pb.mesh, t1, t2, ptr, col, blk, leaf_cells=8)
+    with pytest.raises(np.linalg.LinAlgError):
+        ts.factor(val)

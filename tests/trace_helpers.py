@@ -1,0 +1,28 @@
+"""Trace matrices of the HDG synthetic system on the test meshes (host)."""
+import numpy as np
+
+from conftest import CASES, MESHES
+
+
+def trace_system(pkg, case="F1_1_1.yaml", deg=2, tau_r=1e-2, tau_n=1e-3, mesh=None):
+    from pbte_amd.hdg import assemble_global_matrix, build_ba_sol, build_local_operators, face_sides
+    from pbte_amd.problem import build_problem
+    c = pkg.Case.from_yaml(CASES / case)
+    c.scheme.accflag = 1
+    c.dg.deg = deg
+    c.flow.tau_r, c.flow.tau_n = tau_r, tau_n
+    if mesh is not None:
+        c.mesh.file = str(mesh)
+    pb = build_problem(c)
+    st = np.asarray(c.scheme.stabilisation, dtype=float).copy()
+    if c.scheme.scale_by_hmin:
+        st[1] /= pb.mesh.hmin
+        st[2] /= pb.mesh.hmin
+    _, aa_trace = build_local_operators(pb.mesh, pb.integrals, c.ndof_tri, c.ndof_fc, c.flow.cv,
+                                        c.flow.tau_r, c.tau_c, st, c.scheme.acc_variant)
+    ba = build_ba_sol(pb.mesh, pb.integrals, c.ndof_tri, c.ndof_fc, st, pb.bcdata.bc_type,
+                      c.scheme.acc_variant)
+    K, _ = assemble_global_matrix(pb.mesh, pb.integrals, ba, aa_trace, c.ndof_fc, pb.bcdata.bc_type,
+                                  drop_tol=1e-30, periodic_flip=pb.bcdata.periodic_flip)
+    t1, _, t2, _, _ = face_sides(pb.mesh, pb.bcdata.bc_type)
+    return pb, K, t1, t2, 3 * c.ndof_fc
