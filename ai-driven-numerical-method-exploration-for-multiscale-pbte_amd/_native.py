@@ -41,7 +41,7 @@ class PbteSetup(C.Structure):
 
 class PbteGsisData(C.Structure):
     _fields_ = [("cxv", _f64p), ("cyv", _f64p), ("domega", _f64p),
-                ("inv_aa_sol", _f64p), ("aa_trace", _f64p), ("ba_sol", _f64p),
+                ("inv_aa_sol", _f64p), ("aa_trace", _f64p), ("ba_sol", _f64p),     # device
                 ("tri_fc_fc", _f64p), ("tri_hmin", _f64p), ("tri_faces", _i32p),
                 ("t1", _i32p), ("l1", _i32p), ("t2", _i32p), ("l2", _i32p),
                 ("flip", C.POINTER(C.c_uint8)), ("n_wall", C.c_int32), ("n_wall_th", C.c_int32),
@@ -55,6 +55,15 @@ class PbteTraceDesc(C.Structure):
     _fields_ = [("n_faces", C.c_int32), ("n_cells", C.c_int32), ("blk", C.c_int32),
                 ("leaf_cells", C.c_int32), ("face_t1", _i32p), ("face_t2", _i32p),
                 ("tri_faces", _i32p), ("cell_xy", _f64p), ("bsr_ptr", _i32p), ("bsr_col", _i32p)]
+
+
+class PbteHdgDesc(C.Structure):
+    _fields_ = [("n_tris", C.c_int32), ("nd", C.c_int32), ("nf", C.c_int32), ("n_faces", C.c_int32),
+                ("variant_b", C.c_int32), ("cv", C.c_double), ("tau_r", C.c_double),
+                ("tau_c", C.c_double), ("st", C.c_double * 3), ("drop_tol", C.c_double),
+                ("int_tri_tri", C.c_void_p), ("int_tri_tri_x", C.c_void_p), ("int_tri_tri_y", C.c_void_p),
+                ("int_tri_fc", C.c_void_p), ("int_tri_fc_fc", C.c_void_p), ("int_fc_fc", C.c_void_p),
+                ("tri_normal", C.c_void_p), ("face_kind", C.c_void_p)]
 
 
 _SIGS = {
@@ -106,11 +115,16 @@ _SIGS = {
     "pbte_gsis_source": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbte_gsis_rhs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbte_gsis_local": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
-    "pbte_gsis_precond_setup": (C.c_int, [C.c_void_p, _f64p]),
+    "pbte_gsis_precond_setup": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pbte_gsis_precond_src": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "pbte_gsis_correct": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_hdg_local": (C.c_int, [C.POINTER(PbteHdgDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_char_p, C.c_int32]),
+    "pbte_hdg_assemble": (C.c_int, [C.POINTER(PbteHdgDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, _i64p, C.c_void_p, C.c_char_p, C.c_int32]),
     "pbte_trace_create": (C.c_void_p, [C.POINTER(PbteTraceDesc), C.c_void_p, C.c_char_p, C.c_int32]),
     "pbte_trace_destroy": (None, [C.c_void_p]),
     "pbte_trace_last_error": (C.c_char_p, [C.c_void_p]),

@@ -24,12 +24,99 @@ import ctypes as C
 import numpy as np
 
 from . import _native
-from .constants import BC_NONTHERMALISING, BC_THERMALISING, PI
-from .device import dev_zeros, ptr, stream_handle, torch
-from .hdg import assemble_global_matrix, build_ba_sol, build_local_operators, face_sides
-from .trace_solve import TraceSolver, block_pattern
+from .constants import BC_NONTHERMALISING, BC_PERIODIC, BC_THERMALISING, PI
+from .device import dev_zeros, ptr, stream_handle, to_dev, torch
+from .trace_solve import TraceSolver
 
-__all__ = ["Acceleration"]
+__all__ = ["Acceleration", "build_hdg_device", "face_sides", "trace_pattern"]
+
+
+def face_sides(mesh, bc_type):
+    """Per face the (element, local face) of each side that contributes to
+    its trace row, ``-1`` where absent (``global_solve.py:89-106``): an
+    interior face has its two elements; a periodic face its own element
+    and its partner's; a wall face its own element only."""
+    bc = mesh.face_bc
+    code = np.where(bc >= 0, np.asarray(bc_type)[np.clip(bc, 0, None)], 0)
+    interior = bc < 0
+    periodic = ~interior & (code == BC_PERIODIC)
+    own_t, own_l = mesh.adjacent_arrays()
+    t1 = np.where(interior, mesh.face_tri[:, 0], own_t).astype(np.int64)
+    l1 = np.where(interior, mesh.face_lfc[:, 0], own_l).astype(np.int64)
+    t2 = np.where(interior, mesh.face_tri[:, 1], -1).astype(np.int64)
+    l2 = np.where(interior, mesh.face_lfc[:, 1], -1).astype(np.int64)
+    partner = np.where(periodic, mesh.face_pair, 0).astype(np.int64)
+    t2 = np.where(periodic, own_t[partner], t2)
+    l2 = np.where(periodic, own_l[partner], l2)
+    return t1, l1, t2, l2, periodic
+
+
+def trace_pattern(tri_faces, t1, t2):
+    """Block pattern of the trace matrix: row f couples to the faces of the
+    elements on its sides (its own among them), ascending."""
+    F = t1.size
+    cols = np.concatenate([tri_faces[t1], np.where(t2[:, None] >= 0, tri_faces[np.maximum(t2, 0)], -1)],
+                          axis=1)
+    cols = np.sort(cols, axis=1)
+    keep = cols >= 0
+    keep[:, 1:] &= cols[:, 1:] != cols[:, :-1]
+    counts = keep.sum(axis=1)
+    bptr = np.zeros(F + 1, dtype=np.int32)
+    np.cumsum(counts, out=bptr[1:])
+    return bptr, np.ascontiguousarray(cols[keep], dtype=np.int32)
+
+
+def build_hdg_device(c, mesh, ig, bcdata, st, sides, stream):
+    """The HDG operators and the trace matrix built on the device
+    (``pbte_hdg_local``, ``pbte_hdg_assemble``; ``assembly.py:44-268``):
+    a dict of device tensors ``inv_aa_sol [N,7P,7P]``, ``aa_trace
+    [N,3,7P,3Q]``, ``ba_sol [N,3,3Q,7P]``, ``bsr_val [nnzb,3Q,3Q]``, the host
+    block pattern ``bsr_ptr``, ``bsr_col`` and the count ``nnz`` of nonzero
+    entries of K."""
+    t1, l1, t2, l2, periodic = sides
+    P, Q, N, F = c.ndof_tri, c.ndof_fc, mesh.n_tris, mesh.n_faces
+    bc_type = bcdata.bc_type
+    variant = c.scheme.acc_variant
+    code = np.where(mesh.face_bc >= 0, bc_type[np.clip(mesh.face_bc, 0, None)], 0)
+    fcode = code[mesh.tri_faces]
+    interior = mesh.face_bc[mesh.tri_faces] < 0
+    kind = np.where(interior | (fcode == BC_PERIODIC), 0,
+                    np.where(fcode == BC_THERMALISING, 1, 2)).astype(np.int32)
+    tens = [to_dev(np.ascontiguousarray(a, dtype=np.float64)) for a in
+            (ig.int_tri_tri, ig.int_tri_tri_x, ig.int_tri_tri_y, ig.int_tri_fc, ig.int_tri_fc_fc,
+             ig.int_fc_fc, mesh.tri_normal)]
+    kind_d = to_dev(kind)
+    hd = _native.PbteHdgDesc()
+    hd.n_tris, hd.nd, hd.nf, hd.n_faces = N, P, Q, F
+    hd.variant_b = 1 if variant == "B" else 0
+    hd.cv, hd.tau_r, hd.tau_c = c.flow.cv, c.flow.tau_r, c.tau_c
+    hd.st[:] = [float(v) for v in st]
+    hd.drop_tol = 1e-30 if variant == "A" else 0.0
+    (hd.int_tri_tri, hd.int_tri_tri_x, hd.int_tri_tri_y, hd.int_tri_fc, hd.int_tri_fc_fc,
+     hd.int_fc_fc, hd.tri_normal) = [ptr(t) for t in tens]
+    hd.face_kind = ptr(kind_d)
+    inv_aa_sol = dev_zeros((N, 7 * P, 7 * P))
+    aa_trace = dev_zeros((N, 3, 7 * P, 3 * Q))
+    ba_sol = dev_zeros((N, 3, 3 * Q, 7 * P))
+    err = C.create_string_buffer(512)
+    if _native.lib().pbte_hdg_local(C.byref(hd), ptr(inv_aa_sol), ptr(aa_trace), ptr(ba_sol), stream,
+                                    err, 512) < 0:
+        raise np.linalg.LinAlgError(err.value.decode())
+    mirror = (bcdata.periodic_flip & periodic).astype(np.uint8)
+    bptr, bcol = trace_pattern(mesh.tri_faces.astype(np.int64), t1, t2)
+    side_d = [to_dev(np.ascontiguousarray(a, dtype=np.int32)) for a in (t1, l1, t2, l2, mesh.tri_faces)]
+    mirror_d, bptr_d, bcol_d = to_dev(mirror), to_dev(bptr), to_dev(bcol)
+    blk = 3 * Q
+    bval = dev_zeros((bcol.size, blk, blk))
+    nnz = C.c_int64(0)
+    if _native.lib().pbte_hdg_assemble(C.byref(hd), *[ptr(t) for t in side_d[:4]], ptr(mirror_d),
+                                       ptr(side_d[4]), ptr(bptr_d), ptr(bcol_d), int(bcol.size),
+                                       ptr(aa_trace), ptr(ba_sol), ptr(bval), C.byref(nnz), stream,
+                                       err, 512) < 0:
+        raise _native.NativeError(f"hdg_assemble: {err.value.decode()}")
+    return {"inv_aa_sol": inv_aa_sol, "aa_trace": aa_trace, "ba_sol": ba_sol, "bsr_val": bval,
+            "bsr_ptr": bptr, "bsr_col": bcol, "nnz": int(nnz.value)}
+
 
 class Acceleration:
     def __init__(self, solver):
@@ -44,22 +131,20 @@ class Acceleration:
         self.st = st
         mesh, ig = s.mesh, s.integrals
         bc_type = s.bcdata.bc_type
-        inv_aa_sol, aa_trace = build_local_operators(mesh, ig, self.nd, self.nf, c.flow.cv,
-                                                     c.flow.tau_r, c.tau_c, st, self.variant)
-        # kept on the host for the Krylov moment preconditioner only
-        self.inv_aa_sol = inv_aa_sol if c.scheme.krylov_precond else None
-        ba_sol = build_ba_sol(mesh, ig, self.nd, self.nf, st, bc_type, self.variant)
-        drop = 1e-30 if self.variant == "A" else 0.0
-        self.K, self.nnz = assemble_global_matrix(mesh, ig, ba_sol, aa_trace, self.nf, bc_type,
-                                                  drop_tol=drop,
-                                                  periodic_flip=s.bcdata.periodic_flip)
+        P, N, F, Q = self.nd, s.n_tris, mesh.n_faces, self.nf
+        stream = C.c_void_p(stream_handle())
         t1, l1, t2, l2, periodic = face_sides(mesh, bc_type)
-        bptr, bcol, bval = block_pattern(self.K, 3 * self.nf)
-        self.trace = TraceSolver(mesh, t1, t2, bptr, bcol, 3 * self.nf, stream=C.c_void_p(stream_handle()),
+        ops = build_hdg_device(c, mesh, ig, s.bcdata, st, (t1, l1, t2, l2, periodic), stream)
+        inv_aa_sol, aa_trace, ba_sol = ops["inv_aa_sol"], ops["aa_trace"], ops["ba_sol"]
+        blk = 3 * Q
+        self.n_trace, self.nnz = F * blk, ops["nnz"]
+        self.trace = TraceSolver(mesh, t1, t2, ops["bsr_ptr"], ops["bsr_col"], blk, stream=stream,
                                  leaf_cells=c.performance.trace_leaf)
-        self.trace.factor(bval)
+        self.trace.factor(ops.pop("bsr_val"))
         self.trace_info = self.trace.info()
-        del bval
+        del ops
+        # the T, qx, qy columns of inv(AA_SOL) serve the moment preconditioner
+        self.inv_aa_sol = inv_aa_sol if c.scheme.krylov_precond else None
         flip = (s.bcdata.periodic_flip & periodic).astype(np.uint8)
         code = np.where(mesh.face_bc >= 0, bc_type[np.clip(mesh.face_bc, 0, None)], 0)
         wall_th = np.flatnonzero((mesh.face_bc >= 0) & (code == BC_THERMALISING))
@@ -82,7 +167,9 @@ class Acceleration:
         cxv, cyv, dw = s.vel.flat()
         g = _native.PbteGsisData()
         g.cxv, g.cyv, g.domega = f64(cxv), f64(cyv), f64(dw)
-        g.inv_aa_sol, g.aa_trace, g.ba_sol = f64(inv_aa_sol), f64(aa_trace), f64(ba_sol)
+        g.inv_aa_sol = C.cast(C.c_void_p(ptr(inv_aa_sol)), C.POINTER(C.c_double))
+        g.aa_trace = C.cast(C.c_void_p(ptr(aa_trace)), C.POINTER(C.c_double))
+        g.ba_sol = C.cast(C.c_void_p(ptr(ba_sol)), C.POINTER(C.c_double))
         g.tri_fc_fc, g.tri_hmin = f64(ig.int_tri_fc_fc), f64(mesh.tri_hmin)
         g.tri_faces = i32(mesh.tri_faces)
         g.t1, g.l1, g.t2, g.l2 = i32(t1), i32(l1), i32(t2), i32(l2)
@@ -100,8 +187,9 @@ class Acceleration:
         g.use_hmin = 1 if self.variant == "A" else 0
         ctx = s.ctx
         _native.check(ctx.handle, _native.lib().pbte_gsis_setup(ctx.handle, C.byref(g)), "gsis_setup")
+        torch.cuda.current_stream().synchronize()      # the copies out of the operator buffers
+        del aa_trace, ba_sol
 
-        P, N, F, Q = self.nd, s.n_tris, mesh.n_faces, self.nf
         self.mom7 = dev_zeros((7, P, N))
         self.aa_src_d = dev_zeros((7 * P, N))
         self.ffa_d = dev_zeros((F, 3 * Q))
@@ -188,14 +276,11 @@ class Acceleration:
         self.defect_d.copy_(torch.from_numpy(new.reshape(self.defect_d.shape)))
 
     def diagnostics(self) -> dict:
-        return {"acc_variant": self.variant, "global_matrix_n": int(self.K.shape[0]),
-                "global_matrix_nnz": int(self.K.nnz), "global_solve_count": int(self.solve_count),
+        return {"acc_variant": self.variant, "global_matrix_n": int(self.n_trace),
+                "global_matrix_nnz": int(self.nnz), "global_solve_count": int(self.solve_count),
                 "stabilisation": [float(v) for v in self.st], "defect_omega": float(self.omega),
                 "defect_every": int(self.defect_every), "defect_anderson": int(self.anderson),
                 "defect_outer": len(self.defect_steps),
                 "defect_norm": float(self.defect_d.abs().max()) if self.omega else 0.0,
                 "trace_solve": "device-multifrontal", "trace_ordering": "nested-dissection",
                 "trace_factor_entries": self.trace_info["factor_entries"]}
-
-    def global_matrix(self):
-        return self.K

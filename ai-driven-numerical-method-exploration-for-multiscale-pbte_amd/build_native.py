@@ -19,7 +19,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 LIB = PKG / "libpbte_hip.so"
 OBJDIR = ROOT / "_build"
-SOURCES = [PKG / "csrc" / "pbte_kernels.hip", PKG / "csrc" / "trace_lu.hip", PKG / "csrc" / "schedule.cpp"]
+SOURCES = [PKG / "csrc" / "pbte_kernels.hip", PKG / "csrc" / "trace_lu.hip", PKG / "csrc" / "hdg_setup.hip",
+           PKG / "csrc" / "schedule.cpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

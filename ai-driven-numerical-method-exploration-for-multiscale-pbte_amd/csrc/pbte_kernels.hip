@@ -2089,6 +2089,16 @@ __global__ void __launch_bounds__(256) pad_rows_kernel(const double* in, T* out,
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// out[r][k] = in[r][c0 + k], k < w, for rows of length ld
+__global__ void column_slice_kernel(const double* __restrict__ in, int64_t rows, int ld, int c0, int w,
+                                    double* __restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= rows * w) return;
+    const int64_t r = g / w;
+    out[g] = in[r * ld + c0 + (g - r * w)];
+}
+
 template <typename F>
 int guarded(pbte_ctx* ctx, F&& f)
 {
@@ -3688,15 +3698,17 @@ extern "C" int pbte_gsis_setup(pbte_ctx* ctx, const pbte_gsis_data* g)
             hw[3 * D + d] = cy * (5.0 * cy * cy - 3.0) * w;
         }
         ctx->gs_hw = ctx->upload(hw.data(), hw.size());
-        // stress columns of inv(AA_SOL): [N][7P][4P]
-        std::vector<double> inv4((size_t)N * 7 * P * 4 * P);
-        for (int i = 0; i < N; ++i)
-            for (int p = 0; p < 7 * P; ++p)
-                std::memcpy(&inv4[((size_t)i * 7 * P + p) * 4 * P], g->inv_aa_sol + ((size_t)i * 7 * P + p) * 7 * P + 3 * P,
-                            sizeof(double) * 4 * P);
-        ctx->gs_inv4 = ctx->upload(inv4.data(), inv4.size());
-        ctx->gs_aa_trace = ctx->upload(g->aa_trace, (size_t)N * 3 * 7 * P * 3 * Q);
-        ctx->gs_ba_sol = ctx->upload(g->ba_sol, (size_t)N * 3 * 3 * Q * 7 * P);
+        // stress columns of inv(AA_SOL) (device [N][7P][7P]): [N][7P][4P]
+        ctx->gs_inv4 = ctx->alloc<double>((size_t)N * 7 * P * 4 * P);
+        hipLaunchKernelGGL(column_slice_kernel, dim3(nblocks((int64_t)N * 7 * P * 4 * P, 256)), dim3(256), 0,
+                           ctx->stream, g->inv_aa_sol, (int64_t)N * 7 * P, 7 * P, 3 * P, 4 * P, ctx->gs_inv4);
+        ctx->gs_aa_trace = ctx->alloc<double>((size_t)N * 3 * 7 * P * 3 * Q);
+        ctx->gs_ba_sol = ctx->alloc<double>((size_t)N * 3 * 3 * Q * 7 * P);
+        HIP_TRY(hipMemcpyAsync(ctx->gs_aa_trace, g->aa_trace, sizeof(double) * N * 3 * 7 * P * 3 * Q,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->gs_ba_sol, g->ba_sol, sizeof(double) * N * 3 * 3 * Q * 7 * P,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipGetLastError());
         ctx->gs_t1 = ctx->upload(g->t1, F);
         ctx->gs_l1 = ctx->upload(g->l1, F);
         ctx->gs_t2 = ctx->upload(g->t2, F);
@@ -3797,13 +3809,11 @@ extern "C" int pbte_gsis_precond_setup(pbte_ctx* ctx, const double* inv_aa_sol)
     return guarded(ctx, [&]() {
         if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
         const int P = ctx->P, N = ctx->N;
-        std::vector<double> inv3((size_t)N * 7 * P * 3 * P);
-        for (int i = 0; i < N; ++i)
-            for (int p = 0; p < 7 * P; ++p)
-                std::memcpy(&inv3[((size_t)i * 7 * P + p) * 3 * P], inv_aa_sol + ((size_t)i * 7 * P + p) * 7 * P,
-                            sizeof(double) * 3 * P);
-        if (!ctx->gs_inv3) ctx->gs_inv3 = ctx->alloc<double>(inv3.size());
-        HIP_TRY(hipMemcpy(ctx->gs_inv3, inv3.data(), inv3.size() * sizeof(double), hipMemcpyHostToDevice));
+        // T, qx, qy columns of inv(AA_SOL) (device [N][7P][7P]): [N][7P][3P]
+        if (!ctx->gs_inv3) ctx->gs_inv3 = ctx->alloc<double>((size_t)N * 7 * P * 3 * P);
+        hipLaunchKernelGGL(column_slice_kernel, dim3(nblocks((int64_t)N * 7 * P * 3 * P, 256)), dim3(256), 0,
+                           ctx->stream, inv_aa_sol, (int64_t)N * 7 * P, 7 * P, 0, 3 * P, ctx->gs_inv3);
+        HIP_TRY(hipGetLastError());
         return 0;
     });
 }

@@ -274,8 +274,7 @@ class MomentPreconditioner:
         self.nd, self.nt = solver.ndof, solver.n_tris
         self.nmom = 3 * self.nd * self.nt
         ctx = solver.ctx
-        inv = np.ascontiguousarray(acc.inv_aa_sol, dtype=np.float64)
-        _native.check(ctx.handle, _native.lib().pbte_gsis_precond_setup(ctx.handle, _native.f64(inv)),
+        _native.check(ctx.handle, _native.lib().pbte_gsis_precond_setup(ctx.handle, ptr(acc.inv_aa_sol)),
                       "gsis_precond_setup")
         self.aa_src = dev_zeros((7 * self.nd, self.nt))
         self.ffa = dev_zeros(acc.ffa_d.shape)

@@ -87,7 +87,6 @@ class TraceSolver:
             values = torch.from_numpy(np.ascontiguousarray(values, dtype=np.float64)).to("cuda")
         if values.numel() != self.nnzb * self.blk * self.blk:
             raise ValueError("block values do not match the pattern")
-        self._values = values
         st = self._check(self._lib.pbte_trace_factor(self.handle, ptr(values)), "trace_factor")
         if st > 0:
             msg = self._lib.pbte_trace_last_error(self.handle).decode()

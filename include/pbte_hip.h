@@ -264,13 +264,13 @@ int pbte_vec_axpby(pbte_ctx *ctx, double a, const double *x, double b, double *y
 
 /* ---- GSIS with the HDG synthetic system (acceleration package) -------- */
 
-/* Host arrays read once by pbte_gsis_setup (index orders as the
+/* Arrays read once by pbte_gsis_setup, host unless marked device (index orders as the
  * reference's Acceleration, acceleration/__init__.py:37-92). */
 typedef struct pbte_gsis_data {
     const double *cxv, *cyv, *domega;     /* [D]                            */
-    const double *inv_aa_sol;             /* [N][7P][7P]                    */
-    const double *aa_trace;               /* [N][3][7P][3Q]                 */
-    const double *ba_sol;                 /* [N][3][3Q][7P]                 */
+    const double *inv_aa_sol;             /* device [N][7P][7P]             */
+    const double *aa_trace;               /* device [N][3][7P][3Q]          */
+    const double *ba_sol;                 /* device [N][3][3Q][7P]          */
     const double *tri_fc_fc;              /* int_tri_fc_fc [P][Q][N][3]     */
     const double *tri_hmin;               /* [N]                            */
     const int32_t *tri_faces;             /* [N][3]                         */
@@ -304,7 +304,7 @@ int pbte_gsis_source(pbte_ctx *ctx, const double *mom7, double *aa_src);
 int pbte_gsis_rhs(pbte_ctx *ctx, int32_t precision, const void *vdf, const double *aa_src,
                   double *ffa);
 
-/* Uploads the T, qx, qy columns of inv_aa_sol [N][7P][7P] (host) for the
+/* Copies the T, qx, qy columns of inv_aa_sol [N][7P][7P] (device) for the
  * moment preconditioner.  Part of MomentPreconditioner.__init__
  * (krylov.py:379-390). */
 int pbte_gsis_precond_setup(pbte_ctx *ctx, const double *inv_aa_sol);
@@ -324,6 +324,44 @@ int pbte_gsis_local(pbte_ctx *ctx, const double *aa_src, const double *u_trace, 
 int pbte_gsis_correct(pbte_ctx *ctx, int32_t precision, void *vdf, const double *mom7,
                       const double *uq, const double *defect, double *ts, double *qxs,
                       double *qys, double *temp, double *qx, double *qy, double *mkin);
+
+/* ---- HDG operators built on the device ---------------------------------
+ * The element operators and the trace matrix of the HDG synthetic system.
+ * Replace acceleration/assembly.py:44-268 (build_local_operators,
+ * build_ba_sol, assemble_global_matrix).  All pointers in the descriptor
+ * are device memory, binary64 tensors in the reference's index orders. */
+typedef struct pbte_hdg_desc {
+    int32_t n_tris, nd, nf, n_faces;      /* N, P, Q, F                      */
+    int32_t variant_b;                    /* 1: variant B (scaled by tau_R)  */
+    double cv, tau_r, tau_c;
+    double st[3];                         /* stabilisation (after any h_min
+                                             scaling)                        */
+    double drop_tol;                      /* |K| <= drop_tol set to zero     */
+    const double *int_tri_tri;            /* [P][P][N]                       */
+    const double *int_tri_tri_x, *int_tri_tri_y;   /* [N][P][P]             */
+    const double *int_tri_fc;             /* [N][3][P][P]                    */
+    const double *int_tri_fc_fc;          /* [P][Q][N][3]                    */
+    const double *int_fc_fc;              /* [Q][Q][F]                       */
+    const double *tri_normal;             /* [N][3][2]                       */
+    const int32_t *face_kind;             /* [N][3]: 0 interior or periodic,
+                                             1 thermalising wall, 2 other   */
+} pbte_hdg_desc;
+
+/* inv(AA_SOL) [N][7P][7P] (skipped when NULL), AA_TRACE [N][3][7P][3Q] and
+ * BA_SOL [N][3][3Q][7P] into device buffers.  0, or -2 with the message in
+ * err (a singular element operator among them). */
+int pbte_hdg_local(const pbte_hdg_desc *d, double *inv_aa_sol, double *aa_trace, double *ba_sol,
+                   void *stream, char *err, int32_t errlen);
+
+/* Block values [nnzb][3Q][3Q] of K on the block pattern (bsr_ptr [F+1],
+ * bsr_col [nnzb], device), from the element sides t1/l1, t2/l2 [F]
+ * (-1 absent), mirror [F] (partner side of a reversed periodic pair) and
+ * tri_faces [N][3] (device); *nnz_host = entries left nonzero. */
+int pbte_hdg_assemble(const pbte_hdg_desc *d, const int32_t *t1, const int32_t *l1,
+                      const int32_t *t2, const int32_t *l2, const uint8_t *mirror,
+                      const int32_t *tri_faces, const int32_t *bsr_ptr, const int32_t *bsr_col,
+                      int64_t nnzb, const double *aa_trace, const double *ba_sol, double *bsr_val,
+                      int64_t *nnz_host, void *stream, char *err, int32_t errlen);
 
 /* ---- HDG global trace solve on the device ------------------------------
  * K u = f for the trace matrix of the HDG synthetic system, by a

@@ -1,5 +1,9 @@
-"""HDG macroscopic synthetic system of GSIS: local operators, face operator
-and the sparse trace matrix (host, once per case).
+"""Host restatement of the HDG synthetic system of GSIS: local operators,
+face operator and the sparse trace matrix.
+
+TEST INFRASTRUCTURE ONLY.  The product builds these on the device
+(``csrc/hdg_setup.hip``); the tests compare that build, and the device trace
+solve, with this restatement and SciPy's SuperLU.
 
 Follows ``solver-python/pybte/acceleration/assembly.py:44-268``: seven fields
 per element DOF ``UQ = [T, qx, qy, Lxx, Lxy, Lyx, Lyy]``, three trace fields
@@ -10,12 +14,24 @@ momentum rows for variant A), and the trace matrix assembled block by block
 side, partner rows mirrored across a reversed periodic pair).  The face
 loop of the reference is vectorised over faces; duplicates are summed in
 the reference's block order (diagonal, side 1, side 2; local faces 0..2).
+
+Pinned by: the reference's FROZEN GSIS iteration counts and masses and its
+own GSIS runs with adiabatic walls and finite tau_N (tests/golden), reached
+through the device build, which matches this restatement entry by entry
+(tests/test_gpu_hdg_setup.py).
 """
 from __future__ import annotations
 
 import numpy as np
 
-from .constants import BC_PERIODIC, BC_THERMALISING
+import sys
+from pathlib import Path
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+from __graft_entry__ import load_package  # noqa: E402
+
+load_package()
+from pbte_amd.constants import BC_PERIODIC, BC_THERMALISING  # noqa: E402
 
 __all__ = ["build_local_operators", "build_ba_sol", "mirror_permutation",
            "assemble_global_matrix", "face_sides"]

@@ -5,7 +5,7 @@ from conftest import CASES, MESHES
 
 
 def trace_system(pkg, case="F1_1_1.yaml", deg=2, tau_r=1e-2, tau_n=1e-3, mesh=None):
-    from pbte_amd.hdg import assemble_global_matrix, build_ba_sol, build_local_operators, face_sides
+    from oracle.hdg import assemble_global_matrix, build_ba_sol, build_local_operators, face_sides
     from pbte_amd.problem import build_problem
     c = pkg.Case.from_yaml(CASES / case)
     c.scheme.accflag = 1
