@@ -120,6 +120,9 @@ _SIGS = {
     "pbte_gsis_correct": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_correct_moments": (C.c_int, [C.c_void_p] + [C.c_void_p] * 11),
+    "pbte_gsis_apply_correction": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_int32]),
     "pbte_hdg_local": (C.c_int, [C.POINTER(PbteHdgDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_char_p, C.c_int32]),
     "pbte_hdg_assemble": (C.c_int, [C.POINTER(PbteHdgDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -197,6 +197,19 @@ class Acceleration:
         self.uq_d = dev_zeros((7 * P, N))
         self.mkin_d = dev_zeros((3, P, N))
         self.defect_d = dev_zeros((3, P, N))
+        # The correction of the distribution is left pending: the next sweep
+        # writes every value of the distribution from the corrected moments,
+        # so only the values read before it is written need the correction --
+        # the elements behind adiabatic walls (their emission is computed
+        # first) and behind periodic faces (the sweep reads the partner's
+        # value of the previous iterate where the partner comes later in
+        # its order) -- and the distribution the run hands back.  Not with
+        # cycles the sweep breaks: those read the previous iterate anywhere.
+        self.lazy = int(s.order.cycles_broken.sum()) == 0
+        self.pending = False
+        self.coef_d = dev_zeros((3, P, N)) if self.lazy else None
+        read = np.flatnonzero((mesh.face_bc >= 0) & ((code == BC_NONTHERMALISING) | (code == BC_PERIODIC)))
+        self.boundary_cells_d = to_dev(np.unique(t1[read]).astype(np.int32)) if read.size else None
         self.omega = c.scheme.defect_omega
         self.defect_every = c.scheme.defect_every
         self.anderson = c.scheme.defect_anderson
@@ -222,13 +235,46 @@ class Acceleration:
                       "gsis_local")
         m = s.mom
         defect = ptr(self.defect_d) if self.omega else 0
-        _native.check(h, lib.pbte_gsis_correct(h, prec, ptr(vdf), ptr(self.mom7), ptr(self.uq_d),
-                                               defect, ptr(m.ts_d), ptr(m.qxs_d), ptr(m.qys_d),
-                                               ptr(m.temp_d), ptr(m.qx_d), ptr(m.qy_d),
-                                               ptr(self.mkin_d)), "gsis_correct")
+        if self.lazy:
+            _native.check(h, lib.pbte_gsis_correct_moments(h, ptr(self.mom7), ptr(self.uq_d), defect,
+                                                           ptr(m.ts_d), ptr(m.qxs_d), ptr(m.qys_d),
+                                                           ptr(m.temp_d), ptr(m.qx_d), ptr(m.qy_d),
+                                                           ptr(self.mkin_d), ptr(self.coef_d)),
+                          "gsis_correct_moments")
+            self.pending = True
+        else:
+            _native.check(h, lib.pbte_gsis_correct(h, prec, ptr(vdf), ptr(self.mom7), ptr(self.uq_d),
+                                                   defect, ptr(m.ts_d), ptr(m.qxs_d), ptr(m.qys_d),
+                                                   ptr(m.temp_d), ptr(m.qx_d), ptr(m.qy_d),
+                                                   ptr(self.mkin_d)), "gsis_correct")
         self._ncall += 1
         if self.omega and self.defect_every and self._ncall % self.defect_every == 0:
             self.refresh_defect()
+
+    def settle(self, boundary_only: bool = False) -> None:
+        """Apply the pending correction to the distribution: to every element,
+        or (``boundary_only``) to the elements behind adiabatic walls and
+        periodic faces, the only ones the next step reads before its sweep
+        rewrites them (the correction stays pending for the rest)."""
+        if not self.pending:
+            return
+        s = self.solver
+        vdf = s.sweep_vdf
+        prec = _native.PBTE_FP64 if vdf.dtype == torch.float64 else _native.PBTE_FP32
+        h = s.ctx.handle
+        if boundary_only:
+            cells = self.boundary_cells_d
+            if cells is not None:
+                _native.check(h, _native.lib().pbte_gsis_apply_correction(
+                    h, prec, ptr(vdf), ptr(self.coef_d), ptr(cells), int(cells.numel())), "gsis_apply_correction")
+            return
+        _native.check(h, _native.lib().pbte_gsis_apply_correction(h, prec, ptr(vdf), ptr(self.coef_d), None, 0),
+                      "gsis_apply_correction")
+        self.pending = False
+
+    def swept(self) -> None:
+        """The sweep has rewritten the distribution: nothing is pending."""
+        self.pending = False
 
     def trace_solve(self, ffa=None, out=None) -> None:
         """``K u = ffa`` on the device (``global_solve.py:190-195``); by

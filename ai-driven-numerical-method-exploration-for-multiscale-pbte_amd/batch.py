@@ -128,6 +128,8 @@ def run_source_batch(solvers, callback=None):
                 break
             for k in active:
                 s = tracks[k].s
+                if s.acc is not None:
+                    s.acc.settle(boundary_only=True)
                 if s.bcdata.has_nonthermalising:
                     s.ctx.wall_flux(s.sweep_vdf)
             groups = {}
@@ -137,6 +139,9 @@ def run_source_batch(solvers, callback=None):
             for group in groups.values():
                 batch_of(group).sweep([tracks[k].s.mom for k in group],
                                       [tracks[k].s.sweep_vdf for k in group])
+                for k in group:
+                    if tracks[k].s.acc is not None:
+                        tracks[k].s.acc.swept()
             for k in active:
                 t = tracks[k]
                 s = t.s
@@ -173,6 +178,8 @@ def run_source_batch(solvers, callback=None):
                     t.converged, t.done, t.wall = True, True, now
                 elif t.step >= t.tmax:
                     t.done, t.wall = True, now
+        for t in tracks:
+            t.s.settle()
         torch.cuda.synchronize()
     finally:
         for b in batches.values():

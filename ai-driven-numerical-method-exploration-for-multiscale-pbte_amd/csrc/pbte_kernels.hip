@@ -1454,33 +1454,58 @@ __global__ void __launch_bounds__(256) integrals_kernel(const double* ssraw, con
 // ==========================================================================
 // adiabatic wall emission (_kernels.py:363-397): one thread per (wall face, m)
 // ==========================================================================
-template <typename T>
-__global__ void __launch_bounds__(256) wall_flux_kernel(const T* __restrict__ vdf, const double* cxv,
-                                                        const double* cyv, const double* dw,
-                                                        const double* fcm, const double* nx3,
-                                                        const int32_t* wall_face, const int32_t* wall_tri,
-                                                        const int32_t* wall_lfc, int n_wall, int D,
-                                                        int N, int P, double* flux_wall)
+// sum over the 64 lanes of a wave, butterfly order (fixed, so the result
+// does not depend on the launch)
+__device__ inline double wave_sum64(double v)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n_wall * P) return;
-    const int k = g / P, m = g - (g / P) * P;
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Diffuse adiabatic wall emission (_kernels.py:364-397): one wave per wall
+// face, lanes over directions (each lane its directions d = lane + 64 j in
+// order), the lane partial sums combined by the butterfly; the P moments of
+// a face share each lane's loads of the element's values.
+template <typename T, int P>
+__global__ void __launch_bounds__(64) wall_flux_kernel(const T* __restrict__ vdf, const double* cxv,
+                                                       const double* cyv, const double* dw,
+                                                       const double* fcm, const double* nx3,
+                                                       const int32_t* wall_face, const int32_t* wall_tri,
+                                                       const int32_t* wall_lfc, int D, int N,
+                                                       double* flux_wall)
+{
+    const int k = blockIdx.x, lane = threadIdx.x;
     const int tri = wall_tri[k], lfc = wall_lfc[k];
     const double n1 = -nx3[2 * (tri * 3 + lfc)], n2 = -nx3[2 * (tri * 3 + lfc) + 1];
-    const double* fc = fcm + ((size_t)tri * 3 + lfc) * P * P + (size_t)m * P;
-    double ss1 = 0.0, ss2 = 0.0;
-    for (int d = 0; d < D; ++d) {
+    const double* fc = fcm + ((size_t)tri * 3 + lfc) * P * P;
+    double ss1[P], ss2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < P; ++m) ss1[m] = 0.0;
+    for (int d = lane; d < D; d += 64) {
         const double u = cxv[d] * n1 + cyv[d] * n2;
         if (u < 0.0) {
             const T* f = vdf + ((size_t)d * N + tri) * P;
-            double acc = 0.0;
-            for (int l = 0; l < P; ++l) acc += (double)f[l] * fc[l];
-            ss1 += u * dw[d] * acc;
+            double fv[P];
+#pragma unroll
+            for (int l = 0; l < P; ++l) fv[l] = (double)f[l];
+#pragma unroll
+            for (int m = 0; m < P; ++m) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < P; ++l) acc += fv[l] * fc[m * P + l];
+                ss1[m] += u * dw[d] * acc;
+            }
         } else {
             ss2 -= u * dw[d];
         }
     }
-    flux_wall[(size_t)wall_face[k] * P + m] = -ss1 / ss2;
+    ss2 = wave_sum64(ss2);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const double a = wave_sum64(ss1[m]);
+        if (lane == 0) flux_wall[(size_t)wall_face[k] * P + m] = -a / ss2;
+    }
 }
 
 // ==========================================================================
@@ -1824,103 +1849,169 @@ __global__ void __launch_bounds__(256) periodic_scatter_kernel(const double* __r
 // order: 1, cx, cy (as moments_kernel) and the four third-order weights of
 // the higher-order-term source, cx(5cx^2-3), cy(5cx^2-1), cx(5cy^2-1),
 // cy(5cy^2-3), each times dw (hot_source.py:29-58 summed over l first)
-template <typename T>
+template <typename T, int V>
 __global__ void __launch_bounds__(256) moments7_kernel(const T* __restrict__ vdf, const double* __restrict__ cxv,
                                                        const double* __restrict__ cyv,
                                                        const double* __restrict__ dw,
                                                        const double* __restrict__ hw, int D, int N, int P,
                                                        double* __restrict__ out)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= N * P) return;
-    const int i = g / P, l = g - (g / P) * P;
-    double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0, a6 = 0;
+    // V consecutive (cell, dof) values per thread, one 16-byte word per
+    // direction (V = 1: one value), U words in flight; each value's seven
+    // sums run over the directions in index order
+    using W = typename Vec16<T>::type;
+    constexpr int U = sizeof(T) == 4 ? 8 : 4;
+    const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * V;
+    if (g0 >= (int64_t)N * P) return;
+    double a[7][V];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+#pragma unroll
+        for (int e = 0; e < V; ++e) a[k][e] = 0.0;
+    auto acc = [&](const T* t, int d) {
+        const double w = dw[d], cx = cxv[d], cy = cyv[d];
+        const double h0 = hw[d], h1 = hw[D + d], h2 = hw[2 * D + d], h3 = hw[3 * D + d];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            const double f = (double)t[e];
+            a[0][e] += f * w;
+            a[1][e] += cx * f * w;
+            a[2][e] += cy * f * w;
+            a[3][e] += h0 * f;
+            a[4][e] += h1 * f;
+            a[5][e] += h2 * f;
+            a[6][e] += h3 * f;
+        }
+    };
     const size_t stride = (size_t)N * P;
-    for (int d = 0; d < D; ++d) {
-        const double f = (double)vdf[(size_t)d * stride + g];
-        const double w = dw[d];
-        a0 += f * w;
-        a1 += cxv[d] * f * w;
-        a2 += cyv[d] * f * w;
-        a3 += hw[d] * f;
-        a4 += hw[D + d] * f;
-        a5 += hw[2 * D + d] * f;
-        a6 += hw[3 * D + d] * f;
+    if constexpr (V * sizeof(T) == 16) {
+        const W* f = reinterpret_cast<const W*>(vdf) + g0 / V;
+        const size_t ws = stride / V;
+        int d = 0;
+        for (; d + U <= D; d += U) {
+            W x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) x[u] = f[(size_t)(d + u) * ws];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc(reinterpret_cast<const T*>(&x[u]), d + u);
+        }
+        for (; d < D; ++d) {
+            const W x = f[(size_t)d * ws];
+            acc(reinterpret_cast<const T*>(&x), d);
+        }
+    } else {
+        for (int d = 0; d < D; ++d) {
+            const T x = vdf[(size_t)d * stride + g0];
+            acc(&x, d);
+        }
     }
-    const size_t PN = (size_t)P * N, o = (size_t)l * N + i;
-    out[o] = a0;
-    out[PN + o] = a1;
-    out[2 * PN + o] = a2;
-    out[3 * PN + o] = a3;
-    out[4 * PN + o] = a4;
-    out[5 * PN + o] = a5;
-    out[6 * PN + o] = a6;
-}
-
-// stress rows of the HoT source per (cell, m) (hot_source.py:29-58)
-__global__ void __launch_bounds__(256) hot_work_kernel(const double* __restrict__ m7, const double* __restrict__ gx,
-                                                       const double* __restrict__ gy, int N, int P, double tc5,
-                                                       double* __restrict__ work)
-{
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= N * P) return;
-    const int i = g / P, m = g - (g / P) * P;
     const size_t PN = (size_t)P * N;
-    double pixx = 0, pixy = 0, piyy = 0;
-    for (int l = 0; l < P; ++l) {
-        const double ax = gx[((size_t)i * P + l) * P + m], ay = gy[((size_t)i * P + l) * P + m];
-        const double w1 = m7[3 * PN + (size_t)l * N + i], w2 = m7[4 * PN + (size_t)l * N + i];
-        const double w3 = m7[5 * PN + (size_t)l * N + i], w4 = m7[6 * PN + (size_t)l * N + i];
-        pixx += ax * w1 + ay * w2;
-        pixy += ax * w2 + ay * w3;
-        piyy += ax * w3 + ay * w4;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int64_t g = g0 + e;
+        const int i = (int)(g / P), l = (int)(g - (int64_t)i * P);
+        const size_t o = (size_t)l * N + i;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) out[k * PN + o] = a[k][e];
     }
-    double* o = work + (size_t)i * 4 * P;
-    o[m] = (pixx + 0.5 * piyy) * tc5;
-    o[P + m] = 0.5 * pixy * tc5;
-    o[2 * P + m] = 0.5 * pixy * tc5;
-    o[3 * P + m] = (0.5 * pixx + piyy) * tc5;
 }
 
-// AA_SRC[p][i] = sum_q inv(AA_SOL)[i][p][3P + q] work[i][q]  (hot_source.py:74)
-__global__ void __launch_bounds__(256) hot_apply_kernel(const double* __restrict__ inv4, const double* __restrict__ work,
-                                                        int N, int P, double* __restrict__ aa_src)
+// HoT source (hot_source.py:29-75), one wave per element: the stress rows
+// of the source per dof m (lanes m < P, l summed in order), then
+// AA_SRC[p][i] = sum_q inv(AA_SOL)[i][p][3P + q] work[q] (lanes over p, q in
+// order) from the transposed stress columns invT [N][4P][7P], so that the
+// lanes read consecutive words
+template <int P>
+__global__ void __launch_bounds__(256) hot_source_kernel(const double* __restrict__ m7, const double* __restrict__ gx,
+                                                         const double* __restrict__ gy,
+                                                         const double* __restrict__ invT, int N, double tc5,
+                                                         double* __restrict__ aa_src)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int R = 7 * P;
-    if (g >= N * R) return;
-    const int i = g / R, p = g - (g / R) * R;
-    const double* a = inv4 + ((size_t)i * R + p) * 4 * P;
-    const double* w = work + (size_t)i * 4 * P;
-    double s = 0.0;
-    for (int q = 0; q < 4 * P; ++q) s += a[q] * w[q];
-    aa_src[(size_t)p * N + i] = s;
+    __shared__ double s_work[4][4 * P];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wv;
+    const bool live = i < N;
+    const size_t PN = (size_t)P * N;
+    if (live && lane < P) {
+        const int m = lane;
+        double pixx = 0, pixy = 0, piyy = 0;
+        for (int l = 0; l < P; ++l) {
+            const double ax = gx[((size_t)i * P + l) * P + m], ay = gy[((size_t)i * P + l) * P + m];
+            const double w1 = m7[3 * PN + (size_t)l * N + i], w2 = m7[4 * PN + (size_t)l * N + i];
+            const double w3 = m7[5 * PN + (size_t)l * N + i], w4 = m7[6 * PN + (size_t)l * N + i];
+            pixx += ax * w1 + ay * w2;
+            pixy += ax * w2 + ay * w3;
+            piyy += ax * w3 + ay * w4;
+        }
+        double* o = s_work[wv];
+        o[m] = (pixx + 0.5 * piyy) * tc5;
+        o[P + m] = 0.5 * pixy * tc5;
+        o[2 * P + m] = 0.5 * pixy * tc5;
+        o[3 * P + m] = (0.5 * pixx + piyy) * tc5;
+    }
+    __syncthreads();
+    if (!live) return;
+    constexpr int R = 7 * P;
+    const double* a = invT + (size_t)i * 4 * P * R;
+    for (int p = lane; p < R; p += 64) {
+        double s = 0.0;
+        for (int q = 0; q < 4 * P; ++q) s += a[(size_t)q * R + p] * s_work[wv][q];
+        aa_src[(size_t)p * N + i] = s;
+    }
 }
 
-// wall trace moments (global_solve.py:31-53): per (wall face, face node)
-template <typename T>
-__global__ void __launch_bounds__(256) wall_trace_kernel(const T* __restrict__ vdf, const double* cxv, const double* cyv,
-                                                         const double* dw, const double* __restrict__ wtfc,
-                                                         const int32_t* wtri, int nw, int D, int N, int P, int Q,
-                                                         double* out)
+// out[b][c][r] = in[b][r][c0 + c] for c < w: blocks of `rows` rows of
+// length ld, column slices transposed
+__global__ void transpose_slice_kernel(const double* __restrict__ in, int64_t nblk, int rows, int ld, int c0, int w,
+                                       double* __restrict__ out)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nw * Q) return;
-    const int k = g / Q, m = g - (g / Q) * Q;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)rows * w;
+    if (g >= nblk * per) return;
+    const int64_t b = g / per;
+    const int e = (int)(g - b * per);
+    const int c = e / rows, r = e - (e / rows) * rows;
+    out[g] = in[(b * rows + r) * ld + c0 + c];
+}
+
+// wall trace moments (global_solve.py:31-53): one wave per wall face,
+// lanes over directions (d = lane + 64 j in order), the Q face nodes' three
+// moments combined over lanes by the butterfly
+template <typename T, int P, int Q>
+__global__ void __launch_bounds__(64) wall_trace_kernel(const T* __restrict__ vdf, const double* cxv, const double* cyv,
+                                                        const double* dw, const double* __restrict__ wtfc,
+                                                        const int32_t* wtri, int D, int N, double* out)
+{
+    const int k = blockIdx.x, lane = threadIdx.x;
     const int tri = wtri[k];
-    const double* c = wtfc + ((size_t)k * Q + m) * P;
-    double tb = 0, qxb = 0, qyb = 0;
-    for (int d = 0; d < D; ++d) {
+    const double* c = wtfc + (size_t)k * Q * P;
+    double tb[Q], qxb[Q], qyb[Q];
+#pragma unroll
+    for (int m = 0; m < Q; ++m) tb[m] = qxb[m] = qyb[m] = 0.0;
+    for (int d = lane; d < D; d += 64) {
         const T* f = vdf + ((size_t)d * N + tri) * P;
-        double t = 0.0;
-        for (int l = 0; l < P; ++l) t += (double)f[l] * c[l];
-        tb += t * dw[d];
-        qxb += cxv[d] * t * dw[d];
-        qyb += cyv[d] * t * dw[d];
+        double fv[P];
+#pragma unroll
+        for (int l = 0; l < P; ++l) fv[l] = (double)f[l];
+#pragma unroll
+        for (int m = 0; m < Q; ++m) {
+            double t = 0.0;
+#pragma unroll
+            for (int l = 0; l < P; ++l) t += fv[l] * c[m * P + l];
+            tb[m] += t * dw[d];
+            qxb[m] += cxv[d] * t * dw[d];
+            qyb[m] += cyv[d] * t * dw[d];
+        }
     }
-    out[(size_t)k * 3 * Q + m] = tb;
-    out[(size_t)k * 3 * Q + Q + m] = qxb;
-    out[(size_t)k * 3 * Q + 2 * Q + m] = qyb;
+#pragma unroll
+    for (int m = 0; m < Q; ++m) {
+        const double a = wave_sum64(tb[m]), b = wave_sum64(qxb[m]), e = wave_sum64(qyb[m]);
+        if (lane == 0) {
+            out[(size_t)k * 3 * Q + m] = a;
+            out[(size_t)k * 3 * Q + Q + m] = b;
+            out[(size_t)k * 3 * Q + 2 * Q + m] = e;
+        }
+    }
 }
 
 // trace right-hand side per (face, row) (global_solve.py:128-186)
@@ -2002,22 +2093,27 @@ __global__ void __launch_bounds__(256) precond_src_kernel(const double* __restri
     aa_src[(size_t)p * N + i] = s;
 }
 
-// UQ = AA_SRC + sum_il AA_TRACE[i][il] U_TRACE[face(i, il)]  (local_solve.py:16-26)
-__global__ void __launch_bounds__(256) local_solve_kernel(const double* __restrict__ aa_trace, const double* __restrict__ aa_src,
+// UQ = AA_SRC + sum_il AA_TRACE[i][il] U_TRACE[face(i, il)]  (local_solve.py:16-26):
+// one wave per element, lanes over the 7P rows, the transposed operator
+// aatT [N][3][3Q][7P] read in consecutive words, il then r in order
+__global__ void __launch_bounds__(256) local_solve_kernel(const double* __restrict__ aatT, const double* __restrict__ aa_src,
                                                           const double* __restrict__ u, const int32_t* tri_faces,
                                                           int N, int P, int Q, double* uq)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wv;
+    if (i >= N) return;
     const int R = 7 * P, C = 3 * Q;
-    if (g >= N * R) return;
-    const int i = g / R, p = g - (g / R) * R;
-    double s = 0.0;
-    for (int il = 0; il < 3; ++il) {
-        const double* a = aa_trace + (((size_t)i * 3 + il) * R + p) * C;
-        const double* x = u + (size_t)tri_faces[i * 3 + il] * C;
-        for (int r = 0; r < C; ++r) s += a[r] * x[r];
+    const int f0 = tri_faces[i * 3], f1 = tri_faces[i * 3 + 1], f2 = tri_faces[i * 3 + 2];
+    for (int p = lane; p < R; p += 64) {
+        double s = 0.0;
+        for (int il = 0; il < 3; ++il) {
+            const double* a = aatT + ((size_t)i * 3 + il) * C * R + p;
+            const double* x = u + (size_t)(il == 0 ? f0 : il == 1 ? f1 : f2) * C;
+            for (int r = 0; r < C; ++r) s += a[(size_t)r * R] * x[r];
+        }
+        uq[(size_t)p * N + i] = aa_src[(size_t)p * N + i] + s;
     }
-    uq[(size_t)p * N + i] = aa_src[(size_t)p * N + i] + s;
 }
 
 // damped kinetic correction (correction.py:29-75) per (cell, dof)
@@ -2055,6 +2151,85 @@ __global__ void __launch_bounds__(256) correct_kernel(T* vdf, const double* __re
     ts[o] = l_t + t_vdf;
     qxs[o] = l_qx + qx_vdf;
     qys[o] = l_qy + qy_vdf;
+}
+
+// The moment half of the kinetic correction (correction.py:29-75): the
+// corrected dofs, the kinetic moments mkin and, per (cell, dof), the
+// correction coefficients coef[3][P][N] = (eq, l_qx, l_qy) that
+// apply_correction_kernel adds to the distribution (the same expressions as
+// correct_kernel, so both give the same bits)
+__global__ void __launch_bounds__(256) correct_moments_kernel(const double* __restrict__ m7, const double* __restrict__ uq,
+                                                              const double* __restrict__ defect,
+                                                              const double* hmin, int N, int P, double cv, double pi,
+                                                              double tau_r, double tau_thr, int use_hmin,
+                                                              double* ts, double* qxs, double* qys, double* mkin,
+                                                              double* coef)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= N * P) return;
+    const int i = g / P, m = g - (g / P) * P;
+    const double tau_loc = use_hmin ? tau_r / hmin[i] : tau_r;
+    const double beta = fmin(tau_loc, tau_thr) / tau_loc;
+    const size_t PN = (size_t)P * N, o = (size_t)m * N + i;
+    const double t_vdf = m7[o] / cv, qx_vdf = m7[PN + o], qy_vdf = m7[2 * PN + o];
+    if (mkin) {
+        mkin[o] = t_vdf;
+        mkin[PN + o] = qx_vdf;
+        mkin[2 * PN + o] = qy_vdf;
+    }
+    const double d0 = defect ? defect[o] : 0.0, d1 = defect ? defect[PN + o] : 0.0,
+                 d2 = defect ? defect[2 * PN + o] : 0.0;
+    const double l_t = (uq[o] - d0 - t_vdf) * beta;
+    const double l_qx = (uq[PN + o] - d1 - qx_vdf) * beta;
+    const double l_qy = (uq[2 * PN + o] - d2 - qy_vdf) * beta;
+    coef[o] = l_t * cv / 4.0 / pi;
+    coef[PN + o] = l_qx;
+    coef[2 * PN + o] = l_qy;
+    ts[o] = l_t + t_vdf;
+    qxs[o] = l_qx + qx_vdf;
+    qys[o] = l_qy + qy_vdf;
+}
+
+// The distribution half: f += eq + (cx l_qx + cy l_qy) drift for every
+// direction, on every cell (cells == NULL; V consecutive values per thread,
+// one 16-byte word) or on the listed cells (V = 1, one (cell, dof) per thread)
+template <typename T, int V>
+__global__ void __launch_bounds__(256) apply_correction_kernel(T* vdf, const double* __restrict__ coef,
+                                                               const double* cxv, const double* cyv,
+                                                               const int32_t* __restrict__ cells, int ncells,
+                                                               int D, int N, int P, double drift)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = cells ? (int64_t)ncells * P : (int64_t)N * P / V;
+    if (t >= total) return;
+    const size_t PN = (size_t)P * N, stride = PN;
+    int64_t g0;
+    if (cells) {
+        const int k = (int)(t / P), m = (int)(t - (int64_t)k * P);
+        g0 = (int64_t)cells[k] * P + m;
+    } else {
+        g0 = t * V;
+    }
+    double eq[V], lx[V], ly[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int64_t g = g0 + e;
+        const int i = (int)(g / P), m = (int)(g - (int64_t)i * P);
+        const size_t o = (size_t)m * N + i;
+        eq[e] = coef[o];
+        lx[e] = coef[PN + o];
+        ly[e] = coef[2 * PN + o];
+    }
+    for (int d = 0; d < D; ++d) {
+        T* f = vdf + (size_t)d * stride + g0;
+        T x[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) x[e] = f[e];
+#pragma unroll
+        for (int e = 0; e < V; ++e) x[e] = (T)((double)x[e] + eq[e] + (cxv[d] * lx[e] + cyv[d] * ly[e]) * drift);
+#pragma unroll
+        for (int e = 0; e < V; ++e) f[e] = x[e];
+    }
 }
 
 // element integrals of corrected dofs: temp[i] = sum_m ts[m][i] int_tri[m][i]
@@ -3397,15 +3572,22 @@ extern "C" int pbte_wall_flux(pbte_ctx* ctx, int32_t precision, const void* vdf,
 {
     return guarded(ctx, [&]() {
         if (!ctx->n_wall) return 0;
-        const unsigned g = nblocks((int64_t)ctx->n_wall * ctx->P, 256);
-        if (precision == PBTE_FP64)
-            hipLaunchKernelGGL(wall_flux_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
-                               ctx->cxv, ctx->cyv, ctx->domega, ctx->fcm, ctx->nrm, ctx->wall_face,
-                               ctx->wall_tri, ctx->wall_lfc, ctx->n_wall, ctx->D, ctx->N, ctx->P, flux_wall);
-        else
-            hipLaunchKernelGGL(wall_flux_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (const float*)vdf,
-                               ctx->cxv, ctx->cyv, ctx->domega, ctx->fcm, ctx->nrm, ctx->wall_face,
-                               ctx->wall_tri, ctx->wall_lfc, ctx->n_wall, ctx->D, ctx->N, ctx->P, flux_wall);
+#define WF(PP)                                                                                          \
+    case PP:                                                                                            \
+        if (precision == PBTE_FP64)                                                                     \
+            hipLaunchKernelGGL((wall_flux_kernel<double, PP>), dim3(ctx->n_wall), dim3(64), 0, ctx->stream, \
+                               (const double*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->fcm, ctx->nrm,  \
+                               ctx->wall_face, ctx->wall_tri, ctx->wall_lfc, ctx->D, ctx->N, flux_wall); \
+        else                                                                                            \
+            hipLaunchKernelGGL((wall_flux_kernel<float, PP>), dim3(ctx->n_wall), dim3(64), 0, ctx->stream, \
+                               (const float*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->fcm, ctx->nrm,   \
+                               ctx->wall_face, ctx->wall_tri, ctx->wall_lfc, ctx->D, ctx->N, flux_wall); \
+        break;
+        switch (ctx->P) {
+            WF(3) WF(6) WF(10) WF(15)
+            default: throw std::string("wall_flux: unsupported P");
+        }
+#undef WF
         HIP_TRY(hipGetLastError());
         return 0;
     });
@@ -3699,13 +3881,15 @@ extern "C" int pbte_gsis_setup(pbte_ctx* ctx, const pbte_gsis_data* g)
         }
         ctx->gs_hw = ctx->upload(hw.data(), hw.size());
         // stress columns of inv(AA_SOL) (device [N][7P][7P]): [N][7P][4P]
+        // transposed stress columns invT [N][4P][7P]
         ctx->gs_inv4 = ctx->alloc<double>((size_t)N * 7 * P * 4 * P);
-        hipLaunchKernelGGL(column_slice_kernel, dim3(nblocks((int64_t)N * 7 * P * 4 * P, 256)), dim3(256), 0,
-                           ctx->stream, g->inv_aa_sol, (int64_t)N * 7 * P, 7 * P, 3 * P, 4 * P, ctx->gs_inv4);
+        hipLaunchKernelGGL(transpose_slice_kernel, dim3(nblocks((int64_t)N * 7 * P * 4 * P, 256)), dim3(256), 0,
+                           ctx->stream, g->inv_aa_sol, (int64_t)N, 7 * P, 7 * P, 3 * P, 4 * P, ctx->gs_inv4);
+        // transposed AA_TRACE [N][3][3Q][7P]
         ctx->gs_aa_trace = ctx->alloc<double>((size_t)N * 3 * 7 * P * 3 * Q);
+        hipLaunchKernelGGL(transpose_slice_kernel, dim3(nblocks((int64_t)N * 3 * 7 * P * 3 * Q, 256)), dim3(256), 0,
+                           ctx->stream, g->aa_trace, (int64_t)N * 3, 7 * P, 3 * Q, 0, 3 * Q, ctx->gs_aa_trace);
         ctx->gs_ba_sol = ctx->alloc<double>((size_t)N * 3 * 3 * Q * 7 * P);
-        HIP_TRY(hipMemcpyAsync(ctx->gs_aa_trace, g->aa_trace, sizeof(double) * N * 3 * 7 * P * 3 * Q,
-                               hipMemcpyDeviceToDevice, ctx->stream));
         HIP_TRY(hipMemcpyAsync(ctx->gs_ba_sol, g->ba_sol, sizeof(double) * N * 3 * 3 * Q * 7 * P,
                                hipMemcpyDeviceToDevice, ctx->stream));
         HIP_TRY(hipGetLastError());
@@ -3730,7 +3914,6 @@ extern "C" int pbte_gsis_setup(pbte_ctx* ctx, const pbte_gsis_data* g)
         ctx->gs_wtfc = ctx->upload(wt.data(), wt.size());
         ctx->gs_hmin = ctx->upload(g->tri_hmin, N);
         ctx->gs_tri_faces = ctx->upload(g->tri_faces, (size_t)N * 3);
-        ctx->gs_work = ctx->alloc<double>((size_t)N * 4 * P);
         ctx->gs_wmom = ctx->alloc<double>((size_t)std::max(g->n_wall, 1) * 3 * Q);
         ctx->gs_tc5 = g->tc5;
         ctx->gs_tau_thr = g->tau_thr;
@@ -3746,13 +3929,26 @@ extern "C" int pbte_gsis_moments(pbte_ctx* ctx, int32_t precision, const void* v
 {
     return guarded(ctx, [&]() {
         if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
-        const unsigned g = nblocks((int64_t)ctx->N * ctx->P, 256);
-        if (precision == PBTE_FP64)
-            hipLaunchKernelGGL(moments7_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
-                               ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_hw, ctx->D, ctx->N, ctx->P, mom7);
-        else
-            hipLaunchKernelGGL(moments7_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (const float*)vdf,
-                               ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_hw, ctx->D, ctx->N, ctx->P, mom7);
+        const int64_t np = (int64_t)ctx->N * ctx->P;
+        if (precision == PBTE_FP64) {
+            if (np % 2 == 0)
+                hipLaunchKernelGGL((moments7_kernel<double, 2>), dim3(nblocks(np / 2, 256)), dim3(256), 0,
+                                   ctx->stream, (const double*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_hw,
+                                   ctx->D, ctx->N, ctx->P, mom7);
+            else
+                hipLaunchKernelGGL((moments7_kernel<double, 1>), dim3(nblocks(np, 256)), dim3(256), 0, ctx->stream,
+                                   (const double*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_hw, ctx->D, ctx->N,
+                                   ctx->P, mom7);
+        } else {
+            if (np % 4 == 0)
+                hipLaunchKernelGGL((moments7_kernel<float, 4>), dim3(nblocks(np / 4, 256)), dim3(256), 0,
+                                   ctx->stream, (const float*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_hw,
+                                   ctx->D, ctx->N, ctx->P, mom7);
+            else
+                hipLaunchKernelGGL((moments7_kernel<float, 1>), dim3(nblocks(np, 256)), dim3(256), 0, ctx->stream,
+                                   (const float*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_hw, ctx->D, ctx->N,
+                                   ctx->P, mom7);
+        }
         HIP_TRY(hipGetLastError());
         return 0;
     });
@@ -3763,11 +3959,16 @@ extern "C" int pbte_gsis_source(pbte_ctx* ctx, const double* mom7, double* aa_sr
     return guarded(ctx, [&]() {
         if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
         const int N = ctx->N, P = ctx->P;
-        hipLaunchKernelGGL(hot_work_kernel, dim3(nblocks((int64_t)N * P, 256)), dim3(256), 0, ctx->stream, mom7,
-                           ctx->gx, ctx->gy, N, P, ctx->gs_tc5, ctx->gs_work);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(hot_apply_kernel, dim3(nblocks((int64_t)N * 7 * P, 256)), dim3(256), 0, ctx->stream,
-                           ctx->gs_inv4, ctx->gs_work, N, P, aa_src);
+#define HS(PP)                                                                                            \
+    case PP:                                                                                              \
+        hipLaunchKernelGGL(hot_source_kernel<PP>, dim3((N + 3) / 4), dim3(256), 0, ctx->stream, mom7, ctx->gx, \
+                           ctx->gy, ctx->gs_inv4, N, ctx->gs_tc5, aa_src);                               \
+        break;
+        switch (P) {
+            HS(3) HS(6) HS(10) HS(15)
+            default: throw std::string("gsis_source: unsupported P");
+        }
+#undef HS
         HIP_TRY(hipGetLastError());
         return 0;
     });
@@ -3787,14 +3988,24 @@ extern "C" int pbte_gsis_rhs(pbte_ctx* ctx, int32_t precision, const void* vdf, 
             if (!vdf)
                 hipLaunchKernelGGL(fill_kernel, dim3(nblocks((int64_t)ctx->gs_nw * 3 * Q, 256)), dim3(256), 0,
                                    ctx->stream, ctx->gs_wmom, (int64_t)ctx->gs_nw * 3 * Q, 0.0);
-            else if (precision == PBTE_FP64)
-                hipLaunchKernelGGL(wall_trace_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, (const double*)vdf,
-                                   ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_wtfc, ctx->gs_wtri, ctx->gs_nw, ctx->D,
-                                   N, P, Q, ctx->gs_wmom);
-            else
-                hipLaunchKernelGGL(wall_trace_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, (const float*)vdf,
-                                   ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_wtfc, ctx->gs_wtri, ctx->gs_nw, ctx->D,
-                                   N, P, Q, ctx->gs_wmom);
+            else {
+#define WT3(PP, QQ)                                                                                        \
+    case PP:                                                                                               \
+        if (precision == PBTE_FP64)                                                                        \
+            hipLaunchKernelGGL((wall_trace_kernel<double, PP, QQ>), dim3(ctx->gs_nw), dim3(64), 0, ctx->stream, \
+                               (const double*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_wtfc, ctx->gs_wtri, \
+                               ctx->D, N, ctx->gs_wmom);                                                   \
+        else                                                                                               \
+            hipLaunchKernelGGL((wall_trace_kernel<float, PP, QQ>), dim3(ctx->gs_nw), dim3(64), 0, ctx->stream, \
+                               (const float*)vdf, ctx->cxv, ctx->cyv, ctx->domega, ctx->gs_wtfc, ctx->gs_wtri, \
+                               ctx->D, N, ctx->gs_wmom);                                                   \
+        break;
+                switch (P) {
+                    WT3(3, 2) WT3(6, 3) WT3(10, 4) WT3(15, 5)
+                    default: throw std::string("gsis_rhs: unsupported P");
+                }
+#undef WT3
+            }
             HIP_TRY(hipGetLastError());
             hipLaunchKernelGGL(trace_wall_kernel, dim3(g), dim3(256), 0, ctx->stream, ctx->gs_wmom, ctx->gs_wface,
                                ctx->gs_wnx, ctx->gs_wny, ctx->gs_nw, ctx->gs_nth, Q, ctx->cv, ctx->gs_variant_b, ffa);
@@ -3834,7 +4045,7 @@ extern "C" int pbte_gsis_local(pbte_ctx* ctx, const double* aa_src, const double
     return guarded(ctx, [&]() {
         if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
         const int N = ctx->N, P = ctx->P;
-        hipLaunchKernelGGL(local_solve_kernel, dim3(nblocks((int64_t)N * 7 * P, 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(local_solve_kernel, dim3((N + 3) / 4), dim3(256), 0, ctx->stream,
                            ctx->gs_aa_trace, aa_src, u_trace, ctx->gs_tri_faces, N, P, ctx->Q, uq);
         HIP_TRY(hipGetLastError());
         return 0;
@@ -3860,6 +4071,61 @@ extern "C" int pbte_gsis_correct(pbte_ctx* ctx, int32_t precision, void* vdf, co
         HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL(dof_integrals_kernel, dim3(nblocks(N, 256)), dim3(256), 0, ctx->stream, ts, qxs, qys,
                            ctx->int_tri, N, P, temp, qx, qy);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_correct_moments(pbte_ctx* ctx, const double* mom7, const double* uq, const double* defect,
+                                         double* ts, double* qxs, double* qys, double* temp, double* qx, double* qy,
+                                         double* mkin, double* coef)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
+        const int N = ctx->N, P = ctx->P;
+        hipLaunchKernelGGL(correct_moments_kernel, dim3(nblocks((int64_t)N * P, 256)), dim3(256), 0, ctx->stream,
+                           mom7, uq, defect, ctx->gs_hmin, N, P, ctx->cv, ctx->pi, ctx->tau_r, ctx->gs_tau_thr,
+                           ctx->gs_use_hmin, ts, qxs, qys, mkin, coef);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(dof_integrals_kernel, dim3(nblocks(N, 256)), dim3(256), 0, ctx->stream, ts, qxs, qys,
+                           ctx->int_tri, N, P, temp, qx, qy);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_apply_correction(pbte_ctx* ctx, int32_t precision, void* vdf, const double* coef,
+                                          const int32_t* cells, int32_t ncells)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
+        const int N = ctx->N, P = ctx->P, D = ctx->D;
+        const int64_t np = (int64_t)N * P;
+        const double dr = ctx->gs_drift;
+        if (cells) {
+            if (ncells <= 0) return 0;
+            const unsigned g = nblocks((int64_t)ncells * P, 256);
+            if (precision == PBTE_FP64)
+                hipLaunchKernelGGL((apply_correction_kernel<double, 1>), dim3(g), dim3(256), 0, ctx->stream,
+                                   (double*)vdf, coef, ctx->cxv, ctx->cyv, cells, ncells, D, N, P, dr);
+            else
+                hipLaunchKernelGGL((apply_correction_kernel<float, 1>), dim3(g), dim3(256), 0, ctx->stream,
+                                   (float*)vdf, coef, ctx->cxv, ctx->cyv, cells, ncells, D, N, P, dr);
+        } else if (precision == PBTE_FP64) {
+            if (np % 2 == 0)
+                hipLaunchKernelGGL((apply_correction_kernel<double, 2>), dim3(nblocks(np / 2, 256)), dim3(256), 0,
+                                   ctx->stream, (double*)vdf, coef, ctx->cxv, ctx->cyv, nullptr, 0, D, N, P, dr);
+            else
+                hipLaunchKernelGGL((apply_correction_kernel<double, 1>), dim3(nblocks(np, 256)), dim3(256), 0,
+                                   ctx->stream, (double*)vdf, coef, ctx->cxv, ctx->cyv, nullptr, 0, D, N, P, dr);
+        } else {
+            if (np % 4 == 0)
+                hipLaunchKernelGGL((apply_correction_kernel<float, 4>), dim3(nblocks(np / 4, 256)), dim3(256), 0,
+                                   ctx->stream, (float*)vdf, coef, ctx->cxv, ctx->cyv, nullptr, 0, D, N, P, dr);
+            else
+                hipLaunchKernelGGL((apply_correction_kernel<float, 1>), dim3(nblocks(np, 256)), dim3(256), 0,
+                                   ctx->stream, (float*)vdf, coef, ctx->cxv, ctx->cyv, nullptr, 0, D, N, P, dr);
+        }
         HIP_TRY(hipGetLastError());
         return 0;
     });

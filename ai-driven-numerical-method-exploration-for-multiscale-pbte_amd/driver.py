@@ -153,6 +153,7 @@ class Solver:
     def promote(self) -> None:
         """Continue with binary64 sweeps from the binary32 distribution."""
         if self.sweep_precision == "fp32":
+            self.settle()
             self.vdf_d.copy_(self.vdf32_d)
             self.sweep_precision = "fp64"
 
@@ -161,6 +162,7 @@ class Solver:
         """The distribution ``[D, N, P]`` (binary64) as a host array
         (``HostVdf``): refreshed from HBM on access, or, when it was written
         on the host since, uploaded to HBM instead."""
+        self.settle()
         m = self._mirror
         if m.array is None:
             m.array = np.empty((self.ndir, self.n_tris, self.ndof)).view(HostVdf)
@@ -174,6 +176,8 @@ class Solver:
 
     @vdf.setter
     def vdf(self, value) -> None:
+        if self.acc is not None:
+            self.acc.swept()          # replaced: a pending correction no longer applies
         self.sweep_vdf.copy_(torch.as_tensor(np.ascontiguousarray(value)).to(self.sweep_vdf.dtype))
         self._mirror.dirty = False
 
@@ -241,11 +245,21 @@ class Solver:
         return path
 
     # -- one outer iteration (driver.py:149-161) -------------------------------
+    def settle(self) -> None:
+        """Bring a pending GSIS correction into the distribution (the
+        accelerated step leaves it pending; see ``Acceleration.settle``)."""
+        if self.acc is not None:
+            self.acc.settle()
+
     def step(self) -> None:
         v = self.sweep_vdf
+        if self.acc is not None:
+            self.acc.settle(boundary_only=True)
         if self.bcdata.has_nonthermalising:
             self.ctx.wall_flux(v)
         self.ctx.sweep(self.mom, v)
+        if self.acc is not None:
+            self.acc.swept()
         if self.acc is None:
             self.ctx.moments(v, self.mom)
         else:
@@ -279,6 +293,8 @@ class Solver:
             residuals.append(res)
             masses.append(mass)
             if want_true:
+                if step % every == 0:
+                    self.settle()
                 true_res.append(self.ctx.true_residual(self.mom, self.sweep_vdf)
                                 if step % every == 0 else np.nan)
             if progress_every and step % progress_every == 0:
@@ -304,6 +320,7 @@ class Solver:
                 break
             if step >= tmax:
                 break
+        self.settle()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         extra = None
@@ -355,6 +372,7 @@ class Solver:
         are not exactly those of the corrected distribution), summed in
         HBM by the moment kernel: one copy of ``3 P N`` values to the
         host, not of the distribution."""
+        self.settle()
         c = self.case
         out = c.output_path()
         out.mkdir(parents=True, exist_ok=True)

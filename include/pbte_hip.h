@@ -325,6 +325,21 @@ int pbte_gsis_correct(pbte_ctx *ctx, int32_t precision, void *vdf, const double 
                       const double *uq, const double *defect, double *ts, double *qxs,
                       double *qys, double *temp, double *qx, double *qy, double *mkin);
 
+/* The correction split in two halves, so that a caller can leave the
+ * distribution untouched when the next sweep overwrites it anyway:
+ * pbte_gsis_correct_moments writes the corrected moments (and mkin) like
+ * pbte_gsis_correct and, instead of updating vdf, the coefficients
+ * coef [3][P][N] (device) of the distribution's update;
+ * pbte_gsis_apply_correction applies them to vdf, on every cell (cells ==
+ * NULL) or on the ncells listed (device int32).  Both halves in turn give
+ * the bits of pbte_gsis_correct.  Replace correction.correct_vdf
+ * (correction.py:29-98) together. */
+int pbte_gsis_correct_moments(pbte_ctx *ctx, const double *mom7, const double *uq,
+                              const double *defect, double *ts, double *qxs, double *qys,
+                              double *temp, double *qx, double *qy, double *mkin, double *coef);
+int pbte_gsis_apply_correction(pbte_ctx *ctx, int32_t precision, void *vdf, const double *coef,
+                               const int32_t *cells, int32_t ncells);
+
 /* ---- HDG operators built on the device ---------------------------------
  * The element operators and the trace matrix of the HDG synthetic system.
  * Replace acceleration/assembly.py:44-268 (build_local_operators,
