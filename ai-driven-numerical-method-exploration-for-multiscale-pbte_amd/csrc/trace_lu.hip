@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -56,7 +57,7 @@ struct Front {                // one tree node, device and host
     int64_t ws;               // front matrix offset in its level's workspace (doubles)
     int64_t zv;               // [Z V] rows (nfs x n) in the factor storage
     int64_t w;                // W rows (ncb x nfs) in the factor storage
-    int64_t pan;              // panel scratch (n x KB) and row copy (KB x n) offset
+    int64_t pan;              // panel scratch (n x KB), row copy (KB x n), block inverse (KB x KB)
     int32_t n, nfs, ncb;
     int32_t faces;            // offset of the front's faces (own, then ancestors')
     int32_t piv;              // offset of nfs pivot rows / forward values y
@@ -215,6 +216,175 @@ __global__ void __launch_bounds__(1024) k_panel(const int32_t* __restrict__ node
         if (tid == 0) P[(int64_t)r0 * KB + k] = inv;
         __syncthreads();
     }
+}
+
+// Diagonal-block pivoting (the fast path): the KB x KB diagonal block of
+// the step, inverted in LDS by Gauss-Jordan with interchanges among the
+// block's own rows, stored as the exact inverse (interchanges folded back
+// into its columns), so that no row of the front moves.  One workgroup per
+// front; a block without a nonzero pivot sets the status.
+__global__ void __launch_bounds__(256) k_block_inv(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
+                                                   const double* __restrict__ ws, double* __restrict__ pan, int j0,
+                                                   int32_t* __restrict__ perm, int32_t* __restrict__ status)
+{
+    __shared__ double A[KB][KB + 1];
+    __shared__ int sp[KB];
+    __shared__ int s_p;
+    __shared__ double s_v;
+    const Front f = fr[nodes[blockIdx.x]];
+    if (j0 >= f.nfs) return;
+    const int kbe = min(KB, f.nfs - j0), tid = threadIdx.x, n = f.n;
+    const double* M = ws + f.ws;
+    if (j0 == 0)
+        for (int k = tid; k < f.nfs; k += blockDim.x) perm[f.piv + k] = k;
+    for (int e = tid; e < KB * KB; e += blockDim.x) {
+        const int r = e / KB, c = e - (e / KB) * KB;
+        A[r][c] = (r < kbe && c < kbe) ? M[(int64_t)(j0 + r) * n + j0 + c] : (r == c ? 1.0 : 0.0);
+    }
+    if (tid < KB) sp[tid] = tid;
+    __syncthreads();
+    for (int k = 0; k < kbe; ++k) {
+        if (tid < 64) {
+            // pivot among rows k..kbe-1 of the block: one wave, butterfly on (|a|, row)
+            double v = -1.0;
+            int r = k;
+            if (tid >= k && tid < kbe) {
+                v = fabs(A[tid][k]);
+                r = tid;
+            }
+#pragma unroll
+            for (int m = 32; m > 0; m >>= 1) {
+                const double ov = __shfl_xor(v, m, 64);
+                const int orr = __shfl_xor(r, m, 64);
+                if (ov > v || (ov == v && orr < r)) {
+                    v = ov;
+                    r = orr;
+                }
+            }
+            if (tid == 0) {
+                s_p = r;
+                s_v = v;
+            }
+        }
+        __syncthreads();
+        const int p = s_p;
+        if (!(s_v > 0.0)) {
+            if (tid == 0) atomicMin(status, f.faces);
+            return;
+        }
+        if (p != k) {
+            for (int c = tid; c < KB; c += blockDim.x) {
+                const double t = A[k][c];
+                A[k][c] = A[p][c];
+                A[p][c] = t;
+            }
+            if (tid == 0) {
+                const int t = sp[k];
+                sp[k] = sp[p];
+                sp[p] = t;
+            }
+        }
+        __syncthreads();
+        const double inv = 1.0 / A[k][k];
+        for (int e = tid; e < KB * KB; e += blockDim.x) {
+            const int r = e / KB, c = e - (e / KB) * KB;
+            if (r == k || c == k) continue;
+            A[r][c] = fma(-(A[r][k] * inv), A[k][c], A[r][c]);
+        }
+        __syncthreads();
+        if (tid < KB) {
+            if (tid != k) {
+                A[k][tid] *= inv;
+                A[tid][k] = -A[tid][k] * inv;
+            } else {
+                A[k][k] = inv;
+            }
+        }
+        __syncthreads();
+    }
+    // inverse of the block: Ainv[r][sp[c]] = A[r][c]
+    double* Ai = pan + f.pan + 2 * (int64_t)n * KB;
+    for (int e = tid; e < KB * KB; e += blockDim.x) {
+        const int r = e / KB, c = e - (e / KB) * KB;
+        Ai[r * KB + sp[c]] = A[r][c];
+    }
+}
+
+// The panel of the step from the block inverse: pivot rows P[J] = Ainv,
+// every other row P[i] = -M[i][J] Ainv (the sweep of the panel columns on
+// the block).  blockIdx.x = front, blockIdx.y = chunk of 64 rows.
+__global__ void __launch_bounds__(256) k_panel_apply(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
+                                                     const double* __restrict__ ws, double* __restrict__ pan, int j0)
+{
+    __shared__ double Ai[KB][KB];
+    const Front f = fr[nodes[blockIdx.x]];
+    if (j0 >= f.nfs) return;
+    const int n = f.n, kbe = min(KB, f.nfs - j0), tid = threadIdx.x;
+    const int i0 = blockIdx.y * 64;
+    if (i0 >= n) return;
+    const double* M = ws + f.ws;
+    double* P = pan + f.pan;
+    const double* Ag = pan + f.pan + 2 * (int64_t)n * KB;
+    for (int e = tid; e < KB * KB; e += blockDim.x) Ai[e / KB][e % KB] = Ag[e];
+    __syncthreads();
+    // thread: one row, 8 of the KB columns (4 threads per row... 64 rows x 4)
+    const int r = i0 + (tid >> 2), cq = (tid & 3) * (KB / 4);
+    if (r >= n) return;
+    double out[KB / 4];
+    if (r >= j0 && r < j0 + kbe) {
+#pragma unroll
+        for (int c = 0; c < KB / 4; ++c) out[c] = Ai[r - j0][cq + c];
+    } else {
+#pragma unroll
+        for (int c = 0; c < KB / 4; ++c) out[c] = 0.0;
+        const double* row = M + (int64_t)r * n + j0;
+        for (int k = 0; k < kbe; ++k) {
+            const double x = row[k];
+#pragma unroll
+            for (int c = 0; c < KB / 4; ++c) out[c] = fma(-x, Ai[k][cq + c], out[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < KB / 4; ++c) P[(int64_t)r * KB + cq + c] = cq + c < kbe ? out[c] : 0.0;
+}
+
+// The pivot rows (outside the panel; zero inside it) copied to the row
+// scratch, no interchanges.  blockIdx.x = front, blockIdx.y = 256 columns.
+__global__ void k_copy_rows(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
+                            const double* __restrict__ ws, double* __restrict__ pan, int j0)
+{
+    const Front f = fr[nodes[blockIdx.x]];
+    if (j0 >= f.nfs) return;
+    const int c = blockIdx.y * blockDim.x + threadIdx.x;
+    if (c >= f.n) return;
+    const int kbe = min(KB, f.nfs - j0), n = f.n;
+    const double* M = ws + f.ws;
+    double* T = pan + f.pan + (int64_t)n * KB;
+    const bool in_panel = c >= j0 && c < j0 + kbe;
+    for (int k = 0; k < KB; ++k)
+        T[(int64_t)k * n + c] = (in_panel || k >= kbe) ? 0.0 : M[(int64_t)(j0 + k) * n + c];
+}
+
+// Residual of the factorisation on a test vector: y = K x - b per row, and
+// the row sums of |K| (block rows of the pattern)
+__global__ void k_bsr_residual(const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                               const double* __restrict__ val, int F, int b, const double* __restrict__ x,
+                               const double* __restrict__ rhs, double* __restrict__ res, double* __restrict__ rowabs)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)F * b) return;
+    const int f = (int)(g / b), i = (int)(g - (int64_t)f * b);
+    double s = 0.0, a = 0.0;
+    for (int e = ptr[f]; e < ptr[f + 1]; ++e) {
+        const double* blk = val + ((int64_t)e * b + i) * b;
+        const double* xv = x + (int64_t)col[e] * b;
+        for (int j = 0; j < b; ++j) {
+            s = fma(blk[j], xv[j], s);
+            a += fabs(blk[j]);
+        }
+    }
+    res[g] = s - rhs[g];
+    rowabs[g] = a;
 }
 
 // Row interchanges of the step on the columns outside the panel, then the
@@ -418,6 +588,7 @@ struct Symbolic {
     std::vector<int32_t> face_list, fmap, node_list;
     std::vector<int4> rec;
     std::vector<int2> fw_items, bw_items;
+    std::vector<int32_t> bsr_ptr, bsr_col;
     int64_t fac_size = 0, piv_size = 0, cvec_size = 0;
     int64_t flops = 0;
     int max_depth = 0;
@@ -438,6 +609,8 @@ void build_symbolic(const pbte_trace_desc* d, Symbolic& S)
     S.F = F;
     S.N = N;
     S.b = b;
+    S.bsr_ptr.assign(d->bsr_ptr, d->bsr_ptr + F + 1);
+    S.bsr_col.assign(d->bsr_col, d->bsr_col + d->bsr_ptr[F]);
     // tree by recursive coordinate bisection; cells permuted so that every
     // node holds a contiguous range
     std::vector<int32_t> cell(N);
@@ -550,7 +723,7 @@ void build_symbolic(const pbte_trace_desc* d, Symbolic& S)
         f.ws = L.ws_size;
         L.ws_size += (int64_t)f.n * f.n;
         f.pan = L.pan_size;
-        L.pan_size += 2 * (int64_t)f.n * KB;
+        L.pan_size += 2 * (int64_t)f.n * KB + KB * KB;
         L.nodes.push_back(v);
         L.n_max = std::max(L.n_max, f.n);
         L.nfs_max = std::max(L.nfs_max, f.nfs);
@@ -636,6 +809,9 @@ struct pbte_trace {
     hipStream_t stream = nullptr;
     std::string err;
     bool factored = false;
+    int pivoting = 0;         // 0 diagonal-block pivots, 1 partial pivoting
+    int force_pivoting = -1;  // -1 automatic, else the mode to use
+    double berr = 0.0;        // backward error of the factors on the test vector
     // device
     Front* d_fr = nullptr;
     int32_t *d_faces = nullptr, *d_fmap = nullptr, *d_nodes = nullptr, *d_perm = nullptr, *d_ipiv = nullptr,
@@ -724,6 +900,25 @@ extern "C" void pbte_trace_destroy(pbte_trace* t) { delete t; }
 
 extern "C" const char* pbte_trace_last_error(pbte_trace* t) { return t ? t->err.c_str() : "null trace solver"; }
 
+extern "C" int pbte_trace_options(pbte_trace* t, int32_t pivoting)
+{
+    return tguard(t, [&] {
+        if (pivoting < -1 || pivoting > 1) throw std::string("trace solve: pivoting must be -1, 0 or 1");
+        t->force_pivoting = pivoting;
+        return 0;
+    });
+}
+
+extern "C" int pbte_trace_quality(pbte_trace* t, int32_t* pivoting, double* berr)
+{
+    return tguard(t, [&] {
+        if (!t->factored) throw std::string("trace solve: not factorised");
+        *pivoting = t->pivoting;
+        *berr = t->berr;
+        return 0;
+    });
+}
+
 extern "C" int pbte_trace_info(pbte_trace* t, int64_t* out8)
 {
     return tguard(t, [&] {
@@ -748,79 +943,180 @@ extern "C" int pbte_trace_info(pbte_trace* t, int64_t* out8)
     });
 }
 
-extern "C" int pbte_trace_factor(pbte_trace* t, const double* bsr_val)
+namespace {
+// numeric phase with diagonal-block pivoting (block_pivots) or with partial
+// pivoting among each front's fully summed rows; returns the status word
+int factor_pass(pbte_trace* t, const double* bsr_val, bool block_pivots)
 {
-    return tguard(t, [&] {
-        t->alloc();
-        Symbolic& S = t->S;
-        hipStream_t st = t->stream;
-        const int big = 0x7fffffff;
-        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, t->d_status, big);
-        for (int l = S.max_depth; l >= 0; --l) {
-            const Level& L = S.levels[l];
-            const int nf = (int)L.nodes.size();
-            const int32_t* nodes = t->d_nodes + L.node_off;
-            double* ws = t->d_ws[l & 1];
-            double* wsc = t->d_ws[(l + 1) & 1];
-            hipLaunchKernelGGL(k_fill, dim3(std::min<int64_t>(cdiv(L.ws_size, 256), 65536)), dim3(256), 0, st, ws,
-                               L.ws_size);
-            if (L.rec_n > 0) {
-                const int64_t tot = (int64_t)L.rec_n * S.b * S.b;
-                hipLaunchKernelGGL(k_assemble, dim3(std::min<int64_t>(cdiv(tot, 256), 65536)), dim3(256), 0, st,
-                                   t->d_rec + L.rec_off, L.rec_n, bsr_val, S.b, t->d_fr, ws);
-            }
-            if (L.has_children)
-                for (int s = 0; s < 2; ++s)
-                    hipLaunchKernelGGL(k_extend, dim3(nf, cdiv(L.ncb_child_max, 8)), dim3(256), 0, st, nodes, s,
-                                       t->d_fr, t->d_fmap, S.b, ws, wsc);
-            const int tpr = cdiv(L.n_max, TILE);
-            for (int j0 = 0; j0 < L.nfs_max; j0 += KB) {
+    Symbolic& S = t->S;
+    hipStream_t st = t->stream;
+    const int big = 0x7fffffff;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, t->d_status, big);
+    for (int l = S.max_depth; l >= 0; --l) {
+        const Level& L = S.levels[l];
+        const int nf = (int)L.nodes.size();
+        const int32_t* nodes = t->d_nodes + L.node_off;
+        double* ws = t->d_ws[l & 1];
+        double* wsc = t->d_ws[(l + 1) & 1];
+        hipLaunchKernelGGL(k_fill, dim3(std::min<int64_t>(cdiv(L.ws_size, 256), 65536)), dim3(256), 0, st, ws,
+                           L.ws_size);
+        if (L.rec_n > 0) {
+            const int64_t tot = (int64_t)L.rec_n * S.b * S.b;
+            hipLaunchKernelGGL(k_assemble, dim3(std::min<int64_t>(cdiv(tot, 256), 65536)), dim3(256), 0, st,
+                               t->d_rec + L.rec_off, L.rec_n, bsr_val, S.b, t->d_fr, ws);
+        }
+        if (L.has_children)
+            for (int s = 0; s < 2; ++s)
+                hipLaunchKernelGGL(k_extend, dim3(nf, cdiv(L.ncb_child_max, 8)), dim3(256), 0, st, nodes, s,
+                                   t->d_fr, t->d_fmap, S.b, ws, wsc);
+        const int tpr = cdiv(L.n_max, TILE);
+        for (int j0 = 0; j0 < L.nfs_max; j0 += KB) {
+            if (block_pivots) {
+                hipLaunchKernelGGL(k_block_inv, dim3(nf), dim3(256), 0, st, nodes, t->d_fr, ws, t->d_pan, j0,
+                                   t->d_perm, t->d_status);
+                hipLaunchKernelGGL(k_panel_apply, dim3(nf, cdiv(L.n_max, 64)), dim3(256), 0, st, nodes, t->d_fr, ws,
+                                   t->d_pan, j0);
+                hipLaunchKernelGGL(k_copy_rows, dim3(nf, cdiv(L.n_max, 256)), dim3(256), 0, st, nodes, t->d_fr, ws,
+                                   t->d_pan, j0);
+            } else {
                 hipLaunchKernelGGL(k_panel, dim3(nf), dim3(1024), 0, st, nodes, t->d_fr, ws, t->d_pan, j0,
                                    t->d_ipiv, t->d_perm, t->d_status);
                 hipLaunchKernelGGL(k_swap_rows, dim3(nf, cdiv(L.n_max, 256)), dim3(256), 0, st, nodes, t->d_fr, ws,
                                    t->d_pan, j0, t->d_ipiv);
-                hipLaunchKernelGGL(k_update, dim3(tpr * tpr, nf), dim3(256), 0, st, nodes, t->d_fr, ws, t->d_pan, j0,
-                                   tpr);
             }
-            hipLaunchKernelGGL(k_extract, dim3(nf, cdiv(L.n_max, 4)), dim3(256), 0, st, nodes, t->d_fr, ws,
-                               t->d_fac);
+            hipLaunchKernelGGL(k_update, dim3(tpr * tpr, nf), dim3(256), 0, st, nodes, t->d_fr, ws, t->d_pan, j0,
+                               tpr);
         }
+        hipLaunchKernelGGL(k_extract, dim3(nf, cdiv(L.n_max, 4)), dim3(256), 0, st, nodes, t->d_fr, ws, t->d_fac);
+    }
+    TL_TRY(hipGetLastError());
+    int status = big;
+    TL_TRY(hipMemcpyAsync(&status, t->d_status, sizeof(int), hipMemcpyDeviceToHost, st));
+    TL_TRY(hipStreamSynchronize(st));
+    return status == big ? 0 : 1;
+}
+
+void solve_pass(pbte_trace* t, const double* rhs, double* x);
+
+// normwise backward error max|K x - b| / (max_i sum_j |K_ij| max|x| + max|b|)
+// of the factors on a fixed test vector
+double backward_error(pbte_trace* t, const double* bsr_val)
+{
+    Symbolic& S = t->S;
+    hipStream_t st = t->stream;
+    const int64_t n = (int64_t)S.F * S.b;
+    std::vector<double> hb(n), hx(n), hr(n), ha(n);
+    uint64_t z = 0x9e3779b97f4a7c15ull;
+    for (int64_t i = 0; i < n; ++i) {     // splitmix64, uniform in [-1, 1)
+        z += 0x9e3779b97f4a7c15ull;
+        uint64_t q = z;
+        q = (q ^ (q >> 30)) * 0xbf58476d1ce4e5b9ull;
+        q = (q ^ (q >> 27)) * 0x94d049bb133111ebull;
+        q ^= q >> 31;
+        hb[i] = (double)(q >> 11) * 0x1.0p-52 - 1.0;
+    }
+    double *db = dmalloc<double>(n), *dx = dmalloc<double>(n), *dr = dmalloc<double>(n), *da = dmalloc<double>(n);
+    int32_t *dptr = dmalloc<int32_t>(S.F + 1), *dcol = dmalloc<int32_t>(S.bsr_col.size());
+    auto cleanup = [&] {
+        for (void* p : {(void*)db, (void*)dx, (void*)dr, (void*)da, (void*)dptr, (void*)dcol}) (void)hipFree(p);
+    };
+    try {
+        TL_TRY(hipMemcpyAsync(db, hb.data(), n * 8, hipMemcpyHostToDevice, st));
+        TL_TRY(hipMemcpyAsync(dptr, S.bsr_ptr.data(), (S.F + 1) * 4, hipMemcpyHostToDevice, st));
+        TL_TRY(hipMemcpyAsync(dcol, S.bsr_col.data(), S.bsr_col.size() * 4, hipMemcpyHostToDevice, st));
+        solve_pass(t, db, dx);
+        hipLaunchKernelGGL(k_bsr_residual, dim3(cdiv(n, 256)), dim3(256), 0, st, dptr, dcol, bsr_val, S.F, S.b, dx,
+                           db, dr, da);
         TL_TRY(hipGetLastError());
-        int status = big;
-        TL_TRY(hipMemcpyAsync(&status, t->d_status, sizeof(int), hipMemcpyDeviceToHost, st));
+        TL_TRY(hipMemcpyAsync(hx.data(), dx, n * 8, hipMemcpyDeviceToHost, st));
+        TL_TRY(hipMemcpyAsync(hr.data(), dr, n * 8, hipMemcpyDeviceToHost, st));
+        TL_TRY(hipMemcpyAsync(ha.data(), da, n * 8, hipMemcpyDeviceToHost, st));
         TL_TRY(hipStreamSynchronize(st));
-        if (status != big) {
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+    double rmax = 0, xmax = 0, amax = 0, bmax = 0;
+    bool finite = true;
+    for (int64_t i = 0; i < n; ++i) {
+        finite = finite && std::isfinite(hx[i]) && std::isfinite(hr[i]);
+        rmax = std::max(rmax, std::fabs(hr[i]));
+        xmax = std::max(xmax, std::fabs(hx[i]));
+        amax = std::max(amax, ha[i]);
+        bmax = std::max(bmax, std::fabs(hb[i]));
+    }
+    return finite ? rmax / (amax * xmax + bmax) : INFINITY;
+}
+}  // namespace
+
+extern "C" int pbte_trace_factor(pbte_trace* t, const double* bsr_val)
+{
+    return tguard(t, [&] {
+        t->alloc();
+        t->factored = false;
+        // diagonal-block pivots first; partial pivoting among the fully
+        // summed rows when a block has no pivot or the factors fail the
+        // backward-error test (or as the caller forces)
+        if (t->force_pivoting == 0) {
+            if (factor_pass(t, bsr_val, true) != 0) {
+                t->err = "trace solve: a diagonal block without a nonzero pivot";
+                return 1;
+            }
+            t->factored = true;
+            t->berr = backward_error(t, bsr_val);
+            t->pivoting = 0;
+            return 0;
+        }
+        if (t->force_pivoting < 0 && factor_pass(t, bsr_val, true) == 0) {
+            t->factored = true;
+            t->berr = backward_error(t, bsr_val);
+            if (t->berr <= 1e-13) {
+                t->pivoting = 0;
+                return 0;
+            }
+        }
+        t->factored = false;
+        if (factor_pass(t, bsr_val, false) != 0) {
             t->err = "trace solve: the trace matrix is singular (a front without a nonzero pivot)";
-            t->factored = false;
             return 1;
         }
         t->factored = true;
+        t->berr = backward_error(t, bsr_val);
+        t->pivoting = 1;
         return 0;
     });
 }
+
+namespace {
+void solve_pass(pbte_trace* t, const double* rhs, double* x)
+{
+    Symbolic& S = t->S;
+    hipStream_t st = t->stream;
+    for (int l = S.max_depth; l >= 0; --l) {
+        const Level& L = S.levels[l];
+        const int nf = (int)L.nodes.size();
+        hipLaunchKernelGGL(k_fwd_gather, dim3(nf), dim3(256), (size_t)L.n_max * 8, st, t->d_nodes + L.node_off,
+                           t->d_fr, t->d_faces, t->d_fmap, t->d_perm, S.b, rhs, t->d_y, t->d_cv);
+        if (L.fw_n > 0)
+            hipLaunchKernelGGL(k_fwd_gemv, dim3(L.fw_n), dim3(256), 0, st, t->d_fw + L.fw_off, t->d_fr, t->d_fac,
+                               t->d_y, t->d_cv);
+    }
+    for (int l = 0; l <= S.max_depth; ++l) {
+        const Level& L = S.levels[l];
+        if (L.bw_n > 0)
+            hipLaunchKernelGGL(k_bwd, dim3(L.bw_n), dim3(256), (size_t)L.n_max * 8, st, t->d_bw + L.bw_off,
+                               t->d_fr, t->d_faces, S.b, t->d_fac, t->d_y, x);
+    }
+    TL_TRY(hipGetLastError());
+}
+}  // namespace
 
 extern "C" int pbte_trace_solve(pbte_trace* t, const double* rhs, double* x)
 {
     return tguard(t, [&] {
         if (!t->factored) throw std::string("trace solve: pbte_trace_factor has not succeeded");
-        Symbolic& S = t->S;
-        hipStream_t st = t->stream;
-        for (int l = S.max_depth; l >= 0; --l) {
-            const Level& L = S.levels[l];
-            const int nf = (int)L.nodes.size();
-            hipLaunchKernelGGL(k_fwd_gather, dim3(nf), dim3(256), (size_t)L.n_max * 8, st, t->d_nodes + L.node_off,
-                               t->d_fr, t->d_faces, t->d_fmap, t->d_perm, S.b, rhs, t->d_y, t->d_cv);
-            if (L.fw_n > 0)
-                hipLaunchKernelGGL(k_fwd_gemv, dim3(L.fw_n), dim3(256), 0, st, t->d_fw + L.fw_off, t->d_fr, t->d_fac,
-                                   t->d_y, t->d_cv);
-        }
-        for (int l = 0; l <= S.max_depth; ++l) {
-            const Level& L = S.levels[l];
-            if (L.bw_n > 0)
-                hipLaunchKernelGGL(k_bwd, dim3(L.bw_n), dim3(256), (size_t)L.n_max * 8, st, t->d_bw + L.bw_off,
-                                   t->d_fr, t->d_faces, S.b, t->d_fac, t->d_y, x);
-        }
-        TL_TRY(hipGetLastError());
+        solve_pass(t, rhs, x);
         return 0;
     });
 }

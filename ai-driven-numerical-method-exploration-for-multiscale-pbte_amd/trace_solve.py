@@ -80,6 +80,16 @@ class TraceSolver:
                 "unknowns", "blocks")
         return {k: int(v) for k, v in zip(keys, out)}
 
+    def set_pivoting(self, mode: int) -> None:
+        """-1 automatic (diagonal-block pivots, partial pivoting as the
+        fallback), 0 diagonal-block pivots only, 1 partial pivoting only."""
+        self._check(self._lib.pbte_trace_options(self.handle, int(mode)), "trace_options")
+
+    def quality(self) -> dict:
+        piv, berr = C.c_int32(), C.c_double()
+        self._check(self._lib.pbte_trace_quality(self.handle, C.byref(piv), C.byref(berr)), "trace_quality")
+        return {"pivoting": "block" if piv.value == 0 else "partial", "backward_error": berr.value}
+
     def factor(self, values) -> None:
         """Factorise from the block values (host ``[nnzb, blk, blk]`` array
         or a device tensor of that size)."""

@@ -413,6 +413,19 @@ const char *pbte_trace_last_error(pbte_trace *t);
  * factorisation flops, device workspace bytes, unknowns, assembly blocks. */
 int pbte_trace_info(pbte_trace *t, int64_t *out8);
 
+/* Pivoting of the numeric factorisation: -1 (default) diagonal-block
+ * pivots (interchanges within each 32-column block of a front's fully
+ * summed rows, every step a dense product), with partial pivoting among all
+ * of a front's fully summed rows when a block has no pivot or the normwise
+ * backward error of the factors on a fixed test vector exceeds 1e-13; 0 or
+ * 1 force the one or the other.  No counterpart in the reference (SuperLU's
+ * own pivoting is internal). */
+int pbte_trace_options(pbte_trace *t, int32_t pivoting);
+
+/* The pivoting the factors were built with (0 block, 1 partial) and their
+ * backward error on the test vector. */
+int pbte_trace_quality(pbte_trace *t, int32_t *pivoting, double *berr);
+
 /* Numeric factorisation from the block values bsr_val (device,
  * [nnzb][3Q][3Q] row-major, the pattern given to pbte_trace_create).
  * Returns 0, or 1 when a front has no nonzero pivot (singular K). */

@@ -18,16 +18,21 @@ CASES_TL = [("F1_1_1.yaml", 1, None), ("F1_1_1.yaml", 2, None), ("F1_1_1.yaml", 
 @pytest.mark.parametrize("case,deg,mesh", CASES_TL,
                          ids=lambda v: str(v if not hasattr(v, "name") else v.name))
 @pytest.mark.parametrize("leaf", [4, 16])
-def test_trace_solve_matches_superlu(pkg, case, deg, mesh, leaf):
+@pytest.mark.parametrize("pivoting", [0, 1], ids=["block_pivots", "partial_pivots"])
+def test_trace_solve_matches_superlu(pkg, case, deg, mesh, leaf, pivoting):
     import torch
     from scipy.sparse.linalg import splu
 
     from pbte_amd.trace_solve import TraceSolver, block_pattern
-    print(f"case trace solve {case} deg {deg} mesh {mesh} leaf {leaf}", flush=True)
+    print(f"case trace solve {case} deg {deg} mesh {mesh} leaf {leaf} pivoting {pivoting}", flush=True)
     pb, K, t1, t2, blk = trace_system(pkg, case, deg, mesh=mesh)
     ptr, col, val = block_pattern(K, blk)
     ts = TraceSolver(pb.mesh, t1, t2, ptr, col, blk, leaf_cells=leaf)
+    ts.set_pivoting(pivoting)
     ts.factor(val)
+    q = ts.quality()
+    assert q["pivoting"] == ("block" if pivoting == 0 else "partial")
+    assert q["backward_error"] <= 1e-14
     lu = splu(K.tocsc(), permc_spec="COLAMD")
     rng = np.random.default_rng(7)
     for _ in range(3):
@@ -52,5 +57,5 @@ def test_trace_solve_singular_raises(pkg):
     # zero the block row of one face: its unknowns have no pivot
     val[ptr[5]:ptr[6]] = 0.0
     ts = TraceSolver(pb.mesh, t1, t2, ptr, col, blk, leaf_cells=8)
-    with pytest.raises(np.linalg.LinAlgError):
+    with pytest.raises(np.linalg.LinAlgError):     # both pivotings fail
         ts.factor(val)
