@@ -229,7 +229,7 @@ struct pbte_ctx {
     double* gs_inv3 = nullptr;  // T, qx, qy columns of inv(AA_SOL), moment preconditioner
     double *gs_hw = nullptr, *gs_inv4 = nullptr, *gs_aa_trace = nullptr, *gs_ba_sol = nullptr,
            *gs_wtfc = nullptr, *gs_wnx = nullptr, *gs_wny = nullptr, *gs_hmin = nullptr,
-           *gs_work = nullptr, *gs_wmom = nullptr;
+           *gs_work = nullptr, *gs_wmom = nullptr, *gs_side = nullptr;
     int32_t *gs_t1 = nullptr, *gs_l1 = nullptr, *gs_t2 = nullptr, *gs_l2 = nullptr, *gs_wface = nullptr,
             *gs_wtri = nullptr, *gs_tri_faces = nullptr;
     uint8_t* gs_flip = nullptr;
@@ -2014,33 +2014,44 @@ __global__ void __launch_bounds__(64) wall_trace_kernel(const T* __restrict__ vd
     }
 }
 
-// trace right-hand side per (face, row) (global_solve.py:128-186)
-__global__ void __launch_bounds__(256) trace_rhs_kernel(const double* __restrict__ ba_sol, const double* __restrict__ aa_src,
-                                                        const int32_t* t1, const int32_t* l1, const int32_t* t2,
-                                                        const int32_t* l2, const uint8_t* flip, int F, int N, int P,
-                                                        int Q, double* ffa)
+// trace right-hand side (global_solve.py:128-186) in two passes.  First
+// per (element, local face) side: side[t][l][r] = sum_q BA_SOL[t][l][r][q]
+// AA_SRC[q][t] (q in order), one wave per element, lanes over (l, r), from
+// the transposed baT [N][3][7P][3Q] so that the lanes read consecutive words
+template <int P, int Q>
+__global__ void __launch_bounds__(256) trace_side_kernel(const double* __restrict__ baT,
+                                                         const double* __restrict__ aa_src, int N,
+                                                         double* __restrict__ side)
+{
+    constexpr int R = 3 * Q, C = 7 * P;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t = blockIdx.x * 4 + wv;
+    if (t >= N || lane >= 3 * R) return;
+    const int l = lane / R, r = lane - (lane / R) * R;
+    const double* b = baT + (((size_t)t * 3 + l) * C) * R + r;
+    double s = 0.0;
+    for (int q = 0; q < C; ++q) s += b[(size_t)q * R] * aa_src[(size_t)q * N + t];
+    side[((size_t)t * 3 + l) * R + r] = s;
+}
+
+// then per (face, row): ffa[f][r] = side(t1, l1)[r] + side(t2, l2)[r'], r'
+// mirrored within each field block across a reversed periodic pair
+__global__ void __launch_bounds__(256) trace_rhs_kernel(const double* __restrict__ side, const int32_t* t1,
+                                                        const int32_t* l1, const int32_t* t2, const int32_t* l2,
+                                                        const uint8_t* flip, int F, int Q, double* ffa)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int R = 3 * Q, C = 7 * P;
+    const int R = 3 * Q;
     if (g >= F * R) return;
     const int f = g / R, r = g - (g / R) * R;
-    double s = 0.0;
-    {
-        const int t = t1[f], l = l1[f];
-        const double* b = ba_sol + (((size_t)t * 3 + l) * R + r) * C;
-        for (int q = 0; q < C; ++q) s += b[q] * aa_src[(size_t)q * N + t];
-    }
+    double s = side[((size_t)t1[f] * 3 + l1[f]) * R + r];
     if (t2[f] >= 0) {
-        const int t = t2[f], l = l2[f];
         int rr = r;
         if (flip[f]) {   // mirror_permutation: k -> Q-1-k within each field block
             const int blk = r / Q, k = r - blk * Q;
             rr = blk * Q + (Q - 1 - k);
         }
-        const double* b = ba_sol + (((size_t)t * 3 + l) * R + rr) * C;
-        double s2 = 0.0;
-        for (int q = 0; q < C; ++q) s2 += b[q] * aa_src[(size_t)q * N + t];
-        s += s2;
+        s += side[((size_t)t2[f] * 3 + l2[f]) * R + rr];
     }
     ffa[(size_t)f * R + r] = s;
 }
@@ -3889,9 +3900,11 @@ extern "C" int pbte_gsis_setup(pbte_ctx* ctx, const pbte_gsis_data* g)
         ctx->gs_aa_trace = ctx->alloc<double>((size_t)N * 3 * 7 * P * 3 * Q);
         hipLaunchKernelGGL(transpose_slice_kernel, dim3(nblocks((int64_t)N * 3 * 7 * P * 3 * Q, 256)), dim3(256), 0,
                            ctx->stream, g->aa_trace, (int64_t)N * 3, 7 * P, 3 * Q, 0, 3 * Q, ctx->gs_aa_trace);
+        // transposed BA_SOL [N][3][7P][3Q], and the per-side trace sums
         ctx->gs_ba_sol = ctx->alloc<double>((size_t)N * 3 * 3 * Q * 7 * P);
-        HIP_TRY(hipMemcpyAsync(ctx->gs_ba_sol, g->ba_sol, sizeof(double) * N * 3 * 3 * Q * 7 * P,
-                               hipMemcpyDeviceToDevice, ctx->stream));
+        hipLaunchKernelGGL(transpose_slice_kernel, dim3(nblocks((int64_t)N * 3 * 7 * P * 3 * Q, 256)), dim3(256), 0,
+                           ctx->stream, g->ba_sol, (int64_t)N * 3, 3 * Q, 7 * P, 0, 7 * P, ctx->gs_ba_sol);
+        ctx->gs_side = ctx->alloc<double>((size_t)N * 3 * 3 * Q);
         HIP_TRY(hipGetLastError());
         ctx->gs_t1 = ctx->upload(g->t1, F);
         ctx->gs_l1 = ctx->upload(g->l1, F);
@@ -3979,9 +3992,18 @@ extern "C" int pbte_gsis_rhs(pbte_ctx* ctx, int32_t precision, const void* vdf, 
     return guarded(ctx, [&]() {
         if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
         const int N = ctx->N, P = ctx->P, Q = ctx->Q, F = ctx->F;
+#define TS(PP, QQ)                                                                                     \
+    case PP:                                                                                           \
+        hipLaunchKernelGGL((trace_side_kernel<PP, QQ>), dim3((N + 3) / 4), dim3(256), 0, ctx->stream,  \
+                           ctx->gs_ba_sol, aa_src, N, ctx->gs_side);                                   \
+        break;
+        switch (P) {
+            TS(3, 2) TS(6, 3) TS(10, 4) TS(15, 5)
+            default: throw std::string("gsis_rhs: unsupported P");
+        }
+#undef TS
         hipLaunchKernelGGL(trace_rhs_kernel, dim3(nblocks((int64_t)F * 3 * Q, 256)), dim3(256), 0, ctx->stream,
-                           ctx->gs_ba_sol, aa_src, ctx->gs_t1, ctx->gs_l1, ctx->gs_t2, ctx->gs_l2, ctx->gs_flip, F, N,
-                           P, Q, ffa);
+                           ctx->gs_side, ctx->gs_t1, ctx->gs_l1, ctx->gs_t2, ctx->gs_l2, ctx->gs_flip, F, Q, ffa);
         HIP_TRY(hipGetLastError());
         if (ctx->gs_nw) {
             const unsigned g = nblocks((int64_t)ctx->gs_nw * Q, 256);

@@ -50,8 +50,7 @@ namespace {
     } while (0)
 
 constexpr int KB = 32;        // pivot columns per sweep step
-constexpr int GEMV_ROWS = 16; // rows of a front per solve work item
-constexpr int TILE = 64;      // update tile (rows and columns)
+constexpr int TILE = 128;     // update tile (rows and columns)
 
 struct Front {                // one tree node, device and host
     int64_t ws;               // front matrix offset in its level's workspace (doubles)
@@ -74,6 +73,7 @@ struct Level {
     int32_t rec_off = 0, rec_n = 0;   // original-block records
     int32_t fw_off = 0, fw_n = 0;     // forward work items (front, row0)
     int32_t bw_off = 0, bw_n = 0;     // backward work items
+    int32_t lpr_fw = 64, lpr_bw = 64; // lanes per row of the solve's products
     bool has_children = false;
 };
 
@@ -416,17 +416,21 @@ __global__ void k_swap_rows(const int32_t* __restrict__ nodes, const Front* __re
         T[(int64_t)k * n + c] = (in_panel || k >= kbe) ? 0.0 : M[(int64_t)(j0 + k) * n + c];
 }
 
-// The rank-KB update of the sweep step on 64 x 64 tiles of every front:
+// The rank-KB update of the sweep step on 128 x 128 tiles of every front:
 // outside the panel columns M[i][c] = [i not a pivot row] M[i][c]
 // + sum_k P[i][k] T[k][c]; inside them, the swept panel is stored.
-// blockIdx.x = tile (row-major over the largest front of the level),
-// blockIdx.y = front.
+// 256 threads, 8 x 8 values each (rows ty*8 + a, columns tx*8 + q), the
+// panel and row tiles staged in LDS in chunks of KC columns / rows (the row
+// tile column-interleaved, so that the lanes of a wave read consecutive
+// words).  blockIdx.x = tile (row-major over the largest front of the
+// level), blockIdx.y = front.
+constexpr int KC = 16;
 __global__ void __launch_bounds__(256) k_update(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
                                                 double* __restrict__ ws, const double* __restrict__ pan, int j0,
                                                 int tiles_per_row)
 {
-    __shared__ double sP[TILE][KB + 1];
-    __shared__ double sT[KB][TILE + 1];
+    __shared__ double sP[TILE][KC + 1];
+    __shared__ double sT[KC][TILE];
     const Front f = fr[nodes[blockIdx.y]];
     if (j0 >= f.nfs) return;
     const int n = f.n;
@@ -438,41 +442,44 @@ __global__ void __launch_bounds__(256) k_update(const int32_t* __restrict__ node
     const double* P = pan + f.pan;
     const double* T = P + (int64_t)n * KB;
     double* M = ws + f.ws;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < TILE * KB; e += 256) {
-        const int i = e / KB, k = e - (e / KB) * KB;
-        sP[i][k] = (r0 + i < n) ? P[(int64_t)(r0 + i) * KB + k] : 0.0;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    double acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[a][q] = 0.0;
+    for (int k0 = 0; k0 < kbe; k0 += KC) {
+        __syncthreads();
+        for (int e = tid; e < TILE * KC; e += 256) {
+            const int i = e / KC, k = e - (e / KC) * KC;
+            sP[i][k] = (r0 + i < n) ? P[(int64_t)(r0 + i) * KB + k0 + k] : 0.0;
+        }
+        for (int e = tid; e < TILE * KC; e += 256) {
+            const int k = e / TILE, c = e - (e / TILE) * TILE;       // column c0 + c of row k0 + k
+            sT[k][(c & 7) * 16 + (c >> 3)] = (c0 + c < n) ? T[(int64_t)(k0 + k) * n + c0 + c] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < KC; ++k) {
+            double pa[8], tb[8];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) pa[a] = sP[ty * 8 + a][k];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) tb[q] = sT[k][q * 16 + tx];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[a][q] = fma(pa[a], tb[q], acc[a][q]);
+        }
     }
-    for (int e = tid; e < TILE * KB; e += 256) {
-        const int k = e / TILE, c = e - (e / TILE) * TILE;
-        sT[k][c] = (c0 + c < n) ? T[(int64_t)k * n + c0 + c] : 0.0;
-    }
-    __syncthreads();
-    const int tx = tid & 15, ty = tid >> 4;
-    double acc[4][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc[a][q] = 0.0;
-    for (int k = 0; k < KB; ++k) {
-        double pa[4], tb[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) pa[a] = sP[ty * 4 + a][k];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) tb[q] = sT[k][tx * 4 + q];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[a][q] = fma(pa[a], tb[q], acc[a][q]);
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int i = r0 + ty * 4 + a;
+    for (int a = 0; a < 8; ++a) {
+        const int i = r0 + ty * 8 + a;
         if (i >= n) continue;
         const bool pivot_row = i >= j0 && i < j0 + kbe;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int c = c0 + tx * 4 + q;
+        for (int q = 0; q < 8; ++q) {
+            const int c = c0 + tx * 8 + q;
             if (c >= n) continue;
             double* m = M + (int64_t)i * n + c;
             if (c >= j0 && c < j0 + kbe)
@@ -504,12 +511,6 @@ __global__ void k_extract(const int32_t* __restrict__ nodes, const Front* __rest
 // ---------------------------------------------------------------------------
 // device kernels: solve
 // ---------------------------------------------------------------------------
-__device__ inline double wave_sum(double v)
-{
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
 
 // forward assembly of one front: b1 from the right-hand side plus the
 // children's contributions, y = P b1, c = (contributions on the ancestors'
@@ -536,47 +537,72 @@ __global__ void k_fwd_gather(const int32_t* __restrict__ nodes, const Front* __r
     for (int i = threadIdx.x; i < f.ncb; i += blockDim.x) cv[f.cvec + i] = bf[f.nfs + i];
 }
 
-// c[i] -= W[i] . y for GEMV_ROWS rows of a front per workgroup (4 waves,
-// one row per wave at a time, lanes across the row, fixed-order reduction)
+// sum over the LPR lanes of a row group (butterfly within the group)
+template <int LPR>
+__device__ inline double group_sum(double v)
+{
+#pragma unroll
+    for (int m = LPR / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// dot product of a row with a vector by the LPR lanes of a group, each lane
+// its columns k = lane + LPR j with two accumulators (even, odd j)
+template <int LPR>
+__device__ inline double row_dot(const double* __restrict__ row, const double* __restrict__ v, int ncols, int lane)
+{
+    double s0 = 0.0, s1 = 0.0;
+    int k = lane;
+    for (; k + LPR < ncols; k += 2 * LPR) {
+        s0 = fma(row[k], v[k], s0);
+        s1 = fma(row[k + LPR], v[k + LPR], s1);
+    }
+    if (k < ncols) s0 = fma(row[k], v[k], s0);
+    return group_sum<LPR>(s0 + s1);
+}
+
+// c[i] -= W[i] . y over the rows of a work item: 256 / LPR row groups of
+// LPR lanes, 2 rows per group
+template <int LPR>
 __global__ void __launch_bounds__(256) k_fwd_gemv(const int2* __restrict__ items, const Front* __restrict__ fr,
                                                   const double* __restrict__ fac, const double* __restrict__ y,
                                                   double* __restrict__ cv)
 {
+    constexpr int G = 256 / LPR;
     const int2 it = items[blockIdx.x];
     const Front f = fr[it.x];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
     const double* yv = y + f.piv;
-    for (int r = it.y + wv; r < min(it.y + GEMV_ROWS, f.ncb); r += 4) {
-        const double* row = fac + f.w + (int64_t)r * f.nfs;
-        double s = 0.0;
-        for (int k = lane; k < f.nfs; k += 64) s = fma(row[k], yv[k], s);
-        s = wave_sum(s);
+    for (int r = it.y + grp; r < min(it.y + 2 * G, f.ncb); r += G) {
+        const double s = row_dot<LPR>(fac + f.w + (int64_t)r * f.nfs, yv, f.nfs, lane);
         if (lane == 0) cv[f.cvec + r] -= s;
     }
 }
 
-// x[own faces] = [Z V] [y; -x(ancestors' faces)] for GEMV_ROWS rows of a
-// front per workgroup; the input vector is staged in LDS
+// x[own faces] = [Z V] [y; -x(ancestors' faces)] over the rows of a work
+// item (as k_fwd_gemv); the input vector is staged in LDS
+template <int LPR>
 __global__ void __launch_bounds__(256) k_bwd(const int2* __restrict__ items, const Front* __restrict__ fr,
                                              const int32_t* __restrict__ faces, int b,
                                              const double* __restrict__ fac, const double* __restrict__ y,
                                              double* __restrict__ x)
 {
     extern __shared__ double u[];
+    constexpr int G = 256 / LPR;
     const int2 it = items[blockIdx.x];
     const Front f = fr[it.x];
     for (int j = threadIdx.x; j < f.n; j += blockDim.x)
         u[j] = j < f.nfs ? y[f.piv + j] : -x[(int64_t)faces[f.faces + j / b] * b + j % b];
     __syncthreads();
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int r = it.y + wv; r < min(it.y + GEMV_ROWS, f.nfs); r += 4) {
-        const double* row = fac + f.zv + (int64_t)r * f.n;
-        double s = 0.0;
-        for (int k = lane; k < f.n; k += 64) s = fma(row[k], u[k], s);
-        s = wave_sum(s);
+    const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
+    for (int r = it.y + grp; r < min(it.y + 2 * G, f.nfs); r += G) {
+        const double s = row_dot<LPR>(fac + f.zv + (int64_t)r * f.n, u, f.n, lane);
         if (lane == 0) x[(int64_t)faces[f.faces + r / b] * b + r % b] = s;
     }
 }
+
+// lanes per row for rows of ncols values
+inline int lanes_per_row(int ncols) { return ncols <= 64 ? 8 : ncols <= 160 ? 16 : ncols <= 640 ? 32 : 64; }
 
 // ---------------------------------------------------------------------------
 // host: symbolic phase
@@ -790,10 +816,18 @@ void build_symbolic(const pbte_trace_desc* d, Symbolic& S)
         S.rec.insert(S.rec.end(), recs[l].begin(), recs[l].end());
         L.fw_off = (int32_t)S.fw_items.size();
         L.bw_off = (int32_t)S.bw_items.size();
+        int nfs_max = 0, n_max = 0;
+        for (int v : L.nodes) {
+            nfs_max = std::max(nfs_max, S.fronts[v].nfs);
+            n_max = std::max(n_max, S.fronts[v].n);
+        }
+        L.lpr_fw = lanes_per_row(nfs_max);
+        L.lpr_bw = lanes_per_row(n_max);
+        const int rows_fw = 2 * 256 / L.lpr_fw, rows_bw = 2 * 256 / L.lpr_bw;
         for (int v : L.nodes) {
             const Front& f = S.fronts[v];
-            for (int r = 0; r < f.ncb; r += GEMV_ROWS) S.fw_items.push_back({v, r});
-            for (int r = 0; r < f.nfs; r += GEMV_ROWS) S.bw_items.push_back({v, r});
+            for (int r = 0; r < f.ncb; r += rows_fw) S.fw_items.push_back({v, r});
+            for (int r = 0; r < f.nfs; r += rows_bw) S.bw_items.push_back({v, r});
         }
         L.fw_n = (int32_t)S.fw_items.size() - L.fw_off;
         L.bw_n = (int32_t)S.bw_items.size() - L.bw_off;
@@ -1098,15 +1132,20 @@ void solve_pass(pbte_trace* t, const double* rhs, double* x)
         const int nf = (int)L.nodes.size();
         hipLaunchKernelGGL(k_fwd_gather, dim3(nf), dim3(256), (size_t)L.n_max * 8, st, t->d_nodes + L.node_off,
                            t->d_fr, t->d_faces, t->d_fmap, t->d_perm, S.b, rhs, t->d_y, t->d_cv);
-        if (L.fw_n > 0)
-            hipLaunchKernelGGL(k_fwd_gemv, dim3(L.fw_n), dim3(256), 0, st, t->d_fw + L.fw_off, t->d_fr, t->d_fac,
-                               t->d_y, t->d_cv);
+        if (L.fw_n > 0) {
+            auto fn = L.lpr_fw == 8 ? k_fwd_gemv<8> : L.lpr_fw == 16 ? k_fwd_gemv<16>
+                    : L.lpr_fw == 32 ? k_fwd_gemv<32> : k_fwd_gemv<64>;
+            hipLaunchKernelGGL(fn, dim3(L.fw_n), dim3(256), 0, st, t->d_fw + L.fw_off, t->d_fr, t->d_fac, t->d_y,
+                               t->d_cv);
+        }
     }
     for (int l = 0; l <= S.max_depth; ++l) {
         const Level& L = S.levels[l];
-        if (L.bw_n > 0)
-            hipLaunchKernelGGL(k_bwd, dim3(L.bw_n), dim3(256), (size_t)L.n_max * 8, st, t->d_bw + L.bw_off,
-                               t->d_fr, t->d_faces, S.b, t->d_fac, t->d_y, x);
+        if (L.bw_n > 0) {
+            auto fn = L.lpr_bw == 8 ? k_bwd<8> : L.lpr_bw == 16 ? k_bwd<16> : L.lpr_bw == 32 ? k_bwd<32> : k_bwd<64>;
+            hipLaunchKernelGGL(fn, dim3(L.bw_n), dim3(256), (size_t)L.n_max * 8, st, t->d_bw + L.bw_off, t->d_fr,
+                               t->d_faces, S.b, t->d_fac, t->d_y, x);
+        }
     }
     TL_TRY(hipGetLastError());
 }
