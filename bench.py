@@ -1,51 +1,48 @@
-"""Benchmark of the sweep hot path on one MI355X.
+"""Benchmark of the gray Callaway DG discrete-ordinates hot path on one MI355X.
 
-Workload (default): the Krylov configuration -- unit square, 128 x 128
-structured grid split into 32 768 P2 triangles (Np = 6), 256 ordinates
-(8 Gauss-Legendre polar x 32 azimuth), Kn_R = 1, Kn_N = 0.1, diffuse
-isothermal walls with the north wall at dT = +1.  A step is one transport
-sweep over all 256 directions (FP32 element operators and distribution, the
-FP32-sweep leg of the mixed-precision solve), with the moments already
-resident in HBM.
+Default workload (--workload config2): the GSIS configuration -- a 256 x 256
+triangulated unit square with its interior vertices moved by
+U(-0.25h, 0.25h) (numpy default_rng(0)), 131 072 P2 triangles (Np = 6), 512
+ordinates (16 Gauss-Legendre polar x 32 azimuth), Kn_R = 1e-3, Kn_N = 1e-4,
+one wall at dT = +1 and the opposite wall at 0, the other two sides a
+periodic pair (in place of the specular walls, which the reference does not
+have; for this layout both give the same one-dimensional solution).
 
-Prints one JSON line:
+A step is one transport sweep over all 512 directions in binary32 (element
+inverses and distribution; the sweeps of the FP32 GSIS solve), the moments
+already resident in HBM.  One JSON line:
+
   value          sweep DOF-updates/s over the timed steps (wall clock);
   roofline       algorithmic bytes per sweep / mean sweep-kernel time, the
                  kernel timed with HIP events recorded on its launch stream
                  around each launch; `traffic` is the HBM bytes per launch
                  from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-                 (profiles/sweep_traffic.json), null for other workloads;
-  solve          per Kn_R in {0.01, 0.1, 1, 10} (Kn_N = 0.1 Kn_R): time-to-
-                 solution (setup + run, each also reported) and solves/hour
-                 of the FP64 solve (transport residual below 1e-8) and of the
-                 FP32-sweep + FP64-refinement solve (asked for 1e-6 relative
-                 L2 on T and heat flux, performance.refine_tol), and the
-                 relative L2 differences of T and heat flux between them
-                 (--no-solve skips it);
-  cpu_baseline   the C restatement of the reference's sweep on this host.
+                 (profiles/sweep_traffic.json);
+  solve          the FP64 GSIS solve and the FP32-sweep GSIS solve with the
+                 binary64 finish of the same case, each to the iterate step
+                 1e-8: time-to-solution (setup + run), solves/hour, iterations,
+                 the relative L2 difference of T and of the heat flux between
+                 them, and the time-to-solution speedup;
+  cpu_baseline   the C restatement of the reference's sweep (binary64,
+                 OpenMP over directions) on this host, on a bounded sample:
+                 the same mesh with 16 of the ordinates, at the host's thread
+                 count and at one thread.
 
---workload cis runs the plain-CIS configuration: unit square 16 x 16 split
-into 512 P1 triangles, 64 ordinates (4 x 16), Kn_R = Kn_N = 1, diffuse
-isothermal walls (north dT = +1), FP64 source iteration to 1e-8; value is
-the GPU solves/hour and cpu_baseline the C restatement's solve on the host.
-
---workload gsis runs the GSIS configuration instead: jittered triangulated
-square (interior vertices moved by U(-0.25h, 0.25h), seed 0), P2, 512
-ordinates (16 x 32), Kn_R = 1e-3, Kn_N = 1e-4, one wall at dT = +1, the
-opposite wall at 0, the other two sides a periodic pair (in place of the
-specular walls, which the reference does not have; for this layout both
-give the same one-dimensional solution); one FP64 and one FP32-sweep
-(binary64 finish) GSIS solve to the iterate step 1e-8.  value is the
-solves/hour of the FP32 solve; roofline is the sweep kernel inside those
-solves.  The default mesh is 128 x 128: the 256 x 256 trace system (1.77 M
-unknowns) does not fit the host SuperLU factorisation.
-
---workload batch runs the batched configuration: 64 independent problems
-drawn from default_rng(1) (2k-65k jittered P1/P2 triangles, 64-512
-ordinates, Kn_R log-uniform on [1e-3, 10]); a step is one FP32 sweep of
-every problem, one launch per kernel instance (batch.launch_key); value is the aggregate
-DOF-updates/s, with the problem-by-problem launches timed beside it, and a
-GSIS solve leg of the same draws on coarser meshes (batch_workload).
+Other workloads (each prints its own line):
+--workload sweep   the Krylov configuration: 128 x 128 structured square,
+                   32 768 P2 triangles, 256 ordinates (8 x 32), sweep
+                   DOF-updates/s and FP64 vs FP32+refinement GMRES solves at
+                   Kn_R in {0.01, 0.1, 1, 10} (Kn_N = 0.1 Kn_R).  With --n 362
+                   --deg 3 --npole 16 --nazim 64 --jitter 0.25 --hole 0.2
+                   --tau-r 0.05 it is the large single solve (P3, 1 024
+                   ordinates, a central hot hole);
+--workload cis     the plain-CIS configuration (16 x 16, 512 P1 triangles,
+                   64 ordinates, Kn_R = Kn_N = 1) to 1e-8;
+--workload gsis    the GSIS solves alone at --n;
+--workload batch   64 independent problems drawn from default_rng(1)
+                   (2k-65k jittered P1/P2 triangles, 64-512 ordinates, Kn_R
+                   log-uniform on [1e-3, 10]); a step is one FP32 sweep of
+                   every problem; a GSIS solve leg of the same draws.
 """
 from __future__ import annotations
 
@@ -143,12 +140,12 @@ def gsis_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitte
     }, basedir=str(workdir))
 
 
-def gsis_workload(pkg, args, work):
-    """FP64 and FP32-sweep GSIS solves of the same case; the sweep kernel
-    is timed with HIP events over every sweep of each solve."""
+def gsis_solves(pkg, args, work, n):
+    """FP64 and FP32-sweep GSIS solves of the config-2 case on an n x n mesh;
+    the sweep kernel is timed with HIP events over every sweep of each
+    solve.  Returns the per-precision figures and the comparison."""
     import torch
-    n = args.n
-    res, fields = {}, {}
+    res, fields, shape = {}, {}, None
     for prec in ("fp64", "fp32"):
         case = gsis_case(pkg, n, args.deg, args.npole, args.nazim, 1e-3, 1e-4, prec, work,
                          jitter=args.jitter if args.jitter else 0.25)
@@ -170,7 +167,7 @@ def gsis_workload(pkg, args, work):
         ms = s.ctx.timing_read(case.iteration.tmax)
         s.ctx.timing(0)
         fields[prec] = (rec.temp, np.concatenate([rec.qx, rec.qy]))
-        D, N, P = s.ndir, s.n_tris, s.ndof
+        shape = (s.ndir, s.n_tris, s.ndof)
         k32 = int(rec.diagnostics.get("steps_fp32", 0))
         res[prec] = {"time_to_solution_s": setup + run, "setup_s": setup, "run_s": run,
                      "solves_per_hour": 3600.0 / (setup + run), "iterations": rec.iterations,
@@ -180,14 +177,23 @@ def gsis_workload(pkg, args, work):
                      "sweep_kernel_ms_fp64_mean": float(np.mean(ms[k32:])) if len(ms) > k32 else None,
                      "sweep_kernel_s_total": float(np.sum(ms)) * 1e-3,
                      "trace_unknowns": rec.diagnostics.get("global_matrix_n"),
+                     "trace_solve": rec.diagnostics.get("trace_solve"),
                      "trace_ordering": rec.diagnostics.get("trace_ordering")}
         del s
         torch.cuda.empty_cache()
     (t64, q64), (t32, q32) = fields["fp64"], fields["fp32"]
     res["fp32"]["rel_l2_T_vs_fp64"] = float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64))
     res["fp32"]["rel_l2_q_vs_fp64"] = float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))
+    res["fp32"]["max_abs_T_vs_fp64"] = float(np.abs(t32 - t64).max())
     res["speedup_fp32_vs_fp64"] = res["fp64"]["time_to_solution_s"] / res["fp32"]["time_to_solution_s"]
     res["speedup_fp32_vs_fp64_run"] = res["fp64"]["run_s"] / res["fp32"]["run_s"]
+    return res, shape
+
+
+def gsis_workload(pkg, args, work):
+    """The GSIS solves alone (value: solves/hour of the FP32 solve)."""
+    n = args.n
+    res, (D, N, P) = gsis_solves(pkg, args, work, n)
     t32 = res["fp32"]["sweep_kernel_ms_fp32_mean"] or res["fp32"]["sweep_kernel_ms_fp64_mean"]
     bytes_alg = b_alg(D, N, P, 4, 4)
     workload = (f"gsis jittered channel {n}x{n} -> {N} P{args.deg} triangles, {D} ordinates "
@@ -202,9 +208,87 @@ def gsis_workload(pkg, args, work):
         "roofline": {"bound": "hbm", "achieved": bytes_alg / (t32 * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": bytes_alg / (t32 * 1e-3) / HBM_PEAK, "traffic": None,
                      "bytes_alg_per_launch": bytes_alg},
-        "gsis": res, "cpu_baseline": None,
+        "solve": res, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
+
+
+def config2_workload(pkg, args, work):
+    """The headline line: the config-2 FP32 sweep (value, roofline), the
+    FP64 vs FP32 GSIS solves of the same case (solve) and the CPU sweep
+    (cpu_baseline)."""
+    import torch
+    n = args.n
+    log(f"config 2: jittered channel {n}x{n}, P{args.deg}, {args.npole}x{args.nazim} ordinates")
+    case = gsis_case(pkg, n, args.deg, args.npole, args.nazim, 1e-3, 1e-4, "fp32", work,
+                     jitter=args.jitter if args.jitter else 0.25)
+    case.scheme.accflag = 0                  # the sweep alone: no HDG set-up
+    t0 = time.perf_counter()
+    s = pkg.Solver(case, max_bundle=args.bundle, block=args.block, sweep_variant=args.variant)
+    setup_s = time.perf_counter() - t0
+    D, N, P = s.ndir, s.n_tris, s.ndof
+    log(f"solver set up in {setup_s:.1f} s ({N} triangles); sweeping")
+    out = sweep_line(s, args, D, N, P, "fp32",
+                     f"config 2: jittered square {n}x{n} (+-0.25h, seed 0) -> {N} P{args.deg} triangles, "
+                     f"{D} ordinates ({args.npole}x{args.nazim}), Kn_R 1e-3, Kn_N 1e-4, fp32 sweep")
+    out["setup_s"] = setup_s
+    del s
+    import gc
+    gc.collect()
+    torch.cuda.empty_cache()
+    if not args.no_solve:
+        res, _ = gsis_solves(pkg, args, work, n)
+        out["solve"] = res
+    if not args.no_cpu:
+        c64 = gsis_case(pkg, n, args.deg, 4, 4, 1e-3, 1e-4, "fp64", work,
+                        jitter=args.jitter if args.jitter else 0.25)
+        out["cpu_baseline"] = cpu_baseline(pkg, c64, args.cpu_seconds,
+                                           sample=f"the {n}x{n} mesh with 16 ordinates (4x4) of the 512")
+    print(json.dumps(out), flush=True)
+
+
+def sweep_line(s, args, D, N, P, precision, workload):
+    """Time args.steps sweeps of a set-up solver (after args.warmup) and
+    build the JSON line's sweep fields."""
+    import torch
+    vdf = s.sweep_vdf
+    rng = np.random.default_rng(0)
+    s.mom.set_dofs(rng.random((P, N)), 0.1 * rng.random((P, N)), 0.1 * rng.random((P, N)))
+    for _ in range(args.warmup):
+        s.ctx.sweep(s.mom, vdf)
+    s.ctx.timing(args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.ctx.sweep(s.mom, vdf)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = s.ctx.timing_read(args.steps)
+    s.ctx.timing(0)
+    t_sweep = float(np.mean(ms)) * 1e-3
+    dof = D * N * P
+    s_lu = 4 if precision == "fp32" else 8
+    bytes_alg = b_alg(D, N, P, s_lu, s_lu)
+    return {
+        "metric": METRIC,
+        "value": dof * args.steps / wall,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if precision == "fp32" else "f64",
+        "data": "synthetic moments on the workload's mesh and ordinates",
+        "config": {"workload": workload, "n_tris": N, "ndof": P, "ndir": D, "dof_per_sweep": dof},
+        "roofline": {"bound": "hbm", "achieved": bytes_alg / t_sweep / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": bytes_alg / t_sweep / HBM_PEAK,
+                     "traffic": committed_traffic(workload),
+                     "bytes_alg_per_launch": bytes_alg},
+        "sweep_kernel_ms_mean": t_sweep * 1e3,
+        "sweep_kernel_ms_min": float(np.min(ms)),
+        "schedule": s.ctx.schedule_info(),
+        "tile_shape": s.tile_shape,
+        "factor_mode": s.ctx.mode,
+    }
 
 
 def cis_workload(pkg, args, work):
@@ -442,9 +526,11 @@ def b_alg(D, N, P, s_lu, s_f):
     return D * N * (P * P * s_lu + P + 2 * P * s_f) + N * 8 * (4 * P * P + 3 * P + 6)
 
 
-def cpu_baseline(pkg, case, seconds=15.0, max_sweeps=1000):
-    """The C restatement of the reference sweep (OpenMP over directions,
-    cells in the reference order, precomputed LU) on this host."""
+def cpu_baseline(pkg, case, seconds=15.0, max_sweeps=1000, sample="the same workload"):
+    """The C restatement of the reference sweep (binary64, OpenMP over
+    directions, cells in the reference order, precomputed LU) on this host:
+    about `seconds` of sweeps at the host's thread count (OMP_NUM_THREADS or
+    the CPU count), then as long at one thread."""
     from oracle import cpu
     from pbte_amd.problem import build_problem
     cpu.build()
@@ -453,18 +539,27 @@ def cpu_baseline(pkg, case, seconds=15.0, max_sweeps=1000):
     o.factorise()
     rng = np.random.default_rng(0)
     o.ts[:] = rng.random(o.ts.shape)
-    o.sweep()                                   # warm
-    n, t0 = 0, time.perf_counter()
-    while True:
-        o.sweep()
-        n += 1
-        el = time.perf_counter() - t0
-        if el > seconds or n >= max_sweeps:
-            break
     dof = o.D * o.N * o.P
-    return {"value": dof * n / el, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{n} FP64 sweeps of the same workload after factorisation "
-                      f"({el:.1f} s), OMP_NUM_THREADS={threads}"}
+
+    def timed(nthreads):
+        cpu.lib().orc_set_threads(nthreads)
+        o.sweep()                                   # warm
+        n, t0 = 0, time.perf_counter()
+        while True:
+            o.sweep()
+            n += 1
+            el = time.perf_counter() - t0
+            if el > seconds or n >= max_sweeps:
+                break
+        return dof * n / el, n, el
+
+    v, n, el = timed(threads)
+    v1, n1, el1 = timed(1)
+    cpu.lib().orc_set_threads(threads)
+    return {"value": v, "unit": "DOF-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{n} FP64 sweeps ({el:.1f} s) of {sample} after factorisation, "
+                      f"{threads} OpenMP threads; at 1 thread {n1} sweeps ({el1:.1f} s)",
+            "value_1_thread": v1}
 
 
 def main():
@@ -472,14 +567,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "cis", "gsis", "batch"])
+    ap.add_argument("--n", type=int, default=0, help="mesh size a side (default 256 for config2 and gsis, "
+                                                      "128 for sweep)")
+    ap.add_argument("--workload", default="config2", choices=["config2", "sweep", "cis", "gsis", "batch"])
     ap.add_argument("--batch-count", type=int, default=64, help="problems of the batch workload")
     ap.add_argument("--batch-solve-div", type=int, default=4,
                     help="batch workload: solve leg on meshes this many times coarser a side (0 skips)")
     ap.add_argument("--batch-tmax", type=int, default=5000, help="batch workload: GSIS iteration cap")
     ap.add_argument("--deg", type=int, default=2)
-    ap.add_argument("--npole", type=int, default=8)
+    ap.add_argument("--npole", type=int, default=0, help="polar ordinates (default 16 for config2 and gsis, 8)")
     ap.add_argument("--nazim", type=int, default=32)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--bundle", type=int, default=0)
@@ -519,6 +615,12 @@ def main():
     import torch
 
     work = Path(tempfile.mkdtemp(prefix="pbte_bench_"))
+    big = args.workload in ("config2", "gsis")
+    args.n = args.n or (256 if big else 128)
+    args.npole = args.npole or (16 if big else 8)
+    if args.workload == "config2":
+        config2_workload(pkg, args, work)
+        return
     if args.workload == "cis":
         cis_workload(pkg, args, work)
         return
@@ -526,8 +628,6 @@ def main():
         batch_workload(pkg, args, work)
         return
     if args.workload == "gsis":
-        if args.npole == 8 and args.nazim == 32:
-            args.npole, args.nazim = 16, 32
         gsis_workload(pkg, args, work)
         return
     log(f"mesh {args.n}x{args.n} (hole {args.hole:g}), P{args.deg}, {args.npole}x{args.nazim} ordinates")

@@ -68,6 +68,7 @@ def lib():
         h.orc_sweep.argtypes = [C.POINTER(OrcProblem), _i, _f, _f, _f, _f, _i, _f, _f, C.c_int]
         h.orc_transport_residual.argtypes = [C.POINTER(OrcProblem), _f, _f, _f, _f, _f, _f]
         h.orc_moments.argtypes = [C.POINTER(OrcProblem), _f, _f, _f, _f, _f, _f, _f]
+        h.orc_set_threads.argtypes = [C.c_int]
         h.orc_sum.restype = C.c_double
         h.orc_sum.argtypes = [_f, C.c_int]
         h.orc_residual.restype = C.c_double

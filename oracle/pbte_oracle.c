@@ -21,6 +21,7 @@
  *                          pass-number closed form and topological validity
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -276,6 +277,10 @@ ORC_API void orc_moments(const orc_problem *p, const double *vdf, double *ts, do
 }
 
 /* Sequential sum and the iterate residual (_kernels.py:340-361). */
+/* Threads of the OpenMP loops from now on (the CPU-baseline timing runs
+ * the sweep at the host's thread count and at one thread). */
+ORC_API void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 ORC_API double orc_sum(const double *a, int n)
 {
     double s = 0.0;
