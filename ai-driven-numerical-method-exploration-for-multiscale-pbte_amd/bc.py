@@ -1,9 +1,9 @@
 """Boundary data: type codes, wall temperatures, periodic partners.
 
-Follows ``solver-python/pybte/bc.py:120-170``: symmetry walls are refused,
-periodic faces get the element behind the paired face and whether the pair
-is traversed in reverse; the adiabatic emission ``flux_wall[F, P]`` lives on
-the device and is refreshed by ``pbte_wall_flux`` (``bc.py:47-58``).
+The data of ``solver-python/pybte/bc.py:120-170`` (the same fields and
+exceptions), with the periodic pairing vectorised over faces; the adiabatic
+emission ``flux_wall[F, P]`` lives on the device and is refreshed by
+``pbte_wall_flux`` (``bc.py:47-58``).
 """
 from __future__ import annotations
 
@@ -47,38 +47,55 @@ class BoundaryData:
             self._ctx.wall_flux(vdf)
 
 
+def _periodic_partners(mesh, bc_type, offsets):
+    """For every periodic face: the element behind its partner face and
+    whether the pair runs in opposite senses.  A face's trace nodes run from
+    its first node to its second; shifted by its boundary's offset, the first
+    node lands on the partner's first node (same sense) or on its second
+    (reversed: the partner's nodes are mirrored, which the HDG assembly
+    needs)."""
+    F = mesh.n_faces
+    partner_tri = np.full(F, -1, dtype=np.int32)
+    reversed_pair = np.zeros(F, dtype=bool)
+    bc = mesh.face_bc
+    code = np.where(bc >= 0, bc_type[np.clip(bc, 0, None)], -1)
+    faces = np.flatnonzero(code == BC_PERIODIC)
+    if faces.size == 0:
+        return partner_tri, reversed_pair
+    pair = mesh.face_pair[faces].astype(np.int64)
+    lonely = faces[pair < 0]
+    if lonely.size:
+        raise ValueError(f"periodic face {int(lonely[0])} has no partner face")
+    own_tri, _ = mesh.adjacent_arrays()
+    partner_tri[faces] = own_tri[pair]
+    shifted = mesh.nodes[mesh.face_nodes[faces, 0]] + offsets[bc[faces]]
+    same = np.isclose(shifted, mesh.nodes[mesh.face_nodes[pair, 0]]).all(axis=1)
+    mirrored = np.isclose(shifted, mesh.nodes[mesh.face_nodes[pair, 1]]).all(axis=1)
+    bad = ~(same | mirrored)
+    if bad.any():
+        k = int(np.flatnonzero(bad)[0])
+        f, g, b = int(faces[k]), int(pair[k]), int(bc[faces[k]])
+        raise ValueError(f"periodic face {f} does not meet its partner {g} under the offset "
+                         f"({offsets[b, 0]}, {offsets[b, 1]}) of its boundary")
+    reversed_pair[faces] = mirrored & ~same
+    return partner_tri, reversed_pair
+
+
 def build_boundary_data(mesh, boundaries, ndof_tri: int) -> BoundaryData:
+    """Per boundary its type code and wall temperature; per periodic face
+    its partner element and orientation (``bc.py:120-170``).  Symmetry walls
+    (type 4) are refused with ``NotImplementedError``: the reference carries
+    them through its mesh reader but its transport solve has none."""
     bc_type = np.array([b.code for b in boundaries], dtype=np.int32)
+    if (bc_type == BC_SYMMETRY).any():
+        raise NotImplementedError("symmetry walls (type 4) have no transport boundary condition; "
+                                  "a periodic pair or an adiabatic wall can stand in for one")
     bc_temp = np.array([float(b.temp) for b in boundaries], dtype=np.float64)
-    bc_xoff = np.array([float(b.xoff) for b in boundaries], dtype=np.float64)
-    bc_yoff = np.array([float(b.yoff) for b in boundaries], dtype=np.float64)
-    if np.any(bc_type == BC_SYMMETRY):
-        raise NotImplementedError(
-            "BC_TYP=4 (symmetry) is carried by the mesh but not implemented in the "
-            "transport solve; use a periodic pair or an adiabatic wall instead.")
-    n_faces = mesh.n_faces
-    periodic_tri = np.full(n_faces, -1, dtype=np.int32)
-    periodic_flip = np.zeros(n_faces, dtype=bool)
-    has_periodic = bool(np.any(bc_type == BC_PERIODIC))
-    if has_periodic:
-        for f in range(n_faces):
-            bc = int(mesh.face_bc[f])
-            if bc < 0 or bc_type[bc] != BC_PERIODIC:
-                continue
-            pair = int(mesh.face_pair[f])
-            if pair < 0:
-                raise ValueError(f"periodic face {f} has no partner")
-            tri, _ = mesh.adjacent(pair)
-            periodic_tri[f] = tri
-            a = mesh.nodes[mesh.face_nodes[f, 0]] + np.array([bc_xoff[bc], bc_yoff[bc]])
-            b1 = mesh.nodes[mesh.face_nodes[pair, 0]]
-            b2 = mesh.nodes[mesh.face_nodes[pair, 1]]
-            if np.allclose(a, b2) and not np.allclose(a, b1):
-                periodic_flip[f] = True
-            elif not np.allclose(a, b1):
-                raise ValueError(f"periodic faces {f} and {pair} do not line up under the "
-                                 f"declared offset ({bc_xoff[bc]}, {bc_yoff[bc]})")
-    return BoundaryData(bc_type=bc_type, bc_temp=bc_temp, periodic_tri=periodic_tri,
-                        periodic_flip=periodic_flip,
-                        has_nonthermalising=bool(np.any(bc_type == BC_NONTHERMALISING)),
-                        has_periodic=has_periodic, n_faces=n_faces, ndof=ndof_tri)
+    offsets = np.array([[float(b.xoff), float(b.yoff)] for b in boundaries],
+                       dtype=np.float64).reshape(-1, 2)
+    partner_tri, reversed_pair = _periodic_partners(mesh, bc_type, offsets)
+    return BoundaryData(bc_type=bc_type, bc_temp=bc_temp, periodic_tri=partner_tri,
+                        periodic_flip=reversed_pair,
+                        has_nonthermalising=bool((bc_type == BC_NONTHERMALISING).any()),
+                        has_periodic=bool((bc_type == BC_PERIODIC).any()),
+                        n_faces=mesh.n_faces, ndof=ndof_tri)

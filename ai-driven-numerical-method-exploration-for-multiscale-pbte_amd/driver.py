@@ -17,7 +17,6 @@ The distribution and moments stay in HBM for the whole run; ``vdf`` and the
 """
 from __future__ import annotations
 
-import hashlib
 import time
 import warnings
 import weakref
@@ -26,6 +25,7 @@ from pathlib import Path
 import numpy as np
 
 from .config import Case
+from .restart import read_restart, restart_name, write_restart
 from .device import dev_zeros, require_gpu, torch
 from .io_output import RunRecord, _fields, append_runtime_log, locate_points, sampling_grid, write_tecplot
 from .moments import Moments
@@ -124,7 +124,11 @@ class Solver:
                                       self.bcdata, max_bundle=max_bundle, block=block,
                                       variant=sweep_variant, lds_slots_cap=lds_slots_cap,
                                       max_bytes=max_bytes)
-        self.factorisation_count = self.ndir * self.n_tris
+        # element operators factorised (driver.py:84-85, 155-156; krylov.py:
+        # 119-120): D N once when stored ("precomputed", and the binary32 ones
+        # of "mixed", whose binary64 sweeps refine against the stored binary32
+        # inverse and factorise nothing), plus D N per "onthefly" sweep
+        self._stored_factorisations = self.ndir * self.n_tris if self.ctx.mode != "onthefly" else 0
         self.precision = c.performance.precision
         self.sweep_precision = self.precision     # fp32 runs finish in fp64
         self.vdf_d = dev_zeros((self.ndir, self.n_tris, self.ndof))
@@ -136,13 +140,17 @@ class Solver:
         self.ctx._solver = weakref.ref(self)
         self.bcdata._ctx = self.ctx
         self.t_old_d = dev_zeros(self.n_tris)
-        self.restart_info = self._apply_restart()
         self.acc = None
+        self.restart_info = self._apply_restart()
         if c.scheme.accflag == 1:
             from .acceleration import Acceleration
             self.acc = Acceleration(self)
         torch.cuda.synchronize()
         self.setup_seconds = time.perf_counter() - t0
+
+    @property
+    def factorisation_count(self) -> int:
+        return self._stored_factorisations + self.ctx.factorisations
 
     # -- state views ---------------------------------------------------------
     @property
@@ -209,40 +217,34 @@ class Solver:
         return np.transpose(self.vdf.reshape(nazim, npole, self.n_tris, self.ndof), (3, 2, 1, 0))
 
     # -- restart (driver.py:100-139) ------------------------------------------
-    def _restart_stem(self) -> str:
+    def _restart_default(self) -> Path:
         c = self.case
-        return (f"VDFP{c.dg.deg}T{self.n_tris:3d}"
-                f"NP{c.velmesh.npole:2d}NA{c.velmesh.nazim:2d}").replace(" ", "")
+        return Path(c.basedir) / restart_name(c.dg.deg, self.n_tris, c.velmesh.npole, c.velmesh.nazim)
 
     def _apply_restart(self) -> dict:
+        """Start from a ``VDF*.out`` file when ``restart.enabled``.  With
+        restart off, a file of this run's name in the case directory is an
+        error unless ``restart.error_on_stale`` is off: the Fortran code
+        would have read it without a word."""
         r = self.case.restart
-        default = Path(self.case.basedir) / f"{self._restart_stem()}.out"
+        off = {"enabled": False, "path": None, "sha256": None}
+        default = self._restart_default()
         if not r.enabled:
             if r.error_on_stale and default.exists():
-                raise FileExistsError(f"a restart file {default} is present but restart is "
-                                      "disabled; delete it or set restart.enabled / "
-                                      "restart.error_on_stale explicitly")
-            return {"enabled": False, "path": None, "sha256": None}
-        path = Path(r.path) if r.path else default
-        if not path.is_absolute():
-            path = Path(self.case.basedir) / path
+                raise FileExistsError(f"{default} exists while restart is off; remove it, or set "
+                                      "restart.enabled or turn restart.error_on_stale off")
+            return off
+        path = default if not r.path else Path(r.path)
+        path = path if path.is_absolute() else Path(self.case.basedir) / path
         if not path.exists():
-            raise FileNotFoundError(f"restart enabled but {path} does not exist")
-        raw = path.read_bytes()
-        want = self.ndof * self.n_tris * self.ndir * 8
-        if len(raw) != want:
-            raise ValueError(f"{path}: {len(raw)} bytes, expected {want} "
-                             "(wrong DEG / mesh / angular resolution?)")
-        arr = np.frombuffer(raw, dtype="<f8").reshape(
-            (self.ndof, self.n_tris, self.case.velmesh.npole, self.case.velmesh.nazim), order="F")
-        self.vdf = np.transpose(arr, (3, 2, 1, 0)).reshape(self.ndir, self.n_tris, self.ndof)
-        return {"enabled": True, "path": str(path), "sha256": hashlib.sha256(raw).hexdigest()}
+            raise FileNotFoundError(f"restart.enabled is set and {path} is missing")
+        arr, digest = read_restart(path, (self.ndir, self.n_tris, self.ndof))
+        self.vdf = arr
+        return {"enabled": True, "path": str(path), "sha256": digest}
 
     def save_restart(self, path=None) -> Path:
-        path = Path(path) if path else Path(self.case.basedir) / f"{self._restart_stem()}.out"
-        path.parent.mkdir(parents=True, exist_ok=True)
-        path.write_bytes(self.vdf_fortran().ravel(order="F").tobytes())
-        return path
+        """Write the distribution as a ``VDF*.out`` restart file."""
+        return write_restart(Path(path) if path else self._restart_default(), self.vdf)
 
     # -- one outer iteration (driver.py:149-161) -------------------------------
     def settle(self) -> None:

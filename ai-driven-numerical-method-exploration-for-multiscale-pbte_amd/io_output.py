@@ -10,8 +10,7 @@ from __future__ import annotations
 
 import json
 import platform
-import sys
-from dataclasses import asdict, dataclass, field
+from dataclasses import dataclass, field, fields
 from pathlib import Path
 
 import numpy as np
@@ -84,47 +83,68 @@ class RunRecord:
             return float("nan")
         return float(self.final_residual * rho / (1.0 - rho))
 
+    # properties that the record's serialised forms carry beside its fields
+    DERIVED = ("final_residual", "mass", "contraction", "error_estimate")
+
+    def _arrays(self) -> dict:
+        return {f.name: getattr(self, f.name) for f in fields(self)
+                if isinstance(getattr(self, f.name), np.ndarray)}
+
     def to_dict(self, include_fields: bool = True) -> dict:
-        d = asdict(self)
-        for k, v in list(d.items()):
+        """Every field as plain Python values (arrays as lists; the 2-D DOF
+        arrays ``None`` when ``include_fields`` is false), then the derived
+        quantities of ``DERIVED``."""
+        out = {}
+        for f in fields(self):
+            v = getattr(self, f.name)
             if isinstance(v, np.ndarray):
-                d[k] = None if (not include_fields and v.ndim > 1) else v.tolist()
-        d["final_residual"] = self.final_residual
-        d["mass"] = self.mass
-        d["contraction"] = self.contraction
-        d["error_estimate"] = self.error_estimate
-        return d
+                v = v.tolist() if (include_fields or v.ndim <= 1) else None
+            elif isinstance(v, dict):
+                v = json.loads(json.dumps(v, default=_json_default))
+            out[f.name] = v
+        out.update({name: getattr(self, name) for name in self.DERIVED})
+        return out
 
     def to_json(self, path, include_fields: bool = True) -> Path:
+        """``run_record.json``: ``to_dict`` as indented JSON."""
         path = Path(path)
         path.parent.mkdir(parents=True, exist_ok=True)
-        path.write_text(json.dumps(self.to_dict(include_fields), indent=2))
+        path.write_text(json.dumps(self.to_dict(include_fields), indent=2, default=_json_default))
         return path
 
     def to_npz(self, path) -> Path:
+        """The arrays of the record as they are, and everything else as the
+        JSON string ``meta``."""
         path = Path(path)
         path.parent.mkdir(parents=True, exist_ok=True)
-        payload = {k: v for k, v in asdict(self).items() if isinstance(v, np.ndarray)}
-        payload["meta"] = np.array(json.dumps(self.to_dict(include_fields=False)))
-        np.savez_compressed(path, **payload)
+        meta = json.dumps(self.to_dict(include_fields=False), default=_json_default)
+        np.savez_compressed(path, meta=np.array(meta), **self._arrays())
         return path
 
     @staticmethod
     def environment_fingerprint() -> dict:
-        import numpy as _np
-        try:
-            import scipy
-            scipy_v = scipy.__version__
-        except ImportError:  # pragma: no cover
-            scipy_v = None
-        try:
-            import torch
-            torch_v = torch.__version__
-        except ImportError:  # pragma: no cover
-            torch_v = None
-        return {"python": sys.version.split()[0], "numpy": _np.__version__, "scipy": scipy_v,
-                "numba": None, "torch": torch_v, "platform": platform.platform(),
-                "machine": platform.machine(), "backend": "hip-gfx950"}
+        """Interpreter, the numerical packages' versions (None when absent;
+        numba is never used here) and the machine."""
+        def version(module):
+            try:
+                return __import__(module).__version__
+            except ImportError:
+                return None
+        env = {"python": platform.python_version()}
+        env.update({m: version(m) for m in ("numpy", "scipy")})
+        env["numba"] = None
+        env["torch"] = version("torch")
+        env.update(platform=platform.platform(), machine=platform.machine(), backend="hip-gfx950")
+        return env
+
+
+def _json_default(v):
+    """JSON form of the numpy values a record's dicts may hold."""
+    if isinstance(v, np.ndarray):
+        return v.tolist()
+    if isinstance(v, np.generic):
+        return v.item()
+    raise TypeError(f"{type(v).__name__} is not JSON serialisable")
 
 
 # ---------------------------------------------------------------------------

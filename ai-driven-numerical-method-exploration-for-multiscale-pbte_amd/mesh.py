@@ -45,59 +45,72 @@ class GmshMesh:
     physical_names: dict
 
 
+def _sections(text: str) -> dict:
+    """The ``$Name ... $EndName`` blocks of a gmsh file, as their stripped
+    lines (the first occurrence of each name)."""
+    out, name, body = {}, None, None
+    for raw in text.splitlines():
+        line = raw.strip()
+        if name is not None and line == f"$End{name}":
+            out.setdefault(name, body)
+            name = None
+        elif name is None and line.startswith("$") and not line.startswith("$End"):
+            name, body = line[1:], []
+        elif name is not None:
+            body.append(line)
+    return out
+
+
+def _counted(path, sec, name):
+    """The item lines of a section whose first line is their count."""
+    if not sec:
+        raise ValueError(f"{path}: ${name} section is empty")
+    count = int(sec[0])
+    if len(sec) - 1 < count:
+        raise ValueError(f"{path}: ${name} announces {count} entries and holds {len(sec) - 1}")
+    return sec[1:1 + count]
+
+
 def read_gmsh22(path) -> GmshMesh:
-    """Read ``$Nodes``/``$Elements`` (types 1 and 2 only, >= 2 tags)."""
+    """A gmsh 2.2 ASCII file (``mesh/gmsh_reader.py:45-122``): nodes (ids
+    must run 1..N in order: the node index is the position in the file),
+    line and triangle elements (at least two tags; the first is the
+    physical id), physical names.  Any other element type is an error."""
     path = Path(path)
-    lines = path.read_text().splitlines()
-    nodes = None
-    etype_l, phy_l, en_l = [], [], []
+    secs = _sections(path.read_text())
+    fmt = secs.get("MeshFormat")
+    if fmt:
+        version, ftype = fmt[0].split()[:2]
+        if not version.startswith("2.2"):
+            raise ValueError(f"{path}: gmsh format {version}, this reader takes 2.2")
+        if ftype != "0":
+            raise ValueError(f"{path}: binary gmsh file; this reader takes ASCII")
     names = {}
-    i, n = 0, len(lines)
-    while i < n:
-        tag = lines[i].strip()
-        if tag == "$MeshFormat":
-            ver = lines[i + 1].split()
-            if not ver[0].startswith("2.2"):
-                raise ValueError(f"{path}: expected gmsh ASCII 2.2, got {ver[0]}")
-            if ver[1] != "0":
-                raise ValueError(f"{path}: binary gmsh files are not supported")
-            i += 2
-        elif tag == "$PhysicalNames":
-            cnt = int(lines[i + 1])
-            for k in range(cnt):
-                p = lines[i + 2 + k].split(None, 2)
-                names[(int(p[0]), int(p[1]))] = p[2].strip().strip('"')
-            i += 2 + cnt
-        elif tag == "$Nodes":
-            cnt = int(lines[i + 1])
-            block = np.array([lines[i + 2 + k].split()[:3] for k in range(cnt)],
-                             dtype=np.float64).reshape(cnt, 3)
-            ids = block[:, 0].astype(np.int64)
-            if not np.array_equal(ids, np.arange(1, cnt + 1)):
-                raise ValueError(f"{path}: node ids are not 1..{cnt} in order")
-            nodes = np.ascontiguousarray(block[:, 1:3])
-            i += 2 + cnt
-        elif tag == "$Elements":
-            cnt = int(lines[i + 1])
-            for k in range(cnt):
-                p = lines[i + 2 + k].split()
-                et, ntags = int(p[1]), int(p[2])
-                if et not in (LINE_ELEMENT, TRIANGLE_ELEMENT):
-                    raise ValueError(f"{path}: element {p[0]} has unsupported type {et}")
-                if ntags < 2:
-                    raise ValueError(f"{path}: element {p[0]} has {ntags} tags, need >= 2")
-                nn = 2 if et == LINE_ELEMENT else 3
-                etype_l.append(et)
-                phy_l.append(int(p[3]))
-                en_l.append(tuple(int(v) - 1 for v in p[3 + ntags:3 + ntags + nn]))
-            i += 2 + cnt
-        else:
-            i += 1
-    if nodes is None:
+    for line in _counted(path, secs["PhysicalNames"], "PhysicalNames") if "PhysicalNames" in secs else ():
+        dim, tag, label = line.split(None, 2)
+        names[(int(dim), int(tag))] = label.strip().strip('"')
+    if "Nodes" not in secs:
         raise ValueError(f"{path}: no $Nodes section")
-    return GmshMesh(nodes=nodes, elem_type=np.asarray(etype_l, dtype=np.int64),
-                    elem_phyid=np.asarray(phy_l, dtype=np.int64), elem_nodes=en_l,
-                    physical_names=names)
+    rows = _counted(path, secs["Nodes"], "Nodes")
+    table = np.array([r.split()[:3] for r in rows], dtype=np.float64).reshape(len(rows), 3)
+    if not np.array_equal(table[:, 0], np.arange(1, len(rows) + 1)):
+        raise ValueError(f"{path}: node ids do not run 1..{len(rows)} in file order")
+    kinds, phys, conn = [], [], []
+    for line in _counted(path, secs.get("Elements", ["0"]), "Elements"):
+        tok = [int(v) for v in line.split()]
+        eid, kind, ntags = tok[0], tok[1], tok[2]
+        width = {LINE_ELEMENT: 2, TRIANGLE_ELEMENT: 3}.get(kind)
+        if width is None:
+            raise ValueError(f"{path}: element {eid} is of type {kind}; only lines (1) and "
+                             "triangles (2) are read")
+        if ntags < 2:
+            raise ValueError(f"{path}: element {eid} carries {ntags} tags; the physical and "
+                             "geometrical ones are both needed")
+        kinds.append(kind)
+        phys.append(tok[3])
+        conn.append(tuple(v - 1 for v in tok[3 + ntags:3 + ntags + width]))
+    return GmshMesh(nodes=np.ascontiguousarray(table[:, 1:3]), elem_type=np.asarray(kinds, dtype=np.int64),
+                    elem_phyid=np.asarray(phys, dtype=np.int64), elem_nodes=conn, physical_names=names)
 
 
 def triangle_geometry(nodes: np.ndarray, tri_nodes: np.ndarray):

@@ -55,6 +55,7 @@ class SweepContext:
     def __init__(self):
         self.handle = None
         self.sweep_count = 0
+        self.factorisations = 0                 # element LUs of on-the-fly sweeps
         self.mode = "precomputed"
         self._precomputed = set()               # precisions with stored factors
         self.kernel = "hip"
@@ -259,6 +260,9 @@ class SweepContext:
                 1 if self.use_pbuf else 0, 1 if homogeneous else 0)
         _native.check(self.handle, st, "sweep")
         self.sweep_count += 1
+        if self.mode == "onthefly":
+            # every element operator was built and LU-factorised for this sweep
+            self.factorisations += self.ndir * self.n_tris
 
     def moments(self, vdf, mom, out=None) -> None:
         """Moments of ``vdf`` into ``mom`` (or into ``out`` = 6 device tensors)."""
