@@ -150,6 +150,15 @@ def lib():
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with build_native.py "
                                "(there is no CPU fallback)")
+        # the HIP runtime (PyTorch's) is brought up on the device before the
+        # library's code objects register with it: registered first, the
+        # library's later device allocations have failed with "no
+        # ROCm-capable device".  Without a device (build and host-only
+        # calls) there is nothing to bring up.
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+            torch.cuda.current_stream()
         handle = C.CDLL(str(LIB_PATH))
         for name, (res, args) in _SIGS.items():
             fn = getattr(handle, name)
