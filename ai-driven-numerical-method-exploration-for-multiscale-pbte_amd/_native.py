@@ -128,6 +128,10 @@ _SIGS = {
     "pbte_hdg_assemble": (C.c_int, [C.POINTER(PbteHdgDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                     C.c_void_p, C.c_void_p, _i64p, C.c_void_p, C.c_char_p, C.c_int32]),
+    "pbte_hdg_assemble_static": (C.c_int, [C.POINTER(PbteHdgDesc), C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, _i64p, C.c_void_p,
+                                           C.c_char_p, C.c_int32]),
     "pbte_trace_create": (C.c_void_p, [C.POINTER(PbteTraceDesc), C.c_void_p, C.c_char_p, C.c_int32]),
     "pbte_trace_destroy": (None, [C.c_void_p]),
     "pbte_trace_last_error": (C.c_char_p, [C.c_void_p]),

@@ -23,7 +23,9 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "pbte_hip.h"
 
@@ -401,6 +403,61 @@ extern "C" int pbte_hdg_local(const pbte_hdg_desc* d, double* inv_aa_sol, double
     });
 }
 
+namespace {
+__global__ void k_zero_u64(unsigned long long* p)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
+}
+
+// the assembly's launches: counter cleared (a kernel, which a captured
+// graph may hold), the block-row index of every block, the block values
+void launch_assemble(const HdgArgs& a, const int32_t* t1, const int32_t* l1, const int32_t* t2, const int32_t* l2,
+                     const uint8_t* mirror, const int32_t* tri_faces, const int32_t* bsr_ptr, const int32_t* bsr_col,
+                     int64_t nnzb, const double* aa_trace, const double* ba_sol, double* bsr_val, int32_t* row,
+                     unsigned long long* cnt, hipStream_t st)
+{
+    const int nq = 3 * a.Q;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, cnt);
+    hipLaunchKernelGGL(k_row_of_block, dim3((a.F + 255) / 256), dim3(256), 0, st, bsr_ptr, a.F, row);
+    hipLaunchKernelGGL(k_assemble_trace, dim3(grid_for(nnzb * nq * nq, 256)), dim3(256), 0, st, a, t1, l1, t2, l2,
+                       mirror, tri_faces, row, bsr_col, nnzb, aa_trace, ba_sol, bsr_val, cnt);
+}
+
+// Device scratch of the allocation-free assembly, one copy per GPU for the
+// life of the process: the nonzero counter at offset 0, the block-row
+// indices from offset 256.  Calls that overlap on one GPU share it and race.
+constexpr size_t kScratchBytes = 16u << 20;
+constexpr int64_t kScratchMaxBlocks = (int64_t)(kScratchBytes - 256) / 4;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// the scratch's device address on the current device, looked up once per
+// device (the array's host-side name is not a device address)
+void* scratch_address()
+{
+    static std::mutex mu;
+    static std::vector<void*> addr;
+    int dev = 0;
+    HD_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        HD_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        addr[dev] = p;
+    }
+    return addr[dev];
+}
+
+void read_count(unsigned long long* cnt, int64_t* nnz_host, hipStream_t st)
+{
+    unsigned long long h = 0;
+    HD_TRY(hipGetLastError());
+    HD_TRY(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+    HD_TRY(hipStreamSynchronize(st));
+    if (nnz_host) *nnz_host = (int64_t)h;
+}
+}  // namespace
+
 extern "C" int pbte_hdg_assemble(const pbte_hdg_desc* d, const int32_t* t1, const int32_t* l1, const int32_t* t2,
                                  const int32_t* l2, const uint8_t* mirror, const int32_t* tri_faces,
                                  const int32_t* bsr_ptr, const int32_t* bsr_col, int64_t nnzb,
@@ -410,25 +467,47 @@ extern "C" int pbte_hdg_assemble(const pbte_hdg_desc* d, const int32_t* t1, cons
     return hguard(err, errlen, [&] {
         const HdgArgs a = make_args(d);
         hipStream_t st = static_cast<hipStream_t>(stream);
-        const int nq = 3 * a.Q;
         int32_t* row = nullptr;
         unsigned long long* cnt = nullptr;
         HD_TRY(hipMalloc(&row, std::max<int64_t>(nnzb, 1) * sizeof(int32_t)));
-        HD_TRY(hipMalloc(&cnt, sizeof(unsigned long long)));
-        HD_TRY(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st));
-        hipLaunchKernelGGL(k_row_of_block, dim3((a.F + 255) / 256), dim3(256), 0, st, bsr_ptr, a.F, row);
-        hipLaunchKernelGGL(k_assemble_trace, dim3(grid_for(nnzb * nq * nq, 256)), dim3(256), 0, st, a, t1, l1, t2,
-                           l2, mirror, tri_faces, row, bsr_col, nnzb, aa_trace, ba_sol, bsr_val, cnt);
-        unsigned long long h = 0;
-        hipError_t e1 = hipGetLastError();
-        hipError_t e2 = hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, st);
-        hipError_t e3 = hipStreamSynchronize(st);
+        if (hipMalloc(&cnt, sizeof(unsigned long long)) != hipSuccess) {
+            (void)hipFree(row);
+            throw std::string("hdg_assemble: hipMalloc of the counter failed");
+        }
+        try {
+            launch_assemble(a, t1, l1, t2, l2, mirror, tri_faces, bsr_ptr, bsr_col, nnzb, aa_trace, ba_sol, bsr_val,
+                            row, cnt, st);
+            read_count(cnt, nnz_host, st);
+        } catch (...) {
+            (void)hipFree(row);
+            (void)hipFree(cnt);
+            throw;
+        }
         (void)hipFree(row);
         (void)hipFree(cnt);
-        HD_TRY(e1);
-        HD_TRY(e2);
-        HD_TRY(e3);
-        if (nnz_host) *nnz_host = (int64_t)h;
+        return 0;
+    });
+}
+
+extern "C" int pbte_hdg_assemble_static(const pbte_hdg_desc* d, const int32_t* t1, const int32_t* l1,
+                                        const int32_t* t2, const int32_t* l2, const uint8_t* mirror,
+                                        const int32_t* tri_faces, const int32_t* bsr_ptr, const int32_t* bsr_col,
+                                        int64_t nnzb, const double* aa_trace, const double* ba_sol,
+                                        double* bsr_val, int64_t* nnz_host, void* stream, char* err,
+                                        int32_t errlen)
+{
+    return hguard(err, errlen, [&] {
+        if (nnzb < 0 || nnzb > kScratchMaxBlocks)
+            throw std::string("hdg_assemble_static: ") + std::to_string(nnzb) + " blocks exceed the " +
+                std::to_string(kScratchMaxBlocks) + " the 16 MiB device scratch holds";
+        const HdgArgs a = make_args(d);
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        unsigned char* base = static_cast<unsigned char*>(scratch_address());
+        auto* cnt = reinterpret_cast<unsigned long long*>(base);
+        auto* row = reinterpret_cast<int32_t*>(base + 256);
+        launch_assemble(a, t1, l1, t2, l2, mirror, tri_faces, bsr_ptr, bsr_col, nnzb, aa_trace, ba_sol, bsr_val, row,
+                        cnt, st);
+        read_count(cnt, nnz_host, st);
         return 0;
     });
 }

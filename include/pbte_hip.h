@@ -378,6 +378,21 @@ int pbte_hdg_assemble(const pbte_hdg_desc *d, const int32_t *t1, const int32_t *
                       int64_t nnzb, const double *aa_trace, const double *ba_sol, double *bsr_val,
                       int64_t *nnz_host, void *stream, char *err, int32_t errlen);
 
+/* pbte_hdg_assemble without any allocation: the same launches and the same
+ * bits, with the block-row indices and the nonzero counter in a 16 MiB
+ * file-scope device array (one per GPU, held until the process ends)
+ * instead of hipMalloc'd buffers, so at most 4 194 240 blocks (-2 and a
+ * message beyond).  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, a serving
+ * loop re-assembling K per parameter).  Calls that overlap on one GPU share
+ * the array and race. */
+int pbte_hdg_assemble_static(const pbte_hdg_desc *d, const int32_t *t1, const int32_t *l1,
+                             const int32_t *t2, const int32_t *l2, const uint8_t *mirror,
+                             const int32_t *tri_faces, const int32_t *bsr_ptr,
+                             const int32_t *bsr_col, int64_t nnzb, const double *aa_trace,
+                             const double *ba_sol, double *bsr_val, int64_t *nnz_host,
+                             void *stream, char *err, int32_t errlen);
+
 /* ---- HDG global trace solve on the device ------------------------------
  * K u = f for the trace matrix of the HDG synthetic system, by a
  * nested-dissection multifrontal elimination in binary64 (dense fronts,
