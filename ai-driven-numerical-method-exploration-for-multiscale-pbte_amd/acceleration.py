@@ -209,7 +209,11 @@ class Acceleration:
         self.pending = False
         self.coef_d = dev_zeros((3, P, N)) if self.lazy else None
         read = np.flatnonzero((mesh.face_bc >= 0) & ((code == BC_NONTHERMALISING) | (code == BC_PERIODIC)))
-        self.boundary_cells_d = to_dev(np.unique(t1[read]).astype(np.int32)) if read.size else None
+        bcells = np.unique(t1[read]).astype(np.int32)
+        self.boundary_cells_d = to_dev(bcells) if read.size else None
+        rest = np.setdiff1d(np.arange(N, dtype=np.int32), bcells).astype(np.int32)
+        self.rest_cells_d = to_dev(rest) if self.lazy and rest.size else None
+        self._state_coef: dict = {}       # per distribution buffer: its correction, pending or None
         self.omega = c.scheme.defect_omega
         self.defect_every = c.scheme.defect_every
         self.anderson = c.scheme.defect_anderson
@@ -271,6 +275,53 @@ class Acceleration:
         _native.check(h, _native.lib().pbte_gsis_apply_correction(h, prec, ptr(vdf), ptr(self.coef_d), None, 0),
                       "gsis_apply_correction")
         self.pending = False
+
+    def correct_state(self, vdf, out) -> None:
+        """The correction of a swept distribution for the fixed-point
+        operator of the refined iteration: the corrected moments into
+        ``out = (ts, qxs, qys, temp, qx, qy)``, the correction applied to
+        the boundary elements of ``vdf`` (those the state's emission and
+        periodic blocks are read from) and left pending for the rest
+        (``finish_state``)."""
+        h = self.solver.ctx.handle
+        lib = _native.lib()
+        prec = _native.PBTE_FP64 if vdf.dtype == torch.float64 else _native.PBTE_FP32
+        _native.check(h, lib.pbte_gsis_moments(h, prec, ptr(vdf), ptr(self.mom7)), "gsis_moments")
+        _native.check(h, lib.pbte_gsis_source(h, ptr(self.mom7), ptr(self.aa_src_d)), "gsis_source")
+        _native.check(h, lib.pbte_gsis_rhs(h, prec, ptr(vdf), ptr(self.aa_src_d), ptr(self.ffa_d)), "gsis_rhs")
+        self.trace_solve()
+        _native.check(h, lib.pbte_gsis_local(h, ptr(self.aa_src_d), ptr(self.u_d), ptr(self.uq_d)), "gsis_local")
+        key = vdf.data_ptr()
+        coef = self._state_coef.get(key)
+        if coef is None:
+            coef = self._state_coef[key] = [dev_zeros(self.coef_d.shape), False]
+        _native.check(h, lib.pbte_gsis_correct_moments(h, ptr(self.mom7), ptr(self.uq_d), None,
+                                                       *(ptr(t) for t in out), None, ptr(coef[0])),
+                      "gsis_correct_moments")
+        cells = self.boundary_cells_d
+        if cells is not None:
+            _native.check(h, lib.pbte_gsis_apply_correction(h, prec, ptr(vdf), ptr(coef[0]), ptr(cells),
+                                                            int(cells.numel())), "gsis_apply_correction")
+        coef[1] = True
+        self._ncall += 1
+
+    def finish_state(self, vdf) -> None:
+        """Apply the correction ``correct_state`` left pending to the other
+        elements of ``vdf``."""
+        coef = self._state_coef.get(vdf.data_ptr())
+        if coef is None or not coef[1]:
+            return
+        h = self.solver.ctx.handle
+        prec = _native.PBTE_FP64 if vdf.dtype == torch.float64 else _native.PBTE_FP32
+        cells = self.rest_cells_d
+        if cells is not None:
+            _native.check(h, _native.lib().pbte_gsis_apply_correction(
+                h, prec, ptr(vdf), ptr(coef[0]), ptr(cells), int(cells.numel())), "gsis_apply_correction")
+        coef[1] = False
+
+    def drop_state(self) -> None:
+        """Release the refined iteration's correction buffers."""
+        self._state_coef.clear()
 
     def swept(self) -> None:
         """The sweep has rewritten the distribution: nothing is pending."""

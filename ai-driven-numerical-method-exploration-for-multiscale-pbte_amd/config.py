@@ -14,7 +14,7 @@ Additions of this build, none of which changes the discrete problem:
   ``"numpy"`` are accepted as aliases so that shipped files keep working;
 * ``performance.precision`` -- ``"fp64"`` (every sweep in binary64) or
   ``"fp32"`` (binary32 sweeps inside a binary64 correction);
-* ``performance.refine_switch``, ``refine_tol``, ``autotune``,
+* ``performance.refine_inner``, ``refine_switch``, ``refine_tol``, ``autotune``,
   ``autotune_cache``, ``trace_leaf`` -- documented at their fields;
 * ``scheme.krylov_restart`` -- the GMRES basis size (the reference derives
   it from a byte budget with a floor of 32, ``krylov.py:249-251``).
@@ -202,7 +202,12 @@ class Performance(_Section):
     kernel: str = _key("hip", _kernel)
     threads: int = _key(0, int)
     precision: str = _key("fp64", _lower)
-    # fp32 source iteration (CIS / GSIS): binary32 sweeps until the iterate
+    # fp32 source iteration (CIS / GSIS): binary32 iterations solve for the
+    # correction of a binary64 iterate; each inner solve ends when its step
+    # is this fraction of the binary64 step it corrects
+    refine_inner: float = _key(1.0e-3, float)
+    # fp32 source iteration where the correction does not apply (broken
+    # sweep cycles, the defect repair on): binary32 sweeps until the iterate
     # step falls below this (or stalls), then binary64 sweeps to the tolerance
     refine_switch: float = _key(1.0e-6, float)
     # fp32 Krylov: 0 decides convergence as the binary64 driver does (the
@@ -222,6 +227,7 @@ class Performance(_Section):
         (lambda s: s.threads >= 0, "performance.threads must not be negative"),
         (lambda s: s.precision in ("fp64", "fp32"), "performance.precision: expected 'fp64' or 'fp32'"),
         (lambda s: s.refine_switch > 0.0, "performance.refine_switch must be positive"),
+        (lambda s: 0.0 < s.refine_inner < 1.0, "performance.refine_inner must lie in (0, 1)"),
         (lambda s: s.refine_tol >= 0.0, "performance.refine_tol must not be negative"),
         (lambda s: s.trace_leaf >= 0, "performance.trace_leaf must not be negative"),
     )

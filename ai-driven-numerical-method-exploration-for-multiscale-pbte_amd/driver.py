@@ -186,7 +186,9 @@ class Solver:
     def vdf(self, value) -> None:
         if self.acc is not None:
             self.acc.swept()          # replaced: a pending correction no longer applies
-        self.sweep_vdf.copy_(torch.as_tensor(np.ascontiguousarray(value)).to(self.sweep_vdf.dtype))
+        self.vdf_d.copy_(torch.as_tensor(np.ascontiguousarray(value, dtype=np.float64)))
+        if self.sweep_precision == "fp32":
+            self.vdf32_d.copy_(self.vdf_d)
         self._mirror.dirty = False
 
     def device_vdf(self, vdf):
@@ -271,6 +273,10 @@ class Solver:
         if self.case.scheme.method == "krylov":
             from .krylov import run_krylov
             return run_krylov(self, callback, progress_every)
+        if self.precision == "fp32" and self.sweep_precision == "fp32":
+            from .refine import applicable, run_refined
+            if applicable(self)[0]:
+                return run_refined(self, callback, progress_every)
         return self._run_source(callback, progress_every)
 
     def _run_source(self, callback=None, progress_every: int = 0) -> RunRecord:

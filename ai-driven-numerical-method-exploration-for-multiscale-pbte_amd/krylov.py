@@ -50,10 +50,13 @@ def applicable(solver) -> tuple[bool, str]:
 
 
 class FixedPointOperator:
-    """``u -> F(u)`` on device vectors, one sweep apiece."""
+    """``u -> F(u)`` on device vectors, one sweep apiece; with ``acc`` (an
+    ``acceleration.Acceleration``) each sweep is followed by the GSIS
+    correction, and the state is that of the corrected distribution."""
 
-    def __init__(self, solver):
+    def __init__(self, solver, acc=None):
         self.s = solver
+        self.acc = acc
         self.ctx = solver.ctx
         self.nd, self.nt = solver.ndof, solver.n_tris
         self.nmom = 3 * self.nd * self.nt
@@ -98,7 +101,10 @@ class FixedPointOperator:
         if precision == "fp64":
             self.sweeps64 += 1
         ots, oqxs, oqys, ofw, opb = self._blocks(out)
-        ctx.moments(vdf, None, out=(ots, oqxs, oqys, self._temp, self._qx, self._qy))
+        if self.acc is not None:
+            self.acc.correct_state(vdf, (ots, oqxs, oqys, self._temp, self._qx, self._qy))
+        else:
+            ctx.moments(vdf, None, out=(ots, oqxs, oqys, self._temp, self._qx, self._qy))
         if self.wall:
             ctx.wall_flux(vdf, out=ofw)
         if self.per:
@@ -113,16 +119,19 @@ class FixedPointOperator:
         if self._last == "fp64":
             self.sweeps64 -= 1
 
-    def publish(self, u) -> None:
-        """Copy a state's moment block (and emission) into the solver."""
+    def publish(self, u, cells=None) -> None:
+        """Copy a state's moment block (and emission) into the solver; the
+        element moments are the last product's, or ``cells`` = (temp, qx,
+        qy) when ``u`` is not that product."""
         s = self.s
         ts, qxs, qys, fw, _ = self._blocks(u)
         s.mom.ts_d.copy_(ts.view(self.nd, self.nt))
         s.mom.qxs_d.copy_(qxs.view(self.nd, self.nt))
         s.mom.qys_d.copy_(qys.view(self.nd, self.nt))
-        s.mom.temp_d.copy_(self._temp)
-        s.mom.qx_d.copy_(self._qx)
-        s.mom.qy_d.copy_(self._qy)
+        temp, qx, qy = cells if cells is not None else (self._temp, self._qx, self._qy)
+        s.mom.temp_d.copy_(temp)
+        s.mom.qx_d.copy_(qx)
+        s.mom.qy_d.copy_(qy)
         if self.wall:
             s.ctx.flux_wall.copy_(fw)
 

@@ -169,12 +169,21 @@ def gsis_solves(pkg, args, work, n):
         fields[prec] = (rec.temp, np.concatenate([rec.qx, rec.qy]))
         shape = (s.ndir, s.n_tris, s.ndof)
         k32 = int(rec.diagnostics.get("steps_fp32", 0))
+        # which sweeps ran in binary64: the refined iteration lists its
+        # binary64 steps; the switch runs its binary32 sweeps first
+        is64 = np.ones(len(ms), dtype=bool)
+        if "fp64_steps" in rec.diagnostics:
+            is64[:] = False
+            is64[np.asarray(rec.diagnostics["fp64_steps"], dtype=np.int64) - 1] = True
+        else:
+            is64[:k32] = False
         res[prec] = {"time_to_solution_s": setup + run, "setup_s": setup, "run_s": run,
                      "solves_per_hour": 3600.0 / (setup + run), "iterations": rec.iterations,
                      "converged": rec.converged, "mass": rec.mass, "steps_fp32": k32,
-                     "sweeps": len(ms),
-                     "sweep_kernel_ms_fp32_mean": float(np.mean(ms[:k32])) if k32 else None,
-                     "sweep_kernel_ms_fp64_mean": float(np.mean(ms[k32:])) if len(ms) > k32 else None,
+                     "steps_fp64": int(is64.sum()), "sweeps": len(ms),
+                     "precision_scheme": rec.diagnostics.get("precision_scheme", prec),
+                     "sweep_kernel_ms_fp32_mean": float(np.mean(ms[~is64])) if (~is64).any() else None,
+                     "sweep_kernel_ms_fp64_mean": float(np.mean(ms[is64])) if is64.any() else None,
                      "sweep_kernel_s_total": float(np.sum(ms)) * 1e-3,
                      "trace_unknowns": rec.diagnostics.get("global_matrix_n"),
                      "trace_solve": rec.diagnostics.get("trace_solve"),
