@@ -112,10 +112,11 @@ _SIGS = {
                                  C.c_int64]),
     "pbte_gsis_setup": (C.c_int, [C.c_void_p, C.POINTER(PbteGsisData)]),
     "pbte_gsis_moments": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
-    "pbte_gsis_source": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
-    "pbte_gsis_rhs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
-    "pbte_gsis_local": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
-    "pbte_gsis_precond_setup": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pbte_gsis_source": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_rhs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_local": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_uq": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_gsis_precond_setup": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbte_gsis_precond_src": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "pbte_gsis_correct": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -139,7 +140,7 @@ _SIGS = {
     "pbte_trace_factor": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pbte_trace_options": (C.c_int, [C.c_void_p, C.c_int32]),
     "pbte_trace_quality": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _f64p]),
-    "pbte_trace_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_trace_solve": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -5,14 +5,21 @@ After each sweep, on the device:
 
 1. ``pbte_gsis_moments``  one pass over ``vdf``: the CIS moments and the four
    third-order angular moments of the higher-order-term source;
-2. ``pbte_gsis_source``   HoT stress rows and ``AA_SRC = inv(AA_SOL) work``;
-3. ``pbte_gsis_rhs``      trace right-hand side, wall rows from the kinetic
-   traces (adiabatic flux projected on the tangent);
+2. ``pbte_gsis_source``   HoT stress rows, then through one operator per
+   element the T, qx, qy rows of ``AA_SRC = inv(AA_SOL) work`` and the
+   element sides' sums ``BA_SOL AA_SRC`` of the trace right-hand side;
+3. ``pbte_gsis_rhs``      trace right-hand side from those sums, wall rows
+   from the kinetic traces (adiabatic flux projected on the tangent);
 4. trace solve            ``K u = ffa`` with ``K`` factorised once on the
    device (``trace_solve.TraceSolver``: nested-dissection multifrontal, in
    place of the reference's SuperLU, ``global_solve.py:120-124, 190-195``);
-5. ``pbte_gsis_local``    ``UQ = AA_SRC + AA_TRACE U_TRACE``;
+5. ``pbte_gsis_local``    ``UQ = AA_SRC + AA_TRACE U_TRACE``, the T, qx, qy
+   rows (``pbte_gsis_uq`` gives every row, for the field output);
 6. ``pbte_gsis_correct``  damped blend of ``UQ`` into ``vdf`` and the moments.
+
+On a binary32 distribution (the binary32 iterations of an fp32 run) steps
+2, 4 and 5 read binary32 copies of the element operators and of the trace
+factors; sums and every vector stay binary64.
 
 ``scheme.defect_omega`` (the fixed-point repair of the reference) keeps the
 reference's semantics: with ``omega = 0`` the defect is identically zero.
@@ -187,14 +194,16 @@ class Acceleration:
         g.use_hmin = 1 if self.variant == "A" else 0
         ctx = s.ctx
         _native.check(ctx.handle, _native.lib().pbte_gsis_setup(ctx.handle, C.byref(g)), "gsis_setup")
-        torch.cuda.current_stream().synchronize()      # the copies out of the operator buffers
+        torch.cuda.current_stream().synchronize()      # the operators are built from these buffers
+        # BA_SOL also serves the moment preconditioner's operator
+        self.ba_sol = ba_sol if c.scheme.krylov_precond else None
         del aa_trace, ba_sol
 
-        self.mom7 = dev_zeros((7, P, N))
-        self.aa_src_d = dev_zeros((7 * P, N))
-        self.ffa_d = dev_zeros((F, 3 * Q))
-        self.u_d = dev_zeros((F, 3 * Q))
-        self.uq_d = dev_zeros((7 * P, N))
+        # the step's buffers, one set per distribution precision (the
+        # refined run alternates binary32 and binary64 steps; the field
+        # output reads the last binary64 step's)
+        self._sets: dict = {}
+        self.last = self._set(torch.float64)
         self.mkin_d = dev_zeros((3, P, N))
         self.defect_d = dev_zeros((3, P, N))
         # The correction of the distribution is left pending: the next sweep
@@ -223,37 +232,67 @@ class Acceleration:
         self._and_g: list = []
 
     # -- one acceleration cycle -------------------------------------------
+    def _set(self, dtype) -> dict:
+        st = self._sets.get(dtype)
+        if st is None:
+            P, N = self.nd, self.solver.n_tris
+            F, Q = self.solver.mesh.n_faces, self.nf
+            st = self._sets[dtype] = {"mom7": dev_zeros((7, P, N)), "src": dev_zeros((3 * P, N)),
+                                      "ffa": dev_zeros((F, 3 * Q)), "u": dev_zeros((F, 3 * Q)),
+                                      "uq": dev_zeros((3 * P, N))}
+        return st
+
+    def _stage(self, vdf) -> dict:
+        """Steps 1-5 on a swept distribution: its moments, the element pass,
+        the trace right-hand side, the trace solve and the local recovery
+        of T, qx, qy -- with the binary32 operators and trace factors when
+        the distribution is binary32."""
+        h = self.solver.ctx.handle
+        lib = _native.lib()
+        fp32 = vdf.dtype == torch.float32
+        prec = _native.PBTE_FP32 if fp32 else _native.PBTE_FP64
+        st = self.last = self._set(vdf.dtype)
+        _native.check(h, lib.pbte_gsis_moments(h, prec, ptr(vdf), ptr(st["mom7"])), "gsis_moments")
+        _native.check(h, lib.pbte_gsis_source(h, prec, ptr(st["mom7"]), ptr(st["src"])), "gsis_source")
+        _native.check(h, lib.pbte_gsis_rhs(h, prec, ptr(vdf), ptr(st["ffa"])), "gsis_rhs")
+        self.trace.solve(st["ffa"], st["u"], "fp32" if fp32 else "fp64")
+        _native.check(h, lib.pbte_gsis_local(h, prec, ptr(st["src"]), ptr(st["u"]), ptr(st["uq"])),
+                      "gsis_local")
+        return st
+
     def apply(self) -> None:
         s = self.solver
-        ctx = s.ctx
+        h = s.ctx.handle
         lib = _native.lib()
         vdf = s.sweep_vdf
         prec = _native.PBTE_FP64 if vdf.dtype == torch.float64 else _native.PBTE_FP32
-        h = ctx.handle
-        _native.check(h, lib.pbte_gsis_moments(h, prec, ptr(vdf), ptr(self.mom7)), "gsis_moments")
-        _native.check(h, lib.pbte_gsis_source(h, ptr(self.mom7), ptr(self.aa_src_d)), "gsis_source")
-        _native.check(h, lib.pbte_gsis_rhs(h, prec, ptr(vdf), ptr(self.aa_src_d), ptr(self.ffa_d)),
-                      "gsis_rhs")
-        self.trace_solve()
-        _native.check(h, lib.pbte_gsis_local(h, ptr(self.aa_src_d), ptr(self.u_d), ptr(self.uq_d)),
-                      "gsis_local")
+        st = self._stage(vdf)
         m = s.mom
         defect = ptr(self.defect_d) if self.omega else 0
         if self.lazy:
-            _native.check(h, lib.pbte_gsis_correct_moments(h, ptr(self.mom7), ptr(self.uq_d), defect,
+            _native.check(h, lib.pbte_gsis_correct_moments(h, ptr(st["mom7"]), ptr(st["uq"]), defect,
                                                            ptr(m.ts_d), ptr(m.qxs_d), ptr(m.qys_d),
                                                            ptr(m.temp_d), ptr(m.qx_d), ptr(m.qy_d),
                                                            ptr(self.mkin_d), ptr(self.coef_d)),
                           "gsis_correct_moments")
             self.pending = True
         else:
-            _native.check(h, lib.pbte_gsis_correct(h, prec, ptr(vdf), ptr(self.mom7), ptr(self.uq_d),
+            _native.check(h, lib.pbte_gsis_correct(h, prec, ptr(vdf), ptr(st["mom7"]), ptr(st["uq"]),
                                                    defect, ptr(m.ts_d), ptr(m.qxs_d), ptr(m.qys_d),
                                                    ptr(m.temp_d), ptr(m.qx_d), ptr(m.qy_d),
                                                    ptr(self.mkin_d)), "gsis_correct")
         self._ncall += 1
         if self.omega and self.defect_every and self._ncall % self.defect_every == 0:
             self.refresh_defect()
+
+    def full_uq(self):
+        """UQ [7P, N] of the last binary64 step (of the last step when none
+        ran in binary64), every row: the field output's stress rows."""
+        st = self._sets.get(torch.float64) if torch.float64 in self._sets else self.last
+        uq = dev_zeros((7 * self.nd, self.solver.n_tris))
+        h = self.solver.ctx.handle
+        _native.check(h, _native.lib().pbte_gsis_uq(h, ptr(st["mom7"]), ptr(st["u"]), ptr(uq)), "gsis_uq")
+        return uq
 
     def settle(self, boundary_only: bool = False) -> None:
         """Apply the pending correction to the distribution: to every element,
@@ -286,16 +325,12 @@ class Acceleration:
         h = self.solver.ctx.handle
         lib = _native.lib()
         prec = _native.PBTE_FP64 if vdf.dtype == torch.float64 else _native.PBTE_FP32
-        _native.check(h, lib.pbte_gsis_moments(h, prec, ptr(vdf), ptr(self.mom7)), "gsis_moments")
-        _native.check(h, lib.pbte_gsis_source(h, ptr(self.mom7), ptr(self.aa_src_d)), "gsis_source")
-        _native.check(h, lib.pbte_gsis_rhs(h, prec, ptr(vdf), ptr(self.aa_src_d), ptr(self.ffa_d)), "gsis_rhs")
-        self.trace_solve()
-        _native.check(h, lib.pbte_gsis_local(h, ptr(self.aa_src_d), ptr(self.u_d), ptr(self.uq_d)), "gsis_local")
+        st = self._stage(vdf)
         key = vdf.data_ptr()
         coef = self._state_coef.get(key)
         if coef is None:
             coef = self._state_coef[key] = [dev_zeros(self.coef_d.shape), False]
-        _native.check(h, lib.pbte_gsis_correct_moments(h, ptr(self.mom7), ptr(self.uq_d), None,
+        _native.check(h, lib.pbte_gsis_correct_moments(h, ptr(st["mom7"]), ptr(st["uq"]), None,
                                                        *(ptr(t) for t in out), None, ptr(coef[0])),
                       "gsis_correct_moments")
         cells = self.boundary_cells_d
@@ -327,12 +362,9 @@ class Acceleration:
         """The sweep has rewritten the distribution: nothing is pending."""
         self.pending = False
 
-    def trace_solve(self, ffa=None, out=None) -> None:
-        """``K u = ffa`` on the device (``global_solve.py:190-195``); by
-        default from ``ffa_d`` into ``u_d``."""
-        ffa = self.ffa_d if ffa is None else ffa
-        out = self.u_d if out is None else out
-        self.trace.solve(ffa, out)
+    def trace_solve(self, ffa, out, precision: str = "fp64") -> None:
+        """``K u = ffa`` on the device (``global_solve.py:190-195``)."""
+        self.trace.solve(ffa, out, precision)
 
     @property
     def solve_count(self) -> int:
@@ -341,7 +373,7 @@ class Acceleration:
     # -- the fixed-point repair (acceleration/__init__.py:118-172) ----------
     def refresh_defect(self) -> float:
         P = self.nd
-        o = self.uq_d[:3 * P].view(3, P, -1) - self.mkin_d
+        o = self.last["uq"].view(3, P, -1) - self.mkin_d
         step = float((o - self.defect_d).abs().max())
         if self.anderson:
             self._anderson_step(o)

@@ -1,9 +1,10 @@
 """Batched solves: several independent problems advanced in lockstep, the
 transport sweeps of all of them in one launch of the pipelined kernel
 (``pbte_batch_sweep``).  No counterpart in the reference, which solves one
-case per process; each problem's iterates are the ones ``Solver.run`` gives
-it alone, bit for bit (the same kernels in the same order, only the sweep
-launches are merged).
+case per process; each problem's iterates are the ones its source
+iteration alone (``Solver._run_source``: for fp32 the binary32 sweeps with
+the switch to binary64) gives it, bit for bit (the same kernels in the same
+order, only the sweep launches are merged).
 
 For many small problems: one small problem's sweep fills a few dozen of
 the GPU's 256 CUs (a workgroup per direction), a batch of them fills all.

@@ -10,9 +10,11 @@
 //                          item = slot * B + direction-in-bundle: the items of
 //                          one level are consecutive, so a level's inverses
 //                          stream as 64-lane coalesced 16-byte loads.
-//   item record per bundle [item] int4: cell | (LDS pool slot + 1) << 21 and
-//                          three face words saying where each upwind trace
-//                          lives (LDS slot, global value, wall, nothing).
+//   slot record per schedule class [slot] int4: cell | (LDS pool slot + 1)
+//                          << 21 and three face words saying where each
+//                          upwind trace lives (LDS slot, global value, wall,
+//                          nothing); read by every direction of every bundle
+//                          of the class.
 //   cell record [N][rs]    U, VX, VY (the source, rewritten every sweep), the
 //                          face normals times the face length, info words;
 //                          one 128-byte line per cell in binary32 at P = 6.
@@ -23,7 +25,9 @@
 // interior and periodic face; such directions have the same reference
 // order and the same level schedule, so one workgroup sweeps all of them
 // together and loads each cell's direction-independent data once (one
-// direction per bundle by default).
+// direction per bundle by default).  The bundles of one direction group
+// (the same order and inflow signs) and one size form a schedule class:
+// their levels, slot records and wall lists are built and stored once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -175,11 +179,12 @@ struct pbte_ctx {
     // schedule
     int nb = 0;
     int64_t total_levels = 0, max_width = 0;
+    int n_classes = 0;          // schedule classes (bundles sharing levels and records)
     int32_t *block_bundle = nullptr, *bundle_ndir = nullptr, *bundle_dir_off = nullptr,
             *bundle_dirs = nullptr, *slot_cell = nullptr, *lev_off = nullptr, *lev_ptr = nullptr;
     int64_t *lu_off = nullptr, *item_off = nullptr;
-    int4* irec = nullptr;       // per item (bundle, slot, direction): {cell | pool slot, face words}
-    int4* wcell = nullptr;      // per bundle: inflow wall elements {cell, wall row of faces 0-2}
+    int4* irec = nullptr;       // per (class, slot): {cell | pool slot, face words}
+    int4* wcell = nullptr;      // per class: inflow wall elements {cell, wall row of faces 0-2}
     int32_t* wc_off = nullptr;
     int max_wall_in = 0;        // most inflow wall elements of one bundle
     double* crec64 = nullptr;   // cell records of the sweep
@@ -226,10 +231,16 @@ struct pbte_ctx {
     bool gsis = false;
     int gs_nw = 0, gs_nth = 0, gs_variant_b = 0, gs_use_hmin = 1;
     double gs_tc5 = 0, gs_tau_thr = 1, gs_drift = 0;
-    double* gs_inv3 = nullptr;  // T, qx, qy columns of inv(AA_SOL), moment preconditioner
-    double *gs_hw = nullptr, *gs_inv4 = nullptr, *gs_aa_trace = nullptr, *gs_ba_sol = nullptr,
-           *gs_wtfc = nullptr, *gs_wnx = nullptr, *gs_wny = nullptr, *gs_hmin = nullptr,
-           *gs_work = nullptr, *gs_wmom = nullptr, *gs_side = nullptr;
+    // element operators, K = 3P + 9Q outputs per element: gs_op [N][4P][K]
+    // (stress work -> AA_SRC rows of T, qx, qy and the per-side trace sums),
+    // gs_aat [N][3][3Q][3P] (AA_TRACE rows of T, qx, qy), their binary32
+    // copies (made by the first binary32 step), the rows 3P .. 7P for the
+    // full UQ (gs_op_hi [N][4P][4P], gs_aat_hi [N][3][3Q][4P]) and the
+    // moment preconditioner's gs_op_pc [N][3P][K]
+    double *gs_op = nullptr, *gs_aat = nullptr, *gs_op_hi = nullptr, *gs_aat_hi = nullptr, *gs_op_pc = nullptr;
+    float *gs_op32 = nullptr, *gs_aat32 = nullptr;
+    double *gs_hw = nullptr, *gs_wtfc = nullptr, *gs_wnx = nullptr, *gs_wny = nullptr, *gs_hmin = nullptr,
+           *gs_wmom = nullptr, *gs_side = nullptr;
     int32_t *gs_t1 = nullptr, *gs_l1 = nullptr, *gs_t2 = nullptr, *gs_l2 = nullptr, *gs_wface = nullptr,
             *gs_wtri = nullptr, *gs_tri_faces = nullptr;
     uint8_t* gs_flip = nullptr;
@@ -384,7 +395,7 @@ __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.N) return;
     const int B = a.bundle_ndir[bundle];
-    const int i = a.slot_cell[(size_t)bundle * a.N + s];
+    const int i = a.slot_cell[a.item_off[bundle] + s];
     const int N = a.N;
     const CellGeo<double> g = a.geo[i];
     for (int b = 0; b < B; ++b) {
@@ -475,11 +486,11 @@ template <typename T> struct SweepArgs {
     const T* crec;              // cell records, row stride rs
     const int32_t* other;       // [N][3] element across each face (plain kernel)
     const T* wtr;               // wall vectors over L, [wall row][PV]
-    const int4* wcell;          // per bundle: its inflow wall elements {cell, row of faces 0-2}
+    const int4* wcell;          // per class: its inflow wall elements {cell, row of faces 0-2}
     const int32_t* wc_off;
     const T* pbuf;
     T* vdf;
-    const int4* irec;           // per item (bundle, slot, direction): {cell | pool slot, face words}
+    const int4* irec;           // per (class, slot): {cell | pool slot, face words}
     T* spill_row;               // write target of idle threads (P values)
     const uint8_t* bundle_global;
     int N, F, rs, use_pbuf, homogeneous;
@@ -644,7 +655,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
     const T cx = a.cxv[d], cy = a.cyv[d];
     const int N = a.N;
     const int4* irec = a.irec + a.item_off[bundle];
-    const int4* wc = a.wcell + a.wc_off[bundle];
+    const int4* wc = a.wcell + a.wc_off[2 * bundle];
     const T* lu = a.lu + a.lu_off[bundle];
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const T thermal_on = a.homogeneous ? T(0) : T(1);
@@ -653,7 +664,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
     for (int lv = l0; lv < l1; ++lv) {
         const int s = a.lev_ptr[lv] + c;
         if (s < a.lev_ptr[lv + 1]) {
-            const int4 r = irec[(size_t)s * B + b];
+            const int4 r = irec[s];
             const int i = rec_cell(r.x);
             const int fw[3] = {r.y, r.z, r.w};
             const T* cr = a.crec + (size_t)i * a.rs;
@@ -711,7 +722,7 @@ __device__ __forceinline__ int otf_rhs(const SweepArgs<T>& a, const int4* irec, 
     constexpr int DEG = Deg<P>::v;
     constexpr int Q = DEG + 1;
     constexpr int TQ = Tr<P>::TQ;
-    const int4 r = irec[(size_t)s * B + b];
+    const int4 r = irec[s];
     const int i = rec_cell(r.x);
     const int fw[3] = {r.y, r.z, r.w};
     const T* cr = a.crec + (size_t)i * a.rs;
@@ -772,7 +783,7 @@ __global__ void __launch_bounds__(256) sweep_refine_kernel(SweepArgs<double> a)
     const T cx = a.cxv[d], cy = a.cyv[d];
     const int N = a.N;
     const int4* irec = a.irec + a.item_off[bundle];
-    const int4* wc = a.wcell + a.wc_off[bundle];
+    const int4* wc = a.wcell + a.wc_off[2 * bundle];
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const T thermal_on = a.homogeneous ? T(0) : T(1);
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
@@ -859,7 +870,7 @@ __global__ void __launch_bounds__(256) sweep_otf_kernel(SweepArgs<T> a)
     const double cx64 = a.cxv64[d], cy64 = a.cyv64[d];
     const int N = a.N;
     const int4* irec = a.irec + a.item_off[bundle];
-    const int4* wc = a.wcell + a.wc_off[bundle];
+    const int4* wc = a.wcell + a.wc_off[2 * bundle];
     T* vdf_d = a.vdf + (size_t)d * N * P;
     const T thermal_on = a.homogeneous ? T(0) : T(1);
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
@@ -1015,7 +1026,7 @@ __device__ __forceinline__ void pipe_record(__amdgpu_buffer_rsrc_t irec, int2 sp
     const int sl = span.x + c;
     o.valid = sl < span.y;
     o.slot = o.valid ? sl : span.x;
-    o.r = bits16<int4>(__builtin_amdgcn_raw_buffer_load_b128(irec, (uint32_t)(o.slot * B + b) * 16u, 0, kStream));
+    o.r = bits16<int4>(__builtin_amdgcn_raw_buffer_load_b128(irec, (uint32_t)o.slot * 16u, 0, kStream));
 }
 
 // record-code classes as bit masks over the code: one shift and one test,
@@ -1205,7 +1216,7 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
     const T cx = a.cxv[d], cy = a.cyv[d];
     const int N = a.N;
     const int64_t it0 = a.item_off[bundle];
-    const __amdgpu_buffer_rsrc_t irec = bundle_rsrc(a.irec + it0, (uint32_t)(a.item_off[bundle + 1] - it0) * 16u);
+    const __amdgpu_buffer_rsrc_t irec = bundle_rsrc(a.irec + it0, (uint32_t)N * 16u);
     const int64_t lu0 = a.lu_off[bundle];
     const __amdgpu_buffer_rsrc_t lu = bundle_rsrc(a.lu + lu0, (uint32_t)((a.lu_off[bundle + 1] - lu0) * sizeof(T)));
     T* vdf_d = a.vdf + (size_t)d * N * P;
@@ -1223,7 +1234,7 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         // the summed wall inflow of every inflow wall element of the bundle,
         // per direction (a thermal wall emits nothing in a homogeneous sweep)
         constexpr int PV = pad4(P);
-        const int w0 = a.wc_off[bundle], nw = a.wc_off[bundle + 1] - w0;
+        const int w0 = a.wc_off[2 * bundle], nw = a.wc_off[2 * bundle + 1];
         const T thermal_on = a.homogeneous ? T(0) : T(1);
         for (int k = t; k < nw * B * PV; k += nthr) {
             const int e = k / (B * PV), bb = (k / PV) % B, m = k % PV;
@@ -1916,63 +1927,160 @@ __global__ void __launch_bounds__(256) moments7_kernel(const T* __restrict__ vdf
     }
 }
 
-// HoT source (hot_source.py:29-75), one wave per element: the stress rows
-// of the source per dof m (lanes m < P, l summed in order), then
-// AA_SRC[p][i] = sum_q inv(AA_SOL)[i][p][3P + q] work[q] (lanes over p, q in
-// order) from the transposed stress columns invT [N][4P][7P], so that the
-// lanes read consecutive words
-template <int P>
-__global__ void __launch_bounds__(256) hot_source_kernel(const double* __restrict__ m7, const double* __restrict__ gx,
-                                                         const double* __restrict__ gy,
-                                                         const double* __restrict__ invT, int N, double tc5,
-                                                         double* __restrict__ aa_src)
+// GSIS element operators.  With w the stress work of an element (4P
+// values, the HoT source's stress rows) the accelerated step needs, per
+// element, the rows T, qx, qy of AA_SRC = inv(AA_SOL)[:, 3P:7P] w and the
+// three per-side trace sums BA_SOL[l] AA_SRC (hot_source.py:61-75,
+// global_solve.py:128-150); both are linear in w, so one operator
+// op[i][q][k] (k < 3P: inverse rows, k >= 3P: (l, r) = divmod(k - 3P, 3Q),
+// sum_p BA_SOL[i][l][r][p] inv[i][p][c0 + q], p in order) maps w to all
+// K = 3P + 9Q outputs, and the local recovery needs only AA_TRACE's rows
+// of T, qx, qy (local_solve.py:16-26; correction.py reads those three).
+// Built from the columns c0 .. c0 + W of the inverse: the stress columns
+// (c0 = 3P, W = 4P) for the step, the T, qx, qy columns (c0 = 0, W = 3P)
+// for the moment preconditioner (krylov.py:388-402).
+__global__ void gsis_op_kernel(const double* __restrict__ inv, const double* __restrict__ ba, int64_t N, int P,
+                               int Q, int c0, int W, double* __restrict__ out)
 {
-    __shared__ double s_work[4][4 * P];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + wv;
-    const bool live = i < N;
-    const size_t PN = (size_t)P * N;
-    if (live && lane < P) {
-        const int m = lane;
-        double pixx = 0, pixy = 0, piyy = 0;
-        for (int l = 0; l < P; ++l) {
-            const double ax = gx[((size_t)i * P + l) * P + m], ay = gy[((size_t)i * P + l) * P + m];
-            const double w1 = m7[3 * PN + (size_t)l * N + i], w2 = m7[4 * PN + (size_t)l * N + i];
-            const double w3 = m7[5 * PN + (size_t)l * N + i], w4 = m7[6 * PN + (size_t)l * N + i];
-            pixx += ax * w1 + ay * w2;
-            pixy += ax * w2 + ay * w3;
-            piyy += ax * w3 + ay * w4;
-        }
-        double* o = s_work[wv];
-        o[m] = (pixx + 0.5 * piyy) * tc5;
-        o[P + m] = 0.5 * pixy * tc5;
-        o[2 * P + m] = 0.5 * pixy * tc5;
-        o[3 * P + m] = (0.5 * pixx + piyy) * tc5;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int R = 7 * P, K = 3 * P + 9 * Q;
+    const int64_t per = (int64_t)W * K;
+    if (g >= N * per) return;
+    const int64_t i = g / per;
+    const int e = (int)(g - i * per);
+    const int q = e / K, k = e - (e / K) * K;
+    const double* iv = inv + (size_t)i * R * R;
+    if (k < 3 * P) {
+        out[g] = iv[(size_t)k * R + c0 + q];
+        return;
     }
-    __syncthreads();
-    if (!live) return;
-    constexpr int R = 7 * P;
-    const double* a = invT + (size_t)i * 4 * P * R;
-    for (int p = lane; p < R; p += 64) {
+    const int kk = k - 3 * P, l = kk / (3 * Q), r = kk - (kk / (3 * Q)) * (3 * Q);
+    const double* b = ba + (((size_t)i * 3 + l) * 3 * Q + r) * R;
+    double s = 0.0;
+    for (int pp = 0; pp < R; ++pp) s += b[pp] * iv[(size_t)pp * R + c0 + q];
+    out[g] = s;
+}
+
+// rows r0 .. r0 + nr of the stress columns of the inverse, out[i][q][k] =
+// inv[i][r0 + k][3P + q] (the AA_SRC rows 3P .. 7P of the full recovery)
+__global__ void inv_rows_kernel(const double* __restrict__ inv, int64_t N, int P, int r0, int nr,
+                                double* __restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int R = 7 * P, W = 4 * P;
+    const int64_t per = (int64_t)W * nr;
+    if (g >= N * per) return;
+    const int64_t i = g / per;
+    const int e = (int)(g - i * per);
+    const int q = e / nr, k = e - (e / nr) * nr;
+    out[g] = inv[((size_t)i * R + r0 + k) * R + 3 * P + q];
+}
+
+// rows p0 .. p0 + np of AA_TRACE [N][3][7P][3Q] as out[i][l][r][p - p0]
+__global__ void aat_rows_kernel(const double* __restrict__ aat, int64_t N, int P, int Q, int p0, int np,
+                                double* __restrict__ out)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)9 * Q * np;
+    if (g >= N * per) return;
+    const int64_t i = g / per;
+    const int e = (int)(g - i * per);
+    const int lr = e / np, p = e - (e / np) * np;
+    const int l = lr / (3 * Q), r = lr - (lr / (3 * Q)) * (3 * Q);
+    out[g] = aat[(((size_t)i * 3 + l) * 7 * P + p0 + p) * 3 * Q + r];
+}
+
+template <typename T>
+__global__ void narrow_kernel(const double* __restrict__ in, T* __restrict__ out, int64_t n)
+{
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x)
+        out[g] = (T)in[g];
+}
+
+// the stress work of element i (hot_source.py:29-58): w[q], q < 4P, for
+// dof m, the l sums in order
+template <int P>
+__device__ inline void stress_work(const double* __restrict__ m7, const double* __restrict__ gx,
+                                   const double* __restrict__ gy, int64_t N, int64_t i, int m, double tc5,
+                                   double* w)
+{
+    const size_t PN = (size_t)P * N;
+    double pixx = 0, pixy = 0, piyy = 0;
+    for (int l = 0; l < P; ++l) {
+        const double ax = gx[((size_t)i * P + l) * P + m], ay = gy[((size_t)i * P + l) * P + m];
+        const double w1 = m7[3 * PN + (size_t)l * N + i], w2 = m7[4 * PN + (size_t)l * N + i];
+        const double w3 = m7[5 * PN + (size_t)l * N + i], w4 = m7[6 * PN + (size_t)l * N + i];
+        pixx += ax * w1 + ay * w2;
+        pixy += ax * w2 + ay * w3;
+        piyy += ax * w3 + ay * w4;
+    }
+    w[m] = (pixx + 0.5 * piyy) * tc5;
+    w[P + m] = 0.5 * pixy * tc5;
+    w[2 * P + m] = 0.5 * pixy * tc5;
+    w[3 * P + m] = (0.5 * pixx + piyy) * tc5;
+}
+
+// the element outputs of one workgroup's EB elements from their inputs in
+// LDS (xin[e][q], q < W): src[k][i] (k < 3P) and side[i][k - 3P] (k >= 3P),
+// s = sum_q op[i][q][k] xin[q], q in order; threads over (element, k), so
+// that consecutive threads read consecutive operator words
+template <int EB, typename TO>
+__device__ inline void element_apply(const TO* __restrict__ op, const double (*xin)[64], int64_t N, int64_t i0,
+                                     int P, int Q, int W, double* __restrict__ src, double* __restrict__ side)
+{
+    const int K = 3 * P + 9 * Q;
+    for (int t = threadIdx.x; t < EB * K; t += blockDim.x) {
+        const int e = t / K, k = t - (t / K) * K;
+        const int64_t i = i0 + e;
+        if (i >= N) continue;
+        const TO* a = op + (size_t)i * W * K + k;
         double s = 0.0;
-        for (int q = 0; q < 4 * P; ++q) s += a[(size_t)q * R + p] * s_work[wv][q];
-        aa_src[(size_t)p * N + i] = s;
+        for (int q = 0; q < W; ++q) s += (double)a[(size_t)q * K] * xin[e][q];
+        if (k < 3 * P)
+            src[(size_t)k * N + i] = s;
+        else
+            side[(size_t)i * 9 * Q + k - 3 * P] = s;
     }
 }
 
-// out[b][c][r] = in[b][r][c0 + c] for c < w: blocks of `rows` rows of
-// length ld, column slices transposed
-__global__ void transpose_slice_kernel(const double* __restrict__ in, int64_t nblk, int rows, int ld, int c0, int w,
-                                       double* __restrict__ out)
+// the accelerated step's element pass: the stress work from the third-order
+// moments, then AA_SRC's T, qx, qy rows and the side sums through op
+template <int P, int Q, typename TO>
+__global__ void __launch_bounds__(256) hot_side_kernel(const double* __restrict__ m7, const double* __restrict__ gx,
+                                                       const double* __restrict__ gy, const TO* __restrict__ op,
+                                                       int64_t N, double tc5, double* __restrict__ src,
+                                                       double* __restrict__ side)
 {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t per = (int64_t)rows * w;
-    if (g >= nblk * per) return;
-    const int64_t b = g / per;
-    const int e = (int)(g - b * per);
-    const int c = e / rows, r = e - (e / rows) * rows;
-    out[g] = in[(b * rows + r) * ld + c0 + c];
+    constexpr int EB = 8;
+    static_assert(4 * P <= 64, "stress work exceeds the LDS row");
+    __shared__ double w[EB][64];
+    const int64_t i0 = (int64_t)blockIdx.x * EB;
+    for (int t = threadIdx.x; t < EB * P; t += blockDim.x) {
+        const int e = t / P, m = t - (t / P) * P;
+        if (i0 + e < N) stress_work<P>(m7, gx, gy, N, i0 + e, m, tc5, w[e]);
+    }
+    __syncthreads();
+    element_apply<EB, TO>(op, w, N, i0, P, Q, 4 * P, src, side);
 }
+
+// the moment preconditioner's element pass: sign times the T, qx, qy
+// blocks of r (three [P][N] blocks) through the preconditioner's operator
+template <typename TO>
+__global__ void __launch_bounds__(256) precond_side_kernel(const double* __restrict__ r, double sign,
+                                                           const TO* __restrict__ op, int64_t N, int P, int Q,
+                                                           double* __restrict__ src, double* __restrict__ side)
+{
+    constexpr int EB = 8;
+    __shared__ double w[EB][64];
+    const int64_t i0 = (int64_t)blockIdx.x * EB;
+    for (int t = threadIdx.x; t < EB * 3 * P; t += blockDim.x) {
+        const int e = t / (3 * P), q = t - (t / (3 * P)) * (3 * P);
+        if (i0 + e < N) w[e][q] = sign * r[(size_t)q * N + i0 + e];
+    }
+    __syncthreads();
+    element_apply<EB, TO>(op, w, N, i0, P, Q, 3 * P, src, side);
+}
+
 
 // wall trace moments (global_solve.py:31-53): one wave per wall face,
 // lanes over directions (d = lane + 64 j in order), the Q face nodes' three
@@ -2014,26 +2122,8 @@ __global__ void __launch_bounds__(64) wall_trace_kernel(const T* __restrict__ vd
     }
 }
 
-// trace right-hand side (global_solve.py:128-186) in two passes.  First
-// per (element, local face) side: side[t][l][r] = sum_q BA_SOL[t][l][r][q]
-// AA_SRC[q][t] (q in order), one wave per element, lanes over (l, r), from
-// the transposed baT [N][3][7P][3Q] so that the lanes read consecutive words
-template <int P, int Q>
-__global__ void __launch_bounds__(256) trace_side_kernel(const double* __restrict__ baT,
-                                                         const double* __restrict__ aa_src, int N,
-                                                         double* __restrict__ side)
-{
-    constexpr int R = 3 * Q, C = 7 * P;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int t = blockIdx.x * 4 + wv;
-    if (t >= N || lane >= 3 * R) return;
-    const int l = lane / R, r = lane - (lane / R) * R;
-    const double* b = baT + (((size_t)t * 3 + l) * C) * R + r;
-    double s = 0.0;
-    for (int q = 0; q < C; ++q) s += b[(size_t)q * R] * aa_src[(size_t)q * N + t];
-    side[((size_t)t * 3 + l) * R + r] = s;
-}
-
+// trace right-hand side (global_solve.py:128-186) from the per-side sums
+// side[t][l][r] of the element pass (hot_side_kernel, precond_side_kernel):
 // then per (face, row): ffa[f][r] = side(t1, l1)[r] + side(t2, l2)[r'], r'
 // mirrored within each field block across a reversed periodic pair
 __global__ void __launch_bounds__(256) trace_rhs_kernel(const double* __restrict__ side, const int32_t* t1,
@@ -2087,45 +2177,63 @@ __global__ void __launch_bounds__(256) fill_kernel(double* y, int64_t n, double 
     if (g < n) y[g] = v;
 }
 
-// the moment preconditioner's element source (krylov.py:388-402): the T,
-// qx, qy blocks of r (each [P][N]) times sign through the matching columns
-// of inv(AA_SOL): aa_src[p][i] = sum_q inv(AA_SOL)[i][p][q] sign r[q][i]
-__global__ void __launch_bounds__(256) precond_src_kernel(const double* __restrict__ inv3,
-                                                          const double* __restrict__ r, double sign, int N, int P,
-                                                          double* __restrict__ aa_src)
+// UQ's T, qx, qy rows: uq[p][i] = src[p][i] + sum_il AA_TRACE[i][il][p]
+// U_TRACE[face(i, il)] (local_solve.py:16-26), il then r in order, one
+// thread per (element, p) from aat [N][3][3Q][3P]
+template <typename TO>
+__global__ void __launch_bounds__(256) local3_kernel(const TO* __restrict__ aat, const double* __restrict__ src,
+                                                     const double* __restrict__ u, const int32_t* __restrict__ tri_faces,
+                                                     int64_t N, int P, int Q, double* __restrict__ uq)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int R = 7 * P;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int R = 3 * P, C = 3 * Q;
     if (g >= N * R) return;
-    const int i = g / R, p = g - (g / R) * R;
-    const double* a = inv3 + ((size_t)i * R + p) * 3 * P;
+    const int64_t i = g / R;
+    const int p = (int)(g - i * R);
     double s = 0.0;
-    for (int q = 0; q < 3 * P; ++q) s += a[q] * (sign * r[(size_t)q * N + i]);
-    aa_src[(size_t)p * N + i] = s;
+    for (int il = 0; il < 3; ++il) {
+        const TO* a = aat + ((size_t)i * 3 + il) * C * R + p;
+        const double* x = u + (size_t)tri_faces[i * 3 + il] * C;
+        for (int r = 0; r < C; ++r) s += (double)a[(size_t)r * R] * x[r];
+    }
+    uq[(size_t)p * N + i] = src[(size_t)p * N + i] + s;
 }
 
-// UQ = AA_SRC + sum_il AA_TRACE[i][il] U_TRACE[face(i, il)]  (local_solve.py:16-26):
-// one wave per element, lanes over the 7P rows, the transposed operator
-// aatT [N][3][3Q][7P] read in consecutive words, il then r in order
-__global__ void __launch_bounds__(256) local_solve_kernel(const double* __restrict__ aatT, const double* __restrict__ aa_src,
-                                                          const double* __restrict__ u, const int32_t* tri_faces,
-                                                          int N, int P, int Q, double* uq)
+// the full UQ [7P][N] of a step (its stress work recomputed from m7): rows
+// below 3P as local3_kernel, rows from 3P through op_hi [N][4P][4P] and
+// aat_hi [N][3][3Q][4P]; the field output reads the stress rows
+template <int P, int Q>
+__global__ void __launch_bounds__(256) uq_full_kernel(const double* __restrict__ m7, const double* __restrict__ gx,
+                                                      const double* __restrict__ gy, const double* __restrict__ op,
+                                                      const double* __restrict__ op_hi,
+                                                      const double* __restrict__ aat, const double* __restrict__ aat_hi,
+                                                      const double* __restrict__ u,
+                                                      const int32_t* __restrict__ tri_faces, int64_t N, double tc5,
+                                                      double* __restrict__ uq)
 {
+    constexpr int K = 3 * P + 9 * Q, W = 4 * P, C = 3 * Q;
+    __shared__ double w[4][64];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + wv;
+    const int64_t i = (int64_t)blockIdx.x * 4 + wv;
+    if (i < N && lane < P) stress_work<P>(m7, gx, gy, N, i, lane, tc5, w[wv]);
+    __syncthreads();
     if (i >= N) return;
-    const int R = 7 * P, C = 3 * Q;
-    const int f0 = tri_faces[i * 3], f1 = tri_faces[i * 3 + 1], f2 = tri_faces[i * 3 + 2];
-    for (int p = lane; p < R; p += 64) {
+    for (int k = lane; k < 7 * P; k += 64) {
+        const bool lo = k < 3 * P;
+        const int kk = lo ? k : k - 3 * P, stride = lo ? K : W, np = lo ? 3 * P : W;
+        const double* a = (lo ? op + (size_t)i * W * K : op_hi + (size_t)i * W * W) + kk;
         double s = 0.0;
+        for (int q = 0; q < W; ++q) s += a[(size_t)q * stride] * w[wv][q];
+        double t = 0.0;
         for (int il = 0; il < 3; ++il) {
-            const double* a = aatT + ((size_t)i * 3 + il) * C * R + p;
-            const double* x = u + (size_t)(il == 0 ? f0 : il == 1 ? f1 : f2) * C;
-            for (int r = 0; r < C; ++r) s += a[(size_t)r * R] * x[r];
+            const double* b = (lo ? aat : aat_hi) + ((size_t)i * 3 + il) * C * np + kk;
+            const double* x = u + (size_t)tri_faces[i * 3 + il] * C;
+            for (int r = 0; r < C; ++r) t += b[(size_t)r * np] * x[r];
         }
-        uq[(size_t)p * N + i] = aa_src[(size_t)p * N + i] + s;
+        uq[(size_t)k * N + i] = s + t;
     }
 }
+
 
 // damped kinetic correction (correction.py:29-75) per (cell, dof)
 template <typename T>
@@ -2243,6 +2351,27 @@ __global__ void __launch_bounds__(256) apply_correction_kernel(T* vdf, const dou
     }
 }
 
+// The same update on the listed cells, one thread per (direction, cell, dof):
+// the lists are short (the elements behind adiabatic walls and periodic
+// faces), so the directions are spread over threads rather than looped
+template <typename T>
+__global__ void __launch_bounds__(256) apply_correction_cells_kernel(T* vdf, const double* __restrict__ coef,
+                                                                     const double* cxv, const double* cyv,
+                                                                     const int32_t* __restrict__ cells, int ncells,
+                                                                     int D, int N, int P, double drift)
+{
+    const int64_t per = (int64_t)ncells * P;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * D) return;
+    const int d = (int)(t / per);
+    const int64_t km = t - (int64_t)d * per;
+    const int k = (int)(km / P), m = (int)(km - (int64_t)k * P);
+    const int i = cells[k];
+    const size_t PN = (size_t)P * N, o = (size_t)m * N + i;
+    T* f = vdf + (size_t)d * PN + (size_t)i * P + m;
+    *f = (T)((double)*f + coef[o] + (cxv[d] * coef[PN + o] + cyv[d] * coef[2 * PN + o]) * drift);
+}
+
 // element integrals of corrected dofs: temp[i] = sum_m ts[m][i] int_tri[m][i]
 __global__ void __launch_bounds__(256) dof_integrals_kernel(const double* ts, const double* qxs, const double* qys,
                                                             const double* int_tri, int N, int P, double* temp,
@@ -2274,16 +2403,6 @@ __global__ void __launch_bounds__(256) pad_rows_kernel(const double* in, T* out,
 }
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
-
-// out[r][k] = in[r][c0 + k], k < w, for rows of length ld
-__global__ void column_slice_kernel(const double* __restrict__ in, int64_t rows, int ld, int c0, int w,
-                                    double* __restrict__ out)
-{
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= rows * w) return;
-    const int64_t r = g / w;
-    out[g] = in[r * ld + c0 + (g - r * w)];
-}
 
 template <typename F>
 int guarded(pbte_ctx* ctx, F&& f)
@@ -2703,12 +2822,15 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             }
         }
         std::vector<std::vector<int>> bundles;
-        for (auto& grp : groups) {
+        std::vector<int> bundle_group;
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            const auto& grp = groups[gi];
             size_t k = 0;
             while (k < grp.size()) {
                 int take = bmax;
                 while (take > 1 && (size_t)take > grp.size() - k) take >>= 1;
                 bundles.emplace_back(grp.begin() + k, grp.begin() + k + take);
+                bundle_group.push_back((int)gi);
                 k += take;
             }
         }
@@ -2780,19 +2902,40 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             }
         }
         phase("wall lists, LDS budget");
-        // levels per bundle (all directions of a bundle share them)
-        std::vector<int32_t> h_slot((size_t)nb * N), h_lev_ptr;
+        // Schedule classes: the bundles of one direction group (the same
+        // order and the same inflow sign on every face) and one size have
+        // the same levels, slots, face records, wall lists and pool; each
+        // class is built once and stored once, and its bundles point at it
+        // (the records of a class are then read by all of its bundles, which
+        // the block map below places on one XCD, out of that XCD's L2).
+        std::vector<int> cls_of(nb), rep;
+        {
+            std::map<std::pair<int, int>, int> id;
+            for (int k = 0; k < nb; ++k) {
+                const auto key = std::make_pair(bundle_group[k], (int)bundles[k].size());
+                auto it = id.find(key);
+                if (it == id.end()) {
+                    it = id.emplace(key, (int)rep.size()).first;
+                    rep.push_back(k);
+                }
+                cls_of[k] = it->second;
+            }
+        }
+        const int nc = (int)rep.size();
+        if (say) std::fprintf(stderr, "pbte_ctx_create: %d direction groups, %d bundles, %d schedule classes\n",
+                              (int)groups.size(), nb, nc);
+        std::vector<int32_t> h_slot((size_t)nc * N), h_lev_ptr;
         std::vector<int32_t> h_ndir(nb), h_dir_off(nb + 1, 0), h_dirs;
         std::vector<int64_t> h_lu_off(nb + 1, 0), h_item_off(nb + 1, 0);
-        std::vector<std::vector<int32_t>> slots(nb);
-        std::vector<std::vector<int32_t>> lptr(nb);
-        std::vector<std::vector<int4>> wlists(nb);
-        std::vector<int4> h_rec((size_t)nb * N);
-        std::vector<uint8_t> global_flag(nb, 0);
-        std::vector<int64_t> peak(nb, 0), old_reads(nb, 0), per_reads(nb, 0), slots_used(nb, 0);
-        // levels of every bundle: slots in level order (the reference order
+        std::vector<std::vector<int32_t>> slots(nc);
+        std::vector<std::vector<int32_t>> lptr(nc);
+        std::vector<std::vector<int4>> wlists(nc);
+        std::vector<int4> h_rec((size_t)nc * N);
+        std::vector<uint8_t> global_flag(nc, 0);
+        std::vector<int64_t> peak(nc, 0), old_reads(nc, 0), per_reads(nc, 0), slots_used(nc, 0);
+        // levels of every class: slots in level order (the reference order
         // within a level) and the level boundaries
-        std::vector<std::vector<int32_t>> lcnt(nb);
+        std::vector<std::vector<int32_t>> lcnt(nc);
         auto run_threads = [&](auto&& fn) {
             unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
             std::vector<std::thread> th;
@@ -2803,8 +2946,8 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             std::atomic_int next{0};
             run_threads([&]() {
                 std::vector<int32_t> level_of;
-                for (int k = next++; k < nb; k = next++) {
-                    const int d = bundles[k][0];
+                for (int k = next++; k < nc; k = next++) {
+                    const int d = bundles[rep[k]][0];
                     const int32_t* order = su->order + (size_t)d * N;
                     int nlev = pbte_detail_levels(N, order, su->neighbour, su->tri_faces, su->face_bc,
                                                   su->bc_type, su->periodic_tri, su->nx, su->ny, su->cxv[d],
@@ -2827,9 +2970,10 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             // the pipelined kernel runs one wave per 64 items of the widest
             // level (every wave issues the full load stream of a level)
             int64_t widest = 1;
-            for (int k = 0; k < nb; ++k)
+            for (int k = 0; k < nc; ++k)
                 for (size_t l = 0; l + 1 < lcnt[k].size(); ++l)
-                    widest = std::max<int64_t>(widest, (int64_t)(lcnt[k][l + 1] - lcnt[k][l]) * bundles[k].size());
+                    widest = std::max<int64_t>(widest,
+                                               (int64_t)(lcnt[k][l + 1] - lcnt[k][l]) * bundles[rep[k]].size());
             ctx->block = ctx->items = (int)std::min<int64_t>(256, (widest + 63) / 64 * 64);
             for (auto& bb : bundles)
                 if (ctx->items % (int)bb.size()) ctx->block = ctx->items = 256;
@@ -2840,14 +2984,14 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         // one-direction schedule instead
         std::atomic<bool> pair_overflow{false};
         auto work = [&]() {
-            for (int k = next++; k < nb; k = next++) {
+            for (int k = next++; k < nc; k = next++) {
                 if (pair_overflow.load(std::memory_order_relaxed)) break;
-                const int d = bundles[k][0];
+                const int d = bundles[rep[k]][0];
                 const int32_t* order = su->order + (size_t)d * N;
                 const auto& cnt = lcnt[k];
                 const int nlev = (int)cnt.size() - 1;
                 auto& sl = slots[k];
-                const int B = (int)bundles[k].size();
+                const int B = (int)bundles[rep[k]].size();
                 const int width = ctx->items / B;
                 auto& lp = lptr[k];
                 lp.clear();
@@ -2932,7 +3076,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                     slots_used[k] = overflow ? 0 : used;
                 }
                 global_flag[k] = overflow ? 1 : 0;
-                if (overflow && su->pairs_or_fail && bundles[k].size() > 1) {
+                if (overflow && su->pairs_or_fail && bundles[rep[k]].size() > 1) {
                     pair_overflow = true;
                     break;
                 }
@@ -2961,40 +3105,47 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->pool_slots = (int)std::max<int64_t>(1, *std::max_element(slots_used.begin(), slots_used.end()));
         int64_t maxw = 0;
         std::vector<int4> h_wcell;
-        std::vector<int32_t> h_wc_off(nb + 1, 0);
+        std::vector<int32_t> c_wc_off(nc + 1, 0), c_lev_off(nc, 0);
         for (int k = 0; k < nb; ++k) ctx->max_bundle_dirs = std::max(ctx->max_bundle_dirs, (int)bundles[k].size());
-        std::vector<int32_t> h_lev_off(nb + 1, 0);
-        for (int k = 0; k < nb; ++k) {
-            const int B = (int)bundles[k].size();
-            h_ndir[k] = B;
-            h_dir_off[k + 1] = h_dir_off[k] + B;
-            for (int d : bundles[k]) h_dirs.push_back(d);
+        for (int k = 0; k < nc; ++k) {
+            const int B = (int)bundles[rep[k]].size();
             std::copy(slots[k].begin(), slots[k].end(), h_slot.begin() + (size_t)k * N);
-            // lev_ptr is shared storage: each bundle's boundaries, its last
+            // lev_ptr is shared storage: each class's boundaries, its last
             // entry doubling as the end of its final level
-            h_lev_off[k] = (int32_t)h_lev_ptr.size();
+            c_lev_off[k] = (int32_t)h_lev_ptr.size();
             for (size_t j = 0; j + 1 < lptr[k].size(); ++j) {
                 h_lev_ptr.push_back(lptr[k][j]);
                 maxw = std::max<int64_t>(maxw, (int64_t)(lptr[k][j + 1] - lptr[k][j]) * B);
             }
             h_lev_ptr.push_back(N);
-            h_lu_off[k + 1] = h_lu_off[k] + (int64_t)pad4(P * P) * N * B;
-            h_item_off[k + 1] = h_item_off[k] + (int64_t)N * B;
-            h_wc_off[k + 1] = h_wc_off[k] + (int32_t)wlists[k].size();
+            c_wc_off[k + 1] = c_wc_off[k] + (int32_t)wlists[k].size();
             h_wcell.insert(h_wcell.end(), wlists[k].begin(), wlists[k].end());
         }
-        // lev_off[k]..lev_off[k]+nlev_k-1 index level starts; lev_ptr[j+1] is
-        // the end of level j; the sentinel N closes every bundle's last level.
+        // per bundle: its directions, its factors, and where its class's
+        // levels (lev_off: (start, end) into lev_ptr, lev_ptr[j+1] the end of
+        // level j), records and slots (item_off: N of each per class) and
+        // wall list (wc_off: (start, count)) are
+        std::vector<int32_t> h_lev_se(2 * nb), h_wc_se(2 * nb);
+        std::vector<uint8_t> b_global(nb);
         int64_t tot = 0;
-        for (int k = 0; k < nb; ++k) tot += (int64_t)lptr[k].size() - 1;
+        for (int k = 0; k < nb; ++k) {
+            const int B = (int)bundles[k].size(), c = cls_of[k];
+            h_ndir[k] = B;
+            h_dir_off[k + 1] = h_dir_off[k] + B;
+            for (int d : bundles[k]) h_dirs.push_back(d);
+            h_lu_off[k + 1] = h_lu_off[k] + (int64_t)pad4(P * P) * N * B;
+            h_item_off[k] = (int64_t)c * N;
+            h_lev_se[2 * k] = c_lev_off[c];
+            h_lev_se[2 * k + 1] = c_lev_off[c] + (int32_t)lptr[c].size() - 1;
+            h_wc_se[2 * k] = c_wc_off[c];
+            h_wc_se[2 * k + 1] = c_wc_off[c + 1] - c_wc_off[c];
+            b_global[k] = global_flag[c];
+            tot += (int64_t)lptr[c].size() - 1;
+        }
+        h_item_off[nb] = (int64_t)nc * N;
         ctx->total_levels = tot;
         ctx->max_width = maxw;
-        // the kernels loop lv over [start, end): per-bundle (start, end) pairs
-        std::vector<int32_t> h_lev_se(2 * nb);
-        for (int k = 0; k < nb; ++k) {
-            h_lev_se[2 * k] = h_lev_off[k];
-            h_lev_se[2 * k + 1] = h_lev_off[k] + (int32_t)lptr[k].size() - 1;
-        }
+        ctx->n_classes = nc;
         phase("levels, records, pools");
         // XCD-aware block -> bundle map: blocks are dispatched round robin over
         // 8 XCDs; give each XCD a contiguous range of bundles sorted by the
@@ -3023,35 +3174,19 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->lu_off = ctx->upload(h_lu_off.data(), nb + 1);
         h_wcell.resize(std::max<size_t>(h_wcell.size(), 1), make_int4(0, -1, -1, -1));
         ctx->wcell = ctx->upload(h_wcell.data(), h_wcell.size());
-        ctx->wc_off = ctx->upload(h_wc_off.data(), h_wc_off.size());
-        {
-            // per item (bundle, slot, direction of the bundle): {cell | pool
-            // slot, face words}
-            phase("schedule upload");
-            if (ctx->max_bundle_dirs == 1) {
-                // one direction per bundle: the items are the bundles' slots
-                ctx->irec = ctx->upload(h_rec.data(), h_rec.size());
-            } else {
-                std::vector<int4> h_irec((size_t)h_item_off[nb]);
-                std::atomic<int> nk{0};
-                run_threads([&]() {
-                    for (int k = nk++; k < nb; k = nk++) {
-                        const int B = (int)bundles[k].size();
-                        for (int q = 0; q < N; ++q)
-                            for (int b = 0; b < B; ++b)
-                                h_irec[(size_t)h_item_off[k] + (size_t)q * B + b] = h_rec[(size_t)k * N + q];
-                    }
-                });
-                ctx->irec = ctx->upload(h_irec.data(), h_irec.size());
-            }
-        }
+        ctx->wc_off = ctx->upload(h_wc_se.data(), h_wc_se.size());
+        phase("schedule upload");
+        // per (class, slot): {cell | pool slot, face words}, read by every
+        // direction of every bundle of the class
+        ctx->irec = ctx->upload(h_rec.data(), h_rec.size());
         phase("item records");
-        ctx->bundle_global = ctx->upload(global_flag.data(), global_flag.size());
+        ctx->bundle_global = ctx->upload(b_global.data(), b_global.size());
         for (int k = 0; k < nb; ++k) {
-            ctx->global_bundles += global_flag[k];
-            ctx->pool_peak = std::max(ctx->pool_peak, peak[k]);
-            ctx->old_reads += old_reads[k];
-            ctx->periodic_reads += per_reads[k];
+            const int c = cls_of[k];
+            ctx->global_bundles += global_flag[c];
+            ctx->pool_peak = std::max(ctx->pool_peak, peak[c]);
+            ctx->old_reads += old_reads[c];
+            ctx->periodic_reads += per_reads[c];
         }
         ctx->item_off = ctx->upload(h_item_off.data(), nb + 1);
         ctx->h_lu_off = h_lu_off;
@@ -3204,9 +3339,19 @@ SweepArgs<T> sweep_args(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int 
 }
 
 // whether a context's sweep of this precision runs the pipelined kernel
+// the pipelined kernel addresses a bundle's factors through a buffer
+// descriptor (32-bit byte range and offsets); a bundle whose factors exceed
+// that runs the plain level kernel, which indexes with 64-bit pointers
+template <typename T> bool factors_fit_descriptor(const pbte_ctx* ctx)
+{
+    for (size_t k = 0; k + 1 < ctx->h_lu_off.size(); ++k)
+        if ((uint64_t)(ctx->h_lu_off[k + 1] - ctx->h_lu_off[k]) * sizeof(T) > 0xffffffffull) return false;
+    return true;
+}
+
 template <int P, typename T> bool runs_pipelined(const pbte_ctx* ctx)
 {
-    return ctx->variant != 1 && sweep_lds_bytes<P, T>(ctx) <= ctx->lds_blk;
+    return ctx->variant != 1 && sweep_lds_bytes<P, T>(ctx) <= ctx->lds_blk && factors_fit_descriptor<T>(ctx);
 }
 
 template <int P, typename T>
@@ -3891,19 +4036,21 @@ extern "C" int pbte_gsis_setup(pbte_ctx* ctx, const pbte_gsis_data* g)
             hw[3 * D + d] = cy * (5.0 * cy * cy - 3.0) * w;
         }
         ctx->gs_hw = ctx->upload(hw.data(), hw.size());
-        // stress columns of inv(AA_SOL) (device [N][7P][7P]): [N][7P][4P]
-        // transposed stress columns invT [N][4P][7P]
-        ctx->gs_inv4 = ctx->alloc<double>((size_t)N * 7 * P * 4 * P);
-        hipLaunchKernelGGL(transpose_slice_kernel, dim3(nblocks((int64_t)N * 7 * P * 4 * P, 256)), dim3(256), 0,
-                           ctx->stream, g->inv_aa_sol, (int64_t)N, 7 * P, 7 * P, 3 * P, 4 * P, ctx->gs_inv4);
-        // transposed AA_TRACE [N][3][3Q][7P]
-        ctx->gs_aa_trace = ctx->alloc<double>((size_t)N * 3 * 7 * P * 3 * Q);
-        hipLaunchKernelGGL(transpose_slice_kernel, dim3(nblocks((int64_t)N * 3 * 7 * P * 3 * Q, 256)), dim3(256), 0,
-                           ctx->stream, g->aa_trace, (int64_t)N * 3, 7 * P, 3 * Q, 0, 3 * Q, ctx->gs_aa_trace);
-        // transposed BA_SOL [N][3][7P][3Q], and the per-side trace sums
-        ctx->gs_ba_sol = ctx->alloc<double>((size_t)N * 3 * 3 * Q * 7 * P);
-        hipLaunchKernelGGL(transpose_slice_kernel, dim3(nblocks((int64_t)N * 3 * 7 * P * 3 * Q, 256)), dim3(256), 0,
-                           ctx->stream, g->ba_sol, (int64_t)N * 3, 3 * Q, 7 * P, 0, 7 * P, ctx->gs_ba_sol);
+        // element operators from the device inverse, AA_TRACE and BA_SOL
+        const int K = 3 * P + 9 * Q;
+        ctx->gs_op = ctx->alloc<double>((size_t)N * 4 * P * K);
+        hipLaunchKernelGGL(gsis_op_kernel, dim3(nblocks((int64_t)N * 4 * P * K, 256)), dim3(256), 0, ctx->stream,
+                           g->inv_aa_sol, g->ba_sol, (int64_t)N, P, Q, 3 * P, 4 * P, ctx->gs_op);
+        ctx->gs_op_hi = ctx->alloc<double>((size_t)N * 4 * P * 4 * P);
+        hipLaunchKernelGGL(inv_rows_kernel, dim3(nblocks((int64_t)N * 16 * P * P, 256)), dim3(256), 0, ctx->stream,
+                           g->inv_aa_sol, (int64_t)N, P, 3 * P, 4 * P, ctx->gs_op_hi);
+        ctx->gs_aat = ctx->alloc<double>((size_t)N * 9 * Q * 3 * P);
+        hipLaunchKernelGGL(aat_rows_kernel, dim3(nblocks((int64_t)N * 27 * Q * P, 256)), dim3(256), 0, ctx->stream,
+                           g->aa_trace, (int64_t)N, P, Q, 0, 3 * P, ctx->gs_aat);
+        ctx->gs_aat_hi = ctx->alloc<double>((size_t)N * 9 * Q * 4 * P);
+        hipLaunchKernelGGL(aat_rows_kernel, dim3(nblocks((int64_t)N * 36 * Q * P, 256)), dim3(256), 0, ctx->stream,
+                           g->aa_trace, (int64_t)N, P, Q, 3 * P, 4 * P, ctx->gs_aat_hi);
+        HIP_TRY(hipGetLastError());
         ctx->gs_side = ctx->alloc<double>((size_t)N * 3 * 3 * Q);
         HIP_TRY(hipGetLastError());
         ctx->gs_t1 = ctx->upload(g->t1, F);
@@ -3967,18 +4114,38 @@ extern "C" int pbte_gsis_moments(pbte_ctx* ctx, int32_t precision, const void* v
     });
 }
 
-extern "C" int pbte_gsis_source(pbte_ctx* ctx, const double* mom7, double* aa_src)
+// the binary32 copies of the step's operators, made once
+static void gsis_narrow(pbte_ctx* ctx)
+{
+    if (ctx->gs_op32) return;
+    const int64_t N = ctx->N, P = ctx->P, Q = ctx->Q, K = 3 * P + 9 * Q;
+    const int64_t nop = N * 4 * P * K, naat = N * 9 * Q * 3 * P;
+    ctx->gs_op32 = ctx->alloc<float>((size_t)nop);
+    ctx->gs_aat32 = ctx->alloc<float>((size_t)naat);
+    hipLaunchKernelGGL(narrow_kernel<float>, dim3(4096), dim3(256), 0, ctx->stream, ctx->gs_op, ctx->gs_op32, nop);
+    hipLaunchKernelGGL(narrow_kernel<float>, dim3(4096), dim3(256), 0, ctx->stream, ctx->gs_aat, ctx->gs_aat32, naat);
+    HIP_TRY(hipGetLastError());
+}
+
+extern "C" int pbte_gsis_source(pbte_ctx* ctx, int32_t precision, const double* mom7, double* src)
 {
     return guarded(ctx, [&]() {
         if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
-        const int N = ctx->N, P = ctx->P;
-#define HS(PP)                                                                                            \
-    case PP:                                                                                              \
-        hipLaunchKernelGGL(hot_source_kernel<PP>, dim3((N + 3) / 4), dim3(256), 0, ctx->stream, mom7, ctx->gx, \
-                           ctx->gy, ctx->gs_inv4, N, ctx->gs_tc5, aa_src);                               \
+        if (precision != PBTE_FP64 && precision != PBTE_FP32) throw std::string("gsis_source: bad precision");
+        const int64_t N = ctx->N;
+        if (precision == PBTE_FP32) gsis_narrow(ctx);
+        const dim3 grid((unsigned)((N + 7) / 8));
+#define HS(PP, QQ)                                                                                            \
+    case PP:                                                                                                  \
+        if (precision == PBTE_FP64)                                                                           \
+            hipLaunchKernelGGL((hot_side_kernel<PP, QQ, double>), grid, dim3(256), 0, ctx->stream, mom7, ctx->gx, \
+                               ctx->gy, ctx->gs_op, N, ctx->gs_tc5, src, ctx->gs_side);                       \
+        else                                                                                                  \
+            hipLaunchKernelGGL((hot_side_kernel<PP, QQ, float>), grid, dim3(256), 0, ctx->stream, mom7, ctx->gx, \
+                               ctx->gy, ctx->gs_op32, N, ctx->gs_tc5, src, ctx->gs_side);                     \
         break;
-        switch (P) {
-            HS(3) HS(6) HS(10) HS(15)
+        switch (ctx->P) {
+            HS(3, 2) HS(6, 3) HS(10, 4) HS(15, 5)
             default: throw std::string("gsis_source: unsupported P");
         }
 #undef HS
@@ -3987,21 +4154,11 @@ extern "C" int pbte_gsis_source(pbte_ctx* ctx, const double* mom7, double* aa_sr
     });
 }
 
-extern "C" int pbte_gsis_rhs(pbte_ctx* ctx, int32_t precision, const void* vdf, const double* aa_src, double* ffa)
+extern "C" int pbte_gsis_rhs(pbte_ctx* ctx, int32_t precision, const void* vdf, double* ffa)
 {
     return guarded(ctx, [&]() {
         if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
         const int N = ctx->N, P = ctx->P, Q = ctx->Q, F = ctx->F;
-#define TS(PP, QQ)                                                                                     \
-    case PP:                                                                                           \
-        hipLaunchKernelGGL((trace_side_kernel<PP, QQ>), dim3((N + 3) / 4), dim3(256), 0, ctx->stream,  \
-                           ctx->gs_ba_sol, aa_src, N, ctx->gs_side);                                   \
-        break;
-        switch (P) {
-            TS(3, 2) TS(6, 3) TS(10, 4) TS(15, 5)
-            default: throw std::string("gsis_rhs: unsupported P");
-        }
-#undef TS
         hipLaunchKernelGGL(trace_rhs_kernel, dim3(nblocks((int64_t)F * 3 * Q, 256)), dim3(256), 0, ctx->stream,
                            ctx->gs_side, ctx->gs_t1, ctx->gs_l1, ctx->gs_t2, ctx->gs_l2, ctx->gs_flip, F, Q, ffa);
         HIP_TRY(hipGetLastError());
@@ -4037,38 +4194,70 @@ extern "C" int pbte_gsis_rhs(pbte_ctx* ctx, int32_t precision, const void* vdf, 
     });
 }
 
-extern "C" int pbte_gsis_precond_setup(pbte_ctx* ctx, const double* inv_aa_sol)
+extern "C" int pbte_gsis_precond_setup(pbte_ctx* ctx, const double* inv_aa_sol, const double* ba_sol)
 {
     return guarded(ctx, [&]() {
         if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
-        const int P = ctx->P, N = ctx->N;
-        // T, qx, qy columns of inv(AA_SOL) (device [N][7P][7P]): [N][7P][3P]
-        if (!ctx->gs_inv3) ctx->gs_inv3 = ctx->alloc<double>((size_t)N * 7 * P * 3 * P);
-        hipLaunchKernelGGL(column_slice_kernel, dim3(nblocks((int64_t)N * 7 * P * 3 * P, 256)), dim3(256), 0,
-                           ctx->stream, inv_aa_sol, (int64_t)N * 7 * P, 7 * P, 0, 3 * P, ctx->gs_inv3);
+        const int P = ctx->P, Q = ctx->Q;
+        const int64_t N = ctx->N, K = 3 * P + 9 * Q;
+        if (!ctx->gs_op_pc) ctx->gs_op_pc = ctx->alloc<double>((size_t)N * 3 * P * K);
+        hipLaunchKernelGGL(gsis_op_kernel, dim3(nblocks(N * 3 * P * K, 256)), dim3(256), 0, ctx->stream, inv_aa_sol,
+                           ba_sol, N, P, Q, 0, 3 * P, ctx->gs_op_pc);
         HIP_TRY(hipGetLastError());
         return 0;
     });
 }
 
-extern "C" int pbte_gsis_precond_src(pbte_ctx* ctx, const double* r, double sign, double* aa_src)
+extern "C" int pbte_gsis_precond_src(pbte_ctx* ctx, const double* r, double sign, double* src)
 {
     return guarded(ctx, [&]() {
-        if (!ctx->gs_inv3) throw std::string("pbte_gsis_precond_setup first");
-        hipLaunchKernelGGL(precond_src_kernel, dim3(nblocks((int64_t)ctx->N * 7 * ctx->P, 256)), dim3(256), 0,
-                           ctx->stream, ctx->gs_inv3, r, sign, ctx->N, ctx->P, aa_src);
+        if (!ctx->gs_op_pc) throw std::string("pbte_gsis_precond_setup first");
+        const int64_t N = ctx->N;
+        hipLaunchKernelGGL(precond_side_kernel<double>, dim3((unsigned)((N + 7) / 8)), dim3(256), 0, ctx->stream, r,
+                           sign, ctx->gs_op_pc, N, ctx->P, ctx->Q, src, ctx->gs_side);
         HIP_TRY(hipGetLastError());
         return 0;
     });
 }
 
-extern "C" int pbte_gsis_local(pbte_ctx* ctx, const double* aa_src, const double* u_trace, double* uq)
+extern "C" int pbte_gsis_local(pbte_ctx* ctx, int32_t precision, const double* src, const double* u_trace,
+                               double* uq)
 {
     return guarded(ctx, [&]() {
         if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
-        const int N = ctx->N, P = ctx->P;
-        hipLaunchKernelGGL(local_solve_kernel, dim3((N + 3) / 4), dim3(256), 0, ctx->stream,
-                           ctx->gs_aa_trace, aa_src, u_trace, ctx->gs_tri_faces, N, P, ctx->Q, uq);
+        const int64_t N = ctx->N, P = ctx->P;
+        const unsigned g = nblocks(N * 3 * P, 256);
+        if (precision == PBTE_FP64) {
+            hipLaunchKernelGGL(local3_kernel<double>, dim3(g), dim3(256), 0, ctx->stream, ctx->gs_aat, src, u_trace,
+                               ctx->gs_tri_faces, N, ctx->P, ctx->Q, uq);
+        } else if (precision == PBTE_FP32) {
+            gsis_narrow(ctx);
+            hipLaunchKernelGGL(local3_kernel<float>, dim3(g), dim3(256), 0, ctx->stream, ctx->gs_aat32, src, u_trace,
+                               ctx->gs_tri_faces, N, ctx->P, ctx->Q, uq);
+        } else {
+            throw std::string("gsis_local: bad precision");
+        }
+        HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+extern "C" int pbte_gsis_uq(pbte_ctx* ctx, const double* mom7, const double* u_trace, double* uq)
+{
+    return guarded(ctx, [&]() {
+        if (!ctx->gsis) throw std::string("pbte_gsis_setup first");
+        const int64_t N = ctx->N;
+#define UQ(PP, QQ)                                                                                           \
+    case PP:                                                                                                 \
+        hipLaunchKernelGGL((uq_full_kernel<PP, QQ>), dim3((unsigned)((N + 3) / 4)), dim3(256), 0, ctx->stream, \
+                           mom7, ctx->gx, ctx->gy, ctx->gs_op, ctx->gs_op_hi, ctx->gs_aat, ctx->gs_aat_hi,       \
+                           u_trace, ctx->gs_tri_faces, N, ctx->gs_tc5, uq);                                  \
+        break;
+        switch (ctx->P) {
+            UQ(3, 2) UQ(6, 3) UQ(10, 4) UQ(15, 5)
+            default: throw std::string("gsis_uq: unsupported P");
+        }
+#undef UQ
         HIP_TRY(hipGetLastError());
         return 0;
     });
@@ -4126,12 +4315,12 @@ extern "C" int pbte_gsis_apply_correction(pbte_ctx* ctx, int32_t precision, void
         const double dr = ctx->gs_drift;
         if (cells) {
             if (ncells <= 0) return 0;
-            const unsigned g = nblocks((int64_t)ncells * P, 256);
+            const unsigned g = nblocks((int64_t)ncells * P * D, 256);
             if (precision == PBTE_FP64)
-                hipLaunchKernelGGL((apply_correction_kernel<double, 1>), dim3(g), dim3(256), 0, ctx->stream,
+                hipLaunchKernelGGL(apply_correction_cells_kernel<double>, dim3(g), dim3(256), 0, ctx->stream,
                                    (double*)vdf, coef, ctx->cxv, ctx->cyv, cells, ncells, D, N, P, dr);
             else
-                hipLaunchKernelGGL((apply_correction_kernel<float, 1>), dim3(g), dim3(256), 0, ctx->stream,
+                hipLaunchKernelGGL(apply_correction_cells_kernel<float>, dim3(g), dim3(256), 0, ctx->stream,
                                    (float*)vdf, coef, ctx->cxv, ctx->cyv, cells, ncells, D, N, P, dr);
         } else if (precision == PBTE_FP64) {
             if (np % 2 == 0)

@@ -87,6 +87,12 @@ __global__ void k_fill(double* __restrict__ p, int64_t n)
         p[i] = 0.0;
 }
 
+__global__ void k_narrow(const double* __restrict__ in, float* __restrict__ out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
 __global__ void k_set_int(int32_t* __restrict__ p, int32_t v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
@@ -548,24 +554,25 @@ __device__ inline double group_sum(double v)
 
 // dot product of a row with a vector by the LPR lanes of a group, each lane
 // its columns k = lane + LPR j with two accumulators (even, odd j)
-template <int LPR>
-__device__ inline double row_dot(const double* __restrict__ row, const double* __restrict__ v, int ncols, int lane)
+template <int LPR, typename TF>
+__device__ inline double row_dot(const TF* __restrict__ row, const double* __restrict__ v, int ncols, int lane)
 {
     double s0 = 0.0, s1 = 0.0;
     int k = lane;
     for (; k + LPR < ncols; k += 2 * LPR) {
-        s0 = fma(row[k], v[k], s0);
-        s1 = fma(row[k + LPR], v[k + LPR], s1);
+        s0 = fma((double)row[k], v[k], s0);
+        s1 = fma((double)row[k + LPR], v[k + LPR], s1);
     }
-    if (k < ncols) s0 = fma(row[k], v[k], s0);
+    if (k < ncols) s0 = fma((double)row[k], v[k], s0);
     return group_sum<LPR>(s0 + s1);
 }
 
 // c[i] -= W[i] . y over the rows of a work item: 256 / LPR row groups of
-// LPR lanes, 2 rows per group
-template <int LPR>
+// LPR lanes, 2 rows per group; the factors in TF (binary64, or their
+// binary32 copy), the sums in binary64
+template <int LPR, typename TF>
 __global__ void __launch_bounds__(256) k_fwd_gemv(const int2* __restrict__ items, const Front* __restrict__ fr,
-                                                  const double* __restrict__ fac, const double* __restrict__ y,
+                                                  const TF* __restrict__ fac, const double* __restrict__ y,
                                                   double* __restrict__ cv)
 {
     constexpr int G = 256 / LPR;
@@ -574,17 +581,17 @@ __global__ void __launch_bounds__(256) k_fwd_gemv(const int2* __restrict__ items
     const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
     const double* yv = y + f.piv;
     for (int r = it.y + grp; r < min(it.y + 2 * G, f.ncb); r += G) {
-        const double s = row_dot<LPR>(fac + f.w + (int64_t)r * f.nfs, yv, f.nfs, lane);
+        const double s = row_dot<LPR, TF>(fac + f.w + (int64_t)r * f.nfs, yv, f.nfs, lane);
         if (lane == 0) cv[f.cvec + r] -= s;
     }
 }
 
 // x[own faces] = [Z V] [y; -x(ancestors' faces)] over the rows of a work
 // item (as k_fwd_gemv); the input vector is staged in LDS
-template <int LPR>
+template <int LPR, typename TF>
 __global__ void __launch_bounds__(256) k_bwd(const int2* __restrict__ items, const Front* __restrict__ fr,
                                              const int32_t* __restrict__ faces, int b,
-                                             const double* __restrict__ fac, const double* __restrict__ y,
+                                             const TF* __restrict__ fac, const double* __restrict__ y,
                                              double* __restrict__ x)
 {
     extern __shared__ double u[];
@@ -596,7 +603,7 @@ __global__ void __launch_bounds__(256) k_bwd(const int2* __restrict__ items, con
     __syncthreads();
     const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
     for (int r = it.y + grp; r < min(it.y + 2 * G, f.nfs); r += G) {
-        const double s = row_dot<LPR>(fac + f.zv + (int64_t)r * f.n, u, f.n, lane);
+        const double s = row_dot<LPR, TF>(fac + f.zv + (int64_t)r * f.n, u, f.n, lane);
         if (lane == 0) x[(int64_t)faces[f.faces + r / b] * b + r % b] = s;
     }
 }
@@ -853,12 +860,14 @@ struct pbte_trace {
     int4* d_rec = nullptr;
     int2 *d_fw = nullptr, *d_bw = nullptr;
     double *d_fac = nullptr, *d_ws[2] = {nullptr, nullptr}, *d_pan = nullptr, *d_y = nullptr, *d_cv = nullptr;
+    float* d_fac32 = nullptr;   // binary32 copy of the factors (made by the first binary32 solve)
+    bool fac32_current = false;
     int64_t ws_size[2] = {0, 0}, pan_size = 0;
 
     ~pbte_trace()
     {
         void* ps[] = {d_fr, d_faces, d_fmap, d_nodes, d_perm, d_ipiv, d_status, d_rec, d_fw, d_bw,
-                      d_fac, d_ws[0], d_ws[1], d_pan, d_y, d_cv};
+                      d_fac, d_ws[0], d_ws[1], d_pan, d_y, d_cv, d_fac32};
         for (void* p : ps)
             if (p) (void)hipFree(p);
     }
@@ -1089,6 +1098,7 @@ extern "C" int pbte_trace_factor(pbte_trace* t, const double* bsr_val)
     return tguard(t, [&] {
         t->alloc();
         t->factored = false;
+        t->fac32_current = false;
         // diagonal-block pivots first; partial pivoting among the fully
         // summed rows when a block has no pivot or the factors fail the
         // backward-error test (or as the caller forces)
@@ -1123,7 +1133,8 @@ extern "C" int pbte_trace_factor(pbte_trace* t, const double* bsr_val)
 }
 
 namespace {
-void solve_pass(pbte_trace* t, const double* rhs, double* x)
+template <typename TF>
+void solve_pass_t(pbte_trace* t, const TF* fac, const double* rhs, double* x)
 {
     Symbolic& S = t->S;
     hipStream_t st = t->stream;
@@ -1133,29 +1144,45 @@ void solve_pass(pbte_trace* t, const double* rhs, double* x)
         hipLaunchKernelGGL(k_fwd_gather, dim3(nf), dim3(256), (size_t)L.n_max * 8, st, t->d_nodes + L.node_off,
                            t->d_fr, t->d_faces, t->d_fmap, t->d_perm, S.b, rhs, t->d_y, t->d_cv);
         if (L.fw_n > 0) {
-            auto fn = L.lpr_fw == 8 ? k_fwd_gemv<8> : L.lpr_fw == 16 ? k_fwd_gemv<16>
-                    : L.lpr_fw == 32 ? k_fwd_gemv<32> : k_fwd_gemv<64>;
-            hipLaunchKernelGGL(fn, dim3(L.fw_n), dim3(256), 0, st, t->d_fw + L.fw_off, t->d_fr, t->d_fac, t->d_y,
+            auto fn = L.lpr_fw == 8 ? k_fwd_gemv<8, TF> : L.lpr_fw == 16 ? k_fwd_gemv<16, TF>
+                    : L.lpr_fw == 32 ? k_fwd_gemv<32, TF> : k_fwd_gemv<64, TF>;
+            hipLaunchKernelGGL(fn, dim3(L.fw_n), dim3(256), 0, st, t->d_fw + L.fw_off, t->d_fr, fac, t->d_y,
                                t->d_cv);
         }
     }
     for (int l = 0; l <= S.max_depth; ++l) {
         const Level& L = S.levels[l];
         if (L.bw_n > 0) {
-            auto fn = L.lpr_bw == 8 ? k_bwd<8> : L.lpr_bw == 16 ? k_bwd<16> : L.lpr_bw == 32 ? k_bwd<32> : k_bwd<64>;
+            auto fn = L.lpr_bw == 8 ? k_bwd<8, TF> : L.lpr_bw == 16 ? k_bwd<16, TF>
+                    : L.lpr_bw == 32 ? k_bwd<32, TF> : k_bwd<64, TF>;
             hipLaunchKernelGGL(fn, dim3(L.bw_n), dim3(256), (size_t)L.n_max * 8, st, t->d_bw + L.bw_off, t->d_fr,
-                               t->d_faces, S.b, t->d_fac, t->d_y, x);
+                               t->d_faces, S.b, fac, t->d_y, x);
         }
     }
     TL_TRY(hipGetLastError());
 }
+
+void solve_pass(pbte_trace* t, const double* rhs, double* x) { solve_pass_t<double>(t, t->d_fac, rhs, x); }
 }  // namespace
 
-extern "C" int pbte_trace_solve(pbte_trace* t, const double* rhs, double* x)
+extern "C" int pbte_trace_solve(pbte_trace* t, int32_t precision, const double* rhs, double* x)
 {
     return tguard(t, [&] {
         if (!t->factored) throw std::string("trace solve: pbte_trace_factor has not succeeded");
-        solve_pass(t, rhs, x);
+        if (precision == PBTE_FP64) {
+            solve_pass(t, rhs, x);
+        } else if (precision == PBTE_FP32) {
+            if (!t->d_fac32) t->d_fac32 = dmalloc<float>((size_t)t->S.fac_size);
+            if (!t->fac32_current) {
+                hipLaunchKernelGGL(k_narrow, dim3(4096), dim3(256), 0, t->stream, t->d_fac, t->d_fac32,
+                                   t->S.fac_size);
+                TL_TRY(hipGetLastError());
+                t->fac32_current = true;
+            }
+            solve_pass_t<float>(t, t->d_fac32, rhs, x);
+        } else {
+            throw std::string("trace solve: precision is PBTE_FP64 or PBTE_FP32");
+        }
         return 0;
     });
 }

@@ -397,7 +397,7 @@ class Solver:
             missing = int(np.count_nonzero(triid < 0))
             if missing:
                 warnings.warn(f"{missing} sampling points fell outside the mesh")
-            uq = self.acc.uq_d.cpu().numpy() if self.acc is not None else None
+            uq = self.acc.full_uq().cpu().numpy() if self.acc is not None else None
             m = Moments.zeros(self.ndof, self.n_tris)
             self.ctx.moments(self.sweep_vdf, m)
             fields = _fields(self.mesh, self.basis, m.ts, m.qxs, m.qys, px, py, triid, uq, self.ndof)

@@ -283,24 +283,24 @@ class MomentPreconditioner:
         self.nd, self.nt = solver.ndof, solver.n_tris
         self.nmom = 3 * self.nd * self.nt
         ctx = solver.ctx
-        _native.check(ctx.handle, _native.lib().pbte_gsis_precond_setup(ctx.handle, ptr(acc.inv_aa_sol)),
-                      "gsis_precond_setup")
-        self.aa_src = dev_zeros((7 * self.nd, self.nt))
-        self.ffa = dev_zeros(acc.ffa_d.shape)
-        self.u = dev_zeros(acc.u_d.shape)
-        self.uq = dev_zeros((7 * self.nd, self.nt))
+        _native.check(ctx.handle, _native.lib().pbte_gsis_precond_setup(
+            ctx.handle, ptr(acc.inv_aa_sol), ptr(acc.ba_sol)), "gsis_precond_setup")
+        self.src = dev_zeros((3 * self.nd, self.nt))
+        st = acc.last
+        self.ffa = dev_zeros(st["ffa"].shape)
+        self.u = dev_zeros(st["u"].shape)
+        self.uq = dev_zeros((3 * self.nd, self.nt))
         self.calls = 0
 
     def __call__(self, v) -> None:
         h = self.s.ctx.handle
         lib = _native.lib()
-        _native.check(h, lib.pbte_gsis_precond_src(h, ptr(v), self.sign, ptr(self.aa_src)),
-                      "gsis_precond_src")
-        _native.check(h, lib.pbte_gsis_rhs(h, _native.PBTE_FP64, None, ptr(self.aa_src), ptr(self.ffa)),
-                      "gsis_rhs")
+        fp64 = _native.PBTE_FP64
+        _native.check(h, lib.pbte_gsis_precond_src(h, ptr(v), self.sign, ptr(self.src)), "gsis_precond_src")
+        _native.check(h, lib.pbte_gsis_rhs(h, fp64, None, ptr(self.ffa)), "gsis_rhs")
         self.acc.trace_solve(self.ffa, self.u)
-        _native.check(h, lib.pbte_gsis_local(h, ptr(self.aa_src), ptr(self.u), ptr(self.uq)), "gsis_local")
-        v[:self.nmom].copy_(self.uq[:3 * self.nd].reshape(-1))
+        _native.check(h, lib.pbte_gsis_local(h, fp64, ptr(self.src), ptr(self.u), ptr(self.uq)), "gsis_local")
+        v[:self.nmom].copy_(self.uq.reshape(-1))
         self.calls += 1
 
 

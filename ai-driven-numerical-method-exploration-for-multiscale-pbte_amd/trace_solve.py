@@ -103,9 +103,12 @@ class TraceSolver:
             raise np.linalg.LinAlgError(msg)
         self.factored = True
 
-    def solve(self, rhs, out) -> None:
-        """``out = K^-1 rhs`` (device float64 tensors of ``F * 3Q`` values)."""
+    def solve(self, rhs, out, precision: str = "fp64") -> None:
+        """``out = K^-1 rhs`` (device float64 tensors of ``F * 3Q`` values)
+        from the binary64 factors, or (``precision="fp32"``) from their
+        binary32 copy, sums in binary64."""
         if rhs.numel() != self.n or out.numel() != self.n:
             raise ValueError("trace vectors do not match the system size")
-        self._check(self._lib.pbte_trace_solve(self.handle, ptr(rhs), ptr(out)), "trace_solve")
+        code = {"fp64": _native.PBTE_FP64, "fp32": _native.PBTE_FP32}[precision]
+        self._check(self._lib.pbte_trace_solve(self.handle, code, ptr(rhs), ptr(out)), "trace_solve")
         self.solve_count += 1

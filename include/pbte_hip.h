@@ -294,29 +294,47 @@ int pbte_gsis_setup(pbte_ctx *ctx, const pbte_gsis_data *g);
  * correction._correct_kernel (hot_source.py:29-58, correction.py:40-52). */
 int pbte_gsis_moments(pbte_ctx *ctx, int32_t precision, const void *vdf, double *mom7);
 
-/* AA_SRC [7P][N] = inv(AA_SOL) (stress rows of the HoT source).  Replaces
- * hot_source.hot_source (hot_source.py:61-75). */
-int pbte_gsis_source(pbte_ctx *ctx, const double *mom7, double *aa_src);
+/* The element pass of the accelerated step: the stress work of the HoT
+ * source from mom7, then src [3P][N] = the T, qx, qy rows of AA_SRC =
+ * inv(AA_SOL) work and, kept in the context for pbte_gsis_rhs, the sums
+ * BA_SOL AA_SRC of each element side -- one operator per element
+ * [4P][3P + 9Q] built by pbte_gsis_setup (both are linear in the work).
+ * precision PBTE_FP32 reads binary32 copies of the operators (made on the
+ * first such call; sums in binary64), for the binary32 iterations of the
+ * refined run.  Replaces hot_source.hot_source (hot_source.py:61-75) and
+ * the element half of GlobalSolver.rhs (global_solve.py:128-150). */
+int pbte_gsis_source(pbte_ctx *ctx, int32_t precision, const double *mom7, double *src);
 
-/* Trace right-hand side ffa [F][3Q] (two-sided, periodic mirror, wall rows
- * from the kinetic traces; vdf == NULL stands for a zero distribution, wall
- * rows zero).  Replaces GlobalSolver.rhs (global_solve.py:128-186). */
-int pbte_gsis_rhs(pbte_ctx *ctx, int32_t precision, const void *vdf, const double *aa_src,
-                  double *ffa);
+/* Trace right-hand side ffa [F][3Q] from the side sums of the last
+ * pbte_gsis_source or pbte_gsis_precond_src (two-sided, periodic mirror)
+ * and the wall rows from the kinetic traces of vdf (vdf == NULL stands for
+ * a zero distribution, wall rows zero).  Replaces GlobalSolver.rhs
+ * (global_solve.py:128-186). */
+int pbte_gsis_rhs(pbte_ctx *ctx, int32_t precision, const void *vdf, double *ffa);
 
-/* Copies the T, qx, qy columns of inv_aa_sol [N][7P][7P] (device) for the
- * moment preconditioner.  Part of MomentPreconditioner.__init__
- * (krylov.py:379-390). */
-int pbte_gsis_precond_setup(pbte_ctx *ctx, const double *inv_aa_sol);
+/* Builds the moment preconditioner's element operator from inv_aa_sol
+ * [N][7P][7P] and ba_sol [N][3][3Q][7P] (device): the T, qx, qy columns of
+ * the inverse in place of the stress columns.  Part of
+ * MomentPreconditioner.__init__ (krylov.py:379-390). */
+int pbte_gsis_precond_setup(pbte_ctx *ctx, const double *inv_aa_sol, const double *ba_sol);
 
-/* aa_src [7P][N] = inv(AA_SOL) applied to sign * the T, qx, qy blocks of r
- * (three [P][N] blocks, device).  Replaces the element source of
+/* src [3P][N] and the side sums (as pbte_gsis_source) for the element
+ * source inv(AA_SOL) applied to sign * the T, qx, qy blocks of r (three
+ * [P][N] blocks, device).  Replaces the element source of
  * MomentPreconditioner._apply (krylov.py:392-402). */
-int pbte_gsis_precond_src(pbte_ctx *ctx, const double *r, double sign, double *aa_src);
+int pbte_gsis_precond_src(pbte_ctx *ctx, const double *r, double sign, double *src);
 
-/* UQ [7P][N] = AA_SRC + sum AA_TRACE U_TRACE (u_trace [F][3Q]).  Replaces
- * local_solve.local_solve (local_solve.py:16-26). */
-int pbte_gsis_local(pbte_ctx *ctx, const double *aa_src, const double *u_trace, double *uq);
+/* uq [3P][N] = the T, qx, qy rows of UQ = AA_SRC + sum AA_TRACE U_TRACE
+ * (u_trace [F][3Q]); precision as pbte_gsis_source.  Replaces
+ * local_solve.local_solve (local_solve.py:16-26) for the rows the
+ * correction reads. */
+int pbte_gsis_local(pbte_ctx *ctx, int32_t precision, const double *src, const double *u_trace,
+                    double *uq);
+
+/* The full UQ [7P][N] of a step from its mom7 and u_trace (binary64), the
+ * stress rows included (the field output's Nxx, Nxy, Nyy).  Replaces
+ * local_solve.local_solve (local_solve.py:16-26) where every row is read. */
+int pbte_gsis_uq(pbte_ctx *ctx, const double *mom7, const double *u_trace, double *uq);
 
 /* Damped kinetic correction of vdf and the moments; defect and mkin
  * ([3][P][N]) may be NULL.  Replaces correction.correct_vdf
@@ -446,8 +464,12 @@ int pbte_trace_quality(pbte_trace *t, int32_t *pivoting, double *berr);
  * Returns 0, or 1 when a front has no nonzero pivot (singular K). */
 int pbte_trace_factor(pbte_trace *t, const double *bsr_val);
 
-/* x = K^-1 rhs (device, [F][3Q] each; x must not alias rhs). */
-int pbte_trace_solve(pbte_trace *t, const double *rhs, double *x);
+/* x = K^-1 rhs (device, [F][3Q] each; x must not alias rhs), from the
+ * binary64 factors (precision PBTE_FP64) or from their binary32 copy
+ * (PBTE_FP32: made by the first such solve after each factorisation, sums
+ * in binary64; the correction solves of the refined run's binary32
+ * iterations). */
+int pbte_trace_solve(pbte_trace *t, int32_t precision, const double *rhs, double *x);
 
 #ifdef __cplusplus
 }

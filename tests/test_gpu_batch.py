@@ -70,15 +70,16 @@ def test_batched_sweep_is_bitwise_individual_sweeps(pkg, tmp_path, prec):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_run_source_batch_is_each_solver_alone(pkg, tmp_path, prec):
-    """Lockstep CIS of four problems gives each problem's own run: the same
-    iteration count, residual and mass histories and moments, bit for bit
-    (fp32: with the switch to binary64 sweeps at its own step)."""
+    """Lockstep CIS of four problems gives each problem's own source
+    iteration: the same iteration count, residual and mass histories and
+    moments, bit for bit (fp32: the source iteration with the switch to
+    binary64 sweeps at its own step)."""
     from pbte_amd.batch import run_source_batch
     cases = _cases(pkg, tmp_path, prec)
     for c in cases:
         c.iteration.tmax = 60
     print(f"case batch cis {prec}", flush=True)
-    alone = [pkg.Solver(c).run() for c in cases]
+    alone = [pkg.Solver(c)._run_source() for c in cases]
     recs = run_source_batch([pkg.Solver(c) for c in cases])
     for a, r in zip(alone, recs):
         assert r.iterations == a.iterations and r.converged == a.converged
@@ -91,7 +92,7 @@ def test_run_source_batch_is_each_solver_alone(pkg, tmp_path, prec):
 def test_run_source_batch_mixed_degrees_and_gsis(pkg, tmp_path):
     """Lockstep source iteration over problems of degrees 1 and 2 (one sweep
     launch per degree), two of them GSIS (``accflag: 1``), fp32 with the
-    binary64 finish: each problem's run alone, bit for bit."""
+    binary64 finish: each problem's source iteration alone, bit for bit."""
     from pbte_amd.batch import run_source_batch
     cases = _cases(pkg, tmp_path, "fp32")
     cases[1].dg.deg = 1
@@ -101,7 +102,7 @@ def test_run_source_batch_mixed_degrees_and_gsis(pkg, tmp_path):
     for c in cases:
         c.iteration.tmax = 40
     print("case batch mixed degrees and gsis", flush=True)
-    alone = [pkg.Solver(c).run() for c in cases]
+    alone = [pkg.Solver(c)._run_source() for c in cases]
     recs = run_source_batch([pkg.Solver(c) for c in cases])
     for a, r in zip(alone, recs):
         assert r.scheme == a.scheme and r.deg == a.deg

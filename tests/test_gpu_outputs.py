@@ -32,7 +32,7 @@ def test_write_outputs_after_a_run(pkg, tmp_path, accflag):
     f = np.load(tmp_path / "out" / "field.npz")
     px, py = sampling_grid()
     assert np.array_equal(f["px"], px) and np.array_equal(f["triid"], locate_points(s.mesh, px, py))
-    uq = s.acc.uq_d.cpu().numpy() if accflag else None
+    uq = s.acc.full_uq().cpu().numpy() if accflag else None
     cx, cy, w = s.vel.flat()
     ref = sample_fields(s.mesh, s.basis, s.vdf, cx, cy, w, c.flow.cv, px, py, f["triid"], uq=uq)
     for k, name in enumerate(["T", "qx", "qy", "Nxx", "Nxy", "Nyy"]):
