@@ -57,6 +57,13 @@ class PbteTraceDesc(C.Structure):
                 ("tri_faces", _i32p), ("cell_xy", _f64p), ("bsr_ptr", _i32p), ("bsr_col", _i32p)]
 
 
+class PbteFaceIntegralsDesc(C.Structure):
+    _fields_ = [("n_tris", C.c_int32), ("nd", C.c_int32), ("nf", C.c_int32), ("np_fc", C.c_int32),
+                ("nodes", _f64p), ("tri_nodes", _i32p), ("tri_faces", _i32p), ("face_nodes", _i32p),
+                ("face_len", _f64p), ("other", _i32p), ("off_x", _f64p), ("off_y", _f64p),
+                ("C", _f64p), ("Cf", _f64p), ("mono", _i32p), ("yq", _f64p), ("lpq", _f64p)]
+
+
 class PbteHdgDesc(C.Structure):
     _fields_ = [("n_tris", C.c_int32), ("nd", C.c_int32), ("nf", C.c_int32), ("n_faces", C.c_int32),
                 ("variant_b", C.c_int32), ("cv", C.c_double), ("tau_r", C.c_double),
@@ -141,6 +148,11 @@ _SIGS = {
     "pbte_trace_options": (C.c_int, [C.c_void_p, C.c_int32]),
     "pbte_trace_quality": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _f64p]),
     "pbte_trace_solve": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "pbte_gmsh_read": (C.c_void_p, [C.c_char_p, C.c_char_p, C.c_int32]),
+    "pbte_gmsh_sizes": (C.c_int, [C.c_void_p, _i64p]),
+    "pbte_gmsh_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbte_gmsh_free": (None, [C.c_void_p]),
+    "pbte_face_integrals": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -1,6 +1,6 @@
 """Build the native library in-tree: ``libpbte_hip.so`` (HIP kernels for
-gfx950 plus the host-side scheduler and trace-solve symbolic phase) next to
-this file.
+gfx950 plus the host-side scheduler, trace-solve symbolic phase, mesh reader
+and face integrals) next to this file.
 
 Each source compiles to an object of its own (in parallel, under
 ``_build/`` at the repository root), then one link.
@@ -20,7 +20,7 @@ ROOT = PKG.parent
 LIB = PKG / "libpbte_hip.so"
 OBJDIR = ROOT / "_build"
 SOURCES = [PKG / "csrc" / "pbte_kernels.hip", PKG / "csrc" / "trace_lu.hip", PKG / "csrc" / "hdg_setup.hip",
-           PKG / "csrc" / "schedule.cpp"]
+           PKG / "csrc" / "schedule.cpp", PKG / "csrc" / "host_setup.cpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

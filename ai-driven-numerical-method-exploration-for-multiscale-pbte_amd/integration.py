@@ -16,27 +16,31 @@ Two techniques, as in the reference (``integration.py:199-404``):
   same-element edge tensors;
 * a 15-point Gauss-Legendre rule along each face -- the element x face-basis
   tensor and the element x neighbour tensor (``int_tri_tri_fc``); on a wall
-  face the latter holds ``int phi_M ds`` broadcast over ``L``.
+  face the latter holds ``int phi_M ds`` broadcast over ``L``.  These face
+  loops run in ``pbte_face_integrals`` (``csrc/host_setup.cpp``) on host
+  threads, element by element in the reference's expression order.
 
 The contraction loops keep the reference's summation orders (they matter in
-the last bit); the per-element work is vectorised over elements in chunks,
-which changes no arithmetic.  Monomial powers use ``numpy.power`` (libm
-``pow`` for exponents >= 3, correctly rounded products below).
+the last bit).  Monomial powers x^e are 1, x and x*x for e <= 2 (as numpy's
+power gives them) and libm's ``pow`` for e >= 3; numpy's own power of an
+array is a vector routine on CPUs with AVX-512 that differs from libm's by
+one unit in the last place in a few per cent of cases, so for degrees 3 and
+4 the reference's own tensors depend on the CPU it ran on to that extent.
 """
 from __future__ import annotations
 
-import os
-from concurrent.futures import ThreadPoolExecutor
+import ctypes as C_
 from dataclasses import dataclass
 
 import numpy as np
 
+from . import _native
 from .basis import Basis
 from .constants import BC_PERIODIC
 from .mesh import Mesh
 from .quadrature import gauss_legendre_core, tri_quadrature
 
-__all__ = ["Integrals", "build_integrals", "fac_div"]
+__all__ = ["Integrals", "build_integrals", "face_partners", "fac_div"]
 
 
 def fac_div(ns: int, ne: int, ds: int, de: int) -> float:
@@ -129,9 +133,42 @@ class Integrals:
     cell_ops: np.ndarray | None = None
 
 
+def face_partners(mesh: Mesh, bc_codes=None, bc_xoff=None, bc_yoff=None):
+    """Per (element, local face) the element across -- the interior
+    neighbour, or for a periodic face the element behind the partner face --
+    and the shift of its nodes (the partner boundary's offset); ``-1`` and
+    zero on a wall (``integration.py:321-340``)."""
+    n_tris = mesh.n_tris
+    bc_codes = np.asarray(bc_codes) if bc_codes is not None else np.zeros(0, dtype=int)
+    nbc = bc_codes.size
+    bc_xoff = np.asarray(bc_xoff, dtype=float) if bc_xoff is not None else np.zeros(nbc)
+    bc_yoff = np.asarray(bc_yoff, dtype=float) if bc_yoff is not None else np.zeros(nbc)
+    adj_tri, _ = mesh.adjacent_arrays()
+    other = np.full((n_tris, 3), -1, dtype=np.int32)
+    off_x = np.zeros((n_tris, 3))
+    off_y = np.zeros((n_tris, 3))
+    for j2 in range(3):
+        fcid = mesh.tri_faces[:, j2].astype(np.int64)
+        fbc = mesh.face_bc[fcid].astype(np.int64)
+        per = np.zeros(n_tris, dtype=bool)
+        if bc_codes.size:
+            per = (fbc >= 0) & (bc_codes[np.clip(fbc, 0, None)] == BC_PERIODIC)
+        interior = fbc < 0
+        t1 = mesh.face_tri[fcid, 0]
+        t2 = mesh.face_tri[fcid, 1]
+        o = np.where(t1 == np.arange(n_tris), t2, t1).astype(np.int64)
+        if per.any():
+            pair = mesh.face_pair[fcid[per]].astype(np.int64)
+            o[per] = adj_tri[pair]
+            bcp = mesh.face_bc[pair].astype(np.int64)
+            off_x[per, j2] = bc_xoff[bcp]
+            off_y[per, j2] = bc_yoff[bcp]
+        other[:, j2] = np.where(interior | per, o, -1)
+    return other, off_x, off_y
+
+
 def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
-                    bc_codes=None, bc_xoff=None, bc_yoff=None,
-                    chunk: int = 2048) -> Integrals:
+                    bc_codes=None, bc_xoff=None, bc_yoff=None, threads: int = 0) -> Integrals:
     P = basis.ndof_tri
     Q = basis.ndof_fc
     deg = basis.deg
@@ -208,106 +245,31 @@ def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
     int_fc_fc = ref_fc_fc[:, :, None] * half_len[None, None, :]
     int_fc = ref_fc[:, None] * half_len[None, :]
 
-    # -- face rules ---------------------------------------------------------
+    # -- face integrals (pbte_face_integrals, host threads) ------------------
     yq, lpq = gauss_legendre_core(np_fc)
-    n1, n2 = np_fc, np_fc + 1
-
-    def face_rule(length):
-        absc = (0.0 * (1.0 - yq) + length[:, None] * (1.0 + yq)) / 2.0
-        weig = n2 * n2 * (length[:, None] - 0.0) / ((1.0 - yq * yq) * lpq * lpq) / n1 / n1
-        return absc, weig
-
-    bc_codes = np.asarray(bc_codes) if bc_codes is not None else np.zeros(0, dtype=int)
-    bc_xoff = np.asarray(bc_xoff, dtype=float) if bc_xoff is not None else np.zeros(0)
-    bc_yoff = np.asarray(bc_yoff, dtype=float) if bc_yoff is not None else np.zeros(0)
-
-    # the element across each face: interior neighbour, or for a periodic
-    # face the element behind the partner face shifted by the partner's offset
-    adj_tri, _ = mesh.adjacent_arrays()
-    int_tri_fc_fc = np.zeros((P, Q, n_tris, 3))
-    int_tri_tri_fc = np.zeros((n_tris, 3, P, P))
-
-    def face_slot(j2):
-        fcid = mesh.tri_faces[:, j2].astype(np.int64)
-        fbc = mesh.face_bc[fcid].astype(np.int64)
-        per = np.zeros(n_tris, dtype=bool)
-        if bc_codes.size:
-            per = (fbc >= 0) & (bc_codes[np.clip(fbc, 0, None)] == BC_PERIODIC)
-        interior = fbc < 0
-        t1 = mesh.face_tri[fcid, 0]
-        t2 = mesh.face_tri[fcid, 1]
-        other = np.where(t1 == np.arange(n_tris), t2, t1).astype(np.int64)
-        ox = np.zeros(n_tris)
-        oy = np.zeros(n_tris)
-        if per.any():
-            pair = mesh.face_pair[fcid[per]].astype(np.int64)
-            other[per] = adj_tri[pair]
-            bcp = mesh.face_bc[pair].astype(np.int64)
-            ox[per] = bc_xoff[bcp]
-            oy[per] = bc_yoff[bcp]
-        return fcid, per, interior, other, ox, oy
-
-    slots = [face_slot(j2) for j2 in range(3)]
-
-    def face_chunk(j2, sl):
-        # element-wise arithmetic only: the result of a chunk does not
-        # depend on how the cells are split into chunks or threads
-        fcid, per, interior, other, ox, oy = slots[j2]
-        f = fcid[sl]
-        ln = mesh.face_len[f]
-        absc, weig = face_rule(ln)
-        fx1 = nodes[mesh.face_nodes[f, 0], 0][:, None]
-        fy1 = nodes[mesh.face_nodes[f, 0], 1][:, None]
-        fx2 = nodes[mesh.face_nodes[f, 1], 0][:, None]
-        fy2 = nodes[mesh.face_nodes[f, 1], 1][:, None]
-        lnc = ln[:, None]
-        xp = (fx2 - fx1) / lnc * absc + fx1
-        yp = (fy2 - fy1) / lnc * absc + fy1
-        (bA, cA, dA), (fE, gE, hE) = _affine(x[sl], y[sl])
-        xi = bA[:, None] * xp + cA[:, None] * yp + dA[:, None]
-        eta = fE[:, None] * xp + gE[:, None] * yp + hE[:, None]
-        pm = _eval_mono(C, mono, xi, eta)                  # (c, q, P)
-        t = 2.0 * absc / lnc - 1.0
-        pl = np.zeros(t.shape + (Q,))
-        for k in range(Q):
-            pl += Cf[:, k] * _pow(t, k)[..., None]
-        acc = np.zeros((pm.shape[0], P, Q))
-        for q in range(np_fc):
-            acc += (pm[:, q, :, None] * pl[:, q, None, :]) * weig[:, q, None, None]
-        int_tri_fc_fc[:, :, sl, j2] = np.transpose(acc, (1, 2, 0))
-
-        # element x neighbour (or wall) tensor
-        two = (interior | per)[sl]
-        acc2 = np.zeros((pm.shape[0], P, P))
-        if two.any():
-            o = other[sl]
-            xo = nodes[mesh.tri_nodes[o], 0] + ox[sl][:, None]
-            yo = nodes[mesh.tri_nodes[o], 1] + oy[sl][:, None]
-            (bB, cB, dB), (fF, gF, hF) = _affine(xo, yo)
-            xi2 = bB[:, None] * xp + cB[:, None] * yp + dB[:, None]
-            eta2 = fF[:, None] * xp + gF[:, None] * yp + hF[:, None]
-            pn = _eval_mono(C, mono, xi2, eta2)
-            for q in range(np_fc):
-                acc2 += (pm[:, q, :, None] * pn[:, q, None, :]) * weig[:, q, None, None]
-        if (~two).any():
-            accw = np.zeros((pm.shape[0], P))
-            for q in range(np_fc):
-                accw += pm[:, q, :] * weig[:, q, None]
-            acc2 = np.where(two[:, None, None], acc2, accw[:, :, None])
-        int_tri_tri_fc[sl, j2] = acc2
-
-    # numpy releases the GIL inside its array loops: chunks of cells run on
-    # a thread pool, each writing its own slice
-    tasks = [(j2, slice(s0, min(n_tris, s0 + chunk))) for j2 in range(3)
-             for s0 in range(0, n_tris, chunk)]
-    workers = max(1, min(len(tasks), os.cpu_count() or 1, 16))
-    if workers == 1:
-        for j2, sl in tasks:
-            face_chunk(j2, sl)
-    else:
-        with ThreadPoolExecutor(max_workers=workers) as pool:
-            for fut in [pool.submit(face_chunk, j2, sl) for j2, sl in tasks]:
-                fut.result()
+    other, off_x, off_y = face_partners(mesh, bc_codes, bc_xoff, bc_yoff)
+    int_tri_fc_fc = np.empty((P, Q, n_tris, 3))
+    int_tri_tri_fc = np.empty((n_tris, 3, P, P))
+    keep = [np.ascontiguousarray(nodes, dtype=np.float64),
+            np.ascontiguousarray(mesh.tri_nodes, dtype=np.int32),
+            np.ascontiguousarray(mesh.tri_faces, dtype=np.int32),
+            np.ascontiguousarray(mesh.face_nodes, dtype=np.int32),
+            np.ascontiguousarray(mesh.face_len, dtype=np.float64),
+            other, off_x, off_y,
+            np.ascontiguousarray(C, dtype=np.float64), np.ascontiguousarray(Cf, dtype=np.float64),
+            np.ascontiguousarray(mono, dtype=np.int32),
+            np.ascontiguousarray(yq, dtype=np.float64), np.ascontiguousarray(lpq, dtype=np.float64)]
+    d = _native.PbteFaceIntegralsDesc()
+    d.n_tris, d.nd, d.nf, d.np_fc = n_tris, P, Q, int(np_fc)
+    d.nodes, d.tri_nodes, d.tri_faces, d.face_nodes, d.face_len = (
+        _native.f64(keep[0]), _native.i32(keep[1]), _native.i32(keep[2]), _native.i32(keep[3]),
+        _native.f64(keep[4]))
+    d.other, d.off_x, d.off_y = _native.i32(keep[5]), _native.f64(keep[6]), _native.f64(keep[7])
+    d.C, d.Cf, d.mono = _native.f64(keep[8]), _native.f64(keep[9]), _native.i32(keep[10])
+    d.yq, d.lpq = _native.f64(keep[11]), _native.f64(keep[12])
+    if _native.lib().pbte_face_integrals(C_.byref(d), int_tri_fc_fc.ctypes.data, int_tri_tri_fc.ctypes.data,
+                                         threads) != 0:
+        raise ValueError("pbte_face_integrals: bad arguments")
 
     qax, qay, qw = tri_quadrature(np_tri)
     nodfun_qua_p = _eval_mono(C, mono, qax, qay).T

@@ -22,6 +22,7 @@ reference has no mesher for it.
 """
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 from pathlib import Path
 
@@ -41,76 +42,43 @@ class GmshMesh:
     nodes: np.ndarray
     elem_type: np.ndarray
     elem_phyid: np.ndarray
-    elem_nodes: list
+    elem_nodes: np.ndarray      # [n_elem, 3], -1 after a line's two nodes
     physical_names: dict
 
 
-def _sections(text: str) -> dict:
-    """The ``$Name ... $EndName`` blocks of a gmsh file, as their stripped
-    lines (the first occurrence of each name)."""
-    out, name, body = {}, None, None
-    for raw in text.splitlines():
-        line = raw.strip()
-        if name is not None and line == f"$End{name}":
-            out.setdefault(name, body)
-            name = None
-        elif name is None and line.startswith("$") and not line.startswith("$End"):
-            name, body = line[1:], []
-        elif name is not None:
-            body.append(line)
-    return out
-
-
-def _counted(path, sec, name):
-    """The item lines of a section whose first line is their count."""
-    if not sec:
-        raise ValueError(f"{path}: ${name} section is empty")
-    count = int(sec[0])
-    if len(sec) - 1 < count:
-        raise ValueError(f"{path}: ${name} announces {count} entries and holds {len(sec) - 1}")
-    return sec[1:1 + count]
-
-
 def read_gmsh22(path) -> GmshMesh:
-    """A gmsh 2.2 ASCII file (``mesh/gmsh_reader.py:45-122``): nodes (ids
-    must run 1..N in order: the node index is the position in the file),
-    line and triangle elements (at least two tags; the first is the
-    physical id), physical names.  Any other element type is an error."""
-    path = Path(path)
-    secs = _sections(path.read_text())
-    fmt = secs.get("MeshFormat")
-    if fmt:
-        version, ftype = fmt[0].split()[:2]
-        if not version.startswith("2.2"):
-            raise ValueError(f"{path}: gmsh format {version}, this reader takes 2.2")
-        if ftype != "0":
-            raise ValueError(f"{path}: binary gmsh file; this reader takes ASCII")
-    names = {}
-    for line in _counted(path, secs["PhysicalNames"], "PhysicalNames") if "PhysicalNames" in secs else ():
+    """A gmsh 2.2 ASCII file (``mesh/gmsh_reader.py:45-122``), parsed by
+    ``pbte_gmsh_read``: nodes (ids must run 1..N in order: the node index is
+    the position in the file), line and triangle elements (at least two
+    tags; the first is the physical id), physical names.  Any other element
+    type, a format other than 2.2 ASCII or a short section is a
+    ``ValueError``.  ``elem_nodes`` is ``[n_elem, 3]`` (0-based; -1 after a
+    line's two nodes)."""
+    from . import _native
+    lib = _native.lib()
+    err = C.create_string_buffer(1024)
+    h = lib.pbte_gmsh_read(str(path).encode(), err, 1024)
+    if not h:
+        raise ValueError(err.value.decode())
+    try:
+        n = (C.c_int64 * 3)()
+        lib.pbte_gmsh_sizes(h, n)
+        nn, ne, nb = int(n[0]), int(n[1]), int(n[2])
+        xy = np.empty((nn, 2))
+        kind = np.empty(ne, dtype=np.int32)
+        phy = np.empty(ne, dtype=np.int32)
+        conn = np.empty((ne, 3), dtype=np.int32)
+        names = C.create_string_buffer(max(nb, 1))
+        lib.pbte_gmsh_copy(h, xy.ctypes.data, kind.ctypes.data, phy.ctypes.data, conn.ctypes.data,
+                           C.cast(names, C.c_void_p))
+    finally:
+        lib.pbte_gmsh_free(h)
+    table = {}
+    for line in names.raw[:nb].decode().splitlines():
         dim, tag, label = line.split(None, 2)
-        names[(int(dim), int(tag))] = label.strip().strip('"')
-    if "Nodes" not in secs:
-        raise ValueError(f"{path}: no $Nodes section")
-    rows = _counted(path, secs["Nodes"], "Nodes")
-    table = np.array([r.split()[:3] for r in rows], dtype=np.float64).reshape(len(rows), 3)
-    if not np.array_equal(table[:, 0], np.arange(1, len(rows) + 1)):
-        raise ValueError(f"{path}: node ids do not run 1..{len(rows)} in file order")
-    kinds, phys, conn = [], [], []
-    for line in _counted(path, secs.get("Elements", ["0"]), "Elements"):
-        tok = [int(v) for v in line.split()]
-        eid, kind, ntags = tok[0], tok[1], tok[2]
-        width = {LINE_ELEMENT: 2, TRIANGLE_ELEMENT: 3}.get(kind)
-        if width is None:
-            raise ValueError(f"{path}: element {eid} is of type {kind}; only lines (1) and "
-                             "triangles (2) are read")
-        if ntags < 2:
-            raise ValueError(f"{path}: element {eid} carries {ntags} tags; the physical and "
-                             "geometrical ones are both needed")
-        kinds.append(kind)
-        phys.append(tok[3])
-        conn.append(tuple(v - 1 for v in tok[3 + ntags:3 + ntags + width]))
-    return GmshMesh(nodes=np.ascontiguousarray(table[:, 1:3]), elem_type=np.asarray(kinds, dtype=np.int64),
-                    elem_phyid=np.asarray(phys, dtype=np.int64), elem_nodes=conn, physical_names=names)
+        table[(int(dim), int(tag))] = label.strip().strip('"')
+    return GmshMesh(nodes=xy, elem_type=kind.astype(np.int64), elem_phyid=phy.astype(np.int64),
+                    elem_nodes=conn, physical_names=table)
 
 
 def triangle_geometry(nodes: np.ndarray, tri_nodes: np.ndarray):
@@ -206,9 +174,8 @@ def build_mesh(path, bc_phyid, bc_codes, bc_names=None, bc_xoff=None, bc_yoff=No
         raise ValueError(f"{path}: no triangles")
     line_idx = np.flatnonzero(is_line)
     tri_idx = np.flatnonzero(~is_line)
-    tri_nodes_a = np.asarray([g.elem_nodes[e] for e in tri_idx], dtype=np.int64).reshape(-1, 3)
-    lines_a = (np.asarray([g.elem_nodes[e] for e in line_idx], dtype=np.int64).reshape(-1, 2)
-               if line_idx.size else np.zeros((0, 2), dtype=np.int64))
+    tri_nodes_a = g.elem_nodes[tri_idx].astype(np.int64).reshape(-1, 3)
+    lines_a = g.elem_nodes[line_idx, :2].astype(np.int64).reshape(-1, 2)
     # occurrence stream in element order
     pos_line = line_idx
     pos_tri = tri_idx

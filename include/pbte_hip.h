@@ -358,6 +358,47 @@ int pbte_gsis_correct_moments(pbte_ctx *ctx, const double *mom7, const double *u
 int pbte_gsis_apply_correction(pbte_ctx *ctx, int32_t precision, void *vdf, const double *coef,
                                const int32_t *cells, int32_t ncells);
 
+/* ---- host set-up: mesh file and face integrals ---------------------------
+ * Host-only (no device memory).  A gmsh 2.2 ASCII file is read into a
+ * handle; pbte_gmsh_sizes gives (nodes, elements, bytes of the physical-name
+ * lines) and pbte_gmsh_copy copies out the node coordinates [n][2], element
+ * type (1 line, 2 triangle), first tag (physical id), nodes [n][3] (0-based,
+ * -1 after a line's two) and the $PhysicalNames item lines ('\n'-ended).
+ * Errors (a format other than 2.2 ASCII, node ids not 1..n in order, an
+ * element of another type or with fewer than two tags, a short section)
+ * return NULL with the message in err.  Replaces mesh/gmsh_reader.py:45-122. */
+typedef struct pbte_gmsh pbte_gmsh;
+pbte_gmsh *pbte_gmsh_read(const char *path, char *err, int32_t errlen);
+int pbte_gmsh_sizes(const pbte_gmsh *g, int64_t *out3);
+int pbte_gmsh_copy(const pbte_gmsh *g, double *xy, int32_t *type, int32_t *phyid, int32_t *conn,
+                   char *names);
+void pbte_gmsh_free(pbte_gmsh *g);
+
+/* The face integrals of the nodal basis on every (element, local face):
+ * int_tri_fc_fc [P][Q][N][3] (element basis x face basis) and
+ * int_tri_tri_fc [N][3][P][P] (element basis x the basis of the element
+ * across -- interior neighbour, or periodic partner's element shifted by
+ * off -- or, where other < 0, int phi_m ds in every column), by the
+ * np_fc-point Gauss-Legendre rule (abscissae yq, Legendre derivative lpq).
+ * threads <= 0: up to 16 host threads.  Replaces the face loops of
+ * integration.py:321-404. */
+typedef struct pbte_face_integrals_desc {
+    int32_t n_tris, nd, nf, np_fc;
+    const double *nodes;        /* [n_nodes][2]                              */
+    const int32_t *tri_nodes;   /* [N][3]                                    */
+    const int32_t *tri_faces;   /* [N][3]                                    */
+    const int32_t *face_nodes;  /* [F][2]                                    */
+    const double *face_len;     /* [F]                                       */
+    const int32_t *other;       /* [N][3] element across, -1 on a wall       */
+    const double *off_x, *off_y;/* [N][3] shift of that element's nodes      */
+    const double *C;            /* [P][P] nodal coefficients of the monomials */
+    const double *Cf;           /* [Q][Q] face basis coefficients             */
+    const int32_t *mono;        /* [P][2] monomial exponents (a, b)          */
+    const double *yq, *lpq;     /* [np_fc]                                   */
+} pbte_face_integrals_desc;
+int pbte_face_integrals(const pbte_face_integrals_desc *d, double *int_tri_fc_fc,
+                        double *int_tri_tri_fc, int32_t threads);
+
 /* ---- HDG operators built on the device ---------------------------------
  * The element operators and the trace matrix of the HDG synthetic system.
  * Replace acceleration/assembly.py:44-268 (build_local_operators,

@@ -25,8 +25,8 @@ def test_hole_mesh_topology(pkg, tmp_path):
     g = read_gmsh22(path)
     et = np.asarray(g.elem_type)
     phy = np.asarray(g.elem_phyid)
-    tris = np.asarray([g.elem_nodes[k] for k in np.flatnonzero(et == 2)])
-    hole = np.asarray([g.elem_nodes[k] for k in np.flatnonzero(phy == 16)])
+    tris = g.elem_nodes[np.flatnonzero(et == 2)]
+    hole = g.elem_nodes[np.flatnonzero(phy == 16), :2]
     assert len(hole) > 0 and g.physical_names[(1, 16)].startswith("Hole")
     edges = {}
     for t in tris:
