@@ -47,13 +47,28 @@ class SweepBatch:
     def __del__(self):
         self.close()
 
+    def pipelined(self, prec) -> bool:
+        """Whether every context runs the pipelined kernel in ``prec``
+        (``PBTE_FP64`` / ``PBTE_FP32``), so one launch sweeps them all."""
+        st = _native.lib().pbte_batch_pipelined(self.handle, prec)
+        if st < 0:
+            raise _native.NativeError("batch_pipelined: null batch")
+        return st == 1
+
     def sweep(self, moms, vdfs, homogeneous: bool = False) -> None:
         """One sweep of every context: ``vdfs[k]`` in place from ``moms[k]``
-        (all distributions of one precision), ``sweep_count`` += 1 each."""
+        (all distributions of one precision), ``sweep_count`` += 1 each.
+        Contexts that do not run the pipelined kernel in this precision
+        (factors built on the fly, the plain level kernel) are swept one by
+        one, each as ``SweepContext.sweep`` does."""
         n = len(self.contexts)
         prec = _PREC[vdfs[0].dtype]
         if any(_PREC[v.dtype] != prec for v in vdfs):
             raise ValueError("a batched sweep runs one precision")
+        if not self.pipelined(prec):
+            for c, m, v in zip(self.contexts, moms, vdfs):
+                c.sweep(m, v, homogeneous=homogeneous)
+            return
 
         def arr(vals):
             return (C.c_void_p * n)(*vals)

@@ -661,9 +661,9 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
     const T thermal_on = a.homogeneous ? T(0) : T(1);
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
 
+    const int cstep = (int)blockDim.x / B;
     for (int lv = l0; lv < l1; ++lv) {
-        const int s = a.lev_ptr[lv] + c;
-        if (s < a.lev_ptr[lv + 1]) {
+        for (int s = a.lev_ptr[lv] + c; s < a.lev_ptr[lv + 1]; s += cstep) {
             const int4 r = irec[s];
             const int i = rec_cell(r.x);
             const int fw[3] = {r.y, r.z, r.w};
@@ -792,9 +792,9 @@ __global__ void __launch_bounds__(256) sweep_refine_kernel(SweepArgs<double> a)
     for (int k = t; k < 6 * P * P; k += blockDim.x) ref[k] = a.refop[k];
     __syncthreads();
 
+    const int cstep = (int)blockDim.x / B;
     for (int lv = l0; lv < l1; ++lv) {
-        const int s = a.lev_ptr[lv] + c;
-        if (s < a.lev_ptr[lv + 1]) {
+        for (int s = a.lev_ptr[lv] + c; s < a.lev_ptr[lv + 1]; s += cstep) {
             T rhs[P];
             const int i = otf_rhs<P, T>(a, irec, wc, s, B, b, d, cx, cy, thermal_on, vdf_d, rhs);
             const size_t item = (size_t)s * B + b;
@@ -875,9 +875,9 @@ __global__ void __launch_bounds__(256) sweep_otf_kernel(SweepArgs<T> a)
     const T thermal_on = a.homogeneous ? T(0) : T(1);
     const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
 
+    const int cstep = (int)blockDim.x / B;
     for (int lv = l0; lv < l1; ++lv) {
-        const int s = a.lev_ptr[lv] + c;
-        if (s < a.lev_ptr[lv + 1]) {
+        for (int s = a.lev_ptr[lv] + c; s < a.lev_ptr[lv + 1]; s += cstep) {
             T x[P];
             const int i = otf_rhs<P, T>(a, irec, wc, s, B, b, d, cx, cy, thermal_on, vdf_d, x);
             double A[P][P];
@@ -1197,6 +1197,11 @@ __device__ __forceinline__ void level_barrier(bool global_mode)
     }
 }
 
+// threads of a pipelined workgroup at most (one item each per level)
+template <typename T> struct PipeThreads {
+    static constexpr int max = 256;
+};
+
 // BF: 1 when every bundle holds one direction (the default schedule; the
 // slot and LDS arithmetic then drops its bundle factors), 0 for bundles
 template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
@@ -1327,7 +1332,7 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
 }
 
 template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
-__global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
+__global__ void __launch_bounds__(PipeThreads<T>::max) sweep_pipe_kernel(SweepArgs<T> a)
 {
     sweep_pipe_body<P, T, PFLU, GREAD, DEPTH, BF>(a, blockIdx.x);
 }
@@ -1335,7 +1340,7 @@ __global__ void __launch_bounds__(256) sweep_pipe_kernel(SweepArgs<T> a)
 // several contexts' sweeps in one launch: workgroup k sweeps block
 // bmap[k].y of context bmap[k].x, whose arguments it reads from args
 template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
-__global__ void __launch_bounds__(256) sweep_pipe_batch_kernel(const SweepArgs<T>* __restrict__ args,
+__global__ void __launch_bounds__(PipeThreads<T>::max) sweep_pipe_batch_kernel(const SweepArgs<T>* __restrict__ args,
                                                                const int2* __restrict__ bmap)
 {
     const int2 m = bmap[blockIdx.x];
@@ -2513,8 +2518,13 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->stream = (hipStream_t)stream;
         ctx->cv = su->cv; ctx->vg = su->vg; ctx->tau_r = su->tau_r; ctx->tau_n = su->tau_n;
         ctx->tau_c = su->tau_c; ctx->pi = su->pi;
+        // items per workgroup and level: at most 256 (a binary32 pipelined
+        // workgroup of 512 items, which its registers allow, took 1.85x
+        // longer per level for 1.5x the items on the 256^2 P2 case)
+        const int max_items = 256;
         ctx->block = su->block > 0 ? su->block : 256;
-        if (ctx->block % 64 || ctx->block > 256) throw std::string("block must be a multiple of 64, <= 256");
+        if (ctx->block % 64 || ctx->block > max_items)
+            throw std::string("block must be a multiple of 64, <= ") + std::to_string(max_items);
         ctx->variant = su->sweep_variant;
         if (ctx->variant < 0 || ctx->variant > 3) throw std::string("sweep_variant must be 0-3");
         ctx->items = ctx->block;
@@ -2974,7 +2984,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 for (size_t l = 0; l + 1 < lcnt[k].size(); ++l)
                     widest = std::max<int64_t>(widest,
                                                (int64_t)(lcnt[k][l + 1] - lcnt[k][l]) * bundles[rep[k]].size());
-            ctx->block = ctx->items = (int)std::min<int64_t>(256, (widest + 63) / 64 * 64);
+            ctx->block = ctx->items = (int)std::min<int64_t>(max_items, (widest + 63) / 64 * 64);
             for (auto& bb : bundles)
                 if (ctx->items % (int)bb.size()) ctx->block = ctx->items = 256;
         }
@@ -3351,7 +3361,8 @@ template <typename T> bool factors_fit_descriptor(const pbte_ctx* ctx)
 
 template <int P, typename T> bool runs_pipelined(const pbte_ctx* ctx)
 {
-    return ctx->variant != 1 && sweep_lds_bytes<P, T>(ctx) <= ctx->lds_blk && factors_fit_descriptor<T>(ctx);
+    return ctx->variant != 1 && ctx->block <= PipeThreads<T>::max && sweep_lds_bytes<P, T>(ctx) <= ctx->lds_blk &&
+           factors_fit_descriptor<T>(ctx);
 }
 
 template <int P, typename T>
@@ -3366,17 +3377,17 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
             // binary32 inverses stored (the "mixed" mode): refine from them
             // instead of building and factorising every operator
             if (ctx->have32 && ctx->refop) {
-                hipLaunchKernelGGL((sweep_refine_kernel<P>), dim3(ctx->nb), dim3(ctx->block),
+                hipLaunchKernelGGL((sweep_refine_kernel<P>), dim3(ctx->nb), dim3(std::min(ctx->block, 256)),
                                    6 * P * P * sizeof(double), ctx->stream, a);
                 refined = true;
             }
         }
         if (!refined)
-            hipLaunchKernelGGL((sweep_otf_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
+            hipLaunchKernelGGL((sweep_otf_kernel<P, T>), dim3(ctx->nb), dim3(std::min(ctx->block, 256)), 0, ctx->stream, a);
     } else if (!runs_pipelined<P, T>(ctx)) {
         // the plain level kernel: asked for, or a binary64 sweep whose pool,
         // sized for binary32, exceeds a workgroup's LDS
-        hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(ctx->block), 0, ctx->stream, a);
+        hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(std::min(ctx->block, 256)), 0, ctx->stream, a);
     } else {
         const size_t lds = sweep_lds_bytes<P, T>(ctx);
         constexpr bool pf = P * P * sizeof(T) <= 288;
@@ -3596,6 +3607,22 @@ extern "C" pbte_batch* pbte_batch_create(pbte_ctx* const* ctxs, int32_t n, char*
 }
 
 extern "C" void pbte_batch_destroy(pbte_batch* bt) { delete bt; }
+
+extern "C" int pbte_batch_pipelined(pbte_batch* bt, int32_t precision)
+{
+    if (!bt) return -1;
+    for (pbte_ctx* c : bt->ctxs) {
+        bool ok = false;
+        switch (bt->P) {
+#define BP(PP) case PP: ok = precision == PBTE_FP64 ? (c->have64 && runs_pipelined<PP, double>(c)) \
+                                                    : (c->have32 && runs_pipelined<PP, float>(c)); break;
+            BP(3) BP(6) BP(10) BP(15)
+#undef BP
+        }
+        if (!ok) return 0;
+    }
+    return 1;
+}
 
 extern "C" const char* pbte_batch_last_error(pbte_batch* bt) { return bt ? bt->err.c_str() : "null batch"; }
 

@@ -222,6 +222,10 @@ const char *pbte_batch_last_error(pbte_batch *batch);
 int pbte_batch_sweep(pbte_batch *batch, int32_t precision, const double *const *ts,
                      const double *const *qxs, const double *const *qys, void *const *vdf,
                      const double *const *flux_wall, int32_t homogeneous);
+/* 1 when every context of the batch runs the pipelined kernel in this
+ * precision (pbte_batch_sweep can sweep them together), else 0: the caller
+ * then sweeps those contexts one by one with pbte_sweep. */
+int pbte_batch_pipelined(pbte_batch *batch, int32_t precision);
 
 /* ---- device vector kernels of the Krylov solver (krylov.py:150-246) --- */
 

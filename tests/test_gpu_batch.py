@@ -145,3 +145,30 @@ def test_run_source_batch_matches_oracle(pkg, tmp_path):
         np.testing.assert_allclose(r.mass_history, mass, rtol=0, atol=1e-12)
         err = np.abs(r.temp_dofs - orc.ts).max() / np.abs(orc.ts).max()
         assert err <= 1e-11, err
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_falls_back_for_contexts_off_the_pipelined_kernel(pkg, tmp_path, prec):
+    """Contexts whose sweeps do not run the pipelined kernel -- factors built
+    on the fly (``precompute_inverse: false``) or the plain level kernel
+    (``sweep_variant=1``) -- are swept one by one inside the batch: every
+    problem still gets the iterates of its own source iteration, bit for
+    bit."""
+    from pbte_amd.batch import run_source_batch
+    cases = _cases(pkg, tmp_path, prec)[:3]
+    cases[0].performance.precompute_inverse = False
+    for c in cases:
+        c.iteration.tmax = 30
+    print(f"case batch fallback {prec}", flush=True)
+
+    def solvers():
+        return [pkg.Solver(cases[0]), pkg.Solver(cases[1], sweep_variant=1), pkg.Solver(cases[2])]
+
+    alone = [s._run_source() for s in solvers()]
+    batch = solvers()
+    assert batch[0].ctx.mode == "onthefly"
+    recs = run_source_batch(batch)
+    for a, r in zip(alone, recs):
+        assert r.iterations == a.iterations and r.converged == a.converged
+        np.testing.assert_array_equal(r.residual_history, a.residual_history)
+        np.testing.assert_array_equal(r.temp_dofs, a.temp_dofs)
