@@ -148,6 +148,7 @@ _SIGS = {
     "pbte_trace_options": (C.c_int, [C.c_void_p, C.c_int32]),
     "pbte_trace_quality": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _f64p]),
     "pbte_trace_solve": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "pbte_trace_level": (C.c_int, [C.c_void_p, C.c_int32, _i64p]),
     "pbte_gmsh_read": (C.c_void_p, [C.c_char_p, C.c_char_p, C.c_int32]),
     "pbte_batch_pipelined": (C.c_int, [C.c_void_p, C.c_int32]),
     "pbte_gmsh_sizes": (C.c_int, [C.c_void_p, _i64p]),

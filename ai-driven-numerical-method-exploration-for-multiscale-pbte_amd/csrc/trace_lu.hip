@@ -50,6 +50,11 @@ namespace {
     } while (0)
 
 constexpr int KB = 32;        // pivot columns per sweep step
+
+// stored factor rows are padded to a multiple of 4 values (zeros), so that
+// every row starts 16-byte aligned in binary32 and binary64 and the solve
+// reads it in 16-byte words
+__host__ __device__ constexpr int row_ld(int n) { return (n + 3) & ~3; }
 constexpr int TILE = 128;     // update tile (rows and columns)
 
 struct Front {                // one tree node, device and host
@@ -63,6 +68,7 @@ struct Front {                // one tree node, device and host
     int32_t cvec;             // offset of the ncb forward contributions
     int32_t child[2];         // node ids, -1 when absent
     int32_t cmap[2];          // offsets of the child face -> front position maps
+    int32_t gsrc;             // offset of the n per-row child sources of the forward gather
 };
 
 struct Level {
@@ -73,6 +79,7 @@ struct Level {
     int32_t rec_off = 0, rec_n = 0;   // original-block records
     int32_t fw_off = 0, fw_n = 0;     // forward work items (front, row0)
     int32_t bw_off = 0, bw_n = 0;     // backward work items
+    int32_t gi_off = 0, gi_n = 0;     // forward gather items (front, row0), 256 rows each
     int32_t lpr_fw = 64, lpr_bw = 64; // lanes per row of the solve's products
     bool has_children = false;
 };
@@ -503,13 +510,14 @@ __global__ void k_extract(const int32_t* __restrict__ nodes, const Front* __rest
 {
     const Front f = fr[nodes[blockIdx.x]];
     const double* M = ws + f.ws;
+    const int ldz = row_ld(f.n), ldw = row_ld(f.nfs);
     for (int i = blockIdx.y * 4; i < min(blockIdx.y * 4 + 4, f.n); ++i) {
         if (i < f.nfs) {
-            for (int j = threadIdx.x; j < f.n; j += blockDim.x)
-                fac[f.zv + (int64_t)i * f.n + j] = M[(int64_t)i * f.n + j];
+            for (int j = threadIdx.x; j < ldz; j += blockDim.x)
+                fac[f.zv + (int64_t)i * ldz + j] = j < f.n ? M[(int64_t)i * f.n + j] : 0.0;
         } else {
-            for (int j = threadIdx.x; j < f.nfs; j += blockDim.x)
-                fac[f.w + (int64_t)(i - f.nfs) * f.nfs + j] = -M[(int64_t)i * f.n + j];
+            for (int j = threadIdx.x; j < ldw; j += blockDim.x)
+                fac[f.w + (int64_t)(i - f.nfs) * ldw + j] = j < f.nfs ? -M[(int64_t)i * f.n + j] : 0.0;
         }
     }
 }
@@ -518,29 +526,31 @@ __global__ void k_extract(const int32_t* __restrict__ nodes, const Front* __rest
 // device kernels: solve
 // ---------------------------------------------------------------------------
 
-// forward assembly of one front: b1 from the right-hand side plus the
-// children's contributions, y = P b1, c = (contributions on the ancestors'
-// faces); one workgroup per front, the front vector in LDS
-__global__ void k_fwd_gather(const int32_t* __restrict__ nodes, const Front* __restrict__ fr,
-                             const int32_t* __restrict__ faces, const int32_t* __restrict__ fmap,
-                             const int32_t* __restrict__ perm, int b, const double* __restrict__ rhs,
-                             double* __restrict__ y, double* __restrict__ cv)
+// forward assembly of the fronts of a level, one thread per front row k:
+// the row j it takes (j = perm[k] for the fully summed rows, whose values
+// go to y in pivot order; j = k for the contribution rows, which start the
+// front's contributions c) is b(j) from the right-hand side (0 for an
+// ancestor's row), plus the first child's contribution to it, plus the
+// second's
+__global__ void __launch_bounds__(256) k_fwd_gather(const int2* __restrict__ items, const Front* __restrict__ fr,
+                                                    const int32_t* __restrict__ faces, const int2* __restrict__ gsrc,
+                                                    const int32_t* __restrict__ perm, int b,
+                                                    const double* __restrict__ rhs, double* __restrict__ y,
+                                                    double* __restrict__ cv)
 {
-    extern __shared__ double bf[];
-    const Front f = fr[nodes[blockIdx.x]];
-    for (int i = threadIdx.x; i < f.n; i += blockDim.x)
-        bf[i] = i < f.nfs ? rhs[(int64_t)faces[f.faces + i / b] * b + i % b] : 0.0;
-    __syncthreads();
-    for (int s = 0; s < 2; ++s) {
-        const int c = f.child[s];
-        if (c < 0) continue;
-        const Front ch = fr[c];
-        const int32_t* map = fmap + f.cmap[s];
-        for (int j = threadIdx.x; j < ch.ncb; j += blockDim.x) bf[map[j / b] * b + j % b] += cv[ch.cvec + j];
-        __syncthreads();
-    }
-    for (int k = threadIdx.x; k < f.nfs; k += blockDim.x) y[f.piv + k] = bf[perm[f.piv + k]];
-    for (int i = threadIdx.x; i < f.ncb; i += blockDim.x) cv[f.cvec + i] = bf[f.nfs + i];
+    const int2 it = items[blockIdx.x];
+    const Front f = fr[it.x];
+    const int k = it.y + (int)threadIdx.x;
+    if (k >= f.n) return;
+    const int j = k < f.nfs ? perm[f.piv + k] : k;
+    double v = j < f.nfs ? rhs[(int64_t)faces[f.faces + j / b] * b + j % b] : 0.0;
+    const int2 g = gsrc[f.gsrc + j];
+    if (g.x >= 0) v += cv[g.x];
+    if (g.y >= 0) v += cv[g.y];
+    if (k < f.nfs)
+        y[f.piv + k] = v;
+    else
+        cv[f.cvec + k - f.nfs] = v;
 }
 
 // sum over the LPR lanes of a row group (butterfly within the group)
@@ -552,36 +562,56 @@ __device__ inline double group_sum(double v)
     return v;
 }
 
-// dot product of a row with a vector by the LPR lanes of a group, each lane
-// its columns k = lane + LPR j with two accumulators (even, odd j)
+template <typename TF> struct Wide;
+template <> struct Wide<double> { using type = double2; static constexpr int n = 2; };
+template <> struct Wide<float> { using type = float4; static constexpr int n = 4; };
+
+// dot product of a stored row (ld values, 16-byte aligned, zero past the
+// row's own length) with a vector of ld values by the LPR lanes of a group:
+// each lane reads 16-byte words k = V (lane + LPR j), V = 16 / sizeof(TF),
+// into two accumulators (even, odd j), sums in binary64
 template <int LPR, typename TF>
-__device__ inline double row_dot(const TF* __restrict__ row, const double* __restrict__ v, int ncols, int lane)
+__device__ inline double row_dot(const TF* __restrict__ row, const double* __restrict__ v, int ld, int lane)
 {
+    constexpr int V = Wide<TF>::n;
+    using W = typename Wide<TF>::type;
     double s0 = 0.0, s1 = 0.0;
-    int k = lane;
-    for (; k + LPR < ncols; k += 2 * LPR) {
-        s0 = fma((double)row[k], v[k], s0);
-        s1 = fma((double)row[k + LPR], v[k + LPR], s1);
+    int k = lane * V;
+    for (; k + LPR * V < ld; k += 2 * LPR * V) {
+        const W a = *reinterpret_cast<const W*>(row + k);
+        const W c = *reinterpret_cast<const W*>(row + k + LPR * V);
+        const TF* ea = reinterpret_cast<const TF*>(&a);
+        const TF* ec = reinterpret_cast<const TF*>(&c);
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            s0 = fma((double)ea[j], v[k + j], s0);
+            s1 = fma((double)ec[j], v[k + LPR * V + j], s1);
+        }
     }
-    if (k < ncols) s0 = fma((double)row[k], v[k], s0);
+    if (k < ld) {
+        const W a = *reinterpret_cast<const W*>(row + k);
+        const TF* ea = reinterpret_cast<const TF*>(&a);
+#pragma unroll
+        for (int j = 0; j < V; ++j) s0 = fma((double)ea[j], v[k + j], s0);
+    }
     return group_sum<LPR>(s0 + s1);
 }
 
-// c[i] -= W[i] . y over the rows of a work item: 256 / LPR row groups of
-// LPR lanes, 2 rows per group; the factors in TF (binary64, or their
+// c[i] -= W[i] . y over the rows of a work item (rows it.y .. it.y + R):
+// 256 / LPR row groups of LPR lanes; the factors in TF (binary64, or their
 // binary32 copy), the sums in binary64
 template <int LPR, typename TF>
 __global__ void __launch_bounds__(256) k_fwd_gemv(const int2* __restrict__ items, const Front* __restrict__ fr,
                                                   const TF* __restrict__ fac, const double* __restrict__ y,
-                                                  double* __restrict__ cv)
+                                                  double* __restrict__ cv, int R)
 {
     constexpr int G = 256 / LPR;
     const int2 it = items[blockIdx.x];
     const Front f = fr[it.x];
     const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
     const double* yv = y + f.piv;
-    for (int r = it.y + grp; r < min(it.y + 2 * G, f.ncb); r += G) {
-        const double s = row_dot<LPR, TF>(fac + f.w + (int64_t)r * f.nfs, yv, f.nfs, lane);
+    for (int r = it.y + grp; r < min(it.y + R, f.ncb); r += G) {
+        const double s = row_dot<LPR, TF>(fac + f.w + (int64_t)r * row_ld(f.nfs), yv, row_ld(f.nfs), lane);
         if (lane == 0) cv[f.cvec + r] -= s;
     }
 }
@@ -592,18 +622,19 @@ template <int LPR, typename TF>
 __global__ void __launch_bounds__(256) k_bwd(const int2* __restrict__ items, const Front* __restrict__ fr,
                                              const int32_t* __restrict__ faces, int b,
                                              const TF* __restrict__ fac, const double* __restrict__ y,
-                                             double* __restrict__ x)
+                                             double* __restrict__ x, int R)
 {
     extern __shared__ double u[];
     constexpr int G = 256 / LPR;
     const int2 it = items[blockIdx.x];
     const Front f = fr[it.x];
-    for (int j = threadIdx.x; j < f.n; j += blockDim.x)
-        u[j] = j < f.nfs ? y[f.piv + j] : -x[(int64_t)faces[f.faces + j / b] * b + j % b];
+    const int ld = row_ld(f.n);
+    for (int j = threadIdx.x; j < ld; j += blockDim.x)
+        u[j] = j < f.nfs ? y[f.piv + j] : j < f.n ? -x[(int64_t)faces[f.faces + j / b] * b + j % b] : 0.0;
     __syncthreads();
     const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
-    for (int r = it.y + grp; r < min(it.y + 2 * G, f.nfs); r += G) {
-        const double s = row_dot<LPR, TF>(fac + f.zv + (int64_t)r * f.n, u, f.n, lane);
+    for (int r = it.y + grp; r < min(it.y + R, f.nfs); r += G) {
+        const double s = row_dot<LPR, TF>(fac + f.zv + (int64_t)r * ld, u, ld, lane);
         if (lane == 0) x[(int64_t)faces[f.faces + r / b] * b + r % b] = s;
     }
 }
@@ -620,7 +651,8 @@ struct Symbolic {
     std::vector<Level> levels;           // by depth, 0 = root
     std::vector<int32_t> face_list, fmap, node_list;
     std::vector<int4> rec;
-    std::vector<int2> fw_items, bw_items;
+    std::vector<int2> fw_items, bw_items, gi_items;
+    std::vector<int2> gsrc;              // per front row: the two children's contribution indices, or -1
     std::vector<int32_t> bsr_ptr, bsr_col;
     int64_t fac_size = 0, piv_size = 0, cvec_size = 0;
     int64_t flops = 0;
@@ -745,9 +777,9 @@ void build_symbolic(const pbte_trace_desc* d, Symbolic& S)
         f.child[1] = nodes[v].child[1];
         f.cmap[0] = f.cmap[1] = -1;
         f.zv = fac;
-        fac += (int64_t)f.nfs * f.n;
+        fac += (int64_t)f.nfs * row_ld(f.n);
         f.w = fac;
-        fac += (int64_t)f.ncb * f.nfs;
+        fac += (int64_t)f.ncb * row_ld(f.nfs);
         f.piv = (int32_t)piv;
         piv += f.nfs;
         f.cvec = (int32_t)cvec;
@@ -773,6 +805,8 @@ void build_symbolic(const pbte_trace_desc* d, Symbolic& S)
         Front& f = S.fronts[v];
         const int nfc = f.n / b;
         for (int k = 0; k < nfc; ++k) pos[S.face_list[f.faces + k]] = k;
+        f.gsrc = (int32_t)S.gsrc.size();
+        S.gsrc.resize(S.gsrc.size() + f.n, make_int2(-1, -1));
         for (int s = 0; s < 2; ++s) {
             const int c = f.child[s];
             if (c < 0) continue;
@@ -783,6 +817,10 @@ void build_symbolic(const pbte_trace_desc* d, Symbolic& S)
                 const int g = S.face_list[ch.faces + nown + k];
                 if (pos[g] < 0) throw std::string("trace solve: child face missing from parent front");
                 S.fmap.push_back(pos[g]);
+                for (int e = 0; e < b; ++e) {
+                    int2& src = S.gsrc[(size_t)f.gsrc + (size_t)pos[g] * b + e];
+                    (s == 0 ? src.x : src.y) = ch.cvec + k * b + e;
+                }
             }
             S.levels[nodes[v].depth].has_children = true;
             S.levels[nodes[v].depth].ncb_child_max = std::max(S.levels[nodes[v].depth].ncb_child_max, ch.ncb);
@@ -823,6 +861,10 @@ void build_symbolic(const pbte_trace_desc* d, Symbolic& S)
         S.rec.insert(S.rec.end(), recs[l].begin(), recs[l].end());
         L.fw_off = (int32_t)S.fw_items.size();
         L.bw_off = (int32_t)S.bw_items.size();
+        L.gi_off = (int32_t)S.gi_items.size();
+        for (int v : L.nodes)
+            for (int r = 0; r < S.fronts[v].n; r += 256) S.gi_items.push_back({v, r});
+        L.gi_n = (int32_t)S.gi_items.size() - L.gi_off;
         int nfs_max = 0, n_max = 0;
         for (int v : L.nodes) {
             nfs_max = std::max(nfs_max, S.fronts[v].nfs);
@@ -838,7 +880,7 @@ void build_symbolic(const pbte_trace_desc* d, Symbolic& S)
         }
         L.fw_n = (int32_t)S.fw_items.size() - L.fw_off;
         L.bw_n = (int32_t)S.bw_items.size() - L.bw_off;
-        if ((int64_t)L.n_max * 8 > 64 * 1024)
+        if ((int64_t)row_ld(L.n_max) * 8 > 64 * 1024)
             throw std::string("trace solve: a front exceeds the LDS vector of the solve (n > 8192)");
     }
 }
@@ -858,7 +900,7 @@ struct pbte_trace {
     int32_t *d_faces = nullptr, *d_fmap = nullptr, *d_nodes = nullptr, *d_perm = nullptr, *d_ipiv = nullptr,
             *d_status = nullptr;
     int4* d_rec = nullptr;
-    int2 *d_fw = nullptr, *d_bw = nullptr;
+    int2 *d_fw = nullptr, *d_bw = nullptr, *d_gi = nullptr, *d_gsrc = nullptr;
     double *d_fac = nullptr, *d_ws[2] = {nullptr, nullptr}, *d_pan = nullptr, *d_y = nullptr, *d_cv = nullptr;
     float* d_fac32 = nullptr;   // binary32 copy of the factors (made by the first binary32 solve)
     bool fac32_current = false;
@@ -866,7 +908,7 @@ struct pbte_trace {
 
     ~pbte_trace()
     {
-        void* ps[] = {d_fr, d_faces, d_fmap, d_nodes, d_perm, d_ipiv, d_status, d_rec, d_fw, d_bw,
+        void* ps[] = {d_fr, d_faces, d_fmap, d_nodes, d_perm, d_ipiv, d_status, d_rec, d_fw, d_bw, d_gi, d_gsrc,
                       d_fac, d_ws[0], d_ws[1], d_pan, d_y, d_cv, d_fac32};
         for (void* p : ps)
             if (p) (void)hipFree(p);
@@ -892,6 +934,8 @@ struct pbte_trace {
         up(d_rec, S.rec);
         up(d_fw, S.fw_items);
         up(d_bw, S.bw_items);
+        up(d_gi, S.gi_items);
+        up(d_gsrc, S.gsrc);
         d_perm = dmalloc<int32_t>(S.piv_size);
         d_ipiv = dmalloc<int32_t>(S.piv_size);
         d_status = dmalloc<int32_t>(1);
@@ -899,7 +943,11 @@ struct pbte_trace {
         d_ws[0] = dmalloc<double>(ws_size[0]);
         d_ws[1] = dmalloc<double>(ws_size[1]);
         d_pan = dmalloc<double>(pan_size);
-        d_y = dmalloc<double>(S.piv_size);
+        // y is read past a front's own values up to its padded row length
+        // (against the zero padding of W): four zeros after the last one
+        d_y = dmalloc<double>(S.piv_size + 4);
+        hipLaunchKernelGGL(k_fill, dim3(64), dim3(256), 0, stream, d_y, S.piv_size + 4);
+        TL_TRY(hipGetLastError());
         d_cv = dmalloc<double>(S.cvec_size);
         TL_TRY(hipStreamSynchronize(stream));
     }
@@ -958,6 +1006,32 @@ extern "C" int pbte_trace_quality(pbte_trace* t, int32_t* pivoting, double* berr
         if (!t->factored) throw std::string("trace solve: not factorised");
         *pivoting = t->pivoting;
         *berr = t->berr;
+        return 0;
+    });
+}
+
+extern "C" int pbte_trace_level(pbte_trace* t, int32_t level, int64_t* out8)
+{
+    return tguard(t, [&] {
+        const Symbolic& S = t->S;
+        if (level < 0 || level >= (int)S.levels.size()) throw std::string("trace level out of range");
+        const Level& L = S.levels[level];
+        int64_t nfs = 0, ncb = 0, zv = 0, w = 0;
+        for (int v : L.nodes) {
+            const Front& f = S.fronts[v];
+            nfs += f.nfs;
+            ncb += f.ncb;
+            zv += (int64_t)f.nfs * row_ld(f.n);
+            w += (int64_t)f.ncb * row_ld(f.nfs);
+        }
+        out8[0] = (int64_t)L.nodes.size();
+        out8[1] = L.n_max;
+        out8[2] = nfs;
+        out8[3] = ncb;
+        out8[4] = zv;
+        out8[5] = w;
+        out8[6] = L.fw_n;
+        out8[7] = L.bw_n;
         return 0;
     });
 }
@@ -1133,6 +1207,11 @@ extern "C" int pbte_trace_factor(pbte_trace* t, const double* bsr_val)
 }
 
 namespace {
+// lanes per row of a solve product: the level's choice for binary64 rows;
+// binary32 rows hold half the bytes, so half the lanes give each lane the
+// same 16-byte loads per row (the work items, 2 * 256 / lpr rows, stay)
+template <typename TF> int solve_lanes(int lpr) { return sizeof(TF) == 4 ? std::max(4, lpr / 2) : lpr; }
+
 template <typename TF>
 void solve_pass_t(pbte_trace* t, const TF* fac, const double* rhs, double* x)
 {
@@ -1140,23 +1219,24 @@ void solve_pass_t(pbte_trace* t, const TF* fac, const double* rhs, double* x)
     hipStream_t st = t->stream;
     for (int l = S.max_depth; l >= 0; --l) {
         const Level& L = S.levels[l];
-        const int nf = (int)L.nodes.size();
-        hipLaunchKernelGGL(k_fwd_gather, dim3(nf), dim3(256), (size_t)L.n_max * 8, st, t->d_nodes + L.node_off,
-                           t->d_fr, t->d_faces, t->d_fmap, t->d_perm, S.b, rhs, t->d_y, t->d_cv);
+        hipLaunchKernelGGL(k_fwd_gather, dim3(L.gi_n), dim3(256), 0, st, t->d_gi + L.gi_off, t->d_fr, t->d_faces,
+                           t->d_gsrc, t->d_perm, S.b, rhs, t->d_y, t->d_cv);
         if (L.fw_n > 0) {
-            auto fn = L.lpr_fw == 8 ? k_fwd_gemv<8, TF> : L.lpr_fw == 16 ? k_fwd_gemv<16, TF>
-                    : L.lpr_fw == 32 ? k_fwd_gemv<32, TF> : k_fwd_gemv<64, TF>;
+            const int lpr = solve_lanes<TF>(L.lpr_fw);
+            auto fn = lpr == 4 ? k_fwd_gemv<4, TF> : lpr == 8 ? k_fwd_gemv<8, TF> : lpr == 16 ? k_fwd_gemv<16, TF>
+                    : lpr == 32 ? k_fwd_gemv<32, TF> : k_fwd_gemv<64, TF>;
             hipLaunchKernelGGL(fn, dim3(L.fw_n), dim3(256), 0, st, t->d_fw + L.fw_off, t->d_fr, fac, t->d_y,
-                               t->d_cv);
+                               t->d_cv, 2 * 256 / L.lpr_fw);
         }
     }
     for (int l = 0; l <= S.max_depth; ++l) {
         const Level& L = S.levels[l];
         if (L.bw_n > 0) {
-            auto fn = L.lpr_bw == 8 ? k_bwd<8, TF> : L.lpr_bw == 16 ? k_bwd<16, TF>
-                    : L.lpr_bw == 32 ? k_bwd<32, TF> : k_bwd<64, TF>;
-            hipLaunchKernelGGL(fn, dim3(L.bw_n), dim3(256), (size_t)L.n_max * 8, st, t->d_bw + L.bw_off, t->d_fr,
-                               t->d_faces, S.b, fac, t->d_y, x);
+            const int lpr = solve_lanes<TF>(L.lpr_bw);
+            auto fn = lpr == 4 ? k_bwd<4, TF> : lpr == 8 ? k_bwd<8, TF> : lpr == 16 ? k_bwd<16, TF>
+                    : lpr == 32 ? k_bwd<32, TF> : k_bwd<64, TF>;
+            hipLaunchKernelGGL(fn, dim3(L.bw_n), dim3(256), (size_t)row_ld(L.n_max) * 8, st, t->d_bw + L.bw_off, t->d_fr,
+                               t->d_faces, S.b, fac, t->d_y, x, 2 * 256 / L.lpr_bw);
         }
     }
     TL_TRY(hipGetLastError());

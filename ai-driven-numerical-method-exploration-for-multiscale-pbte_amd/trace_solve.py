@@ -80,6 +80,18 @@ class TraceSolver:
                 "unknowns", "blocks")
         return {k: int(v) for k, v in zip(keys, out)}
 
+    def levels(self) -> list:
+        """Per level of the dissection tree (root first): fronts, widest
+        front, fully summed and contribution rows, stored entries of
+        [Z V] and W, and the solve's forward and backward work items."""
+        keys = ("fronts", "n_max", "rows", "contrib_rows", "zv_entries", "w_entries", "fw_items", "bw_items")
+        out = []
+        for lv in range(self.info()["levels"]):
+            v = (C.c_int64 * 8)()
+            self._check(self._lib.pbte_trace_level(self.handle, lv, v), "trace_level")
+            out.append({k: int(x) for k, x in zip(keys, v)})
+        return out
+
     def set_pivoting(self, mode: int) -> None:
         """-1 automatic (diagonal-block pivots, partial pivoting as the
         fallback), 0 diagonal-block pivots only, 1 partial pivoting only."""

@@ -500,6 +500,11 @@ int pbte_trace_info(pbte_trace *t, int64_t *out8);
  * own pivoting is internal). */
 int pbte_trace_options(pbte_trace *t, int32_t pivoting);
 
+/* Per level of the dissection tree (0 = root): fronts, widest front, fully
+ * summed rows, contribution rows, stored [Z V] and W entries (padded rows),
+ * forward and backward work items of the solve. */
+int pbte_trace_level(pbte_trace *t, int32_t level, int64_t *out8);
+
 /* The pivoting the factors were built with (0 block, 1 partial) and their
  * backward error on the test vector. */
 int pbte_trace_quality(pbte_trace *t, int32_t *pivoting, double *berr);
