@@ -35,7 +35,7 @@ from .constants import BC_NONTHERMALISING, BC_PERIODIC, BC_THERMALISING, PI
 from .device import dev_zeros, ptr, stream_handle, to_dev, torch
 from .trace_solve import TraceSolver
 
-__all__ = ["Acceleration", "build_hdg_device", "face_sides", "trace_pattern"]
+__all__ = ["Acceleration", "build_hdg_device", "face_sides", "prepare", "trace_pattern"]
 
 
 def face_sides(mesh, bc_type):
@@ -125,29 +125,44 @@ def build_hdg_device(c, mesh, ig, bcdata, st, sides, stream):
             "bsr_ptr": bptr, "bsr_col": bcol, "nnz": int(nnz.value)}
 
 
+def prepare(case, problem, stream) -> dict:
+    """The part of the set-up that needs no sweep context: the HDG operators
+    on the device and the factorised trace matrix.  ``Solver`` runs it on a
+    worker thread while the sweep context is built (the host schedule and
+    this device work overlap; every native call releases the GIL)."""
+    c = case
+    mesh, ig, bcdata = problem.mesh, problem.integrals, problem.bcdata
+    st = np.asarray(c.scheme.stabilisation, dtype=float).copy()
+    if c.scheme.scale_by_hmin:
+        st[1] /= mesh.hmin
+        st[2] /= mesh.hmin
+    stream = C.c_void_p(stream)
+    sides = face_sides(mesh, bcdata.bc_type)
+    ops = build_hdg_device(c, mesh, ig, bcdata, st, sides, stream)
+    trace = TraceSolver(mesh, sides[0], sides[2], ops["bsr_ptr"], ops["bsr_col"], 3 * c.ndof_fc, stream=stream,
+                        leaf_cells=c.performance.trace_leaf)
+    trace.factor(ops.pop("bsr_val"))
+    return {"st": st, "sides": sides, "ops": ops, "trace": trace}
+
+
 class Acceleration:
-    def __init__(self, solver):
+    def __init__(self, solver, prepared: dict | None = None):
         c = solver.case
         self.solver = s = solver
         self.variant = c.scheme.acc_variant
         self.nd, self.nf = c.ndof_tri, c.ndof_fc
-        st = np.asarray(c.scheme.stabilisation, dtype=float).copy()
-        if c.scheme.scale_by_hmin:
-            st[1] /= s.mesh.hmin
-            st[2] /= s.mesh.hmin
-        self.st = st
+        if prepared is None:
+            prepared = prepare(c, s.problem, stream_handle())
+        self.st = st = prepared["st"]
         mesh, ig = s.mesh, s.integrals
         bc_type = s.bcdata.bc_type
         P, N, F, Q = self.nd, s.n_tris, mesh.n_faces, self.nf
-        stream = C.c_void_p(stream_handle())
-        t1, l1, t2, l2, periodic = face_sides(mesh, bc_type)
-        ops = build_hdg_device(c, mesh, ig, s.bcdata, st, (t1, l1, t2, l2, periodic), stream)
+        t1, l1, t2, l2, periodic = prepared["sides"]
+        ops = prepared.pop("ops")
         inv_aa_sol, aa_trace, ba_sol = ops["inv_aa_sol"], ops["aa_trace"], ops["ba_sol"]
         blk = 3 * Q
         self.n_trace, self.nnz = F * blk, ops["nnz"]
-        self.trace = TraceSolver(mesh, t1, t2, ops["bsr_ptr"], ops["bsr_col"], blk, stream=stream,
-                                 leaf_cells=c.performance.trace_leaf)
-        self.trace.factor(ops.pop("bsr_val"))
+        self.trace = prepared["trace"]
         self.trace_info = self.trace.info()
         del ops
         # the T, qx, qy columns of inv(AA_SOL) serve the moment preconditioner

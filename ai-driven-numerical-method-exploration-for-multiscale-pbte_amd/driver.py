@@ -20,13 +20,14 @@ from __future__ import annotations
 import time
 import warnings
 import weakref
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
 
 from .config import Case
 from .restart import read_restart, restart_name, write_restart
-from .device import dev_zeros, require_gpu, torch
+from .device import dev_zeros, require_gpu, stream_handle, torch
 from .io_output import RunRecord, _fields, append_runtime_log, locate_points, sampling_grid, write_tecplot
 from .moments import Moments
 from .problem import build_problem
@@ -120,10 +121,28 @@ class Solver:
                               "the run's precision); the default schedule is used")
             else:
                 max_bundle, block, sweep_variant = (self.tile_shape[k] for k in ("max_bundle", "block", "variant"))
-        self.ctx = SweepContext.build(c, self.mesh, self.integrals, self.vel, self.order,
-                                      self.bcdata, max_bundle=max_bundle, block=block,
-                                      variant=sweep_variant, lds_slots_cap=lds_slots_cap,
-                                      max_bytes=max_bytes)
+        # GSIS: the HDG operators and the trace factorisation (device work
+        # and the trace solver's host symbolic phase) are built on a worker
+        # thread while this thread builds the sweep context (host schedule)
+        prep = None
+        if c.scheme.accflag == 1:
+            from .acceleration import prepare
+            pool = ThreadPoolExecutor(max_workers=1)
+            prep = pool.submit(prepare, c, pb, stream_handle())
+            pool.shutdown(wait=False)
+        try:
+            self.ctx = SweepContext.build(c, self.mesh, self.integrals, self.vel, self.order,
+                                          self.bcdata, max_bundle=max_bundle, block=block,
+                                          variant=sweep_variant, lds_slots_cap=lds_slots_cap,
+                                          max_bytes=max_bytes)
+        except BaseException:
+            if prep is not None:          # let the worker finish its device work first
+                try:
+                    prep.result()
+                except Exception:
+                    pass
+            raise
+        prepared = prep.result() if prep is not None else None
         # element operators factorised (driver.py:84-85, 155-156; krylov.py:
         # 119-120): D N once when stored ("precomputed", and the binary32 ones
         # of "mixed", whose binary64 sweeps refine against the stored binary32
@@ -144,7 +163,7 @@ class Solver:
         self.restart_info = self._apply_restart()
         if c.scheme.accflag == 1:
             from .acceleration import Acceleration
-            self.acc = Acceleration(self)
+            self.acc = Acceleration(self, prepared=prepared)
         torch.cuda.synchronize()
         self.setup_seconds = time.perf_counter() - t0
 

@@ -3,6 +3,7 @@ mesh, bases, ordinates, integral tensors, sweep order, boundary data.
 Needs no device."""
 from __future__ import annotations
 
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 
 from .basis import Basis, build_basis
@@ -35,10 +36,14 @@ def build_problem(case, on_cycle: str | None = None, validate_order: bool = Fals
                       [b.yoff for b in c.boundaries])
     basis = build_basis(c.dg.deg)
     vel = build_velocity_mesh(c.velmesh.npole, c.velmesh.nazim, c.flow.vg)
-    integrals = build_integrals(mesh, basis, c.np_tri, c.np_fc, bc_codes=c.bc_codes(),
-                                bc_xoff=[b.xoff for b in c.boundaries],
-                                bc_yoff=[b.yoff for b in c.boundaries])
-    order = build_sweep_order(mesh, vel, on_cycle=on_cycle or c.scheme.on_cycle)
+    # the sweep orders (native, threaded) are built on a worker thread while
+    # this one builds the integral tensors
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        fut = pool.submit(build_sweep_order, mesh, vel, on_cycle=on_cycle or c.scheme.on_cycle)
+        integrals = build_integrals(mesh, basis, c.np_tri, c.np_fc, bc_codes=c.bc_codes(),
+                                    bc_xoff=[b.xoff for b in c.boundaries],
+                                    bc_yoff=[b.yoff for b in c.boundaries])
+        order = fut.result()
     if validate_order:
         validate_sweep_order(mesh, vel, order)
     bcdata = build_boundary_data(mesh, c.boundaries, c.ndof_tri)

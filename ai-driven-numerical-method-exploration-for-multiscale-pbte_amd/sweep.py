@@ -22,7 +22,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native
-from .constants import PI
+from .constants import BC_PERIODIC, PI
 from .device import dev_zeros, ptr, stream_handle, torch
 
 __all__ = ["SweepContext"]
@@ -41,13 +41,10 @@ def _other_lfc(mesh, bcdata) -> np.ndarray:
     lfc_other = np.where(t0 == own, mesh.face_lfc[f, 1], mesh.face_lfc[f, 0])
     out[interior] = lfc_other[interior]
     if bcdata.has_periodic:
-        adj_tri, adj_lfc = mesh.adjacent_arrays()
-        for i in range(n):
-            for il in range(3):
-                fc = int(f[i, il])
-                bc = int(mesh.face_bc[fc])
-                if bc >= 0 and bcdata.bc_type[bc] == 3:
-                    out[i, il] = adj_lfc[int(mesh.face_pair[fc])]
+        _, adj_lfc = mesh.adjacent_arrays()
+        bc = mesh.face_bc[f]
+        per = (bc >= 0) & (np.asarray(bcdata.bc_type)[np.clip(bc, 0, None)] == BC_PERIODIC)
+        out[per] = adj_lfc[mesh.face_pair[f[per]]]
     return out
 
 
