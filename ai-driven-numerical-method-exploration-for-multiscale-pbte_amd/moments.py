@@ -62,7 +62,11 @@ class Moments:
 
     @property
     def mass(self) -> float:
-        """Sequential index-order sum of ``temp`` (``_kernels.py:340-347``)."""
+        """Sum of ``temp`` (``_kernels.py:340-347``): on the device in a
+        fixed two-pass order (per-workgroup partial sums over a grid-stride
+        partition, then the partials in block order), so a run repeats bit
+        for bit but the order is not the reference's sequential one; on the
+        host (no context) in index order."""
         if self._ctx is not None:
             return self._ctx.mass(self)
         return float(sum(self.temp.tolist()))
@@ -79,7 +83,8 @@ def compute_moments(vdf, cxv, cyv, domega, int_tri, cv, mom: Moments, ctx=None) 
 
 
 def residual_iterate(temp, t_old, ctx=None) -> float:
-    """``sqrt(sum (T - T_old)^2 / sum T^2)`` summed in index order."""
+    """``sqrt(sum (T - T_old)^2 / sum T^2)`` (``_kernels.py:340-361``),
+    summed on the device in the fixed two-pass order of ``Moments.mass``."""
     if ctx is None:
         raise TypeError("residual_iterate needs the solver's SweepContext")
     return ctx.residual_iterate(temp, t_old)[0]
