@@ -222,6 +222,44 @@ def gsis_workload(pkg, args, work):
     print(json.dumps(out), flush=True)
 
 
+def trace_workload(pkg, args, work):
+    """The GSIS trace solve of the config-2 mesh alone: its dissection per
+    level and args.steps solves from the binary64 factors and from their
+    binary32 copy (after args.warmup each), timed with HIP events on the
+    solver's stream.  The trace system does not depend on the ordinates, so
+    the solver is built with 4 x 4 of them."""
+    import torch
+    case = gsis_case(pkg, args.n, args.deg, 4, 4, 1e-3, 1e-4, "fp32", work,
+                     jitter=args.jitter if args.jitter else 0.25)
+    log(f"trace: jittered channel {args.n}x{args.n}, P{args.deg}")
+    s = pkg.Solver(case)
+    tr = s.acc.trace
+    st = s.acc.last
+    rhs = torch.randn(st["ffa"].shape, dtype=torch.float64, device="cuda",
+                      generator=torch.Generator(device="cuda").manual_seed(0))
+    u = torch.zeros_like(st["u"])
+    times = {}
+    for prec in ("fp64", "fp32"):
+        for _ in range(args.warmup):
+            tr.solve(rhs, u, prec)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(args.steps)]
+        for a, b in ev:
+            a.record()
+            tr.solve(rhs, u, prec)
+            b.record()
+        torch.cuda.synchronize()
+        times[prec] = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    info = tr.info()
+    out = {"workload": f"gsis trace solve, jittered channel {args.n}x{args.n}, P{args.deg}",
+           "unknowns": info["unknowns"], "fronts": info["nodes"], "factor_entries": info["factor_entries"],
+           "solve_ms_fp64": times["fp64"], "solve_ms_fp32": times["fp32"],
+           "factor_GBps_fp64": 8.0 * info["factor_entries"] / times["fp64"] / 1e6,
+           "factor_GBps_fp32": 4.0 * info["factor_entries"] / times["fp32"] / 1e6,
+           "levels": tr.levels(), "steps": args.steps, "warmup": args.warmup}
+    print(json.dumps(out), flush=True)
+
+
 def config2_workload(pkg, args, work):
     """The headline line: the config-2 FP32 sweep (value, roofline), the
     FP64 vs FP32 GSIS solves of the same case (solve) and the CPU sweep
@@ -578,7 +616,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=0, help="mesh size a side (default 256 for config2 and gsis, "
                                                       "128 for sweep)")
-    ap.add_argument("--workload", default="config2", choices=["config2", "sweep", "cis", "gsis", "batch"])
+    ap.add_argument("--workload", default="config2", choices=["config2", "sweep", "cis", "gsis", "batch", "trace"])
     ap.add_argument("--batch-count", type=int, default=64, help="problems of the batch workload")
     ap.add_argument("--batch-solve-div", type=int, default=4,
                     help="batch workload: solve leg on meshes this many times coarser a side (0 skips)")
@@ -624,7 +662,7 @@ def main():
     import torch
 
     work = Path(tempfile.mkdtemp(prefix="pbte_bench_"))
-    big = args.workload in ("config2", "gsis")
+    big = args.workload in ("config2", "gsis", "trace")
     args.n = args.n or (256 if big else 128)
     args.npole = args.npole or (16 if big else 8)
     if args.workload == "config2":
@@ -638,6 +676,9 @@ def main():
         return
     if args.workload == "gsis":
         gsis_workload(pkg, args, work)
+        return
+    if args.workload == "trace":
+        trace_workload(pkg, args, work)
         return
     log(f"mesh {args.n}x{args.n} (hole {args.hole:g}), P{args.deg}, {args.npole}x{args.nazim} ordinates")
     case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, args.tau_r, 0.1 * args.tau_r,
