@@ -115,6 +115,8 @@ _SIGS = {
     "pbte_vec_cgs2_end": (C.c_int, [C.c_void_p, _f64p]),
     "pbte_vec_defect": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_int64]),
+    "pbte_vec_relax": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_int64,
+                                 C.POINTER(C.c_double)]),
     "pbte_vec_axpby": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_double, C.c_void_p,
                                  C.c_int64]),
     "pbte_gsis_setup": (C.c_int, [C.c_void_p, C.POINTER(PbteGsisData)]),

@@ -243,8 +243,10 @@ class Acceleration:
         self.anderson = c.scheme.defect_anderson
         self._ncall = 0
         self.defect_steps: list[float] = []
-        self._and_d: list = []
-        self._and_g: list = []
+        self._o_d = None                  # the offset o = UQ[:3P] - M_kin
+        self._and_pairs = None            # Anderson window on the device
+        self._and_work = None
+        self._and_rows: list[int] = []    # window rows, oldest first
 
     # -- one acceleration cycle -------------------------------------------
     def _set(self, dtype) -> dict:
@@ -387,37 +389,83 @@ class Acceleration:
 
     # -- the fixed-point repair (acceleration/__init__.py:118-172) ----------
     def refresh_defect(self) -> float:
-        P = self.nd
-        o = self.last["uq"].view(3, P, -1) - self.mkin_d
-        step = float((o - self.defect_d).abs().max())
+        """Move the defect toward the current offset ``o = UQ[:3P] - M_kin``
+        and return ``max |o - defect|`` before the move: relaxation
+        ``defect += omega (o - defect)``, or an Anderson step on the map
+        ``defect -> o`` (``scheme.defect_anderson``).  Every vector
+        operation runs on the device (``pbte_vec_*``)."""
+        ctx = self.solver.ctx
+        if self._o_d is None:
+            self._o_d = dev_zeros(self.defect_d.numel())
+        o = self._o_d
+        ctx.defect(self.last["uq"], self.mkin_d, None, o)
         if self.anderson:
+            step = ctx.relax(0.0, o, self.defect_d)
             self._anderson_step(o)
         else:
-            self.defect_d.add_(self.omega * (o - self.defect_d))
+            step = ctx.relax(self.omega, o, self.defect_d)
         self.defect_steps.append(step)
         return step
 
     def _anderson_step(self, o) -> None:
-        self._and_d.append(self.defect_d.reshape(-1).cpu().numpy().copy())
-        self._and_g.append(o.reshape(-1).cpu().numpy().copy())
-        if len(self._and_d) > self.anderson + 1:
-            self._and_d.pop(0)
-            self._and_g.pop(0)
-        if len(self._and_d) == 1:
-            self.defect_d.add_(self.omega * (o - self.defect_d))
+        """Anderson (type II) on ``d -> o(d)`` over the last
+        ``defect_anderson + 1`` pairs (acceleration/__init__.py:149-172).
+        The pairs stay on the device.  ``min |dR gamma - R_last|`` (``R =
+        G - D``, ``dR`` its successive differences) is solved through a thin
+        QR of ``dR`` -- Gram-Schmidt twice on the device -- and numpy's
+        least squares on the small triangle, with the singular-value cut-off
+        numpy would use for ``dR`` itself; then ``d = G_last - dG gamma`` on
+        the device.  A non-finite result empties the window and relaxes."""
+        ctx = self.solver.ctx
+        n = o.numel()
+        W = self.anderson + 1
+        if self._and_pairs is None:
+            self._and_pairs = dev_zeros((2, W, n))     # defect rows, offset rows
+            self._and_work = dev_zeros((3, W, n))      # R, differences of G, Q
+        Dw, Gw = self._and_pairs[0], self._and_pairs[1]
+        rows = self._and_rows
+        free = rows.pop(0) if len(rows) == W else min(set(range(W)) - set(rows))
+        Dw[free].copy_(self.defect_d.view(-1))
+        Gw[free].copy_(o.view(-1))
+        rows.append(free)
+        m = len(rows)
+        if m == 1:
+            ctx.relax(self.omega, o, self.defect_d)
             return
-        D = np.asarray(self._and_d)
-        G = np.asarray(self._and_g)
-        R = G - D
-        dR = (R[1:] - R[:-1]).T
-        gamma, *_ = np.linalg.lstsq(dR, R[-1], rcond=None)
-        new = G[-1] - (G[1:] - G[:-1]).T @ gamma
-        if not np.all(np.isfinite(new)):
-            self._and_d.clear()
-            self._and_g.clear()
-            self.defect_d.add_(self.omega * (o - self.defect_d))
+        R, dG, Q = self._and_work[0], self._and_work[1], self._and_work[2]
+        for j, r in enumerate(rows):
+            ctx.defect(Gw[r], Dw[r], None, R[j])
+        k = m - 1
+        Rk = np.zeros((k, k))
+        for j in range(k):
+            ctx.defect(R[j + 1], R[j], None, Q[j])
+            if j == 0:
+                nrm = ctx.norm(Q[0])
+            else:
+                Rk[:j, j], nrm = ctx.cgs2(Q, j, Q[j])
+            Rk[j, j] = nrm
+            if nrm > 0.0:
+                ctx.axpby(1.0 / nrm, Q[j], 0.0, Q[j])
+        b = ctx.multidot(Q, k, R[k])
+        gamma = np.linalg.lstsq(Rk, b, rcond=np.finfo(np.float64).eps * max(n, k))[0]
+        for j in range(k):
+            ctx.defect(Gw[rows[j + 1]], Gw[rows[j]], None, dG[j])
+        new = R[0]
+        new.copy_(Gw[rows[-1]])
+        ctx.update(dG, k, gamma, new)
+        if not (np.all(np.isfinite(gamma)) and np.isfinite(ctx.relax(0.0, new, new))):
+            rows.clear()
+            ctx.relax(self.omega, o, self.defect_d)
             return
-        self.defect_d.copy_(torch.from_numpy(new.reshape(self.defect_d.shape)))
+        self.defect_d.view(-1).copy_(new)
+
+    def _defect_norm(self) -> float:
+        """``max |defect|`` on the device (the offset buffer, cleared, as
+        the zero it is measured against)."""
+        if self._o_d is None:
+            self._o_d = dev_zeros(self.defect_d.numel())
+        self._o_d.zero_()
+        return self.solver.ctx.relax(0.0, self.defect_d, self._o_d)
 
     def diagnostics(self) -> dict:
         return {"acc_variant": self.variant, "global_matrix_n": int(self.n_trace),
@@ -425,6 +473,6 @@ class Acceleration:
                 "stabilisation": [float(v) for v in self.st], "defect_omega": float(self.omega),
                 "defect_every": int(self.defect_every), "defect_anderson": int(self.anderson),
                 "defect_outer": len(self.defect_steps),
-                "defect_norm": float(self.defect_d.abs().max()) if self.omega else 0.0,
+                "defect_norm": self._defect_norm() if self.omega else 0.0,
                 "trace_solve": "device-multifrontal", "trace_ordering": "nested-dissection",
                 "trace_factor_entries": self.trace_info["factor_entries"]}

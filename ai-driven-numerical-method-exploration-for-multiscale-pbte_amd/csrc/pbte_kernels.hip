@@ -1829,6 +1829,51 @@ __global__ void __launch_bounds__(256) fixed_point_defect_kernel(const double* v
     out[i] = o;
 }
 
+// max of two magnitudes where a NaN wins, as numpy's max does
+__device__ inline double nan_max(double a, double b) { return (a != a || a > b) ? a : b; }
+
+// The relaxation of the GSIS fixed-point repair
+// (acceleration/__init__.py:144-147): per workgroup the NaN-propagating max
+// of |o - d| over a grid-stride partition, and, when omega != 0,
+// d += omega (o - d) in numpy's rounding (difference, product, sum, each
+// rounded; separate statements, so nothing is contracted into an fma).
+__global__ void __launch_bounds__(256) relax_kernel(double omega, const double* __restrict__ o, double* d,
+                                                    int64_t n, double* partial)
+{
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double di = d[i];
+        const double t = o[i] - di;
+        m = nan_max(fabs(t), m);
+        if (omega != 0.0) {
+            const double u = omega * t;
+            d[i] = di + u;
+        }
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] = nan_max(sh[threadIdx.x], sh[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) max_final_kernel(const double* partial, int nbk, double* out)
+{
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (int k = threadIdx.x; k < nbk; k += 256) m = nan_max(partial[k], m);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] = nan_max(sh[threadIdx.x], sh[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
 // periodic partner DOFs as Krylov state: gather vdf[d][ptri[k]][:] into a
 // binary64 block [D][K][P], and scatter such a block into pbuf[d][pface[k]][:]
 template <typename T>
@@ -4005,6 +4050,30 @@ extern "C" int pbte_vec_defect(pbte_ctx* ctx, const double* v, const double* t, 
         hipLaunchKernelGGL(fixed_point_defect_kernel, dim3(nblocks(n, 256)), dim3(256), 0, ctx->stream, v, t, g, out,
                            n);
         HIP_TRY(hipGetLastError());
+        return 0;
+    });
+}
+
+// step_host = max |o - d| (NaN if any entry is NaN), then d += omega (o - d)
+// unless omega == 0; waits for the stream
+extern "C" int pbte_vec_relax(pbte_ctx* ctx, double omega, const double* o, double* d, int64_t n,
+                              double* step_host)
+{
+    return guarded(ctx, [&]() {
+        if (n <= 0) {
+            *step_host = 0.0;
+            return 0;
+        }
+        const int nbk = (int)std::min<int64_t>(kDotBlocks, std::max<int64_t>(1, (n + 255) / 256));
+        if ((size_t)nbk + 1 > ctx->red_len) throw std::string("reduction buffer too small");
+        hipLaunchKernelGGL(relax_kernel, dim3(nbk), dim3(256), 0, ctx->stream, omega, o, d, n, ctx->red);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->red, nbk, ctx->red + nbk);
+        HIP_TRY(hipGetLastError());
+        double* h = ctx->host_staging(1);
+        HIP_TRY(hipMemcpyAsync(h, ctx->red + nbk, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        *step_host = h[0];
         return 0;
     });
 }

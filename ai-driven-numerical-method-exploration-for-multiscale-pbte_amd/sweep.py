@@ -366,6 +366,14 @@ class SweepContext:
                                            ptr(out), out.numel())
         _native.check(self.handle, st, "defect")
 
+    def relax(self, omega: float, o, d) -> float:
+        """``max |o - d|`` (NaN when an entry is NaN), then
+        ``d += omega (o - d)`` unless ``omega`` is 0; waits for the stream."""
+        step = C.c_double()
+        st = _native.lib().pbte_vec_relax(self.handle, float(omega), ptr(o), ptr(d), d.numel(), C.byref(step))
+        _native.check(self.handle, st, "relax")
+        return step.value
+
     def residual_iterate_async(self, temp_d, t_old_d, out_d) -> None:
         """Residual and mass into the device pair ``out_d`` without waiting."""
         st = _native.lib().pbte_residual_iterate_async(self.handle, ptr(temp_d), ptr(t_old_d), ptr(out_d))

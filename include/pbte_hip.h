@@ -263,6 +263,14 @@ int pbte_vec_cgs2_end(pbte_ctx *ctx, double *out_host);
 int pbte_vec_defect(pbte_ctx *ctx, const double *v, const double *t, const double *g, double *out,
                     int64_t n);
 
+/* The relaxation of the GSIS fixed-point repair
+ * (acceleration/__init__.py:144-147, refresh_defect): *step_host =
+ * max |o - d| (NaN when any entry is NaN, as numpy's max), then, unless
+ * omega == 0, d += omega (o - d) in numpy's rounding; waits for the
+ * stream.  With omega == 0 and o == d it tests o for non-finite entries. */
+int pbte_vec_relax(pbte_ctx *ctx, double omega, const double *o, double *d, int64_t n,
+                   double *step_host);
+
 /* y[i] = a x[i] + b y[i] */
 int pbte_vec_axpby(pbte_ctx *ctx, double a, const double *x, double b, double *y, int64_t n);
 
