@@ -14,6 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpbte_hip.so"
 
 PBTE_FP64 = 0
 PBTE_FP32 = 1
+PBTE_FP64_FP32 = 2          # pbte_factorise: both precisions from one pass
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)

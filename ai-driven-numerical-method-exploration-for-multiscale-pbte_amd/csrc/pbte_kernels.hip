@@ -289,6 +289,7 @@ struct FactorArgs {
     const int32_t *slot_cell, *bundle_ndir, *bundle_dir_off, *bundle_dirs;
     const int64_t *lu_off, *item_off;
     const double *cxv, *cyv, *mass, *gx, *gy, *fcm;
+    const double *refop, *cellop;      // reference-element matrices, 8 scalars per cell
     const CellGeo<double>* geo;
     double inv_tau_c;
     int N;
@@ -301,13 +302,17 @@ struct FactorArgs {
 // its DGETF2 factorisation with the reference's pivoting (_kernels.py:43-76),
 // in binary64; the diagonal holds reciprocals (the solve multiplies by
 // them).  Returns 0 or the first zero pivot (1-based), like INFO.
-template <int P>
-__device__ __forceinline__ int element_lu(const double* __restrict__ mass, const double* __restrict__ gx,
-                                          const double* __restrict__ gy, const double* __restrict__ fcm,
-                                          const CellGeo<double>& g, double inv_tau_c, int N, int i, double cx,
-                                          double cy, double (&A)[P][P], uint64_t& pv)
+// The entries (m, l) of the element's tensors come from get(m, l, E):
+// E[0] mass, E[1] / E[2] the x / y streaming terms, E[3..5] the three
+// face terms.
+struct ElemEntries {
+    double e[6];
+};
+
+template <int P, typename Get>
+__device__ __forceinline__ int element_lu_from(Get get, const CellGeo<double>& g, double inv_tau_c, double cx,
+                                               double cy, double (&A)[P][P], uint64_t& pv)
 {
-    const double* fc = fcm + (size_t)i * 3 * P * P;
     double w[3];
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
@@ -318,11 +323,11 @@ __device__ __forceinline__ int element_lu(const double* __restrict__ mass, const
     for (int l = 0; l < P; ++l)
 #pragma unroll
         for (int m = 0; m < P; ++m) {
-            double v = (inv_tau_c * mass[((size_t)m * P + l) * N + i] -
-                        cx * gx[((size_t)i * P + m) * P + l] - cy * gy[((size_t)i * P + m) * P + l]);
-            v = v + w[0] * fc[(0 * P + m) * P + l];
-            v = v + w[1] * fc[(1 * P + m) * P + l];
-            v = v + w[2] * fc[(2 * P + m) * P + l];
+            const ElemEntries E = get(m, l);
+            double v = (inv_tau_c * E.e[0] - cx * E.e[1] - cy * E.e[2]);
+            v = v + w[0] * E.e[3];
+            v = v + w[1] * E.e[4];
+            v = v + w[2] * E.e[5];
             A[m][l] = v;
         }
     pv = 0;
@@ -362,6 +367,59 @@ __device__ __forceinline__ int element_lu(const double* __restrict__ mass, const
         A[k][k] = rcp;
     }
     return info;
+}
+
+// element_lu_from on the element tensors as stored: mass [P][P][N],
+// streaming [N][P][P], faces [N][3][P][P]
+template <int P>
+__device__ __forceinline__ int element_lu(const double* __restrict__ mass, const double* __restrict__ gx,
+                                          const double* __restrict__ gy, const double* __restrict__ fcm,
+                                          const CellGeo<double>& g, double inv_tau_c, int N, int i, double cx,
+                                          double cy, double (&A)[P][P], uint64_t& pv)
+{
+    const double* fc = fcm + (size_t)i * 3 * P * P;
+    return element_lu_from<P>(
+        [&](int m, int l) {
+            ElemEntries E;
+            E.e[0] = mass[((size_t)m * P + l) * N + i];
+            E.e[1] = gx[((size_t)i * P + m) * P + l];
+            E.e[2] = gy[((size_t)i * P + m) * P + l];
+#pragma unroll
+            for (int il = 0; il < 3; ++il) E.e[3 + il] = fc[(il * P + m) * P + l];
+            return E;
+        },
+        g, inv_tau_c, cx, cy, A, pv);
+}
+
+// element_lu_from on tensors rebuilt from the reference-element matrices
+// ref [6][P][P] (mass, d/dx, d/dy, faces 1-3) and the 8 scalars of the cell
+// co (2|T|, the four inverse-Jacobian terms, the three scaled face
+// lengths), in integration.py's products one rounding at a time (separate
+// statements: no fma), so every entry equals the stored tensor's bit for bit
+template <int P>
+__device__ __forceinline__ int element_lu_ref(const double* ref, const double (&co)[8], const CellGeo<double>& g,
+                                              double inv_tau_c, double cx, double cy, double (&A)[P][P],
+                                              uint64_t& pv)
+{
+    constexpr int PP = P * P;
+    return element_lu_from<P>(
+        [&](int m, int l) {
+            const int e = m * P + l;
+            ElemEntries E;
+            E.e[0] = ref[e] * co[0];
+            const double x1 = co[1] * ref[PP + e];
+            const double x2 = co[2] * ref[2 * PP + e];
+            const double xs = x1 + x2;
+            E.e[1] = xs * co[0];
+            const double y1 = co[3] * ref[PP + e];
+            const double y2 = co[4] * ref[2 * PP + e];
+            const double ys = y1 + y2;
+            E.e[2] = ys * co[0];
+#pragma unroll
+            for (int il = 0; il < 3; ++il) E.e[3 + il] = co[5 + il] * ref[(3 + il) * PP + e];
+            return E;
+        },
+        g, inv_tau_c, cx, cy, A, pv);
 }
 
 // v = inv(A) v from element_lu's factors, DGETRS's loop order
@@ -421,6 +479,77 @@ __global__ void __launch_bounds__(256) factor_kernel(FactorArgs a)
                 if (a.lu64) a.lu64[base + lu_idx<double>(r * P + j, item, NB)] = v[r];
                 if (a.lu32) a.lu32[base + lu_idx<float>(r * P + j, item, NB)] = (float)v[r];
             }
+        }
+    }
+}
+
+// The same factorisation and inverse, one wave per 64 consecutive items of
+// a bundle (item = slot * B + direction), the operators rebuilt from the
+// reference-element matrices (LDS) and 8 scalars per cell (bit for bit the
+// stored tensors), each item's inverse staged in LDS as it is solved and
+// then written as whole 16-byte words, word plane by word plane, so a wave
+// stores 1 KiB runs; binary64 and binary32 copies from one pass when both
+// are asked for.  Grid: (ceil(N * max B / 64), bundles).
+template <int P>
+__global__ void __launch_bounds__(64) factor_ref_kernel(FactorArgs a)
+{
+    constexpr int PP = P * P;
+    constexpr int PV = pad4(PP);
+    __shared__ double ref[6 * PP];
+    __shared__ double inv[PP][64];
+    const int bundle = blockIdx.y;
+    const int t = threadIdx.x;
+    for (int k = t; k < 6 * PP; k += 64) ref[k] = a.refop[k];
+    __syncthreads();
+    const int B = a.bundle_ndir[bundle];
+    const size_t NB = (size_t)a.N * B;
+    const size_t item = (size_t)blockIdx.x * 64 + t;
+    if ((size_t)blockIdx.x * 64 >= NB) return;          // uniform over the block
+    const bool valid = item < NB;
+    if (valid) {
+        const int s = (int)(item / B), b = (int)(item % B);
+        const int i = a.slot_cell[a.item_off[bundle] + s];
+        const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
+        double co[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) co[k] = a.cellop[(size_t)i * 8 + k];
+        const CellGeo<double> g = a.geo[i];
+        double A[P][P];
+        uint64_t pv;
+        if (element_lu_ref<P>(ref, co, g, a.inv_tau_c, a.cxv[d], a.cyv[d], A, pv)) atomicAdd(a.singular, 1);
+#pragma unroll 1
+        for (int j = 0; j < P; ++j) {
+            double v[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) v[k] = k == j ? 1.0 : 0.0;
+            element_lu_solve<P>(A, pv, v);
+#pragma unroll
+            for (int r = 0; r < P; ++r) inv[r * P + j][t] = v[r];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int64_t base = a.lu_off[bundle];
+    if (a.lu64) {
+        double2* o = reinterpret_cast<double2*>(a.lu64 + base);
+#pragma unroll
+        for (int w = 0; w < PV / 2; ++w) {
+            double2 x;
+            x.x = 2 * w < PP ? inv[2 * w][t] : 0.0;
+            x.y = 2 * w + 1 < PP ? inv[2 * w + 1][t] : 0.0;
+            o[(size_t)w * NB + item] = x;
+        }
+    }
+    if (a.lu32) {
+        float4* o = reinterpret_cast<float4*>(a.lu32 + base);
+#pragma unroll
+        for (int w = 0; w < PV / 4; ++w) {
+            float4 x;
+            x.x = 4 * w < PP ? (float)inv[4 * w][t] : 0.0f;
+            x.y = 4 * w + 1 < PP ? (float)inv[4 * w + 1][t] : 0.0f;
+            x.z = 4 * w + 2 < PP ? (float)inv[4 * w + 2][t] : 0.0f;
+            x.w = 4 * w + 3 < PP ? (float)inv[4 * w + 3][t] : 0.0f;
+            o[(size_t)w * NB + item] = x;
         }
     }
 }
@@ -3321,8 +3450,17 @@ void launch_factor(pbte_ctx* ctx, double* lu64, float* lu32)
     a.cxv = ctx->cxv; a.cyv = ctx->cyv; a.mass = ctx->mass; a.gx = ctx->gx; a.gy = ctx->gy; a.fcm = ctx->fcm;
     a.geo = ctx->geo64; a.inv_tau_c = 1.0 / ctx->tau_c; a.N = ctx->N;
     a.lu64 = lu64; a.lu32 = lu32; a.singular = ctx->counter;
-    dim3 grid(nblocks(ctx->N, 128), ctx->nb);
-    hipLaunchKernelGGL(factor_kernel<P>, grid, dim3(128), 0, ctx->stream, a);
+    a.refop = ctx->refop; a.cellop = ctx->cellop;
+    if (ctx->refop && ctx->cellop) {
+        // the padded factor storage of a bundle holds its word planes whole
+        static_assert(pad4(P * P) % 4 == 0, "16-byte word planes");
+        const int64_t items = (int64_t)ctx->N * ctx->max_bundle_dirs;
+        dim3 grid((unsigned)((items + 63) / 64), ctx->nb);
+        hipLaunchKernelGGL(factor_ref_kernel<P>, grid, dim3(64), 0, ctx->stream, a);
+    } else {
+        dim3 grid(nblocks(ctx->N, 128), ctx->nb);
+        hipLaunchKernelGGL(factor_kernel<P>, grid, dim3(128), 0, ctx->stream, a);
+    }
     HIP_TRY(hipGetLastError());
 }
 
@@ -3506,13 +3644,15 @@ void launch_source(pbte_ctx* ctx, const double* ts, const double* qxs, const dou
 extern "C" int pbte_factorise(pbte_ctx* ctx, int32_t precision)
 {
     return guarded(ctx, [&]() {
-        if (precision != PBTE_FP64 && precision != PBTE_FP32) throw std::string("bad precision");
+        if (precision != PBTE_FP64 && precision != PBTE_FP32 && precision != PBTE_FP64_FP32)
+            throw std::string("bad precision");
         double* l64 = nullptr;
         float* l32 = nullptr;
-        if (precision == PBTE_FP64) {
+        if (precision != PBTE_FP32) {
             if (!ctx->lu64) ctx->lu64 = ctx->alloc<double>((size_t)ctx->lu_elems);
             l64 = ctx->lu64;
-        } else {
+        }
+        if (precision != PBTE_FP64) {
             if (!ctx->lu32) ctx->lu32 = ctx->alloc<float>((size_t)ctx->lu_elems);
             l32 = ctx->lu32;
         }
@@ -3526,7 +3666,8 @@ extern "C" int pbte_factorise(pbte_ctx* ctx, int32_t precision)
         int32_t bad = 0;
         HIP_TRY(hipMemcpyAsync(&bad, ctx->counter, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (precision == PBTE_FP64) ctx->have64 = true; else ctx->have32 = true;
+        if (l64) ctx->have64 = true;
+        if (l32) ctx->have32 = true;
         return (int)bad;
     });
 }

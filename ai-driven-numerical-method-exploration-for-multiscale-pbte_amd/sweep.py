@@ -169,9 +169,7 @@ class SweepContext:
             self.factorise("fp32")
             self.mode = "mixed"
         elif precompute and nbytes * (1.5 if self.precision == "fp32" else 1.0) <= max_bytes:
-            self.factorise("fp64")
-            if self.precision == "fp32":
-                self.factorise("fp32")
+            self.factorise("fp64+fp32" if self.precision == "fp32" else "fp64")
         return self
 
     def __del__(self):
@@ -208,12 +206,15 @@ class SweepContext:
         return float(self.ndir) * n_tris * ndof * ndof * 8.0
 
     def factorise(self, precision: str = "fp64") -> None:
-        code = _native.PBTE_FP64 if precision == "fp64" else _native.PBTE_FP32
+        """Factors of every element operator in ``precision``: "fp64",
+        "fp32", or "fp64+fp32" (both from one binary64 pass)."""
+        code = {"fp64": _native.PBTE_FP64, "fp32": _native.PBTE_FP32,
+                "fp64+fp32": _native.PBTE_FP64_FP32}[precision]
         bad = _native.check(self.handle, _native.lib().pbte_factorise(self.handle, code), "factorise")
         if bad:
             raise np.linalg.LinAlgError(
                 f"{bad} element operators are singular; check TAU_C and the mesh")
-        self._precomputed.add(precision)
+        self._precomputed.update(precision.split("+"))
         self.mode = "precomputed"
 
     def ensure_pbuf(self, dtype):

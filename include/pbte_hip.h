@@ -27,6 +27,8 @@ typedef struct pbte_ctx pbte_ctx;
 
 /* precision selector of the per-(cell, direction) data a sweep touches */
 enum { PBTE_FP64 = 0, PBTE_FP32 = 1 };
+/* pbte_factorise only: both precisions' factors from one binary64 pass */
+enum { PBTE_FP64_FP32 = 2 };
 
 typedef struct pbte_dims {
     int32_t ndir;     /* D = npole * nazim, d = j1 + npole * j2          */
@@ -127,9 +129,13 @@ int pbte_schedule_info(pbte_ctx *ctx, int64_t *out6);
 int pbte_timing(pbte_ctx *ctx, int32_t capacity);
 int pbte_timing_read(pbte_ctx *ctx, float *ms, int32_t n);
 
-/* Factorise A_SOL for every (direction, element) in the given precision.
- * Replaces SweepContext.factorise / _kernels.build_operators
- * (sweep.py:116-129, _kernels.py:43-148).  Returns 0, or the number of
+/* Factorise A_SOL for every (direction, element) in the given precision
+ * (PBTE_FP64_FP32: the binary64 factors and their binary32 copy from one
+ * pass).  Replaces SweepContext.factorise / _kernels.build_operators
+ * (sweep.py:116-129, _kernels.py:43-148).  The operators are built from the
+ * reference-element matrices and cell scalars of pbte_set_cell_operators
+ * when those are set (the same values as the element tensors, bit for
+ * bit), else from the element tensors.  Returns 0, or the number of
  * singular element operators (> 0). */
 int pbte_factorise(pbte_ctx *ctx, int32_t precision);
 
