@@ -222,6 +222,11 @@ class Performance(_Section):
     autotune_cache: str = _key("", lambda v: str(v or ""))
     # elements per leaf of the trace solve's nested dissection, 0 = 16
     trace_leaf: int = _key(0, int)
+    # binary32 sweeps: "fused" builds and factorises each element operator
+    # in binary32 inside the sweep (no binary32 factors stored or read, the
+    # reference's on-the-fly mode); "stored" reads binary32 factors formed
+    # once in binary64.  Fused falls back to stored where it cannot run.
+    fp32_operator: str = _key("fused", _lower)
 
     RULES = (
         (lambda s: s.threads >= 0, "performance.threads must not be negative"),
@@ -230,6 +235,8 @@ class Performance(_Section):
         (lambda s: 0.0 < s.refine_inner < 1.0, "performance.refine_inner must lie in (0, 1)"),
         (lambda s: s.refine_tol >= 0.0, "performance.refine_tol must not be negative"),
         (lambda s: s.trace_leaf >= 0, "performance.trace_leaf must not be negative"),
+        (lambda s: s.fp32_operator in ("fused", "stored"),
+         "performance.fp32_operator: expected 'fused' or 'stored'"),
     )
 
 

@@ -28,6 +28,7 @@
 // direction per bundle by default).  The bundles of one direction group
 // (the same order and inflow signs) and one size form a schedule class:
 // their levels, slot records and wall lists are built and stored once.
+#include "ref_element.h"
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -90,6 +91,11 @@ template <int P> struct Lay {
     static constexpr int R_N = 3 * P, R_INFO = 3 * P + 6;
     static constexpr int CR = pad4(3 * P + 9);
     static constexpr int CRL = pad4(3 * P + 6);  // the part every item loads
+    // the fused binary32 sweep's five element scalars follow the info words
+    // (in the row's padding): area2 / tau_c and area2 times the four affine
+    // gradient coefficients; that sweep loads the whole row
+    static constexpr int R_FUSE = 3 * P + 9;
+    static constexpr int CRF = pad4(3 * P + 14);
 };
 
 // row stride of the cell records: a multiple of 128 bytes, so a record
@@ -102,6 +108,11 @@ __host__ inline int rec_stride_rt(int P, size_t sz)
 {
     return (int)(((size_t)pad4(3 * P + 9) * sz + 127) / 128 * 128 / sz);
 }
+static_assert(rec_stride<3, float>() >= Lay<3>::CRF && rec_stride<6, float>() >= Lay<6>::CRF &&
+                  rec_stride<10, float>() >= Lay<10>::CRF && rec_stride<15, float>() >= Lay<15>::CRF &&
+                  rec_stride<3, double>() >= Lay<3>::CRF && rec_stride<6, double>() >= Lay<6>::CRF &&
+                  rec_stride<10, double>() >= Lay<10>::CRF && rec_stride<15, double>() >= Lay<15>::CRF,
+              "the fused element scalars fit the padding of every cell record");
 
 // LU entry k of item `item` (slot * B + b) of a bundle with NB items, laid
 // out [k / V][item][V] with V = 16 / sizeof(T): lanes holding consecutive
@@ -209,6 +220,13 @@ struct pbte_ctx {
     // matrix-free element operators (the refined binary64 sweep)
     double* refop = nullptr;    // [6][P][P]
     double* cellop = nullptr;   // [N][8]
+    // the fused binary32 sweep: the reference-element matrices [3][P][P] and
+    // the unit-edge mass matrix [Q][Q] of this context (fused_ok once the face
+    // tensors are checked, fused_tables once these equal the compiled
+    // tables), on when fused32 is set; face lengths [N][3]
+    std::vector<double> h_fref;
+    std::vector<double> h_flen;
+    bool fused_ok = false, fused32 = false, have_cellops = false, fused_tables = false;
     // scratch
     double* src64 = nullptr;   // source of the transport residual
     double* ssraw = nullptr;
@@ -762,6 +780,82 @@ __device__ __forceinline__ void apply_inverse(T (&x)[P], E&& e)
     for (int r = 0; r < P; ++r) x[r] = y[r];
 }
 
+__device__ __forceinline__ float recip(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double recip(double v) { return 1.0 / v; }
+
+// the fused element solve: A x = b with A built where it is used from the
+// reference-element matrices (mass M, D_xi, D_eta and the unit-edge mass
+// matrix Kf of a face, compile-time tables whose zero entries drop out) and
+// the element's five scalars in its cell record,
+//   A = (area2 / tau_c) M - area2 (cx ax + cy ay) D_xi - area2 (cx bx + cy by) D_eta
+//       + sum over faces of w+ L Kf on the face's nodes,  w+ = max(c.n, 0)
+// (_kernels.py:106-125 builds the same operator from the element tensors,
+// which are these products), then factorised without pivoting: A + A^T is
+// positive definite (the mass term, plus the upwind boundary term: the
+// volume advection term is skew up to the boundary integral), so no leading
+// block is singular.  Doolittle elimination with the right-hand side
+// carried along, then back substitution.
+template <typename T> __device__ __forceinline__ void fma_const(T& acc, double k, T v)
+{
+    if (k != 0.0) acc += (T)k * v;
+}
+
+template <int P> constexpr double unit_edge(int r, int q)
+{
+    return RefElem<P>::E0[face_node(Deg<P>::v, 0, r) * P + face_node(Deg<P>::v, 0, q)];
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void fused_solve(T (&x)[P], const T* cr, T cx, T cy)
+{
+    using L = Lay<P>;
+    using R = RefElem<P>;
+    constexpr int DEG = Deg<P>::v;
+    constexpr int Q = DEG + 1;
+    const T am = cr[L::R_FUSE];
+    const T bx = -(cx * cr[L::R_FUSE + 1] + cy * cr[L::R_FUSE + 2]);
+    const T by = -(cx * cr[L::R_FUSE + 3] + cy * cr[L::R_FUSE + 4]);
+    T A[P][P];
+#pragma unroll
+    for (int m = 0; m < P; ++m)
+#pragma unroll
+        for (int l = 0; l < P; ++l) {
+            T v = T(-0.0);      // -0 + x folds to x
+            fma_const(v, R::M[m * P + l], am);
+            fma_const(v, R::Dx[m * P + l], bx);
+            fma_const(v, R::Dy[m * P + l], by);
+            A[m][l] = v;
+        }
+#pragma unroll
+    for (int il = 0; il < 3; ++il) {
+        const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
+        const T w = T(0.5) * (sp + fabs(sp));
+#pragma unroll
+        for (int r = 0; r < Q; ++r)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fma_const(A[face_node(DEG, il, r)][face_node(DEG, il, q)], unit_edge<P>(r, q), w);
+    }
+    T rd[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        rd[k] = recip(A[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < P; ++i) {
+            const T f = A[i][k] * rd[k];
+#pragma unroll
+            for (int j = k + 1; j < P; ++j) A[i][j] -= f * A[k][j];
+            x[i] -= f * x[k];
+        }
+    }
+#pragma unroll
+    for (int i = P - 1; i >= 0; --i) {
+        T v = x[i];
+#pragma unroll
+        for (int j = i + 1; j < P; ++j) v -= A[i][j] * x[j];
+        x[i] = v * rd[i];
+    }
+}
+
 __device__ __forceinline__ int bits_of(float v) { return __float_as_int(v); }
 __device__ __forceinline__ int bits_of(double v) { return (int)(uint32_t)(uint64_t)__double_as_longlong(v); }
 
@@ -1105,15 +1199,22 @@ struct PipeRec {
     int4 r;                 // {cell | pool slot, face words}
 };
 
-template <int P, typename T, bool PFLU> struct PipeData {
+// how the pipelined sweep applies the element operator: the stored inverse
+// read in the solve (LU_READ) or gathered with the cell record
+// (LU_PREFETCH), or the operator built from the reference-element matrices
+// and factorised in registers (LU_FUSED, binary32)
+enum { LU_READ = 0, LU_PREFETCH = 1, LU_FUSED = 2 };
+template <int P, typename T> constexpr int lu_stored_mode() { return P * P * sizeof(T) <= 288 ? LU_PREFETCH : LU_READ; }
+
+template <int P, typename T, int LUM> struct PipeData {
     using L = Lay<P>;
     bool valid;
     int cell, slot, fw[3];
     int lds[3];         // LDS offset of each face's upwind trace (the zero slot when none)
     int wv;             // LDS offset of the element's wall inflow (the zero slot when none)
     int wst;            // LDS pool slot this item writes, or -1
-    T cr[L::CRL];       // the cell record without the info words
-    T lu[PFLU ? L::LU : 4];
+    T cr[LUM == LU_FUSED ? L::CRF : L::CRL];    // the cell record (without the info words unless fused)
+    T lu[LUM == LU_PREFETCH ? L::LU : 4];
 };
 
 // what the solve of a level reads from LDS (and, rarely, global memory)
@@ -1171,9 +1272,9 @@ __device__ __forceinline__ int rec_lds_mask(int use_pbuf)
 // in-order vmcnt counter would make an earlier use wait for every
 // prefetch in flight).  The inverse streams through a buffer descriptor:
 // one lane offset, the k-groups a scalar offset apart.
-template <int P, typename T, bool PFLU>
+template <int P, typename T, int LUM>
 __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec& r, int B, int b,
-                                            __amdgpu_buffer_rsrc_t lu, PipeData<P, T, PFLU>& o)
+                                            __amdgpu_buffer_rsrc_t lu, PipeData<P, T, LUM>& o)
 {
     using L = Lay<P>;
     constexpr int W = Tr<P>::W;
@@ -1202,8 +1303,8 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
         wv = code_in(kRecWall, code) ? (unsigned)a.wbase + (val * B + b) * PV : wv;
     }
     o.wv = (int)wv;
-    load16<T, L::CRL>(o.cr, a.crec + (size_t)i * a.rs);
-    if constexpr (PFLU) {
+    load16<T, (LUM == LU_FUSED ? L::CRF : L::CRL)>(o.cr, a.crec + (size_t)i * a.rs);
+    if constexpr (LUM == LU_PREFETCH) {
         constexpr int w = 16 / sizeof(T);
         const uint32_t voff = (uint32_t)(r.slot * B + b) * 16u;
         const uint32_t gstride = (uint32_t)a.N * (uint32_t)B * 16u;
@@ -1224,8 +1325,8 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
 // global reads (explicit periodic buffer, a previous-sweep value of a
 // broken cycle, a bundle running with full barriers) behind a wave-uniform
 // branch with the wait inside it
-template <int P, typename T, bool PFLU, bool GREAD>
-__device__ __forceinline__ void pipe_read(const SweepArgs<T>& a, const PipeData<P, T, PFLU>& dC, int d, const T* lds,
+template <int P, typename T, int LUM, bool GREAD>
+__device__ __forceinline__ void pipe_read(const SweepArgs<T>& a, const PipeData<P, T, LUM>& dC, int d, const T* lds,
                                           const T* vdf_d, PipeIn<P, T>& in)
 {
     using L = Lay<P>;
@@ -1263,8 +1364,8 @@ __device__ __forceinline__ void pipe_read(const SweepArgs<T>& a, const PipeData<
 }
 
 // second half: source, wall inflow, upwind faces, the inverse, the stores
-template <int P, typename T, bool PFLU>
-__device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeData<P, T, PFLU>& dC,
+template <int P, typename T, int LUM>
+__device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeData<P, T, LUM>& dC,
                                              const PipeIn<P, T>& in, T cx, T cy, int B, int b, T* lds, T* vdf_d,
                                              const T* lub)
 {
@@ -1284,7 +1385,9 @@ __device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeDa
         const T w = T(0.5) * (sp - fabs(sp));
         face_update<P, T>(x, il, w, a.kref, in.u[il]);
     }
-    if constexpr (PFLU) {
+    if constexpr (LUM == LU_FUSED) {
+        fused_solve<P, T>(x, cr, cx, cy);
+    } else if constexpr (LUM == LU_PREFETCH) {
         apply_inverse<P, T>(x, [&](int k) { return dC.lu[k]; });
     } else {
         const size_t item = (size_t)dC.slot * B + b, NB = (size_t)a.N * B;
@@ -1333,7 +1436,7 @@ template <typename T> struct PipeThreads {
 
 // BF: 1 when every bundle holds one direction (the default schedule; the
 // slot and LDS arithmetic then drops its bundle factors), 0 for bundles
-template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
+template <int P, typename T, int LUM, bool GREAD, int DEPTH, int BF>
 __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
 {
     constexpr int W = Tr<P>::W;
@@ -1404,15 +1507,15 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
     // while the loads issue); in binary64 the loads first (the read values
     // held across the gather would push the binary64 register sets into
     // the accumulation registers)
-    auto level = [&](const PipeData<P, T, PFLU>& dS, PipeRec& rN, int2& sN, int lvN, PipeData<P, T, PFLU>& dG,
+    auto level = [&](const PipeData<P, T, LUM>& dS, PipeRec& rN, int2& sN, int lvN, PipeData<P, T, LUM>& dG,
                      const PipeRec& rG) {
         PipeIn<P, T> in;
-        if constexpr (sizeof(T) == 4) pipe_read<P, T, PFLU, GREAD>(a, dS, d, lds, vdf_d, in);
+        if constexpr (sizeof(T) == 4) pipe_read<P, T, LUM, GREAD>(a, dS, d, lds, vdf_d, in);
         pipe_record(irec, sN, c, B, b, rN);
         sN = span(lvN);
-        pipe_gather<P, T, PFLU>(a, rG, B, b, lu, dG);
-        if constexpr (sizeof(T) != 4) pipe_read<P, T, PFLU, GREAD>(a, dS, d, lds, vdf_d, in);
-        pipe_compute<P, T, PFLU>(a, dS, in, cx, cy, B, b, lds, vdf_d, a.lu + lu0);
+        pipe_gather<P, T, LUM>(a, rG, B, b, lu, dG);
+        if constexpr (sizeof(T) != 4) pipe_read<P, T, LUM, GREAD>(a, dS, d, lds, vdf_d, in);
+        pipe_compute<P, T, LUM>(a, dS, in, cx, cy, B, b, lds, vdf_d, a.lu + lu0);
         level_barrier(global_mode);
     };
     if constexpr (DEPTH == 2) {
@@ -1422,14 +1525,14 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         // that no copy forces a wait
         int2 S0 = span(l0 + 2), S1 = span(l0 + 3);
         PipeRec R0, R1;
-        PipeData<P, T, PFLU> D0, D1;
+        PipeData<P, T, LUM> D0, D1;
         // the prologue issues its loads in the steady-state order (records
         // of the level after next before the gather of the next level):
         // vmcnt is in order, and the loop-entry state must not ask for more
         // waiting
         pipe_record(irec, span(l0), c, B, b, R0);
         pipe_record(irec, span(l0 + 1), c, B, b, R1);
-        pipe_gather<P, T, PFLU>(a, R0, B, b, lu, D0);
+        pipe_gather<P, T, LUM>(a, R0, B, b, lu, D0);
         // two levels per trip, with no exit between them: an odd level count
         // runs one empty level (span (0, 0)) at the end.  A mid-trip exit
         // would make the compiler rotate the register sets with copies, and a
@@ -1445,12 +1548,12 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         // level l0 + k mod 3.
         int2 S0 = span(l0 + 6), S1 = span(l0 + 4), S2 = span(l0 + 5);
         PipeRec R0, R1, R2;
-        PipeData<P, T, PFLU> D0, D1, D2;
+        PipeData<P, T, LUM> D0, D1, D2;
         pipe_record(irec, span(l0), c, B, b, R0);
         pipe_record(irec, span(l0 + 1), c, B, b, R1);
-        pipe_gather<P, T, PFLU>(a, R0, B, b, lu, D0);
+        pipe_gather<P, T, LUM>(a, R0, B, b, lu, D0);
         pipe_record(irec, span(l0 + 2), c, B, b, R2);
-        pipe_gather<P, T, PFLU>(a, R1, B, b, lu, D1);
+        pipe_gather<P, T, LUM>(a, R1, B, b, lu, D1);
         pipe_record(irec, span(l0 + 3), c, B, b, R0);
         for (int lv = l0; lv < l1; lv += 3) {
             level(D0, R1, S1, lv + 7, D2, R2);     // solve lv; record lv+4; gather lv+2
@@ -1460,20 +1563,20 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
     }
 }
 
-template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
+template <int P, typename T, int LUM, bool GREAD, int DEPTH, int BF>
 __global__ void __launch_bounds__(PipeThreads<T>::max) sweep_pipe_kernel(SweepArgs<T> a)
 {
-    sweep_pipe_body<P, T, PFLU, GREAD, DEPTH, BF>(a, blockIdx.x);
+    sweep_pipe_body<P, T, LUM, GREAD, DEPTH, BF>(a, blockIdx.x);
 }
 
 // several contexts' sweeps in one launch: workgroup k sweeps block
 // bmap[k].y of context bmap[k].x, whose arguments it reads from args
-template <int P, typename T, bool PFLU, bool GREAD, int DEPTH, int BF>
+template <int P, typename T, int LUM, bool GREAD, int DEPTH, int BF>
 __global__ void __launch_bounds__(PipeThreads<T>::max) sweep_pipe_batch_kernel(const SweepArgs<T>* __restrict__ args,
                                                                const int2* __restrict__ bmap)
 {
     const int2 m = bmap[blockIdx.x];
-    sweep_pipe_body<P, T, PFLU, GREAD, DEPTH, BF>(args[m.x], m.y);
+    sweep_pipe_body<P, T, LUM, GREAD, DEPTH, BF>(args[m.x], m.y);
 }
 
 // the wall vector of each adiabatic wall face from this iterate's emission
@@ -2169,6 +2272,24 @@ __global__ void aat_rows_kernel(const double* __restrict__ aat, int64_t N, int P
     out[g] = aat[(((size_t)i * 3 + l) * 7 * P + p0 + p) * 3 * Q + r];
 }
 
+// the fused sweep's five element scalars (area2 / tau_c, area2 times the
+// gradient coefficients) from the cell operators [N][8] into the cell
+// records' padding at column rf, both precisions
+__global__ void __launch_bounds__(256) fused_scalars_kernel(const double* __restrict__ cell, double tau_c, int N,
+                                                            int rf, double* crec64, int rs64, float* crec32,
+                                                            int rs32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double* c = cell + (size_t)i * 8;
+    const double v[5] = {c[0] / tau_c, c[0] * c[1], c[0] * c[3], c[0] * c[2], c[0] * c[4]};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        crec64[(size_t)i * rs64 + rf + k] = v[k];
+        crec32[(size_t)i * rs32 + rf + k] = (float)v[k];
+    }
+}
+
 template <typename T>
 __global__ void narrow_kernel(const double* __restrict__ in, T* __restrict__ out, int64_t n)
 {
@@ -2640,7 +2761,7 @@ std::vector<double> reversed_edge_matrix(int deg)
 template <int P>
 int pipe_blocks_per_cu(int depth, bool one_dir, int block)
 {
-    constexpr bool pf = P * P * sizeof(float) <= 288;
+    constexpr int pf = lu_stored_mode<P, float>();
     const void* fn =
         depth == 2 ? (one_dir ? reinterpret_cast<const void*>(sweep_pipe_kernel<P, float, pf, false, 2, 1>)
                               : reinterpret_cast<const void*>(sweep_pipe_kernel<P, float, pf, false, 2, 0>))
@@ -2863,7 +2984,39 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         if (kworst > 1e-11)
             throw std::string("face blocks are not a length times the unit-edge mass matrix "
                               "(curved or non-affine faces?)");
+        {
+            // the fused sweep's outflow term: every face mass block
+            // int_tri_fc[i, il] is its length times one unit-edge mass
+            // matrix Kf on the face's nodes (own node order) and zero
+            // elsewhere; otherwise that sweep stays off
+            std::vector<double> kf((size_t)Q * Q, 0.0);
+            bool ok = N > 0 && flen[0] > 0.0;
+            if (ok)
+                for (int r = 0; r < Q; ++r)
+                    for (int q = 0; q < Q; ++q)
+                        kf[(size_t)r * Q + q] = su->fcm[(size_t)face_node(deg, 0, r) * P + face_node(deg, 0, q)] / flen[0];
+            for (int i = 0; i < N && ok; ++i)
+                for (int il = 0; il < 3 && ok; ++il) {
+                    const double* fm = su->fcm + ((size_t)i * 3 + il) * P * P;
+                    const double len = flen[(size_t)i * 3 + il];
+                    double bmax = 0.0, e = 0.0;
+                    std::vector<int> on(P, 0);
+                    for (int r = 0; r < Q; ++r) on[face_node(deg, il, r)] = 1 + r;
+                    for (int m = 0; m < P; ++m)
+                        for (int l = 0; l < P; ++l) {
+                            const double v = fm[m * P + l];
+                            const double want = on[m] && on[l] ? len * kf[(size_t)(on[m] - 1) * Q + on[l] - 1] : 0.0;
+                            bmax = std::max(bmax, std::fabs(v));
+                            e = std::max(e, std::fabs(v - want));
+                        }
+                    ok = bmax > 0.0 && e <= 1e-11 * bmax;
+                }
+            ctx->fused_ok = ok;
+            ctx->h_fref.assign((size_t)3 * P * P + (size_t)Q * Q, 0.0);
+            for (int k = 0; k < Q * Q; ++k) ctx->h_fref[(size_t)3 * P * P + k] = kf[k];
+        }
         ctx->geo64 = ctx->upload(g64.data(), N);
+        ctx->h_flen.assign(flen.begin(), flen.end());
         ctx->K64 = ctx->upload(K.data(), K.size());
         ctx->bvec64 = ctx->upload(bvec.data(), bvec.size());
         ctx->other = ctx->upload(h_other.data(), h_other.size());
@@ -3493,7 +3646,7 @@ template <int P, typename T> size_t sweep_lds_bytes(const pbte_ctx* ctx)
            sizeof(T);
 }
 
-template <int P, typename T, bool PF, bool GR, int DEPTH>
+template <int P, typename T, int PF, bool GR, int DEPTH>
 void launch_pipe(pbte_ctx* ctx, const SweepArgs<T>& a, size_t lds)
 {
     auto* fn = ctx->max_bundle_dirs == 1 ? sweep_pipe_kernel<P, T, PF, GR, DEPTH, 1>
@@ -3542,10 +3695,20 @@ template <typename T> bool factors_fit_descriptor(const pbte_ctx* ctx)
     return true;
 }
 
+template <int P, typename T> bool pipelined_shape(const pbte_ctx* ctx)
+{
+    return ctx->variant != 1 && ctx->block <= PipeThreads<T>::max && sweep_lds_bytes<P, T>(ctx) <= ctx->lds_blk;
+}
+
 template <int P, typename T> bool runs_pipelined(const pbte_ctx* ctx)
 {
-    return ctx->variant != 1 && ctx->block <= PipeThreads<T>::max && sweep_lds_bytes<P, T>(ctx) <= ctx->lds_blk &&
-           factors_fit_descriptor<T>(ctx);
+    return pipelined_shape<P, T>(ctx) && factors_fit_descriptor<T>(ctx);
+}
+
+// a binary32 sweep of this context runs the fused kernel
+template <int P, typename T> bool runs_fused(const pbte_ctx* ctx)
+{
+    return std::is_same<T, float>::value && P <= 10 && ctx->fused32 && pipelined_shape<P, T>(ctx);
 }
 
 template <int P, typename T>
@@ -3554,7 +3717,21 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
     const SweepArgs<T> a = sweep_args<P, T>(ctx, pbuf, vdf, use_pbuf, homogeneous);
     const bool timed = 2 * (size_t)ctx->ev_used + 1 < ctx->events.size();
     if (timed) HIP_TRY(hipEventRecord(ctx->events[2 * ctx->ev_used], ctx->stream));
-    if (onthefly) {
+    if (runs_fused<P, T>(ctx)) {
+        // binary32 with the element operators built and factorised in the
+        // sweep (stored binary32 factors, if any, are not read)
+        if constexpr (std::is_same<T, float>::value && P <= 10) {
+            const size_t lds = sweep_lds_bytes<P, T>(ctx);
+            const bool gread = ctx->global_bundles > 0 || ctx->old_reads > 0 || (use_pbuf && ctx->periodic_reads > 0);
+            if (ctx->depth == 3) {
+                if (gread) launch_pipe<P, T, LU_FUSED, true, 3>(ctx, a, lds);
+                else launch_pipe<P, T, LU_FUSED, false, 3>(ctx, a, lds);
+            } else {
+                if (gread) launch_pipe<P, T, LU_FUSED, true, 2>(ctx, a, lds);
+                else launch_pipe<P, T, LU_FUSED, false, 2>(ctx, a, lds);
+            }
+        }
+    } else if (onthefly) {
         bool refined = false;
         if constexpr (std::is_same<T, double>::value && P <= 10) {
             // binary32 inverses stored (the "mixed" mode): refine from them
@@ -3573,7 +3750,7 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
         hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(std::min(ctx->block, 256)), 0, ctx->stream, a);
     } else {
         const size_t lds = sweep_lds_bytes<P, T>(ctx);
-        constexpr bool pf = P * P * sizeof(T) <= 288;
+        constexpr int pf = lu_stored_mode<P, T>();
         const bool gread = ctx->global_bundles > 0 || ctx->old_reads > 0 || (use_pbuf && ctx->periodic_reads > 0);
         if (ctx->depth == 3) {
             if (gread) launch_pipe<P, T, pf, true, 3>(ctx, a, lds);
@@ -3684,7 +3861,8 @@ extern "C" int pbte_sweep(pbte_ctx* ctx, int32_t precision, const double* ts, co
             run_sweep<double>(ctx, ts, qxs, qys, flux_wall, (const double*)pbuf, (double*)vdf, use_pbuf,
                               homogeneous);
         } else if (precision == PBTE_FP32) {
-            if (!ctx->have32) throw std::string("FP32 factors not built; call pbte_factorise");
+            if (!ctx->have32 && !ctx->fused32)
+                throw std::string("FP32 factors not built; call pbte_factorise or pbte_set_fused");
             run_sweep<float>(ctx, ts, qxs, qys, flux_wall, (const float*)pbuf, (float*)vdf, use_pbuf,
                              homogeneous);
         } else {
@@ -3693,6 +3871,28 @@ extern "C" int pbte_sweep(pbte_ctx* ctx, int32_t precision, const double* ts, co
         return 0;
     });
 }
+
+namespace {
+// the context's reference-element matrices and unit-edge matrix against
+// the fused sweep's compile-time tables
+template <int P> bool fused_tables_match(const std::vector<double>& f)
+{
+    using R = RefElem<P>;
+    constexpr int PP = P * P, Q = Deg<P>::v + 1;
+    double big = 0.0, err = 0.0;
+    for (int k = 0; k < PP; ++k) {
+        const double t[3] = {R::M[k], R::Dx[k], R::Dy[k]};
+        for (int j = 0; j < 3; ++j) {
+            big = std::max(big, std::fabs(t[j]));
+            err = std::max(err, std::fabs(f[(size_t)j * PP + k] - t[j]));
+        }
+    }
+    for (int r = 0; r < Q; ++r)
+        for (int q = 0; q < Q; ++q)
+            err = std::max(err, std::fabs(f[(size_t)3 * PP + r * Q + q] - unit_edge<P>(r, q)));
+    return big > 0.0 && err <= 1e-12 * big;
+}
+}  // namespace
 
 extern "C" int pbte_set_cell_operators(pbte_ctx* ctx, const double* ref, const double* cell)
 {
@@ -3703,6 +3903,50 @@ extern "C" int pbte_set_cell_operators(pbte_ctx* ctx, const double* ref, const d
         if (!ctx->cellop) ctx->cellop = ctx->alloc<double>(nc);
         HIP_TRY(hipMemcpy(ctx->refop, ref, nr * sizeof(double), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(ctx->cellop, cell, nc * sizeof(double), hipMemcpyHostToDevice));
+        // the fused sweep's data: the reference-element mass and derivative
+        // matrices, and per element area2 / tau_c and area2 times the
+        // gradient coefficients (the element tensors are these products:
+        // mass = area2 M, gx = area2 (a D_xi + b D_eta), gy = area2 (c D_xi
+        // + d D_eta)), written into the padding of the cell records
+        const int P = ctx->P, PP = P * P;
+        for (int k = 0; k < 3 * PP; ++k) ctx->h_fref[k] = ref[k];
+        bool tables = false;
+        switch (P) {
+        case 3: tables = fused_tables_match<3>(ctx->h_fref); break;
+        case 6: tables = fused_tables_match<6>(ctx->h_fref); break;
+        case 10: tables = fused_tables_match<10>(ctx->h_fref); break;
+        }
+        ctx->fused_tables = tables;
+        hipLaunchKernelGGL(fused_scalars_kernel, dim3(nblocks(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->cellop,
+                           ctx->tau_c, ctx->N, 3 * P + 9, ctx->crec64, ctx->rs64, ctx->crec32, ctx->rs32);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->have_cellops = true;
+        return 0;
+    });
+}
+
+extern "C" int pbte_set_fused(pbte_ctx* ctx, int32_t on)
+{
+    return guarded(ctx, [&]() {
+        if (!on) {
+            ctx->fused32 = false;
+            return 0;
+        }
+        if (ctx->P > 10) throw std::string("fused sweep: element degree above 3");
+        if (!ctx->have_cellops) throw std::string("fused sweep: pbte_set_cell_operators first");
+        if (!ctx->fused_ok)
+            throw std::string("fused sweep: face mass blocks are not a length times one unit-edge matrix");
+        if (!ctx->fused_tables)
+            throw std::string("fused sweep: the reference-element matrices differ from the compiled tables");
+        bool shape = false;
+        switch (ctx->P) {
+        case 3: shape = pipelined_shape<3, float>(ctx); break;
+        case 6: shape = pipelined_shape<6, float>(ctx); break;
+        case 10: shape = pipelined_shape<10, float>(ctx); break;
+        }
+        if (!shape) throw std::string("fused sweep: the context does not run the pipelined kernel");
+        ctx->fused32 = true;
         return 0;
     });
 }
@@ -3801,7 +4045,9 @@ extern "C" int pbte_batch_pipelined(pbte_batch* bt, int32_t precision)
         bool ok = false;
         switch (bt->P) {
 #define BP(PP) case PP: ok = precision == PBTE_FP64 ? (c->have64 && runs_pipelined<PP, double>(c)) \
-                                                    : (c->have32 && runs_pipelined<PP, float>(c)); break;
+                                                    : (runs_fused<PP, float>(c) == runs_fused<PP, float>(bt->ctxs[0]) && \
+                                                       (runs_fused<PP, float>(c) || (c->have32 && runs_pipelined<PP, float>(c)))); \
+                        break;
             BP(3) BP(6) BP(10) BP(15)
 #undef BP
         }
@@ -3812,7 +4058,7 @@ extern "C" int pbte_batch_pipelined(pbte_batch* bt, int32_t precision)
 
 extern "C" const char* pbte_batch_last_error(pbte_batch* bt) { return bt ? bt->err.c_str() : "null batch"; }
 
-template <int P, typename T, bool PF, bool GR, int DEPTH>
+template <int P, typename T, int PF, bool GR, int DEPTH>
 void launch_pipe_batch(pbte_batch* bt, bool one_dir, size_t lds, hipStream_t st)
 {
     auto* fn = one_dir ? sweep_pipe_batch_kernel<P, T, PF, GR, DEPTH, 1> : sweep_pipe_batch_kernel<P, T, PF, GR, DEPTH, 0>;
@@ -3830,9 +4076,12 @@ void batch_sweep(pbte_batch* bt, T* const* vdf, int homogeneous)
     std::vector<unsigned char> host((size_t)n * sizeof(SweepArgs<T>));
     size_t lds = 0;
     bool gread = false, one_dir = true;
+    const bool fused = runs_fused<P, T>(bt->ctxs[0]);
     for (int k = 0; k < n; ++k) {
         pbte_ctx* c = bt->ctxs[k];
-        if (!runs_pipelined<P, T>(c))
+        if (runs_fused<P, T>(c) != fused)
+            throw std::string("batch: fused and stored-factor binary32 contexts in one launch");
+        if (!fused && !runs_pipelined<P, T>(c))
             throw std::string("batch: a context does not run the pipelined kernel in this precision");
         const SweepArgs<T> a = sweep_args<P, T>(c, nullptr, vdf[k], 0, homogeneous);
         std::memcpy(host.data() + (size_t)k * sizeof(SweepArgs<T>), &a, sizeof(a));
@@ -3847,9 +4096,19 @@ void batch_sweep(pbte_batch* bt, T* const* vdf, int homogeneous)
         HIP_TRY(hipMemcpy(bt->args, host.data(), host.size(), hipMemcpyHostToDevice));
         bt->host_args.swap(host);
     }
-    constexpr bool pf = P * P * sizeof(T) <= 288;
+    constexpr int pf = lu_stored_mode<P, T>();
     const hipStream_t st = bt->ctxs[0]->stream;
-    if (bt->ctxs[0]->depth == 3) {
+    if (fused) {
+        if constexpr (std::is_same<T, float>::value && P <= 10) {
+            if (bt->ctxs[0]->depth == 3) {
+                if (gread) launch_pipe_batch<P, T, LU_FUSED, true, 3>(bt, one_dir, lds, st);
+                else launch_pipe_batch<P, T, LU_FUSED, false, 3>(bt, one_dir, lds, st);
+            } else {
+                if (gread) launch_pipe_batch<P, T, LU_FUSED, true, 2>(bt, one_dir, lds, st);
+                else launch_pipe_batch<P, T, LU_FUSED, false, 2>(bt, one_dir, lds, st);
+            }
+        }
+    } else if (bt->ctxs[0]->depth == 3) {
         if (gread) launch_pipe_batch<P, T, pf, true, 3>(bt, one_dir, lds, st);
         else launch_pipe_batch<P, T, pf, false, 3>(bt, one_dir, lds, st);
     } else {
@@ -3873,7 +4132,7 @@ extern "C" int pbte_batch_sweep(pbte_batch* bt, int32_t precision, const double*
                 if (!c->have64) throw std::string("FP64 factors not built in a batch context");
                 prep_sweep<double>(c, ts[k], qxs[k], qys[k], flux_wall[k]);
             } else if (precision == PBTE_FP32) {
-                if (!c->have32) throw std::string("FP32 factors not built in a batch context");
+                if (!c->have32 && !c->fused32) throw std::string("FP32 factors not built in a batch context");
                 prep_sweep<float>(c, ts[k], qxs[k], qys[k], flux_wall[k]);
             } else {
                 throw std::string("bad precision");

@@ -167,17 +167,14 @@ def face_partners(mesh: Mesh, bc_codes=None, bc_xoff=None, bc_yoff=None):
     return other, off_x, off_y
 
 
-def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
-                    bc_codes=None, bc_xoff=None, bc_yoff=None, threads: int = 0) -> Integrals:
+def reference_element(basis: Basis):
+    """The reference-element integrals every element tensor scales:
+    ``(ref_tri, ref_tri_tri, ref_dx, ref_dy, r1, r2, r3)`` -- the basis
+    integrals, the mass matrix, the two derivative matrices and the three
+    edge matrices (``integration.py:199-260``)."""
     P = basis.ndof_tri
-    Q = basis.ndof_fc
-    deg = basis.deg
     mono = basis.mono
     C = basis.nodfun_tri
-    Cf = basis.nodfun_fc
-    nodes = mesh.nodes
-    n_tris = mesh.n_tris
-    area2 = 2.0 * mesh.tri_area
     ix, iy = mono[:, 0], mono[:, 1]
 
     e_vec = np.array([fac_div(1, int(iy[m]), int(ix[m]) + 1, int(ix[m] + iy[m]) + 2)
@@ -205,14 +202,24 @@ def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
             if im == 0 and il == 0:
                 e_e3[m, l] = 1.0 / float(jm + jl + 1)
 
-    ref_tri = _contract1(C, e_vec)
-    ref_tri_tri = _contract2(C, e_mass, True)
-    ref_dx = _contract2(C, e_dx, True)
-    ref_dy = _contract2(C, e_dy, True)
-    r1 = _contract2(C, e_e1, False)
-    r2 = _contract2(C, e_e2, False)
-    r3 = _contract2(C, e_e3, False)
+    return (_contract1(C, e_vec), _contract2(C, e_mass, True), _contract2(C, e_dx, True),
+            _contract2(C, e_dy, True), _contract2(C, e_e1, False), _contract2(C, e_e2, False),
+            _contract2(C, e_e3, False))
 
+
+def build_integrals(mesh: Mesh, basis: Basis, np_tri: int, np_fc: int,
+                    bc_codes=None, bc_xoff=None, bc_yoff=None, threads: int = 0) -> Integrals:
+    P = basis.ndof_tri
+    Q = basis.ndof_fc
+    deg = basis.deg
+    mono = basis.mono
+    C = basis.nodfun_tri
+    Cf = basis.nodfun_fc
+    nodes = mesh.nodes
+    n_tris = mesh.n_tris
+    area2 = 2.0 * mesh.tri_area
+    sqrt2 = np.sqrt(2.0)
+    ref_tri, ref_tri_tri, ref_dx, ref_dy, r1, r2, r3 = reference_element(basis)
     int_tri = ref_tri[:, None] * area2[None, :]
     int_tri_tri = ref_tri_tri[:, :, None] * area2[None, None, :]
 

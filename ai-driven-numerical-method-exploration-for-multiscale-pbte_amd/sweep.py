@@ -59,6 +59,7 @@ class SweepContext:
         self.use_pbuf = False
         self.pbuf = None
         self.sweep_variant = 0
+        self.fused = False                      # binary32 sweeps build their element operators
         self.one_dir = True
         self._solver = None                     # weak reference to the Solver that owns this context
 
@@ -162,15 +163,33 @@ class SweepContext:
         nbytes = self.operator_bytes(self.n_tris, self.nd)
         self.mode = "onthefly"
         self._precomputed = set()
-        if precompute and self.precision == "fp32" and 1.5 * nbytes > max_bytes >= 0.5 * nbytes:
+        # binary32 sweeps fused: each element operator built and factorised
+        # in the sweep, so only the binary64 factors are stored
+        self.fused = (self.precision == "fp32" and case.performance.fp32_operator == "fused"
+                      and self.set_fused(True))
+        # bytes of the stored factors: binary64, plus binary32 for fp32 runs
+        # that read stored binary32 factors
+        need = nbytes * (1.5 if self.precision == "fp32" and not self.fused else 1.0)
+        if precompute and self.precision == "fp32" and need > max_bytes >= 0.5 * nbytes:
             # binary32 factors fit, binary64 ones beside them do not: the
-            # binary32 sweeps use stored factors, the binary64 sweeps of the
-            # refinement build theirs on the fly
+            # binary32 sweeps use stored (or fused) factors, the binary64
+            # sweeps of the refinement refine from the binary32 ones
             self.factorise("fp32")
             self.mode = "mixed"
-        elif precompute and nbytes * (1.5 if self.precision == "fp32" else 1.0) <= max_bytes:
-            self.factorise("fp64+fp32" if self.precision == "fp32" else "fp64")
+        elif precompute and need <= max_bytes:
+            self.factorise("fp64+fp32" if self.precision == "fp32" and not self.fused else "fp64")
         return self
+
+    def set_fused(self, on: bool) -> bool:
+        """Switch the fused binary32 sweep on (True when it runs here: the
+        context's elements and schedule allow it) or off."""
+        h = self.handle
+        if not on:
+            _native.check(h, _native.lib().pbte_set_fused(h, 0), "set_fused")
+            self.fused = False
+            return False
+        self.fused = _native.lib().pbte_set_fused(h, 1) == 0
+        return self.fused
 
     def __del__(self):
         self.close()
@@ -252,13 +271,14 @@ class SweepContext:
         pb = None
         if self.use_pbuf:
             pb = self.ensure_pbuf(vdf.dtype)
-        stored = ("fp64" if prec == _native.PBTE_FP64 else "fp32") in self._precomputed
+        fused = self.fused and prec == _native.PBTE_FP32
+        stored = ("fp64" if prec == _native.PBTE_FP64 else "fp32") in self._precomputed and not fused
         fn = _native.lib().pbte_sweep if stored else _native.lib().pbte_sweep_onthefly
         st = fn(self.handle, prec, ptr(ts), ptr(qxs), ptr(qys), ptr(vdf), ptr(fw), ptr(pb),
                 1 if self.use_pbuf else 0, 1 if homogeneous else 0)
         _native.check(self.handle, st, "sweep")
         self.sweep_count += 1
-        if self.mode == "onthefly":
+        if self.mode == "onthefly" or fused:
             # every element operator was built and LU-factorised for this sweep
             self.factorisations += self.ndir * self.n_tris
 

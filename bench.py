@@ -60,8 +60,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12
+VALU_F32_PEAK = 157.3e12        # MI355X published FP32 vector peak (packed; 78.6e12 unpacked)
 METRIC = "solves/hour + sweep DOF-updates/s, FP32+refine vs FP64 GPU at 1e-6 rel L2"
 TRAFFIC_FILE = ROOT / "profiles" / "sweep_traffic.json"
+FP32_OPERATOR = "fused"         # --fp32-operator
 
 
 def log(msg):
@@ -107,7 +109,8 @@ def make_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitte
         "boundaries": walls,
         "restart": {"enabled": False, "error_on_stale": False},
         "output": {"field": False, "runtime_log": False, "run_record": False},
-        "performance": {"precision": precision, "refine_tol": 1e-6 if precision == "fp32" else 0.0},
+        "performance": {"precision": precision, "refine_tol": 1e-6 if precision == "fp32" else 0.0,
+                        "fp32_operator": FP32_OPERATOR},
     }, basedir=str(workdir))
 
 
@@ -136,7 +139,7 @@ def gsis_case(pkg, n, deg, npole, nazim, tau_r, tau_n, precision, workdir, jitte
         ],
         "restart": {"enabled": False, "error_on_stale": False},
         "output": {"field": False, "runtime_log": False, "run_record": False},
-        "performance": {"precision": precision},
+        "performance": {"precision": precision, "fp32_operator": FP32_OPERATOR},
     }, basedir=str(workdir))
 
 
@@ -275,10 +278,13 @@ def config2_workload(pkg, args, work):
     setup_s = time.perf_counter() - t0
     D, N, P = s.ndir, s.n_tris, s.ndof
     log(f"solver set up in {setup_s:.1f} s ({N} triangles); sweeping")
-    out = sweep_line(s, args, D, N, P, "fp32",
-                     f"config 2: jittered square {n}x{n} (+-0.25h, seed 0) -> {N} P{args.deg} triangles, "
-                     f"{D} ordinates ({args.npole}x{args.nazim}), Kn_R 1e-3, Kn_N 1e-4, fp32 sweep")
+    wl = (f"config 2: jittered square {n}x{n} (+-0.25h, seed 0) -> {N} P{args.deg} triangles, "
+          f"{D} ordinates ({args.npole}x{args.nazim}), Kn_R 1e-3, Kn_N 1e-4, fp32 sweep")
+    out = sweep_line(s, args, D, N, P, "fp32", wl)
     out["setup_s"] = setup_s
+    if s.ctx.fused and not args.no_stored:
+        log("the same sweep with stored binary32 inverses")
+        out["sweep_stored"] = stored_sweep(s, args, D, N, P, wl)
     del s
     import gc
     gc.collect()
@@ -294,28 +300,73 @@ def config2_workload(pkg, args, work):
     print(json.dumps(out), flush=True)
 
 
-def sweep_line(s, args, D, N, P, precision, workload):
-    """Time args.steps sweeps of a set-up solver (after args.warmup) and
-    build the JSON line's sweep fields."""
+def timed_sweeps(s, steps, warmup):
+    """(wall seconds, per-launch kernel ms) of `steps` sweeps after `warmup`."""
     import torch
     vdf = s.sweep_vdf
-    rng = np.random.default_rng(0)
-    s.mom.set_dofs(rng.random((P, N)), 0.1 * rng.random((P, N)), 0.1 * rng.random((P, N)))
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         s.ctx.sweep(s.mom, vdf)
-    s.ctx.timing(args.steps)
+    s.ctx.timing(steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         s.ctx.sweep(s.mom, vdf)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = s.ctx.timing_read(args.steps)
+    ms = s.ctx.timing_read(steps)
     s.ctx.timing(0)
-    t_sweep = float(np.mean(ms)) * 1e-3
-    dof = D * N * P
+    return wall, ms
+
+
+def roofline(s, D, N, P, precision, t_sweep, workload):
+    """The sweep's roofline object.  `achieved` prices the workload at the
+    survey's algorithmic bytes of the precomputed-LU sweep (B_alg) over the
+    measured kernel time; a fused binary32 sweep (element operators built
+    and factorised in registers) reads none of B_alg's factor bytes, so its
+    own bytes and its VALU rate are given beside it."""
     s_lu = 4 if precision == "fp32" else 8
     bytes_alg = b_alg(D, N, P, s_lu, s_lu)
+    out = {"bound": "hbm", "achieved": bytes_alg / t_sweep / 1e9, "peak": HBM_PEAK / 1e9,
+           "unit": "GB/s", "frac": bytes_alg / t_sweep / HBM_PEAK,
+           "traffic": committed_traffic(workload), "bytes_alg_per_launch": bytes_alg}
+    if precision == "fp32" and s.ctx.fused:
+        own = b_alg(D, N, P, 0, s_lu)
+        flops = D * N * fused_flops({3: 1, 6: 2, 10: 3, 15: 4}[P])
+        out.update({
+            "basis": "B_alg of the precomputed-LU sweep of this workload over the fused sweep's time",
+            "kernel_bytes_alg_per_launch": own, "kernel_bytes_frac": own / t_sweep / HBM_PEAK,
+            "valu": {"flops_per_launch": flops, "achieved": flops / t_sweep / 1e12,
+                     "peak": VALU_F32_PEAK / 1e12, "unit": "TFLOP/s", "frac": flops / t_sweep / VALU_F32_PEAK}})
+    return out
+
+
+def fused_flops(deg):
+    """Floating-point operations of one fused binary32 element solve (item):
+    the right-hand side (source, three inflow faces), A from the nonzero
+    reference-element entries and the three outflow faces, elimination
+    without pivoting and back substitution; an FMA counts two."""
+    from pbte_amd.basis import build_basis
+    from pbte_amd.integration import reference_element
+    basis = build_basis(deg)
+    P, Q = basis.ndof_tri, basis.ndof_fc
+    _, m, dx, dy, _, _, _ = reference_element(basis)
+    nz = sum(int(np.count_nonzero(np.abs(a) > 1e-14 * np.abs(a).max())) for a in (m, dx, dy))
+    face = 5 + 2 * Q * Q + 2 * Q
+    rhs = 5 * P + 3 * face
+    build = 6 + 2 * nz + 3 * (5 + 2 * Q * Q)
+    lu = sum(1 + (P - k - 1) * (3 + 2 * (P - k - 1)) for k in range(P))
+    back = sum(2 * (P - 1 - i) + 1 for i in range(P))
+    return rhs + build + lu + back
+
+
+def sweep_line(s, args, D, N, P, precision, workload):
+    """Time args.steps sweeps of a set-up solver (after args.warmup) and
+    build the JSON line's sweep fields."""
+    rng = np.random.default_rng(0)
+    s.mom.set_dofs(rng.random((P, N)), 0.1 * rng.random((P, N)), 0.1 * rng.random((P, N)))
+    wall, ms = timed_sweeps(s, args.steps, args.warmup)
+    t_sweep = float(np.mean(ms)) * 1e-3
+    dof = D * N * P
     return {
         "metric": METRIC,
         "value": dof * args.steps / wall,
@@ -326,16 +377,29 @@ def sweep_line(s, args, D, N, P, precision, workload):
         "dtype": "f32" if precision == "fp32" else "f64",
         "data": "synthetic moments on the workload's mesh and ordinates",
         "config": {"workload": workload, "n_tris": N, "ndof": P, "ndir": D, "dof_per_sweep": dof},
-        "roofline": {"bound": "hbm", "achieved": bytes_alg / t_sweep / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": bytes_alg / t_sweep / HBM_PEAK,
-                     "traffic": committed_traffic(workload),
-                     "bytes_alg_per_launch": bytes_alg},
+        "roofline": roofline(s, D, N, P, precision, t_sweep, workload),
         "sweep_kernel_ms_mean": t_sweep * 1e3,
         "sweep_kernel_ms_min": float(np.min(ms)),
         "schedule": s.ctx.schedule_info(),
         "tile_shape": s.tile_shape,
         "factor_mode": s.ctx.mode,
+        "fp32_operator": "fused" if precision == "fp32" and s.ctx.fused else "stored",
     }
+
+
+def stored_sweep(s, args, D, N, P, workload):
+    """The same binary32 sweep with stored binary32 inverses (formed in
+    binary64, read by the kernel): its time and HBM roofline beside the
+    fused sweep's."""
+    s.ctx.set_fused(False)
+    s.ctx.factorise("fp32")
+    wall, ms = timed_sweeps(s, args.steps, args.warmup)
+    t = float(np.mean(ms)) * 1e-3
+    bytes_alg = b_alg(D, N, P, 4, 4)
+    return {"sweep_kernel_ms_mean": t * 1e3, "dof_updates_per_s": D * N * P / t,
+            "roofline": {"bound": "hbm", "achieved": bytes_alg / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": bytes_alg / t / HBM_PEAK, "traffic": committed_traffic(workload + ", stored"),
+                         "bytes_alg_per_launch": bytes_alg}}
 
 
 def cis_workload(pkg, args, work):
@@ -637,6 +701,10 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="pick the sweep's tile shapes by timing them (performance.autotune)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stored", action="store_true",
+                    help="config2: skip timing the stored-inverse binary32 sweep beside the fused one")
+    ap.add_argument("--fp32-operator", default="fused", choices=["fused", "stored"],
+                    help="binary32 sweeps: element operators fused into the sweep or stored inverses")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true", help="skip the Krylov solve leg")
     ap.add_argument("--solve-kn", default="0.01,0.1,1,10",
@@ -660,6 +728,8 @@ def main():
     from __graft_entry__ import load_package
     pkg = load_package()
     import torch
+    global FP32_OPERATOR
+    FP32_OPERATOR = args.fp32_operator
 
     work = Path(tempfile.mkdtemp(prefix="pbte_bench_"))
     big = args.workload in ("config2", "gsis", "trace")
@@ -721,16 +791,14 @@ def main():
         "data": "synthetic moments on the workload's mesh and ordinates",
         "config": {"workload": workload, "n_tris": N, "ndof": P, "ndir": D, "dof_per_sweep": dof,
                    "jitter": args.jitter},
-        "roofline": {"bound": "hbm", "achieved": bytes_alg / t_sweep / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": bytes_alg / t_sweep / HBM_PEAK,
-                     "traffic": committed_traffic(workload),
-                     "bytes_alg_per_launch": bytes_alg},
+        "roofline": roofline(s, D, N, P, args.precision, t_sweep, workload),
         "sweep_kernel_ms_mean": t_sweep * 1e3,
         "sweep_kernel_ms_min": float(np.min(ms)),
         "schedule": s.ctx.schedule_info(),
         "tile_shape": s.tile_shape,
         "setup_s": setup_s,
         "factor_mode": s.ctx.mode,
+        "fp32_operator": "fused" if args.precision == "fp32" and s.ctx.fused else "stored",
     }
     if s.ctx.mode == "mixed":
         # binary64 sweeps of the refinement build their operators on the fly

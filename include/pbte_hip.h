@@ -207,8 +207,23 @@ int pbte_periodic_scatter(pbte_ctx *ctx, int32_t precision, const double *in, vo
  * coefficients (y3-y1)/s, (y1-y2)/s, (x1-x3)/s, (x2-x1)/s, and the edge
  * lengths len12, len23/sqrt(2), len31.  With both set and binary32 factors
  * built, pbte_sweep_onthefly in binary64 applies inv32 refined twice
- * against A formed from these. */
+ * against A formed from these.  The same data (ref's first three matrices,
+ * per element area2/tau_c and area2 times the four gradient coefficients)
+ * feeds the fused binary32 sweep of pbte_set_fused. */
 int pbte_set_cell_operators(pbte_ctx *ctx, const double *ref, const double *cell);
+
+/* The reference's on-the-fly mode (sweep.py:104-110, _kernels.py:242-269:
+ * every element operator built and factorised in the sweep) for binary32
+ * sweeps, fused into the pipelined sweep: on != 0 makes every binary32
+ * pbte_sweep / pbte_sweep_onthefly (and batched binary32 sweep) build A
+ * from the reference-element matrices and the element's scalars in
+ * registers and solve it by elimination without pivoting (A + A^T is
+ * positive definite), so no binary32 factors are stored or read.  Needs
+ * pbte_set_cell_operators first, P <= 10, straight-sided elements (every
+ * face mass block a length times one unit-edge matrix, checked when the
+ * context is created) and a context that runs the pipelined kernel; else
+ * -1 with the reason in pbte_last_error.  on == 0 switches it off. */
+int pbte_set_fused(pbte_ctx *ctx, int32_t on);
 
 /* ---- batched sweeps (no counterpart in the reference) ------------------
  * The sweeps of several independent problems in one launch of the

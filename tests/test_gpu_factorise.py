@@ -64,8 +64,9 @@ def test_reference_matrix_factors_equal_tensor_factors(pkg, deg):
     c = pkg.Case.from_yaml(CASES / "F2_1_1.yaml")
     c.dg.deg = deg
     c.performance.precision = "fp32"
+    c.performance.fp32_operator = "stored"
     s = pkg.Solver(c)
-    assert s.ctx.mode == "precomputed"
+    assert s.ctx.mode == "precomputed" and not s.ctx.fused
     plain = dataclasses.replace(s.integrals, ref_ops=None, cell_ops=None)
     print(f"case factor paths deg {deg}", flush=True)
     old = sweep.SweepContext.build(c, s.mesh, plain, s.vel, s.order, s.bcdata, precompute=False)
