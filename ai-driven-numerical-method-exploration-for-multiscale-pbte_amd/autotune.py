@@ -125,6 +125,10 @@ def tune(case, problem, candidates=CANDIDATES, reps: int = 4, max_bytes: float |
         finally:
             ctx.close()
     if not timed:
+        # remembered: every later solver of this signature keeps the
+        # default schedule without building the candidates again
+        _memo[memo_key] = None
+        torch.cuda.empty_cache()
         return None
     best = min(timed, key=lambda r: r["ms"])
     entry = {**best, "timed": timed, "candidates": [list(c) for c in cands]}

@@ -38,12 +38,14 @@ __all__ = ["Solver", "HostVdf"]
 
 class _Mirror:
     """Which copy of a solver's distribution is newer: the host array
-    (``dirty``, written through ``Solver.vdf``) or the device tensor."""
+    (``dirty``, written through ``Solver.vdf``) or the device tensor
+    (``stale``: a device step changed it since the host array was read)."""
 
     def __init__(self, solver):
         self._solver = weakref.ref(solver)     # no cycle: the solver frees its HBM on del
         self.array = None
         self.dirty = False
+        self.stale = False
 
     @property
     def solver(self):
@@ -60,6 +62,14 @@ class _Mirror:
         """The device state into the host array, in place."""
         np.asarray(self.array)[...] = self.solver.sweep_vdf.detach().to(torch.float64).cpu().numpy()
         self.dirty = False
+        self.stale = False
+
+    def refresh(self) -> None:
+        """Before a host write: a held array that device steps have left
+        behind is brought up to date first, so the write lands on the
+        current distribution (as on the reference's live array)."""
+        if self.stale and self.array is not None and not self.dirty:
+            self.pull()
 
 
 class HostVdf(np.ndarray):
@@ -80,7 +90,12 @@ class HostVdf(np.ndarray):
         if self._mirror is not None:
             self._mirror.dirty = True
 
+    def _refresh(self):
+        if self._mirror is not None:
+            self._mirror.refresh()
+
     def __setitem__(self, key, value):
+        self._refresh()
         super().__setitem__(key, value)
         self._touch()
 
@@ -89,6 +104,7 @@ def _inplace(name):
     base = getattr(np.ndarray, name)
 
     def op(self, other):
+        self._refresh()
         out = base(self, other)
         self._touch()
         return out
@@ -209,6 +225,7 @@ class Solver:
         if self.sweep_precision == "fp32":
             self.vdf32_d.copy_(self.vdf_d)
         self._mirror.dirty = False
+        self._mirror.stale = True
 
     def device_vdf(self, vdf):
         """The device tensor a device operation runs on for ``vdf``, and
@@ -274,7 +291,14 @@ class Solver:
         if self.acc is not None:
             self.acc.settle()
 
+    def device_changed(self) -> None:
+        """Host writes through ``vdf`` reach the device first; the host
+        array is behind the device after (``step``, ``run``, device sweeps)."""
+        self._mirror.push()
+        self._mirror.stale = True
+
     def step(self) -> None:
+        self.device_changed()
         v = self.sweep_vdf
         if self.acc is not None:
             self.acc.settle(boundary_only=True)
@@ -289,6 +313,7 @@ class Solver:
             self.acc.apply()
 
     def run(self, callback=None, progress_every: int = 0) -> RunRecord:
+        self.device_changed()
         if self.case.scheme.method == "krylov":
             from .krylov import run_krylov
             return run_krylov(self, callback, progress_every)

@@ -260,6 +260,9 @@ class SweepContext:
         ``src`` optionally overrides the moment source with a tuple of device
         ``[P, N]`` binary64 tensors ``(ts, qxs, qys)``."""
         vdf, back = self._device(vdf)
+        solver = self._solver() if back is None and self._solver is not None else None
+        if solver is not None and vdf is solver.sweep_vdf:
+            solver.device_changed()             # the owner's device state moves on
         self._sweep(mom, vdf, homogeneous, flux_wall, src)
         if back is not None:
             back()

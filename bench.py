@@ -280,6 +280,8 @@ def config2_workload(pkg, args, work):
     log(f"solver set up in {setup_s:.1f} s ({N} triangles); sweeping")
     wl = (f"config 2: jittered square {n}x{n} (+-0.25h, seed 0) -> {N} P{args.deg} triangles, "
           f"{D} ordinates ({args.npole}x{args.nazim}), Kn_R 1e-3, Kn_N 1e-4, fp32 sweep")
+    if not s.ctx.fused:
+        wl += ", stored"
     out = sweep_line(s, args, D, N, P, "fp32", wl)
     out["setup_s"] = setup_s
     if s.ctx.fused and not args.no_stored:
@@ -780,6 +782,8 @@ def main():
     hole = f" with a central hole r={args.hole:g}" if args.hole > 0 else ""
     workload = (f"square {args.n}x{args.n}{hole} -> {N} P{args.deg} triangles, "
                 f"{D} ordinates ({args.npole}x{args.nazim}), {args.precision} sweep")
+    if args.precision == "fp32" and not s.ctx.fused:
+        workload += ", stored"
     out = {
         "metric": METRIC,
         "value": dof * args.steps / wall,

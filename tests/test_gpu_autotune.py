@@ -46,3 +46,45 @@ def test_autotuned_run_is_the_untuned_run(pkg, tmp_path, prec):
     autotune._memo.clear()
     again = autotune.tune(tuned, s.problem)
     assert again == table[key]
+
+
+def test_mixed_mode_autotune_is_bitwise_untuned(pkg, tmp_path):
+    """An fp32 case whose binary64 factors do not fit ("mixed" factor
+    mode): the tuner times its binary32 sweeps, and the tuned run is the
+    untuned run bit for bit."""
+    from pbte_amd.problem import build_problem
+    base = _case(pkg, "F2_1_1", performance__precision="fp32", iteration__tmax=20)
+    pb = build_problem(base)
+    nbytes = float(pb.vel.ndir) * pb.mesh.n_tris * base.ndof_tri ** 2 * 8.0
+    tuned = _case(pkg, "F2_1_1", performance__precision="fp32", performance__autotune=True,
+                  iteration__tmax=20)
+    s = pkg.Solver(tuned, max_bytes=0.9 * nbytes)
+    assert s.ctx.mode == "mixed"
+    assert s.tile_shape is not None
+    r1 = s.run()
+    r0 = pkg.Solver(base, max_bytes=0.9 * nbytes).run()
+    np.testing.assert_array_equal(r1.residual_history, r0.residual_history)
+    np.testing.assert_array_equal(r1.temp_dofs, r0.temp_dofs)
+
+
+def test_untimeable_case_keeps_default_schedule_once(pkg, monkeypatch):
+    """An fp64 on-the-fly case stores no factors: nothing can be timed, the
+    tuner returns None (the default schedule is kept), and a second solver
+    of the same case does not build the candidates again."""
+    from pbte_amd import autotune, sweep
+    c = _case(pkg, "F1_1_1", performance__autotune=True, performance__precompute_inverse=False,
+              iteration__tmax=3)
+    autotune._memo.clear()
+    s = pkg.Solver(c)
+    assert s.tile_shape is None and s.ctx.mode == "onthefly"
+    built = []
+    orig = sweep.SweepContext.build.__func__
+
+    def counting(cls, *a, **kw):
+        built.append(kw.get("max_bundle"))
+        return orig(cls, *a, **kw)
+    monkeypatch.setattr(sweep.SweepContext, "build", classmethod(counting))
+    s2 = pkg.Solver(c)
+    assert s2.tile_shape is None
+    assert len(built) == 1                   # the solver's own context only
+    assert s2.run().iterations == 3

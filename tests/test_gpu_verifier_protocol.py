@@ -73,3 +73,26 @@ def test_host_array_sweep_is_in_place_and_equals_the_device_sweep(pkg):
     s.ctx.moments(v, s.mom)
     assert np.array_equal(s.vdf_d[3].cpu().numpy(), 2.0 * got[3])
     assert np.array_equal(s.vdf_d[2].cpu().numpy(), got[2])
+
+
+def test_held_host_distribution_follows_device_steps(pkg):
+    """A ``solver.vdf`` reference held across device steps behaves as the
+    reference's live array: a partial write after ``step()`` lands on the
+    stepped distribution (not on the copy read before the steps), and the
+    next step sweeps from it."""
+    c = pkg.Case.from_yaml(CASES / "F2_1_1.yaml")
+    c.output.field = c.output.run_record = c.output.runtime_log = False
+    a, b = pkg.Solver(c), pkg.Solver(c)
+    held = a.vdf
+    for s in (a, b):
+        s.step()
+        s.step()
+    held[0, 0, 0] = 0.25                     # write through the held array
+    want = np.array(b.vdf)
+    want[0, 0, 0] = 0.25
+    assert np.array_equal(np.asarray(a.vdf), want)
+    b.vdf[0, 0, 0] = 0.25
+    a.step()
+    b.step()
+    assert np.array_equal(np.asarray(a.vdf), np.asarray(b.vdf))
+    assert np.array_equal(np.asarray(held), np.asarray(a.vdf))
