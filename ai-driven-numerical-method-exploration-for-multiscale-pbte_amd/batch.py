@@ -16,8 +16,9 @@ import time
 
 from . import _native
 from .device import dev_zeros, ptr, torch
+from .krylov import SweepRequest
 
-__all__ = ["SweepBatch", "launch_key", "run_source_batch"]
+__all__ = ["SweepBatch", "launch_key", "run_refined_batch", "run_source_batch"]
 
 _PREC = {torch.float64: _native.PBTE_FP64, torch.float32: _native.PBTE_FP32}
 
@@ -61,36 +62,59 @@ class SweepBatch:
         Contexts that do not run the pipelined kernel in this precision
         (factors built on the fly, the plain level kernel) are swept one by
         one, each as ``SweepContext.sweep`` does."""
+        self.sweep_requests([SweepRequest(c, v, homogeneous, None, None)
+                             for c, v in zip(self.contexts, vdfs)], moms)
+
+    def sweep_requests(self, reqs, moms=None) -> None:
+        """The sweeps ``reqs[k]`` (``krylov.SweepRequest`` of context k: its
+        distribution, moment source -- ``src`` or ``moms[k]`` --, flux_wall,
+        and the explicit periodic buffer when the context's ``use_pbuf`` is
+        set), all of one precision and one ``homogeneous`` flag, in one
+        launch when every context runs the pipelined kernel; else one by
+        one.  Each context's result is its ``SweepContext.sweep``, bit for
+        bit."""
         n = len(self.contexts)
-        prec = _PREC[vdfs[0].dtype]
-        if any(_PREC[v.dtype] != prec for v in vdfs):
-            raise ValueError("a batched sweep runs one precision")
+        prec = _PREC[reqs[0].vdf.dtype]
+        hom = reqs[0].homogeneous
+        if any(_PREC[r.vdf.dtype] != prec or r.homogeneous != hom for r in reqs):
+            raise ValueError("a batched sweep runs one precision and one source flag")
         if not self.pipelined(prec):
-            for c, m, v in zip(self.contexts, moms, vdfs):
-                c.sweep(m, v, homogeneous=homogeneous)
+            for k, r in enumerate(reqs):
+                r.ctx.sweep(None if moms is None else moms[k], r.vdf, homogeneous=hom,
+                            flux_wall=r.flux_wall, src=r.src)
             return
 
         def arr(vals):
             return (C.c_void_p * n)(*vals)
 
-        ts = arr([ptr(m.ts_d) for m in moms])
-        qx = arr([ptr(m.qxs_d) for m in moms])
-        qy = arr([ptr(m.qys_d) for m in moms])
-        vd = arr([ptr(v) for v in vdfs])
-        fw = arr([ptr(c.flux_wall) for c in self.contexts])
-        st = _native.lib().pbte_batch_sweep(self.handle, prec, ts, qx, qy, vd, fw, 1 if homogeneous else 0)
+        srcs = [r.src if r.src is not None else (moms[k].ts_d, moms[k].qxs_d, moms[k].qys_d)
+                for k, r in enumerate(reqs)]
+        ts = arr([ptr(s[0]) for s in srcs])
+        qx = arr([ptr(s[1]) for s in srcs])
+        qy = arr([ptr(s[2]) for s in srcs])
+        vd = arr([ptr(r.vdf) for r in reqs])
+        fw = arr([ptr(r.ctx.flux_wall if r.flux_wall is None else r.flux_wall) for r in reqs])
+        pb = arr([ptr(r.ctx.ensure_pbuf(r.vdf.dtype)) if r.ctx.use_pbuf else None for r in reqs])
+        for r in reqs:
+            solver = r.ctx._solver() if r.ctx._solver is not None else None
+            if solver is not None and r.vdf is solver.sweep_vdf:
+                solver.device_changed()
+        st = _native.lib().pbte_batch_sweep(self.handle, prec, ts, qx, qy, vd, fw, pb, 1 if hom else 0)
         if st < 0:
             msg = _native.lib().pbte_batch_last_error(self.handle)
             raise _native.NativeError(f"batch_sweep: {msg.decode() if msg else 'unknown error'}")
-        for c in self.contexts:
-            c.sweep_count += 1
+        for r in reqs:
+            r.ctx.sweep_count += 1
+            if r.ctx.fused and prec == _native.PBTE_FP32:
+                r.ctx.factorisations += r.ctx.ndir * r.ctx.n_tris
 
 
 def launch_key(ctx) -> tuple:
     """Contexts with equal keys share a launch: one kernel instance
     (element degree, pipeline depth, and whether every bundle holds one
-    direction, whose instance drops the bundle arithmetic)."""
-    return (ctx.nd, 2 if ctx.sweep_variant == 2 else 3, ctx.one_dir)
+    direction, whose instance drops the bundle arithmetic, and whether its
+    binary32 sweeps are fused)."""
+    return (ctx.nd, 2 if ctx.sweep_variant == 2 else 3, ctx.one_dir, ctx.fused)
 
 
 class _Track:
@@ -210,3 +234,70 @@ def run_source_batch(solvers, callback=None):
         recs.append(s._record(t.step, t.converged, t.residuals, t.masses, None, t.wall,
                               scheme="GSIS" if s.case.scheme.accflag == 1 else "CIS", extra=extra))
     return recs
+
+
+def run_refined_batch(solvers, callback=None):
+    """The fp32 source iteration as a binary64 defect correction
+    (``refine.refined_steps``, CIS or GSIS) of every solver in lockstep;
+    returns their ``RunRecord`` objects in order.
+
+    Every problem advances to its next request: the sweeps asked for are
+    grouped by precision, source flag and ``launch_key`` and each group is
+    swept in one launch; the residual reads of all problems share one
+    wait.  Each problem's iterates are those of its refined iteration alone,
+    bit for bit.  ``callback(k, step, res, solver)`` after each recorded
+    step of problem k."""
+    from .krylov import SweepRequest as _Sweep
+    from .refine import ReadRequest, applicable, refined_steps
+    for s in solvers:
+        if s.precision != "fp32" or s.sweep_precision != "fp32" or s.case.scheme.method == "krylov":
+            raise ValueError("run_refined_batch runs fp32 source iteration (refined)")
+        ok, why = applicable(s)
+        if not ok:
+            raise ValueError(f"run_refined_batch: {why}")
+    gens = {}
+    for k, s in enumerate(solvers):
+        cb = None if callback is None else (lambda step, res, solver, k=k: callback(k, step, res, solver))
+        gens[k] = refined_steps(s, cb)
+    msgs = {k: None for k in gens}
+    recs = {}
+    batches = {}
+
+    def batch_of(group):
+        key = tuple(group)
+        if key not in batches:
+            batches[key] = SweepBatch([solvers[k].ctx for k in group])
+        return batches[key]
+
+    try:
+        while gens:
+            reqs = {}
+            for k, g in list(gens.items()):
+                try:
+                    reqs[k] = g.send(msgs[k])
+                except StopIteration as stop:
+                    recs[k] = stop.value
+                    del gens[k]
+            groups = {}
+            reads = []
+            for k, r in reqs.items():
+                if isinstance(r, _Sweep):
+                    key = (r.vdf.dtype, r.homogeneous) + launch_key(r.ctx)
+                    groups.setdefault(key, []).append(k)
+                    msgs[k] = None
+                elif isinstance(r, ReadRequest):
+                    reads.append(k)
+                else:
+                    raise TypeError(f"unknown request {type(r).__name__}")
+            for group in groups.values():
+                batch_of(group).sweep_requests([reqs[k] for k in group])
+            if reads:
+                vals = torch.stack([reqs[k].slot for k in reads]).cpu().numpy()
+                for k, (res, mass) in zip(reads, vals):
+                    msgs[k] = (float(res), float(mass))
+    finally:
+        for g in gens.values():
+            g.close()
+        for b in batches.values():
+            b.close()
+    return [recs[k] for k in range(len(solvers))]

@@ -1569,15 +1569,31 @@ __global__ void __launch_bounds__(PipeThreads<T>::max) sweep_pipe_kernel(SweepAr
     sweep_pipe_body<P, T, LUM, GREAD, DEPTH, BF>(a, blockIdx.x);
 }
 
-// several contexts' sweeps in one launch: workgroup k sweeps block
-// bmap[k].y of context bmap[k].x, whose arguments it reads from args
+// several contexts' sweeps in one persistent launch fed by a device work
+// queue: the work items are the (context, block) pairs of bmap, longest
+// bundles first; each resident workgroup takes the next item from the
+// queue head (one atomic per item), sweeps block bmap[k].y of context
+// bmap[k].x with that context's arguments, and stops once the queue is
+// empty.  The head is zeroed by queue_reset_kernel before the launch.
 template <int P, typename T, int LUM, bool GREAD, int DEPTH, int BF>
-__global__ void __launch_bounds__(PipeThreads<T>::max) sweep_pipe_batch_kernel(const SweepArgs<T>* __restrict__ args,
-                                                               const int2* __restrict__ bmap)
+__global__ void __launch_bounds__(PipeThreads<T>::max) sweep_queue_kernel(const SweepArgs<T>* __restrict__ args,
+                                                          const int2* __restrict__ bmap, int nitems,
+                                                          int* __restrict__ head)
 {
-    const int2 m = bmap[blockIdx.x];
-    sweep_pipe_body<P, T, LUM, GREAD, DEPTH, BF>(args[m.x], m.y);
+    __shared__ int next;
+    for (;;) {
+        // every wave is past the previous item (its LDS pool and `next`)
+        __syncthreads();
+        if (threadIdx.x == 0) next = atomicAdd(head, 1);
+        __syncthreads();
+        const int k = next;
+        if (k >= nitems) break;
+        const int2 m = bmap[k];
+        sweep_pipe_body<P, T, LUM, GREAD, DEPTH, BF>(args[m.x], m.y);
+    }
 }
+
+__global__ void queue_reset_kernel(int* head) { *head = 0; }
 
 // the wall vector of each adiabatic wall face from this iterate's emission
 // flux_wall (bc.py:47-58, _kernels.py:364-397), over the face length
@@ -3972,20 +3988,23 @@ extern "C" int pbte_sweep_onthefly(pbte_ctx* ctx, int32_t precision, const doubl
 
 // ==========================================================================
 // batched sweeps: the bundles of several contexts (independent problems of
-// one element degree, built on one stream) in one launch of the pipelined
-// kernel; each workgroup finds its context and bundle in a block map
+// one element degree, built on one stream) in one persistent launch of the
+// pipelined kernel, whose workgroups take (context, bundle) work items from
+// a device queue, longest first
 // ==========================================================================
 struct pbte_batch {
     std::vector<pbte_ctx*> ctxs;
-    int2* bmap = nullptr;           // device [blocks] {context, block}
+    int2* bmap = nullptr;           // device [blocks] {context, block}: the work items, longest first
     void* args = nullptr;           // device SweepArgs<T>[n]
+    int* head = nullptr;            // device work-queue head
     std::vector<unsigned char> host_args;
-    int nblk = 0, block = 0, P = 0;
+    int nblk = 0, block = 0, P = 0, ncu = 0;
     std::string err;
     ~pbte_batch()
     {
         if (bmap) (void)hipFree(bmap);
         if (args) (void)hipFree(args);
+        if (head) (void)hipFree(head);
     }
 };
 
@@ -4029,6 +4048,10 @@ extern "C" pbte_batch* pbte_batch_create(pbte_ctx* const* ctxs, int32_t n, char*
         HIP_TRY(hipMemcpy(bt->bmap, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice));
         const size_t abytes = (size_t)n * std::max(sizeof(SweepArgs<double>), sizeof(SweepArgs<float>));
         HIP_TRY(hipMalloc(&bt->args, abytes));
+        HIP_TRY(hipMalloc((void**)&bt->head, sizeof(int)));
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&bt->ncu, hipDeviceAttributeMultiprocessorCount, dev));
         return bt;
     } catch (const std::string& m) {
         delete bt;
@@ -4061,16 +4084,26 @@ extern "C" const char* pbte_batch_last_error(pbte_batch* bt) { return bt ? bt->e
 template <int P, typename T, int PF, bool GR, int DEPTH>
 void launch_pipe_batch(pbte_batch* bt, bool one_dir, size_t lds, hipStream_t st)
 {
-    auto* fn = one_dir ? sweep_pipe_batch_kernel<P, T, PF, GR, DEPTH, 1> : sweep_pipe_batch_kernel<P, T, PF, GR, DEPTH, 0>;
+    auto* fn = one_dir ? sweep_queue_kernel<P, T, PF, GR, DEPTH, 1> : sweep_queue_kernel<P, T, PF, GR, DEPTH, 0>;
     if (lds > 65536)
         HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
-    hipLaunchKernelGGL(fn, dim3(bt->nblk), dim3(bt->block), lds, st, static_cast<const SweepArgs<T>*>(bt->args),
-                       bt->bmap);
+    // as many workgroups as the CUs hold at once (registers and LDS), at
+    // most one per work item
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn), bt->block, lds) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        per_cu = 1;
+    }
+    const int grid = std::max(1, std::min(bt->nblk, std::max(per_cu, 1) * std::max(bt->ncu, 1)));
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, st, bt->head);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(bt->block), lds, st, static_cast<const SweepArgs<T>*>(bt->args),
+                       bt->bmap, bt->nblk, bt->head);
 }
 
 template <int P, typename T>
-void batch_sweep(pbte_batch* bt, T* const* vdf, int homogeneous)
+void batch_sweep(pbte_batch* bt, T* const* vdf, const void* const* pbuf, int homogeneous)
 {
     const int n = (int)bt->ctxs.size();
     std::vector<unsigned char> host((size_t)n * sizeof(SweepArgs<T>));
@@ -4083,10 +4116,12 @@ void batch_sweep(pbte_batch* bt, T* const* vdf, int homogeneous)
             throw std::string("batch: fused and stored-factor binary32 contexts in one launch");
         if (!fused && !runs_pipelined<P, T>(c))
             throw std::string("batch: a context does not run the pipelined kernel in this precision");
-        const SweepArgs<T> a = sweep_args<P, T>(c, nullptr, vdf[k], 0, homogeneous);
+        const T* pb = pbuf ? static_cast<const T*>(pbuf[k]) : nullptr;
+        const int use_pbuf = pb != nullptr;
+        const SweepArgs<T> a = sweep_args<P, T>(c, pb, vdf[k], use_pbuf, homogeneous);
         std::memcpy(host.data() + (size_t)k * sizeof(SweepArgs<T>), &a, sizeof(a));
         lds = std::max(lds, sweep_lds_bytes<P, T>(c));
-        gread = gread || c->global_bundles > 0 || c->old_reads > 0;
+        gread = gread || c->global_bundles > 0 || c->old_reads > 0 || (use_pbuf && c->periodic_reads > 0);
         one_dir = one_dir && c->max_bundle_dirs == 1;
     }
     // the arguments change only with the distributions and the source flag:
@@ -4120,7 +4155,7 @@ void batch_sweep(pbte_batch* bt, T* const* vdf, int homogeneous)
 
 extern "C" int pbte_batch_sweep(pbte_batch* bt, int32_t precision, const double* const* ts, const double* const* qxs,
                                 const double* const* qys, void* const* vdf, const double* const* flux_wall,
-                                int32_t homogeneous)
+                                const void* const* pbuf, int32_t homogeneous)
 {
     if (!bt) return -1;
     try {
@@ -4141,18 +4176,18 @@ extern "C" int pbte_batch_sweep(pbte_batch* bt, int32_t precision, const double*
         if (precision == PBTE_FP64) {
             auto v = reinterpret_cast<double* const*>(vdf);
             switch (bt->P) {
-            case 3: batch_sweep<3, double>(bt, v, homogeneous); break;
-            case 6: batch_sweep<6, double>(bt, v, homogeneous); break;
-            case 10: batch_sweep<10, double>(bt, v, homogeneous); break;
-            case 15: batch_sweep<15, double>(bt, v, homogeneous); break;
+            case 3: batch_sweep<3, double>(bt, v, pbuf, homogeneous); break;
+            case 6: batch_sweep<6, double>(bt, v, pbuf, homogeneous); break;
+            case 10: batch_sweep<10, double>(bt, v, pbuf, homogeneous); break;
+            case 15: batch_sweep<15, double>(bt, v, pbuf, homogeneous); break;
             }
         } else {
             auto v = reinterpret_cast<float* const*>(vdf);
             switch (bt->P) {
-            case 3: batch_sweep<3, float>(bt, v, homogeneous); break;
-            case 6: batch_sweep<6, float>(bt, v, homogeneous); break;
-            case 10: batch_sweep<10, float>(bt, v, homogeneous); break;
-            case 15: batch_sweep<15, float>(bt, v, homogeneous); break;
+            case 3: batch_sweep<3, float>(bt, v, pbuf, homogeneous); break;
+            case 6: batch_sweep<6, float>(bt, v, pbuf, homogeneous); break;
+            case 10: batch_sweep<10, float>(bt, v, pbuf, homogeneous); break;
+            case 15: batch_sweep<15, float>(bt, v, pbuf, homogeneous); break;
             }
         }
         return 0;

@@ -35,8 +35,8 @@ import numpy as np
 from . import _native
 from .device import dev_zeros, ptr, torch
 
-__all__ = ["FixedPointOperator", "MomentPreconditioner", "applicable", "gmres_full", "run_krylov",
-           "krylov_basis_cap"]
+__all__ = ["FixedPointOperator", "MomentPreconditioner", "SweepRequest", "applicable", "gmres_full",
+           "run_krylov", "krylov_basis_cap"]
 
 _ENOUGH_FACTOR = 1.0e-2
 _FLOOR = 1.0e-14
@@ -47,6 +47,22 @@ def applicable(solver) -> tuple[bool, str]:
         return False, ("the sweep ordering has broken cycles: part of the inflow is lagged, "
                        "so one sweep is not a function of the state alone")
     return True, ""
+
+
+class SweepRequest:
+    """One sweep a fixed-point product asks for: ``vdf`` in place from the
+    moment source ``src`` (and ``flux_wall``; the context's explicit
+    periodic buffer when its ``use_pbuf`` is set), with or without the wall
+    sources.  ``run()`` sweeps it alone; a batch driver may sweep several
+    problems' requests in one launch instead (the same bits)."""
+
+    __slots__ = ("ctx", "vdf", "homogeneous", "flux_wall", "src")
+
+    def __init__(self, ctx, vdf, homogeneous, flux_wall, src):
+        self.ctx, self.vdf, self.homogeneous, self.flux_wall, self.src = ctx, vdf, homogeneous, flux_wall, src
+
+    def run(self) -> None:
+        self.ctx.sweep(None, self.vdf, homogeneous=self.homogeneous, flux_wall=self.flux_wall, src=self.src)
 
 
 class FixedPointOperator:
@@ -87,6 +103,13 @@ class FixedPointOperator:
 
     def apply(self, u, out, precision: str = "fp64", homogeneous: bool = False) -> None:
         """``out = F(u)`` (``homogeneous``: without the wall sources, ``T u``)."""
+        self.begin(u, precision, homogeneous).run()
+        self.end(out, precision)
+
+    def begin(self, u, precision: str = "fp64", homogeneous: bool = False) -> "SweepRequest":
+        """First half of :meth:`apply`: the periodic partners of ``u`` into
+        the explicit buffer; returns the sweep to run (alone, or batched
+        with other problems' sweeps), after which :meth:`end` finishes."""
         s, ctx = self.s, self.ctx
         vdf = s.vdf_d if precision == "fp64" else s.vdf32_d
         code = _native.PBTE_FP64 if precision == "fp64" else _native.PBTE_FP32
@@ -95,7 +118,15 @@ class FixedPointOperator:
             buf = ctx.ensure_pbuf(vdf.dtype)
             _native.check(ctx.handle, _native.lib().pbte_periodic_scatter(
                 ctx.handle, code, ptr(pb), ptr(buf)), "periodic_scatter")
-        ctx.sweep(None, vdf, homogeneous=homogeneous, flux_wall=fw, src=(ts, qxs, qys))
+        return SweepRequest(ctx, vdf, homogeneous, fw, (ts, qxs, qys))
+
+    def end(self, out, precision: str = "fp64") -> None:
+        """Second half of :meth:`apply`, after the sweep: the moments (or the
+        GSIS correction), the wall emission and the periodic partners of
+        the swept distribution into ``out``."""
+        s, ctx = self.s, self.ctx
+        vdf = s.vdf_d if precision == "fp64" else s.vdf32_d
+        code = _native.PBTE_FP64 if precision == "fp64" else _native.PBTE_FP32
         self.sweeps += 1
         self._last = precision
         if precision == "fp64":

@@ -40,7 +40,7 @@ import numpy as np
 from .device import dev_zeros, torch
 from .krylov import FixedPointOperator
 
-__all__ = ["applicable", "run_refined"]
+__all__ = ["ReadRequest", "applicable", "drive", "refined_steps", "run_refined"]
 
 # inner steps without a 10 % improvement that end an inner solve (a
 # contraction slower than 0.9 per this many steps is taken as the floor)
@@ -60,7 +60,44 @@ def applicable(solver) -> tuple[bool, str]:
     return True, ""
 
 
+class ReadRequest:
+    """A device pair ``(residual, mass)`` the iteration needs on the host:
+    the driver sends back the two values (one wait for every problem of a
+    batch)."""
+
+    __slots__ = ("slot",)
+
+    def __init__(self, slot):
+        self.slot = slot
+
+
+def drive(steps):
+    """Run a step generator alone: each sweep as it is asked for, each
+    read at once; returns the generator's result."""
+    msg = None
+    try:
+        while True:
+            req = steps.send(msg)
+            if isinstance(req, ReadRequest):
+                v = req.slot.cpu().numpy()
+                msg = (float(v[0]), float(v[1]))
+            else:
+                req.run()
+                msg = None
+    except StopIteration as stop:
+        return stop.value
+
+
 def run_refined(solver, callback=None, progress_every: int = 0):
+    return drive(refined_steps(solver, callback, progress_every))
+
+
+def refined_steps(solver, callback=None, progress_every: int = 0):
+    """The refined iteration as a generator: it yields the sweeps it needs
+    (``krylov.SweepRequest``: sweep, then resume) and the residual reads
+    (``ReadRequest``: resume with ``(residual, mass)``), and returns the
+    ``RunRecord``.  ``drive`` runs it alone; ``batch.run_refined_batch``
+    runs several in lockstep with their sweeps batched."""
     c = solver.case
     tol, tmax = c.iteration.tol, c.iteration.tmax
     eta = c.performance.refine_inner
@@ -79,6 +116,7 @@ def run_refined(solver, callback=None, progress_every: int = 0):
         fx, r, e, tmp = dev_zeros(n), dev_zeros(n), dev_zeros(n), dev_zeros(n)
         tfx, qxfx, qyfx = (dev_zeros(solver.n_tris) for _ in range(3))
         cur, prev = dev_zeros(solver.n_tris), dev_zeros(solver.n_tris)
+        slot = dev_zeros(2)
         converged = False
 
         def record(res, mass, fp64):
@@ -96,11 +134,13 @@ def run_refined(solver, callback=None, progress_every: int = 0):
 
         while True:
             # binary64 step from x: fx = F(x), and the reference's residual
-            op.apply(x, fx, "fp64")
+            yield op.begin(x, "fp64")
+            op.end(fx, "fp64")
             tfx.copy_(op._temp)
             qxfx.copy_(op._qx)
             qyfx.copy_(op._qy)
-            res, mass = ctx.residual_iterate(tfx, tx)
+            ctx.residual_iterate_async(tfx, tx, slot)
+            res, mass = yield ReadRequest(slot)
             record(res, mass, True)
             if want_true:
                 op.publish(fx)
@@ -120,14 +160,16 @@ def run_refined(solver, callback=None, progress_every: int = 0):
             target = max(eta * res, 0.8 * tol)
             anchor, since, k = np.inf, 0, 0
             while len(residuals) < tmax:
-                op.apply(e, tmp, "fp32", homogeneous=True)
+                yield op.begin(e, "fp32", homogeneous=True)
+                op.end(tmp, "fp32")
                 ctx.axpby(1.0, r, 1.0, tmp)
                 prev.copy_(cur)
                 cur.copy_(tfx)
                 ctx.axpby(1.0, op._temp, 1.0, cur)
                 e, tmp = tmp, e
                 k += 1
-                rin, mass = ctx.residual_iterate(cur, prev)
+                ctx.residual_iterate_async(cur, prev, slot)
+                rin, mass = yield ReadRequest(slot)
                 record(rin, mass, False)
                 if rin < target:
                     break

@@ -170,10 +170,11 @@ class SweepContext:
         # bytes of the stored factors: binary64, plus binary32 for fp32 runs
         # that read stored binary32 factors
         need = nbytes * (1.5 if self.precision == "fp32" and not self.fused else 1.0)
-        if precompute and self.precision == "fp32" and need > max_bytes >= 0.5 * nbytes:
-            # binary32 factors fit, binary64 ones beside them do not: the
+        if precompute and need > max_bytes >= 0.5 * nbytes:
+            # binary32 factors fit, binary64 ones (beside them) do not: the
             # binary32 sweeps use stored (or fused) factors, the binary64
-            # sweeps of the refinement refine from the binary32 ones
+            # sweeps refine from the binary32 ones (fp64 runs included:
+            # binary64 results without binary64 factors)
             self.factorise("fp32")
             self.mode = "mixed"
         elif precompute and need <= max_bytes:

@@ -714,6 +714,8 @@ def main():
     ap.add_argument("--solve-precisions", default="fp64,fp32",
                     help="precisions of the Krylov solve leg, in order")
     ap.add_argument("--restart", type=int, default=30, help="GMRES restart length of the solve leg")
+    ap.add_argument("--precond", action="store_true",
+                    help="solve leg: GMRES preconditioned by the GSIS moment solve (accflag 1, krylov_precond)")
     ap.add_argument("--solve-tmax", type=int, default=20000, help="product cap of the solve leg")
     ap.add_argument("--solve-max-bytes", type=float, default=0.0,
                     help="factor byte cap of the solve leg (0: half the device memory)")
@@ -853,8 +855,10 @@ def solve_times(pkg, args, work):
         res = {}
         for prec in precs:
             case = make_case(pkg, args.n, args.deg, args.npole, args.nazim, tau_r, 0.1 * tau_r, prec,
-                             work, jitter=args.jitter, hole=args.hole, tmax=args.solve_tmax)
+                             work, jitter=args.jitter, hole=args.hole, tmax=args.solve_tmax,
+                             accflag=1 if args.precond else 0)
             case.scheme.krylov_restart = args.restart
+            case.scheme.krylov_precond = bool(args.precond)
 
             def progress(step, r, solver, prec=prec, tau_r=tau_r):
                 if step % 200 == 0:
@@ -871,6 +875,7 @@ def solve_times(pkg, args, work):
                          "iterations": rec.iterations, "converged": rec.converged,
                          "sweeps_fp64": d.get("sweeps_fp64"), "sweeps_fp32": d.get("sweeps_fp32"),
                          "factor_mode": s.ctx.mode, "restart": args.restart,
+                         "preconditioner": "GSIS moment solve" if args.precond else None,
                          "solves_per_hour": 3600.0 / tts, "temp": rec.temp,
                          "q": np.concatenate([rec.qx, rec.qy])}
             log(f"solve Kn_R {tau_r:g} {prec}: " + json.dumps(

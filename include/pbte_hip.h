@@ -226,23 +226,26 @@ int pbte_set_cell_operators(pbte_ctx *ctx, const double *ref, const double *cell
 int pbte_set_fused(pbte_ctx *ctx, int32_t on);
 
 /* ---- batched sweeps (no counterpart in the reference) ------------------
- * The sweeps of several independent problems in one launch of the
- * pipelined sweep kernel: every workgroup finds its context and bundle in a
- * block map.  The contexts must share the element degree, the stream and the
- * pipeline depth and run the pipelined kernel in the precision swept; no
- * explicit periodic buffer.  pbte_batch_sweep takes per context the moment
- * sources, the distribution (of the precision) and flux_wall (NULL without
- * adiabatic walls), runs each context's source and wall-vector kernels and
- * then the one sweep launch; each context's sweep is the one pbte_sweep
- * computes, bit for bit.  For many small problems, whose workgroups alone
- * leave most of the CUs idle. */
+ * The sweeps of several independent problems in one persistent launch of
+ * the pipelined sweep kernel fed by a device work queue: the work items are
+ * the contexts' (context, bundle) pairs, longest bundles first; each
+ * resident workgroup takes the next item with one atomic until the queue is
+ * empty.  The contexts must share the element degree, the stream and the
+ * pipeline depth and run the pipelined kernel in the precision swept (all
+ * fused or all with stored factors in binary32).  pbte_batch_sweep takes
+ * per context the moment sources, the distribution (of the precision),
+ * flux_wall (NULL without adiabatic walls) and the explicit periodic buffer
+ * (pbuf NULL, or a NULL entry: none; as pbte_sweep's use_pbuf), runs each
+ * context's source and wall-vector kernels and then the one sweep launch;
+ * each context's sweep is the one pbte_sweep computes, bit for bit.  For
+ * many small problems, whose workgroups alone leave most of the CUs idle. */
 typedef struct pbte_batch pbte_batch;
 pbte_batch *pbte_batch_create(pbte_ctx *const *ctxs, int32_t n, char *err, int32_t errlen);
 void pbte_batch_destroy(pbte_batch *batch);
 const char *pbte_batch_last_error(pbte_batch *batch);
 int pbte_batch_sweep(pbte_batch *batch, int32_t precision, const double *const *ts,
                      const double *const *qxs, const double *const *qys, void *const *vdf,
-                     const double *const *flux_wall, int32_t homogeneous);
+                     const double *const *flux_wall, const void *const *pbuf, int32_t homogeneous);
 /* 1 when every context of the batch runs the pipelined kernel in this
  * precision (pbte_batch_sweep can sweep them together), else 0: the caller
  * then sweeps those contexts one by one with pbte_sweep. */

@@ -172,3 +172,34 @@ def test_batch_falls_back_for_contexts_off_the_pipelined_kernel(pkg, tmp_path, p
         assert r.iterations == a.iterations and r.converged == a.converged
         np.testing.assert_array_equal(r.residual_history, a.residual_history)
         np.testing.assert_array_equal(r.temp_dofs, a.temp_dofs)
+
+
+def test_run_refined_batch_is_each_refined_run_alone(pkg, tmp_path):
+    """The fp32 defect-correction iteration of five problems in lockstep --
+    degrees 1 and 2, thermalising, adiabatic and periodic walls (explicit
+    periodic buffers in the batched launch), two of them GSIS -- gives each
+    problem's own refined run, bit for bit: iteration counts, residual and
+    mass histories, binary64 steps, moments."""
+    from pbte_amd.batch import run_refined_batch
+    cases = _cases(pkg, tmp_path, "fp32")
+    cases.append(_case(pkg, "F3_1_1", performance__precision="fp32", dg__deg=1))
+    cases[1].dg.deg = 1
+    for k in (0, 2):
+        cases[k].scheme.accflag = 1
+    for c in cases:
+        c.iteration.tmax = 400
+    print("case batch refined", flush=True)
+    alone = [pkg.Solver(c).run() for c in cases]
+    for a in alone:
+        assert a.diagnostics["precision_scheme"] == "defect-correction"
+    calls = []
+    solvers = [pkg.Solver(c) for c in cases]
+    recs = run_refined_batch(solvers, callback=lambda k, step, res, s: calls.append(k))
+    for a, r in zip(alone, recs):
+        assert r.scheme == a.scheme and r.iterations == a.iterations and r.converged == a.converged
+        np.testing.assert_array_equal(r.residual_history, a.residual_history)
+        np.testing.assert_array_equal(r.mass_history, a.mass_history)
+        np.testing.assert_array_equal(r.temp_dofs, a.temp_dofs)
+        np.testing.assert_array_equal(r.qx, a.qx)
+        assert r.diagnostics["fp64_steps"] == a.diagnostics["fp64_steps"]
+    assert len(calls) == sum(r.iterations for r in recs)

@@ -317,14 +317,15 @@ def test_onthefly_sweep_matches_oracle(pkg, tag, deg):
     assert _rel(s.vdf, o.vdf) <= 1e-12
 
 
-@pytest.mark.parametrize("tag,deg", [("F1_1_1", 3), ("F2_1_1", 2), ("F3_1_1", 1)])
-def test_mixed_mode_refined_sweep_matches_oracle(pkg, tag, deg):
-    """The "mixed" factor mode (fp32 run whose binary64 factors do not fit
-    beside the binary32 ones): the binary64 sweep refines the stored
+@pytest.mark.parametrize("tag,deg,prec", [("F1_1_1", 3, "fp32"), ("F2_1_1", 2, "fp32"), ("F3_1_1", 1, "fp32"),
+                                          ("F2_1_1", 3, "fp64")])
+def test_mixed_mode_refined_sweep_matches_oracle(pkg, tag, deg, prec):
+    """The "mixed" factor mode (binary64 factors do not fit, binary32 ones
+    do; fp32 and fp64 runs): the binary64 sweep refines the stored
     binary32 inverse against the matrix-free operator; thermalising,
     adiabatic and periodic walls, degrees 1-3, against the oracle."""
     import torch
-    c = _case(pkg, tag, performance__precision="fp32", dg__deg=deg)
+    c = _case(pkg, tag, performance__precision=prec, dg__deg=deg)
     from pbte_amd.problem import build_problem
     pb = build_problem(c)
     nbytes = float(pb.vel.ndir) * pb.mesh.n_tris * c.ndof_tri ** 2 * 8.0
