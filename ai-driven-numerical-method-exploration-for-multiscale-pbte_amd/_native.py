@@ -150,6 +150,7 @@ _SIGS = {
     "pbte_trace_info": (C.c_int, [C.c_void_p, _i64p]),
     "pbte_trace_factor": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pbte_trace_options": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pbte_trace_graphs": (C.c_int, [C.c_void_p, C.c_int32]),
     "pbte_trace_quality": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _f64p]),
     "pbte_trace_solve": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "pbte_trace_level": (C.c_int, [C.c_void_p, C.c_int32, _i64p]),

@@ -905,9 +905,30 @@ struct pbte_trace {
     float* d_fac32 = nullptr;   // binary32 copy of the factors (made by the first binary32 solve)
     bool fac32_current = false;
     int64_t ws_size[2] = {0, 0}, pan_size = 0;
+    // the solve's launches (two per level forward, one backward) recorded
+    // once per (precision, rhs, x) as a HIP graph on a private stream and
+    // replayed on `stream`; dropped when the factors change
+    struct SolveGraph {
+        int precision;
+        const double* rhs;
+        double* x;
+        hipGraphExec_t exec;
+    };
+    std::vector<SolveGraph> graphs;
+    hipStream_t cap_stream = nullptr;
+    bool use_graphs = true;
+    static constexpr size_t kMaxGraphs = 8;
+
+    void drop_graphs()
+    {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
+    }
 
     ~pbte_trace()
     {
+        drop_graphs();
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
         void* ps[] = {d_fr, d_faces, d_fmap, d_nodes, d_perm, d_ipiv, d_status, d_rec, d_fw, d_bw, d_gi, d_gsrc,
                       d_fac, d_ws[0], d_ws[1], d_pan, d_y, d_cv, d_fac32};
         for (void* p : ps)
@@ -990,6 +1011,15 @@ extern "C" pbte_trace* pbte_trace_create(const pbte_trace_desc* desc, void* stre
 extern "C" void pbte_trace_destroy(pbte_trace* t) { delete t; }
 
 extern "C" const char* pbte_trace_last_error(pbte_trace* t) { return t ? t->err.c_str() : "null trace solver"; }
+
+extern "C" int pbte_trace_graphs(pbte_trace* t, int32_t on)
+{
+    return tguard(t, [&] {
+        t->use_graphs = on != 0;
+        if (!t->use_graphs) t->drop_graphs();
+        return 0;
+    });
+}
 
 extern "C" int pbte_trace_options(pbte_trace* t, int32_t pivoting)
 {
@@ -1171,6 +1201,7 @@ extern "C" int pbte_trace_factor(pbte_trace* t, const double* bsr_val)
 {
     return tguard(t, [&] {
         t->alloc();
+        t->drop_graphs();
         t->factored = false;
         t->fac32_current = false;
         // diagonal-block pivots first; partial pivoting among the fully
@@ -1213,10 +1244,9 @@ namespace {
 template <typename TF> int solve_lanes(int lpr) { return sizeof(TF) == 4 ? std::max(4, lpr / 2) : lpr; }
 
 template <typename TF>
-void solve_pass_t(pbte_trace* t, const TF* fac, const double* rhs, double* x)
+void solve_pass_t(pbte_trace* t, const TF* fac, const double* rhs, double* x, hipStream_t st)
 {
     Symbolic& S = t->S;
-    hipStream_t st = t->stream;
     for (int l = S.max_depth; l >= 0; --l) {
         const Level& L = S.levels[l];
         hipLaunchKernelGGL(k_fwd_gather, dim3(L.gi_n), dim3(256), 0, st, t->d_gi + L.gi_off, t->d_fr, t->d_faces,
@@ -1242,7 +1272,39 @@ void solve_pass_t(pbte_trace* t, const TF* fac, const double* rhs, double* x)
     TL_TRY(hipGetLastError());
 }
 
-void solve_pass(pbte_trace* t, const double* rhs, double* x) { solve_pass_t<double>(t, t->d_fac, rhs, x); }
+void solve_pass(pbte_trace* t, const double* rhs, double* x) { solve_pass_t<double>(t, t->d_fac, rhs, x, t->stream); }
+
+// one solve on t->stream: replayed from its graph when one was recorded
+// for these buffers, else recorded now (at most kMaxGraphs per factors;
+// past that, or with graphs off, launched directly).  Recording captures
+// the launches on a private stream without running them; the graph then
+// runs on t->stream in its order like any launch.
+template <typename TF>
+void solve_graphed(pbte_trace* t, int precision, const TF* fac, const double* rhs, double* x)
+{
+    if (t->use_graphs) {
+        for (const auto& g : t->graphs)
+            if (g.precision == precision && g.rhs == rhs && g.x == x) {
+                TL_TRY(hipGraphLaunch(g.exec, t->stream));
+                return;
+            }
+        if (t->graphs.size() < pbte_trace::kMaxGraphs) {
+            if (!t->cap_stream) TL_TRY(hipStreamCreateWithFlags(&t->cap_stream, hipStreamNonBlocking));
+            hipGraph_t graph = nullptr;
+            TL_TRY(hipStreamBeginCapture(t->cap_stream, hipStreamCaptureModeRelaxed));
+            solve_pass_t<TF>(t, fac, rhs, x, t->cap_stream);
+            TL_TRY(hipStreamEndCapture(t->cap_stream, &graph));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            TL_TRY(e);
+            t->graphs.push_back({precision, rhs, x, exec});
+            TL_TRY(hipGraphLaunch(exec, t->stream));
+            return;
+        }
+    }
+    solve_pass_t<TF>(t, fac, rhs, x, t->stream);
+}
 }  // namespace
 
 extern "C" int pbte_trace_solve(pbte_trace* t, int32_t precision, const double* rhs, double* x)
@@ -1250,7 +1312,7 @@ extern "C" int pbte_trace_solve(pbte_trace* t, int32_t precision, const double* 
     return tguard(t, [&] {
         if (!t->factored) throw std::string("trace solve: pbte_trace_factor has not succeeded");
         if (precision == PBTE_FP64) {
-            solve_pass(t, rhs, x);
+            solve_graphed<double>(t, PBTE_FP64, t->d_fac, rhs, x);
         } else if (precision == PBTE_FP32) {
             if (!t->d_fac32) t->d_fac32 = dmalloc<float>((size_t)t->S.fac_size);
             if (!t->fac32_current) {
@@ -1259,7 +1321,7 @@ extern "C" int pbte_trace_solve(pbte_trace* t, int32_t precision, const double* 
                 TL_TRY(hipGetLastError());
                 t->fac32_current = true;
             }
-            solve_pass_t<float>(t, t->d_fac32, rhs, x);
+            solve_graphed<float>(t, PBTE_FP32, t->d_fac32, rhs, x);
         } else {
             throw std::string("trace solve: precision is PBTE_FP64 or PBTE_FP32");
         }

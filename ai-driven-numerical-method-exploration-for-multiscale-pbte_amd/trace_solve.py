@@ -97,6 +97,11 @@ class TraceSolver:
         fallback), 0 diagonal-block pivots only, 1 partial pivoting only."""
         self._check(self._lib.pbte_trace_options(self.handle, int(mode)), "trace_options")
 
+    def set_graphs(self, on: bool) -> None:
+        """Record each (precision, rhs, x) solve as a HIP graph and replay
+        it (the default), or launch every kernel directly."""
+        self._check(self._lib.pbte_trace_graphs(self.handle, 1 if on else 0), "trace_graphs")
+
     def quality(self) -> dict:
         piv, berr = C.c_int32(), C.c_double()
         self._check(self._lib.pbte_trace_quality(self.handle, C.byref(piv), C.byref(berr)), "trace_quality")

@@ -459,6 +459,7 @@ def cis_workload(pkg, args, work):
 
 BATCH_CELLS = (2048, 8192, 32768, 65536)
 BATCH_DIRS = {64: (4, 16), 128: (8, 16), 256: (8, 32), 512: (16, 32)}
+BATCH_FACTOR_BYTES = 2.0e9      # per-problem cap on stored element factors in the batch solve leg
 
 
 def batch_draws(count=64, seed=1):
@@ -491,13 +492,11 @@ def batch_workload(pkg, args, work):
     wall time per step; the same sweeps launched problem by problem are
     timed beside it.
 
-    Solve leg (--batch-solve-div k, default 4; 0 skips it): the same 64
-    draws on meshes k times coarser in each direction (n // k cells a
-    side, at least 4), GSIS with FP32 sweeps and the binary64 finish to the
-    iterate step 1e-8, all 64 in lockstep (batch.run_source_batch); the
-    aggregate solves/hour and each problem's iterations and time to
-    tolerance.  The full-size GSIS solves are not run: each GSIS iteration
-    solves the HDG trace system with SuperLU on the host."""
+    Solve leg (--batch-solve-div k, default 1: the drawn sizes; 0 skips
+    it): the same 64 draws (n // k cells a side, at least 4), GSIS with
+    FP32 sweeps inside the binary64 defect correction to the iterate step
+    1e-8, all 64 in lockstep (batch_solves); the aggregate solves/hour and
+    each problem's iterations and time to tolerance."""
     import torch
     from pbte_amd.batch import SweepBatch, launch_key
     draws = batch_draws(args.batch_count, 1)
@@ -510,7 +509,8 @@ def batch_workload(pkg, args, work):
         case = make_case(pkg, d["n"], d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp32", work,
                          jitter=0.25, seed=1, method="source")
         case.performance.autotune = args.autotune
-        s = pkg.Solver(case, max_bundle=args.bundle)
+        # binary32 sweeps only: fused, no element factors stored
+        s = pkg.Solver(case, max_bundle=args.bundle, max_bytes=0.0 if FP32_OPERATOR == "fused" else None)
         P, N = s.ndof, s.n_tris
         s.mom.set_dofs(rng.random((P, N)), 0.1 * rng.random((P, N)), 0.1 * rng.random((P, N)))
         solvers.append(s)
@@ -597,38 +597,86 @@ def batch_workload(pkg, args, work):
 
 
 def batch_solves(pkg, args, work, draws):
-    """GSIS FP32 + binary64 finish of every drawn problem, in lockstep, on
-    meshes args.batch_solve_div times coarser a side."""
-    import torch
-    from pbte_amd.batch import run_source_batch
-    div = args.batch_solve_div
-    log(f"batch: building {len(draws)} GSIS problems (meshes / {div} a side)")
-    t0 = time.perf_counter()
-    solvers = []
-    for k, d in enumerate(draws):
-        npole, nazim = BATCH_DIRS[d["ndir"]]
-        case = make_case(pkg, max(4, d["n"] // div), d["deg"], npole, nazim, d["tau_r"], d["tau_n"], "fp32",
-                         work, jitter=0.25, seed=1, method="source", accflag=1, tmax=args.batch_tmax)
-        case.performance.autotune = args.autotune
-        solvers.append(pkg.Solver(case))
-        if k % 8 == 7:
-            log(f"batch: {k + 1} GSIS problems set up ({time.perf_counter() - t0:.1f} s)")
-    setup = time.perf_counter() - t0
-    log(f"batch: {len(solvers)} GSIS problems set up in {setup:.1f} s; solving")
-    def progress(k, step, res, s):
-        if step % 200 == 0 and k == 0:
-            log(f"batch: step {step}")
+    """GSIS of every drawn problem (meshes args.batch_solve_div times
+    coarser a side; 1: the drawn sizes), fp32 as the binary64 defect
+    correction, in lockstep (batch.run_refined_batch: each problem's sweeps
+    merged into the persistent work-queue launches); problems whose sweep
+    ordering breaks cycles run the fp32 switch scheme in a second lockstep
+    group (batch.run_source_batch).  Problems are admitted in draw order
+    while the device memory in use stays below --batch-mem-frac of the
+    total (the solve's own buffers, binary32 trace-factor copies among
+    them, are allocated as it runs); each admitted group is solved, freed,
+    and the next admitted.  Binary64 factors are stored for problems whose
+    factors take at most BATCH_FACTOR_BYTES, binary32 ones (the refined
+    binary64 sweep) below twice that, none above (the binary64 sweeps
+    build their operators); binary32 sweeps are fused."""
+    import gc
 
-    t1 = time.perf_counter()
-    recs = run_source_batch(solvers, callback=progress)
-    torch.cuda.synchronize()
-    run = time.perf_counter() - t1
-    per = [{"n_tris": s.n_tris, "ndof": s.ndof, "ndir": s.ndir, "iterations": r.iterations,
-            "converged": r.converged, "time_to_tolerance_s": r.wall_clock,
-            "steps_fp32": r.diagnostics.get("steps_fp32")} for s, r in zip(solvers, recs)]
-    return {"mesh_div": div, "setup_s": setup, "run_s": run,
-            "solves_per_hour": 3600.0 * len(recs) / (setup + run),
-            "converged": int(sum(r.converged for r in recs)), "problems": per}
+    import torch
+    from pbte_amd.batch import run_refined_batch, run_source_batch
+    from pbte_amd.refine import applicable
+    div = args.batch_solve_div
+    log(f"batch: {len(draws)} GSIS problems (meshes / {div} a side)")
+    total = torch.cuda.mem_get_info()[1]
+    per = [None] * len(draws)
+    groups = []
+    setup_all = run_all = 0.0
+    k = 0
+
+    def progress(j, step, res, s):
+        if step % 200 == 0 and j == 0:
+            log(f"batch: first problem of the group at step {step}")
+
+    while k < len(draws):
+        t0 = time.perf_counter()
+        solvers, idx = [], []
+        while k < len(draws):
+            d = draws[k]
+            npole, nazim = BATCH_DIRS[d["ndir"]]
+            case = make_case(pkg, max(4, d["n"] // div), d["deg"], npole, nazim, d["tau_r"], d["tau_n"],
+                             "fp32", work, jitter=0.25, seed=1, method="source", accflag=1, tmax=args.batch_tmax)
+            case.performance.autotune = args.autotune
+            solvers.append(pkg.Solver(case, max_bytes=BATCH_FACTOR_BYTES))
+            idx.append(k)
+            k += 1
+            free, _ = torch.cuda.mem_get_info()
+            if total - free > args.batch_mem_frac * total:
+                break
+        setup = time.perf_counter() - t0
+        used = (total - torch.cuda.mem_get_info()[0]) / 2**30
+        refined = [j for j, s in enumerate(solvers) if applicable(s)[0]]
+        switch = [j for j in range(len(solvers)) if j not in refined]
+        log(f"batch: problems {idx[0]}-{idx[-1]} set up in {setup:.1f} s ({used:.1f} GiB in use); solving "
+            f"({len(refined)} refined, {len(switch)} with the switch scheme)")
+        t1 = time.perf_counter()
+        recs = [None] * len(solvers)
+        if refined:
+            for j, r in zip(refined, run_refined_batch([solvers[j] for j in refined], callback=progress)):
+                recs[j] = r
+        if switch:
+            for j, r in zip(switch, run_source_batch([solvers[j] for j in switch])):
+                recs[j] = r
+        torch.cuda.synchronize()
+        run = time.perf_counter() - t1
+        log(f"batch: group solved in {run:.1f} s")
+        for j, (s, r) in enumerate(zip(solvers, recs)):
+            per[idx[j]] = {"n_tris": s.n_tris, "ndof": s.ndof, "ndir": s.ndir, "tau_r": s.case.flow.tau_r,
+                           "iterations": r.iterations, "converged": r.converged,
+                           "time_to_tolerance_s": r.wall_clock,
+                           "scheme": r.diagnostics.get("precision_scheme", "switch"),
+                           "factor_mode": s.ctx.mode, "steps_fp32": r.diagnostics.get("steps_fp32"),
+                           "group": len(groups)}
+        groups.append({"problems": len(solvers), "setup_s": setup, "run_s": run, "device_gib": used})
+        setup_all += setup
+        run_all += run
+        del solvers, recs
+        gc.collect()
+        torch.cuda.empty_cache()
+    return {"mesh_div": div, "setup_s": setup_all, "run_s": run_all,
+            "solves_per_hour": 3600.0 * len(draws) / (setup_all + run_all),
+            "converged": int(sum(p["converged"] for p in per)),
+            "refined": int(sum(p["scheme"] == "defect-correction" for p in per)),
+            "groups": groups, "problems": per}
 
 
 def b_alg(D, N, P, s_lu, s_f):
@@ -684,9 +732,11 @@ def main():
                                                       "128 for sweep)")
     ap.add_argument("--workload", default="config2", choices=["config2", "sweep", "cis", "gsis", "batch", "trace"])
     ap.add_argument("--batch-count", type=int, default=64, help="problems of the batch workload")
-    ap.add_argument("--batch-solve-div", type=int, default=4,
+    ap.add_argument("--batch-solve-div", type=int, default=1,
                     help="batch workload: solve leg on meshes this many times coarser a side (0 skips)")
     ap.add_argument("--batch-tmax", type=int, default=5000, help="batch workload: GSIS iteration cap")
+    ap.add_argument("--batch-mem-frac", type=float, default=0.55,
+                    help="batch solve leg: admit problems while this fraction of device memory is in use")
     ap.add_argument("--deg", type=int, default=2)
     ap.add_argument("--npole", type=int, default=0, help="polar ordinates (default 16 for config2 and gsis, 8)")
     ap.add_argument("--nazim", type=int, default=32)

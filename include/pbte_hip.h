@@ -553,6 +553,15 @@ int pbte_trace_factor(pbte_trace *t, const double *bsr_val);
  * iterations). */
 int pbte_trace_solve(pbte_trace *t, int32_t precision, const double *rhs, double *x);
 
+/* The solve's kernel launches (per level of the dissection tree) recorded
+ * as a HIP graph the first time a (precision, rhs, x) triple is solved and
+ * replayed after (up to 8 graphs per factorisation, recorded on a private
+ * stream, replayed on the solver's stream; a factorisation drops them).
+ * on = 0 launches every kernel directly (the same values).  No counterpart
+ * in the reference: the solve is launch-bound for small systems, and a
+ * batch of problems solves one per problem per iteration. */
+int pbte_trace_graphs(pbte_trace *t, int32_t on);
+
 #ifdef __cplusplus
 }
 #endif

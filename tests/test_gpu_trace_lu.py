@@ -59,3 +59,32 @@ def test_trace_solve_singular_raises(pkg):
     ts = TraceSolver(pb.mesh, t1, t2, ptr, col, blk, leaf_cells=8)
     with pytest.raises(np.linalg.LinAlgError):     # both pivotings fail
         ts.factor(val)
+
+
+def test_graph_replays_are_the_direct_solve(pkg):
+    """The solve recorded as a HIP graph and replayed (several times, with
+    other device work and waits between the replays, in both precisions,
+    again after a new factorisation) gives the directly launched solve's
+    bits on every replay."""
+    import torch
+
+    from pbte_amd.trace_solve import TraceSolver, block_pattern
+    pb, K, t1, t2, blk = trace_system(pkg, "F3_1_1.yaml", 2)
+    ptr, col, val = block_pattern(K, blk)
+    direct = TraceSolver(pb.mesh, t1, t2, ptr, col, blk, leaf_cells=4)
+    direct.set_graphs(False)
+    graphed = TraceSolver(pb.mesh, t1, t2, ptr, col, blk, leaf_cells=4)
+    rng = np.random.default_rng(11)
+    bd = torch.empty(K.shape[0], dtype=torch.float64, device="cuda")
+    xa, xb = torch.empty_like(bd), torch.empty_like(bd)
+    for scale in (1.0, 3.0):
+        direct.factor(val * scale)
+        graphed.factor(val * scale)
+        for rep in range(4):
+            bd.copy_(torch.from_numpy(rng.standard_normal(K.shape[0])))
+            for prec in ("fp64", "fp32"):
+                direct.solve(bd, xa, prec)
+                graphed.solve(bd, xb, prec)
+                junk = torch.randn(4096, device="cuda", dtype=torch.float64).sum()   # other work between replays
+                torch.cuda.synchronize()
+                assert torch.equal(xa, xb), (scale, rep, prec, float(junk))
