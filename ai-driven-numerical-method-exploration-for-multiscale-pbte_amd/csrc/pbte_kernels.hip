@@ -91,11 +91,10 @@ template <int P> struct Lay {
     static constexpr int R_N = 3 * P, R_INFO = 3 * P + 6;
     static constexpr int CR = pad4(3 * P + 9);
     static constexpr int CRL = pad4(3 * P + 6);  // the part every item loads
-    // the fused binary32 sweep's five element scalars follow the info words
-    // (in the row's padding): area2 / tau_c and area2 times the four affine
-    // gradient coefficients; that sweep loads the whole row
+    // the fused binary32 sweep's element scalar area2 / tau_c follows the
+    // info words (in the row's padding); that sweep loads the row up to it
     static constexpr int R_FUSE = 3 * P + 9;
-    static constexpr int CRF = pad4(3 * P + 14);
+    static constexpr int CRF = pad4(3 * P + 10);
 };
 
 // row stride of the cell records: a multiple of 128 bytes, so a record
@@ -226,7 +225,8 @@ struct pbte_ctx {
     // tables), on when fused32 is set; face lengths [N][3]
     std::vector<double> h_fref;
     std::vector<double> h_flen;
-    bool fused_ok = false, fused32 = false, have_cellops = false, fused_tables = false;
+    bool fused_ok = false, fused32 = false, have_cellops = false, fused_tables = false, fused_geometry = false;
+    bool split = false;         // fused sweeps at P <= 6 in the split kernel where its LDS fits
     // scratch
     double* src64 = nullptr;   // source of the transport residual
     double* ssraw = nullptr;
@@ -644,6 +644,7 @@ template <typename T> struct SweepArgs {
     int lane_zero;              // 0 at run time; makes a uniform load per-lane
     int pool;                   // LDS pool slots
     int wbase;                  // LDS offset (in values) of the wall inflow vectors
+    int fbase;                  // LDS offset (in values) of the split sweep's factor buffers
     int nthr;                   // threads that sweep (the context's workgroup size; a batch
                                 // launches the widest of its contexts)
     T kref[25];                 // Kr, Q x Q, row major
@@ -805,17 +806,20 @@ template <int P> constexpr double unit_edge(int r, int q)
     return RefElem<P>::E0[face_node(Deg<P>::v, 0, r) * P + face_node(Deg<P>::v, 0, q)];
 }
 
+// A of one item from its element scalar am = area2/tau_c and the
+// length-scaled outward normals nl of faces 0, 1, 2 (edges 12, 23, 31).
+// The advection coefficients -area2 (c . grad xi) and -area2 (c . grad eta)
+// of D_xi and D_eta are the flux speeds c . (n L) through faces 2 and 0:
+// area2 grad xi = -(n L) of edge 31, area2 grad eta = -(n L) of edge 12
+// (checked per element when the scalars are set)
 template <int P, typename T>
-__device__ __forceinline__ void fused_solve(T (&x)[P], const T* cr, T cx, T cy)
+__device__ __forceinline__ void fused_build(T (&A)[P][P], T am, const T* nl, T cx, T cy)
 {
-    using L = Lay<P>;
     using R = RefElem<P>;
     constexpr int DEG = Deg<P>::v;
     constexpr int Q = DEG + 1;
-    const T am = cr[L::R_FUSE];
-    const T bx = -(cx * cr[L::R_FUSE + 1] + cy * cr[L::R_FUSE + 2]);
-    const T by = -(cx * cr[L::R_FUSE + 3] + cy * cr[L::R_FUSE + 4]);
-    T A[P][P];
+    const T bx = cx * nl[4] + cy * nl[5];
+    const T by = cx * nl[0] + cy * nl[1];
 #pragma unroll
     for (int m = 0; m < P; ++m)
 #pragma unroll
@@ -828,32 +832,59 @@ __device__ __forceinline__ void fused_solve(T (&x)[P], const T* cr, T cx, T cy)
         }
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
-        const T sp = cx * cr[L::R_N + 2 * il] + cy * cr[L::R_N + 2 * il + 1];
+        const T sp = cx * nl[2 * il] + cy * nl[2 * il + 1];
         const T w = T(0.5) * (sp + fabs(sp));
 #pragma unroll
         for (int r = 0; r < Q; ++r)
 #pragma unroll
             for (int q = 0; q < Q; ++q) fma_const(A[face_node(DEG, il, r)][face_node(DEG, il, q)], unit_edge<P>(r, q), w);
     }
-    T rd[P];
+}
+
+// Doolittle elimination in place: the multipliers below the diagonal, U
+// above it, the reciprocal of U's diagonal on it
+template <int P, typename T>
+__device__ __forceinline__ void fused_eliminate(T (&A)[P][P])
+{
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-        rd[k] = recip(A[k][k]);
+        const T rk = recip(A[k][k]);
 #pragma unroll
         for (int i = k + 1; i < P; ++i) {
-            const T f = A[i][k] * rd[k];
+            const T f = A[i][k] * rk;
 #pragma unroll
             for (int j = k + 1; j < P; ++j) A[i][j] -= f * A[k][j];
-            x[i] -= f * x[k];
+            A[i][k] = f;
         }
+        A[k][k] = rk;
     }
+}
+
+// x = A^-1 x from fused_eliminate's factors, read through e(i, j)
+template <int P, typename T, class E>
+__device__ __forceinline__ void fused_substitute(T (&x)[P], E&& e)
+{
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+#pragma unroll
+        for (int i = k + 1; i < P; ++i) x[i] -= e(i, k) * x[k];
 #pragma unroll
     for (int i = P - 1; i >= 0; --i) {
         T v = x[i];
 #pragma unroll
-        for (int j = i + 1; j < P; ++j) v -= A[i][j] * x[j];
-        x[i] = v * rd[i];
+        for (int j = i + 1; j < P; ++j) v -= e(i, j) * x[j];
+        x[i] = v * e(i, i);
     }
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void fused_solve(T (&x)[P], const T* cr, T cx, T cy)
+{
+    using L = Lay<P>;
+    T A[P][P];
+    fused_build<P, T>(A, cr[L::R_FUSE], cr + L::R_N, cx, cy);
+    fused_eliminate<P, T>(A);
+    fused_substitute<P, T>(x, [&](int i, int j) { return A[i][j]; });
 }
 
 __device__ __forceinline__ int bits_of(float v) { return __float_as_int(v); }
@@ -1203,7 +1234,7 @@ struct PipeRec {
 // read in the solve (LU_READ) or gathered with the cell record
 // (LU_PREFETCH), or the operator built from the reference-element matrices
 // and factorised in registers (LU_FUSED, binary32)
-enum { LU_READ = 0, LU_PREFETCH = 1, LU_FUSED = 2 };
+enum { LU_READ = 0, LU_PREFETCH = 1, LU_FUSED = 2, LU_SPLIT = 3 };
 template <int P, typename T> constexpr int lu_stored_mode() { return P * P * sizeof(T) <= 288 ? LU_PREFETCH : LU_READ; }
 
 template <int P, typename T, int LUM> struct PipeData {
@@ -1367,7 +1398,7 @@ __device__ __forceinline__ void pipe_read(const SweepArgs<T>& a, const PipeData<
 template <int P, typename T, int LUM>
 __device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeData<P, T, LUM>& dC,
                                              const PipeIn<P, T>& in, T cx, T cy, int B, int b, T* lds, T* vdf_d,
-                                             const T* lub)
+                                             const T* lub, const T* fv = nullptr)
 {
     using L = Lay<P>;
     constexpr int DEG = Deg<P>::v;
@@ -1387,6 +1418,8 @@ __device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeDa
     }
     if constexpr (LUM == LU_FUSED) {
         fused_solve<P, T>(x, cr, cx, cy);
+    } else if constexpr (LUM == LU_SPLIT) {
+        fused_substitute<P, T>(x, [&](int i, int j) { return fv[i * P + j]; });
     } else if constexpr (LUM == LU_PREFETCH) {
         apply_inverse<P, T>(x, [&](int k) { return dC.lu[k]; });
     } else {
@@ -1483,9 +1516,11 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
     }
     __syncthreads();
     if (!sweeps) {
-        // the loops below run whole trips of DEPTH levels, one barrier each
+        // the loops below run whole trips (two levels at depth 2, six at
+        // depth 3), one barrier per level
         const int nlev = l1 > l0 ? l1 - l0 : 0;
-        const int nbar = (nlev + DEPTH - 1) / DEPTH * DEPTH;
+        const int trip = DEPTH == 3 ? 6 : DEPTH;
+        const int nbar = (nlev + trip - 1) / trip * trip;
         for (int k = 0; k < nbar; ++k) level_barrier(global_mode);
         return;
     }
@@ -1543,9 +1578,10 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         }
     } else {
         // the gather two levels ahead of the solve and the records two levels
-        // ahead of the gather (three register sets each, three levels per
-        // trip): every load has two level times to arrive.  Set k holds
-        // level l0 + k mod 3.
+        // ahead of the gather (three register sets each, six levels per
+        // trip: the loop's back edge, where the sets are renamed, costs a
+        // wait for the loads in flight once per trip): every load has two
+        // level times to arrive.  Set k holds level l0 + k mod 3.
         int2 S0 = span(l0 + 6), S1 = span(l0 + 4), S2 = span(l0 + 5);
         PipeRec R0, R1, R2;
         PipeData<P, T, LUM> D0, D1, D2;
@@ -1555,12 +1591,182 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         pipe_record(irec, span(l0 + 2), c, B, b, R2);
         pipe_gather<P, T, LUM>(a, R1, B, b, lu, D1);
         pipe_record(irec, span(l0 + 3), c, B, b, R0);
-        for (int lv = l0; lv < l1; lv += 3) {
+        for (int lv = l0; lv < l1; lv += 6) {
             level(D0, R1, S1, lv + 7, D2, R2);     // solve lv; record lv+4; gather lv+2
             level(D1, R2, S2, lv + 8, D0, R0);     // solve lv+1; record lv+5; gather lv+3
             level(D2, R0, S0, lv + 9, D1, R1);     // solve lv+2; record lv+6; gather lv+4
+            level(D0, R1, S1, lv + 10, D2, R2);
+            level(D1, R2, S2, lv + 11, D0, R0);
+            level(D2, R0, S0, lv + 12, D1, R1);
         }
     }
+}
+
+// ----------------------------------------------------------------------------
+// the split fused sweep (binary32, P <= 6): a workgroup of twice the level
+// width, its first half of waves solving level lv (upwind traces, right-hand
+// side, substitution, stores) while its second half builds and factorises
+// the element operators of level lv + 1 and leaves them in one of two LDS
+// factor buffers, [buffer][group of 4 values][lane][4].  The consumer's
+// data (records, the cell record without the element scalars) and the
+// producer's (records, the normals and element scalars) each stream in
+// their own three-level pipeline; both halves take the same barriers, one
+// per level and one for the first level's factors.  The two halves of a
+// SIMD interleave: the factorisation runs while the solve waits on memory
+// and barriers.  The values are the one-wave fused sweep's, bit for bit
+// (the same build, elimination and substitution).
+// ----------------------------------------------------------------------------
+template <int P> constexpr int fac_vals() { return pad4(P * P); }
+template <int P> constexpr int prod_lo() { return (3 * P) & ~3; }                    // first value a producer loads
+template <int P> constexpr int prod_vals() { return pad4(3 * P + 10) - prod_lo<P>(); }
+
+template <int P, typename T> struct ProdData {
+    T g[prod_vals<P>()];     // cell record values prod_lo .. pad4(3P + 14)
+};
+
+template <int P, typename T>
+__device__ __forceinline__ void prod_gather(const SweepArgs<T>& a, const PipeRec& r, ProdData<P, T>& o)
+{
+    load16<T, prod_vals<P>()>(o.g, a.crec + (size_t)rec_cell(r.r.x) * a.rs + prod_lo<P>());
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void prod_factor(const ProdData<P, T>& dP, T cx, T cy, T* fac, int stride)
+{
+    using L = Lay<P>;
+    constexpr int FV = fac_vals<P>();
+    T A[P][P];
+    fused_build<P, T>(A, dP.g[L::R_FUSE - prod_lo<P>()], dP.g + (L::R_N - prod_lo<P>()), cx, cy);
+    fused_eliminate<P, T>(A);
+    T f[FV];
+#pragma unroll
+    for (int k = 0; k < FV; ++k) f[k] = k < P * P ? A[k / P][k % P] : T(0);
+#pragma unroll
+    for (int g = 0; g < FV / 4; ++g) lds_store<T, 4>(fac + (size_t)g * stride, f + 4 * g);
+}
+
+template <int P, typename T, bool GREAD, int BF>
+__device__ __forceinline__ void sweep_split_body(const SweepArgs<T>& a, int blk)
+{
+    constexpr int FV = fac_vals<P>();
+    constexpr int W = Tr<P>::W;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int bundle = a.block_bundle[blk];
+    const int B = BF ? 1 : a.bundle_ndir[bundle];
+    T* lds = reinterpret_cast<T*>(lds_raw);
+    const bool global_mode = a.bundle_global[bundle] != 0;
+    const int half = (int)blockDim.x / 2;
+    const int t = threadIdx.x;
+    // the halves are whole waves (half is a multiple of 64): uniform per wave
+    const bool producer = __builtin_amdgcn_readfirstlane(t) >= half;
+    const int tr = producer ? t - half : t;
+    const int b = BF ? 0 : tr % B;
+    const int c = BF ? tr : tr / B;
+    const int d = a.bundle_dirs[a.bundle_dir_off[bundle] + b];
+    const T cx = a.cxv[d], cy = a.cyv[d];
+    const int N = a.N;
+    const int64_t it0 = a.item_off[bundle];
+    const __amdgpu_buffer_rsrc_t irec = bundle_rsrc(a.irec + it0, (uint32_t)N * 16u);
+    T* vdf_d = a.vdf + (size_t)d * N * P;
+    const int l0 = a.lev_off[2 * bundle], l1 = a.lev_off[2 * bundle + 1];
+    const int nthr = a.nthr;
+    const bool sweeps = __builtin_amdgcn_readfirstlane(tr) < nthr;
+    if (sweeps && !producer) {
+        T* z = lds + (size_t)a.pool * B * W;
+        for (int k = tr; k < B * W; k += nthr) z[k] = T(0);
+        constexpr int PV = pad4(P);
+        const int w0 = a.wc_off[2 * bundle], nw = a.wc_off[2 * bundle + 1];
+        const T thermal_on = a.homogeneous ? T(0) : T(1);
+        for (int k = tr; k < nw * B * PV; k += nthr) {
+            const int e = k / (B * PV), bb = (k / PV) % B, m = k % PV;
+            const int4 ent = a.wcell[w0 + e];
+            const int db = a.bundle_dirs[a.bundle_dir_off[bundle] + bb];
+            const T* cr = a.crec + (size_t)ent.x * a.rs;
+            lds[a.wbase + k] = m < P ? wall_inflow<P, T>(cr, ent, a.cxv[db], a.cyv[db], a.wtr, thermal_on, m) : T(0);
+        }
+    }
+    __syncthreads();
+    // the lane's factor values in buffer k: group g at fac(k) + g * 4 * half
+    T* fac_base = lds + a.fbase + (size_t)tr * 4;
+    const int fstride = 4 * half;
+    auto fac = [&](int lv) { return fac_base + (size_t)((lv - l0) & 1) * (FV / 4) * fstride; };
+    const int nlev = l1 > l0 ? l1 - l0 : 0;
+    const int ntrip = (nlev + 2) / 3;
+    if (!sweeps) {
+        for (int k = 0; k < 1 + 3 * ntrip; ++k) level_barrier(global_mode);
+        return;
+    }
+    const int32_t* lp = a.lev_ptr + (t & a.lane_zero);
+    auto span = [&](int lv) {
+        const int k = min(lv, l1 - 1);
+        const int2 v = make_int2(lp[k], lp[k + 1]);
+        return lv < l1 ? v : make_int2(0, 0);
+    };
+    if (producer) {
+        // production n: the factors of level l0 + n into buffer n & 1 while
+        // the consumers solve level l0 + n - 1; its records were read four
+        // productions ahead, its cell data gathered two ahead (register set
+        // n mod 3); one barrier each, 1 + 3 ntrip in all
+        auto produce = [&](const ProdData<P, T>& dS, PipeRec& rN, int2& sN, int lvN, ProdData<P, T>& dG,
+                           const PipeRec& rG, int p) {
+            pipe_record(irec, sN, c, B, b, rN);
+            sN = span(lvN);
+            prod_gather<P, T>(a, rG, dG);
+            prod_factor<P, T>(dS, cx, cy, fac(p), fstride);
+            level_barrier(global_mode);
+        };
+        int2 S0 = span(l0 + 6), S1 = span(l0 + 4), S2 = span(l0 + 5);
+        PipeRec R0, R1, R2;
+        ProdData<P, T> D0, D1, D2;
+        pipe_record(irec, span(l0), c, B, b, R0);
+        pipe_record(irec, span(l0 + 1), c, B, b, R1);
+        prod_gather<P, T>(a, R0, D0);
+        pipe_record(irec, span(l0 + 2), c, B, b, R2);
+        prod_gather<P, T>(a, R1, D1);
+        pipe_record(irec, span(l0 + 3), c, B, b, R0);
+        produce(D0, R1, S1, l0 + 7, D2, R2, l0);
+        for (int n = 1; n < 1 + 3 * ntrip; n += 3) {
+            produce(D1, R2, S2, l0 + n + 7, D0, R0, l0 + n);
+            produce(D2, R0, S0, l0 + n + 8, D1, R1, l0 + n + 1);
+            produce(D0, R1, S1, l0 + n + 9, D2, R2, l0 + n + 2);
+        }
+    } else {
+        auto level = [&](const PipeData<P, T, LU_SPLIT>& dS, PipeRec& rN, int2& sN, int lvN,
+                         PipeData<P, T, LU_SPLIT>& dG, const PipeRec& rG, int lv) {
+            PipeIn<P, T> in;
+            pipe_read<P, T, LU_SPLIT, GREAD>(a, dS, d, lds, vdf_d, in);
+            T fv[FV];
+            const T* fb = fac(lv);
+#pragma unroll
+            for (int g = 0; g < FV / 4; ++g) lds_load<T, 4>(fv + 4 * g, fb + (size_t)g * fstride);
+            pipe_record(irec, sN, c, B, b, rN);
+            sN = span(lvN);
+            pipe_gather<P, T, LU_SPLIT>(a, rG, B, b, irec, dG);
+            pipe_compute<P, T, LU_SPLIT>(a, dS, in, cx, cy, B, b, lds, vdf_d, nullptr, fv);
+            level_barrier(global_mode);
+        };
+        level_barrier(global_mode);            // the producers' first level
+        int2 S0 = span(l0 + 6), S1 = span(l0 + 4), S2 = span(l0 + 5);
+        PipeRec R0, R1, R2;
+        PipeData<P, T, LU_SPLIT> D0, D1, D2;
+        pipe_record(irec, span(l0), c, B, b, R0);
+        pipe_record(irec, span(l0 + 1), c, B, b, R1);
+        pipe_gather<P, T, LU_SPLIT>(a, R0, B, b, irec, D0);
+        pipe_record(irec, span(l0 + 2), c, B, b, R2);
+        pipe_gather<P, T, LU_SPLIT>(a, R1, B, b, irec, D1);
+        pipe_record(irec, span(l0 + 3), c, B, b, R0);
+        for (int n = 0; n < 3 * ntrip; n += 3) {
+            level(D0, R1, S1, l0 + n + 7, D2, R2, l0 + n);
+            level(D1, R2, S2, l0 + n + 8, D0, R0, l0 + n + 1);
+            level(D2, R0, S0, l0 + n + 9, D1, R1, l0 + n + 2);
+        }
+    }
+}
+
+template <int P, typename T, bool GREAD, int BF>
+__global__ void __launch_bounds__(512) sweep_split_kernel(SweepArgs<T> a)
+{
+    sweep_split_body<P, T, GREAD, BF>(a, blockIdx.x);
 }
 
 template <int P, typename T, int LUM, bool GREAD, int DEPTH, int BF>
@@ -2288,22 +2494,32 @@ __global__ void aat_rows_kernel(const double* __restrict__ aat, int64_t N, int P
     out[g] = aat[(((size_t)i * 3 + l) * 7 * P + p0 + p) * 3 * Q + r];
 }
 
-// the fused sweep's five element scalars (area2 / tau_c, area2 times the
-// gradient coefficients) from the cell operators [N][8] into the cell
-// records' padding at column rf, both precisions
+// the fused sweep's element scalar area2 / tau_c from the cell operators
+// [N][8] (area2, the gradient coefficients (y3-y1)/s, (y1-y2)/s, (x1-x3)/s,
+// (x2-x1)/s, ...) into the cell records' padding at column rf, both
+// precisions; and the check that area2 times the gradient coefficients are
+// minus the length-scaled normals the sweep reads at column rn (faces 2 and
+// 0), counting elements off by more than 1e-10 of the largest in *bad
 __global__ void __launch_bounds__(256) fused_scalars_kernel(const double* __restrict__ cell, double tau_c, int N,
-                                                            int rf, double* crec64, int rs64, float* crec32,
-                                                            int rs32)
+                                                            int rf, int rn, double* crec64, int rs64, float* crec32,
+                                                            int rs32, int32_t* bad)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const double* c = cell + (size_t)i * 8;
-    const double v[5] = {c[0] / tau_c, c[0] * c[1], c[0] * c[3], c[0] * c[2], c[0] * c[4]};
+    const double am = c[0] / tau_c;
+    crec64[(size_t)i * rs64 + rf] = am;
+    crec32[(size_t)i * rs32 + rf] = (float)am;
+    const double* nl = crec64 + (size_t)i * rs64 + rn;
+    const double g[4] = {c[0] * c[1], c[0] * c[3], c[0] * c[2], c[0] * c[4]};
+    const double n[4] = {nl[4], nl[5], nl[0], nl[1]};
+    double big = 0.0, err = 0.0;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        crec64[(size_t)i * rs64 + rf + k] = v[k];
-        crec32[(size_t)i * rs32 + rf + k] = (float)v[k];
+    for (int k = 0; k < 4; ++k) {
+        big = fmax(big, fabs(n[k]));
+        err = fmax(err, fabs(g[k] + n[k]));
     }
+    if (!(err <= 1e-10 * big)) atomicAdd(bad, 1);
 }
 
 template <typename T>
@@ -3692,6 +3908,7 @@ SweepArgs<T> sweep_args(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int 
     a.lane_zero = 0;
     a.pool = ctx->pool_slots;
     a.wbase = (ctx->pool_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W;
+    a.fbase = a.wbase + ctx->max_wall_in * ctx->max_bundle_dirs * pad4(P);
     a.nthr = ctx->block;
     for (int k = 0; k < ctx->Q * ctx->Q; ++k) a.kref[k] = S::kref(ctx)[k];
     a.mass = ctx->mass; a.gx = ctx->gx; a.gy = ctx->gy; a.fcm = ctx->fcm; a.geo = ctx->geo64;
@@ -3727,6 +3944,30 @@ template <int P, typename T> bool runs_fused(const pbte_ctx* ctx)
     return std::is_same<T, float>::value && P <= 10 && ctx->fused32 && pipelined_shape<P, T>(ctx);
 }
 
+// LDS bytes of the split fused sweep: the pipelined kernel's plus two
+// factor buffers of the consumer width
+template <int P, typename T> size_t split_lds_bytes(const pbte_ctx* ctx)
+{
+    return sweep_lds_bytes<P, T>(ctx) + (size_t)2 * fac_vals<P>() * ctx->block * sizeof(T);
+}
+
+// ... and runs it as the split kernel (producer and consumer halves)
+template <int P, typename T> bool runs_split(const pbte_ctx* ctx)
+{
+    return P <= 6 && runs_fused<P, T>(ctx) && ctx->split && ctx->block % 64 == 0 && 2 * ctx->block <= 512 &&
+           split_lds_bytes<P, T>(ctx) <= ctx->lds_blk;
+}
+
+template <int P, typename T, bool GR>
+void launch_split(pbte_ctx* ctx, const SweepArgs<T>& a, size_t lds)
+{
+    auto* fn = ctx->max_bundle_dirs == 1 ? sweep_split_kernel<P, T, GR, 1> : sweep_split_kernel<P, T, GR, 0>;
+    if (lds > 65536)
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    hipLaunchKernelGGL(fn, dim3(ctx->nb), dim3(2 * ctx->block), lds, ctx->stream, a);
+}
+
 template <int P, typename T>
 void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous, bool onthefly)
 {
@@ -3739,10 +3980,19 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
         if constexpr (std::is_same<T, float>::value && P <= 10) {
             const size_t lds = sweep_lds_bytes<P, T>(ctx);
             const bool gread = ctx->global_bundles > 0 || ctx->old_reads > 0 || (use_pbuf && ctx->periodic_reads > 0);
-            if (ctx->depth == 3) {
+            bool split = false;
+            if constexpr (P <= 6) {
+                if (runs_split<P, T>(ctx)) {
+                    const size_t slds = split_lds_bytes<P, T>(ctx);
+                    if (gread) launch_split<P, T, true>(ctx, a, slds);
+                    else launch_split<P, T, false>(ctx, a, slds);
+                    split = true;
+                }
+            }
+            if (!split && ctx->depth == 3) {
                 if (gread) launch_pipe<P, T, LU_FUSED, true, 3>(ctx, a, lds);
                 else launch_pipe<P, T, LU_FUSED, false, 3>(ctx, a, lds);
-            } else {
+            } else if (!split) {
                 if (gread) launch_pipe<P, T, LU_FUSED, true, 2>(ctx, a, lds);
                 else launch_pipe<P, T, LU_FUSED, false, 2>(ctx, a, lds);
             }
@@ -3933,12 +4183,39 @@ extern "C" int pbte_set_cell_operators(pbte_ctx* ctx, const double* ref, const d
         case 10: tables = fused_tables_match<10>(ctx->h_fref); break;
         }
         ctx->fused_tables = tables;
+        HIP_TRY(hipMemsetAsync(ctx->counter, 0, sizeof(int32_t), ctx->stream));
         hipLaunchKernelGGL(fused_scalars_kernel, dim3(nblocks(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->cellop,
-                           ctx->tau_c, ctx->N, 3 * P + 9, ctx->crec64, ctx->rs64, ctx->crec32, ctx->rs32);
+                           ctx->tau_c, ctx->N, 3 * P + 9, 3 * P, ctx->crec64, ctx->rs64, ctx->crec32, ctx->rs32,
+                           ctx->counter);
         HIP_TRY(hipGetLastError());
+        int32_t bad = 0;
+        HIP_TRY(hipMemcpyAsync(&bad, ctx->counter, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->fused_geometry = bad == 0;
         ctx->have_cellops = true;
         return 0;
+    });
+}
+
+extern "C" int pbte_sweep_path(pbte_ctx* ctx, int32_t precision)
+{
+    return guarded(ctx, [&]() {
+        if (precision != PBTE_FP64 && precision != PBTE_FP32) throw std::string("bad precision");
+        const bool f32 = precision == PBTE_FP32;
+        const bool stored = f32 ? ctx->have32 : ctx->have64;
+        int path = 0;
+        switch (ctx->P) {
+#define SP(PP)                                                                                                  \
+    case PP:                                                                                                    \
+        if (f32 && runs_split<PP, float>(ctx)) path = 4;                                                        \
+        else if (f32 && runs_fused<PP, float>(ctx)) path = 3;                                                   \
+        else if (!stored) path = f32 ? 5 : (ctx->have32 && ctx->refop && PP <= 10 ? 6 : 5);                     \
+        else path = (f32 ? runs_pipelined<PP, float>(ctx) : runs_pipelined<PP, double>(ctx)) ? 2 : 1;           \
+        break;
+            SP(3) SP(6) SP(10) SP(15)
+#undef SP
+        }
+        return path;
     });
 }
 
@@ -3949,12 +4226,16 @@ extern "C" int pbte_set_fused(pbte_ctx* ctx, int32_t on)
             ctx->fused32 = false;
             return 0;
         }
+        if (on != 1 && on != 2) throw std::string("fused sweep: on is 0, 1 or 2");
+        ctx->split = on == 2;
         if (ctx->P > 10) throw std::string("fused sweep: element degree above 3");
         if (!ctx->have_cellops) throw std::string("fused sweep: pbte_set_cell_operators first");
         if (!ctx->fused_ok)
             throw std::string("fused sweep: face mass blocks are not a length times one unit-edge matrix");
         if (!ctx->fused_tables)
             throw std::string("fused sweep: the reference-element matrices differ from the compiled tables");
+        if (!ctx->fused_geometry)
+            throw std::string("fused sweep: the gradient coefficients are not the faces' normals (face order?)");
         bool shape = false;
         switch (ctx->P) {
         case 3: shape = pipelined_shape<3, float>(ctx); break;

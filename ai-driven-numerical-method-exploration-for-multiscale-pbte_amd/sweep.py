@@ -181,15 +181,24 @@ class SweepContext:
             self.factorise("fp64+fp32" if self.precision == "fp32" and not self.fused else "fp64")
         return self
 
+    _PATHS = {1: "plain", 2: "pipelined", 3: "fused", 4: "fused-split", 5: "onthefly", 6: "refined"}
+
+    def sweep_path(self, precision: str = "fp32") -> str:
+        """The kernel a sweep of this precision runs: "plain", "pipelined"
+        (stored factors), "fused", "fused-split", "onthefly" or "refined"."""
+        code = _native.PBTE_FP64 if precision == "fp64" else _native.PBTE_FP32
+        return self._PATHS[_native.check(self.handle, _native.lib().pbte_sweep_path(self.handle, code), "sweep_path")]
+
     def set_fused(self, on: bool) -> bool:
         """Switch the fused binary32 sweep on (True when it runs here: the
-        context's elements and schedule allow it) or off."""
+        context's elements and schedule allow it; "split": in the split
+        kernel where it fits) or off."""
         h = self.handle
         if not on:
             _native.check(h, _native.lib().pbte_set_fused(h, 0), "set_fused")
             self.fused = False
             return False
-        self.fused = _native.lib().pbte_set_fused(h, 1) == 0
+        self.fused = _native.lib().pbte_set_fused(h, 2 if on == "split" else 1) == 0
         return self.fused
 
     def __del__(self):

@@ -276,6 +276,8 @@ def config2_workload(pkg, args, work):
     t0 = time.perf_counter()
     s = pkg.Solver(case, max_bundle=args.bundle, block=args.block, sweep_variant=args.variant)
     setup_s = time.perf_counter() - t0
+    if args.split and s.ctx.fused:
+        s.ctx.set_fused("split")
     D, N, P = s.ndir, s.n_tris, s.ndof
     log(f"solver set up in {setup_s:.1f} s ({N} triangles); sweeping")
     wl = (f"config 2: jittered square {n}x{n} (+-0.25h, seed 0) -> {N} P{args.deg} triangles, "
@@ -386,6 +388,7 @@ def sweep_line(s, args, D, N, P, precision, workload):
         "tile_shape": s.tile_shape,
         "factor_mode": s.ctx.mode,
         "fp32_operator": "fused" if precision == "fp32" and s.ctx.fused else "stored",
+        "sweep_path": s.ctx.sweep_path(precision),
     }
 
 
@@ -757,6 +760,8 @@ def main():
                     help="config2: skip timing the stored-inverse binary32 sweep beside the fused one")
     ap.add_argument("--fp32-operator", default="fused", choices=["fused", "stored"],
                     help="binary32 sweeps: element operators fused into the sweep or stored inverses")
+    ap.add_argument("--split", action="store_true",
+                    help="fused binary32 sweeps in the split kernel (producer and consumer halves)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-solve", action="store_true", help="skip the Krylov solve leg")
     ap.add_argument("--solve-kn", default="0.01,0.1,1,10",
@@ -811,6 +816,8 @@ def main():
     t0 = time.perf_counter()
     s = pkg.Solver(case, max_bundle=args.bundle, block=args.block, sweep_variant=args.variant)
     setup_s = time.perf_counter() - t0
+    if args.split and s.ctx.fused:
+        s.ctx.set_fused("split")
     log(f"solver set up in {setup_s:.1f} s ({s.n_tris} triangles, factor mode {s.ctx.mode}); sweeping")
     D, N, P = s.ndir, s.n_tris, s.ndof
     vdf = s.sweep_vdf
@@ -855,6 +862,7 @@ def main():
         "setup_s": setup_s,
         "factor_mode": s.ctx.mode,
         "fp32_operator": "fused" if args.precision == "fp32" and s.ctx.fused else "stored",
+        "sweep_path": s.ctx.sweep_path(args.precision),
     }
     if s.ctx.mode == "mixed":
         # binary64 sweeps of the refinement build their operators on the fly

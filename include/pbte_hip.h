@@ -222,8 +222,21 @@ int pbte_set_cell_operators(pbte_ctx *ctx, const double *ref, const double *cell
  * pbte_set_cell_operators first, P <= 10, straight-sided elements (every
  * face mass block a length times one unit-edge matrix, checked when the
  * context is created) and a context that runs the pipelined kernel; else
- * -1 with the reason in pbte_last_error.  on == 0 switches it off. */
+ * -1 with the reason in pbte_last_error.  on == 0 switches it off.  With
+ * on == 2, P <= 6 and room in the LDS for two factor buffers, a sweep runs
+ * the split kernel: a workgroup of twice the width whose second half
+ * builds and factorises the next level's operators into LDS while the
+ * first half solves the current level (the same values; slower than one
+ * wave per item on the configurations measured, whose vector-memory
+ * address unit is the busier resource). */
 int pbte_set_fused(pbte_ctx *ctx, int32_t on);
+
+/* Which kernel a sweep of this precision runs (pbte_sweep for stored
+ * factors, else pbte_sweep_onthefly): 1 plain level kernel, 2 pipelined
+ * with stored factors, 3 fused, 4 split fused, 5 on the fly, 6 binary64
+ * refined from the binary32 inverse ("mixed").  No counterpart in the
+ * reference: for tests and benchmark records. */
+int pbte_sweep_path(pbte_ctx *ctx, int32_t precision);
 
 /* ---- batched sweeps (no counterpart in the reference) ------------------
  * The sweeps of several independent problems in one persistent launch of

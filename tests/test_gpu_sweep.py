@@ -123,6 +123,7 @@ def test_fused_fp32_sweep_matches_oracle(pkg, tag, deg):
     c = _case(pkg, tag, performance__precision="fp32", dg__deg=deg)
     s, o = _pair(pkg, c)
     assert s.ctx.fused and s.ctx.mode == "precomputed" and s.ctx._precomputed == {"fp64"}
+    assert s.ctx.sweep_path("fp32") == "fused"
     _seed_state(s, o, seed=31)
     for _ in range(2):
         s.ctx.sweep(s.mom, s.vdf32_d)
@@ -142,6 +143,31 @@ def test_fused_sweep_bitwise_across_schedules(pkg, bundle, depth):
         s = pkg.Solver(c, max_bundle=b, sweep_variant=dp)
         assert s.ctx.fused
         rng = np.random.default_rng(41)
+        s.vdf = rng.standard_normal(s.sweep_vdf.shape)
+        s.mom.set_dofs(*(rng.standard_normal((s.ndof, s.n_tris)) for _ in range(3)))
+        s.ctx.flux_wall.copy_(torch.from_numpy(rng.standard_normal(s.ctx.flux_wall.shape)))
+        for _ in range(2):
+            s.ctx.sweep(s.mom, s.sweep_vdf)
+        outs.append(s.vdf)
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("tag,deg,bundle", [("F1_1_1", 1, 1), ("F1_1_1", 2, 2), ("F2_1_1", 2, 1),
+                                            ("F3_1_1", 2, 2), ("F3_1_1", 1, 1)])
+def test_split_fused_sweep_is_the_one_wave_fused_sweep(pkg, tag, deg, bundle):
+    """The split kernel (the second half of the workgroup factorising the
+    next level's operators into LDS while the first half solves) gives the
+    one-wave fused sweep's bits: thermalising, adiabatic and periodic walls
+    (previous-sweep reads), degrees 1 and 2, one and two directions per
+    workgroup, two sweeps in a row."""
+    import torch
+    c = _case(pkg, tag, performance__precision="fp32", dg__deg=deg)
+    outs = []
+    for mode in ("split", True):
+        s = pkg.Solver(c, max_bundle=bundle)
+        assert s.ctx.set_fused(mode)
+        assert s.ctx.sweep_path("fp32") == ("fused-split" if mode == "split" else "fused")
+        rng = np.random.default_rng(51)
         s.vdf = rng.standard_normal(s.sweep_vdf.shape)
         s.mom.set_dofs(*(rng.standard_normal((s.ndof, s.n_tris)) for _ in range(3)))
         s.ctx.flux_wall.copy_(torch.from_numpy(rng.standard_normal(s.ctx.flux_wall.shape)))
