@@ -90,6 +90,7 @@ _SIGS = {
     "pbte_set_cell_operators": (C.c_int, [C.c_void_p, _f64p, _f64p]),
     "pbte_set_fused": (C.c_int, [C.c_void_p, C.c_int32]),
     "pbte_sweep_path": (C.c_int, [C.c_void_p, C.c_int32]),
+    "pbte_staging": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "pbte_schedule_info": (C.c_int, [C.c_void_p, _i64p]),
     "pbte_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "pbte_timing_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int32]),

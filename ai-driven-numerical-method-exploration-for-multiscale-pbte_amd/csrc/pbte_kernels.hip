@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -46,6 +47,7 @@
 #include <string>
 #include <thread>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "pbte_hip.h"
@@ -208,8 +210,15 @@ struct pbte_ctx {
     int64_t global_bundles = 0;
     int variant = 0;            // 0 pipelined level sweep, 1 plain level sweep
     int max_bundle_dirs = 1;
-    int64_t old_reads = 0;      // previous-sweep reads (broken cycles)
-    int64_t periodic_reads = 0; // upwind reads across periodic faces
+    // upwind reads by kind: previous-sweep values across non-periodic faces
+    // (broken cycles), previous-sweep values across periodic faces, every
+    // periodic face; and of each the reads whose class is not staged
+    int64_t np_old_reads = 0, per_old_reads = 0, periodic_reads = 0;
+    int64_t np_old_unstaged = 0, per_old_unstaged = 0, per_unstaged = 0;
+    int32_t* stage_list = nullptr;  // per class: staged elements, then staged periodic faces
+    int4* stage_off = nullptr;      // per bundle: {start, elements, non-periodic ones, periodic faces}
+    int stage_slots[2] = {0, 0};    // most stage slots of a bundle without / with an explicit periodic buffer
+    bool staging = true;            // pipelined launches stage those reads where their LDS fits
     std::vector<int64_t> h_lu_off;
     int64_t lu_elems = 0, item_elems = 0;
     // factors
@@ -640,7 +649,10 @@ template <typename T> struct SweepArgs {
     const int4* irec;           // per (class, slot): {cell | pool slot, face words}
     T* spill_row;               // write target of idle threads (P values)
     const uint8_t* bundle_global;
+    const int32_t* stage_list;  // per class: the elements its staged faces read, then its staged periodic faces
+    const int4* stage_off;      // per bundle: {its class's start in stage_list, elements, periodic faces, 0}
     int N, F, rs, use_pbuf, homogeneous;
+    int stage_on;               // the stage slots follow the spill slot in LDS
     int lane_zero;              // 0 at run time; makes a uniform load per-lane
     int pool;                   // LDS pool slots
     int wbase;                  // LDS offset (in values) of the wall inflow vectors
@@ -674,13 +686,27 @@ template <typename T> struct SweepArgs {
 //   UP_GLOBAL/PER_GLOBAL  this sweep's values from global memory (value: the
 //                         element), only in a bundle whose LDS pool would
 //                         overflow, which then runs with full barriers
-// With an explicit periodic buffer every periodic face reads that instead.
+//   OLD_ST / PER_OLD_ST   OLD / PER_OLD with the value staged: the sweep's
+//                         prologue copies the traces every such face reads
+//                         into LDS stage slots (value: the stage slot in
+//                         bits 0-12; PER_OLD_ST also the periodic face's
+//                         stage slot for an explicit periodic buffer in bits
+//                         13-25)
+//   PER_LDS_ST            PER_LDS (pool slot in bits 0-12) with the periodic
+//                         face's stage slot in bits 13-25
+// With an explicit periodic buffer every periodic face reads that instead
+// (a staged one from its periodic stage slot).  A launch whose LDS has no
+// room for the stage slots reads the staged faces from global memory (the
+// element across from the element's neighbour list).
 enum {
-    REC_UP_LDS = 0, REC_ADIABATIC = 2, REC_THERMAL = 3, REC_OLD = 4, REC_PER_LDS = 5, REC_PER_OLD = 7,
-    REC_UP_GLOBAL = 8, REC_PER_GLOBAL = 9, REC_NONE = 15
+    REC_UP_LDS = 0, REC_OLD_ST = 1, REC_ADIABATIC = 2, REC_THERMAL = 3, REC_OLD = 4, REC_PER_LDS = 5,
+    REC_PER_OLD_ST = 6, REC_PER_OLD = 7, REC_UP_GLOBAL = 8, REC_PER_GLOBAL = 9, REC_PER_LDS_ST = 10, REC_NONE = 15
 };
 constexpr int kFaceValBits = 26;
 constexpr int kFaceValMask = (1 << kFaceValBits) - 1;
+constexpr int kStageBits = 13;
+constexpr int kStageMax = 1 << kStageBits;      // stage slots of one class (each list)
+constexpr int kStageMask = kStageMax - 1;
 __host__ __device__ constexpr int face_word(int code, int ilp, int val)
 {
     return (int)(((unsigned)code << 28) | ((unsigned)ilp << kFaceValBits) | (unsigned)val);
@@ -701,9 +727,11 @@ __host__ __device__ constexpr int rec_pool(int x) { return (int)((unsigned)x >> 
 // no short-circuit control flow
 __host__ __device__ constexpr int code_bit(int c) { return 1 << c; }
 constexpr int kRecWall = code_bit(REC_ADIABATIC) | code_bit(REC_THERMAL);
+constexpr int kRecStaged = code_bit(REC_OLD_ST) | code_bit(REC_PER_OLD_ST) | code_bit(REC_PER_LDS_ST);
 constexpr int kRecNeighbour = code_bit(REC_UP_LDS) | code_bit(REC_OLD) | code_bit(REC_PER_LDS) |
-                              code_bit(REC_PER_OLD) | code_bit(REC_UP_GLOBAL) | code_bit(REC_PER_GLOBAL);
-constexpr int kRecPeriodic = code_bit(REC_PER_LDS) | code_bit(REC_PER_OLD) | code_bit(REC_PER_GLOBAL);
+                              code_bit(REC_PER_OLD) | code_bit(REC_UP_GLOBAL) | code_bit(REC_PER_GLOBAL) | kRecStaged;
+constexpr int kRecPeriodic = code_bit(REC_PER_LDS) | code_bit(REC_PER_OLD) | code_bit(REC_PER_GLOBAL) |
+                             code_bit(REC_PER_OLD_ST) | code_bit(REC_PER_LDS_ST);
 __device__ __forceinline__ bool code_in(int mask, int code) { return ((mask >> code) & 1) != 0; }
 
 // value m of the direction's source plus the wall inflow v
@@ -944,7 +972,7 @@ __global__ void __launch_bounds__(256) sweep_kernel(SweepArgs<T> a)
                 const T w = T(0.5) * (sp - fabs(sp));
                 T u[TQ];
                 {
-                    const bool per = code == REC_PER_LDS || code == REC_PER_OLD || code == REC_PER_GLOBAL;
+                    const bool per = code_in(kRecPeriodic, code);
                     const T* fe = (per && a.use_pbuf)
                                       ? a.pbuf + ((size_t)d * a.F + (bits_of(cr[L::R_INFO + il]) >> 4)) * P
                                       : vdf_d + (size_t)a.other[(size_t)i * 3 + il] * P;
@@ -1293,9 +1321,55 @@ __device__ __forceinline__ void pipe_record(__amdgpu_buffer_rsrc_t irec, int2 sp
 // record-code classes as bit masks over the code: one shift and one test,
 // no short-circuit control flow
 // faces whose inflow trace comes from the LDS pool
-__device__ __forceinline__ int rec_lds_mask(int use_pbuf)
+// how a face word selects its LDS trace (masks over the record codes,
+// uniform per launch): the codes read from LDS, those whose slot is in bits
+// 13-25, those whose slot counts from the first stage slot
+struct FaceSel { int lds, hi, stage; };
+__device__ __forceinline__ FaceSel face_sel(int use_pbuf, int stage_on)
 {
-    return use_pbuf ? code_bit(REC_UP_LDS) : (code_bit(REC_UP_LDS) | code_bit(REC_PER_LDS));
+    FaceSel f{code_bit(REC_UP_LDS), 0, 0};
+    if (!use_pbuf) f.lds |= code_bit(REC_PER_LDS) | code_bit(REC_PER_LDS_ST);
+    if (stage_on) {
+        f.lds |= kRecStaged;
+        f.stage = code_bit(REC_OLD_ST) | code_bit(REC_PER_OLD_ST) | (use_pbuf ? code_bit(REC_PER_LDS_ST) : 0);
+        f.hi = use_pbuf ? code_bit(REC_PER_OLD_ST) | code_bit(REC_PER_LDS_ST) : 0;
+    }
+    return f;
+}
+
+// the prologue's copy of the staged traces: per stage slot and direction of
+// the bundle the three face traces of an element's previous-sweep values or
+// of a periodic face's explicit-buffer entry, laid out as a pool slot.  A
+// class's list holds the elements of its staged previous-sweep reads (those
+// that some non-periodic face reads first), then its staged periodic faces;
+// without an explicit periodic buffer the stage slots are the elements, with
+// one the elements that non-periodic faces read, then the periodic faces.
+template <int P, typename T>
+__device__ __forceinline__ void stage_traces(const SweepArgs<T>& a, int bundle, int B, int t, int nthr, T* lds)
+{
+    constexpr int DEG = Deg<P>::v;
+    constexpr int Q = DEG + 1;
+    constexpr int TQ = Tr<P>::TQ;
+    constexpr int W = Tr<P>::W;
+    if (!a.stage_on) return;
+    const int4 so = a.stage_off[bundle];          // {start, elements, non-periodic elements, periodic faces}
+    const int nel = a.use_pbuf ? so.z : so.y;
+    const int n = nel + (a.use_pbuf ? so.w : 0);
+    const int32_t* lst = a.stage_list + so.x;
+    T* st = lds + (size_t)(a.pool + 2) * B * W;
+    const int32_t* dirs = a.bundle_dirs + a.bundle_dir_off[bundle];
+    for (int k = t; k < n * B * W; k += nthr) {
+        const int e = k / (B * W), bb = (k / W) % B, m = k % W;
+        const int il = m / TQ, q = m % TQ;
+        T v = T(0);
+        if (il < 3 && q < Q) {
+            const int db = dirs[bb];
+            const T* src = e < nel ? a.vdf + ((size_t)db * a.N + lst[e]) * P
+                                   : a.pbuf + ((size_t)db * a.F + lst[so.y + e - nel]) * P;
+            v = src[face_node(DEG, il, q)];
+        }
+        st[k] = v;
+    }
 }
 
 // gather: only loads whose addresses come from the record of two levels
@@ -1320,17 +1394,20 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
     o.fw[1] = r.r.z;
     o.fw[2] = r.r.w;
     const unsigned zero = ((unsigned)a.pool * B + b) * W;
-    const int lds_mask = rec_lds_mask(a.use_pbuf);
+    const FaceSel fs = face_sel(a.use_pbuf, a.stage_on);
+    const unsigned stage0 = (unsigned)a.pool + 2u;
     unsigned wv = zero;
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
-        // a pool face reads its upwind slot, every other face the all-zero
+        // a pool or stage face reads its slot, every other face the all-zero
         // slot; a wall face names the element's wall inflow (selects, no
         // control flow)
         const int code = fw_code(o.fw[il]);
         const unsigned val = (unsigned)fw_val(o.fw[il]);
-        const unsigned nbr = (val * B + b) * W + (unsigned)fw_ilp(o.fw[il]) * TQ;
-        o.lds[il] = (int)(code_in(lds_mask, code) ? nbr : zero);
+        unsigned slot = code_in(fs.hi, code) ? val >> kStageBits : val & (unsigned)kStageMask;
+        slot += code_in(fs.stage, code) ? stage0 : 0u;
+        const unsigned nbr = (slot * B + b) * W + (unsigned)fw_ilp(o.fw[il]) * TQ;
+        o.lds[il] = (int)(code_in(fs.lds, code) ? nbr : zero);
         wv = code_in(kRecWall, code) ? (unsigned)a.wbase + (val * B + b) * PV : wv;
     }
     o.wv = (int)wv;
@@ -1372,8 +1449,7 @@ __device__ __forceinline__ void pipe_read(const SweepArgs<T>& a, const PipeData<
     }
     lds_load<T, PV>(in.v, lds + dC.wv);
     if constexpr (GREAD) {
-        const int lds_mask = rec_lds_mask(a.use_pbuf);
-        const int gmask = dC.valid ? kRecNeighbour & ~lds_mask : 0;
+        const int gmask = dC.valid ? kRecNeighbour & ~face_sel(a.use_pbuf, a.stage_on).lds : 0;
         const bool g0 = code_in(gmask, in.code[0]), g1 = code_in(gmask, in.code[1]),
                    g2 = code_in(gmask, in.code[2]);
         if (__builtin_amdgcn_ballot_w64(g0 || g1 || g2) != 0) {
@@ -1381,10 +1457,14 @@ __device__ __forceinline__ void pipe_read(const SweepArgs<T>& a, const PipeData<
             for (int il = 0; il < 3; ++il) {
                 if (code_in(gmask, in.code[il])) {
                     const int ilp = fw_ilp(dC.fw[il]);
+                    // a staged face's value names a stage slot: the
+                    // element across comes from the neighbour list
+                    const int nb = code_in(kRecStaged, in.code[il]) ? a.other[(size_t)dC.cell * 3 + il]
+                                                                    : fw_val(dC.fw[il]);
                     const T* fe = (code_in(kRecPeriodic, in.code[il]) && a.use_pbuf)
                                       ? a.pbuf + ((size_t)d * a.F +
                                                   (bits_of(a.crec[(size_t)dC.cell * a.rs + L::R_INFO + il]) >> 4)) * P
-                                      : vdf_d + (size_t)fw_val(dC.fw[il]) * P;
+                                      : vdf_d + (size_t)nb * P;
 #pragma unroll
                     for (int q = 0; q < Q; ++q) in.u[il][q] = fe[face_node(DEG, ilp, q)];
                 }
@@ -1476,7 +1556,7 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int bundle = a.block_bundle[blk];
     const int B = BF ? 1 : a.bundle_ndir[bundle];
-    // LDS: pool slots, one all-zero slot, one spill slot, the wall inflow
+    // LDS: pool slots, one all-zero slot, one spill slot, the stage slots, the wall inflow
     T* lds = reinterpret_cast<T*>(lds_raw);
     const bool global_mode = a.bundle_global[bundle] != 0;
     const int t = threadIdx.x;
@@ -1513,6 +1593,7 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
             const T* cr = a.crec + (size_t)ent.x * a.rs;
             lds[a.wbase + k] = m < P ? wall_inflow<P, T>(cr, ent, a.cxv[db], a.cyv[db], a.wtr, thermal_on, m) : T(0);
         }
+        stage_traces<P, T>(a, bundle, B, t, nthr, lds);
     }
     __syncthreads();
     if (!sweeps) {
@@ -1684,6 +1765,7 @@ __device__ __forceinline__ void sweep_split_body(const SweepArgs<T>& a, int blk)
             const T* cr = a.crec + (size_t)ent.x * a.rs;
             lds[a.wbase + k] = m < P ? wall_inflow<P, T>(cr, ent, a.cxv[db], a.cyv[db], a.wtr, thermal_on, m) : T(0);
         }
+        stage_traces<P, T>(a, bundle, B, tr, nthr, lds);
     }
     __syncthreads();
     // the lane's factor values in buffer k: group g at fac(k) + g * 4 * half
@@ -3501,7 +3583,15 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         std::vector<std::vector<int4>> wlists(nc);
         std::vector<int4> h_rec((size_t)nc * N);
         std::vector<uint8_t> global_flag(nc, 0);
-        std::vector<int64_t> peak(nc, 0), old_reads(nc, 0), per_reads(nc, 0), slots_used(nc, 0);
+        std::vector<int64_t> peak(nc, 0), slots_used(nc, 0);
+        // per class: reads by kind (previous-sweep non-periodic, previous-
+        // sweep periodic, periodic), whether its staged reads are encoded as
+        // such, and its stage list with {elements, non-periodic ones,
+        // periodic faces}
+        std::vector<std::array<int64_t, 3>> kind_reads(nc, {0, 0, 0});
+        std::vector<uint8_t> staged(nc, 0);
+        std::vector<std::vector<int32_t>> stage_lists(nc);
+        std::vector<std::array<int32_t, 3>> stage_cnt(nc, {0, 0, 0});
         // levels of every class: slots in level order (the reference order
         // within a level) and the level boundaries
         std::vector<std::vector<int32_t>> lcnt(nc);
@@ -3649,17 +3739,70 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                     pair_overflow = true;
                     break;
                 }
+                // stage lists (a bundle with a pool only): the elements whose
+                // previous-sweep values some face reads, those a non-periodic
+                // face reads first, and the periodic faces read
+                std::unordered_map<int32_t, int32_t> el_idx, pf_idx;
+                if (!overflow) {
+                    std::vector<int32_t> np_el, per_el, pfs;
+                    std::unordered_map<int32_t, int32_t> seen;        // element -> 1 non-periodic, 2 periodic only
+                    for (int pass = 0; pass < 2; ++pass)
+                        for (int q = 0; q < N; ++q)
+                            for (int il = 0; il < 3; ++il) {
+                                const Read& r = rd[(size_t)q * 3 + il];
+                                if (pass == 0 && r.code == REC_OLD && !seen.count(r.val)) {
+                                    seen[r.val] = 1;
+                                    np_el.push_back(r.val);
+                                }
+                                if (pass == 1 && r.code == REC_PER_OLD && !seen.count(r.val)) {
+                                    seen[r.val] = 2;
+                                    per_el.push_back(r.val);
+                                }
+                                if (pass == 1 && (r.code == REC_PER_OLD || r.code == REC_PER_LDS)) {
+                                    const int fc = su->tri_faces[(size_t)sl[q] * 3 + il];
+                                    if (!pf_idx.count(fc)) {
+                                        pf_idx[fc] = (int32_t)pfs.size();
+                                        pfs.push_back(fc);
+                                    }
+                                }
+                            }
+                    const int nel = (int)(np_el.size() + per_el.size()), nnp = (int)np_el.size();
+                    const int npf = (int)pfs.size();
+                    if (nel + npf > 0 && nel < kStageMax && nnp + npf < kStageMax) {
+                        auto& lst = stage_lists[k];
+                        lst = np_el;
+                        lst.insert(lst.end(), per_el.begin(), per_el.end());
+                        for (int e = 0; e < nel; ++e) el_idx[lst[e]] = e;
+                        lst.insert(lst.end(), pfs.begin(), pfs.end());
+                        stage_cnt[k] = {nel, nnp, npf};
+                        staged[k] = 1;
+                    } else {
+                        pf_idx.clear();
+                    }
+                }
                 for (int q = 0; q < N; ++q) {
                     int w[3];
                     for (int il = 0; il < 3; ++il) {
                         Read r = rd[(size_t)q * 3 + il];
+                        if (r.code == REC_OLD) ++kind_reads[k][0];
+                        if (r.code == REC_PER_OLD) ++kind_reads[k][1];
+                        if (r.code == REC_PER_LDS || r.code == REC_PER_OLD) ++kind_reads[k][2];
                         if (r.code == REC_UP_LDS || r.code == REC_PER_LDS) {
                             if (overflow) r = {r.code == REC_UP_LDS ? REC_UP_GLOBAL : REC_PER_GLOBAL, sl[r.val], r.ilp};
                             else r.val = sid[r.val];
                         }
+                        if (staged[k]) {
+                            // the periodic face's slot with an explicit buffer
+                            // follows the non-periodic elements' slots
+                            const bool per = r.code == REC_PER_OLD || r.code == REC_PER_LDS;
+                            const int hi = per ? (stage_cnt[k][1] + pf_idx.at(su->tri_faces[(size_t)sl[q] * 3 + il]))
+                                                     << kStageBits
+                                               : 0;
+                            if (r.code == REC_OLD) r = {REC_OLD_ST, el_idx.at(r.val), r.ilp};
+                            else if (r.code == REC_PER_OLD) r = {REC_PER_OLD_ST, el_idx.at(r.val) | hi, r.ilp};
+                            else if (r.code == REC_PER_LDS) r = {REC_PER_LDS_ST, r.val | hi, r.ilp};
+                        }
                         w[il] = face_word(r.code, r.ilp, r.val);
-                        if (r.code == REC_OLD || r.code == REC_PER_OLD) ++old_reads[k];
-                        if (r.code == REC_PER_LDS || r.code == REC_PER_OLD || r.code == REC_PER_GLOBAL) ++per_reads[k];
                     }
                     const int wst = overflow ? -1 : sid[q];
                     h_rec[(size_t)k * N + q] =
@@ -3750,13 +3893,32 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->irec = ctx->upload(h_rec.data(), h_rec.size());
         phase("item records");
         ctx->bundle_global = ctx->upload(b_global.data(), b_global.size());
+        std::vector<int32_t> h_stage, c_stage_off(nc, 0);
+        for (int c = 0; c < nc; ++c) {
+            c_stage_off[c] = (int32_t)h_stage.size();
+            h_stage.insert(h_stage.end(), stage_lists[c].begin(), stage_lists[c].end());
+        }
+        std::vector<int4> h_stage_off(nb);
         for (int k = 0; k < nb; ++k) {
             const int c = cls_of[k];
             ctx->global_bundles += global_flag[c];
             ctx->pool_peak = std::max(ctx->pool_peak, peak[c]);
-            ctx->old_reads += old_reads[c];
-            ctx->periodic_reads += per_reads[c];
+            ctx->np_old_reads += kind_reads[c][0];
+            ctx->per_old_reads += kind_reads[c][1];
+            ctx->periodic_reads += kind_reads[c][2];
+            if (!staged[c]) {
+                ctx->np_old_unstaged += kind_reads[c][0];
+                ctx->per_old_unstaged += kind_reads[c][1];
+                ctx->per_unstaged += kind_reads[c][2];
+            }
+            const auto& sc = stage_cnt[c];
+            h_stage_off[k] = make_int4(c_stage_off[c], sc[0], sc[1], sc[2]);
+            ctx->stage_slots[0] = std::max(ctx->stage_slots[0], sc[0]);
+            ctx->stage_slots[1] = std::max(ctx->stage_slots[1], sc[1] + sc[2]);
         }
+        h_stage.resize(std::max<size_t>(h_stage.size(), 1), 0);
+        ctx->stage_list = ctx->upload(h_stage.data(), h_stage.size());
+        ctx->stage_off = ctx->upload(h_stage_off.data(), h_stage_off.size());
         ctx->item_off = ctx->upload(h_item_off.data(), nb + 1);
         ctx->h_lu_off = h_lu_off;
         ctx->lu_elems = h_lu_off[nb];
@@ -3870,12 +4032,33 @@ template <> struct SweepData<float> {
     static const float* kref(const pbte_ctx* c) { return c->kref32; }
 };
 
-// LDS bytes of the pipelined kernel: pool, zero and spill slots, wall traces
-template <int P, typename T> size_t sweep_lds_bytes(const pbte_ctx* ctx)
+// LDS bytes of the pipelined kernel: pool, zero and spill slots, stage
+// slots, wall traces
+template <int P, typename T> size_t sweep_lds_bytes(const pbte_ctx* ctx, int stage = 0)
 {
-    return ((size_t)(ctx->pool_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W +
+    return ((size_t)(ctx->pool_slots + 2 + stage) * ctx->max_bundle_dirs * Tr<P>::W +
             (size_t)ctx->max_wall_in * ctx->max_bundle_dirs * pad4(P)) *
            sizeof(T);
+}
+
+// stage slots a pipelined launch holds (all the context's staged reads, or
+// none when they do not fit beside the rest of its LDS, extra bytes
+// included: its staged faces then read global memory)
+template <int P, typename T> int stage_slots(const pbte_ctx* ctx, int use_pbuf, size_t extra = 0)
+{
+    const int n = ctx->stage_slots[use_pbuf ? 1 : 0];
+    return ctx->staging && n > 0 && sweep_lds_bytes<P, T>(ctx, n) + extra <= ctx->lds_blk ? n : 0;
+}
+
+// whether a pipelined launch reads global memory for upwind values (the
+// kernel instance with the global-read branch)
+bool needs_gread(const pbte_ctx* ctx, int use_pbuf, bool stage_on)
+{
+    if (ctx->global_bundles > 0) return true;
+    const int64_t np_old = stage_on ? ctx->np_old_unstaged : ctx->np_old_reads;
+    const int64_t per = use_pbuf ? (stage_on ? ctx->per_unstaged : ctx->periodic_reads)
+                                 : (stage_on ? ctx->per_old_unstaged : ctx->per_old_reads);
+    return np_old + per > 0;
 }
 
 template <int P, typename T, int PF, bool GR, int DEPTH>
@@ -3891,7 +4074,7 @@ void launch_pipe(pbte_ctx* ctx, const SweepArgs<T>& a, size_t lds)
 
 // the kernel arguments of one context's sweep
 template <int P, typename T>
-SweepArgs<T> sweep_args(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous)
+SweepArgs<T> sweep_args(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous, int stage = 0)
 {
     using S = SweepData<T>;
     SweepArgs<T> a{};
@@ -3905,9 +4088,10 @@ SweepArgs<T> sweep_args(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int 
     a.irec = ctx->irec;
     a.spill_row = reinterpret_cast<T*>(ctx->spill_row);
     a.bundle_global = ctx->bundle_global;
+    a.stage_list = ctx->stage_list; a.stage_off = ctx->stage_off; a.stage_on = stage > 0;
     a.lane_zero = 0;
     a.pool = ctx->pool_slots;
-    a.wbase = (ctx->pool_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W;
+    a.wbase = (ctx->pool_slots + 2 + stage) * ctx->max_bundle_dirs * Tr<P>::W;
     a.fbase = a.wbase + ctx->max_wall_in * ctx->max_bundle_dirs * pad4(P);
     a.nthr = ctx->block;
     for (int k = 0; k < ctx->Q * ctx->Q; ++k) a.kref[k] = S::kref(ctx)[k];
@@ -3971,22 +4155,26 @@ void launch_split(pbte_ctx* ctx, const SweepArgs<T>& a, size_t lds)
 template <int P, typename T>
 void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homogeneous, bool onthefly)
 {
-    const SweepArgs<T> a = sweep_args<P, T>(ctx, pbuf, vdf, use_pbuf, homogeneous);
+    // the split kernel's factor buffers follow the rest of its LDS (a
+    // fused context runs the fused kernels on either entry point)
+    bool split = false;
+    if constexpr (P <= 6) split = runs_split<P, T>(ctx);
+    const size_t extra = split ? split_lds_bytes<P, T>(ctx) - sweep_lds_bytes<P, T>(ctx) : 0;
+    const bool piped = runs_fused<P, T>(ctx) || (!onthefly && runs_pipelined<P, T>(ctx));
+    const int stage = piped ? stage_slots<P, T>(ctx, use_pbuf, extra) : 0;
+    const SweepArgs<T> a = sweep_args<P, T>(ctx, pbuf, vdf, use_pbuf, homogeneous, stage);
+    const size_t lds = sweep_lds_bytes<P, T>(ctx, stage);
+    const bool gread = needs_gread(ctx, use_pbuf, stage > 0);
     const bool timed = 2 * (size_t)ctx->ev_used + 1 < ctx->events.size();
     if (timed) HIP_TRY(hipEventRecord(ctx->events[2 * ctx->ev_used], ctx->stream));
     if (runs_fused<P, T>(ctx)) {
         // binary32 with the element operators built and factorised in the
         // sweep (stored binary32 factors, if any, are not read)
         if constexpr (std::is_same<T, float>::value && P <= 10) {
-            const size_t lds = sweep_lds_bytes<P, T>(ctx);
-            const bool gread = ctx->global_bundles > 0 || ctx->old_reads > 0 || (use_pbuf && ctx->periodic_reads > 0);
-            bool split = false;
             if constexpr (P <= 6) {
-                if (runs_split<P, T>(ctx)) {
-                    const size_t slds = split_lds_bytes<P, T>(ctx);
-                    if (gread) launch_split<P, T, true>(ctx, a, slds);
-                    else launch_split<P, T, false>(ctx, a, slds);
-                    split = true;
+                if (split) {
+                    if (gread) launch_split<P, T, true>(ctx, a, lds + extra);
+                    else launch_split<P, T, false>(ctx, a, lds + extra);
                 }
             }
             if (!split && ctx->depth == 3) {
@@ -4015,9 +4203,7 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
         // sized for binary32, exceeds a workgroup's LDS
         hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(std::min(ctx->block, 256)), 0, ctx->stream, a);
     } else {
-        const size_t lds = sweep_lds_bytes<P, T>(ctx);
         constexpr int pf = lu_stored_mode<P, T>();
-        const bool gread = ctx->global_bundles > 0 || ctx->old_reads > 0 || (use_pbuf && ctx->periodic_reads > 0);
         if (ctx->depth == 3) {
             if (gread) launch_pipe<P, T, pf, true, 3>(ctx, a, lds);
             else launch_pipe<P, T, pf, false, 3>(ctx, a, lds);
@@ -4219,6 +4405,21 @@ extern "C" int pbte_sweep_path(pbte_ctx* ctx, int32_t precision)
     });
 }
 
+extern "C" int pbte_staging(pbte_ctx* ctx, int32_t on, int64_t* out4)
+{
+    return guarded(ctx, [&]() {
+        if (on > 1) throw std::string("staging: on must be -1, 0 or 1");
+        if (on >= 0) ctx->staging = on != 0;
+        if (out4) {
+            out4[0] = ctx->stage_slots[0];
+            out4[1] = ctx->stage_slots[1];
+            out4[2] = ctx->np_old_reads + ctx->per_old_reads - ctx->np_old_unstaged - ctx->per_old_unstaged;
+            out4[3] = ctx->periodic_reads - ctx->per_unstaged;
+        }
+        return ctx->staging ? 1 : 0;
+    });
+}
+
 extern "C" int pbte_set_fused(pbte_ctx* ctx, int32_t on)
 {
     return guarded(ctx, [&]() {
@@ -4399,10 +4600,11 @@ void batch_sweep(pbte_batch* bt, T* const* vdf, const void* const* pbuf, int hom
             throw std::string("batch: a context does not run the pipelined kernel in this precision");
         const T* pb = pbuf ? static_cast<const T*>(pbuf[k]) : nullptr;
         const int use_pbuf = pb != nullptr;
-        const SweepArgs<T> a = sweep_args<P, T>(c, pb, vdf[k], use_pbuf, homogeneous);
+        const int stage = stage_slots<P, T>(c, use_pbuf);
+        const SweepArgs<T> a = sweep_args<P, T>(c, pb, vdf[k], use_pbuf, homogeneous, stage);
         std::memcpy(host.data() + (size_t)k * sizeof(SweepArgs<T>), &a, sizeof(a));
-        lds = std::max(lds, sweep_lds_bytes<P, T>(c));
-        gread = gread || c->global_bundles > 0 || c->old_reads > 0 || (use_pbuf && c->periodic_reads > 0);
+        lds = std::max(lds, sweep_lds_bytes<P, T>(c, stage));
+        gread = gread || needs_gread(c, use_pbuf, stage > 0);
         one_dir = one_dir && c->max_bundle_dirs == 1;
     }
     // the arguments change only with the distributions and the source flag:

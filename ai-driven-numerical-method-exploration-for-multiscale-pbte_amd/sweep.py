@@ -189,6 +189,16 @@ class SweepContext:
         code = _native.PBTE_FP64 if precision == "fp64" else _native.PBTE_FP32
         return self._PATHS[_native.check(self.handle, _native.lib().pbte_sweep_path(self.handle, code), "sweep_path")]
 
+    def staging(self, on=None) -> dict:
+        """LDS staging of the pipelined sweeps' previous-sweep and explicit
+        periodic-buffer reads: switch it on or off (``on`` True/False; None
+        leaves it) and return the setting with the schedule's stage sizes."""
+        out = (C.c_int64 * 4)()
+        st = _native.check(self.handle, _native.lib().pbte_staging(
+            self.handle, -1 if on is None else (1 if on else 0), out), "staging")
+        return {"on": bool(st), "slots": int(out[0]), "slots_pbuf": int(out[1]),
+                "staged_old_reads": int(out[2]), "staged_periodic_reads": int(out[3])}
+
     def set_fused(self, on: bool) -> bool:
         """Switch the fused binary32 sweep on (True when it runs here: the
         context's elements and schedule allow it; "split": in the split

@@ -278,6 +278,8 @@ def config2_workload(pkg, args, work):
     setup_s = time.perf_counter() - t0
     if args.split and s.ctx.fused:
         s.ctx.set_fused("split")
+    if args.no_staging:
+        s.ctx.staging(False)
     D, N, P = s.ndir, s.n_tris, s.ndof
     log(f"solver set up in {setup_s:.1f} s ({N} triangles); sweeping")
     wl = (f"config 2: jittered square {n}x{n} (+-0.25h, seed 0) -> {N} P{args.deg} triangles, "
@@ -389,6 +391,7 @@ def sweep_line(s, args, D, N, P, precision, workload):
         "factor_mode": s.ctx.mode,
         "fp32_operator": "fused" if precision == "fp32" and s.ctx.fused else "stored",
         "sweep_path": s.ctx.sweep_path(precision),
+        "staging": s.ctx.staging(),
     }
 
 
@@ -760,6 +763,9 @@ def main():
                     help="config2: skip timing the stored-inverse binary32 sweep beside the fused one")
     ap.add_argument("--fp32-operator", default="fused", choices=["fused", "stored"],
                     help="binary32 sweeps: element operators fused into the sweep or stored inverses")
+    ap.add_argument("--no-staging", action="store_true",
+                    help="read the previous-sweep and periodic-buffer upwind values from global memory "
+                         "instead of staging them in LDS")
     ap.add_argument("--split", action="store_true",
                     help="fused binary32 sweeps in the split kernel (producer and consumer halves)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -818,6 +824,8 @@ def main():
     setup_s = time.perf_counter() - t0
     if args.split and s.ctx.fused:
         s.ctx.set_fused("split")
+    if args.no_staging:
+        s.ctx.staging(False)
     log(f"solver set up in {setup_s:.1f} s ({s.n_tris} triangles, factor mode {s.ctx.mode}); sweeping")
     D, N, P = s.ndir, s.n_tris, s.ndof
     vdf = s.sweep_vdf
@@ -863,6 +871,7 @@ def main():
         "factor_mode": s.ctx.mode,
         "fp32_operator": "fused" if args.precision == "fp32" and s.ctx.fused else "stored",
         "sweep_path": s.ctx.sweep_path(args.precision),
+        "staging": s.ctx.staging(),
     }
     if s.ctx.mode == "mixed":
         # binary64 sweeps of the refinement build their operators on the fly

@@ -238,6 +238,21 @@ int pbte_set_fused(pbte_ctx *ctx, int32_t on);
  * reference: for tests and benchmark records. */
 int pbte_sweep_path(pbte_ctx *ctx, int32_t precision);
 
+/* Staging of the rare global upwind reads of the pipelined sweeps: the
+ * previous-sweep values a face reads across a broken cycle or a periodic
+ * pair whose partner is swept later, and the explicit periodic buffer's
+ * entries, copied into LDS stage slots by each workgroup's prologue (the
+ * same values: results are bitwise those of the global reads).  A launch
+ * whose LDS has no room for its stage slots reads them from global memory.
+ * on = 1 stages (the default), 0 reads global memory, -1 leaves the
+ * setting; out4 (may be NULL) receives the most stage slots of a bundle
+ * without and with an explicit periodic buffer, the previous-sweep reads
+ * and the periodic reads of the staged schedule classes.  Returns 1 when
+ * staging is on, 0 when off, -1 with the reason in pbte_last_error.  No
+ * counterpart in the reference (its sweep reads every value from the
+ * distribution array). */
+int pbte_staging(pbte_ctx *ctx, int32_t on, int64_t *out4);
+
 /* ---- batched sweeps (no counterpart in the reference) ------------------
  * The sweeps of several independent problems in one persistent launch of
  * the pipelined sweep kernel fed by a device work queue: the work items are

@@ -285,6 +285,41 @@ def test_pipelined_sweep_is_bitwise_plain_sweep(pkg, tag, bundle, cap, depth):
         assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("prec,operator,bundle,use_pbuf", [("fp64", "stored", 1, False), ("fp64", "stored", 2, True),
+                                                            ("fp32", "fused", 1, False), ("fp32", "fused", 2, True),
+                                                            ("fp32", "stored", 2, False), ("fp32", "stored", 1, True)])
+def test_staged_reads_are_bitwise_global_reads(pkg, prec, operator, bundle, use_pbuf):
+    """The periodic partners swept later (previous-sweep values) and the
+    explicit periodic buffer's entries, staged in LDS by the sweep's
+    prologue, give the global reads' values bit for bit."""
+    import torch
+    c = _case(pkg, "F3_1_1", performance__precision=prec, performance__fp32_operator=operator)
+    s = pkg.Solver(c, max_bundle=bundle)
+    info = s.ctx.staging()
+    print(prec, operator, bundle, use_pbuf, info, s.ctx.sweep_path(prec), flush=True)
+    assert info["on"] and info["staged_periodic_reads"] > 0 and info["staged_old_reads"] > 0
+    assert info["slots"] > 0 and info["slots_pbuf"] > 0
+    outs = []
+    for on in (True, False):
+        s.ctx.staging(on)
+        rng = np.random.default_rng(23)
+        s.vdf = rng.standard_normal(s.sweep_vdf.shape)
+        s.mom.set_dofs(*(rng.standard_normal((s.ndof, s.n_tris)) for _ in range(3)))
+        s.ctx.flux_wall.copy_(torch.from_numpy(rng.standard_normal(s.ctx.flux_wall.shape)))
+        if use_pbuf:
+            pb = s.ctx.ensure_pbuf(s.sweep_vdf.dtype)
+            pb.copy_(torch.from_numpy(rng.standard_normal(tuple(pb.shape))))
+            s.ctx.use_pbuf = True
+        try:
+            for _ in range(2):
+                s.ctx.sweep(s.mom, s.sweep_vdf)
+        finally:
+            s.ctx.use_pbuf = False
+        outs.append(s.vdf)
+    assert s.ctx.staging(True)["on"]
+    assert np.array_equal(outs[0], outs[1])
+
+
 def test_broken_cycle_sweep_matches_oracle(pkg):
     """Inflow lagged by a broken cycle reads the previous sweep's value (the
     reference order emits the element before its upwind neighbour)."""
