@@ -47,7 +47,6 @@
 #include <string>
 #include <thread>
 #include <type_traits>
-#include <unordered_map>
 #include <vector>
 
 #include "pbte_hip.h"
@@ -649,8 +648,9 @@ template <typename T> struct SweepArgs {
     const int4* irec;           // per (class, slot): {cell | pool slot, face words}
     T* spill_row;               // write target of idle threads (P values)
     const uint8_t* bundle_global;
-    const int32_t* stage_list;  // per class: the elements its staged faces read, then its staged periodic faces
-    const int4* stage_off;      // per bundle: {its class's start in stage_list, elements, periodic faces, 0}
+    const int32_t* stage_list;  // per class: its staged reads, (element or face) << 2 | face across
+    const int4* stage_off;      // per bundle: {its class's start in stage_list, previous-sweep reads
+                                // across non-periodic faces, across periodic faces, periodic reads}
     int N, F, rs, use_pbuf, homogeneous;
     int stage_on;               // the stage slots follow the spill slot in LDS
     int lane_zero;              // 0 at run time; makes a uniform load per-lane
@@ -687,13 +687,13 @@ template <typename T> struct SweepArgs {
 //                         element), only in a bundle whose LDS pool would
 //                         overflow, which then runs with full barriers
 //   OLD_ST / PER_OLD_ST   OLD / PER_OLD with the value staged: the sweep's
-//                         prologue copies the traces every such face reads
-//                         into LDS stage slots (value: the stage slot in
-//                         bits 0-12; PER_OLD_ST also the periodic face's
+//                         prologue copies the trace every such face reads
+//                         into an LDS stage slot (value: the stage slot in
+//                         bits 0-12; PER_OLD_ST also the face's stage slot
+//                         for an explicit periodic buffer in bits 13-25)
+//   PER_LDS_ST            PER_LDS (pool slot in bits 0-12) with the face's
 //                         stage slot for an explicit periodic buffer in bits
-//                         13-25)
-//   PER_LDS_ST            PER_LDS (pool slot in bits 0-12) with the periodic
-//                         face's stage slot in bits 13-25
+//                         13-25
 // With an explicit periodic buffer every periodic face reads that instead
 // (a staged one from its periodic stage slot).  A launch whose LDS has no
 // room for the stage slots reads the staged faces from global memory (the
@@ -1338,12 +1338,13 @@ __device__ __forceinline__ FaceSel face_sel(int use_pbuf, int stage_on)
 }
 
 // the prologue's copy of the staged traces: per stage slot and direction of
-// the bundle the three face traces of an element's previous-sweep values or
-// of a periodic face's explicit-buffer entry, laid out as a pool slot.  A
-// class's list holds the elements of its staged previous-sweep reads (those
-// that some non-periodic face reads first), then its staged periodic faces;
-// without an explicit periodic buffer the stage slots are the elements, with
-// one the elements that non-periodic faces read, then the periodic faces.
+// the bundle the one face trace a staged read takes, from the previous
+// sweep's values of the element across or, for a periodic face with an
+// explicit periodic buffer, from the buffer's entry of the face.  A class's
+// list holds its staged reads in three runs: previous-sweep values across
+// non-periodic faces, across periodic faces, then every periodic face (the
+// buffer's entries); the stage slots are the first two runs without an
+// explicit periodic buffer, the first and the third with one.
 template <int P, typename T>
 __device__ __forceinline__ void stage_traces(const SweepArgs<T>& a, int bundle, int B, int t, int nthr, T* lds)
 {
@@ -1352,23 +1353,19 @@ __device__ __forceinline__ void stage_traces(const SweepArgs<T>& a, int bundle, 
     constexpr int TQ = Tr<P>::TQ;
     constexpr int W = Tr<P>::W;
     if (!a.stage_on) return;
-    const int4 so = a.stage_off[bundle];          // {start, elements, non-periodic elements, periodic faces}
-    const int nel = a.use_pbuf ? so.z : so.y;
-    const int n = nel + (a.use_pbuf ? so.w : 0);
+    const int4 so = a.stage_off[bundle];
+    const int n = so.y + (a.use_pbuf ? so.w : so.z);
     const int32_t* lst = a.stage_list + so.x;
     T* st = lds + (size_t)(a.pool + 2) * B * W;
     const int32_t* dirs = a.bundle_dirs + a.bundle_dir_off[bundle];
-    for (int k = t; k < n * B * W; k += nthr) {
-        const int e = k / (B * W), bb = (k / W) % B, m = k % W;
-        const int il = m / TQ, q = m % TQ;
-        T v = T(0);
-        if (il < 3 && q < Q) {
-            const int db = dirs[bb];
-            const T* src = e < nel ? a.vdf + ((size_t)db * a.N + lst[e]) * P
-                                   : a.pbuf + ((size_t)db * a.F + lst[so.y + e - nel]) * P;
-            v = src[face_node(DEG, il, q)];
-        }
-        st[k] = v;
+    for (int k = t; k < n * B * TQ; k += nthr) {
+        const int e = k / (B * TQ), bb = (k / TQ) % B, q = k % TQ;
+        const bool buf = a.use_pbuf && e >= so.y;
+        const int ent = lst[buf ? e + so.z : e];
+        const int idx = ent >> 2, ilp = ent & 3;
+        const int db = dirs[bb];
+        const T* src = buf ? a.pbuf + ((size_t)db * a.F + idx) * P : a.vdf + ((size_t)db * a.N + idx) * P;
+        st[k] = q < Q ? src[face_node(DEG, ilp, q < Q ? q : 0)] : T(0);
     }
 }
 
@@ -1395,7 +1392,7 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
     o.fw[2] = r.r.w;
     const unsigned zero = ((unsigned)a.pool * B + b) * W;
     const FaceSel fs = face_sel(a.use_pbuf, a.stage_on);
-    const unsigned stage0 = (unsigned)a.pool + 2u;
+    const unsigned stage0 = ((unsigned)a.pool + 2u) * B * W + (unsigned)b * TQ;
     unsigned wv = zero;
 #pragma unroll
     for (int il = 0; il < 3; ++il) {
@@ -1404,9 +1401,9 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
         // control flow)
         const int code = fw_code(o.fw[il]);
         const unsigned val = (unsigned)fw_val(o.fw[il]);
-        unsigned slot = code_in(fs.hi, code) ? val >> kStageBits : val & (unsigned)kStageMask;
-        slot += code_in(fs.stage, code) ? stage0 : 0u;
-        const unsigned nbr = (slot * B + b) * W + (unsigned)fw_ilp(o.fw[il]) * TQ;
+        const unsigned slot = code_in(fs.hi, code) ? val >> kStageBits : val & (unsigned)kStageMask;
+        const unsigned nbr = code_in(fs.stage, code) ? stage0 + slot * B * TQ
+                                                      : (slot * B + b) * W + (unsigned)fw_ilp(o.fw[il]) * TQ;
         o.lds[il] = (int)(code_in(fs.lds, code) ? nbr : zero);
         wv = code_in(kRecWall, code) ? (unsigned)a.wbase + (val * B + b) * PV : wv;
     }
@@ -3586,8 +3583,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         std::vector<int64_t> peak(nc, 0), slots_used(nc, 0);
         // per class: reads by kind (previous-sweep non-periodic, previous-
         // sweep periodic, periodic), whether its staged reads are encoded as
-        // such, and its stage list with {elements, non-periodic ones,
-        // periodic faces}
+        // such, and its stage list with the lengths of its three runs
         std::vector<std::array<int64_t, 3>> kind_reads(nc, {0, 0, 0});
         std::vector<uint8_t> staged(nc, 0);
         std::vector<std::vector<int32_t>> stage_lists(nc);
@@ -3739,47 +3735,24 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                     pair_overflow = true;
                     break;
                 }
-                // stage lists (a bundle with a pool only): the elements whose
-                // previous-sweep values some face reads, those a non-periodic
-                // face reads first, and the periodic faces read
-                std::unordered_map<int32_t, int32_t> el_idx, pf_idx;
-                if (!overflow) {
-                    std::vector<int32_t> np_el, per_el, pfs;
-                    std::unordered_map<int32_t, int32_t> seen;        // element -> 1 non-periodic, 2 periodic only
-                    for (int pass = 0; pass < 2; ++pass)
-                        for (int q = 0; q < N; ++q)
-                            for (int il = 0; il < 3; ++il) {
-                                const Read& r = rd[(size_t)q * 3 + il];
-                                if (pass == 0 && r.code == REC_OLD && !seen.count(r.val)) {
-                                    seen[r.val] = 1;
-                                    np_el.push_back(r.val);
-                                }
-                                if (pass == 1 && r.code == REC_PER_OLD && !seen.count(r.val)) {
-                                    seen[r.val] = 2;
-                                    per_el.push_back(r.val);
-                                }
-                                if (pass == 1 && (r.code == REC_PER_OLD || r.code == REC_PER_LDS)) {
-                                    const int fc = su->tri_faces[(size_t)sl[q] * 3 + il];
-                                    if (!pf_idx.count(fc)) {
-                                        pf_idx[fc] = (int32_t)pfs.size();
-                                        pfs.push_back(fc);
-                                    }
-                                }
-                            }
-                    const int nel = (int)(np_el.size() + per_el.size()), nnp = (int)np_el.size();
-                    const int npf = (int)pfs.size();
-                    if (nel + npf > 0 && nel < kStageMax && nnp + npf < kStageMax) {
-                        auto& lst = stage_lists[k];
-                        lst = np_el;
-                        lst.insert(lst.end(), per_el.begin(), per_el.end());
-                        for (int e = 0; e < nel; ++e) el_idx[lst[e]] = e;
-                        lst.insert(lst.end(), pfs.begin(), pfs.end());
-                        stage_cnt[k] = {nel, nnp, npf};
+                // stage lists (a bundle with a pool only): one entry per
+                // staged read, (element or periodic face) << 2 | face across,
+                // in three runs: previous-sweep reads across non-periodic
+                // faces, across periodic faces, every periodic read
+                {
+                    int64_t n_np = 0, n_po = 0, n_per = 0;
+                    for (size_t f = 0; f < (size_t)N * 3; ++f) {
+                        n_np += rd[f].code == REC_OLD;
+                        n_po += rd[f].code == REC_PER_OLD;
+                        n_per += rd[f].code == REC_PER_OLD || rd[f].code == REC_PER_LDS;
+                    }
+                    if (!overflow && n_np + n_po + n_per > 0 && n_np + n_po < kStageMax && n_np + n_per < kStageMax) {
+                        stage_cnt[k] = {(int32_t)n_np, (int32_t)n_po, (int32_t)n_per};
+                        stage_lists[k].assign((size_t)(n_np + n_po + n_per), 0);
                         staged[k] = 1;
-                    } else {
-                        pf_idx.clear();
                     }
                 }
+                int32_t e_np = 0, e_po = 0, e_per = 0;
                 for (int q = 0; q < N; ++q) {
                     int w[3];
                     for (int il = 0; il < 3; ++il) {
@@ -3791,16 +3764,27 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                             if (overflow) r = {r.code == REC_UP_LDS ? REC_UP_GLOBAL : REC_PER_GLOBAL, sl[r.val], r.ilp};
                             else r.val = sid[r.val];
                         }
-                        if (staged[k]) {
-                            // the periodic face's slot with an explicit buffer
-                            // follows the non-periodic elements' slots
-                            const bool per = r.code == REC_PER_OLD || r.code == REC_PER_LDS;
-                            const int hi = per ? (stage_cnt[k][1] + pf_idx.at(su->tri_faces[(size_t)sl[q] * 3 + il]))
-                                                     << kStageBits
-                                               : 0;
-                            if (r.code == REC_OLD) r = {REC_OLD_ST, el_idx.at(r.val), r.ilp};
-                            else if (r.code == REC_PER_OLD) r = {REC_PER_OLD_ST, el_idx.at(r.val) | hi, r.ilp};
-                            else if (r.code == REC_PER_LDS) r = {REC_PER_LDS_ST, r.val | hi, r.ilp};
+                        if (staged[k] && (r.code == REC_OLD || r.code == REC_PER_OLD || r.code == REC_PER_LDS)) {
+                            // stage slots: without an explicit periodic
+                            // buffer the first two runs, with one the
+                            // first and the third
+                            auto& lst = stage_lists[k];
+                            const int n_np = stage_cnt[k][0], n_po = stage_cnt[k][1];
+                            int hi = 0;
+                            if (r.code != REC_OLD) {
+                                const int fc = su->tri_faces[(size_t)sl[q] * 3 + il];
+                                lst[(size_t)n_np + n_po + e_per] = fc << 2 | r.ilp;
+                                hi = (n_np + e_per++) << kStageBits;
+                            }
+                            if (r.code == REC_OLD) {
+                                lst[e_np] = r.val << 2 | r.ilp;
+                                r = {REC_OLD_ST, e_np++, r.ilp};
+                            } else if (r.code == REC_PER_OLD) {
+                                lst[(size_t)n_np + e_po] = r.val << 2 | r.ilp;
+                                r = {REC_PER_OLD_ST, (n_np + e_po++) | hi, r.ilp};
+                            } else {
+                                r = {REC_PER_LDS_ST, r.val | hi, r.ilp};
+                            }
                         }
                         w[il] = face_word(r.code, r.ilp, r.val);
                     }
@@ -3913,8 +3897,8 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
             }
             const auto& sc = stage_cnt[c];
             h_stage_off[k] = make_int4(c_stage_off[c], sc[0], sc[1], sc[2]);
-            ctx->stage_slots[0] = std::max(ctx->stage_slots[0], sc[0]);
-            ctx->stage_slots[1] = std::max(ctx->stage_slots[1], sc[1] + sc[2]);
+            ctx->stage_slots[0] = std::max(ctx->stage_slots[0], sc[0] + sc[1]);
+            ctx->stage_slots[1] = std::max(ctx->stage_slots[1], sc[0] + sc[2]);
         }
         h_stage.resize(std::max<size_t>(h_stage.size(), 1), 0);
         ctx->stage_list = ctx->upload(h_stage.data(), h_stage.size());
@@ -4032,11 +4016,17 @@ template <> struct SweepData<float> {
     static const float* kref(const pbte_ctx* c) { return c->kref32; }
 };
 
+// LDS values of the stage slots (one face trace per slot and direction)
+template <int P> size_t stage_values(const pbte_ctx* ctx, int stage)
+{
+    return pad4(stage * ctx->max_bundle_dirs * Tr<P>::TQ);
+}
+
 // LDS bytes of the pipelined kernel: pool, zero and spill slots, stage
 // slots, wall traces
 template <int P, typename T> size_t sweep_lds_bytes(const pbte_ctx* ctx, int stage = 0)
 {
-    return ((size_t)(ctx->pool_slots + 2 + stage) * ctx->max_bundle_dirs * Tr<P>::W +
+    return ((size_t)(ctx->pool_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W + stage_values<P>(ctx, stage) +
             (size_t)ctx->max_wall_in * ctx->max_bundle_dirs * pad4(P)) *
            sizeof(T);
 }
@@ -4091,7 +4081,7 @@ SweepArgs<T> sweep_args(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int 
     a.stage_list = ctx->stage_list; a.stage_off = ctx->stage_off; a.stage_on = stage > 0;
     a.lane_zero = 0;
     a.pool = ctx->pool_slots;
-    a.wbase = (ctx->pool_slots + 2 + stage) * ctx->max_bundle_dirs * Tr<P>::W;
+    a.wbase = (ctx->pool_slots + 2) * ctx->max_bundle_dirs * Tr<P>::W + (int)stage_values<P>(ctx, stage);
     a.fbase = a.wbase + ctx->max_wall_in * ctx->max_bundle_dirs * pad4(P);
     a.nthr = ctx->block;
     for (int k = 0; k < ctx->Q * ctx->Q; ++k) a.kref[k] = S::kref(ctx)[k];
