@@ -3621,6 +3621,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 }
             });
         }
+        phase("levels");
         if (ctx->variant != 1 && su->block <= 0) {
             // the pipelined kernel runs one wave per 64 items of the widest
             // level (every wave issues the full load stream of a level)
@@ -3638,7 +3639,13 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         // at the first bundle whose pool overflows: the caller builds the
         // one-direction schedule instead
         std::atomic<bool> pair_overflow{false};
+        struct Read { int code, val, ilp; };
         auto work = [&]() {
+            // per-thread work arrays, reused by every class the thread builds
+            // (a fresh allocation per class page-faults tens of MB per thread)
+            std::vector<int32_t> sub_of((size_t)N), pos((size_t)N), slot_of((size_t)N), last_reader((size_t)N);
+            std::vector<int32_t> sid((size_t)N);
+            std::vector<Read> rd((size_t)N * 3);
             for (int k = next++; k < nc; k = next++) {
                 if (pair_overflow.load(std::memory_order_relaxed)) break;
                 const int d = bundles[rep[k]][0];
@@ -3654,20 +3661,16 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                     for (int s = cnt[l]; s < cnt[l + 1]; s += width) lp.push_back(s);
                 lp.push_back(N);
                 // sublevel of every slot
-                std::vector<int32_t> sub_of((size_t)N);
                 for (size_t j = 0; j + 1 < lp.size(); ++j)
                     for (int q = lp[j]; q < lp[j + 1]; ++q) sub_of[q] = (int32_t)j;
                 // per-slot records: the element and, per local face, what the
                 // sweep reads through it (REC_*).  This sweep's upwind values
                 // come from the LDS pool slot of the upwind element; a bundle
                 // whose pool would overflow reads them from global memory.
-                std::vector<int32_t> pos((size_t)N), slot_of((size_t)N);
                 for (int p = 0; p < N; ++p) pos[order[p]] = p;
                 for (int q = 0; q < N; ++q) slot_of[sl[q]] = q;
                 const double cxd = su->cxv[d], cyd = su->cyv[d];
-                struct Read { int code, val, ilp; };
-                std::vector<Read> rd((size_t)N * 3);
-                std::vector<int32_t> last_reader((size_t)N, -1);
+                std::fill(last_reader.begin(), last_reader.end(), -1);
                 auto& wlst = wlists[k];
                 wlst.clear();
                 for (int q = 0; q < N; ++q) {
@@ -3707,7 +3710,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
                 }
                 // pool slots: interval allocation in slot order; a slot is free
                 // again once the sublevel of its last reader has passed
-                std::vector<int32_t> sid((size_t)N, -1);
+                std::fill(sid.begin(), sid.end(), -1);
                 bool overflow = false;
                 {
                     using Busy = std::pair<int32_t, int32_t>;     // (last reader sublevel, slot)
@@ -3842,7 +3845,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         ctx->total_levels = tot;
         ctx->max_width = maxw;
         ctx->n_classes = nc;
-        phase("levels, records, pools");
+        phase("records, pools");
         // XCD-aware block -> bundle map: blocks are dispatched round robin over
         // 8 XCDs; give each XCD a contiguous range of bundles sorted by the
         // in-plane angle so its L2 sees similar sweep fronts.

@@ -179,13 +179,16 @@ int pbte_detail_levels(int n, const int32_t* order, const int32_t* nb, const int
                        const double* nx, const double* ny, double cx, double cy,
                        std::vector<int32_t>& level_of)
 {
-    std::vector<int32_t> pos((size_t)n);
+    // work arrays kept per thread across calls (one call per direction
+    // class; fresh ones page-fault on every call)
+    static thread_local std::vector<int32_t> pos, lower;
+    pos.resize((size_t)n);
     for (int k = 0; k < n; ++k) pos[order[k]] = k;
     level_of.assign((size_t)n, 0);
     // edges point forward in the reference order, so one pass in that order
     // finalises each element's level before it is used.  Old-value reads
     // i -> j are pushed forward as a lower bound on j.
-    std::vector<int32_t> lower((size_t)n, 0);
+    lower.assign((size_t)n, 0);
     int maxlev = 0;
     for (int k = 0; k < n; ++k) {
         int i = order[k];
