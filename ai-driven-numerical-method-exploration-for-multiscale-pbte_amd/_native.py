@@ -86,6 +86,10 @@ _SIGS = {
     "pbte_batch_sweep": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_void_p), C.c_int32]),
+    "pbte_batch_sweep_async": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                         C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                         C.POINTER(C.c_void_p), C.c_int32]),
+    "pbte_batch_join": (C.c_int, [C.c_void_p]),
     "pbte_last_error": (C.c_char_p, [C.c_void_p]),
     "pbte_set_cell_operators": (C.c_int, [C.c_void_p, _f64p, _f64p]),
     "pbte_set_fused": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -56,23 +56,29 @@ class SweepBatch:
             raise _native.NativeError("batch_pipelined: null batch")
         return st == 1
 
-    def sweep(self, moms, vdfs, homogeneous: bool = False) -> None:
+    def join(self) -> None:
+        """The contexts' stream waits for this batch's last launch."""
+        _join_all([self])
+
+    def sweep(self, moms, vdfs, homogeneous: bool = False, join: bool = True) -> None:
         """One sweep of every context: ``vdfs[k]`` in place from ``moms[k]``
         (all distributions of one precision), ``sweep_count`` += 1 each.
         Contexts that do not run the pipelined kernel in this precision
         (factors built on the fly, the plain level kernel) are swept one by
         one, each as ``SweepContext.sweep`` does."""
         self.sweep_requests([SweepRequest(c, v, homogeneous, None, None)
-                             for c, v in zip(self.contexts, vdfs)], moms)
+                             for c, v in zip(self.contexts, vdfs)], moms, join=join)
 
-    def sweep_requests(self, reqs, moms=None) -> None:
+    def sweep_requests(self, reqs, moms=None, join: bool = True) -> None:
         """The sweeps ``reqs[k]`` (``krylov.SweepRequest`` of context k: its
         distribution, moment source -- ``src`` or ``moms[k]`` --, flux_wall,
         and the explicit periodic buffer when the context's ``use_pbuf`` is
         set), all of one precision and one ``homogeneous`` flag, in one
         launch when every context runs the pipelined kernel; else one by
         one.  Each context's result is its ``SweepContext.sweep``, bit for
-        bit."""
+        bit.  ``join=False`` leaves the launch running beside the contexts'
+        stream (launches of several batches then overlap): call ``join``
+        before the stream uses the results."""
         n = len(self.contexts)
         prec = _PREC[reqs[0].vdf.dtype]
         hom = reqs[0].homogeneous
@@ -99,7 +105,8 @@ class SweepBatch:
             solver = r.ctx._solver() if r.ctx._solver is not None else None
             if solver is not None and r.vdf is solver.sweep_vdf:
                 solver.device_changed()
-        st = _native.lib().pbte_batch_sweep(self.handle, prec, ts, qx, qy, vd, fw, pb, 1 if hom else 0)
+        fn = _native.lib().pbte_batch_sweep if join else _native.lib().pbte_batch_sweep_async
+        st = fn(self.handle, prec, ts, qx, qy, vd, fw, pb, 1 if hom else 0)
         if st < 0:
             msg = _native.lib().pbte_batch_last_error(self.handle)
             raise _native.NativeError(f"batch_sweep: {msg.decode() if msg else 'unknown error'}")
@@ -107,6 +114,13 @@ class SweepBatch:
             r.ctx.sweep_count += 1
             if r.ctx.fused and prec == _native.PBTE_FP32:
                 r.ctx.factorisations += r.ctx.ndir * r.ctx.n_tris
+
+
+def _join_all(batches) -> None:
+    for b in batches:
+        if _native.lib().pbte_batch_join(b.handle) < 0:
+            msg = _native.lib().pbte_batch_last_error(b.handle)
+            raise _native.NativeError(f"batch_join: {msg.decode() if msg else 'unknown error'}")
 
 
 def launch_key(ctx) -> tuple:
@@ -176,9 +190,15 @@ def run_source_batch(solvers, callback=None):
             for k in active:
                 s = tracks[k].s
                 groups.setdefault((s.sweep_precision,) + launch_key(s.ctx), []).append(k)
+            # the groups' launches overlap on the device; the stream joins
+            # them all before the per-problem work
+            launched = []
             for group in groups.values():
-                batch_of(group).sweep([tracks[k].s.mom for k in group],
-                                      [tracks[k].s.sweep_vdf for k in group])
+                b = batch_of(group)
+                b.sweep([tracks[k].s.mom for k in group], [tracks[k].s.sweep_vdf for k in group], join=False)
+                launched.append(b)
+            _join_all(launched)
+            for group in groups.values():
                 for k in group:
                     if tracks[k].s.acc is not None:
                         tracks[k].s.acc.swept()
@@ -289,8 +309,12 @@ def run_refined_batch(solvers, callback=None):
                     reads.append(k)
                 else:
                     raise TypeError(f"unknown request {type(r).__name__}")
+            launched = []
             for group in groups.values():
-                batch_of(group).sweep_requests([reqs[k] for k in group])
+                b = batch_of(group)
+                b.sweep_requests([reqs[k] for k in group], join=False)
+                launched.append(b)
+            _join_all(launched)
             if reads:
                 vals = torch.stack([reqs[k].slot for k in reads]).cpu().numpy()
                 for k, (res, mass) in zip(reads, vals):

@@ -4470,16 +4470,33 @@ extern "C" int pbte_sweep_onthefly(pbte_ctx* ctx, int32_t precision, const doubl
 struct pbte_batch {
     std::vector<pbte_ctx*> ctxs;
     int2* bmap = nullptr;           // device [blocks] {context, block}: the work items, longest first
-    void* args = nullptr;           // device SweepArgs<T>[n]
+    void* args = nullptr;           // device SweepArgs<T>[n] of the next launch (an entry of the cache)
     int* head = nullptr;            // device work-queue head
-    std::vector<unsigned char> host_args;
+    // argument sets already on the device (a lockstep iteration alternates
+    // between a few distribution buffers): never rewritten, so a launch
+    // still reading one is never disturbed; least recently used dropped
+    struct ArgSet { std::vector<unsigned char> host; void* dev; uint64_t used; };
+    std::vector<ArgSet> arg_cache;
+    uint64_t arg_clock = 0;
+    static constexpr size_t kArgSets = 8;
+    // the batch's own stream: its launch waits for the contexts' stream up
+    // to the sweep's preparation, and the contexts' stream waits for the
+    // launch at the join, so the launches of several batches overlap
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    bool pending = false;           // a launch the contexts' stream has not joined yet
     int nblk = 0, block = 0, P = 0, ncu = 0;
     std::string err;
     ~pbte_batch()
     {
+        if (pending && !ctxs.empty()) (void)hipStreamWaitEvent(ctxs[0]->stream, ev_out, 0);
+        if (stream) (void)hipStreamSynchronize(stream);
         if (bmap) (void)hipFree(bmap);
-        if (args) (void)hipFree(args);
+        for (auto& e : arg_cache) (void)hipFree(e.dev);
         if (head) (void)hipFree(head);
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -4521,9 +4538,10 @@ extern "C" pbte_batch* pbte_batch_create(pbte_ctx* const* ctxs, int32_t n, char*
         bt->nblk = (int)h.size();
         HIP_TRY(hipMalloc((void**)&bt->bmap, h.size() * sizeof(int2)));
         HIP_TRY(hipMemcpy(bt->bmap, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice));
-        const size_t abytes = (size_t)n * std::max(sizeof(SweepArgs<double>), sizeof(SweepArgs<float>));
-        HIP_TRY(hipMalloc(&bt->args, abytes));
         HIP_TRY(hipMalloc((void**)&bt->head, sizeof(int)));
+        HIP_TRY(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&bt->ev_in, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&bt->ev_out, hipEventDisableTiming));
         int dev = 0;
         HIP_TRY(hipGetDevice(&dev));
         HIP_TRY(hipDeviceGetAttribute(&bt->ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -4601,14 +4619,35 @@ void batch_sweep(pbte_batch* bt, T* const* vdf, const void* const* pbuf, int hom
         one_dir = one_dir && c->max_bundle_dirs == 1;
     }
     // the arguments change only with the distributions and the source flag:
-    // upload (and wait for the previous launch to have read them) only then
-    if (host != bt->host_args) {
-        HIP_TRY(hipStreamSynchronize(bt->ctxs[0]->stream));
-        HIP_TRY(hipMemcpy(bt->args, host.data(), host.size(), hipMemcpyHostToDevice));
-        bt->host_args.swap(host);
+    // a set seen before is on the device already
+    bt->args = nullptr;
+    for (auto& e : bt->arg_cache)
+        if (e.host == host) {
+            bt->args = e.dev;
+            e.used = ++bt->arg_clock;
+        }
+    if (!bt->args) {
+        if (bt->arg_cache.size() >= pbte_batch::kArgSets) {
+            // the least recently used set: its last launch has finished once
+            // the batch stream is idle
+            auto old = std::min_element(bt->arg_cache.begin(), bt->arg_cache.end(),
+                                        [](const auto& a, const auto& b) { return a.used < b.used; });
+            HIP_TRY(hipStreamSynchronize(bt->stream));
+            HIP_TRY(hipFree(old->dev));
+            bt->arg_cache.erase(old);
+        }
+        void* dev = nullptr;
+        HIP_TRY(hipMalloc(&dev, host.size()));
+        HIP_TRY(hipMemcpy(dev, host.data(), host.size(), hipMemcpyHostToDevice));
+        bt->arg_cache.push_back({host, dev, ++bt->arg_clock});
+        bt->args = dev;
     }
     constexpr int pf = lu_stored_mode<P, T>();
-    const hipStream_t st = bt->ctxs[0]->stream;
+    // the launch follows the contexts' stream up to here (sources, wall
+    // vectors) on the batch's stream
+    HIP_TRY(hipEventRecord(bt->ev_in, bt->ctxs[0]->stream));
+    HIP_TRY(hipStreamWaitEvent(bt->stream, bt->ev_in, 0));
+    const hipStream_t st = bt->stream;
     if (fused) {
         if constexpr (std::is_same<T, float>::value && P <= 10) {
             if (bt->ctxs[0]->depth == 3) {
@@ -4627,14 +4666,39 @@ void batch_sweep(pbte_batch* bt, T* const* vdf, const void* const* pbuf, int hom
         else launch_pipe_batch<P, T, pf, false, 2>(bt, one_dir, lds, st);
     }
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(bt->ev_out, bt->stream));
+    bt->pending = true;
 }
 
-extern "C" int pbte_batch_sweep(pbte_batch* bt, int32_t precision, const double* const* ts, const double* const* qxs,
-                                const double* const* qys, void* const* vdf, const double* const* flux_wall,
-                                const void* const* pbuf, int32_t homogeneous)
+// the contexts' stream waits for the batch's last launch
+void batch_join(pbte_batch* bt)
+{
+    if (!bt->pending) return;
+    HIP_TRY(hipStreamWaitEvent(bt->ctxs[0]->stream, bt->ev_out, 0));
+    bt->pending = false;
+}
+
+extern "C" int pbte_batch_join(pbte_batch* bt)
 {
     if (!bt) return -1;
     try {
+        batch_join(bt);
+        return 0;
+    } catch (const std::string& m) {
+        bt->err = m;
+        return -1;
+    }
+}
+
+extern "C" int pbte_batch_sweep_async(pbte_batch* bt, int32_t precision, const double* const* ts,
+                                      const double* const* qxs, const double* const* qys, void* const* vdf,
+                                      const double* const* flux_wall, const void* const* pbuf, int32_t homogeneous)
+{
+    if (!bt) return -1;
+    try {
+        // a previous launch of this batch still reads its contexts' cell
+        // records, which the preparation below rewrites
+        batch_join(bt);
         const int n = (int)bt->ctxs.size();
         for (int k = 0; k < n; ++k) {
             pbte_ctx* c = bt->ctxs[k];
@@ -4671,6 +4735,14 @@ extern "C" int pbte_batch_sweep(pbte_batch* bt, int32_t precision, const double*
         bt->err = m;
         return -1;
     }
+}
+
+extern "C" int pbte_batch_sweep(pbte_batch* bt, int32_t precision, const double* const* ts, const double* const* qxs,
+                                const double* const* qys, void* const* vdf, const double* const* flux_wall,
+                                const void* const* pbuf, int32_t homogeneous)
+{
+    const int st = pbte_batch_sweep_async(bt, precision, ts, qxs, qys, vdf, flux_wall, pbuf, homogeneous);
+    return st < 0 ? st : pbte_batch_join(bt);
 }
 
 extern "C" int pbte_moments(pbte_ctx* ctx, int32_t precision, const void* vdf, double* ts, double* qxs,

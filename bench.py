@@ -529,8 +529,11 @@ def batch_workload(pkg, args, work):
     batches = [(SweepBatch([solvers[k].ctx for k in g]), g) for g in groups.values()]
 
     def step_batched():
+        # the groups' launches overlap on the device, joined at the step's end
         for b, g in batches:
-            b.sweep([solvers[k].mom for k in g], [solvers[k].sweep_vdf for k in g])
+            b.sweep([solvers[k].mom for k in g], [solvers[k].sweep_vdf for k in g], join=False)
+        for b, _ in batches:
+            b.join()
 
     def step_alone():
         for s in solvers:

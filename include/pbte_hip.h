@@ -274,6 +274,18 @@ const char *pbte_batch_last_error(pbte_batch *batch);
 int pbte_batch_sweep(pbte_batch *batch, int32_t precision, const double *const *ts,
                      const double *const *qxs, const double *const *qys, void *const *vdf,
                      const double *const *flux_wall, const void *const *pbuf, int32_t homogeneous);
+/* pbte_batch_sweep in two halves.  The launch runs on the batch's own
+ * stream after the contexts' stream has reached it; _async returns without
+ * making the contexts' stream wait for it, so the launches of several
+ * batches (groups of problems that need different kernel instances)
+ * overlap on the device; pbte_batch_join makes the contexts' stream wait
+ * for the batch's last launch (the next _async of the same batch joins
+ * first).  Join before using the swept distributions on that stream. */
+int pbte_batch_sweep_async(pbte_batch *batch, int32_t precision, const double *const *ts,
+                           const double *const *qxs, const double *const *qys, void *const *vdf,
+                           const double *const *flux_wall, const void *const *pbuf,
+                           int32_t homogeneous);
+int pbte_batch_join(pbte_batch *batch);
 /* 1 when every context of the batch runs the pipelined kernel in this
  * precision (pbte_batch_sweep can sweep them together), else 0: the caller
  * then sweeps those contexts one by one with pbte_sweep. */
