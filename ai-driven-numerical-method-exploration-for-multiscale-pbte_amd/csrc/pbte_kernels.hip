@@ -1374,7 +1374,7 @@ __device__ __forceinline__ void stage_traces(const SweepArgs<T>& a, int bundle, 
 // in-order vmcnt counter would make an earlier use wait for every
 // prefetch in flight).  The inverse streams through a buffer descriptor:
 // one lane offset, the k-groups a scalar offset apart.
-template <int P, typename T, int LUM>
+template <int P, typename T, int LUM, int BF>
 __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec& r, int B, int b,
                                             __amdgpu_buffer_rsrc_t lu, PipeData<P, T, LUM>& o)
 {
@@ -1402,8 +1402,16 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
         const int code = fw_code(o.fw[il]);
         const unsigned val = (unsigned)fw_val(o.fw[il]);
         const unsigned slot = code_in(fs.hi, code) ? val >> kStageBits : val & (unsigned)kStageMask;
-        const unsigned nbr = code_in(fs.stage, code) ? stage0 + slot * B * TQ
-                                                      : (slot * B + b) * W + (unsigned)fw_ilp(o.fw[il]) * TQ;
+        const bool st = code_in(fs.stage, code);
+        unsigned nbr;
+        if constexpr (BF) {
+            // one direction per bundle: a stage slot holds one trace, a pool
+            // slot three; one multiply-add with selected constants
+            nbr = slot * (st ? (unsigned)TQ : (unsigned)W) + (st ? stage0 : (unsigned)fw_ilp(o.fw[il]) * TQ);
+        } else {
+            // (the selected-factor form measured slower in bundles)
+            nbr = st ? stage0 + slot * B * TQ : (slot * B + b) * W + (unsigned)fw_ilp(o.fw[il]) * TQ;
+        }
         o.lds[il] = (int)(code_in(fs.lds, code) ? nbr : zero);
         wv = code_in(kRecWall, code) ? (unsigned)a.wbase + (val * B + b) * PV : wv;
     }
@@ -1626,7 +1634,7 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         if constexpr (sizeof(T) == 4) pipe_read<P, T, LUM, GREAD>(a, dS, d, lds, vdf_d, in);
         pipe_record(irec, sN, c, B, b, rN);
         sN = span(lvN);
-        pipe_gather<P, T, LUM>(a, rG, B, b, lu, dG);
+        pipe_gather<P, T, LUM, BF>(a, rG, B, b, lu, dG);
         if constexpr (sizeof(T) != 4) pipe_read<P, T, LUM, GREAD>(a, dS, d, lds, vdf_d, in);
         pipe_compute<P, T, LUM>(a, dS, in, cx, cy, B, b, lds, vdf_d, a.lu + lu0);
         level_barrier(global_mode);
@@ -1645,7 +1653,7 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         // waiting
         pipe_record(irec, span(l0), c, B, b, R0);
         pipe_record(irec, span(l0 + 1), c, B, b, R1);
-        pipe_gather<P, T, LUM>(a, R0, B, b, lu, D0);
+        pipe_gather<P, T, LUM, BF>(a, R0, B, b, lu, D0);
         // two levels per trip, with no exit between them: an odd level count
         // runs one empty level (span (0, 0)) at the end.  A mid-trip exit
         // would make the compiler rotate the register sets with copies, and a
@@ -1665,9 +1673,9 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         PipeData<P, T, LUM> D0, D1, D2;
         pipe_record(irec, span(l0), c, B, b, R0);
         pipe_record(irec, span(l0 + 1), c, B, b, R1);
-        pipe_gather<P, T, LUM>(a, R0, B, b, lu, D0);
+        pipe_gather<P, T, LUM, BF>(a, R0, B, b, lu, D0);
         pipe_record(irec, span(l0 + 2), c, B, b, R2);
-        pipe_gather<P, T, LUM>(a, R1, B, b, lu, D1);
+        pipe_gather<P, T, LUM, BF>(a, R1, B, b, lu, D1);
         pipe_record(irec, span(l0 + 3), c, B, b, R0);
         for (int lv = l0; lv < l1; lv += 6) {
             level(D0, R1, S1, lv + 7, D2, R2);     // solve lv; record lv+4; gather lv+2
@@ -1820,7 +1828,7 @@ __device__ __forceinline__ void sweep_split_body(const SweepArgs<T>& a, int blk)
             for (int g = 0; g < FV / 4; ++g) lds_load<T, 4>(fv + 4 * g, fb + (size_t)g * fstride);
             pipe_record(irec, sN, c, B, b, rN);
             sN = span(lvN);
-            pipe_gather<P, T, LU_SPLIT>(a, rG, B, b, irec, dG);
+            pipe_gather<P, T, LU_SPLIT, BF>(a, rG, B, b, irec, dG);
             pipe_compute<P, T, LU_SPLIT>(a, dS, in, cx, cy, B, b, lds, vdf_d, nullptr, fv);
             level_barrier(global_mode);
         };
@@ -1830,9 +1838,9 @@ __device__ __forceinline__ void sweep_split_body(const SweepArgs<T>& a, int blk)
         PipeData<P, T, LU_SPLIT> D0, D1, D2;
         pipe_record(irec, span(l0), c, B, b, R0);
         pipe_record(irec, span(l0 + 1), c, B, b, R1);
-        pipe_gather<P, T, LU_SPLIT>(a, R0, B, b, irec, D0);
+        pipe_gather<P, T, LU_SPLIT, BF>(a, R0, B, b, irec, D0);
         pipe_record(irec, span(l0 + 2), c, B, b, R2);
-        pipe_gather<P, T, LU_SPLIT>(a, R1, B, b, irec, D1);
+        pipe_gather<P, T, LU_SPLIT, BF>(a, R1, B, b, irec, D1);
         pipe_record(irec, span(l0 + 3), c, B, b, R0);
         for (int n = 0; n < 3 * ntrip; n += 3) {
             level(D0, R1, S1, l0 + n + 7, D2, R2, l0 + n);
