@@ -1321,6 +1321,16 @@ __device__ __forceinline__ void pipe_record(__amdgpu_buffer_rsrc_t irec, int2 sp
 // record-code classes as bit masks over the code: one shift and one test,
 // no short-circuit control flow
 // faces whose inflow trace comes from the LDS pool
+// item (slot, direction b) of a bundle of B directions as an LDS row index:
+// slot B + b, in 24-bit multiplies (slots and wall-list indices are far
+// below 2^24) so that no 64-bit multiply-add is needed; one direction per
+// bundle: the slot itself
+template <int BF> __device__ __forceinline__ unsigned bundle_row(unsigned slot, int B, int b)
+{
+    if constexpr (BF) return slot;
+    else return __umul24(slot, (unsigned)B) + (unsigned)b;
+}
+
 // how a face word selects its LDS trace (masks over the record codes,
 // uniform per launch): the codes read from LDS, those whose slot is in bits
 // 13-25, those whose slot counts from the first stage slot
@@ -1410,13 +1420,17 @@ __device__ __forceinline__ void pipe_gather(const SweepArgs<T>& a, const PipeRec
             nbr = slot * (st ? (unsigned)TQ : (unsigned)W) + (st ? stage0 : (unsigned)fw_ilp(o.fw[il]) * TQ);
         } else {
             // (the selected-factor form measured slower in bundles)
-            nbr = st ? stage0 + slot * B * TQ : (slot * B + b) * W + (unsigned)fw_ilp(o.fw[il]) * TQ;
+            nbr = st ? stage0 + __umul24(slot, (unsigned)(B * TQ))
+                     : __umul24(bundle_row<BF>(slot, B, b), (unsigned)W) + (unsigned)fw_ilp(o.fw[il]) * TQ;
         }
         o.lds[il] = (int)(code_in(fs.lds, code) ? nbr : zero);
-        wv = code_in(kRecWall, code) ? (unsigned)a.wbase + (val * B + b) * PV : wv;
+        // a wall face's value is its element's index in the bundle's wall
+        // list, far below 2^24 (the list lives in LDS)
+        wv = code_in(kRecWall, code) ? (unsigned)a.wbase + bundle_row<BF>(val & 0xffffffu, B, b) * PV : wv;
     }
     o.wv = (int)wv;
-    load16<T, (LUM == LU_FUSED ? L::CRF : L::CRL)>(o.cr, a.crec + (size_t)i * a.rs);
+    // the element's row: element below 2^21, stride below 2^8
+    load16<T, (LUM == LU_FUSED ? L::CRF : L::CRL)>(o.cr, a.crec + __umul24((unsigned)i, (unsigned)a.rs));
     if constexpr (LUM == LU_PREFETCH) {
         constexpr int w = 16 / sizeof(T);
         const uint32_t voff = (uint32_t)(r.slot * B + b) * 16u;
@@ -1480,7 +1494,7 @@ __device__ __forceinline__ void pipe_read(const SweepArgs<T>& a, const PipeData<
 }
 
 // second half: source, wall inflow, upwind faces, the inverse, the stores
-template <int P, typename T, int LUM>
+template <int P, typename T, int LUM, int BF>
 __device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeData<P, T, LUM>& dC,
                                              const PipeIn<P, T>& in, T cx, T cy, int B, int b, T* lds, T* vdf_d,
                                              const T* lub, const T* fv = nullptr)
@@ -1520,7 +1534,7 @@ __device__ __forceinline__ void pipe_compute(const SweepArgs<T>& a, const PipeDa
         constexpr int W = Tr<P>::W;
         const int spill = a.pool + 1;
         const int sl = (dC.valid && dC.wst >= 0) ? dC.wst : spill;
-        const int wr = (sl * B + b) * W;
+        const int wr = (int)(bundle_row<BF>((unsigned)sl, B, b) * W);
         T* o = dC.valid ? vdf_d + (size_t)dC.cell * P : a.spill_row;
 #pragma unroll
         for (int m = 0; m < P; ++m) o[m] = x[m];
@@ -1636,7 +1650,7 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         sN = span(lvN);
         pipe_gather<P, T, LUM, BF>(a, rG, B, b, lu, dG);
         if constexpr (sizeof(T) != 4) pipe_read<P, T, LUM, GREAD>(a, dS, d, lds, vdf_d, in);
-        pipe_compute<P, T, LUM>(a, dS, in, cx, cy, B, b, lds, vdf_d, a.lu + lu0);
+        pipe_compute<P, T, LUM, BF>(a, dS, in, cx, cy, B, b, lds, vdf_d, a.lu + lu0);
         level_barrier(global_mode);
     };
     if constexpr (DEPTH == 2) {
@@ -1829,7 +1843,7 @@ __device__ __forceinline__ void sweep_split_body(const SweepArgs<T>& a, int blk)
             pipe_record(irec, sN, c, B, b, rN);
             sN = span(lvN);
             pipe_gather<P, T, LU_SPLIT, BF>(a, rG, B, b, irec, dG);
-            pipe_compute<P, T, LU_SPLIT>(a, dS, in, cx, cy, B, b, lds, vdf_d, nullptr, fv);
+            pipe_compute<P, T, LU_SPLIT, BF>(a, dS, in, cx, cy, B, b, lds, vdf_d, nullptr, fv);
             level_barrier(global_mode);
         };
         level_barrier(global_mode);            // the producers' first level
