@@ -3646,15 +3646,21 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         phase("levels");
         if (ctx->variant != 1 && su->block <= 0) {
             // the pipelined kernel runs one wave per 64 items of the widest
-            // level (every wave issues the full load stream of a level)
+            // level (every wave issues the full load stream of a level), at
+            // most three waves: a level's time grows with the waves that run
+            // it, steeply from three to four (256 x 256 P2, 512 ordinates,
+            // paired directions, one MI355X: 1.04 / 1.19 / 1.53 us per level
+            // at two / three / four waves; the sweep 2.69 / 2.33 / 2.44 ms
+            // with the sublevels that the narrower width adds)
+            constexpr int kAutoItems = 192;
             int64_t widest = 1;
             for (int k = 0; k < nc; ++k)
                 for (size_t l = 0; l + 1 < lcnt[k].size(); ++l)
                     widest = std::max<int64_t>(widest,
                                                (int64_t)(lcnt[k][l + 1] - lcnt[k][l]) * bundles[rep[k]].size());
-            ctx->block = ctx->items = (int)std::min<int64_t>(max_items, (widest + 63) / 64 * 64);
+            ctx->block = ctx->items = (int)std::min<int64_t>(kAutoItems, (widest + 63) / 64 * 64);
             for (auto& bb : bundles)
-                if (ctx->items % (int)bb.size()) ctx->block = ctx->items = 256;
+                if (ctx->items % (int)bb.size()) ctx->block = ctx->items = kAutoItems;
         }
         std::atomic_int next{0};
         // a paired schedule asked to fail on overflow (pairs_or_fail) stops
