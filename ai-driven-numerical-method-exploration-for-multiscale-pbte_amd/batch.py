@@ -145,7 +145,7 @@ def launch_key(ctx) -> tuple:
     (element degree, pipeline depth, and whether every bundle holds one
     direction, whose instance drops the bundle arithmetic, and whether its
     binary32 sweeps are fused)."""
-    return (ctx.nd, 2 if ctx.sweep_variant == 2 else 3, ctx.one_dir, ctx.fused)
+    return (ctx.nd, ctx.sweep_variant if ctx.sweep_variant in (2, 4) else 3, ctx.one_dir, ctx.fused)
 
 
 class _Track:

@@ -1619,7 +1619,7 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         // the loops below run whole trips (two levels at depth 2, six at
         // depth 3), one barrier per level
         const int nlev = l1 > l0 ? l1 - l0 : 0;
-        const int trip = DEPTH == 3 ? 6 : DEPTH;
+        const int trip = DEPTH == 4 ? 12 : (DEPTH == 3 ? 6 : DEPTH);
         const int nbar = (nlev + trip - 1) / trip * trip;
         for (int k = 0; k < nbar; ++k) level_barrier(global_mode);
         return;
@@ -1675,6 +1675,37 @@ __device__ __forceinline__ void sweep_pipe_body(const SweepArgs<T>& a, int blk)
         for (int lv = l0; lv < l1; lv += 2) {
             level(D0, R0, S0, lv + 4, D1, R1);     // solve lv; record lv+2; gather lv+1
             level(D1, R1, S1, lv + 5, D0, R0);     // solve lv+1; record lv+3; gather lv+2
+        }
+    } else if constexpr (DEPTH == 4) {
+        // the gather three levels ahead of the solve and the records two
+        // levels ahead of the gather (four data sets, three record and span
+        // sets, twelve levels per trip): every gathered load has three level
+        // times to arrive.  Data set k holds level l0 + k mod 4, record and
+        // span set k the levels l0 + k mod 3.
+        int2 S0 = span(l0 + 6), S1 = span(l0 + 7), S2 = span(l0 + 5);
+        PipeRec R0, R1, R2;
+        PipeData<P, T, LUM> D0, D1, D2, D3;
+        pipe_record(irec, span(l0), c, B, b, R0);
+        pipe_record(irec, span(l0 + 1), c, B, b, R1);
+        pipe_gather<P, T, LUM, BF>(a, R0, B, b, lu, D0);
+        pipe_record(irec, span(l0 + 2), c, B, b, R2);
+        pipe_gather<P, T, LUM, BF>(a, R1, B, b, lu, D1);
+        pipe_record(irec, span(l0 + 3), c, B, b, R0);
+        pipe_gather<P, T, LUM, BF>(a, R2, B, b, lu, D2);
+        pipe_record(irec, span(l0 + 4), c, B, b, R1);
+        for (int lv = l0; lv < l1; lv += 12) {
+            level(D0, R2, S2, lv + 8, D3, R0);     // solve lv; record lv+5; gather lv+3
+            level(D1, R0, S0, lv + 9, D0, R1);     // solve lv+1; record lv+6; gather lv+4
+            level(D2, R1, S1, lv + 10, D1, R2);
+            level(D3, R2, S2, lv + 11, D2, R0);
+            level(D0, R0, S0, lv + 12, D3, R1);
+            level(D1, R1, S1, lv + 13, D0, R2);
+            level(D2, R2, S2, lv + 14, D1, R0);
+            level(D3, R0, S0, lv + 15, D2, R1);
+            level(D0, R1, S1, lv + 16, D3, R2);
+            level(D1, R2, S2, lv + 17, D0, R0);
+            level(D2, R0, S0, lv + 18, D1, R1);
+            level(D3, R1, S1, lv + 19, D2, R2);
         }
     } else {
         // the gather two levels ahead of the solve and the records two levels
@@ -3154,7 +3185,7 @@ extern "C" pbte_ctx* pbte_ctx_create(const pbte_dims* dims, const pbte_setup* su
         if (ctx->block % 64 || ctx->block > max_items)
             throw std::string("block must be a multiple of 64, <= ") + std::to_string(max_items);
         ctx->variant = su->sweep_variant;
-        if (ctx->variant < 0 || ctx->variant > 3) throw std::string("sweep_variant must be 0-3");
+        if (ctx->variant < 0 || ctx->variant > 4) throw std::string("sweep_variant must be 0-4");
         ctx->items = ctx->block;
         const int deg = Q - 1;
 
@@ -4198,7 +4229,10 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
                     else launch_split<P, T, false>(ctx, a, lds + extra);
                 }
             }
-            if (!split && ctx->depth == 3) {
+            if (!split && ctx->depth == 4) {
+                if (gread) launch_pipe<P, T, LU_FUSED, true, 4>(ctx, a, lds);
+                else launch_pipe<P, T, LU_FUSED, false, 4>(ctx, a, lds);
+            } else if (!split && ctx->depth == 3) {
                 if (gread) launch_pipe<P, T, LU_FUSED, true, 3>(ctx, a, lds);
                 else launch_pipe<P, T, LU_FUSED, false, 3>(ctx, a, lds);
             } else if (!split) {
@@ -4225,7 +4259,8 @@ void launch_sweep(pbte_ctx* ctx, const T* pbuf, T* vdf, int use_pbuf, int homoge
         hipLaunchKernelGGL((sweep_kernel<P, T>), dim3(ctx->nb), dim3(std::min(ctx->block, 256)), 0, ctx->stream, a);
     } else {
         constexpr int pf = lu_stored_mode<P, T>();
-        if (ctx->depth == 3) {
+        // (the four-level pipeline is built for the fused sweep only)
+        if (ctx->depth >= 3) {
             if (gread) launch_pipe<P, T, pf, true, 3>(ctx, a, lds);
             else launch_pipe<P, T, pf, false, 3>(ctx, a, lds);
         } else {
@@ -4678,7 +4713,10 @@ void batch_sweep(pbte_batch* bt, T* const* vdf, const void* const* pbuf, int hom
     const hipStream_t st = bt->stream;
     if (fused) {
         if constexpr (std::is_same<T, float>::value && P <= 10) {
-            if (bt->ctxs[0]->depth == 3) {
+            if (bt->ctxs[0]->depth == 4) {
+                if (gread) launch_pipe_batch<P, T, LU_FUSED, true, 4>(bt, one_dir, lds, st);
+                else launch_pipe_batch<P, T, LU_FUSED, false, 4>(bt, one_dir, lds, st);
+            } else if (bt->ctxs[0]->depth == 3) {
                 if (gread) launch_pipe_batch<P, T, LU_FUSED, true, 3>(bt, one_dir, lds, st);
                 else launch_pipe_batch<P, T, LU_FUSED, false, 3>(bt, one_dir, lds, st);
             } else {
@@ -4686,7 +4724,7 @@ void batch_sweep(pbte_batch* bt, T* const* vdf, const void* const* pbuf, int hom
                 else launch_pipe_batch<P, T, LU_FUSED, false, 2>(bt, one_dir, lds, st);
             }
         }
-    } else if (bt->ctxs[0]->depth == 3) {
+    } else if (bt->ctxs[0]->depth >= 3) {
         if (gread) launch_pipe_batch<P, T, pf, true, 3>(bt, one_dir, lds, st);
         else launch_pipe_batch<P, T, pf, false, 3>(bt, one_dir, lds, st);
     } else {

@@ -131,11 +131,11 @@ def test_fused_fp32_sweep_matches_oracle(pkg, tag, deg):
         assert _rel(s.vdf, o.vdf) <= 2e-5
 
 
-@pytest.mark.parametrize("bundle,depth", [(1, 3), (2, 3), (4, 3), (2, 2)])
+@pytest.mark.parametrize("bundle,depth", [(1, 3), (2, 3), (4, 3), (2, 2), (1, 4), (2, 4)])
 def test_fused_sweep_bitwise_across_schedules(pkg, bundle, depth):
     """The fused sweep's per-item arithmetic does not depend on the
-    schedule: bundles of 1, 2 or 4 directions and pipeline depths 2 and 3
-    give the same bits."""
+    schedule: bundles of 1, 2 or 4 directions and pipeline depths 2, 3 and
+    4 give the same bits."""
     import torch
     c = _case(pkg, "F3_1_1", performance__precision="fp32")
     outs = []
