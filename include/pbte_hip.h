@@ -70,8 +70,11 @@ typedef struct pbte_setup {
                                              gather 2 levels ahead), 1
                                              plain level sweep, 2 / 3
                                              pipelined with the gather 1 /
-                                             2 levels ahead (all give the
-                                             same bits)                    */
+                                             2 levels ahead, 4 the fused
+                                             binary32 sweep with the gather
+                                             3 levels ahead (other sweeps
+                                             of such a context: 2 ahead);
+                                             all give the same bits        */
     int32_t lds_slots_cap;                /* > 0 caps the LDS trace pool
                                              slots (exercises the global
                                              fallback); 0 = automatic      */
